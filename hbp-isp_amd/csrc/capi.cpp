@@ -1,0 +1,495 @@
+// extern "C" entry points of libhbp_hip.so (declared in include/hbp_hip.h).
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "../../include/hbp_hip.h"
+#include "hbp_internal.h"
+
+namespace hbp {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+void *Pool::get(size_t bytes) {
+    size_t n = (bytes + 255) & ~(size_t)255;
+    size_t best = (size_t)-1, bi = 0;
+    for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].n >= n && free_[i].n <= 2 * n && free_[i].n < best) {
+            best = free_[i].n;
+            bi = i;
+        }
+    Blk b;
+    if (best != (size_t)-1) {
+        b = free_[bi];
+        free_.erase(free_.begin() + (long)bi);
+    } else {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            // give cached blocks back and retry once
+            for (auto &f : free_) (void)hipFree(f.p);
+            for (auto &f : free_) reserved_ -= f.n;
+            free_.clear();
+            e = hipMalloc(&p, n);
+            if (e != hipSuccess)
+                throw HipError{"device allocation of " + std::to_string(n) + " bytes failed: " + hipGetErrorString(e)};
+        }
+        b = Blk{p, n};
+        reserved_ += n;
+    }
+    used_.push_back(b);
+    in_use_ += b.n;
+    return b.p;
+}
+
+void Pool::put(void *p) {
+    for (size_t i = 0; i < used_.size(); ++i)
+        if (used_[i].p == p) {
+            in_use_ -= used_[i].n;
+            free_.push_back(used_[i]);
+            used_.erase(used_.begin() + (long)i);
+            return;
+        }
+}
+
+void Pool::release_all() {
+    for (auto &f : free_) (void)hipFree(f.p);
+    for (auto &u : used_) (void)hipFree(u.p);
+    free_.clear();
+    used_.clear();
+    in_use_ = reserved_ = 0;
+}
+
+int crtrees_coop_blocks(int num_cu);
+
+}  // namespace hbp
+
+struct hbp_ctx {
+    hbp::Ctx c;
+};
+struct hbp_volume {
+    hbp::Volume v;
+};
+struct hbp_pyramid {
+    hbp::Pyramid p;
+};
+struct hbp_bp {
+    hbp::BP b;
+};
+
+using namespace hbp;
+
+#define HBP_API_BEGIN try {
+#define HBP_API_END                       \
+    }                                     \
+    catch (const HipError &e) {           \
+        set_error(e.msg);                 \
+        return -1;                        \
+    }                                     \
+    catch (const std::bad_alloc &) {      \
+        set_error("host allocation failed"); \
+        return -2;                        \
+    }                                     \
+    catch (...) {                         \
+        set_error("unknown error");       \
+        return -3;                        \
+    }                                     \
+    return 0;
+
+static void use_device(hbp_ctx *ctx) { HBP_CHECK(hipSetDevice(ctx->c.device)); }
+
+extern "C" {
+
+const char *hbp_last_error(void) { return hbp::g_last_error.c_str(); }
+
+int hbp_ctx_create(int device, hbp_ctx **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HBP_CHECK(hipGetDeviceCount(&n));
+    HBP_REQUIRE(device >= 0 && device < n, "no such HIP device");
+    HBP_CHECK(hipSetDevice(device));
+    auto *ctx = new hbp_ctx();
+    ctx->c.device = device;
+    hipDeviceProp_t prop;
+    HBP_CHECK(hipGetDeviceProperties(&prop, device));
+    ctx->c.num_cu = prop.multiProcessorCount;
+    HBP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    ctx->c.coop_blocks = crtrees_coop_blocks(ctx->c.num_cu);
+    *out = ctx;
+    HBP_API_END
+}
+
+int hbp_ctx_destroy(hbp_ctx *ctx) {
+    HBP_API_BEGIN
+    if (!ctx) return 0;
+    use_device(ctx);
+    HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    ctx->c.scratch.release();
+    ctx->c.counters.release();
+    ctx->c.pool.release_all();
+    HBP_CHECK(hipStreamDestroy(ctx->c.stream));
+    delete ctx;
+    HBP_API_END
+}
+
+int hbp_ctx_sync(hbp_ctx *ctx) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx, "ctx is NULL");
+    HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    HBP_API_END
+}
+
+void *hbp_ctx_stream(hbp_ctx *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
+long long hbp_ctx_pool_bytes(hbp_ctx *ctx) { return ctx ? (long long)ctx->c.pool.bytes_reserved() : 0; }
+
+// ---------------------------------------------------------------- (a)
+static void check_algo(const char *algo) {
+    HBP_REQUIRE(algo && std::strcmp(algo, "Census") == 0,
+                "only the \"Census\" cost is implemented (\"AD\" and \"MC-CNN\" are not)");
+}
+
+int hbp_cost_volume_dev(hbp_ctx *ctx, const uint8_t *d_l, const uint8_t *d_r, int W, int H, int D, const char *algo,
+                        hbp_volume **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && out && d_l && d_r, "NULL argument");
+    *out = nullptr;
+    check_algo(algo);
+    use_device(ctx);
+    auto *v = new hbp_volume();
+    try {
+        volume_build(ctx->c, v->v, d_l, d_r, W, H, D);
+    } catch (...) {
+        delete v;
+        throw;
+    }
+    *out = v;
+    HBP_API_END
+}
+
+int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *l, const uint8_t *r, int W, int H, int D, const char *algo,
+                    hbp_volume **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && out && l && r, "NULL argument");
+    *out = nullptr;
+    check_algo(algo);
+    HBP_REQUIRE(W > 0 && H > 0, "bad image size");
+    use_device(ctx);
+    size_t bytes = (size_t)W * H * 3;
+    DevBuf<uint8_t> dl(ctx->c.pool, bytes), dr(ctx->c.pool, bytes);
+    HBP_CHECK(hipMemcpyAsync(dl.p, l, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    HBP_CHECK(hipMemcpyAsync(dr.p, r, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    auto *v = new hbp_volume();
+    try {
+        volume_build(ctx->c, v->v, dl.p, dr.p, W, H, D);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    } catch (...) {
+        delete v;
+        throw;
+    }
+    *out = v;
+    HBP_API_END
+}
+
+int hbp_cost_to_right(hbp_ctx *ctx, hbp_volume *vol) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && vol, "NULL argument");
+    HBP_REQUIRE(!vol->v.right, "the volume is already the right view");
+    use_device(ctx);
+    volume_to_right(ctx->c, vol->v);
+    HBP_API_END
+}
+
+int hbp_volume_info(const hbp_volume *vol, int *W, int *H, int *D, float *tau, int *pitch) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(vol, "NULL volume");
+    if (W) *W = vol->v.W;
+    if (H) *H = vol->v.H;
+    if (D) *D = vol->v.D;
+    if (tau) *tau = vol->v.tau;
+    if (pitch) *pitch = vol->v.pitch;
+    HBP_API_END
+}
+
+float *hbp_volume_dev_wta(hbp_volume *vol) { return vol ? vol->v.dwta.p : nullptr; }
+const uint8_t *hbp_volume_dev_u8(const hbp_volume *vol) { return vol ? vol->v.vol.p : nullptr; }
+float *hbp_volume_dev_f32(hbp_ctx *ctx, hbp_volume *vol) {
+    try {
+        if (!ctx || !vol) return nullptr;
+        use_device(ctx);
+        float *p = volume_f32(ctx->c, vol->v);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+        return p;
+    } catch (const HipError &e) {
+        set_error(e.msg);
+        return nullptr;
+    }
+}
+
+int hbp_volume_download(hbp_ctx *ctx, const hbp_volume *vol, uint8_t *vh, float *wh) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && vol, "NULL argument");
+    use_device(ctx);
+    const Volume &v = vol->v;
+    if (vh)
+        HBP_CHECK(hipMemcpy2DAsync(vh, (size_t)v.W, v.vol.p, (size_t)v.pitch, (size_t)v.W, (size_t)v.H * v.D,
+                                   hipMemcpyDeviceToHost, ctx->c.stream));
+    if (wh)
+        HBP_CHECK(hipMemcpyAsync(wh, v.dwta.p, sizeof(float) * (size_t)v.W * v.H, hipMemcpyDeviceToHost,
+                                 ctx->c.stream));
+    HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    HBP_API_END
+}
+
+int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol) {
+    HBP_API_BEGIN
+    if (!vol) return 0;
+    if (ctx) {
+        use_device(ctx);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    }
+    delete vol;
+    HBP_API_END
+}
+
+// ---------------------------------------------------------------- (b)
+static void check_seg_args(int link, int num_nb, double sigma) {
+    HBP_REQUIRE(link == 2, "only CentroidLink (2) is implemented");
+    HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
+    HBP_REQUIRE(sigma <= 1.17549435e-38, "only sigma = 0 (no Gaussian pre-blur) is implemented");
+}
+
+int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int link, int num_nb, double sigma,
+                          hbp_pyramid **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && out && d_bgr, "NULL argument");
+    *out = nullptr;
+    check_seg_args(link, num_nb, sigma);
+    use_device(ctx);
+    auto *p = new hbp_pyramid();
+    try {
+        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb);
+    } catch (...) {
+        delete p;
+        throw;
+    }
+    *out = p;
+    HBP_API_END
+}
+
+int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int link, int num_nb, double sigma,
+                      hbp_pyramid **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && out && bgr, "NULL argument");
+    *out = nullptr;
+    check_seg_args(link, num_nb, sigma);
+    HBP_REQUIRE(W > 0 && H > 0, "bad image size");
+    use_device(ctx);
+    size_t bytes = (size_t)W * H * 3;
+    DevBuf<uint8_t> d(ctx->c.pool, bytes);
+    HBP_CHECK(hipMemcpyAsync(d.p, bgr, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    auto *p = new hbp_pyramid();
+    try {
+        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb);
+    } catch (...) {
+        delete p;
+        throw;
+    }
+    *out = p;
+    HBP_API_END
+}
+
+int hbp_pyramid_levels(const hbp_pyramid *pyr) { return pyr ? pyr->p.levels() : -1; }
+
+int hbp_pyramid_level_size(const hbp_pyramid *pyr, int level, int *V, int *E) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(pyr && level >= 0 && level < pyr->p.levels(), "bad level");
+    if (V) *V = pyr->p.V[level];
+    if (E) *E = pyr->p.E[level];
+    HBP_API_END
+}
+
+int *hbp_pyramid_dev_clus(hbp_pyramid *pyr) { return pyr ? pyr->p.clus.p : nullptr; }
+float *hbp_pyramid_dev_mean(hbp_pyramid *pyr) { return pyr ? pyr->p.mean.p : nullptr; }
+int *hbp_pyramid_dev_bd(hbp_pyramid *pyr) { return pyr ? (int *)pyr->p.bd.p : nullptr; }
+int *hbp_pyramid_dev_bdl(hbp_pyramid *pyr) { return pyr ? pyr->p.bdl.p : nullptr; }
+
+int hbp_pyramid_download_level(hbp_ctx *ctx, const hbp_pyramid *pyr, int level, int *clus, float *mean, int *bd,
+                               int *bdl) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && pyr && level >= 0 && level < pyr->p.levels(), "bad argument");
+    use_device(ctx);
+    const Pyramid &p = pyr->p;
+    size_t V = (size_t)p.V[level], E = (size_t)p.E[level];
+    hipStream_t s = ctx->c.stream;
+    if (clus) HBP_CHECK(hipMemcpyAsync(clus, p.clus.p + p.voff[level], sizeof(int) * V, hipMemcpyDeviceToHost, s));
+    if (mean)
+        HBP_CHECK(hipMemcpyAsync(mean, p.mean.p + 6 * p.voff[level], sizeof(float) * 6 * V, hipMemcpyDeviceToHost, s));
+    if (bd) HBP_CHECK(hipMemcpyAsync(bd, p.bd.p + p.eoff[level], sizeof(int2) * E, hipMemcpyDeviceToHost, s));
+    if (bdl) HBP_CHECK(hipMemcpyAsync(bdl, p.bdl.p + p.eoff[level], sizeof(int) * E, hipMemcpyDeviceToHost, s));
+    HBP_CHECK(hipStreamSynchronize(s));
+    HBP_API_END
+}
+
+int hbp_pyramid_free(hbp_ctx *ctx, hbp_pyramid *pyr) {
+    HBP_API_BEGIN
+    if (!pyr) return 0;
+    if (ctx) {
+        use_device(ctx);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    }
+    delete pyr;
+    HBP_API_END
+}
+
+int hbp_crtrees_get_clus(hbp_ctx *ctx, int *d_nn, int *d_clus, int n) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && d_nn && d_clus && n > 0, "bad argument");
+    use_device(ctx);
+    crtrees_labels(ctx->c, d_nn, n, d_clus);
+    HBP_API_END
+}
+
+int hbp_crtrees_compact(hbp_ctx *ctx, int *d_clus, int n, int *num) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && d_clus && n > 0, "bad argument");
+    use_device(ctx);
+    int v = compact_labels(ctx->c, d_clus, n);
+    if (num) *num = v;
+    HBP_API_END
+}
+
+// ---------------------------------------------------------------- (c)
+int hbp_bp_create(hbp_ctx *ctx, const hbp_pyramid *pyr, const hbp_volume *vol, float alpha, float tau_s,
+                  int min_level, int use_cost_vol, hbp_bp **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && pyr && vol && out, "NULL argument");
+    *out = nullptr;
+    HBP_REQUIRE(std::isfinite(alpha) && std::isfinite(tau_s), "alpha and tau_smooth must be finite");
+    use_device(ctx);
+    auto *b = new hbp_bp();
+    b->b.pyr = &pyr->p;
+    b->b.vol = &vol->v;
+    b->b.lambda = alpha;
+    b->b.tau_s = tau_s;
+    b->b.min_level = min_level;
+    b->b.use_cost_vol = use_cost_vol ? 1 : 0;
+    try {
+        bp_setup(ctx->c, b->b);
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    *out = b;
+    HBP_API_END
+}
+
+int hbp_bp_step(hbp_ctx *ctx, hbp_bp *bp, int max_sweeps, int *done) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && bp, "NULL argument");
+    use_device(ctx);
+    int d = bp_step(ctx->c, bp->b, max_sweeps);
+    if (done) *done = d;
+    HBP_API_END
+}
+
+int hbp_bp_total_sweeps(const hbp_bp *bp) { return bp ? (int)bp->b.sched.size() : -1; }
+int hbp_bp_level(const hbp_bp *bp) { return bp ? bp->b.cur : -1; }
+
+int hbp_bp_level_size(const hbp_bp *bp, int level, int *V, int *E) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(bp && level >= 0 && level < bp->b.L, "bad level");
+    if (V) *V = bp->b.lv[level].V;
+    if (E) *E = (int)bp->b.lv[level].E;
+    HBP_API_END
+}
+
+int hbp_bp_run(hbp_ctx *ctx, hbp_bp *bp, float *disp, float *plane3) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && bp, "NULL argument");
+    use_device(ctx);
+    BP &b = bp->b;
+    bp_step(ctx->c, b, 1 << 30);
+    bp_map(ctx->c, b);
+    size_t n = (size_t)b.N;
+    if (disp) HBP_CHECK(hipMemcpyAsync(disp, b.disp.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->c.stream));
+    if (plane3)
+        HBP_CHECK(hipMemcpyAsync(plane3, b.plane3.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->c.stream));
+    if (disp || plane3) HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    HBP_API_END
+}
+
+float *hbp_bp_dev_disp(hbp_bp *bp) { return bp ? bp->b.disp.p : nullptr; }
+float *hbp_bp_dev_plane(hbp_bp *bp) { return bp ? bp->b.plane3.p : nullptr; }
+
+int hbp_bp_get_state(hbp_ctx *ctx, hbp_bp *bp, float *planes, float *cost, float *belief, float *msg) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && bp, "NULL argument");
+    use_device(ctx);
+    bp_get_state(ctx->c, bp->b, planes, cost, belief, msg);
+    HBP_API_END
+}
+
+int hbp_bp_free(hbp_ctx *ctx, hbp_bp *bp) {
+    HBP_API_BEGIN
+    if (!bp) return 0;
+    if (ctx) {
+        use_device(ctx);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    }
+    delete bp;
+    HBP_API_END
+}
+
+int hbp_schedule_sweeps(int levels, int min_level, int labels) { return bp_total_sweeps(levels, min_level, labels); }
+
+// ---------------------------------------------------------------- (d)
+int hbp_lr_check_fill_dev(hbp_ctx *ctx, const float *dl, const float *dr, float *pl, int W, int H, uint8_t *fl,
+                          uint8_t *fr, uint8_t *f2) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && dl && dr && pl && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    size_t n = (size_t)W * H;
+    DevBuf<uint8_t> tmp(ctx->c.pool, 3 * n);
+    lr_check_fill(ctx->c, dl, dr, pl, W, H, fl ? fl : tmp.p, fr ? fr : tmp.p + n, f2 ? f2 : tmp.p + 2 * n);
+    HBP_API_END
+}
+
+int hbp_lr_check_fill(hbp_ctx *ctx, const float *dl, const float *dr, float *pl, int W, int H, uint8_t *fl,
+                      uint8_t *fr, uint8_t *f2) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && dl && dr && pl && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    size_t n = (size_t)W * H;
+    hipStream_t s = ctx->c.stream;
+    DevBuf<float> d(ctx->c.pool, 5 * n);
+    DevBuf<uint8_t> f(ctx->c.pool, 3 * n);
+    HBP_CHECK(hipMemcpyAsync(d.p, dl, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    HBP_CHECK(hipMemcpyAsync(d.p + n, dr, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    HBP_CHECK(hipMemcpyAsync(d.p + 2 * n, pl, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    lr_check_fill(ctx->c, d.p, d.p + n, d.p + 2 * n, W, H, f.p, f.p + n, f.p + 2 * n);
+    HBP_CHECK(hipMemcpyAsync(pl, d.p + 2 * n, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    if (fl) HBP_CHECK(hipMemcpyAsync(fl, f.p, n, hipMemcpyDeviceToHost, s));
+    if (fr) HBP_CHECK(hipMemcpyAsync(fr, f.p + n, n, hipMemcpyDeviceToHost, s));
+    if (f2) HBP_CHECK(hipMemcpyAsync(f2, f.p + 2 * n, n, hipMemcpyDeviceToHost, s));
+    HBP_CHECK(hipStreamSynchronize(s));
+    HBP_API_END
+}
+
+int hbp_plane_to_disp(hbp_ctx *ctx, const float *plane3, int W, int H, float *disp) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && plane3 && disp && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    size_t n = (size_t)W * H;
+    hipStream_t s = ctx->c.stream;
+    DevBuf<float> d(ctx->c.pool, 4 * n);
+    HBP_CHECK(hipMemcpyAsync(d.p, plane3, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    plane_to_disp(ctx->c, d.p, W, H, d.p + 3 * n);
+    HBP_CHECK(hipMemcpyAsync(disp, d.p + 3 * n, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HBP_CHECK(hipStreamSynchronize(s));
+    HBP_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+// Shared declarations of the HBP-ISP HIP library (gfx950 / CDNA4).
+//
+// All device arithmetic in this library is compiled with -ffp-contract=off and IEEE division
+// and square root, so every float expression rounds exactly as written; the scalar rules
+// below (cube root, exp, floor-to-int, fixed-point sums, LCG) are the pinned forms of the
+// reference's FP64 libm calls and unordered atomics.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hbp {
+
+// ---------------------------------------------------------------------------------------
+// error handling: every C-ABI entry point returns 0 or a negative code and records a message
+// ---------------------------------------------------------------------------------------
+void set_error(const std::string &msg);
+struct HipError {
+    std::string msg;
+};
+#define HBP_CHECK(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            throw ::hbp::HipError{std::string(#expr) + ": " + hipGetErrorString(_e) + " (" + \
+                                  __FILE__ + ":" + std::to_string(__LINE__) + ")"};        \
+    } while (0)
+#define HBP_REQUIRE(cond, msg)                          \
+    do {                                                \
+        if (!(cond)) throw ::hbp::HipError{std::string(msg)}; \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------
+// pinned scalar rules (device + host)
+// ---------------------------------------------------------------------------------------
+
+// Cube root on the Lab f(t) domain (216/24389, 1]: Newton y <- (2y + x/y^2)/3 from
+// 0.45 + 0.55x, eight steps.
+__host__ __device__ inline double cbrt_newton(double x) {
+    double y = 0.45 + 0.55 * x;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) y = (2.0 * y + x / (y * y)) / 3.0;
+    return y;
+}
+
+// exp(x), x <= 0: Cody-Waite split of ln2 (high part with 32 trailing zero bits), degree-13
+// Taylor polynomial on |r| <= ln2/2 (Horner), scaled by 2^n.
+__host__ __device__ inline double exp_cody_waite(double x) {
+    if (x < -745.0) return 0.0;
+    const double inv_ln2 = 1.4426950408889634;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    double n = rint(x * inv_ln2);
+    double r = (x - n * ln2_hi) - n * ln2_lo;
+    const double c[14] = {1.0,
+                          1.0,
+                          0.5,
+                          0.16666666666666666,
+                          0.041666666666666664,
+                          0.008333333333333333,
+                          0.001388888888888889,
+                          0.0001984126984126984,
+                          2.48015873015873e-05,
+                          2.7557319223985893e-06,
+                          2.755731922398589e-07,
+                          2.505210838544172e-08,
+                          2.08767569878681e-09,
+                          1.6059043836821613e-10};
+    double p = c[13];
+#pragma unroll
+    for (int k = 12; k >= 0; --k) p = c[k] + r * p;
+    return ldexp(p, (int)n);
+}
+
+// floor to int with NaN -> 0 and saturation (the semantics of __float2int_rd).
+__host__ __device__ inline int floor_to_int(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT_MAX;
+    if (x < -2147483648.0f) return INT_MIN;
+    return (int)floorf(x);
+}
+
+// int64 -> float, round to nearest even, from integer operations only.
+__host__ __device__ inline float int64_to_float_rne(int64_t q) {
+    if (q == 0) return 0.0f;
+    bool neg = q < 0;
+    uint64_t u = neg ? (uint64_t)0 - (uint64_t)q : (uint64_t)q;
+#ifdef __HIP_DEVICE_COMPILE__
+    int msb = 63 - __clzll((long long)u);
+#else
+    int msb = 63 - __builtin_clzll(u);
+#endif
+    float f;
+    if (msb <= 23) {
+        f = (float)(uint32_t)u;
+    } else {
+        int sh = msb - 23;
+        uint64_t m = u >> sh;
+        uint64_t rem = u & ((1ull << sh) - 1ull);
+        uint64_t half = 1ull << (sh - 1);
+        if (rem > half || (rem == half && (m & 1ull))) m += 1ull;
+        f = ldexpf((float)(uint32_t)m, sh);
+    }
+    return neg ? -f : f;
+}
+
+// Cluster sums of per-pixel plane costs are exact int64 sums of q = rint(v * 2^s); the cost is
+// int64_to_float_rne(sum) * 2^-s, or the quiet NaN when any term was NaN.
+int fixed_point_shift(float lambda, float tau, int use_cost_vol, long npix);
+__host__ __device__ inline int64_t to_fixed(float v, int s) { return (int64_t)rintf(ldexpf(v, s)); }
+__host__ __device__ inline float from_fixed(int64_t q, bool nan, int s) {
+    if (nan) return __int_as_float(0x7FC00000);
+    return ldexpf(int64_to_float_rne(q), -s);
+}
+
+// BSD LCG: s' = (1103515245 s + 12345) mod 2^31 in uint32 arithmetic.
+__host__ __device__ inline uint32_t lcg_next(uint32_t s) { return (1103515245u * s + 12345u) % 2147483648u; }
+__host__ __device__ inline float lcg_uniform(uint32_t &s, float f) {
+    s = lcg_next(s);
+    return (float)s / 2147483648.0f * f;
+}
+// unsigned modulo with a zero divisor mapped to 0
+__host__ __device__ inline uint32_t umod0(uint32_t s, int m) { return (uint32_t)m == 0u ? 0u : s % (uint32_t)m; }
+
+// ---------------------------------------------------------------------------------------
+// constants of the method (hbp_isp.hpp:265-283)
+// ---------------------------------------------------------------------------------------
+constexpr int kP = 4;         // particles kept per superpixel
+constexpr int kK = 3 * kP;    // best, prior and candidate slots
+constexpr float kCensusTau = 25.0f;
+
+// packed pixel coordinate (x | y << 16)
+__host__ __device__ inline uint32_t pack_xy(int x, int y) { return (uint32_t)x | ((uint32_t)y << 16); }
+__host__ __device__ inline int unpack_x(uint32_t p) { return (int)(p & 0xffffu); }
+__host__ __device__ inline int unpack_y(uint32_t p) { return (int)(p >> 16); }
+
+// ---------------------------------------------------------------------------------------
+// device memory: a per-context caching pool (blocks are reused by exact size)
+// ---------------------------------------------------------------------------------------
+class Pool {
+public:
+    void *get(size_t bytes);
+    void put(void *p);
+    void release_all();
+    ~Pool() { release_all(); }
+    size_t bytes_in_use() const { return in_use_; }
+    size_t bytes_reserved() const { return reserved_; }
+
+private:
+    struct Blk {
+        void *p;
+        size_t n;
+    };
+    std::vector<Blk> free_, used_;
+    size_t in_use_ = 0, reserved_ = 0;
+};
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    Pool *pool = nullptr;
+    DevBuf() = default;
+    DevBuf(Pool &pl, size_t count) { alloc(pl, count); }
+    void alloc(Pool &pl, size_t count) {
+        release();
+        pool = &pl;
+        n = count;
+        p = count ? (T *)pl.get(count * sizeof(T)) : nullptr;
+    }
+    void release() {
+        if (p && pool) pool->put(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), pool(o.pool) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            pool = o.pool;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    int num_cu = 256;
+    int coop_blocks = 0;   // co-resident blocks of the cooperative kernels
+    // scratch reused by every call on this context
+    DevBuf<uint8_t> scratch;
+    DevBuf<int> counters;  // small device counters (sizes, flags)
+    int *host_counters = nullptr;  // pinned mirror
+    void *scratch_bytes(size_t bytes) {
+        if (scratch.n < bytes) scratch.alloc(pool, bytes + (bytes >> 3));
+        return scratch.p;
+    }
+};
+
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+inline long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+}  // namespace hbp

@@ -1,0 +1,114 @@
+// Internal objects of the HBP-ISP HIP library: cost volume, superpixel pyramid, BP state.
+#pragma once
+#include "hbp_common.h"
+
+namespace hbp {
+
+// Cost volume of one stereo pair (StereoCost / StereoData of the reference).
+struct Volume {
+    int W = 0, H = 0, D = 0, pitch = 0;
+    long plane = 0;          // bytes between disparity planes (H * pitch)
+    float tau = kCensusTau;  // truncation (stereo_cost.cu:433)
+    bool right = false;      // cost_to_right applied
+    DevBuf<uint32_t> cl, cr; // census words of the left / right image [H][W]
+    DevBuf<uint8_t> vol;     // u8 [D][H][pitch]
+    DevBuf<float> dwta;      // WTA disparity [H][W]
+    DevBuf<float> f32;       // f32 [D][H][W], materialised only for the float* getter
+};
+
+// Superpixel pyramid in the reference's packed ISP format (segment.cu:884-905): per level l,
+// clus[V_l] (id at level l+1), mean[6][V_l] (sums of L, a, b, x, y, w), bd[E_l] (s, t),
+// bdl[E_l]; the levels follow each other in the four packed arrays.
+struct Pyramid {
+    int W = 0, H = 0, num_nb = 4;
+    std::vector<int> V, E;          // per level
+    std::vector<long> voff, eoff;   // element offsets of each level in the packed arrays
+    DevBuf<int> clus;
+    DevBuf<float> mean;
+    DevBuf<int2> bd;
+    DevBuf<int> bdl;
+    int levels() const { return (int)V.size(); }
+};
+
+// Per-level structures BP reads (HBP_ISP ctor + run_ms prologue, hbp_isp.hpp:288-386,
+// hbp_isp.cu:1319-1325), and the level's BP state.
+struct BPLevel {
+    int V = 0, Vh = 1;
+    long E = 0;
+    const int *clus = nullptr;     // into the pyramid
+    const int2 *bd = nullptr;      // into the pyramid
+    DevBuf<uint2> pix;             // (packed xy, cluster) sorted by (cluster, pixel id) [N]
+    DevBuf<int> pix_off;           // [V+1]
+    DevBuf<int> off;               // CSR offsets by source [V+1]
+    DevBuf<int> rev;               // reverse edge [E]
+    DevBuf<float> c0, dpos;        // per edge: max(dc0, .01) * bdl and max(dc1, 1) [E]
+    DevBuf<float2> mxy;            // normalised mean position [V]
+    DevBuf<int> child, child_off;  // nodes grouped by parent [V], [Vh+1]
+    DevBuf<float4> plane;          // [V][K]
+    DevBuf<float> cost, belief;    // [V][K]
+    DevBuf<uint4> point;           // packed xy of the last RANSAC triple [V]
+};
+
+// one BP sweep of the schedule of hbp_g (hbp_isp.cu:1283-1311)
+struct Sweep {
+    int level, kind;   // kind: 1 RANSAC, 2 spatial propagation, 3 random adjustment
+    float dz, dn;      // max_dz / max_dn of a random sweep
+    bool first, last;  // first / last sweep of its level
+};
+
+struct BP {
+    const Pyramid *pyr = nullptr;
+    const Volume *vol = nullptr;
+    int W = 0, H = 0, D = 0, L = 0, min_level = 2, use_cost_vol = 1;
+    long N = 0;
+    float lambda = 1000.1f, tau_s = 0.6f, tau = kCensusTau;
+    int fx_shift = 0;
+    std::vector<BPLevel> lv;
+    DevBuf<float4> msg;            // [E][8] persistent messages of the current level (2 x float4)
+    DevBuf<float> msg_t;           // [E][12] messages of the current sweep
+    DevBuf<unsigned long long> acc;  // [V][12] fixed-point plane-cost sums
+    DevBuf<unsigned int> accnan;     // [V] NaN bits per slot
+    DevBuf<uint32_t> s_ransac, s_neighbor, s_child;
+    // schedule state (hbp_g, hbp_isp.cu:1283-1311)
+    int cur = -1;
+    bool inited = false;
+    int outer = 0, plane_iter = 0, iter_bp = 0;
+    float max_dz = 0, max_dn = 0;
+    int sweeps = 0;
+    bool mapped = false;
+    std::vector<Sweep> sched;
+    size_t next = 0;
+    DevBuf<float4> best;           // per superpixel plane chosen by map_msg
+    DevBuf<float> disp, plane3;    // outputs of map_msg [N], [N][3]
+};
+
+// cost.hip
+void volume_build(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D);
+void volume_to_right(Ctx &c, Volume &v);
+float *volume_f32(Ctx &c, Volume &v);
+
+// pyramid.hip
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb);
+void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label);
+int compact_labels(Ctx &c, int *d_label, int V);
+
+// bp.hip
+void bp_setup(Ctx &c, BP &b);
+int bp_step(Ctx &c, BP &b, int max_sweeps);
+void bp_map(Ctx &c, BP &b);
+void bp_get_state(Ctx &c, BP &b, float *planes, float *cost, float *belief, float *msg);
+int bp_total_sweeps(int levels, int min_level, int D);
+
+// lr.hip
+void lr_check_fill(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
+                   uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2);
+void plane_to_disp(Ctx &c, const float *d_plane3, int W, int H, float *d_disp);
+
+// primitives.hip
+void scan_exclusive_int(Ctx &c, const int *d_in, int *d_out, long n, int *d_total);
+void sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, int *vals, int *vals_tmp,
+                    long n, int end_bit);
+void sort_pairs_u32(Ctx &c, unsigned int *keys, unsigned int *keys_tmp, int *vals, int *vals_tmp, long n,
+                    int end_bit);
+
+}  // namespace hbp

@@ -1,0 +1,549 @@
+// Stage (b): CRTrees superpixel pyramid (segment.cu, CRTrees.cu, imutil.cu of the reference).
+//
+// Per level: one cooperative kernel runs the whole CRTrees iteration with device-side loop
+// control (the reference polls a flag from the host after every kernel); then compaction,
+// the edge reduction (sort of (s', t') keys, run-length dedupe, boundary-length sums), the
+// per-parent sums in a fixed child order and the next level's 1-NN.  The host reads back two
+// counts per level (V', E') to size the next level.
+#include <hip/hip_cooperative_groups.h>
+
+#include <cmath>
+
+#include "hbp_internal.h"
+
+namespace cg = cooperative_groups;
+
+namespace hbp {
+
+__constant__ double c_srgb_lut[256];
+
+// ----------------------------------------------------------------------------------------
+// Lab conversion (imutil.cu:4-40 BGR_to_LAB, img_uchar3_to_float4 170-185) and the level-0
+// mean array (init_data_mean, segment.cu:856-861 -> img_struct_to_array imutil.cu:92-114).
+// ----------------------------------------------------------------------------------------
+__global__ void lab_kernel(const uint8_t *__restrict__ bgr, int W, int H, float4 *__restrict__ lab,
+                           float *__restrict__ mean) {
+    long n = (long)W * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double b = c_srgb_lut[bgr[3 * i]], g = c_srgb_lut[bgr[3 * i + 1]], r = c_srgb_lut[bgr[3 * i + 2]];
+        double x = (r * 0.412453 + g * 0.357580 + b * 0.180423) / 0.950456;
+        double y = (r * 0.212671 + g * 0.715160 + b * 0.072169) / 1.0;
+        double z = (r * 0.019334 + g * 0.119193 + b * 0.950227) / 1.088754;
+        const double epsilon = 216 / 24389.0;
+        const double k = 841 / 108.0;
+        const double kb = 16 / 116.0;
+        x = (x > epsilon) ? cbrt_newton(x) : (k * x) + kb;
+        y = (y > epsilon) ? cbrt_newton(y) : (k * y) + kb;
+        z = (z > epsilon) ? cbrt_newton(z) : (k * z) + kb;
+        float L = (float)((116 * y) - 16);
+        float A = (float)(500 * (x - y));
+        float B = (float)(200 * (y - z));
+        lab[i] = make_float4(L, A, B, 1.0f);
+        mean[i] = L;
+        mean[n + i] = A;
+        mean[2 * n + i] = B;
+        mean[3 * n + i] = (float)(int)(i % W);
+        mean[4 * n + i] = (float)(int)(i / W);
+        mean[5 * n + i] = 1.0f;
+    }
+}
+
+// segment.cu:14-22
+__device__ __forceinline__ float dist_px(float4 s, float4 t) {
+    float dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
+    float d = dx * dx + dy * dy + dz * dz;
+    float h = dx * 1.f;
+    d += h > 0.f ? h : 0.f;
+    return d;
+}
+
+// compute_1nn_grid_k (segment.cu:57-195) with bdl = 1 (segment.cu:846-850): neighbours in the
+// order u, d, l, r (+ ul, ur, dr, dl), first strict minimum, border slots are self-loops.
+__global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int num_nb, int *__restrict__ nn,
+                               float *__restrict__ dist_min, int2 *__restrict__ bd, int *__restrict__ bdl) {
+    long n = (long)W * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int x = (int)(i % W), y = (int)(i / W);
+        float4 v = lab[i];
+        float best = FLT_MAX;
+        int arg = 0;
+        bool ok[8] = {y > 0, y < H - 1, x > 0, x < W - 1, y > 0 && x > 0, y > 0 && x < W - 1,
+                      y < H - 1 && x < W - 1, y < H - 1 && x > 0};
+        long nb[8] = {i - W, i + W, i - 1, i + 1, i - W - 1, i - W + 1, i + W + 1, i + W - 1};
+        for (int k = 0; k < num_nb; ++k) {
+            long j = ok[k] ? nb[k] : i;
+            if (ok[k]) {
+                float d = dist_px(v, lab[j]);
+                if (d < best) {
+                    best = d;
+                    arg = (int)j;
+                }
+            }
+            bd[num_nb * i + k] = make_int2((int)i, (int)j);
+            bdl[num_nb * i + k] = 1;
+        }
+        nn[i] = arg;
+        dist_min[i] = best;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// CRTrees::get_clus (CRTrees.cu:315-354) as one kernel.  Phases are grid-stride loops over
+// the nodes separated by grid barriers; the loop flags live in device memory and every block
+// reads the same value after the barrier, so all blocks leave each loop together.
+//   A = nnc, B = nnn, M = max_d, C = clus; nnw is the working copy of nn.
+// re_link reads a snapshot of A (writes go to B), the double-buffered form of the in-place
+// kernel; the oracle checks both forms give the same labels.
+// ----------------------------------------------------------------------------------------
+struct CRTBuf {
+    int *nnw, *A, *B, *M, *C, *mask, *maskc, *indeg, *flags, *status;
+};
+
+template <bool GRID>
+__device__ __forceinline__ void crt_sync() {
+    if (GRID) cg::this_grid().sync();
+    else __syncthreads();
+}
+
+// loop flags: a block-wide OR, then one device-scope atomic per block; read back with an
+// agent-scope atomic load after the barrier.
+__device__ __forceinline__ void flag_or(int *flags, int slot, int local) {
+    if (__syncthreads_or(local) && threadIdx.x == 0) atomicOr(&flags[slot], 1);
+}
+__device__ __forceinline__ int flag_read(int *flags, int slot) {
+    return __hip_atomic_load(&flags[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flag_clear(int *flags, int slot) {
+    __hip_atomic_store(&flags[slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool GRID>
+__global__ __launch_bounds__(256) void crtrees_kernel(const int *__restrict__ nn0, int V, CRTBuf s,
+                                                      int *__restrict__ label) {
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nth = (long)gridDim.x * blockDim.x;
+    int *flags = s.flags;
+    int fi = 0;
+    for (long i = tid; i < V; i += nth) {
+        int v = nn0[i];
+        s.nnw[i] = v;
+        s.A[i] = v;
+        s.mask[i] = 1;
+    }
+    if (tid < 4) flag_clear(flags, (int)tid);
+    crt_sync<GRID>();
+    const int kMaxRounds = 64, kMaxJumps = 64;
+    int rounds = 0;
+    for (;;) {
+        // get_in_degree (CRTrees.cu:287-303)
+        for (long i = tid; i < V; i += nth) s.indeg[i] = 0;
+        crt_sync<GRID>();
+        for (long i = tid; i < V; i += nth)
+            if (s.mask[i]) atomicAdd(&s.indeg[s.A[i]], 1);
+        crt_sync<GRID>();
+        int loc = 0;
+        for (long i = tid; i < V; i += nth)
+            if (s.mask[i] && s.indeg[i] > 1) loc = 1;
+        flag_or(flags, fi, loc);
+        if (tid == 0) flag_clear(flags, (fi + 1) & 3);
+        crt_sync<GRID>();
+        bool more = flag_read(flags, fi) != 0;
+        fi = (fi + 1) & 3;
+        if (!more) break;
+        if (++rounds > kMaxRounds) {
+            if (tid == 0) *s.status = 1;
+            break;
+        }
+        // de_link_and_compute_mask (CRTrees.cu:58-74, 304-311)
+        for (long i = tid; i < V; i += nth) {
+            int m = s.mask[i];
+            s.maskc[i] = m;
+            int dl = s.indeg[i] > 1;
+            s.mask[i] = dl;
+            if (dl) s.A[i] = (int)i;
+        }
+        crt_sync<GRID>();
+        // pointer_jumping (CRTrees.cu:314-333): ping-pong between A and B; the caller keeps
+        // reading buffer A, which holds the result of the last even iteration.
+        {
+            int *p = s.A, *q = s.B;
+            for (int it = 0;; ++it) {
+                int jl = 0;
+                for (long i = tid; i < V; i += nth) {
+                    if (!s.maskc[i]) continue;
+                    int a = p[i], b = p[a];
+                    q[i] = b;
+                    if (!s.mask[a] && s.mask[b]) jl = 1;
+                }
+                flag_or(flags, fi, jl);
+                if (tid == 0) flag_clear(flags, (fi + 1) & 3);
+                crt_sync<GRID>();
+                bool jp = flag_read(flags, fi) != 0;
+                fi = (fi + 1) & 3;
+                int *t = p;
+                p = q;
+                q = t;
+                if (!jp) break;
+                if (it >= kMaxJumps) {
+                    if (tid == 0) *s.status = 2;
+                    break;
+                }
+            }
+        }
+        // re_link (CRTrees.cu:178-201), reading A, writing B
+        for (long i = tid; i < V; i += nth) {
+            int v = s.A[i];
+            if (s.mask[i]) {
+                int a = s.nnw[i];
+                v = a;
+                if (!s.mask[a]) v = s.A[a];
+            } else if (!s.maskc[i]) {
+                int a = s.A[i];
+                if (s.maskc[a]) v = s.A[a];
+            }
+            s.B[i] = v;
+        }
+        crt_sync<GRID>();
+        // re_link_all (CRTrees.cu:203-225)
+        for (long i = tid; i < V; i += nth) {
+            int a = s.B[i];
+            if (s.mask[i]) s.nnw[i] = a;
+            else if (s.mask[a]) s.nnw[i] = s.B[a];
+        }
+        crt_sync<GRID>();
+        for (long i = tid; i < V; i += nth) s.A[i] = s.nnw[i];   // CRTrees.cu:423
+        crt_sync<GRID>();
+    }
+    // pointer_jumping_max (CRTrees.cu:274-298)
+    for (long i = tid; i < V; i += nth) s.M[i] = s.A[i];
+    crt_sync<GRID>();
+    int *p = s.A, *q = s.B, *mx = s.M, *cl = s.C;
+    for (int it = 0;; ++it) {
+        int jl = 0;
+        for (long i = tid; i < V; i += nth) {
+            int a = p[i];
+            int m = mx[i];
+            int c = max(m, mx[a]);
+            cl[i] = c;
+            q[i] = p[a];
+            if (c != m) jl = 1;
+        }
+        flag_or(flags, fi, jl);
+        if (tid == 0) flag_clear(flags, (fi + 1) & 3);
+        crt_sync<GRID>();
+        bool jp = flag_read(flags, fi) != 0;
+        fi = (fi + 1) & 3;
+        int *t = p;
+        p = q;
+        q = t;
+        t = mx;
+        mx = cl;
+        cl = t;
+        if (!jp) break;
+        if (it >= kMaxJumps) {
+            if (tid == 0) *s.status = 3;
+            break;
+        }
+    }
+    for (long i = tid; i < V; i += nth) label[i] = mx[i];
+}
+
+int crtrees_coop_blocks(int num_cu) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, crtrees_kernel<true>, 256, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (per_cu > 4) per_cu = 4;
+    return per_cu * num_cu;
+}
+
+// CRTrees::compact_clus_label (CRTrees.cu:387-402): rank of each label among the labels used.
+__global__ void mark_labels_kernel(const int *__restrict__ label, int V, int *__restrict__ pred) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (long)gridDim.x * blockDim.x)
+        pred[label[i]] = 1;
+}
+__global__ void remap_labels_kernel(int *__restrict__ label, int V, const int *__restrict__ pos) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (long)gridDim.x * blockDim.x)
+        label[i] = pos[label[i]];
+}
+
+// ----------------------------------------------------------------------------------------
+// Edge reduction (reduce_boundary_dist, segment.cu:1050-1091)
+// ----------------------------------------------------------------------------------------
+__global__ void map_edges_kernel(const int2 *__restrict__ bd, const int *__restrict__ bdl,
+                                 const int *__restrict__ clus, long E, int bits,
+                                 unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
+        int2 b = bd[e];
+        unsigned long long s = (unsigned)clus[b.x], t = (unsigned)clus[b.y];
+        keys[e] = (s << bits) | t;
+        vals[e] = bdl[e];
+    }
+}
+// last element of a run of equal keys with s' != t' (is_diff_bd, segment.cu:320-334)
+__global__ void run_flags_kernel(const unsigned long long *__restrict__ keys, long E, int bits,
+                                 int *__restrict__ flag) {
+    unsigned long long tmask = (1ull << bits) - 1ull;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
+        unsigned long long k = keys[e];
+        bool self = (k >> bits) == (k & tmask);
+        flag[e] = (!self && (e == E - 1 || keys[e + 1] != k)) ? 1 : 0;
+    }
+}
+// reduce_bdl (segment.cu:368-386) + scatter_if (1085): every non-self element adds its bdl to
+// its run's slot (integer sums); the run's last element writes the edge.
+__global__ void scatter_edges_kernel(const unsigned long long *__restrict__ keys, const int *__restrict__ vals,
+                                     const int *__restrict__ flag, const int *__restrict__ pos, long E, int bits,
+                                     int2 *__restrict__ bd_out, int *__restrict__ bdl_out) {
+    unsigned long long tmask = (1ull << bits) - 1ull;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
+        unsigned long long k = keys[e];
+        int s = (int)(k >> bits), t = (int)(k & tmask);
+        if (s == t) continue;
+        atomicAdd(&bdl_out[pos[e]], vals[e]);
+        if (flag[e]) bd_out[pos[e]] = make_int2(s, t);
+    }
+}
+
+__global__ void fill_int_kernel(int *__restrict__ p, long n, int v) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void iota_kernel(int *__restrict__ p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = (int)i;
+}
+__global__ void count_keys_kernel(const int *__restrict__ key, long n, int *__restrict__ cnt) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[key[i]], 1);
+}
+__global__ void count_src_kernel(const int2 *__restrict__ bd, long E, int *__restrict__ cnt) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[bd[e].x], 1);
+}
+
+// reduce_data_mean (segment.cu:1134-1153) in the pinned order (children ascending, left fold
+// from 0) + sum_to_mean (491-503) + compute_dist_inner_max (1116-1128, max from 0).
+__global__ void parent_sums_kernel(const float *__restrict__ mean, int V, const int *__restrict__ child,
+                                   const int *__restrict__ child_off, int Vn, const float *__restrict__ dist_min,
+                                   float *__restrict__ sums, float *__restrict__ mu, float *__restrict__ dist_max) {
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < Vn; p += (long)gridDim.x * blockDim.x) {
+        int b = child_off[p], e = child_off[p + 1];
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float dm = 0.f;
+        for (int j = b; j < e; ++j) {
+            int c = child[j];
+#pragma unroll
+            for (int ch = 0; ch < 6; ++ch) acc[ch] += mean[(long)ch * V + c];
+            dm = fmaxf(dm, dist_min[c]);
+        }
+#pragma unroll
+        for (int ch = 0; ch < 6; ++ch) sums[(long)ch * Vn + p] = acc[ch];
+#pragma unroll
+        for (int ch = 0; ch < 5; ++ch) mu[(long)ch * Vn + p] = acc[ch] / acc[5];
+        dist_max[p] = dm;
+    }
+}
+
+// segment.cu:27-37
+__device__ __forceinline__ float dist_ch(float s, float n, int c) {
+    float d = (s - n) * (s - n);
+    if (c == 0) {
+        float h = (s - n) * 1.f;
+        d += h > 0.f ? h : 0.f;
+    }
+    return d;
+}
+
+// compute_dist_mean_k (508-534), pernalize_dist_k (582-612), compute_1nn (1211-1231):
+// per node over its out-edges (sorted by t): minimum distance, ties -> largest t.
+__global__ void level_nn_kernel(const float *__restrict__ mu, int Vn, const int2 *__restrict__ bd,
+                                const int *__restrict__ off, const float *__restrict__ dist_max,
+                                int *__restrict__ nn, float *__restrict__ dist_min) {
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < Vn; s += (long)gridDim.x * blockDim.x) {
+        float L = mu[s], A = mu[Vn + s], B = mu[2L * Vn + s];
+        float best = FLT_MAX;
+        int arg = 0;
+        for (int e = off[s]; e < off[s + 1]; ++e) {
+            int t = bd[e].y;
+            float d = dist_ch(L, mu[t], 0) + dist_ch(A, mu[Vn + t], 1) + dist_ch(B, mu[2L * Vn + t], 2);
+            float delta = d - dist_max[t];
+            if (delta > 0) d += delta;
+            if (d < best) {
+                best = d;
+                arg = t;
+            } else if (d == best && t > arg) {
+                arg = t;
+            }
+        }
+        nn[s] = arg;
+        dist_min[s] = best;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------------------
+static int grid_for(const Ctx &c, long n, int block = 256) {
+    long b = (n + block - 1) / block;
+    long cap = (long)c.num_cu * 16;
+    return (int)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+static int bits_for(long n) {
+    int b = 1;
+    while ((1L << b) < n) ++b;
+    return b;
+}
+
+static void upload_srgb_lut(Ctx &c) {
+    static double lut[256];
+    static bool init = false;
+    if (!init) {
+        for (int v = 0; v < 256; ++v) {
+            double cc = (double)(float)v / 255.0;
+            lut[v] = (cc > 0.04045) ? std::pow((cc + 0.055) / 1.055, 2.4) : cc / 12.92;
+        }
+        init = true;
+    }
+    HBP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_srgb_lut), lut, sizeof(lut), 0, hipMemcpyHostToDevice, c.stream));
+}
+
+void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label) {
+    DevBuf<int> buf(c.pool, (size_t)V * 8 + 8);
+    CRTBuf s;
+    s.nnw = buf.p;
+    s.A = buf.p + (size_t)V;
+    s.B = buf.p + 2 * (size_t)V;
+    s.M = buf.p + 3 * (size_t)V;
+    s.C = buf.p + 4 * (size_t)V;
+    s.mask = buf.p + 5 * (size_t)V;
+    s.maskc = buf.p + 6 * (size_t)V;
+    s.indeg = buf.p + 7 * (size_t)V;
+    s.flags = buf.p + 8 * (size_t)V;
+    s.status = buf.p + 8 * (size_t)V + 4;
+    HBP_CHECK(hipMemsetAsync(s.status, 0, sizeof(int), c.stream));
+    if (V <= 8192) {
+        hipLaunchKernelGGL(crtrees_kernel<false>, dim3(1), dim3(256), 0, c.stream, d_nn, V, s, d_label);
+        HBP_CHECK(hipGetLastError());
+    } else {
+        int blocks = c.coop_blocks;
+        long need = ((long)V + 255) / 256;
+        if (need < blocks) blocks = (int)need;
+        void *args[] = {(void *)&d_nn, (void *)&V, (void *)&s, (void *)&d_label};
+        HBP_CHECK(hipLaunchCooperativeKernel((const void *)crtrees_kernel<true>, dim3(blocks), dim3(256), args, 0,
+                                             c.stream));
+    }
+    int status = 0;
+    HBP_CHECK(hipMemcpyAsync(&status, s.status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    HBP_REQUIRE(status == 0, "CRTrees iteration did not converge within its bounds");
+}
+
+int compact_labels(Ctx &c, int *d_label, int V) {
+    DevBuf<int> pred(c.pool, (size_t)V + 1), pos(c.pool, (size_t)V + 1);
+    HBP_CHECK(hipMemsetAsync(pred.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
+    hipLaunchKernelGGL(mark_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pred.p);
+    scan_exclusive_int(c, pred.p, pos.p, V + 1, nullptr);   // pred[V] = 0, so pos[V] = count
+    hipLaunchKernelGGL(remap_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pos.p);
+    int Vn = 0;
+    HBP_CHECK(hipMemcpyAsync(&Vn, pos.p + V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    return Vn;
+}
+
+// SegHAC::run_ms (segment.cu:1279-1321) for CentroidLink, LabelISPFormat, target_clus = 1.
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb) {
+    HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
+    HBP_REQUIRE(W >= 2 && H >= 2, "image must be at least 2x2");
+    long N = (long)W * H;
+    HBP_REQUIRE(N < (1L << 24), "images are limited to 2^24 pixels");
+    p.W = W;
+    p.H = H;
+    p.num_nb = num_nb;
+    p.V.clear();
+    p.E.clear();
+    p.voff.clear();
+    p.eoff.clear();
+    // capacities asserted by the reference (segment.cu:891-892)
+    long vcap = 2 * N, ecap = 2L * num_nb * N;
+    p.clus.alloc(c.pool, vcap);
+    p.mean.alloc(c.pool, 6 * vcap);
+    p.bd.alloc(c.pool, ecap);
+    p.bdl.alloc(c.pool, ecap);
+    upload_srgb_lut(c);
+
+    DevBuf<float4> lab(c.pool, N);
+    DevBuf<int> nn(c.pool, N);
+    DevBuf<float> dist_min(c.pool, N);
+    hipLaunchKernelGGL(lab_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, d_bgr, W, H, lab.p, p.mean.p);
+    hipLaunchKernelGGL(nn_grid_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, lab.p, W, H, num_nb, nn.p,
+                       dist_min.p, p.bd.p, p.bdl.p);
+    HBP_CHECK(hipGetLastError());
+    lab.release();
+
+    int V = (int)N;
+    long E = num_nb * N;
+    long vo = 0, eo = 0;
+    while (V > 1) {
+        int *clus = p.clus.p + vo;
+        crtrees_labels(c, nn.p, V, clus);
+        int Vn = compact_labels(c, clus, V);
+        p.V.push_back(V);
+        p.E.push_back((int)E);
+        p.voff.push_back(vo);
+        p.eoff.push_back(eo);
+        if (Vn == 1) break;
+        long vo2 = vo + V, eo2 = eo + E;
+        HBP_REQUIRE(vo2 + Vn <= vcap && eo2 + E <= ecap, "pyramid capacity exceeded");
+        // --- edge reduction
+        int bits = bits_for(Vn);
+        DevBuf<unsigned long long> keys(c.pool, E), keys2(c.pool, E);
+        DevBuf<int> vals(c.pool, E), vals2(c.pool, E), flag(c.pool, E + 1), pos(c.pool, E + 1);
+        hipLaunchKernelGGL(map_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, p.bd.p + eo,
+                           p.bdl.p + eo, clus, E, bits, keys.p, vals.p);
+        sort_pairs_u64(c, keys.p, keys2.p, vals.p, vals2.p, E, 2 * bits);
+        HBP_CHECK(hipMemsetAsync(flag.p + E, 0, sizeof(int), c.stream));
+        hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p, E, bits, flag.p);
+        scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);
+        int En = 0;
+        HBP_CHECK(hipMemcpyAsync(&En, pos.p + E, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HBP_CHECK(hipStreamSynchronize(c.stream));
+        HBP_REQUIRE(eo2 + En <= ecap, "pyramid edge capacity exceeded");
+        HBP_CHECK(hipMemsetAsync(p.bdl.p + eo2, 0, sizeof(int) * (size_t)En, c.stream));
+        hipLaunchKernelGGL(scatter_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p, vals.p,
+                           flag.p, pos.p, E, bits, p.bd.p + eo2, p.bdl.p + eo2);
+        keys.release();
+        keys2.release();
+        vals.release();
+        vals2.release();
+        flag.release();
+        pos.release();
+        // --- children grouped by parent, ascending child index (stable sort by parent)
+        DevBuf<unsigned int> ck(c.pool, V), ck2(c.pool, V);
+        DevBuf<int> cv(c.pool, V), cv2(c.pool, V), cnt(c.pool, (size_t)Vn + 1), coff(c.pool, (size_t)Vn + 1);
+        HBP_CHECK(hipMemcpyAsync(ck.p, clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
+        hipLaunchKernelGGL(iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, cv.p, (long)V);
+        sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits);
+        HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
+        hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, clus, (long)V, cnt.p);
+        scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
+        // --- sums, means, dist_max
+        DevBuf<float> mu(c.pool, 5L * Vn), dist_max(c.pool, Vn);
+        hipLaunchKernelGGL(parent_sums_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, p.mean.p + 6 * vo, V,
+                           cv.p, coff.p, Vn, dist_min.p, p.mean.p + 6 * vo2, mu.p, dist_max.p);
+        // --- CSR of the new edges, then the next level's 1-NN
+        HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
+        hipLaunchKernelGGL(count_src_kernel, dim3(grid_for(c, En)), dim3(256), 0, c.stream, p.bd.p + eo2, (long)En,
+                           cnt.p);
+        scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
+        hipLaunchKernelGGL(level_nn_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, mu.p, Vn, p.bd.p + eo2,
+                           coff.p, dist_max.p, nn.p, dist_min.p);
+        HBP_CHECK(hipGetLastError());
+        V = Vn;
+        E = En;
+        vo = vo2;
+        eo = eo2;
+    }
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace hbp

@@ -1,0 +1,269 @@
+"""ctypes bindings to libhbp_hip.so (the C ABI in include/hbp_hip.h).
+
+The library is loaded from hbp-isp_amd/lib/; nothing falls back to another implementation:
+a missing library, a missing symbol or a failing call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libhbp_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "hbp_hip.h")
+
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_pp = C.POINTER(C.c_void_p)
+
+
+class HbpError(RuntimeError):
+    pass
+
+
+SIGNATURES = {
+    "hbp_ctx_create": (C.c_int, [C.c_int, _pp]),
+    "hbp_ctx_destroy": (C.c_int, [_vp]),
+    "hbp_ctx_sync": (C.c_int, [_vp]),
+    "hbp_ctx_stream": (_vp, [_vp]),
+    "hbp_last_error": (C.c_char_p, []),
+    "hbp_ctx_pool_bytes": (C.c_longlong, [_vp]),
+    "hbp_cost_volume": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
+    "hbp_cost_volume_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
+    "hbp_cost_to_right": (C.c_int, [_vp, _vp]),
+    "hbp_volume_info": (C.c_int, [_vp, _ip, _ip, _ip, _fp, _ip]),
+    "hbp_volume_dev_wta": (_vp, [_vp]),
+    "hbp_volume_dev_f32": (_vp, [_vp, _vp]),
+    "hbp_volume_dev_u8": (_vp, [_vp]),
+    "hbp_volume_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hbp_volume_free": (C.c_int, [_vp, _vp]),
+    "hbp_pyramid_build": (C.c_int, [_vp, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _pp]),
+    "hbp_pyramid_build_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _pp]),
+    "hbp_pyramid_levels": (C.c_int, [_vp]),
+    "hbp_pyramid_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "hbp_pyramid_dev_clus": (_vp, [_vp]),
+    "hbp_pyramid_dev_mean": (_vp, [_vp]),
+    "hbp_pyramid_dev_bd": (_vp, [_vp]),
+    "hbp_pyramid_dev_bdl": (_vp, [_vp]),
+    "hbp_pyramid_download_level": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "hbp_pyramid_free": (C.c_int, [_vp, _vp]),
+    "hbp_crtrees_get_clus": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "hbp_crtrees_compact": (C.c_int, [_vp, _vp, C.c_int, _ip]),
+    "hbp_bp_create": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int, _pp]),
+    "hbp_bp_step": (C.c_int, [_vp, _vp, C.c_int, _ip]),
+    "hbp_bp_total_sweeps": (C.c_int, [_vp]),
+    "hbp_bp_level": (C.c_int, [_vp]),
+    "hbp_bp_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "hbp_bp_run": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hbp_bp_dev_disp": (_vp, [_vp]),
+    "hbp_bp_dev_plane": (_vp, [_vp]),
+    "hbp_bp_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "hbp_bp_free": (C.c_int, [_vp, _vp]),
+    "hbp_schedule_sweeps": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "hbp_lr_check_fill": (C.c_int, [_vp, _f32, _f32, _f32, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "hbp_lr_check_fill_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "hbp_plane_to_disp": (C.c_int, [_vp, _f32, C.c_int, C.c_int, _f32]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libhbp_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HbpError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C hbp-isp_amd)")
+    lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _np_ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise HbpError(load().hbp_last_error().decode())
+
+
+class Ctx:
+    """One device context (one HIP stream, one device memory pool)."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.hbp_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.lib = lib
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hbp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        _check(self.lib.hbp_ctx_sync(self.h))
+
+    # ---- (a)
+    def cost_volume(self, left, right, D, right_view=False):
+        return Volume(self, left, right, D, right_view)
+
+    # ---- (b)
+    def pyramid(self, bgr, num_nb=4):
+        return Pyramid(self, bgr, num_nb)
+
+    # ---- (c)
+    def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
+        return BP(self, pyr, vol, alpha, tau_s, min_level, use_cost_vol)
+
+    # ---- (d)
+    def lr_check_fill(self, dispL, dispR, planeL):
+        H, W = dispL.shape
+        plane = np.ascontiguousarray(planeL, dtype=np.float32).copy()
+        fl = np.zeros((H, W), np.uint8)
+        fr = np.zeros((H, W), np.uint8)
+        f2 = np.zeros((H, W), np.uint8)
+        _check(self.lib.hbp_lr_check_fill(self.h, np.ascontiguousarray(dispL, np.float32),
+                                          np.ascontiguousarray(dispR, np.float32), plane, W, H,
+                                          _np_ptr(fl), _np_ptr(fr), _np_ptr(f2)))
+        return plane, fl, fr, f2
+
+    def plane_to_disp(self, plane3):
+        H, W = plane3.shape[:2]
+        d = np.zeros((H, W), np.float32)
+        _check(self.lib.hbp_plane_to_disp(self.h, np.ascontiguousarray(plane3, np.float32), W, H, d))
+        return d
+
+
+class Volume:
+    def __init__(self, ctx: Ctx, left, right, D, right_view=False):
+        self.ctx = ctx
+        H, W = left.shape[:2]
+        h = C.c_void_p()
+        _check(ctx.lib.hbp_cost_volume(ctx.h, np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
+                                       b"Census", C.byref(h)))
+        self.h = h
+        self.W, self.H, self.D = W, H, D
+        if right_view:
+            _check(ctx.lib.hbp_cost_to_right(ctx.h, self.h))
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.hbp_volume_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+    def download(self):
+        vol = np.zeros((self.D, self.H, self.W), np.uint8)
+        wta = np.zeros((self.H, self.W), np.float32)
+        _check(self.ctx.lib.hbp_volume_download(self.ctx.h, self.h, _np_ptr(vol), _np_ptr(wta)))
+        return vol, wta
+
+
+class Pyramid:
+    def __init__(self, ctx: Ctx, bgr, num_nb=4):
+        self.ctx = ctx
+        H, W = bgr.shape[:2]
+        h = C.c_void_p()
+        _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, 2, num_nb, 0.0, C.byref(h)))
+        self.h = h
+        self.W, self.H = W, H
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.hbp_pyramid_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+    @property
+    def levels(self):
+        return self.ctx.lib.hbp_pyramid_levels(self.h)
+
+    def level(self, l):
+        V, E = C.c_int(), C.c_int()
+        _check(self.ctx.lib.hbp_pyramid_level_size(self.h, l, C.byref(V), C.byref(E)))
+        V, E = V.value, E.value
+        clus = np.zeros(V, np.int32)
+        mean = np.zeros(6 * V, np.float32)
+        bd = np.zeros(2 * E, np.int32)
+        bdl = np.zeros(E, np.int32)
+        _check(self.ctx.lib.hbp_pyramid_download_level(self.ctx.h, self.h, l, _np_ptr(clus), _np_ptr(mean),
+                                                       _np_ptr(bd), _np_ptr(bdl)))
+        return dict(V=V, E=E, clus=clus, mean=mean.reshape(6, V), bd=bd.reshape(E, 2), bdl=bdl)
+
+
+class BP:
+    def __init__(self, ctx: Ctx, pyr: Pyramid, vol: Volume, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
+        self.ctx, self.pyr, self.vol = ctx, pyr, vol
+        h = C.c_void_p()
+        _check(ctx.lib.hbp_bp_create(ctx.h, pyr.h, vol.h, alpha, tau_s, min_level, use_cost_vol, C.byref(h)))
+        self.h = h
+        self.W, self.H = pyr.W, pyr.H
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.hbp_bp_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+    @property
+    def total_sweeps(self):
+        return self.ctx.lib.hbp_bp_total_sweeps(self.h)
+
+    @property
+    def level(self):
+        return self.ctx.lib.hbp_bp_level(self.h)
+
+    def step(self, n=1 << 30):
+        done = C.c_int()
+        _check(self.ctx.lib.hbp_bp_step(self.ctx.h, self.h, n, C.byref(done)))
+        return done.value
+
+    def run(self):
+        disp = np.zeros((self.H, self.W), np.float32)
+        plane = np.zeros((self.H, self.W, 3), np.float32)
+        _check(self.ctx.lib.hbp_bp_run(self.ctx.h, self.h, _np_ptr(disp), _np_ptr(plane)))
+        return disp, plane
+
+    def state(self):
+        l = self.level
+        V, E = C.c_int(), C.c_int()
+        _check(self.ctx.lib.hbp_bp_level_size(self.h, l, C.byref(V), C.byref(E)))
+        V, E = V.value, E.value
+        planes = np.zeros(12 * V * 4, np.float32)
+        cost = np.zeros(12 * V, np.float32)
+        belief = np.zeros(12 * V, np.float32)
+        msg = np.zeros(E * 8, np.float32)
+        _check(self.ctx.lib.hbp_bp_get_state(self.ctx.h, self.h, _np_ptr(planes), _np_ptr(cost), _np_ptr(belief),
+                                             _np_ptr(msg)))
+        return dict(level=l, V=V, E=E, planes=planes.reshape(12, V, 4), cost=cost.reshape(12, V),
+                    belief=belief.reshape(12, V), msg=msg.reshape(E, 8))
+
+
+def exported_symbols():
+    """Names declared in include/hbp_hip.h (for the export check)."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(hbp_[a-z0-9_]+)\s*\(", txt)))

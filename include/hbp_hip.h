@@ -1,0 +1,128 @@
+/* hbp_hip.h - C ABI of the MI355X (gfx950) HBP-ISP stereo library, libhbp_hip.so.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status (0 = success, < 0 = error, the
+ * message from hbp_last_error()).  Host buffers are caller-owned; device memory is owned by
+ * the library (a caching pool per context).  All work of a context runs in order on the
+ * context's own HIP stream.  Each entry point names the reference interface it replaces
+ * (file:line in the HBP-ISP sources); the C++ drop-in classes in hbp-isp_amd/api call these.
+ *
+ * Layouts: images are BGR u8 interleaved [H][W][3] (cv::Mat CV_8UC3); disparities f32 [H][W]
+ * (CV_32FC1); planes f32 [H][W][3] (CV_32FC3, a b c of d = a*x + b*y + c); the cost volume
+ * is u8 [D][H][pitch] on the device (pitch = W rounded up to 64) and f32 [D][H][W] through
+ * hbp_volume_dev_f32 / the float* getter.
+ */
+#ifndef HBP_HIP_H
+#define HBP_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hbp_ctx hbp_ctx;
+typedef struct hbp_volume hbp_volume;
+typedef struct hbp_pyramid hbp_pyramid;
+typedef struct hbp_bp hbp_bp;
+
+/* ---- context: replaces the implicit device/legacy stream of the reference
+ *      (cudaSetDevice(0), main_stereo_hbp_isp.cu:31) */
+int hbp_ctx_create(int device, hbp_ctx **out);
+int hbp_ctx_destroy(hbp_ctx *ctx);
+int hbp_ctx_sync(hbp_ctx *ctx);
+void *hbp_ctx_stream(hbp_ctx *ctx); /* hipStream_t of the context */
+const char *hbp_last_error(void);
+long long hbp_ctx_pool_bytes(hbp_ctx *ctx); /* device bytes reserved by the context's pool */
+
+/* ---- (a) cost volume: StereoCost(imgL, imgR, labels, vol_dir) + compute_stereo_cost("Census")
+ *      (stereo_cost.hpp:90-116, stereo_cost.cu:421-445).  algo must be "Census". */
+int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *bgr_left, const uint8_t *bgr_right, int width,
+                    int height, int labels, const char *algo, hbp_volume **out);
+/* same, images already in device memory */
+int hbp_cost_volume_dev(hbp_ctx *ctx, const uint8_t *d_bgr_left, const uint8_t *d_bgr_right,
+                        int width, int height, int labels, const char *algo, hbp_volume **out);
+/* StereoCost::cost_to_right (stereo_cost.cu:488-516): the volume and WTA of the right view */
+int hbp_cost_to_right(hbp_ctx *ctx, hbp_volume *vol);
+/* getters g_labels / g_im_size / g_tau (stereo_cost.hpp:27-41) and the layout */
+int hbp_volume_info(const hbp_volume *vol, int *width, int *height, int *labels, float *tau,
+                    int *pitch);
+/* g_img_D_d (stereo_cost.hpp:47-51): device f32 WTA disparity [H][W] */
+float *hbp_volume_dev_wta(hbp_volume *vol);
+/* g_img_cost_d (stereo_cost.hpp:42-46): device f32 [D][H][W] volume, built on first call */
+float *hbp_volume_dev_f32(hbp_ctx *ctx, hbp_volume *vol);
+/* device u8 volume [D][H][pitch] */
+const uint8_t *hbp_volume_dev_u8(const hbp_volume *vol);
+/* copy to host: vol u8 [D][H][W] (may be NULL), wta f32 [H][W] (may be NULL) */
+int hbp_volume_download(hbp_ctx *ctx, const hbp_volume *vol, uint8_t *vol_host, float *wta_host);
+int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol);
+
+/* ---- (b) superpixel pyramid: SegHAC(src, link, num_nb, sigma, LabelISPFormat, 1, false) +
+ *      run_ms() (segment.hpp:248-309, segment.cu:1279-1321).  link must be 2 (CentroidLink),
+ *      sigma 0, num_nb 4 or 8. */
+int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int width, int height, int link,
+                      int num_nb, double sigma, hbp_pyramid **out);
+int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int width, int height, int link,
+                          int num_nb, double sigma, hbp_pyramid **out);
+/* g_num_clus_isp().size() */
+int hbp_pyramid_levels(const hbp_pyramid *pyr);
+/* g_num_clus_isp()[level], g_num_bd_isp()[level] (segment.hpp:127-136) */
+int hbp_pyramid_level_size(const hbp_pyramid *pyr, int level, int *num_clus, int *num_bd);
+/* g_isp_clus_d / g_isp_mean_d / g_isp_bd_d / g_isp_bdl_d (segment.hpp:137-156): packed device
+ * arrays (int, float [6][V] per level, int2 as int pairs, int) */
+int *hbp_pyramid_dev_clus(hbp_pyramid *pyr);
+float *hbp_pyramid_dev_mean(hbp_pyramid *pyr);
+int *hbp_pyramid_dev_bd(hbp_pyramid *pyr);
+int *hbp_pyramid_dev_bdl(hbp_pyramid *pyr);
+/* copy one level to host: clus [V], mean [6][V] (sums), bd [E][2], bdl [E]; any may be NULL */
+int hbp_pyramid_download_level(hbp_ctx *ctx, const hbp_pyramid *pyr, int level, int *clus,
+                               float *mean, int *bd, int *bdl);
+int hbp_pyramid_free(hbp_ctx *ctx, hbp_pyramid *pyr);
+
+/* ---- CRTrees on device arrays (CRTrees.hpp:16-78) */
+/* CRTrees::get_clus(nn_d, clus_d, N): cluster representative per node, nn_d unchanged */
+int hbp_crtrees_get_clus(hbp_ctx *ctx, int *d_nn, int *d_clus, int n);
+/* CRTrees::compact_clus_label(clus_d, N): ranks in place, returns the cluster count */
+int hbp_crtrees_compact(hbp_ctx *ctx, int *d_clus, int n, int *num_clus);
+
+/* ---- (c) hierarchical BP: HBP_ISP(isp_data, stereo_data, alpha, tau_smooth)
+ *      (hbp_isp.hpp:288-386) with StereoData(stereo_cost, use_cost_vol) (hbp_isp.hpp:91-105).
+ *      The pyramid and volume are borrowed and must outlive the BP object. */
+int hbp_bp_create(hbp_ctx *ctx, const hbp_pyramid *pyr, const hbp_volume *vol, float alpha,
+                  float tau_smooth, int min_level, int use_cost_vol, hbp_bp **out);
+/* run up to max_sweeps sweeps of HBP_ISP::run_ms (hbp_isp.cu:1313-1346); *done = sweeps run */
+int hbp_bp_step(hbp_ctx *ctx, hbp_bp *bp, int max_sweeps, int *done);
+/* total sweeps the schedule holds, the level being worked on (min_level-1 when finished) */
+int hbp_bp_total_sweeps(const hbp_bp *bp);
+int hbp_bp_level(const hbp_bp *bp);
+int hbp_bp_level_size(const hbp_bp *bp, int level, int *num_clus, int *num_bd);
+/* map_msg (hbp_isp.cu:1081-1105): disp f32 [H][W] and plane f32 [H][W][3] on the host
+ * (either may be NULL); runs the remaining sweeps first */
+int hbp_bp_run(hbp_ctx *ctx, hbp_bp *bp, float *disp, float *plane3);
+/* device pointers of the last map: disp [H][W], plane [H][W][3] */
+float *hbp_bp_dev_disp(hbp_bp *bp);
+float *hbp_bp_dev_plane(hbp_bp *bp);
+/* state of the current level after the last sweep, for checks: planes [12][V][4], cost and
+ * belief [12][V], messages [E][8]; slots 8..11 of planes may already hold the next sweep's
+ * candidates */
+int hbp_bp_get_state(hbp_ctx *ctx, hbp_bp *bp, float *planes, float *cost, float *belief,
+                     float *msg);
+int hbp_bp_free(hbp_ctx *ctx, hbp_bp *bp);
+/* sweeps of the schedule for `levels` pyramid levels, min_level and D labels */
+int hbp_schedule_sweeps(int levels, int min_level, int labels);
+
+/* ---- (d) PostProcess(imL, imR) (post_process.hpp:47-55)
+ * left_right_check (post_process.cu:47-110): host buffers; plane_left is filled in place;
+ * fail_left / fail_right / fail2_left (u8 [H][W], 255 = failed) may be NULL. */
+int hbp_lr_check_fill(hbp_ctx *ctx, const float *disp_left, const float *disp_right,
+                      float *plane_left, int width, int height, uint8_t *fail_left,
+                      uint8_t *fail_right, uint8_t *fail2_left);
+/* same on device buffers */
+int hbp_lr_check_fill_dev(hbp_ctx *ctx, const float *d_disp_left, const float *d_disp_right,
+                          float *d_plane_left, int width, int height, uint8_t *d_fail_left,
+                          uint8_t *d_fail_right, uint8_t *d_fail2_left);
+/* plane_label_to_disp (post_process.cu:373-390), host buffers */
+int hbp_plane_to_disp(hbp_ctx *ctx, const float *plane3, int width, int height, float *disp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

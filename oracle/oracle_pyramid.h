@@ -1,0 +1,32 @@
+// Oracle (test infrastructure only): pyramid data shared by the oracle's stages.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace orc {
+
+// One pushed level in the reference's packed ISP format (segment.cu:884-905):
+// clus[V] -> id at the next level, mean[6][V] = per-node sums of (L, a, b, x, y, w)
+// (level 0: the pixel values), bd[E][2] edges (s, t), bdl[E] boundary lengths.
+struct Level {
+    int V = 0;
+    long E = 0;
+    std::vector<int> clus;
+    std::vector<float> mean;
+    std::vector<int> bd;
+    std::vector<int> bdl;
+};
+
+struct Pyramid {
+    int W = 0, H = 0;
+    std::vector<Level> levels;
+};
+
+void lab_image(const uint8_t *bgr, int W, int H, float *lab4);
+void nn_grid(const float *lab4, int W, int H, int num_nb, int *nn, float *dist_min, int *bd);
+void cc_max(const int *nn, int V, int *label);
+int compact(int *label, int V);
+void crtrees_literal(const int *nn, int V, int order, int *out);
+void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P);
+
+}  // namespace orc

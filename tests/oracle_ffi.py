@@ -1,0 +1,159 @@
+"""ctypes bindings to the oracle (oracle/lib/libhbp_oracle.so): test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "lib", "libhbp_oracle.so")
+
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle")])
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_pin_cbrt": (C.c_double, [C.c_double]),
+        "orc_pin_exp": (C.c_double, [C.c_double]),
+        "orc_i64_to_f32": (C.c_float, [C.c_int64]),
+        "orc_f2i_rd": (C.c_int, [C.c_float]),
+        "orc_fixed_shift": (C.c_int, [C.c_float, C.c_float, C.c_int, C.c_int]),
+        "orc_amin_fold": (C.c_float, [C.c_float, C.c_float]),
+        "orc_grey": (None, [_u8p, C.c_int, C.c_int, _u8p]),
+        "orc_census": (None, [_u8p, C.c_int, C.c_int, _u32p]),
+        "orc_hamming": (None, [_u32p, _u32p, C.c_int, C.c_int, C.c_int, C.c_int, _u8p]),
+        "orc_wta": (None, [_u8p, C.c_int, C.c_int, C.c_int, _f32p]),
+        "orc_vol_to_right": (None, [_u8p, C.c_int, C.c_int, C.c_int]),
+        "orc_cost_volume": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, _u8p, _f32p]),
+        "orc_lab": (None, [_u8p, C.c_int, C.c_int, _f32p]),
+        "orc_nn_grid": (None, [_f32p, C.c_int, C.c_int, C.c_int, _i32p, _f32p, _i32p]),
+        "orc_cc_max": (None, [_i32p, C.c_int, _i32p]),
+        "orc_crtrees_literal": (None, [_i32p, C.c_int, C.c_int, _i32p]),
+        "orc_compact": (C.c_int, [_i32p, C.c_int]),
+        "orc_pyramid_build": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int]),
+        "orc_pyr_levels": (C.c_int, [C.c_void_p]),
+        "orc_pyr_level_size": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "orc_pyr_get_level": (None, [C.c_void_p, C.c_int, _i32p, _f32p, _i32p, _i32p]),
+        "orc_pyr_free": (None, [C.c_void_p]),
+        "orc_bp_new": (C.c_void_p, [C.c_void_p, _u8p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                    C.c_float, C.c_float, C.c_int, C.c_int]),
+        "orc_bp_step": (C.c_int, [C.c_void_p, C.c_int]),
+        "orc_bp_level": (C.c_int, [C.c_void_p]),
+        "orc_bp_sweeps_done": (C.c_int, [C.c_void_p]),
+        "orc_bp_level_size": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "orc_bp_get_state": (None, [C.c_void_p, _f32p, _f32p, _f32p, _f32p]),
+        "orc_bp_map": (None, [C.c_void_p, _f32p, _f32p]),
+        "orc_bp_free": (None, [C.c_void_p]),
+        "orc_bp_total_sweeps": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "orc_lr_consistency": (None, [_f32p, _f32p, C.c_int, C.c_int, _u8p, _u8p]),
+        "orc_dilate3": (None, [_u8p, C.c_int, C.c_int, _u8p]),
+        "orc_lr_fill": (None, [_u8p, _u8p, _f32p, C.c_int, C.c_int]),
+        "orc_plane_to_disp": (None, [_f32p, C.c_int, C.c_int, _f32p]),
+        "orc_left_right_check": (None, [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, _u8p, _u8p, _u8p]),
+        "orc_plain_pair": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                     C.c_int, C.c_int, _f32p, _f32p, C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def cost_volume(left, right, D, right_view=False):
+    H, W = left.shape[:2]
+    vol = np.zeros((D, H, W), np.uint8)
+    dwta = np.zeros((H, W), np.float32)
+    rc = lib.orc_cost_volume(np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
+                             int(right_view), vol, dwta)
+    assert rc == 0
+    return vol, dwta
+
+
+class Pyramid:
+    def __init__(self, bgr, num_nb=4):
+        H, W = bgr.shape[:2]
+        self.W, self.H = W, H
+        self.h = lib.orc_pyramid_build(np.ascontiguousarray(bgr), W, H, num_nb)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_pyr_free(self.h)
+            self.h = None
+
+    @property
+    def levels(self):
+        return lib.orc_pyr_levels(self.h)
+
+    def level(self, l):
+        V, E = C.c_int(), C.c_int()
+        lib.orc_pyr_level_size(self.h, l, C.byref(V), C.byref(E))
+        V, E = V.value, E.value
+        clus = np.zeros(V, np.int32)
+        mean = np.zeros(6 * V, np.float32)
+        bd = np.zeros(2 * E, np.int32)
+        bdl = np.zeros(E, np.int32)
+        lib.orc_pyr_get_level(self.h, l, clus, mean, bd, bdl)
+        return dict(V=V, E=E, clus=clus, mean=mean.reshape(6, V), bd=bd.reshape(E, 2), bdl=bdl)
+
+
+class BP:
+    def __init__(self, pyr: Pyramid, vol, dwta, D, alpha=1000.1, tau_s=0.6, min_level=2,
+                 use_cost_vol=1, tau=25.0):
+        self.pyr, self.vol, self.dwta = pyr, np.ascontiguousarray(vol), np.ascontiguousarray(dwta)
+        self.W, self.H, self.D = pyr.W, pyr.H, D
+        self.h = lib.orc_bp_new(pyr.h, self.vol, self.dwta, pyr.W, pyr.H, D, tau, alpha, tau_s,
+                                min_level, use_cost_vol)
+        assert self.h, "invalid min_level"
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_bp_free(self.h)
+            self.h = None
+
+    def step(self, n=1 << 30):
+        return lib.orc_bp_step(self.h, n)
+
+    @property
+    def level(self):
+        return lib.orc_bp_level(self.h)
+
+    def state(self):
+        l = self.level
+        V, E = C.c_int(), C.c_int()
+        lib.orc_bp_level_size(self.h, l, C.byref(V), C.byref(E))
+        V, E = V.value, E.value
+        planes = np.zeros(12 * V * 4, np.float32)
+        cost = np.zeros(12 * V, np.float32)
+        belief = np.zeros(12 * V, np.float32)
+        msg = np.zeros(E * 8, np.float32)
+        lib.orc_bp_get_state(self.h, planes, cost, belief, msg)
+        return dict(level=l, V=V, E=E, planes=planes.reshape(12, V, 4), cost=cost.reshape(12, V),
+                    belief=belief.reshape(12, V), msg=msg.reshape(E, 8))
+
+    def map(self):
+        disp = np.zeros((self.H, self.W), np.float32)
+        plane3 = np.zeros((self.H, self.W, 3), np.float32)
+        lib.orc_bp_map(self.h, disp, plane3)
+        return disp, plane3
+
+
+def plain_pair(left, right, D, alpha=1000.1, tau_s=0.6, min_level=2, bp_levels=0):
+    H, W = left.shape[:2]
+    disp = np.zeros((H, W), np.float32)
+    plane3 = np.zeros((H, W, 3), np.float32)
+    L = C.c_int()
+    ml = lib.orc_plain_pair(np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D, alpha,
+                            tau_s, min_level, bp_levels, disp, plane3, C.byref(L))
+    return disp, plane3, L.value, ml

@@ -1,0 +1,272 @@
+"""GPU parity: the HIP path (through the C ABI of libhbp_hip.so) against the oracle on the same
+seeded synthetic pairs.  Integer/byte outputs must be bit-exact; float outputs are compared bit
+for bit as well (the pinned orders make them deterministic), NaNs matching NaNs."""
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from hbp_amd import lib as H
+from hbp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same_bits(a, b):
+    """float arrays equal bit for bit, any NaN equal to any NaN"""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a.view(np.uint32)[~na], b.view(np.uint32)[~nb])
+
+
+def mismatch(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    both = np.isnan(a) & np.isnan(b)
+    return (a.view(np.uint32) != b.view(np.uint32)) & ~both
+
+
+# ---------------------------------------------------------------- (a)
+COST_CASES = [(1, 64, 48, 16), (2, 37, 23, 5), (3, 450, 375, 64), (4, 130, 70, 100), (5, 1030, 40, 256)]
+
+
+@pytest.mark.parametrize("seed,W,H,D", COST_CASES)
+@pytest.mark.parametrize("right_view", [False, True])
+def test_cost_volume_bit_exact(ctx, seed, W, H, D, right_view):
+    say(f"cost volume seed={seed} {W}x{H} D={D} right={right_view}")
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 6))
+    vo, wo = O.cost_volume(left, right, D, right_view)
+    v = ctx.cost_volume(left, right, D, right_view)
+    vg, wg = v.download()
+    assert np.array_equal(vg, vo)
+    assert np.array_equal(wg, wo)
+
+
+def test_cost_volume_large_pair(ctx):
+    say("cost volume 1472x984 D=256")
+    left, right, _ = synth.make_pair(11, 1472, 984, 256, (20, 40))
+    vo, wo = O.cost_volume(left, right, 256)
+    v = ctx.cost_volume(left, right, 256)
+    vg, wg = v.download()
+    assert np.array_equal(vg, vo)
+    assert np.array_equal(wg, wo)
+
+
+def test_cost_volume_errors(ctx):
+    img = np.zeros((10, 8, 3), np.uint8)
+    with pytest.raises(H.HbpError):
+        ctx.cost_volume(img, img, 4)          # narrower than the census window
+    img = np.zeros((20, 20, 3), np.uint8)
+    with pytest.raises(H.HbpError):
+        ctx.cost_volume(img, img, 21)         # D > W
+
+
+# ---------------------------------------------------------------- (b)
+PYR_CASES = [(1, 64, 48), (2, 37, 23), (6, 120, 90), (3, 450, 375)]
+
+
+@pytest.mark.parametrize("seed,W,H", PYR_CASES)
+def test_pyramid_levels_bit_exact(ctx, seed, W, H):
+    say(f"pyramid seed={seed} {W}x{H}")
+    left, _, _ = synth.make_pair(seed, W, H, 16, (3, 8))
+    po = O.Pyramid(left)
+    pg = ctx.pyramid(left)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+
+
+def test_pyramid_large_pair(ctx):
+    say("pyramid 1472x984")
+    left, _, _ = synth.make_pair(11, 1472, 984, 256, (20, 40))
+    po = O.Pyramid(left)
+    pg = ctx.pyramid(left)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+
+
+def _hip():
+    return C.CDLL("libamdhip64.so")
+
+
+def test_crtrees_device_api_matches_literal_model(ctx):
+    """CRTrees::get_clus / compact_clus_label on device arrays (CRTrees.hpp:16-78) against the
+    oracle's literal model, on random functional graphs small (one block) and large (grid)."""
+    hip = _hip()
+    rng = np.random.default_rng(5)
+    for V in (7, 300, 5000, 20000, 200000):
+        nn = rng.integers(0, V, V).astype(np.int32)
+        say(f"crtrees V={V}")
+        ref = np.zeros(V, np.int32)
+        O.lib.orc_crtrees_literal(nn, V, 0, ref)
+        d_nn, d_cl = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(d_nn), C.c_size_t(4 * V)) == 0
+        assert hip.hipMalloc(C.byref(d_cl), C.c_size_t(4 * V)) == 0
+        try:
+            assert hip.hipMemcpy(d_nn, nn.ctypes.data_as(C.c_void_p), C.c_size_t(4 * V), 1) == 0
+            assert ctx.lib.hbp_crtrees_get_clus(ctx.h, d_nn, d_cl, V) == 0, ctx.lib.hbp_last_error()
+            out = np.zeros(V, np.int32)
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d_cl, C.c_size_t(4 * V), 2) == 0
+            assert np.array_equal(out, ref)
+            n = C.c_int()
+            assert ctx.lib.hbp_crtrees_compact(ctx.h, d_cl, V, C.byref(n)) == 0
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d_cl, C.c_size_t(4 * V), 2) == 0
+            refc = ref.copy()
+            assert n.value == O.lib.orc_compact(refc, V)
+            assert np.array_equal(out, refc)
+            nn2 = np.zeros(V, np.int32)
+            assert hip.hipMemcpy(nn2.ctypes.data_as(C.c_void_p), d_nn, C.c_size_t(4 * V), 2) == 0
+            assert np.array_equal(nn2, nn)          # nn is left unchanged
+        finally:
+            hip.hipFree(d_nn)
+            hip.hipFree(d_cl)
+
+
+# ---------------------------------------------------------------- (c)
+def _bp_pair(seed, W, H, D, planes=(3, 8)):
+    left, right, _ = synth.make_pair(seed, W, H, D, planes)
+    return left, right
+
+
+@pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2)])
+def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml):
+    say(f"bp state seed={seed} {W}x{H} D={D} min_level={ml}")
+    left, right = _bp_pair(seed, W, H, D)
+    vo, wo = O.cost_volume(left, right, D)
+    po = O.Pyramid(left)
+    bo = O.BP(po, vo, wo, D, min_level=ml)
+    vg = ctx.cost_volume(left, right, D)
+    pg = ctx.pyramid(left)
+    bg = ctx.bp(pg, vg, min_level=ml)
+    assert bg.total_sweeps == O.lib.orc_bp_total_sweeps(po.levels, ml, D)
+    for sweep in range(12):
+        assert bo.step(1) == 1 and bg.step(1) == 1
+        so, sg = bo.state(), bg.state()
+        assert so["level"] == sg["level"]
+        assert same_bits(so["cost"], sg["cost"]), f"cost differs after sweep {sweep}"
+        assert same_bits(so["belief"], sg["belief"]), f"belief differs after sweep {sweep}"
+        assert same_bits(so["planes"][:8], sg["planes"][:8]), f"planes differ after sweep {sweep}"
+        assert same_bits(so["msg"], sg["msg"]), f"messages differ after sweep {sweep}"
+
+
+@pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2),
+                                           (8, 160, 120, 48, 3)])
+def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
+    say(f"bp full seed={seed} {W}x{H} D={D} min_level={ml}")
+    left, right = _bp_pair(seed, W, H, D)
+    vo, wo = O.cost_volume(left, right, D)
+    po = O.Pyramid(left)
+    bo = O.BP(po, vo, wo, D, min_level=ml)
+    bo.step()
+    do, plo = bo.map()
+    vg = ctx.cost_volume(left, right, D)
+    pg = ctx.pyramid(left)
+    bg = ctx.bp(pg, vg, min_level=ml)
+    dg, plg = bg.run()
+    assert same_bits(do, dg), f"{mismatch(do, dg).mean():.4%} of disparities differ"
+    assert same_bits(plo, plg)
+
+
+def test_bp_use_cost_vol_false(ctx):
+    say("bp use_cost_vol=0")
+    left, right = _bp_pair(3, 80, 60, 16)
+    vo, wo = O.cost_volume(left, right, 16)
+    po = O.Pyramid(left)
+    bo = O.BP(po, vo, wo, 16, min_level=2, use_cost_vol=0, alpha=1.3)
+    bo.step()
+    do, plo = bo.map()
+    bg = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, 16), alpha=1.3, min_level=2, use_cost_vol=0)
+    dg, plg = bg.run()
+    assert same_bits(do, dg) and same_bits(plo, plg)
+
+
+def test_plain_path_cfg1_bit_exact(ctx):
+    """450x375, D=64, reference defaults (alpha 1000.1, tau_s 0.6, min_level 2)."""
+    say("plain path 450x375 D=64")
+    left, right, gt = synth.make_pair(1, **{k: v for k, v in synth.PLAIN_SMALL.items()})
+    t = time.time()
+    do, plo, L, ml = O.plain_pair(left, right, 64)
+    say(f"  oracle {time.time() - t:.1f}s levels={L}")
+    vg = ctx.cost_volume(left, right, 64)
+    pg = ctx.pyramid(left)
+    t = time.time()
+    dg, plg = ctx.bp(pg, vg, min_level=2).run()
+    say(f"  gpu bp {time.time() - t:.3f}s")
+    frac = mismatch(do, dg).mean()
+    say(f"  disparity mismatch fraction {frac:.6f}")
+    assert frac == 0.0
+    assert same_bits(plo, plg)
+
+
+def test_bp_is_deterministic(ctx):
+    left, right = _bp_pair(4, 120, 90, 32)
+    vg = ctx.cost_volume(left, right, 32)
+    pg = ctx.pyramid(left)
+    d1, p1 = ctx.bp(pg, vg, min_level=2).run()
+    d2, p2 = ctx.bp(pg, vg, min_level=2).run()
+    assert same_bits(d1, d2) and same_bits(p1, p2)
+
+
+def test_bp_rejects_bad_min_level(ctx):
+    left, right = _bp_pair(4, 64, 48, 16)
+    vg = ctx.cost_volume(left, right, 16)
+    pg = ctx.pyramid(left)
+    with pytest.raises(H.HbpError):
+        ctx.bp(pg, vg, min_level=0)
+    with pytest.raises(H.HbpError):
+        ctx.bp(pg, vg, min_level=pg.levels)
+
+
+# ---------------------------------------------------------------- (d)
+@pytest.mark.parametrize("seed,W,H,D", [(1, 64, 48, 16), (5, 120, 90, 32)])
+def test_left_right_check_and_fill_bit_exact(ctx, seed, W, H, D):
+    say(f"lr check seed={seed} {W}x{H}")
+    left, right = _bp_pair(seed, W, H, D)
+    # left view
+    vo, wo = O.cost_volume(left, right, D)
+    bol = O.BP(O.Pyramid(left), vo, wo, D, min_level=2)
+    bol.step()
+    dl, pl = bol.map()
+    # right view: pyramid of the right image, right-view volume
+    vr, wr = O.cost_volume(left, right, D, right_view=True)
+    bor = O.BP(O.Pyramid(right), vr, wr, D, min_level=2)
+    bor.step()
+    dr, pr = bor.map()
+    # GPU right view
+    gr = ctx.bp(ctx.pyramid(right), ctx.cost_volume(left, right, D, right_view=True), min_level=2)
+    dgr, pgr = gr.run()
+    assert same_bits(dr, dgr) and same_bits(pr, pgr)
+    # oracle check + fill
+    plo = pl.copy()
+    fl, fr, f2 = (np.zeros((H, W), np.uint8) for _ in range(3))
+    O.lib.orc_left_right_check(dl, dr, plo, pr.copy(), W, H, fl, fr, f2)
+    dispo = np.zeros((H, W), np.float32)
+    O.lib.orc_plane_to_disp(plo, W, H, dispo)
+    plg, flg, frg, f2g = ctx.lr_check_fill(dl, dr, pl)
+    assert np.array_equal(fl, flg) and np.array_equal(fr, frg) and np.array_equal(f2, f2g)
+    assert same_bits(plo, plg)
+    assert same_bits(dispo, ctx.plane_to_disp(plg))

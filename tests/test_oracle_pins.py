@@ -1,0 +1,171 @@
+"""CPU tests: pin the oracle's scalar rules and stage (a) against libm, closed forms and
+properties a plausible mistake would break (wrong sign, swapped operands, wrong offsets)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from hbp_amd import synth
+
+L = O.lib
+
+
+def ulp_diff(a, b):
+    ia = np.frombuffer(np.float64(a).tobytes(), np.int64)[0]
+    ib = np.frombuffer(np.float64(b).tobytes(), np.int64)[0]
+    return abs(int(ia) - int(ib))
+
+
+def test_pin_cbrt_matches_libm_within_one_ulp():
+    eps = 216 / 24389.0
+    xs = np.concatenate([np.linspace(eps * 1.0000001, 1.0, 20001), [1.0, 0.5, 0.125, 0.027]])
+    worst = max(ulp_diff(L.orc_pin_cbrt(float(x)), np.cbrt(x)) for x in xs)
+    assert worst <= 1
+    assert L.orc_pin_cbrt(0.125) == 0.5
+    assert L.orc_pin_cbrt(1.0) == 1.0
+
+
+def test_pin_exp_matches_libm_within_one_ulp():
+    xs = np.concatenate([np.linspace(-40.0, 0.0, 20001), [0.0, -1.0, -0.5, -30.6, -1e-9]])
+    worst = max(ulp_diff(L.orc_pin_exp(float(x)), math.exp(x)) for x in xs)
+    assert worst <= 1
+    assert L.orc_pin_exp(0.0) == 1.0
+    assert L.orc_pin_exp(-800.0) == 0.0
+
+
+def test_i64_to_f32_is_round_to_nearest_even():
+    rng = np.random.default_rng(0)
+    vals = list(rng.integers(-(1 << 62), 1 << 62, 20000)) + [0, 1, -1, (1 << 24) + 1, (1 << 24) + 3,
+                                                              (1 << 25) + 2, (1 << 25) + 6, -((1 << 24) + 1),
+                                                              (1 << 62) - 1, 16777217, 33554435]
+    for v in vals:
+        assert np.float32(L.orc_i64_to_f32(int(v))) == np.float32(np.int64(v)), v
+
+
+def test_floor_to_int_semantics():
+    assert L.orc_f2i_rd(float("nan")) == 0
+    assert L.orc_f2i_rd(float("inf")) == 2**31 - 1
+    assert L.orc_f2i_rd(float("-inf")) == -(2**31)
+    assert L.orc_f2i_rd(3e9) == 2**31 - 1
+    assert L.orc_f2i_rd(-0.5) == -1
+    assert L.orc_f2i_rd(2.999999) == 2
+    assert L.orc_f2i_rd(-3.0) == -3
+
+
+def test_fixed_point_shift_bounds_the_worst_sum():
+    for lam, tau, ucv, n in [(1000.1, 25.0, 1, 168750), (1.3, 25.0, 1, 1448448), (1e6, 25.0, 1, 8294400),
+                             (0.05, 25.0, 0, 100)]:
+        s = L.orc_fixed_shift(lam, tau, ucv, n)
+        vmax = abs(lam) * (tau if ucv else 1.0) * 1.0001
+        assert vmax * n * 2.0**s < 2.0**62.5
+        assert vmax * n * 2.0**s > 2.0**58   # not needlessly coarse
+
+
+def test_amin_fold_is_min_for_ordinary_values():
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        v = rng.normal(size=4).astype(np.float32) * 100
+        m = np.float32(np.finfo(np.float32).max)
+        for x in v:
+            m = L.orc_amin_fold(float(m), float(x))
+        assert np.float32(m) == v.min()
+
+
+def test_amin_fold_nan_rules():
+    fm = float(np.finfo(np.float32).max)
+    nan = float("nan")
+    # positive value after NaN replaces it (signed-int min), NaN after a value replaces it
+    # (unsigned max), so the fold is order dependent exactly as the atomics are
+    assert L.orc_amin_fold(L.orc_amin_fold(fm, nan), 2.0) == 2.0
+    assert math.isnan(L.orc_amin_fold(L.orc_amin_fold(fm, 1.0), nan))
+    assert L.orc_amin_fold(L.orc_amin_fold(fm, -3.0), nan) == -3.0
+
+
+def test_grey_closed_form():
+    bgr = np.array([[[0, 0, 0], [255, 0, 0], [0, 255, 0], [0, 0, 255], [255, 255, 255], [10, 20, 30]]],
+                   np.uint8).reshape(1, 6, 3)
+    g = np.zeros(6, np.uint8)
+    L.orc_grey(np.ascontiguousarray(bgr), 6, 1, g)
+    # 0.114 B + 0.587 G + 0.299 R, truncated: 29.07 -> 29, 149.685 -> 149, 76.245 -> 76,
+    # 1.14 + 11.74 + 8.97 = 21.85 -> 21; white sums to 255.
+    assert list(g) == [0, 29, 149, 76, 255, 21]
+
+
+def census(grey):
+    H, W = grey.shape
+    out = np.zeros((H, W), np.uint32)
+    L.orc_census(np.ascontiguousarray(grey), W, H, out)
+    return out
+
+
+def test_census_constant_image_interior_is_all_ones():
+    c = census(np.full((20, 30), 77, np.uint8))
+    assert np.all(c[3:-3, 4:-4] == 0x7FFFFFFF)
+    # at the top-left corner the zero padding wins every comparison that involves it
+    assert c[0, 0] != 0x7FFFFFFF
+
+
+def test_census_ramp_bits():
+    # I(x, y) = x + 10: for pairs (x-4+m, x+4-m) the comparison holds iff m >= 4, so the
+    # three full rows set bits m = 4..8 and the centre row (m < 4) sets none
+    g = np.tile(np.arange(40, dtype=np.uint8) + 10, (20, 1))
+    c = census(g)[10, 20]
+    expect = 0
+    for k in range(3):
+        expect |= 0x1F << (k * 9 + 4)
+    assert c == expect
+    # vertical ramp: I = y; rows k < 3 compare y-3+k >= y+3-k -> false
+    g2 = np.tile((np.arange(20, dtype=np.uint8) + 10)[:, None], (1, 40))
+    c2 = census(g2)[10, 20]
+    assert c2 == (0xF << 27)
+
+
+def test_hamming_direction_and_border_rule():
+    rng = np.random.default_rng(3)
+    H, W, D, k = 16, 48, 12, 5
+    left = rng.integers(0, 255, (H, W, 3), dtype=np.uint8)
+    right = np.zeros_like(left)
+    right[:, : W - k] = left[:, k:]           # right(x) = left(x + k): true disparity k
+    vol, wta = O.cost_volume(left, right, D)
+    assert np.all(vol[k][3:-3, k + 4: W - 4] == 0)
+    assert np.all(wta[3:-3, k + 8: W - 8] == k)
+    # x < d reads CR(0)
+    gl = np.zeros((H, W), np.uint8)
+    gr = np.zeros((H, W), np.uint8)
+    L.orc_grey(np.ascontiguousarray(left), W, H, gl)
+    L.orc_grey(np.ascontiguousarray(right), W, H, gr)
+    cl, cr = census(gl), census(gr)
+    for d in range(1, D):
+        for x in range(d):
+            pc = np.array([bin(int(a) ^ int(b)).count("1") for a, b in zip(cl[:, x], cr[:, 0])])
+            assert np.array_equal(vol[d, :, x], np.minimum(pc, 25))
+
+
+def test_wta_ties_take_smallest_d():
+    D, H, W = 6, 1, 3
+    vol = np.full((D, H, W), 9, np.uint8)
+    vol[2, 0, 0] = vol[4, 0, 0] = 1
+    vol[:, 0, 1] = 5
+    vol[5, 0, 2] = 0
+    wta = np.zeros((H, W), np.float32)
+    L.orc_wta(vol, W, H, D, wta)
+    assert list(wta[0]) == [2.0, 0.0, 5.0]
+
+
+def test_volume_to_right_closed_form():
+    left, right, _ = synth.make_pair(7, 40, 20, 10, (2, 3))
+    vl, _ = O.cost_volume(left, right, 10)
+    vr, wr = O.cost_volume(left, right, 10, right_view=True)
+    W = 40
+    for d in range(10):
+        idx = np.minimum(np.arange(W) + d, W - 1)
+        assert np.array_equal(vr[d], vl[d][:, idx])
+    assert np.array_equal(wr, np.argmin(vr, axis=0).astype(np.float32))
+
+
+def test_cost_volume_rejects_bad_shapes():
+    img = np.zeros((10, 8, 3), np.uint8)
+    vol = np.zeros((4, 10, 8), np.uint8)
+    wta = np.zeros((10, 8), np.float32)
+    assert L.orc_cost_volume(img, img, 8, 10, 4, 0, vol, wta) == -1   # W < 9
