@@ -809,6 +809,11 @@ static void launch_plane_cost(Ctx &c, BP &b, BPLevel &B, int slot0) {
     constexpr int CH = 256 * R;
     int grid = div_up(b.N, CH);
     const Volume &v = *b.vol;
+    // algorithmic bytes (pixel id 4 B + 2 cost bytes per plane and pixel; per superpixel the
+    // planes read, the sums and counts written)
+    double bytes = (double)b.N * (4 + 2 * NS) + (double)B.V * (16 * NS + 8 + 4 * NS);
+    int lvl = (int)(&B - &b.lv[0]);
+    ProfScope ps(c, "plane_cost_L" + std::to_string(lvl), bytes);
     hipLaunchKernelGGL((plane_cost_kernel<NS, R>), dim3(grid), dim3(256), 0, c.stream, B.pix.p, b.N, slot0,
                        B.plane.p, v.vol.p, v.pitch, v.plane, b.D, v.dwta.p, b.W, b.lambda, b.tau, b.use_cost_vol,
                        b.fx_shift, b.acc.p, b.accnan.p);
@@ -827,14 +832,19 @@ static void run_sweep(Ctx &c, BP &b, const Sweep &sw, int next_kind, float ndz, 
         hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, B.V)), dim3(256), 0, c.stream, B.V, B.cost.p,
                            B.belief.p, b.acc.p, b.accnan.p, b.fx_shift);
     } else {
-        if (sw.kind == kSpatial)
+        if (sw.kind == kSpatial) {
+            ProfScope pss(c, "spatial_L" + std::to_string(l), (16.0 * P + 28) * B.V);
             hipLaunchKernelGGL(spatial_kernel, dim3(grid_for(c, B.V)), dim3(256), 0, c.stream, a, b.s_neighbor.p,
                                b.s_child.p);
+        }
         launch_plane_cost<4, 8>(c, b, B, 2 * P);
     }
-    if (B.E > 0)
+    if (B.E > 0) {
+        ProfScope ps(c, "edge_L" + std::to_string(l), 96.0 * B.E + (16.0 * K + 32 + 16) * B.V);
         hipLaunchKernelGGL(edge_kernel, dim3(div_up(B.E, 16)), dim3(256), 0, c.stream, B.E, B.bd, B.c0.p, B.dpos.p,
                            B.plane.p, B.belief.p, B.point.p, b.msg.p, b.tau_s, b.msg_t.p);
+    }
+    ProfScope psn(c, "node_L" + std::to_string(l), 84.0 * B.E + (32.0 * K + 16.0 * K + 16) * B.V);
     hipLaunchKernelGGL(node_kernel, dim3(grid_for(c, B.V, 128)), dim3(128), 0, c.stream, a, b.msg_t.p, b.msg.p,
                        b.acc.p, b.accnan.p, b.fx_shift, sw.kind, next_kind, ndz, ndn, v.dwta.p, b.s_ransac.p,
                        b.s_neighbor.p, b.s_child.p);

@@ -145,6 +145,45 @@ int hbp_ctx_sync(hbp_ctx *ctx) {
 void *hbp_ctx_stream(hbp_ctx *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
 long long hbp_ctx_pool_bytes(hbp_ctx *ctx) { return ctx ? (long long)ctx->c.pool.bytes_reserved() : 0; }
 
+int hbp_profile_enable(hbp_ctx *ctx, int on) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx, "ctx is NULL");
+    ctx->c.prof.flush();
+    ctx->c.prof.on = on != 0;
+    HBP_API_END
+}
+
+int hbp_profile_reset(hbp_ctx *ctx) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx, "ctx is NULL");
+    ctx->c.prof.reset();
+    HBP_API_END
+}
+
+int hbp_profile_read(hbp_ctx *ctx, const char *kernel, double *ms, long long *launches, double *bytes) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && kernel, "NULL argument");
+    ctx->c.prof.flush();
+    KernelStat s;
+    for (size_t i = 0; i < ctx->c.prof.names.size(); ++i)
+        if (ctx->c.prof.names[i] == kernel) s = ctx->c.prof.stats[i];
+    if (ms) *ms = s.ms;
+    if (launches) *launches = s.launches;
+    if (bytes) *bytes = s.bytes;
+    HBP_API_END
+}
+
+int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen) {
+    if (!ctx) return -1;
+    std::string all;
+    for (size_t i = 0; i < ctx->c.prof.names.size(); ++i) all += (i ? "," : "") + ctx->c.prof.names[i];
+    if (buf && buflen > 0) {
+        std::strncpy(buf, all.c_str(), (size_t)buflen - 1);
+        buf[buflen - 1] = 0;
+    }
+    return (int)all.size() + 1;
+}
+
 // ---------------------------------------------------------------- (a)
 static void check_algo(const char *algo) {
     HBP_REQUIRE(algo && std::strcmp(algo, "Census") == 0,

@@ -167,6 +167,7 @@ __global__ void volume_to_f32_kernel(const uint8_t *__restrict__ vol, int W, int
 // ----------------------------------------------------------------------------------------
 void census_launch(Ctx &c, const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, uint32_t *cl, uint32_t *cr) {
     dim3 grid(div_up(W, CT_W), div_up(H, CT_H));
+    ProfScope ps(c, "census", 14.0 * W * H);
     hipLaunchKernelGGL(census_kernel, grid, dim3(256), 0, c.stream, bgrL, bgrR, W, H, cl, cr);
     HBP_CHECK(hipGetLastError());
 }
@@ -175,6 +176,8 @@ void hamming_launch(Ctx &c, Volume &v, bool right) {
     dim3 grid(div_up(v.pitch, HM_SEG), v.H);
     int nwin = HM_SEG + v.D - 1;
     size_t lds = sizeof(uint32_t) * (size_t)nwin * (right ? 2 : 1);
+    // algorithmic bytes: census words in (8N), the u8 volume out (D*N), WTA out (4N)
+    ProfScope ps(c, right ? "hamming_wta_right" : "hamming_wta", (double)(8 + v.D + 4) * v.W * v.H);
     if (right)
         hipLaunchKernelGGL(hamming_wta_kernel<true>, grid, dim3(HM_THREADS), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
                            v.D, v.pitch, v.plane, v.vol.p, v.dwta.p);

@@ -194,8 +194,69 @@ struct DevBuf {
     }
 };
 
+// Per-kernel timing with HIP events recorded on the context's stream around selected
+// launches, with the launch's algorithmic HBM bytes; off unless enabled through the C ABI.
+struct KernelStat {
+    double ms = 0.0, bytes = 0.0;
+    long long launches = 0;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<std::string> names;
+    std::vector<KernelStat> stats;
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    int id_of(const std::string &n) {
+        for (size_t i = 0; i < names.size(); ++i)
+            if (names[i] == n) return (int)i;
+        names.push_back(n);
+        stats.emplace_back();
+        return (int)names.size() - 1;
+    }
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void flush() {
+        for (auto &p : pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                stats[p.id].ms += ms;
+                stats[p.id].bytes += p.bytes;
+                stats[p.id].launches += 1;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        flush();
+        for (auto &s : stats) s = KernelStat();
+    }
+    ~Profiler() {
+        for (auto &p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 struct Ctx {
     int device = 0;
+    Profiler prof;
     hipStream_t stream = nullptr;
     Pool pool;
     int num_cu = 256;
@@ -208,6 +269,31 @@ struct Ctx {
         if (scratch.n < bytes) scratch.alloc(pool, bytes + (bytes >> 3));
         return scratch.p;
     }
+};
+
+// Times the launches issued while it is alive (when profiling is on).
+class ProfScope {
+public:
+    ProfScope(Ctx &c, const std::string &name, double bytes) : c_(c) {
+        if (!c.prof.on) return;
+        id_ = c.prof.id_of(name);
+        a_ = c.prof.take();
+        b_ = c.prof.take();
+        bytes_ = bytes;
+        if (a_ && b_) (void)hipEventRecord(a_, c.stream);
+    }
+    ~ProfScope() {
+        if (!c_.prof.on || !a_ || !b_) return;
+        (void)hipEventRecord(b_, c_.stream);
+        c_.prof.pending.push_back({id_, a_, b_, bytes_});
+        if (c_.prof.pending.size() > 4096) c_.prof.flush();
+    }
+
+private:
+    Ctx &c_;
+    int id_ = -1;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+    double bytes_ = 0.0;
 };
 
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }

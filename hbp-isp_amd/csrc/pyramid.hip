@@ -421,6 +421,7 @@ void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label) {
     s.flags = buf.p + 8 * (size_t)V;
     s.status = buf.p + 8 * (size_t)V + 4;
     HBP_CHECK(hipMemsetAsync(s.status, 0, sizeof(int), c.stream));
+    ProfScope ps(c, "crtrees", 16.0 * V);
     if (V <= 8192) {
         hipLaunchKernelGGL(crtrees_kernel<false>, dim3(1), dim3(256), 0, c.stream, d_nn, V, s, d_label);
         HBP_CHECK(hipGetLastError());

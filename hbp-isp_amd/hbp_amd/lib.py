@@ -34,6 +34,11 @@ SIGNATURES = {
     "hbp_ctx_stream": (_vp, [_vp]),
     "hbp_last_error": (C.c_char_p, []),
     "hbp_ctx_pool_bytes": (C.c_longlong, [_vp]),
+    "hbp_profile_enable": (C.c_int, [_vp, C.c_int]),
+    "hbp_profile_reset": (C.c_int, [_vp]),
+    "hbp_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_double)]),
+    "hbp_profile_names": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "hbp_cost_volume": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_volume_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_to_right": (C.c_int, [_vp, _vp]),
@@ -119,6 +124,24 @@ class Ctx:
 
     def sync(self):
         _check(self.lib.hbp_ctx_sync(self.h))
+
+    # ---- kernel timing (HIP events on the context stream)
+    def profile(self, on=True):
+        _check(self.lib.hbp_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        _check(self.lib.hbp_profile_reset(self.h))
+
+    def profile_stats(self):
+        n = self.lib.hbp_profile_names(self.h, None, 0)
+        buf = C.create_string_buffer(max(n, 1))
+        self.lib.hbp_profile_names(self.h, buf, n)
+        out = {}
+        for name in filter(None, buf.value.decode().split(",")):
+            ms, cnt, by = C.c_double(), C.c_longlong(), C.c_double()
+            _check(self.lib.hbp_profile_read(self.h, name.encode(), C.byref(ms), C.byref(cnt), C.byref(by)))
+            out[name] = dict(ms=ms.value, launches=cnt.value, bytes=by.value)
+        return out
 
     # ---- (a)
     def cost_volume(self, left, right, D, right_view=False):

@@ -32,6 +32,15 @@ int hbp_ctx_sync(hbp_ctx *ctx);
 void *hbp_ctx_stream(hbp_ctx *ctx); /* hipStream_t of the context */
 const char *hbp_last_error(void);
 long long hbp_ctx_pool_bytes(hbp_ctx *ctx); /* device bytes reserved by the context's pool */
+/* kernel timing with HIP events on the context's stream (no counterpart in the reference, which
+ * times stages with host clocks, main_stereo_hbp_isp.cu:36-72): enable / reset, then read the
+ * summed event time, launch count and algorithmic HBM bytes of one kernel name; names lists the
+ * recorded names separated by ',' (returns the length needed) */
+int hbp_profile_enable(hbp_ctx *ctx, int on);
+int hbp_profile_reset(hbp_ctx *ctx);
+int hbp_profile_read(hbp_ctx *ctx, const char *kernel, double *total_ms, long long *launches,
+                     double *bytes);
+int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen);
 
 /* ---- (a) cost volume: StereoCost(imgL, imgR, labels, vol_dir) + compute_stereo_cost("Census")
  *      (stereo_cost.hpp:90-116, stereo_cost.cu:421-445).  algo must be "Census". */

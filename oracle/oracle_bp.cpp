@@ -10,6 +10,9 @@
 #include <cstring>
 #include <numeric>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "hbp_oracle.h"
 #include "oracle_math.h"
@@ -158,23 +161,64 @@ static inline float plane_px_cost(const BP &b, const F4 &pl, int pxid) {
 }
 
 // compute_plane_cost (hbp_isp.cu:1159-1174) with read_plane_cost (177-228); cluster sums in
-// int64 fixed point.
+// int64 fixed point.  The sorted pixel list is cut into equal ranges, one per thread; a thread
+// folds runs of one cluster locally and adds each run to the cluster's total atomically (the
+// sums are exact integers, so the split does not change them).
 static void plane_cost(BP &b, int l, int s_c) {
     BPLevel &B = b.lv[l];
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int c = 0; c < B.V; ++c) {
-        int64_t acc[P] = {0, 0, 0, 0};
-        int nan[P] = {0, 0, 0, 0};
-        for (int j = B.pix_off[c]; j < B.pix_off[c + 1]; ++j) {
+    std::vector<int64_t> acc((size_t)B.V * P, 0);
+    std::vector<int> nanf((size_t)B.V * P, 0);
+    std::vector<int> cl_of(b.N);
+    for (int c = 0; c < B.V; ++c)
+        for (int j = B.pix_off[c]; j < B.pix_off[c + 1]; ++j) cl_of[j] = c;
+    const long N = b.N;
+#pragma omp parallel
+    {
+        int nt = 1, t = 0;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+        t = omp_get_thread_num();
+#endif
+        long j0 = N * t / nt, j1 = N * (t + 1) / nt;
+        int crun = -1;
+        int64_t racc[P] = {0, 0, 0, 0};
+        int rnan[P] = {0, 0, 0, 0};
+        F4 pl[P];
+        auto flush = [&]() {
+            if (crun < 0) return;
+            for (int p = 0; p < P; ++p) {
+                int64_t v = racc[p];
+#pragma omp atomic
+                acc[(size_t)crun * P + p] += v;
+                if (rnan[p]) {
+#pragma omp atomic write
+                    nanf[(size_t)crun * P + p] = 1;
+                }
+            }
+        };
+        for (long j = j0; j < j1; ++j) {
+            int c = cl_of[j];
+            if (c != crun) {
+                flush();
+                crun = c;
+                for (int p = 0; p < P; ++p) {
+                    racc[p] = 0;
+                    rnan[p] = 0;
+                    pl[p] = B.plane[(long)(s_c + p) * B.V + c];
+                }
+            }
             int pxid = B.px[j];
             for (int p = 0; p < P; ++p) {
-                float v = plane_px_cost(b, B.plane[(long)(s_c + p) * B.V + c], pxid);
-                if (v != v) nan[p] = 1;
-                else acc[p] += to_fixed(v, b.fx_shift);
+                float v = plane_px_cost(b, pl[p], pxid);
+                if (v != v) rnan[p] = 1;
+                else racc[p] += to_fixed(v, b.fx_shift);
             }
         }
-        for (int p = 0; p < P; ++p) B.cost[(long)(s_c + p) * B.V + c] = from_fixed(acc[p], nan[p], b.fx_shift);
+        flush();
     }
+    for (int c = 0; c < B.V; ++c)
+        for (int p = 0; p < P; ++p)
+            B.cost[(long)(s_c + p) * B.V + c] = from_fixed(acc[(size_t)c * P + p], nanf[(size_t)c * P + p], b.fx_shift);
 }
 
 // init_bp (hbp_isp.cu:930-950) with copy_prior_plane (56-73) and init_rand_seed_k (570-591).
