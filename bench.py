@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
+    ap.add_argument("--split", action="store_true",
+                    help="time the sweeps as one kernel per phase instead of one launch per level")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stereo path runs one pair per GPU and this build has "
@@ -69,13 +71,14 @@ def main():
 
     info = {}
 
-    def one_pair():
+    def one_pair(fused=not args.split):
         vol, pyr, bp = C.c_void_p(), C.c_void_p(), C.c_void_p()
         H._check(lib.hbp_cost_volume_dev(ctx.h, d_l, d_r, W, Hh, D, b"Census", C.byref(vol)))
         H._check(lib.hbp_pyramid_build_dev(ctx.h, d_l, W, Hh, 2, 4, 0.0, C.byref(pyr)))
         L = lib.hbp_pyramid_levels(pyr)
         ml = max(1, L - 1 - wl["bp_levels"]) if wl["bp_levels"] > 0 else 2
         H._check(lib.hbp_bp_create(ctx.h, pyr, vol, 1000.1, 0.6, ml, 1, C.byref(bp)))
+        H._check(lib.hbp_bp_set_fused(bp, int(fused)))
         info.update(levels=L, min_level=ml, sweeps=lib.hbp_bp_total_sweeps(bp))
         H._check(lib.hbp_bp_run(ctx.h, bp, None, None))
         lib.hbp_bp_free(ctx.h, bp)
@@ -98,6 +101,11 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     stats = ctx.profile_stats()
+    # one more pair with one kernel per BP phase, outside the timed region: per-phase breakdown
+    ctx.profile_reset()
+    one_pair(fused=False)
+    ctx.sync()
+    phase_stats = ctx.profile_stats()
     ctx.profile(False)
 
     pairs_per_s = args.steps / elapsed
@@ -123,13 +131,35 @@ def main():
                        gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
                for g, v in sorted(groups.items(), key=lambda kv: -kv[1]["ms"])}
 
+    phases = {}
+    for name, v in phase_stats.items():
+        g = name.split("_L")[0]
+        a = phases.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
+        a["ms"] += v["ms"]
+        a["bytes"] += v["bytes"]
+        a["launches"] += v["launches"]
+    phases = {g: dict(ms_per_pair=round(v["ms"], 3), launches=v["launches"],
+                      gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
+              for g, v in sorted(phases.items(), key=lambda kv: -kv[1]["ms"])}
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_plane_cost_large.json")
     if args.workload == "large" and os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
         except Exception:
-            traffic = None
+            phases = {}
+    for name, v in phase_stats.items():
+        g = name.split("_L")[0]
+        a = phases.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
+        a["ms"] += v["ms"]
+        a["bytes"] += v["bytes"]
+        a["launches"] += v["launches"]
+    phases = {g: dict(ms_per_pair=round(v["ms"], 3), launches=v["launches"],
+                      gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
+              for g, v in sorted(phases.items(), key=lambda kv: -kv[1]["ms"])}
+
+    traffic = None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -159,6 +189,7 @@ def main():
                      "traffic": traffic, "algorithmic_bytes_per_launch": round(avg_bytes),
                      "avg_launch_us": round(1e3 * avg_ms, 2), "launches": gd["launches"]},
         "kernels": kernels,
+        "phases_split_pass": phases,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -18,13 +18,22 @@
 // HBM layout per level: planes [V][12] float4, cost/belief [V][12] f32, messages [E][8] f32
 // (the 8 slots the next sweep reads), sweep messages [E][12] f32, pixels sorted by superpixel
 // as (packed x|y<<16, cluster) pairs, CSR offsets/reverse ids/edge weights per edge.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
 #include <cmath>
 
 #include "hbp_internal.h"
 
+namespace cg = cooperative_groups;
+
 namespace hbp {
 
 constexpr int P = kP, K = kK;
+
+// replicas of each fixed-point accumulator: chunk k adds into replica k mod rrep, which spreads
+// the atomics of the few large superpixels of the coarse levels over rrep addresses
+static inline int acc_replicas(int V) { return V <= 65536 ? 8 : 1; }
 
 // ----------------------------------------------------------------------------------------
 // prologue kernels
@@ -197,13 +206,11 @@ __device__ void random_node(int id, const float2 *__restrict__ mxy, float max_dz
 
 enum SweepKind { kNone = 0, kRansac = 1, kSpatial = 2, kRandom = 3 };
 
-// ----------------------------------------------------------------------------------------
-// level init: init_bp (hbp_isp.cu:930-950) with copy_prior_plane (56-73) and
-// init_rand_seed_k (570-591), plus the first sweep's RANSAC candidates.
-// ----------------------------------------------------------------------------------------
-struct LevelArgs {
+// Every pointer and constant one sweep reads or writes (level structures, parent level,
+// volume, parameters, state).  Passed by value to the kernels.
+struct SweepArgs {
     int V, Vh, W, D;
-    long E;
+    long E, N;
     const int *clus;
     const int2 *bd;
     const uint2 *pix;
@@ -219,89 +226,106 @@ struct LevelArgs {
     float *cost;
     float *belief;
     uint4 *point;
-    // parent level (spatial propagation)
+    // parent level (spatial propagation, priors)
     const int2 *bd_h;
     const int *off_h;
     const float4 *plane_h;
+    // cost volume and parameters
+    const uint8_t *vol;
+    int pitch;
+    long vplane;
+    const float *dwta;
+    float lambda, tau, tau_s;
+    int use_cost_vol, shift;
+    // state
+    float4 *msg;                 // [E][2] float4
+    float *msg_t;                // [E][12]
+    unsigned long long *acc;     // [V][rrep][12] fixed-point sums, rrep replicas
+    int rrep;
+    unsigned int *accnan;        // [V]
+    uint32_t *s_ransac, *s_neighbor, *s_child;
 };
 
-__global__ void init_level_kernel(LevelArgs a, const float *__restrict__ dwta, uint32_t *__restrict__ s_ransac,
-                                  uint32_t *__restrict__ s_neighbor, uint32_t *__restrict__ s_child,
-                                  unsigned long long *__restrict__ acc, unsigned int *__restrict__ accnan) {
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < a.V; id += (long)gridDim.x * blockDim.x) {
-        int cc = a.clus[id];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            float4 pl = a.plane_h[(long)cc * K + p];
-            a.plane[id * K + p] = pl;
-            a.plane[id * K + p + P] = pl;
-            a.plane[id * K + p + 2 * P] = pl;
-        }
-        s_neighbor[id] = (uint32_t)a.off[id];
-        s_ransac[id] = (uint32_t)a.pix_off[id];
-        if (id < a.Vh) s_child[id] = (uint32_t)a.child_off[id];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            acc[id * K + k] = 0ull;
-            a.belief[id * K + k] = 0.f;
-            a.cost[id * K + k] = 0.f;
-        }
-        accnan[id] = 0u;
-        ransac_node((int)id, a.pix, a.pix_off, dwta, a.W, s_ransac, a.plane, a.point);
-    }
+// the schedule of one sweep as the device reads it
+struct SweepDesc {
+    int kind, next_kind;
+    float ndz, ndn;
+};
+
+constexpr int CH12 = 256 * 2, CH4 = 256 * 8;                    // pixels per plane-cost chunk
+constexpr int MAXC12 = CH12 / 2 + 2, MAXC4 = CH4 / 2 + 2;        // superpixels a chunk can touch
+constexpr size_t PC_LDS_BYTES =
+    (MAXC12 * 12 * 8 + MAXC12 * 4) > (MAXC4 * 4 * 8 + MAXC4 * 4) ? (MAXC12 * 12 * 8 + MAXC12 * 4)
+                                                                 : (MAXC4 * 4 * 8 + MAXC4 * 4);
+
+__device__ __forceinline__ int64_t acc_sum(const SweepArgs &a, long id, int k) {
+    unsigned long long t = 0ull;
+    for (int r = 0; r < a.rrep; ++r) t += a.acc[(id * a.rrep + r) * K + k];
+    return (int64_t)t;
+}
+__device__ __forceinline__ void acc_zero(const SweepArgs &a, long id, int k) {
+    for (int r = 0; r < a.rrep; ++r) a.acc[(id * a.rrep + r) * K + k] = 0ull;
 }
 
-__global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+// init_bp (hbp_isp.cu:930-950): copy_prior_plane (56-73), init_rand_seed_k (570-591), plus the
+// first sweep's RANSAC candidates.
+__device__ void init_node(const SweepArgs &a, long id) {
+    int cc = a.clus[id];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        float4 pl = a.plane_h[(long)cc * K + p];
+        a.plane[id * K + p] = pl;
+        a.plane[id * K + p + P] = pl;
+        a.plane[id * K + p + 2 * P] = pl;
+    }
+    a.s_neighbor[id] = (uint32_t)a.off[id];
+    a.s_ransac[id] = (uint32_t)a.pix_off[id];
+    if (id < a.Vh) a.s_child[id] = (uint32_t)a.child_off[id];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        acc_zero(a, id, k);
+        a.belief[id * K + k] = 0.f;
+        a.cost[id * K + k] = 0.f;
+    }
+    a.accnan[id] = 0u;
+    ransac_node((int)id, a.pix, a.pix_off, a.dwta, a.W, a.s_ransac, a.plane, a.point);
 }
 
 // costs of slots 0..2P-1 after init (compute_plane_cost(0), (P)); belief = cost
-__global__ void init_cost_kernel(int V, float *__restrict__ cost, float *__restrict__ belief,
-                                 unsigned long long *__restrict__ acc, unsigned int *__restrict__ accnan, int shift) {
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < V; id += (long)gridDim.x * blockDim.x) {
-        unsigned int nb = accnan[id];
+__device__ void init_cost_node(const SweepArgs &a, long id) {
+    unsigned int nb = a.accnan[id];
 #pragma unroll
-        for (int k = 0; k < 2 * P; ++k) {
-            float c = from_fixed((int64_t)acc[id * K + k], (nb >> k) & 1u, shift);
-            cost[id * K + k] = c;
-            belief[id * K + k] = c;
-            acc[id * K + k] = 0ull;
-        }
-        accnan[id] = nb & ~0xffu;
+    for (int k = 0; k < 2 * P; ++k) {
+        float c = from_fixed(acc_sum(a, id, k), (nb >> k) & 1u, a.shift);
+        a.cost[id * K + k] = c;
+        a.belief[id * K + k] = c;
+        acc_zero(a, id, k);
     }
+    a.accnan[id] = nb & ~0xffu;
 }
 
 // ----------------------------------------------------------------------------------------
-// plane cost: read_plane_cost (hbp_isp.cu:177-228) + the per-superpixel sums.
-// Each 256-thread block takes CH = 256*R consecutive pixels of the sorted list; a lane sums
+// plane cost: read_plane_cost (hbp_isp.cu:177-228) + the per-superpixel sums, for one chunk
+// of CH = 256*R consecutive pixels of the sorted list (one 256-thread block).  A lane sums
 // its R consecutive pixels while the superpixel stays the same and flushes into LDS int64
 // counters indexed by superpixel - first superpixel of the chunk (superpixels at BP levels
 // hold >= 2 pixels, so a chunk touches at most CH/2 + 1 of them); the block then adds each
-// counter to the global sum with one atomic.
+// counter to replica (chunk mod rrep) of the global sum with one atomic.
 // ----------------------------------------------------------------------------------------
 template <int NS, int R>
-__global__ __launch_bounds__(256) void plane_cost_kernel(const uint2 *__restrict__ pix, long N, int slot0,
-                                                         const float4 *__restrict__ plane,
-                                                         const uint8_t *__restrict__ vol, int pitch, long vplane,
-                                                         int D, const float *__restrict__ dwta, int W, float lambda,
-                                                         float tau, int use_cost_vol, int shift,
-                                                         unsigned long long *__restrict__ acc,
-                                                         unsigned int *__restrict__ accnan) {
+__device__ void plane_cost_chunk(const SweepArgs &a, long chunk, int slot0, unsigned long long *sacc,
+                                 unsigned int *snan) {
     constexpr int CH = 256 * R;
     constexpr int MAXC = CH / 2 + 2;
-    __shared__ unsigned long long sacc[MAXC * NS];
-    __shared__ unsigned int snan[MAXC];
-    const long base = (long)blockIdx.x * CH;
-    if (base >= N) return;
+    const long base = chunk * CH;
     for (int i = threadIdx.x; i < MAXC * NS; i += 256) sacc[i] = 0ull;
     for (int i = threadIdx.x; i < MAXC; i += 256) snan[i] = 0u;
-    const int c_first = (int)pix[base].y;
-    const long last = (base + CH < N ? base + CH : N) - 1;
-    const int nclus = (int)pix[last].y - c_first + 1;
+    const int c_first = (int)a.pix[base].y;
+    const long last = (base + CH < a.N ? base + CH : a.N) - 1;
+    const int nclus = (int)a.pix[last].y - c_first + 1;
     __syncthreads();
 
-    long j0 = base + (long)threadIdx.x * R;
+    const long j0 = base + (long)threadIdx.x * R;
     int crun = -1;
     int64_t racc[NS];
     unsigned int rnan = 0u;
@@ -310,8 +334,8 @@ __global__ __launch_bounds__(256) void plane_cost_kernel(const uint2 *__restrict
     for (int k = 0; k < NS; ++k) racc[k] = 0;
     for (int r = 0; r < R; ++r) {
         long j = j0 + r;
-        if (j >= N) break;
-        uint2 q = pix[j];
+        if (j >= a.N) break;
+        uint2 q = a.pix[j];
         int c = (int)q.y;
         if (c != crun) {
             if (crun >= 0) {
@@ -324,34 +348,34 @@ __global__ __launch_bounds__(256) void plane_cost_kernel(const uint2 *__restrict
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
                 racc[k] = 0;
-                pl[k] = plane[(long)c * K + slot0 + k];
+                pl[k] = a.plane[(long)c * K + slot0 + k];
             }
             rnan = 0u;
             crun = c;
         }
         int x = unpack_x(q.x), y = unpack_y(q.x);
         float fx = (float)x, fy = (float)y;
-        long pofs = (long)y * pitch + x;
+        long pofs = (long)y * a.pitch + x;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             float df = pl[k].x * fx + pl[k].y * fy + pl[k].z;
             float cost;
-            if (use_cost_vol) {
+            if (a.use_cost_vol) {
                 int d = floor_to_int(df);
-                if (d < 0 || (long long)d + 1 >= D) {
-                    cost = tau;
+                if (d < 0 || (long long)d + 1 >= a.D) {
+                    cost = a.tau;
                 } else {
-                    float v0 = (float)vol[(long)d * vplane + pofs];
-                    float v1 = (float)vol[(long)(d + 1) * vplane + pofs];
+                    float v0 = (float)a.vol[(long)d * a.vplane + pofs];
+                    float v1 = (float)a.vol[(long)(d + 1) * a.vplane + pofs];
                     cost = ((float)(d + 1) - df) * v0 + (df - (float)d) * v1;
                 }
             } else {
-                float Dw = dwta[(long)y * W + x];
+                float Dw = a.dwta[(long)y * a.W + x];
                 cost = fabsf(Dw - df) <= 1.f ? 0.f : 1.f;
             }
-            float v = lambda * cost;
+            float v = a.lambda * cost;
             if (v != v) rnan |= 1u << (slot0 + k);
-            else racc[k] += to_fixed(v, shift);
+            else racc[k] += to_fixed(v, a.shift);
         }
     }
     if (crun >= 0) {
@@ -362,34 +386,31 @@ __global__ __launch_bounds__(256) void plane_cost_kernel(const uint2 *__restrict
         if (rnan) atomicOr(&snan[lc], rnan);
     }
     __syncthreads();
+    const int rep = (int)(chunk % a.rrep);
     for (int i = threadIdx.x; i < nclus * NS; i += 256) {
         unsigned long long v = sacc[i];
         int lc = i / NS, k = i % NS;
-        if (v) atomicAdd(&acc[(long)(c_first + lc) * K + slot0 + k], v);
+        if (v) atomicAdd(&a.acc[((long)(c_first + lc) * a.rrep + rep) * K + slot0 + k], v);
     }
     for (int i = threadIdx.x; i < nclus; i += 256)
-        if (snan[i]) atomicOr(&accnan[c_first + i], snan[i]);
+        if (snan[i]) atomicOr(&a.accnan[c_first + i], snan[i]);
+    __syncthreads();
 }
 
 // ----------------------------------------------------------------------------------------
-// messages: passing_msg_k (hbp_isp.cu:292-304), smooth_msg_k (306-355), normalize_msg_k
-// (273-290).  16 lanes per edge e = (s, t); lane y < 12 computes the message into s for s's
-// slot y from t's 8 best+prior particles.
+// messages for 16 edges (one 256-thread block): passing_msg_k (hbp_isp.cu:292-304),
+// smooth_msg_k (306-355), normalize_msg_k (273-290).  Lane y < 12 of edge e = (s, t) computes
+// the message into s for s's slot y from t's 8 best + prior particles.
 // ----------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void edge_kernel(long E, const int2 *__restrict__ bd, const float *__restrict__ c0v,
-                                                   const float *__restrict__ dposv,
-                                                   const float4 *__restrict__ plane,
-                                                   const float *__restrict__ belief, const uint4 *__restrict__ point,
-                                                   const float4 *__restrict__ msg, float tau_s,
-                                                   float *__restrict__ msg_t) {
-    const long e = (long)blockIdx.x * 16 + (threadIdx.x >> 4);
+__device__ void edge_group(const SweepArgs &a, long grp) {
+    const long e = grp * 16 + (threadIdx.x >> 4);
     const int y = threadIdx.x & 15;
-    const bool live = e < E && y < K;
+    const bool live = e < a.E && y < K;
     float out = 0.f;
     if (live) {
-        int2 b = bd[e];
+        int2 b = a.bd[e];
         int s = b.x, t = b.y;
-        uint4 ps = point[s], pt = point[t];
+        uint4 ps = a.point[s], pt = a.point[t];
         float X[6], Y[6];
         X[0] = (float)unpack_x(ps.x); Y[0] = (float)unpack_y(ps.x);
         X[1] = (float)unpack_x(ps.y); Y[1] = (float)unpack_y(ps.y);
@@ -397,10 +418,10 @@ __global__ __launch_bounds__(256) void edge_kernel(long E, const int2 *__restric
         X[3] = (float)unpack_x(pt.x); Y[3] = (float)unpack_y(pt.x);
         X[4] = (float)unpack_x(pt.y); Y[4] = (float)unpack_y(pt.y);
         X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
-        float c0 = c0v[e];
-        float dpos = dposv[e];
-        float4 psl = plane[(long)s * K + y];
-        float4 m0 = msg[2 * e], m1 = msg[2 * e + 1];
+        float c0 = a.c0[e];
+        float dpos = a.dpos[e];
+        float4 psl = a.plane[(long)s * K + y];
+        float4 m0 = a.msg[2 * e], m1 = a.msg[2 * e + 1];
         float mh[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
         float minimum = FLT_MAX;
 #pragma unroll
@@ -409,15 +430,15 @@ __global__ __launch_bounds__(256) void edge_kernel(long E, const int2 *__restric
                 out += minimum;
                 minimum = FLT_MAX;
             }
-            float m = belief[(long)t * K + p] - mh[p];
-            float4 ptl = plane[(long)t * K + p];
+            float m = a.belief[(long)t * K + p] - mh[p];
+            float4 ptl = a.plane[(long)t * K + p];
             float dpx = psl.x - ptl.x;
             float dpy = psl.y - ptl.y;
             float dpz = psl.z - ptl.z;
             float phi = fabsf(dpx * X[0] + dpy * Y[0] + dpz);
 #pragma unroll
             for (int j = 1; j < 6; ++j) phi = phi + fabsf(dpx * X[j] + dpy * Y[j] + dpz);
-            phi = fminf(phi / 3 / dpos, tau_s);
+            phi = fminf(phi / 3 / dpos, a.tau_s);
             float v = m + c0 * phi;
             if (v < minimum) minimum = v;
         }
@@ -425,11 +446,11 @@ __global__ __launch_bounds__(256) void edge_kernel(long E, const int2 *__restric
     }
     // normalize_msg_k: mean over the 12 slots, summed in slot order
     float mean = 0.f;
-    const int g = threadIdx.x & ~15;
+    const int g = (threadIdx.x & 63) & ~15;
 #pragma unroll
-    for (int k = 0; k < K; ++k) mean += __shfl(out, (g & 63) + k, 64);
+    for (int k = 0; k < K; ++k) mean += __shfl(out, g + k, 64);
     mean /= K;
-    if (live) msg_t[e * K + y] = out - mean;
+    if (live) a.msg_t[e * K + y] = out - mean;
 }
 
 // ----------------------------------------------------------------------------------------
@@ -437,105 +458,97 @@ __global__ __launch_bounds__(256) void edge_kernel(long E, const int2 *__restric
 // (1059-1066), get_belief_order (432-450), select_belief (452-472), select_msg (474-493);
 // then the parents' seed advance of a spatial sweep and the next sweep's candidates.
 // ----------------------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void node_kernel(LevelArgs a, const float *__restrict__ msg_t,
-                                                   float4 *__restrict__ msg, unsigned long long *__restrict__ acc,
-                                                   unsigned int *__restrict__ accnan, int shift, int this_kind,
-                                                   int next_kind, float max_dz, float max_dn,
-                                                   const float *__restrict__ dwta, uint32_t *__restrict__ s_ransac,
-                                                   uint32_t *__restrict__ s_neighbor,
-                                                   uint32_t *__restrict__ s_child) {
-    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < a.V; s += (long)gridDim.x * blockDim.x) {
-        float cost[K], bel[K];
-        unsigned int nb = accnan[s];
+__device__ void node_update(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn) {
+    float cost[K], bel[K];
+    unsigned int nb = a.accnan[s];
 #pragma unroll
-        for (int k = 0; k < 2 * P; ++k) cost[k] = a.cost[s * K + k];
+    for (int k = 0; k < 2 * P; ++k) cost[k] = a.cost[s * K + k];
 #pragma unroll
-        for (int k = 2 * P; k < K; ++k) {
-            cost[k] = from_fixed((int64_t)acc[s * K + k], (nb >> k) & 1u, shift);
-            acc[s * K + k] = 0ull;
+    for (int k = 2 * P; k < K; ++k) {
+        cost[k] = from_fixed(acc_sum(a, s, k), (nb >> k) & 1u, a.shift);
+        acc_zero(a, s, k);
+    }
+    a.accnan[s] = 0u;
+#pragma unroll
+    for (int k = 0; k < K; ++k) bel[k] = cost[k];
+    const int eb = a.off[s], ee = a.off[s + 1];
+    for (int e = eb; e < ee; ++e) {
+        const float4 *mt = (const float4 *)(a.msg_t + (long)e * K);
+        float4 u0 = mt[0], u1 = mt[1], u2 = mt[2];
+        bel[0] += u0.x; bel[1] += u0.y; bel[2] += u0.z; bel[3] += u0.w;
+        bel[4] += u1.x; bel[5] += u1.y; bel[6] += u1.z; bel[7] += u1.w;
+        bel[8] += u2.x; bel[9] += u2.y; bel[10] += u2.z; bel[11] += u2.w;
+    }
+    int order[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) order[k] = 0;
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i)
+#pragma unroll
+        for (int j = i + 1; j < K; ++j) {
+            if (bel[j] < bel[i]) order[i]++;
+            else order[j]++;
         }
-        accnan[s] = 0u;
+    // in-place compaction (select_belief / select_msg): output slot j < nsel takes the j-th
+    // selected slot in slot order, slots j >= nsel keep their own value.
+    bool sel[K];
+    int pos[K];
+    int nsel = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) bel[k] = cost[k];
-        const int eb = a.off[s], ee = a.off[s + 1];
-        for (int e = eb; e < ee; ++e) {
-            const float4 *mt = (const float4 *)(msg_t + (long)e * K);
-            float4 u0 = mt[0], u1 = mt[1], u2 = mt[2];
-            bel[0] += u0.x; bel[1] += u0.y; bel[2] += u0.z; bel[3] += u0.w;
-            bel[4] += u1.x; bel[5] += u1.y; bel[6] += u1.z; bel[7] += u1.w;
-            bel[8] += u2.x; bel[9] += u2.y; bel[10] += u2.z; bel[11] += u2.w;
-        }
-        int order[K];
+    for (int k = 0; k < K; ++k) {
+        sel[k] = order[k] < P;
+        pos[k] = nsel;
+        nsel += sel[k] ? 1 : 0;
+    }
+    float4 *pl = a.plane + s * K;
+    float4 pin[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) order[k] = 0;
+    for (int k = 0; k < K; ++k) pin[k] = pl[k];
 #pragma unroll
-        for (int i = 0; i < K - 1; ++i)
+    for (int j = 0; j < K; ++j) {
+        float cv = cost[j], bv = bel[j];
+        float4 pv = pin[j];
 #pragma unroll
-            for (int j = i + 1; j < K; ++j) {
-                if (bel[j] < bel[i]) order[i]++;
-                else order[j]++;
+        for (int k = j; k < K; ++k)
+            if (sel[k] && pos[k] == j) {
+                cv = cost[k];
+                bv = bel[k];
+                pv = pin[k];
             }
-        // in-place compaction (select_belief / select_msg): output slot j < nsel takes the j-th
-        // selected slot in slot order, slots j >= nsel keep their own value.
-        bool sel[K];
-        int pos[K];
-        int nsel = 0;
+        pl[j] = pv;
+        a.cost[s * K + j] = cv;
+        a.belief[s * K + j] = bv;
+    }
+    // messages over s's particles -> reverse edge, compacted by s's order
+    for (int e = eb; e < ee; ++e) {
+        const float4 *mt = (const float4 *)(a.msg_t + (long)e * K);
+        float4 u0 = mt[0], u1 = mt[1], u2 = mt[2];
+        float m[K] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w, u2.x, u2.y, u2.z, u2.w};
+        float o[2 * P];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            sel[k] = order[k] < P;
-            pos[k] = nsel;
-            nsel += sel[k] ? 1 : 0;
-        }
-        float4 *pl = a.plane + s * K;
-        float4 pin[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) pin[k] = pl[k];
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            float cv = cost[j], bv = bel[j];
-            float4 pv = pin[j];
+        for (int j = 0; j < 2 * P; ++j) {
+            float v = m[j];
 #pragma unroll
             for (int k = j; k < K; ++k)
-                if (sel[k] && pos[k] == j) {
-                    cv = cost[k];
-                    bv = bel[k];
-                    pv = pin[k];
-                }
-            pl[j] = pv;
-            a.cost[s * K + j] = cv;
-            a.belief[s * K + j] = bv;
+                if (sel[k] && pos[k] == j) v = m[k];
+            o[j] = v;
         }
-        // messages over s's particles -> reverse edge, compacted by s's order
-        for (int e = eb; e < ee; ++e) {
-            const float4 *mt = (const float4 *)(msg_t + (long)e * K);
-            float4 u0 = mt[0], u1 = mt[1], u2 = mt[2];
-            float m[K] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w, u2.x, u2.y, u2.z, u2.w};
-            float o[2 * P];
-#pragma unroll
-            for (int j = 0; j < 2 * P; ++j) {
-                float v = m[j];
-#pragma unroll
-                for (int k = j; k < K; ++k)
-                    if (sel[k] && pos[k] == j) v = m[k];
-                o[j] = v;
-            }
-            long r = a.rev[e];
-            msg[2 * r] = make_float4(o[0], o[1], o[2], o[3]);
-            msg[2 * r + 1] = make_float4(o[4], o[5], o[6], o[7]);
-        }
-        if (this_kind == kSpatial && s < a.Vh) {
-            uint32_t sc = s_child[s], sn = s_neighbor[s];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                sc = lcg_next(sc);
-                sn = lcg_next(sn);
-            }
-            s_child[s] = sc;
-            s_neighbor[s] = sn;
-        }
-        if (next_kind == kRansac) ransac_node((int)s, a.pix, a.pix_off, dwta, a.W, s_ransac, a.plane, a.point);
-        else if (next_kind == kRandom) random_node((int)s, a.mxy, max_dz, max_dn, s_ransac, a.plane);
+        long r = a.rev[e];
+        a.msg[2 * r] = make_float4(o[0], o[1], o[2], o[3]);
+        a.msg[2 * r + 1] = make_float4(o[4], o[5], o[6], o[7]);
     }
+    if (this_kind == kSpatial && s < a.Vh) {
+        uint32_t sc = a.s_child[s], sn = a.s_neighbor[s];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            sc = lcg_next(sc);
+            sn = lcg_next(sn);
+        }
+        a.s_child[s] = sc;
+        a.s_neighbor[s] = sn;
+    }
+    if (next_kind == kRansac) ransac_node((int)s, a.pix, a.pix_off, a.dwta, a.W, a.s_ransac, a.plane, a.point);
+    else if (next_kind == kRandom) random_node((int)s, a.mxy, max_dz, max_dn, a.s_ransac, a.plane);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -543,23 +556,76 @@ __global__ __launch_bounds__(128) void node_kernel(LevelArgs a, const float *__r
 // slot-0 plane of a random child (select_child_plane) of a random neighbour of P at the parent
 // level (neighbor_candidate_plane), each draw being the (p+1)-th LCG step of that parent's seed.
 // ----------------------------------------------------------------------------------------
-__global__ void spatial_kernel(LevelArgs a, const uint32_t *__restrict__ s_neighbor,
-                               const uint32_t *__restrict__ s_child) {
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < a.V; id += (long)gridDim.x * blockDim.x) {
-        int Pp = a.clus[id];
-        int nb_pos = a.off_h[Pp], nb_num = a.off_h[Pp + 1] - nb_pos;
-        uint32_t sn = s_neighbor[Pp];
-        for (int p = 0; p < P; ++p) {
-            sn = lcg_next(sn);
-            int r = (int)umod0(sn, nb_num);
-            int t = a.bd_h[r + nb_pos].y;
-            uint32_t sc = s_child[t];
-            for (int q = 0; q <= p; ++q) sc = lcg_next(sc);
-            int cpos = a.child_off[t], ccnt = a.child_off[t + 1] - cpos;
-            int r2 = (int)umod0(sc, ccnt);
-            int c = a.child[cpos + r2];
-            a.plane[id * K + 2 * P + p] = a.plane[(long)c * K];
+__device__ void spatial_node(const SweepArgs &a, long id) {
+    int Pp = a.clus[id];
+    int nb_pos = a.off_h[Pp], nb_num = a.off_h[Pp + 1] - nb_pos;
+    uint32_t sn = a.s_neighbor[Pp];
+    for (int p = 0; p < P; ++p) {
+        sn = lcg_next(sn);
+        int r = (int)umod0(sn, nb_num);
+        int t = a.bd_h[r + nb_pos].y;
+        uint32_t sc = a.s_child[t];
+        for (int q = 0; q <= p; ++q) sc = lcg_next(sc);
+        int cpos = a.child_off[t], ccnt = a.child_off[t + 1] - cpos;
+        int r2 = (int)umod0(sc, ccnt);
+        int c = a.child[cpos + r2];
+        a.plane[id * K + 2 * P + p] = a.plane[(long)c * K];
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// one kernel per phase (used for per-kernel measurements and step-by-step runs)
+// ----------------------------------------------------------------------------------------
+#define GRID_STRIDE(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+__global__ __launch_bounds__(256) void init_level_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) init_node(a, id); }
+__global__ __launch_bounds__(256) void init_cost_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) init_cost_node(a, id); }
+__global__ __launch_bounds__(256) void spatial_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) spatial_node(a, id); }
+__global__ __launch_bounds__(128) void node_kernel(SweepArgs a, int this_kind, int next_kind, float ndz, float ndn) {
+    GRID_STRIDE(s, a.V) node_update(a, s, this_kind, next_kind, ndz, ndn);
+}
+template <int NS, int R>
+__global__ __launch_bounds__(256) void plane_cost_kernel(SweepArgs a, int slot0) {
+    extern __shared__ unsigned long long lds_pc[];
+    constexpr int MAXC = 256 * R / 2 + 2;
+    plane_cost_chunk<NS, R>(a, blockIdx.x, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
+}
+__global__ __launch_bounds__(256) void edge_kernel(SweepArgs a) { edge_group(a, blockIdx.x); }
+
+__global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ----------------------------------------------------------------------------------------
+// all sweeps [s_begin, s_end) of one level in one cooperative launch: per sweep
+//   (spatial candidates | grid barrier)  plane-cost chunks + message groups  | barrier
+//   node updates (+ next candidates)  | barrier
+// Work items are spread over the blocks in grid-stride order.  With first_costed, the first
+// sweep's candidate costs were computed by the level init and its plane-cost phase is skipped.
+// ----------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bp_level_kernel(SweepArgs a, const SweepDesc *__restrict__ sw, int s_begin,
+                                                       int s_end, int first_costed) {
+    extern __shared__ unsigned long long lds_pc[];
+    auto grid = cg::this_grid();
+    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gth = (long)gridDim.x * blockDim.x;
+    const long nch4 = (a.N + CH4 - 1) / CH4;
+    const long neg = (a.E + 15) / 16;
+    for (int i = s_begin; i < s_end; ++i) {
+        const SweepDesc d = sw[i];
+        if (d.kind == kSpatial) {
+            for (long id = gtid; id < a.V; id += gth) spatial_node(a, id);
+            grid.sync();
         }
+        const long nch = (first_costed && i == s_begin) ? 0 : nch4;
+        for (long w = blockIdx.x; w < nch + neg; w += gridDim.x) {
+            if (w < nch) plane_cost_chunk<4, 8>(a, w, 2 * P, lds_pc, (unsigned int *)(lds_pc + MAXC4 * 4));
+            else edge_group(a, w - nch);
+        }
+        grid.sync();
+        for (long s = gtid; s < a.V; s += gth) node_update(a, s, d.kind, d.next_kind, d.ndz, d.ndn);
+        grid.sync();
     }
 }
 
@@ -668,14 +734,16 @@ static std::vector<Sweep> schedule(int L, int min_level, int D) {
 }
 int bp_total_sweeps(int levels, int min_level, int D) { return (int)schedule(levels, min_level, D).size(); }
 
-static LevelArgs level_args(BP &b, int l) {
+static SweepArgs sweep_args(BP &b, int l) {
     BPLevel &B = b.lv[l];
-    LevelArgs a{};
+    const Volume &v = *b.vol;
+    SweepArgs a{};
     a.V = B.V;
     a.Vh = B.Vh;
     a.W = b.W;
     a.D = b.D;
     a.E = B.E;
+    a.N = b.N;
     a.clus = B.clus;
     a.bd = B.bd;
     a.pix = B.pix.p;
@@ -697,6 +765,23 @@ static LevelArgs level_args(BP &b, int l) {
         a.off_h = Hh.off.p;
         a.plane_h = Hh.plane.p;
     }
+    a.vol = v.vol.p;
+    a.pitch = v.pitch;
+    a.vplane = v.plane;
+    a.dwta = v.dwta.p;
+    a.lambda = b.lambda;
+    a.tau = b.tau;
+    a.tau_s = b.tau_s;
+    a.use_cost_vol = b.use_cost_vol;
+    a.shift = b.fx_shift;
+    a.msg = b.msg.p;
+    a.msg_t = b.msg_t.p;
+    a.acc = b.acc.p;
+    a.rrep = acc_replicas(B.V);
+    a.accnan = b.accnan.p;
+    a.s_ransac = b.s_ransac.p;
+    a.s_neighbor = b.s_neighbor.p;
+    a.s_child = b.s_child.p;
     return a;
 }
 
@@ -789,7 +874,10 @@ void bp_setup(Ctx &c, BP &b) {
     }
     b.msg.alloc(c.pool, (size_t)maxE * 2);
     b.msg_t.alloc(c.pool, (size_t)maxE * K);
-    b.acc.alloc(c.pool, (size_t)maxV * K);
+    size_t acc_n = 0;
+    for (int l = b.min_level; l < b.L; ++l)
+        acc_n = std::max(acc_n, (size_t)b.lv[l].V * acc_replicas(b.lv[l].V) * K);
+    b.acc.alloc(c.pool, acc_n);
     b.accnan.alloc(c.pool, (size_t)maxV);
     b.s_ransac.alloc(c.pool, maxV);
     b.s_neighbor.alloc(c.pool, maxV);
@@ -804,57 +892,115 @@ void bp_setup(Ctx &c, BP &b) {
     HBP_CHECK(hipStreamSynchronize(c.stream));
 }
 
+
+// algorithmic HBM bytes per launch (pixel id 4 B + 2 cost bytes per plane and pixel; per
+// superpixel the planes read and the sums written)
+static double plane_cost_bytes(long N, int V, int NS) { return (double)N * (4 + 2 * NS) + (double)V * (16 * NS + 8 + 4 * NS); }
+static double edge_bytes(long E, int V) { return 96.0 * E + (16.0 * K + 48) * V; }
+static double node_bytes(long E, int V) { return 84.0 * E + (48.0 * K + 16) * V; }
+
 template <int NS, int R>
-static void launch_plane_cost(Ctx &c, BP &b, BPLevel &B, int slot0) {
+static void launch_plane_cost(Ctx &c, const SweepArgs &a, int l, int slot0) {
     constexpr int CH = 256 * R;
-    int grid = div_up(b.N, CH);
-    const Volume &v = *b.vol;
-    // algorithmic bytes (pixel id 4 B + 2 cost bytes per plane and pixel; per superpixel the
-    // planes read, the sums and counts written)
-    double bytes = (double)b.N * (4 + 2 * NS) + (double)B.V * (16 * NS + 8 + 4 * NS);
-    int lvl = (int)(&B - &b.lv[0]);
-    ProfScope ps(c, "plane_cost_L" + std::to_string(lvl), bytes);
-    hipLaunchKernelGGL((plane_cost_kernel<NS, R>), dim3(grid), dim3(256), 0, c.stream, B.pix.p, b.N, slot0,
-                       B.plane.p, v.vol.p, v.pitch, v.plane, b.D, v.dwta.p, b.W, b.lambda, b.tau, b.use_cost_vol,
-                       b.fx_shift, b.acc.p, b.accnan.p);
+    ProfScope ps(c, "plane_cost_L" + std::to_string(l), plane_cost_bytes(a.N, a.V, NS));
+    hipLaunchKernelGGL((plane_cost_kernel<NS, R>), dim3(div_up(a.N, CH)), dim3(256), PC_LDS_BYTES, c.stream, a,
+                       slot0);
 }
 
-static void run_sweep(Ctx &c, BP &b, const Sweep &sw, int next_kind, float ndz, float ndn) {
+// one sweep as separate kernels
+static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float ndz, float ndn) {
     int l = sw.level;
-    BPLevel &B = b.lv[l];
-    LevelArgs a = level_args(b, l);
-    const Volume &v = *b.vol;
+    SweepArgs a = sweep_args(b, l);
     if (sw.first) {
-        HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)B.E, c.stream));
-        hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, B.V)), dim3(256), 0, c.stream, a, v.dwta.p,
-                           b.s_ransac.p, b.s_neighbor.p, b.s_child.p, b.acc.p, b.accnan.p);
-        launch_plane_cost<12, 2>(c, b, B, 0);
-        hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, B.V)), dim3(256), 0, c.stream, B.V, B.cost.p,
-                           B.belief.p, b.acc.p, b.accnan.p, b.fx_shift);
+        HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)a.E, c.stream));
+        hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
+        launch_plane_cost<12, 2>(c, a, l, 0);
+        hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
     } else {
         if (sw.kind == kSpatial) {
-            ProfScope pss(c, "spatial_L" + std::to_string(l), (16.0 * P + 28) * B.V);
-            hipLaunchKernelGGL(spatial_kernel, dim3(grid_for(c, B.V)), dim3(256), 0, c.stream, a, b.s_neighbor.p,
-                               b.s_child.p);
+            ProfScope pss(c, "spatial_L" + std::to_string(l), (16.0 * P + 28) * a.V);
+            hipLaunchKernelGGL(spatial_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
         }
-        launch_plane_cost<4, 8>(c, b, B, 2 * P);
+        launch_plane_cost<4, 8>(c, a, l, 2 * P);
     }
-    if (B.E > 0) {
-        ProfScope ps(c, "edge_L" + std::to_string(l), 96.0 * B.E + (16.0 * K + 32 + 16) * B.V);
-        hipLaunchKernelGGL(edge_kernel, dim3(div_up(B.E, 16)), dim3(256), 0, c.stream, B.E, B.bd, B.c0.p, B.dpos.p,
-                           B.plane.p, B.belief.p, B.point.p, b.msg.p, b.tau_s, b.msg_t.p);
+    if (a.E > 0) {
+        ProfScope ps(c, "edge_L" + std::to_string(l), edge_bytes(a.E, a.V));
+        hipLaunchKernelGGL(edge_kernel, dim3(div_up(a.E, 16)), dim3(256), 0, c.stream, a);
     }
-    ProfScope psn(c, "node_L" + std::to_string(l), 84.0 * B.E + (32.0 * K + 16.0 * K + 16) * B.V);
-    hipLaunchKernelGGL(node_kernel, dim3(grid_for(c, B.V, 128)), dim3(128), 0, c.stream, a, b.msg_t.p, b.msg.p,
-                       b.acc.p, b.accnan.p, b.fx_shift, sw.kind, next_kind, ndz, ndn, v.dwta.p, b.s_ransac.p,
-                       b.s_neighbor.p, b.s_child.p);
+    {
+        ProfScope psn(c, "node_L" + std::to_string(l), node_bytes(a.E, a.V));
+        hipLaunchKernelGGL(node_kernel, dim3(grid_for(c, a.V, 128)), dim3(128), 0, c.stream, a, sw.kind, next_kind,
+                           ndz, ndn);
+    }
     HBP_CHECK(hipGetLastError());
+}
+
+int bp_level_coop_blocks(int num_cu) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_level_kernel, 256, PC_LDS_BYTES) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (per_cu > 4) per_cu = 4;
+    return per_cu * num_cu;
+}
+
+// sweeps [i0, i1) of one level in one cooperative launch
+static void run_sweeps_fused(Ctx &c, BP &b, size_t i0, size_t i1) {
+    const Sweep &s0 = b.sched[i0];
+    int l = s0.level;
+    SweepArgs a = sweep_args(b, l);
+    // the device copy of the level's schedule
+    std::vector<SweepDesc> h(i1 - i0);
+    for (size_t i = i0; i < i1; ++i) {
+        const Sweep &sw = b.sched[i];
+        SweepDesc d{sw.kind, kNone, 0.f, 0.f};
+        if (!sw.last && i + 1 < b.sched.size()) {
+            d.next_kind = b.sched[i + 1].kind;
+            d.ndz = b.sched[i + 1].dz;
+            d.ndn = b.sched[i + 1].dn;
+        }
+        h[i - i0] = d;
+    }
+    b.sweep_desc.alloc(c.pool, h.size() * sizeof(SweepDesc));
+    HBP_CHECK(hipMemcpyAsync(b.sweep_desc.p, h.data(), h.size() * sizeof(SweepDesc), hipMemcpyHostToDevice,
+                             c.stream));
+    if (s0.first) {
+        // level init as separate kernels (its 12-slot plane cost needs more registers)
+        HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)a.E, c.stream));
+        hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
+        launch_plane_cost<12, 2>(c, a, l, 0);
+        hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
+        HBP_CHECK(hipGetLastError());
+    }
+    long work = std::max({(long)div_up(a.N, CH4), (long)div_up(a.E, 16), (long)div_up(a.V, 256), 1L});
+    int blocks = (int)std::min<long>(work, c.bp_coop_blocks);
+    const SweepDesc *dsw = (const SweepDesc *)b.sweep_desc.p;
+    int sb = 0, se = (int)(i1 - i0), fc = s0.first ? 1 : 0;
+    void *args[] = {(void *)&a, (void *)&dsw, (void *)&sb, (void *)&se, (void *)&fc};
+    double bytes = (double)(i1 - i0) * (plane_cost_bytes(a.N, a.V, 4) + edge_bytes(a.E, a.V) + node_bytes(a.E, a.V));
+    ProfScope ps(c, "bp_level_L" + std::to_string(l), bytes);
+    HBP_CHECK(hipLaunchCooperativeKernel((const void *)bp_level_kernel, dim3(blocks), dim3(256), args, PC_LDS_BYTES,
+                                         c.stream));
 }
 
 int bp_step(Ctx &c, BP &b, int max_sweeps) {
     int done = 0;
     while (done < max_sweeps && b.next < b.sched.size()) {
         const Sweep &sw = b.sched[b.next];
+        if (b.fused) {
+            // the rest of this level (or as much of it as max_sweeps allows) in one launch
+            size_t i1 = b.next;
+            while (i1 < b.sched.size() && b.sched[i1].level == sw.level && (long)(i1 - b.next) < max_sweeps - done &&
+                   (i1 == b.next || !b.sched[i1].first))
+                ++i1;
+            run_sweeps_fused(c, b, b.next, i1);
+            int n = (int)(i1 - b.next);
+            b.cur = sw.level;
+            b.next = i1;
+            done += n;
+            b.sweeps += n;
+            continue;
+        }
         int next_kind = kNone;
         float ndz = 0.f, ndn = 0.f;
         if (!sw.last && b.next + 1 < b.sched.size()) {
@@ -863,7 +1009,7 @@ int bp_step(Ctx &c, BP &b, int max_sweeps) {
             ndz = nx.dz;
             ndn = nx.dn;
         }
-        run_sweep(c, b, sw, next_kind, ndz, ndn);
+        run_sweep_split(c, b, sw, next_kind, ndz, ndn);
         b.cur = sw.level;
         b.next++;
         done++;

@@ -260,7 +260,8 @@ struct Ctx {
     hipStream_t stream = nullptr;
     Pool pool;
     int num_cu = 256;
-    int coop_blocks = 0;   // co-resident blocks of the cooperative kernels
+    int coop_blocks = 0;     // co-resident blocks of the CRTrees cooperative kernel
+    int bp_coop_blocks = 0;  // co-resident blocks of the fused BP level kernel
     // scratch reused by every call on this context
     DevBuf<uint8_t> scratch;
     DevBuf<int> counters;  // small device counters (sizes, flags)

@@ -71,10 +71,9 @@ struct BP {
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;
-    bool inited = false;
-    int outer = 0, plane_iter = 0, iter_bp = 0;
-    float max_dz = 0, max_dn = 0;
     int sweeps = 0;
+    bool fused = true;             // one cooperative launch per level, else one kernel per phase
+    DevBuf<uint8_t> sweep_desc;    // device copy of the schedule of the running launch
     bool mapped = false;
     std::vector<Sweep> sched;
     size_t next = 0;
@@ -94,6 +93,7 @@ int compact_labels(Ctx &c, int *d_label, int V);
 
 // bp.hip
 void bp_setup(Ctx &c, BP &b);
+int bp_level_coop_blocks(int num_cu);
 int bp_step(Ctx &c, BP &b, int max_sweeps);
 void bp_map(Ctx &c, BP &b);
 void bp_get_state(Ctx &c, BP &b, float *planes, float *cost, float *belief, float *msg);

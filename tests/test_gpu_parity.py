@@ -152,16 +152,17 @@ def _bp_pair(seed, W, H, D, planes=(3, 8)):
     return left, right
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2)])
-def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml):
-    say(f"bp state seed={seed} {W}x{H} D={D} min_level={ml}")
+def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml, fused):
+    say(f"bp state seed={seed} {W}x{H} D={D} min_level={ml} fused={fused}")
     left, right = _bp_pair(seed, W, H, D)
     vo, wo = O.cost_volume(left, right, D)
     po = O.Pyramid(left)
     bo = O.BP(po, vo, wo, D, min_level=ml)
     vg = ctx.cost_volume(left, right, D)
     pg = ctx.pyramid(left)
-    bg = ctx.bp(pg, vg, min_level=ml)
+    bg = ctx.bp(pg, vg, min_level=ml, fused=fused)
     assert bg.total_sweeps == O.lib.orc_bp_total_sweeps(po.levels, ml, D)
     for sweep in range(12):
         assert bo.step(1) == 1 and bg.step(1) == 1
@@ -173,10 +174,11 @@ def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml):
         assert same_bits(so["msg"], sg["msg"]), f"messages differ after sweep {sweep}"
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2),
                                            (8, 160, 120, 48, 3)])
-def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
-    say(f"bp full seed={seed} {W}x{H} D={D} min_level={ml}")
+def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml, fused):
+    say(f"bp full seed={seed} {W}x{H} D={D} min_level={ml} fused={fused}")
     left, right = _bp_pair(seed, W, H, D)
     vo, wo = O.cost_volume(left, right, D)
     po = O.Pyramid(left)
@@ -185,7 +187,7 @@ def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
     do, plo = bo.map()
     vg = ctx.cost_volume(left, right, D)
     pg = ctx.pyramid(left)
-    bg = ctx.bp(pg, vg, min_level=ml)
+    bg = ctx.bp(pg, vg, min_level=ml, fused=fused)
     dg, plg = bg.run()
     assert same_bits(do, dg), f"{mismatch(do, dg).mean():.4%} of disparities differ"
     assert same_bits(plo, plg)
@@ -222,12 +224,25 @@ def test_plain_path_cfg1_bit_exact(ctx):
     assert same_bits(plo, plg)
 
 
-def test_bp_is_deterministic(ctx):
+def test_bp_is_deterministic_and_fused_equals_split(ctx):
     left, right = _bp_pair(4, 120, 90, 32)
     vg = ctx.cost_volume(left, right, 32)
     pg = ctx.pyramid(left)
     d1, p1 = ctx.bp(pg, vg, min_level=2).run()
     d2, p2 = ctx.bp(pg, vg, min_level=2).run()
+    d3, p3 = ctx.bp(pg, vg, min_level=2, fused=False).run()
+    assert same_bits(d1, d2) and same_bits(p1, p2)
+    assert same_bits(d1, d3) and same_bits(p1, p3)
+
+
+def test_large_pair_fused_equals_split(ctx):
+    say("large pair fused vs split")
+    left, right, _ = synth.make_pair(11, 1472, 984, 256, (20, 40))
+    vg = ctx.cost_volume(left, right, 256)
+    pg = ctx.pyramid(left)
+    ml = max(1, pg.levels - 5)
+    d1, p1 = ctx.bp(pg, vg, min_level=ml).run()
+    d2, p2 = ctx.bp(pg, vg, min_level=ml, fused=False).run()
     assert same_bits(d1, d2) and same_bits(p1, p2)
 
 
