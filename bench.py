@@ -142,24 +142,14 @@ def main():
                       gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
               for g, v in sorted(phases.items(), key=lambda kv: -kv[1]["ms"])}
 
+    # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_plane_cost_large.json")
-    if args.workload == "large" and os.path.exists(tfile):
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
+    if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
-        except Exception:
-            phases = {}
-    for name, v in phase_stats.items():
-        g = name.split("_L")[0]
-        a = phases.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
-        a["ms"] += v["ms"]
-        a["bytes"] += v["bytes"]
-        a["launches"] += v["launches"]
-    phases = {g: dict(ms_per_pair=round(v["ms"], 3), launches=v["launches"],
-                      gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
-              for g, v in sorted(phases.items(), key=lambda kv: -kv[1]["ms"])}
-
-    traffic = None
+        except (OSError, ValueError):
+            traffic = None
 
     cpu = None
     if not args.no_cpu_baseline:
