@@ -42,8 +42,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
-    ap.add_argument("--split", action="store_true",
-                    help="time the sweeps as one kernel per phase instead of one launch per level")
+    ap.add_argument("--mode", choices=["split", "fused"], default="split",
+                    help="split: two kernels per BP sweep; fused: one cooperative launch per BP level")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stereo path runs one pair per GPU and this build has "
@@ -71,7 +71,7 @@ def main():
 
     info = {}
 
-    def one_pair(fused=not args.split):
+    def one_pair(fused=args.mode == "fused"):
         vol, pyr, bp = C.c_void_p(), C.c_void_p(), C.c_void_p()
         H._check(lib.hbp_cost_volume_dev(ctx.h, d_l, d_r, W, Hh, D, b"Census", C.byref(vol)))
         H._check(lib.hbp_pyramid_build_dev(ctx.h, d_l, W, Hh, 2, 4, 0.0, C.byref(pyr)))
@@ -85,27 +85,38 @@ def main():
         lib.hbp_pyramid_free(ctx.h, pyr)
         lib.hbp_volume_free(ctx.h, vol)
 
+    def timed(steps):
+        lat = []
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            one_pair()
+            ctx.sync()
+            lat.append(time.perf_counter() - ts)
+        ctx.sync()
+        return time.perf_counter() - t0, lat
+
     for _ in range(args.warmup):
         one_pair()
     ctx.sync()
+    # timed region: HIP events around the dominant kernel's launches only
+    dom = "bp_level" if args.mode == "fused" else "plane_cost"
+    ctx.profile(True)
+    ctx.profile_only(dom)
+    ctx.profile_reset()
+    elapsed, lat = timed(args.steps)
+    stats = ctx.profile_stats()
+    ctx.profile(False)
+    ctx.profile_only(None)
+    # the same steps without any events (the events' own cost)
+    elapsed_plain, _ = timed(args.steps)
+    # one more pair with every kernel timed, outside the timed region: per-kernel breakdown
     ctx.profile(True)
     ctx.profile_reset()
-    lat = []
+    one_pair()
     ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        one_pair()
-        ctx.sync()
-        lat.append(time.perf_counter() - ts)
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    stats = ctx.profile_stats()
-    # one more pair with one kernel per BP phase, outside the timed region: per-phase breakdown
-    ctx.profile_reset()
-    one_pair(fused=False)
-    ctx.sync()
-    phase_stats = ctx.profile_stats()
+    pair_stats = ctx.profile_stats()
     ctx.profile(False)
 
     pairs_per_s = args.steps / elapsed
@@ -114,33 +125,23 @@ def main():
     p50 = statistics.median(lat_ms)
     p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
 
-    # kernels grouped by name without the level suffix
-    groups = {}
-    for name, s in stats.items():
-        g = name.split("_L")[0]
-        a = groups.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
-        a["ms"] += s["ms"]
-        a["bytes"] += s["bytes"]
-        a["launches"] += s["launches"]
-    dom = max(groups, key=lambda g: groups[g]["ms"])
-    gd = groups[dom]
+    def group(st):
+        out = {}
+        for name, v in st.items():
+            g = name.split("_L")[0]
+            a = out.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
+            a["ms"] += v["ms"]
+            a["bytes"] += v["bytes"]
+            a["launches"] += v["launches"]
+        return out
+
+    gd = group(stats).get(dom, dict(ms=0.0, bytes=0.0, launches=0))
     avg_ms = gd["ms"] / max(gd["launches"], 1)
     avg_bytes = gd["bytes"] / max(gd["launches"], 1)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    kernels = {g: dict(ms_per_step=round(v["ms"] / args.steps, 3), launches_per_step=v["launches"] // args.steps,
+    kernels = {g: dict(ms_per_pair=round(v["ms"], 3), launches=v["launches"],
                        gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
-               for g, v in sorted(groups.items(), key=lambda kv: -kv[1]["ms"])}
-
-    phases = {}
-    for name, v in phase_stats.items():
-        g = name.split("_L")[0]
-        a = phases.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
-        a["ms"] += v["ms"]
-        a["bytes"] += v["bytes"]
-        a["launches"] += v["launches"]
-    phases = {g: dict(ms_per_pair=round(v["ms"], 3), launches=v["launches"],
-                      gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
-              for g, v in sorted(phases.items(), key=lambda kv: -kv[1]["ms"])}
+               for g, v in sorted(group(pair_stats).items(), key=lambda kv: -kv[1]["ms"])}
 
     # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     traffic = None
@@ -163,6 +164,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "ms_per_step_without_events": round(1e3 * elapsed_plain / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -173,13 +175,12 @@ def main():
         "p99_ms": round(p99, 3),
         "config": {"workload": wl["name"], "width": W, "height": Hh, "max_disp": D, "levels": info["levels"],
                    "min_level": info["min_level"], "sweeps": info["sweeps"], "alpha": 1000.1,
-                   "tau_smooth": 0.6, "seed": args.seed},
+                   "tau_smooth": 0.6, "seed": args.seed, "mode": args.mode},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": round(avg_bytes),
                      "avg_launch_us": round(1e3 * avg_ms, 2), "launches": gd["launches"]},
-        "kernels": kernels,
-        "phases_split_pass": phases,
+        "kernels_one_pair": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

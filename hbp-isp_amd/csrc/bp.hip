@@ -2,22 +2,30 @@
 // (hbp_isp.cu / hbp_isp.hpp of the reference).
 //
 // One BP sweep (generate_plane + compute_plane_cost + belief_propagation, hbp_isp.cu:1302-1308)
-// is three kernels on one stream:
-//   plane_cost_kernel  per pixel: cost of the 4 candidate planes, summed per superpixel in
-//                      int64 fixed point (segmented in LDS, one global atomic per chunk and
-//                      cluster), so the sums are independent of the reduction order;
-//   edge_kernel        per (edge, target slot): the min-sum message of smooth_msg_k and the
-//                      mean removal of normalize_msg_k (16 lanes per edge);
-//   node_kernel        per superpixel: belief = cost + incoming messages in edge order, rank
-//                      of the 12 beliefs, compaction of planes/costs/beliefs and of the
-//                      outgoing messages (select_*), written straight to the reverse edge
-//                      (ping_pang_msg), and the next sweep's RANSAC / random candidates.
-// Spatial-propagation sweeps add spatial_kernel, which resolves the parent/neighbour/child
-// chain of select_child_plane + neighbor_candidate_plane + set_candidate_plane_k per child.
+// is two kernels on one stream:
+//   cost_edge_kernel  (1) per pixel: the cost of the 4 candidate planes of its superpixel,
+//                     summed per superpixel as int64 fixed point, so the sums do not depend
+//                     on the reduction order; (2) per (edge, target slot): the min-sum message
+//                     of smooth_msg_k and the mean removal of normalize_msg_k.  On spatial
+//                     sweeps the candidates of select_child_plane + neighbor_candidate_plane
+//                     are resolved inside this kernel.
+//   node_kernel       per superpixel, 16 lanes (one per slot): belief = cost + incoming
+//                     messages in edge order, rank of the 12 beliefs, compaction of planes /
+//                     costs / beliefs and of the outgoing messages (select_*), written to the
+//                     reverse edge (ping_pang_msg), and the next sweep's RANSAC / random
+//                     candidates.
+//
+// Plane costs have two layouts.  Coarse levels (>= 32 pixels per superpixel on average) cut
+// each superpixel's sorted pixel run into segments of up to 512 pixels; a wavefront owns a
+// segment, loads the superpixel's planes once, reads 64 consecutive pixels per step, reduces
+// across lanes and adds one int64 per plane to the superpixel's sum.  Fine levels use chunks
+// of consecutive sorted pixels per 256-thread block with a segmented wavefront scan and LDS
+// counters per superpixel.
 //
 // HBM layout per level: planes [V][12] float4, cost/belief [V][12] f32, messages [E][8] f32
-// (the 8 slots the next sweep reads), sweep messages [E][12] f32, pixels sorted by superpixel
-// as (packed x|y<<16, cluster) pairs, CSR offsets/reverse ids/edge weights per edge.
+// (the 8 slots the next sweep reads), sweep messages [E][12] f32, pixels sorted by
+// (superpixel, pixel id) as packed x|y<<16 [N] and superpixel id [N], CSR offsets / reverse
+// ids / edge weights per edge, fixed-point sums [V][rrep][12] int64.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
@@ -30,10 +38,12 @@ namespace cg = cooperative_groups;
 namespace hbp {
 
 constexpr int P = kP, K = kK;
+constexpr int SEG_R = 8, SEG = 64 * SEG_R;   // pixels per plane-cost segment (coarse levels)
+constexpr int COARSE_MIN_PIX = 32;           // average pixels per superpixel of a coarse level
 
-// replicas of each fixed-point accumulator: chunk k adds into replica k mod rrep, which spreads
-// the atomics of the few large superpixels of the coarse levels over rrep addresses
-static inline int acc_replicas(int V) { return V <= 65536 ? 8 : 1; }
+// replicas of each fixed-point accumulator: work item i adds into replica i mod rrep, which
+// spreads the atomics of the few large superpixels of the coarse levels over rrep addresses
+static inline int acc_replicas(int V) { return V <= 4096 ? 16 : (V <= 65536 ? 4 : 1); }
 
 // ----------------------------------------------------------------------------------------
 // prologue kernels
@@ -49,24 +59,42 @@ __global__ void compose_kernel(unsigned int *__restrict__ im_clus, const int *__
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         im_clus[i] = (unsigned)clus[im_clus[i]];
 }
-__global__ void count_u32_kernel(const unsigned int *__restrict__ key, long n, int *__restrict__ cnt) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[key[i]], 1);
-}
-__global__ void count_i32_kernel(const int *__restrict__ key, long n, int *__restrict__ cnt) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[key[i]], 1);
+// off[k] = first j with keys[j] >= k, k in [0, nkeys]; keys ascending (a CSR offset array
+// without atomics: every key run writes the offsets of the keys it starts)
+__global__ void lower_bound_kernel(const unsigned int *__restrict__ keys, long n, int nkeys, int *__restrict__ off) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        long k1 = keys[j];
+        long k0 = j == 0 ? 0 : (long)keys[j - 1] + 1;
+        for (long k = k0; k <= k1 && k <= nkeys; ++k) off[k] = (int)j;
+        if (j == n - 1)
+            for (long k = k1 + 1; k <= nkeys; ++k) off[k] = (int)n;
+    }
 }
 __global__ void count_src_kernel2(const int2 *__restrict__ bd, long E, int *__restrict__ cnt) {
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x)
         atomicAdd(&cnt[bd[e].x], 1);
 }
-// setup_D4 (hbp_isp.cu:871-891): (packed xy, cluster) in (cluster, pixel id) order
+// setup_D4 (hbp_isp.cu:871-891): pixels in (cluster, pixel id) order as packed xy + cluster
 __global__ void pix_kernel(const unsigned int *__restrict__ keys, const int *__restrict__ ids, long n, int W,
-                           uint2 *__restrict__ pix) {
+                           uint32_t *__restrict__ pxy, uint32_t *__restrict__ pcl) {
     for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
         int id = ids[j];
-        pix[j] = make_uint2(pack_xy(id % W, id / W), keys[j]);
+        pxy[j] = pack_xy(id % W, id / W);
+        pcl[j] = keys[j];
+    }
+}
+// segments of at most SEG pixels of one superpixel: count, then (first, end, cluster, index)
+__global__ void seg_count_kernel(const int *__restrict__ pix_off, int V, int *__restrict__ cnt) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c <= V; c += (long)gridDim.x * blockDim.x)
+        cnt[c] = c < V ? (pix_off[c + 1] - pix_off[c] + SEG - 1) / SEG : 0;
+}
+__global__ void seg_fill_kernel(const int *__restrict__ pix_off, const int *__restrict__ sego, int V,
+                                int4 *__restrict__ seg) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < V; c += (long)gridDim.x * blockDim.x) {
+        int b = pix_off[c], e = pix_off[c + 1];
+        int o = sego[c];
+        for (int i = 0; b + i * SEG < e; ++i)
+            seg[o + i] = make_int4(b + i * SEG, min(b + (i + 1) * SEG, e), (int)c, i);
     }
 }
 // reverse_bd_id_k (hbp_isp.cu:154-175): first r in t's CSR range with bd[r].y == s
@@ -145,63 +173,62 @@ __device__ __forceinline__ float4 fit_plane(float4 A, float4 B, float4 C) {
     return make_float4(-nx, -ny, nx * A.x + ny * A.y + A.z, 0.f);
 }
 
-// ransac_candidate_plane (hbp_isp.cu:592-640) for one superpixel
-__device__ void ransac_node(int id, const uint2 *__restrict__ pix, const int *__restrict__ pix_off,
+// one plane of ransac_candidate_plane (hbp_isp.cu:592-640): three LCG draws from seed s
+// (advanced in place), three distinct pixels of the superpixel, the plane through their WTA
+// disparities.  q returns the packed pixels.
+__device__ __forceinline__ float4 ransac_plane(uint32_t &s, const uint32_t *__restrict__ pxy, int pos, int n,
+                                               const float *__restrict__ dwta, int W, uint4 &q) {
+    s = lcg_next(s);
+    int p0 = (int)umod0(s, n);
+    s = lcg_next(s);
+    int p1 = (int)umod0(s, n - 1);
+    s = lcg_next(s);
+    int p2 = (int)umod0(s, n - 2);
+    three_diff(p0, p1, p2, n);
+    q.x = pxy[pos + p0];
+    q.y = pxy[pos + p1];
+    q.z = pxy[pos + p2];
+    int x0 = unpack_x(q.x), y0 = unpack_y(q.x), x1 = unpack_x(q.y), y1 = unpack_y(q.y), x2 = unpack_x(q.z),
+        y2 = unpack_y(q.z);
+    float4 A = make_float4((float)x0, (float)y0, dwta[(long)y0 * W + x0], 0.f);
+    float4 B = make_float4((float)x1, (float)y1, dwta[(long)y1 * W + x1], 0.f);
+    float4 C = make_float4((float)x2, (float)y2, dwta[(long)y2 * W + x2], 0.f);
+    return fit_plane(A, B, C);
+}
+
+// ransac_candidate_plane for the candidate slots 2P..3P-1 of one superpixel
+__device__ void ransac_node(int id, const uint32_t *__restrict__ pxy, const int *__restrict__ pix_off,
                             const float *__restrict__ dwta, int W, uint32_t *__restrict__ s_ransac,
                             float4 *__restrict__ plane, uint4 *__restrict__ point) {
     int pos = pix_off[id];
     int n = pix_off[id + 1] - pos;
     uint32_t s = s_ransac[id];
-    uint32_t q0 = 0, q1 = 0, q2 = 0;
-    for (int p = 2 * P; p < 3 * P; ++p) {
-        s = lcg_next(s);
-        int p0 = (int)umod0(s, n);
-        s = lcg_next(s);
-        int p1 = (int)umod0(s, n - 1);
-        s = lcg_next(s);
-        int p2 = (int)umod0(s, n - 2);
-        three_diff(p0, p1, p2, n);
-        q0 = pix[pos + p0].x;
-        q1 = pix[pos + p1].x;
-        q2 = pix[pos + p2].x;
-        int x0 = unpack_x(q0), y0 = unpack_y(q0), x1 = unpack_x(q1), y1 = unpack_y(q1), x2 = unpack_x(q2),
-            y2 = unpack_y(q2);
-        float4 A = make_float4((float)x0, (float)y0, dwta[(long)y0 * W + x0], 0.f);
-        float4 B = make_float4((float)x1, (float)y1, dwta[(long)y1 * W + x1], 0.f);
-        float4 C = make_float4((float)x2, (float)y2, dwta[(long)y2 * W + x2], 0.f);
-        plane[(long)id * K + p] = fit_plane(A, B, C);
-    }
-    point[id] = make_uint4(q0, q1, q2, 0u);
+    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+    for (int p = 2 * P; p < 3 * P; ++p) plane[(long)id * K + p] = ransac_plane(s, pxy, pos, n, dwta, W, q);
+    point[id] = q;
     s_ransac[id] = s;
 }
 
-// random_candidate_plane (hbp_isp.cu:641-674) with normalize_plane (495-503)
-__device__ void random_node(int id, const float2 *__restrict__ mxy, float max_dz, float max_dn,
-                            uint32_t *__restrict__ s_ransac, float4 *__restrict__ plane) {
-    uint32_t s = s_ransac[id];
-    float2 m = mxy[id];
-    float x = m.x, y = m.y;
-    for (int p = 2 * P; p < 3 * P; ++p) {
-        float4 pl = plane[(long)id * K + p];
-        float z = pl.x * x + pl.y * y + pl.z;
-        float dz = lcg_uniform(s, max_dz);
-        z += dz;
-        float dnx = lcg_uniform(s, max_dn);
-        float dny = lcg_uniform(s, max_dn);
-        float dnz = lcg_uniform(s, max_dn);
-        float norm = sqrtf(pl.x * pl.x + pl.y * pl.y + pl.z * pl.z);
-        pl.x /= norm;
-        pl.y /= norm;
-        pl.z /= norm;
-        pl.x += dnx;
-        pl.y += dny;
-        pl.z += dnz;
-        float a = -pl.x / pl.z;
-        float b = -pl.y / pl.z;
-        float c = z - a * x - b * y;
-        plane[(long)id * K + p] = make_float4(a, b, c, 0.f);
-    }
-    s_ransac[id] = s;
+// one plane of random_candidate_plane (hbp_isp.cu:641-674) with normalize_plane (495-503):
+// four draws from seed s (advanced in place) perturb plane pl around the mean position (x, y)
+__device__ __forceinline__ float4 random_plane(uint32_t &s, float4 pl, float x, float y, float max_dz, float max_dn) {
+    float z = pl.x * x + pl.y * y + pl.z;
+    float dz = lcg_uniform(s, max_dz);
+    z += dz;
+    float dnx = lcg_uniform(s, max_dn);
+    float dny = lcg_uniform(s, max_dn);
+    float dnz = lcg_uniform(s, max_dn);
+    float norm = sqrtf(pl.x * pl.x + pl.y * pl.y + pl.z * pl.z);
+    pl.x /= norm;
+    pl.y /= norm;
+    pl.z /= norm;
+    pl.x += dnx;
+    pl.y += dny;
+    pl.z += dnz;
+    float a = -pl.x / pl.z;
+    float b = -pl.y / pl.z;
+    float c = z - a * x - b * y;
+    return make_float4(a, b, c, 0.f);
 }
 
 enum SweepKind { kNone = 0, kRansac = 1, kSpatial = 2, kRandom = 3 };
@@ -213,8 +240,11 @@ struct SweepArgs {
     long E, N;
     const int *clus;
     const int2 *bd;
-    const uint2 *pix;
+    const uint32_t *pxy;         // packed x | y << 16 of the sorted pixels [N]
+    const uint32_t *pcl;         // superpixel of the sorted pixels [N]
     const int *pix_off;
+    const int4 *seg;             // coarse levels: (first, end, superpixel, index) [nseg]
+    const int *sego;             // coarse levels: first segment of each superpixel [V+1]
     const int *off;
     const int *rev;
     const float *c0;
@@ -230,10 +260,10 @@ struct SweepArgs {
     const int2 *bd_h;
     const int *off_h;
     const float4 *plane_h;
-    // cost volume and parameters
-    const uint8_t *vol;
-    int pitch;
-    long vplane;
+    // census words of the pair [H][W] (the cost volume's entries are recomputed from them),
+    // view of the volume, WTA disparities and parameters
+    const uint32_t *cl, *cr;
+    int right;
     const float *dwta;
     float lambda, tau, tau_s;
     int use_cost_vol, shift;
@@ -252,8 +282,8 @@ struct SweepDesc {
     float ndz, ndn;
 };
 
-constexpr int CH12 = 256 * 2, CH4 = 256 * 8;                    // pixels per plane-cost chunk
-constexpr int MAXC12 = CH12 / 2 + 2, MAXC4 = CH4 / 2 + 2;        // superpixels a chunk can touch
+constexpr int CH_R12 = 2, CH_R4 = 8;                              // 256-pixel steps per chunk
+constexpr int MAXC12 = 256 * CH_R12 / 2 + 2, MAXC4 = 256 * CH_R4 / 2 + 2;  // superpixels a chunk can touch
 constexpr size_t PC_LDS_BYTES =
     (MAXC12 * 12 * 8 + MAXC12 * 4) > (MAXC4 * 4 * 8 + MAXC4 * 4) ? (MAXC12 * 12 * 8 + MAXC12 * 4)
                                                                  : (MAXC4 * 4 * 8 + MAXC4 * 4);
@@ -288,7 +318,7 @@ __device__ void init_node(const SweepArgs &a, long id) {
         a.cost[id * K + k] = 0.f;
     }
     a.accnan[id] = 0u;
-    ransac_node((int)id, a.pix, a.pix_off, a.dwta, a.W, a.s_ransac, a.plane, a.point);
+    ransac_node((int)id, a.pxy, a.pix_off, a.dwta, a.W, a.s_ransac, a.plane, a.point);
 }
 
 // costs of slots 0..2P-1 after init (compute_plane_cost(0), (P)); belief = cost
@@ -305,92 +335,203 @@ __device__ void init_cost_node(const SweepArgs &a, long id) {
 }
 
 // ----------------------------------------------------------------------------------------
-// plane cost: read_plane_cost (hbp_isp.cu:177-228) + the per-superpixel sums, for one chunk
-// of CH = 256*R consecutive pixels of the sorted list (one 256-thread block).  A lane sums
-// its R consecutive pixels while the superpixel stays the same and flushes into LDS int64
-// counters indexed by superpixel - first superpixel of the chunk (superpixels at BP levels
-// hold >= 2 pixels, so a chunk touches at most CH/2 + 1 of them); the block then adds each
-// counter to replica (chunk mod rrep) of the global sum with one atomic.
+// spatial propagation (hbp_isp.cu:1225-1243): candidate p of node id with parent Pp is the
+// slot-0 plane of a random child (select_child_plane) of a random neighbour of Pp at the parent
+// level (neighbor_candidate_plane); each draw is the (p+1)-th LCG step of that parent's seed.
+// Parents at BP levels have at least one neighbour and one child.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ float4 spatial_candidate(const SweepArgs &a, long id, int p) {
+    int Pp = a.clus[id];
+    int nb_pos = a.off_h[Pp], nb_num = a.off_h[Pp + 1] - nb_pos;
+    uint32_t sn = a.s_neighbor[Pp];
+    for (int q = 0; q <= p; ++q) sn = lcg_next(sn);
+    int r = (int)umod0(sn, nb_num);
+    int t = a.bd_h[r + nb_pos].y;
+    uint32_t sc = a.s_child[t];
+    for (int q = 0; q <= p; ++q) sc = lcg_next(sc);
+    int cpos = a.child_off[t], ccnt = a.child_off[t + 1] - cpos;
+    int r2 = (int)umod0(sc, ccnt);
+    int c = a.child[cpos + r2];
+    return a.plane[(long)c * K];
+}
+__device__ void spatial_node(const SweepArgs &a, long id) {
+    for (int p = 0; p < P; ++p) a.plane[id * K + 2 * P + p] = spatial_candidate(a, id, p);
+}
+
+// Entry (d, y, x) of the cost volume, 0 <= d < D, recomputed from the census words exactly as
+// the Hamming kernel stores it: left view min(popc(CL(x) ^ CR(max(x-d, 0))), 25); right view
+// (cost_to_right) the left entry at x' = min(x+d, W-1).  clx = CL(x) of the pixel's row.
+__device__ __forceinline__ float census_entry(const SweepArgs &a, const uint32_t *clrow, const uint32_t *crrow,
+                                              uint32_t clx, int x, int d) {
+    uint32_t l, r;
+    if (!a.right) {
+        int xr = x - d;
+        l = clx;
+        r = crrow[xr < 0 ? 0 : xr];
+    } else {
+        int xl = x + d;
+        xl = xl > a.W - 1 ? a.W - 1 : xl;
+        l = clrow[xl];
+        r = crrow[xl - d];
+    }
+    int c = __popc(l ^ r);
+    return (float)(c < 25 ? c : 25);
+}
+
+// read_plane_cost (hbp_isp.cu:177-228) of one pixel and plane, times lambda
+__device__ __forceinline__ float pixel_plane_cost(const SweepArgs &a, float4 pl, int x, int y, float fx, float fy,
+                                                  const uint32_t *clrow, const uint32_t *crrow, uint32_t clx) {
+    float df = pl.x * fx + pl.y * fy + pl.z;
+    float cost;
+    if (a.use_cost_vol) {
+        int d = floor_to_int(df);
+        if (d < 0 || (long long)d + 1 >= a.D) {
+            cost = a.tau;
+        } else {
+            float v0 = census_entry(a, clrow, crrow, clx, x, d);
+            float v1 = census_entry(a, clrow, crrow, clx, x, d + 1);
+            cost = ((float)(d + 1) - df) * v0 + (df - (float)d) * v1;
+        }
+    } else {
+        float Dw = a.dwta[(long)y * a.W + x];
+        cost = fabsf(Dw - df) <= 1.f ? 0.f : 1.f;
+    }
+    return a.lambda * cost;
+}
+
+// ----------------------------------------------------------------------------------------
+// plane cost, coarse levels: one wavefront per segment (<= SEG pixels of one superpixel).
+// Lane l reads pixels first + l + 64 r; the planes are wave-uniform.  The per-lane int64 sums
+// are reduced across the wavefront and added to replica (segment mod rrep) of the
+// superpixel's sums.  On spatial sweeps lanes 0..P-1 resolve the candidates first and the
+// superpixel's first segment stores them.
+// ----------------------------------------------------------------------------------------
+template <int NS>
+__device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
+    const int lane = threadIdx.x & 63;
+    const int4 sg = a.seg[w];
+    if (sg.y <= sg.x) return;   // past the last segment (the array is padded with empty ones)
+    const int c = __builtin_amdgcn_readfirstlane(sg.z);
+    float4 pl[NS];
+    if (spatial) {
+        float4 cand = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < P) cand = spatial_candidate(a, c, lane);
+        if (sg.w == 0 && lane < P) a.plane[(long)c * K + 2 * P + lane] = cand;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            pl[k] = make_float4(__shfl(cand.x, k), __shfl(cand.y, k), __shfl(cand.z, k), 0.f);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pl[k] = a.plane[(long)c * K + slot0 + k];
+    }
+    int64_t acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0;
+    unsigned int nanm = 0u;
+    const int end = sg.y;
+#pragma unroll 4
+    for (int r = 0; r < SEG_R; ++r) {
+        const int j = sg.x + r * 64 + lane;
+        if (j < end) {
+            uint32_t q = a.pxy[j];
+            int x = unpack_x(q), y = unpack_y(q);
+            float fx = (float)x, fy = (float)y;
+            const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
+            const uint32_t clx = a.right ? 0u : clrow[x];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                float v = pixel_plane_cost(a, pl[k], x, y, fx, fy, clrow, crrow, clx);
+                if (v != v) nanm |= 1u << (slot0 + k);
+                else acc[k] += to_fixed(v, a.shift);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) nanm |= __shfl_xor(nanm, o);
+    unsigned long long mine = 0ull;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        if (lane == k) mine = (unsigned long long)acc[k];
+    const int rep = w % a.rrep;
+    if (lane < NS && mine) atomicAdd(&a.acc[((long)c * a.rrep + rep) * K + slot0 + lane], mine);
+    if (lane == 0 && nanm) atomicOr(&a.accnan[c], nanm);
+}
+
+// ----------------------------------------------------------------------------------------
+// plane cost, fine levels: one 256-thread block per chunk of 256 R consecutive sorted pixels
+// (lane-consecutive reads).  Each step's per-pixel values are summed per superpixel run with a
+// segmented wavefront scan; run tails add into LDS int64 counters indexed by superpixel -
+// first superpixel of the chunk (superpixels at BP levels hold >= 2 pixels, so a chunk touches
+// at most 128 R + 1 of them); the block then adds each counter to replica (chunk mod rrep)
+// of the global sum.
 // ----------------------------------------------------------------------------------------
 template <int NS, int R>
-__device__ void plane_cost_chunk(const SweepArgs &a, long chunk, int slot0, unsigned long long *sacc,
-                                 unsigned int *snan) {
+__device__ void pc_chunk(const SweepArgs &a, long chunk, int slot0, unsigned long long *sacc, unsigned int *snan) {
     constexpr int CH = 256 * R;
     constexpr int MAXC = CH / 2 + 2;
+    const int lane = threadIdx.x & 63;
     const long base = chunk * CH;
     for (int i = threadIdx.x; i < MAXC * NS; i += 256) sacc[i] = 0ull;
     for (int i = threadIdx.x; i < MAXC; i += 256) snan[i] = 0u;
-    const int c_first = (int)a.pix[base].y;
+    const int c_first = (int)a.pcl[base];
     const long last = (base + CH < a.N ? base + CH : a.N) - 1;
-    const int nclus = (int)a.pix[last].y - c_first + 1;
+    const int nclus = (int)a.pcl[last] - c_first + 1;
     __syncthreads();
-
-    const long j0 = base + (long)threadIdx.x * R;
-    int crun = -1;
-    int64_t racc[NS];
-    unsigned int rnan = 0u;
-    float4 pl[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) racc[k] = 0;
     for (int r = 0; r < R; ++r) {
-        long j = j0 + r;
-        if (j >= a.N) break;
-        uint2 q = a.pix[j];
-        int c = (int)q.y;
-        if (c != crun) {
-            if (crun >= 0) {
-                int lc = crun - c_first;
+        const long j = base + (long)r * 256 + threadIdx.x;
+        const bool valid = j < a.N;
+        int c = -1;
+        int64_t v[NS];
+        unsigned int nanm = 0u;
 #pragma unroll
-                for (int k = 0; k < NS; ++k)
-                    if (racc[k]) atomicAdd(&sacc[lc * NS + k], (unsigned long long)racc[k]);
-                if (rnan) atomicOr(&snan[lc], rnan);
-            }
+        for (int k = 0; k < NS; ++k) v[k] = 0;
+        if (valid) {
+            c = (int)a.pcl[j];
+            uint32_t q = a.pxy[j];
+            int x = unpack_x(q), y = unpack_y(q);
+            float fx = (float)x, fy = (float)y;
+            const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
+            const uint32_t clx = a.right ? 0u : clrow[x];
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
-                racc[k] = 0;
-                pl[k] = a.plane[(long)c * K + slot0 + k];
+                float f = pixel_plane_cost(a, a.plane[(long)c * K + slot0 + k], x, y, fx, fy, clrow, crrow, clx);
+                if (f != f) nanm |= 1u << (slot0 + k);
+                else v[k] = to_fixed(f, a.shift);
             }
-            rnan = 0u;
-            crun = c;
         }
-        int x = unpack_x(q.x), y = unpack_y(q.x);
-        float fx = (float)x, fy = (float)y;
-        long pofs = (long)y * a.pitch + x;
+        // inclusive scan within runs of equal superpixel (runs are contiguous: pixels sorted)
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            float df = pl[k].x * fx + pl[k].y * fy + pl[k].z;
-            float cost;
-            if (a.use_cost_vol) {
-                int d = floor_to_int(df);
-                if (d < 0 || (long long)d + 1 >= a.D) {
-                    cost = a.tau;
-                } else {
-                    float v0 = (float)a.vol[(long)d * a.vplane + pofs];
-                    float v1 = (float)a.vol[(long)(d + 1) * a.vplane + pofs];
-                    cost = ((float)(d + 1) - df) * v0 + (df - (float)d) * v1;
-                }
-            } else {
-                float Dw = a.dwta[(long)y * a.W + x];
-                cost = fabsf(Dw - df) <= 1.f ? 0.f : 1.f;
+        for (int o = 1; o < 64; o <<= 1) {
+            int cu = __shfl_up(c, o);
+            bool same = lane >= o && cu == c;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                int64_t u = __shfl_up(v[k], o);
+                if (same) v[k] += u;
             }
-            float v = a.lambda * cost;
-            if (v != v) rnan |= 1u << (slot0 + k);
-            else racc[k] += to_fixed(v, a.shift);
+            unsigned int nu = __shfl_up(nanm, o);
+            if (same) nanm |= nu;
         }
-    }
-    if (crun >= 0) {
-        int lc = crun - c_first;
+        int cd = __shfl_down(c, 1);
+        bool tail = lane == 63 || cd != c;
+        const int lc = c - c_first;
+        if (valid && tail && lc < MAXC) {
 #pragma unroll
-        for (int k = 0; k < NS; ++k)
-            if (racc[k]) atomicAdd(&sacc[lc * NS + k], (unsigned long long)racc[k]);
-        if (rnan) atomicOr(&snan[lc], rnan);
+            for (int k = 0; k < NS; ++k)
+                if (v[k]) atomicAdd(&sacc[lc * NS + k], (unsigned long long)v[k]);
+            if (nanm) atomicOr(&snan[lc], nanm);
+        }
     }
     __syncthreads();
     const int rep = (int)(chunk % a.rrep);
     for (int i = threadIdx.x; i < nclus * NS; i += 256) {
-        unsigned long long v = sacc[i];
+        unsigned long long s = sacc[i];
         int lc = i / NS, k = i % NS;
-        if (v) atomicAdd(&a.acc[((long)(c_first + lc) * a.rrep + rep) * K + slot0 + k], v);
+        if (s) atomicAdd(&a.acc[((long)(c_first + lc) * a.rrep + rep) * K + slot0 + k], s);
     }
     for (int i = threadIdx.x; i < nclus; i += 256)
         if (snan[i]) atomicOr(&a.accnan[c_first + i], snan[i]);
@@ -398,13 +539,15 @@ __device__ void plane_cost_chunk(const SweepArgs &a, long chunk, int slot0, unsi
 }
 
 // ----------------------------------------------------------------------------------------
-// messages for 16 edges (one 256-thread block): passing_msg_k (hbp_isp.cu:292-304),
-// smooth_msg_k (306-355), normalize_msg_k (273-290).  Lane y < 12 of edge e = (s, t) computes
-// the message into s for s's slot y from t's 8 best + prior particles.
+// messages for 4 edges (one wavefront): passing_msg_k (hbp_isp.cu:292-304), smooth_msg_k
+// (306-355), normalize_msg_k (273-290).  Lane y < 12 of edge e = (s, t) computes the message
+// into s for s's slot y from t's 8 best + prior particles.  On spatial sweeps whose candidates
+// are resolved in this kernel, s's candidate planes are resolved here too.
 // ----------------------------------------------------------------------------------------
-__device__ void edge_group(const SweepArgs &a, long grp) {
-    const long e = grp * 16 + (threadIdx.x >> 4);
-    const int y = threadIdx.x & 15;
+__device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
+    const int lane = threadIdx.x & 63;
+    const long e = wv * 4 + (lane >> 4);
+    const int y = lane & 15;
     const bool live = e < a.E && y < K;
     float out = 0.f;
     if (live) {
@@ -420,7 +563,7 @@ __device__ void edge_group(const SweepArgs &a, long grp) {
         X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
         float c0 = a.c0[e];
         float dpos = a.dpos[e];
-        float4 psl = a.plane[(long)s * K + y];
+        float4 psl = (spatial && y >= 2 * P) ? spatial_candidate(a, s, y - 2 * P) : a.plane[(long)s * K + y];
         float4 m0 = a.msg[2 * e], m1 = a.msg[2 * e + 1];
         float mh[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
         float minimum = FLT_MAX;
@@ -446,7 +589,7 @@ __device__ void edge_group(const SweepArgs &a, long grp) {
     }
     // normalize_msg_k: mean over the 12 slots, summed in slot order
     float mean = 0.f;
-    const int g = (threadIdx.x & 63) & ~15;
+    const int g = lane & ~15;
 #pragma unroll
     for (int k = 0; k < K; ++k) mean += __shfl(out, g + k, 64);
     mean /= K;
@@ -454,90 +597,92 @@ __device__ void edge_group(const SweepArgs &a, long grp) {
 }
 
 // ----------------------------------------------------------------------------------------
-// node update: aggregate_belief (hbp_isp.cu:1048-1058) in edge order, ping_pang_msg
-// (1059-1066), get_belief_order (432-450), select_belief (452-472), select_msg (474-493);
-// then the parents' seed advance of a spatial sweep and the next sweep's candidates.
+// node update with 16 lanes per superpixel s, lane k < 12 owning slot k: aggregate_belief
+// (hbp_isp.cu:1048-1058) in edge order, ping_pang_msg (1059-1066), get_belief_order
+// (432-450), select_belief (452-472), select_msg (474-493); then the parents' seed advance of
+// a spatial sweep and the next sweep's candidates.  Every location a lane stores was read by
+// that same lane or reaches it through a cross-lane shuffle, so no lane overwrites data
+// another lane has still to read.
 // ----------------------------------------------------------------------------------------
-__device__ void node_update(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn) {
-    float cost[K], bel[K];
-    unsigned int nb = a.accnan[s];
-#pragma unroll
-    for (int k = 0; k < 2 * P; ++k) cost[k] = a.cost[s * K + k];
-#pragma unroll
-    for (int k = 2 * P; k < K; ++k) {
-        cost[k] = from_fixed(acc_sum(a, s, k), (nb >> k) & 1u, a.shift);
-        acc_zero(a, s, k);
-    }
-    a.accnan[s] = 0u;
-#pragma unroll
-    for (int k = 0; k < K; ++k) bel[k] = cost[k];
+__device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn) {
+    constexpr int MAXDEG = 16;   // incoming messages held in registers (the rest are re-read)
+    const int lane = threadIdx.x & 63;
+    const int k = lane & 15;
+    const int g = lane & ~15;
+    const bool own = k < K;
+    const int kk = own ? k : 0;
+    // independent loads first
     const int eb = a.off[s], ee = a.off[s + 1];
-    for (int e = eb; e < ee; ++e) {
-        const float4 *mt = (const float4 *)(a.msg_t + (long)e * K);
-        float4 u0 = mt[0], u1 = mt[1], u2 = mt[2];
-        bel[0] += u0.x; bel[1] += u0.y; bel[2] += u0.z; bel[3] += u0.w;
-        bel[4] += u1.x; bel[5] += u1.y; bel[6] += u1.z; bel[7] += u1.w;
-        bel[8] += u2.x; bel[9] += u2.y; bel[10] += u2.z; bel[11] += u2.w;
+    const int deg = ee - eb;
+    const int ppos = a.pix_off[s], pn = a.pix_off[s + 1] - ppos;
+    // the NaN bits of the candidate sums and the RANSAC seed: lane 2P reads (and rewrites)
+    // them, the others get them by shuffle
+    unsigned int nb = 0u;
+    uint32_t s0 = 0u;
+    if (k == 2 * P) {
+        nb = a.accnan[s];
+        s0 = a.s_ransac[s];
     }
-    int order[K];
+    float cost = kk < 2 * P ? a.cost[s * K + kk] : 0.f;
+    int64_t fsum = kk >= 2 * P ? acc_sum(a, s, kk) : 0;
+    const float4 pin = a.plane[s * K + kk];
+    const int rv = (k < deg && k < MAXDEG) ? a.rev[eb + k] : 0;
+    float mt[MAXDEG];
 #pragma unroll
-    for (int k = 0; k < K; ++k) order[k] = 0;
-#pragma unroll
-    for (int i = 0; i < K - 1; ++i)
-#pragma unroll
-        for (int j = i + 1; j < K; ++j) {
-            if (bel[j] < bel[i]) order[i]++;
-            else order[j]++;
-        }
-    // in-place compaction (select_belief / select_msg): output slot j < nsel takes the j-th
-    // selected slot in slot order, slots j >= nsel keep their own value.
-    bool sel[K];
-    int pos[K];
-    int nsel = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        sel[k] = order[k] < P;
-        pos[k] = nsel;
-        nsel += sel[k] ? 1 : 0;
+    for (int i = 0; i < MAXDEG; ++i) mt[i] = i < deg ? a.msg_t[(long)(eb + i) * K + kk] : 0.f;
+    if (k == 2 * P) a.accnan[s] = 0u;
+    nb = __shfl(nb, g + 2 * P);
+    if (kk >= 2 * P) {
+        cost = from_fixed(fsum, (nb >> kk) & 1u, a.shift);
+        acc_zero(a, s, kk);
     }
-    float4 *pl = a.plane + s * K;
-    float4 pin[K];
+    // belief = cost + incoming messages in edge order
+    float bel = cost;
 #pragma unroll
-    for (int k = 0; k < K; ++k) pin[k] = pl[k];
+    for (int i = 0; i < MAXDEG; ++i)
+        if (i < deg) bel += mt[i];
+    for (int e = eb + MAXDEG; e < ee; ++e) bel += a.msg_t[(long)e * K + kk];
+    // rank: order[k] = #{j < k: !(bel[k] < bel[j])} + #{j > k: bel[j] < bel[k]}
+    int order = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        float cv = cost[j], bv = bel[j];
-        float4 pv = pin[j];
-#pragma unroll
-        for (int k = j; k < K; ++k)
-            if (sel[k] && pos[k] == j) {
-                cv = cost[k];
-                bv = bel[k];
-                pv = pin[k];
-            }
-        pl[j] = pv;
-        a.cost[s * K + j] = cv;
-        a.belief[s * K + j] = bv;
+        float bj = __shfl(bel, g + j);
+        if (j < k) order += (bel < bj) ? 0 : 1;
+        else if (j > k) order += (bj < bel) ? 1 : 0;
+    }
+    const bool sel = own && order < P;
+    const uint32_t m = (uint32_t)(__ballot(sel) >> g) & 0xfffu;
+    const int nsel = __popc(m);
+    // output slot k takes the k-th selected slot (k < nsel) or keeps its own value
+    int src = k;
+    if (k < nsel) {
+        uint32_t mm = m;
+        for (int i = 0; i < k; ++i) mm &= mm - 1u;
+        src = __ffs(mm) - 1;
+    }
+    const float cost_n = __shfl(cost, g + src), bel_n = __shfl(bel, g + src);
+    const float4 pl_n = make_float4(__shfl(pin.x, g + src), __shfl(pin.y, g + src), __shfl(pin.z, g + src),
+                                    __shfl(pin.w, g + src));
+    if (own) {
+        a.plane[s * K + k] = pl_n;
+        a.cost[s * K + k] = cost_n;
+        a.belief[s * K + k] = bel_n;
     }
     // messages over s's particles -> reverse edge, compacted by s's order
-    for (int e = eb; e < ee; ++e) {
-        const float4 *mt = (const float4 *)(a.msg_t + (long)e * K);
-        float4 u0 = mt[0], u1 = mt[1], u2 = mt[2];
-        float m[K] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w, u2.x, u2.y, u2.z, u2.w};
-        float o[2 * P];
+    float *msg = (float *)a.msg;
 #pragma unroll
-        for (int j = 0; j < 2 * P; ++j) {
-            float v = m[j];
-#pragma unroll
-            for (int k = j; k < K; ++k)
-                if (sel[k] && pos[k] == j) v = m[k];
-            o[j] = v;
+    for (int i = 0; i < MAXDEG; ++i) {
+        if (i < deg) {
+            const float o = __shfl(mt[i], g + src);
+            const int r = __shfl(rv, g + i);
+            if (k < 2 * P) msg[(long)r * 8 + k] = o;
         }
-        long r = a.rev[e];
-        a.msg[2 * r] = make_float4(o[0], o[1], o[2], o[3]);
-        a.msg[2 * r + 1] = make_float4(o[4], o[5], o[6], o[7]);
     }
-    if (this_kind == kSpatial && s < a.Vh) {
+    for (int e = eb + MAXDEG; e < ee; ++e) {
+        const float o = __shfl(a.msg_t[(long)e * K + kk], g + src);
+        if (k < 2 * P) msg[(long)a.rev[e] * 8 + k] = o;
+    }
+    if (this_kind == kSpatial && s < a.Vh && k == 0) {
         uint32_t sc = a.s_child[s], sn = a.s_neighbor[s];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -547,50 +692,62 @@ __device__ void node_update(const SweepArgs &a, long s, int this_kind, int next_
         a.s_child[s] = sc;
         a.s_neighbor[s] = sn;
     }
-    if (next_kind == kRansac) ransac_node((int)s, a.pix, a.pix_off, a.dwta, a.W, a.s_ransac, a.plane, a.point);
-    else if (next_kind == kRandom) random_node((int)s, a.mxy, max_dz, max_dn, a.s_ransac, a.plane);
-}
-
-// ----------------------------------------------------------------------------------------
-// spatial propagation (hbp_isp.cu:1225-1243): for child id with parent P, candidate p is the
-// slot-0 plane of a random child (select_child_plane) of a random neighbour of P at the parent
-// level (neighbor_candidate_plane), each draw being the (p+1)-th LCG step of that parent's seed.
-// ----------------------------------------------------------------------------------------
-__device__ void spatial_node(const SweepArgs &a, long id) {
-    int Pp = a.clus[id];
-    int nb_pos = a.off_h[Pp], nb_num = a.off_h[Pp + 1] - nb_pos;
-    uint32_t sn = a.s_neighbor[Pp];
-    for (int p = 0; p < P; ++p) {
-        sn = lcg_next(sn);
-        int r = (int)umod0(sn, nb_num);
-        int t = a.bd_h[r + nb_pos].y;
-        uint32_t sc = a.s_child[t];
-        for (int q = 0; q <= p; ++q) sc = lcg_next(sc);
-        int cpos = a.child_off[t], ccnt = a.child_off[t + 1] - cpos;
-        int r2 = (int)umod0(sc, ccnt);
-        int c = a.child[cpos + r2];
-        a.plane[id * K + 2 * P + p] = a.plane[(long)c * K];
+    // next candidates: lane p in [2P, 3P) makes candidate p from the seed after the draws of
+    // candidates 2P..p-1; lane 2P also stores the seed after all of them
+    if (next_kind == kRansac || next_kind == kRandom) {
+        s0 = __shfl(s0, g + 2 * P);
+        if (k >= 2 * P && own) {
+            const int steps = next_kind == kRansac ? 3 : 4;
+            uint32_t sk = s0;
+            for (int i = 0; i < steps * (k - 2 * P); ++i) sk = lcg_next(sk);
+            if (next_kind == kRansac) {
+                uint4 q = make_uint4(0u, 0u, 0u, 0u);
+                a.plane[s * K + k] = ransac_plane(sk, a.pxy, ppos, pn, a.dwta, a.W, q);
+                if (k == K - 1) a.point[s] = q;
+            } else {
+                float2 mxy = a.mxy[s];
+                a.plane[s * K + k] = random_plane(sk, pl_n, mxy.x, mxy.y, max_dz, max_dn);
+            }
+            if (k == 2 * P) {
+                uint32_t sf = s0;
+                for (int i = 0; i < steps * P; ++i) sf = lcg_next(sf);
+                a.s_ransac[s] = sf;
+            }
+        }
     }
 }
 
 // ----------------------------------------------------------------------------------------
-// one kernel per phase (used for per-kernel measurements and step-by-step runs)
+// kernels
 // ----------------------------------------------------------------------------------------
 #define GRID_STRIDE(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
 
 __global__ __launch_bounds__(256) void init_level_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) init_node(a, id); }
 __global__ __launch_bounds__(256) void init_cost_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) init_cost_node(a, id); }
 __global__ __launch_bounds__(256) void spatial_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) spatial_node(a, id); }
-__global__ __launch_bounds__(128) void node_kernel(SweepArgs a, int this_kind, int next_kind, float ndz, float ndn) {
-    GRID_STRIDE(s, a.V) node_update(a, s, this_kind, next_kind, ndz, ndn);
+__global__ __launch_bounds__(256) void node_kernel(SweepArgs a, int this_kind, int next_kind, float ndz, float ndn) {
+    const long s = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
+    if (s < a.V) node_group(a, s, this_kind, next_kind, ndz, ndn);
 }
-template <int NS, int R>
-__global__ __launch_bounds__(256) void plane_cost_kernel(SweepArgs a, int slot0) {
+
+// Plane costs of slots [slot0, slot0+NS) and, for NS == P, the sweep's messages.  Coarse
+// levels: wavefronts [0, npc) take segments and the rest take 4 edges each.  Fine levels:
+// blocks [0, npc) take chunks and the waves of the other blocks take 4 edges each.
+template <int NS, bool COARSE>
+__global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, int spatial, int npc) {
     extern __shared__ unsigned long long lds_pc[];
-    constexpr int MAXC = 256 * R / 2 + 2;
-    plane_cost_chunk<NS, R>(a, blockIdx.x, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (COARSE) {
+        const int w = blockIdx.x * 4 + wave;
+        if (w < npc) pc_segment<NS>(a, w, slot0, spatial != 0);
+        else if (NS == P) edge_wave(a, w - npc, spatial != 0);
+    } else {
+        constexpr int R = NS == P ? CH_R4 : CH_R12;
+        constexpr int MAXC = 256 * R / 2 + 2;
+        if ((int)blockIdx.x < npc) pc_chunk<NS, R>(a, blockIdx.x, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
+        else if (NS == P) edge_wave(a, (long)(blockIdx.x - npc) * 4 + wave, false);
+    }
 }
-__global__ __launch_bounds__(256) void edge_kernel(SweepArgs a) { edge_group(a, blockIdx.x); }
 
 __global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -599,32 +756,44 @@ __global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
 
 // ----------------------------------------------------------------------------------------
 // all sweeps [s_begin, s_end) of one level in one cooperative launch: per sweep
-//   (spatial candidates | grid barrier)  plane-cost chunks + message groups  | barrier
-//   node updates (+ next candidates)  | barrier
-// Work items are spread over the blocks in grid-stride order.  With first_costed, the first
-// sweep's candidate costs were computed by the level init and its plane-cost phase is skipped.
+//   (fine levels, spatial sweeps: candidates | grid barrier)
+//   plane costs + messages | barrier | node updates (+ next candidates) | barrier
+// Work items are spread over the wavefronts / blocks in grid-stride order.  With first_costed,
+// the first sweep's candidate costs were computed by the level init and its plane-cost work is
+// skipped.
 // ----------------------------------------------------------------------------------------
+template <bool COARSE>
 __global__ __launch_bounds__(256) void bp_level_kernel(SweepArgs a, const SweepDesc *__restrict__ sw, int s_begin,
-                                                       int s_end, int first_costed) {
+                                                       int s_end, int first_costed, int npc) {
     extern __shared__ unsigned long long lds_pc[];
     auto grid = cg::this_grid();
-    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long gth = (long)gridDim.x * blockDim.x;
-    const long nch4 = (a.N + CH4 - 1) / CH4;
-    const long neg = (a.E + 15) / 16;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const long nwaves = (long)gridDim.x * 4;
+    const long nedge = (a.E + 3) / 4;
+    const long ngroups = ((long)a.V * 16 + 255) / 256 * 4;   // node waves
     for (int i = s_begin; i < s_end; ++i) {
         const SweepDesc d = sw[i];
-        if (d.kind == kSpatial) {
-            for (long id = gtid; id < a.V; id += gth) spatial_node(a, id);
-            grid.sync();
-        }
-        const long nch = (first_costed && i == s_begin) ? 0 : nch4;
-        for (long w = blockIdx.x; w < nch + neg; w += gridDim.x) {
-            if (w < nch) plane_cost_chunk<4, 8>(a, w, 2 * P, lds_pc, (unsigned int *)(lds_pc + MAXC4 * 4));
-            else edge_group(a, w - nch);
+        const bool spatial = d.kind == kSpatial;
+        const int np = (first_costed && i == s_begin) ? 0 : npc;
+        if (COARSE) {
+            for (long w = (long)blockIdx.x * 4 + wave; w < np + nedge; w += nwaves) {
+                if (w < np) pc_segment<P>(a, (int)w, 2 * P, spatial);
+                else edge_wave(a, w - np, spatial);
+            }
+        } else {
+            if (spatial) {
+                for (long id = (long)blockIdx.x * 256 + threadIdx.x; id < a.V; id += nwaves * 64) spatial_node(a, id);
+                grid.sync();
+            }
+            for (long b = blockIdx.x; b < np; b += gridDim.x)
+                pc_chunk<P, CH_R4>(a, b, 2 * P, lds_pc, (unsigned int *)(lds_pc + MAXC4 * P));
+            for (long w = (long)blockIdx.x * 4 + wave; w < nedge; w += nwaves) edge_wave(a, w, false);
         }
         grid.sync();
-        for (long s = gtid; s < a.V; s += gth) node_update(a, s, d.kind, d.next_kind, d.ndz, d.ndn);
+        for (long w = (long)blockIdx.x * 4 + wave; w < ngroups; w += nwaves) {
+            const long s = w * 4 + ((threadIdx.x & 63) >> 4);
+            if (s < a.V) node_group(a, s, d.kind, d.next_kind, d.ndz, d.ndn);
+        }
         grid.sync();
     }
 }
@@ -654,12 +823,13 @@ __global__ void map_best_kernel(int V, const float *__restrict__ belief, const f
         best[c] = plane[c * K + bp];
     }
 }
-__global__ void map_paint_kernel(const uint2 *__restrict__ pix, long N, const float4 *__restrict__ best, int W, int D,
-                                 float *__restrict__ disp, float *__restrict__ plane3) {
+__global__ void map_paint_kernel(const uint32_t *__restrict__ pxy, const uint32_t *__restrict__ pcl, long N,
+                                 const float4 *__restrict__ best, int W, int D, float *__restrict__ disp,
+                                 float *__restrict__ plane3) {
     for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long)gridDim.x * blockDim.x) {
-        uint2 q = pix[j];
-        int x = unpack_x(q.x), y = unpack_y(q.x);
-        float4 pl = best[q.y];
+        uint32_t q = pxy[j];
+        int x = unpack_x(q), y = unpack_y(q);
+        float4 pl = best[pcl[j]];
         float d = pl.x * (float)x + pl.y * (float)y + pl.z;
         d = fminf(fmaxf(d, 0.0f), (float)(D - 1));
         long i = (long)y * W + x;
@@ -746,8 +916,11 @@ static SweepArgs sweep_args(BP &b, int l) {
     a.N = b.N;
     a.clus = B.clus;
     a.bd = B.bd;
-    a.pix = B.pix.p;
+    a.pxy = B.pxy.p;
+    a.pcl = B.pcl.p;
     a.pix_off = B.pix_off.p;
+    a.seg = B.seg.p;
+    a.sego = B.sego.p;
     a.off = B.off.p;
     a.rev = B.rev.p;
     a.c0 = B.c0.p;
@@ -765,9 +938,9 @@ static SweepArgs sweep_args(BP &b, int l) {
         a.off_h = Hh.off.p;
         a.plane_h = Hh.plane.p;
     }
-    a.vol = v.vol.p;
-    a.pitch = v.pitch;
-    a.vplane = v.plane;
+    a.cl = v.cl.p;
+    a.cr = v.cr.p;
+    a.right = v.right ? 1 : 0;
     a.dwta = v.dwta.p;
     a.lambda = b.lambda;
     a.tau = b.tau;
@@ -822,15 +995,29 @@ void bp_setup(Ctx &c, BP &b) {
             HBP_CHECK(hipMemcpyAsync(keys.p, im_clus.p, sizeof(unsigned) * N, hipMemcpyDeviceToDevice, c.stream));
             hipLaunchKernelGGL(iota_i32_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, ids.p, N);
             sort_pairs_u32(c, keys.p, keys2.p, ids.p, ids2.p, N, bits_for(V));
-            B.pix.alloc(c.pool, N);
+            B.pxy.alloc(c.pool, N);
+            B.pcl.alloc(c.pool, N);
             hipLaunchKernelGGL(pix_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, keys.p, ids.p, N, b.W,
-                               B.pix.p);
-            DevBuf<int> cnt(c.pool, (size_t)V + 1);
+                               B.pxy.p, B.pcl.p);
             B.pix_off.alloc(c.pool, (size_t)V + 1);
-            HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
-            hipLaunchKernelGGL(count_u32_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, im_clus.p, N, cnt.p);
-            scan_exclusive_int(c, cnt.p, B.pix_off.p, V + 1, nullptr);
+            hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, keys.p, N, V,
+                               B.pix_off.p);
+            // coarse levels: segments of <= SEG pixels per superpixel
+            B.coarse = N >= (long)COARSE_MIN_PIX * V;
+            if (B.coarse) {
+                DevBuf<int> scnt(c.pool, (size_t)V + 1);
+                B.sego.alloc(c.pool, (size_t)V + 1);
+                B.nseg_max = (int)(div_up(N, SEG) + V);
+                B.seg.alloc(c.pool, (size_t)B.nseg_max);
+                HBP_CHECK(hipMemsetAsync(B.seg.p, 0, sizeof(int4) * (size_t)B.nseg_max, c.stream));
+                hipLaunchKernelGGL(seg_count_kernel, dim3(grid_for(c, (long)V + 1)), dim3(256), 0, c.stream,
+                                   B.pix_off.p, V, scnt.p);
+                scan_exclusive_int(c, scnt.p, B.sego.p, V + 1, nullptr);
+                hipLaunchKernelGGL(seg_fill_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.pix_off.p,
+                                   B.sego.p, V, B.seg.p);
+            }
             // CSR by source, reverse edges, edge weights, mean positions
+            DevBuf<int> cnt(c.pool, (size_t)V + 1);
             B.off.alloc(c.pool, (size_t)V + 1);
             HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
             hipLaunchKernelGGL(count_src_kernel2, dim3(grid_for(c, E)), dim3(256), 0, c.stream, B.bd, E, cnt.p);
@@ -846,16 +1033,14 @@ void bp_setup(Ctx &c, BP &b) {
             // children grouped by parent (sort_clus, hbp_isp.cu:985-1001)
             {
                 DevBuf<unsigned int> ck(c.pool, V), ck2(c.pool, V);
-                DevBuf<int> cv2(c.pool, V), ccnt(c.pool, (size_t)B.Vh + 1);
+                DevBuf<int> cv2(c.pool, V);
                 B.child.alloc(c.pool, V);
                 B.child_off.alloc(c.pool, (size_t)B.Vh + 1);
                 HBP_CHECK(hipMemcpyAsync(ck.p, B.clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
                 hipLaunchKernelGGL(iota_i32_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.child.p, (long)V);
                 sort_pairs_u32(c, ck.p, ck2.p, B.child.p, cv2.p, V, bits_for(B.Vh));
-                HBP_CHECK(hipMemsetAsync(ccnt.p, 0, sizeof(int) * ((size_t)B.Vh + 1), c.stream));
-                hipLaunchKernelGGL(count_i32_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.clus, (long)V,
-                                   ccnt.p);
-                scan_exclusive_int(c, ccnt.p, B.child_off.p, B.Vh + 1, nullptr);
+                hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, ck.p, (long)V,
+                                   B.Vh, B.child_off.p);
             }
             B.plane.alloc(c.pool, (size_t)V * K);
             B.cost.alloc(c.pool, (size_t)V * K);
@@ -893,53 +1078,82 @@ void bp_setup(Ctx &c, BP &b) {
 }
 
 
-// algorithmic HBM bytes per launch (pixel id 4 B + 2 cost bytes per plane and pixel; per
-// superpixel the planes read and the sums written)
-static double plane_cost_bytes(long N, int V, int NS) { return (double)N * (4 + 2 * NS) + (double)V * (16 * NS + 8 + 4 * NS); }
+// algorithmic HBM bytes per launch: per pixel its packed position, its left census word and
+// one right census word (12 B; the right census row a pixel's planes gather from is read once
+// per row); per superpixel the planes read, the int64 sums and the NaN word written; per edge
+// the message inputs (6 packed points, weights, 8 prior messages, 8 beliefs and 8 planes of t)
+// and 12 outputs
+static double plane_cost_bytes(long N, int V, int NS) { return 12.0 * N + (double)V * (24.0 * NS + 4); }
 static double edge_bytes(long E, int V) { return 96.0 * E + (16.0 * K + 48) * V; }
 static double node_bytes(long E, int V) { return 84.0 * E + (48.0 * K + 16) * V; }
 
-template <int NS, int R>
-static void launch_plane_cost(Ctx &c, const SweepArgs &a, int l, int slot0) {
-    constexpr int CH = 256 * R;
-    ProfScope ps(c, "plane_cost_L" + std::to_string(l), plane_cost_bytes(a.N, a.V, NS));
-    hipLaunchKernelGGL((plane_cost_kernel<NS, R>), dim3(div_up(a.N, CH)), dim3(256), PC_LDS_BYTES, c.stream, a,
-                       slot0);
+// plane costs of slots [slot0, slot0 + NS) (+ the sweep's messages when with_edges)
+template <int NS>
+static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l, int slot0, bool spatial,
+                             bool costs, bool with_edges) {
+    long nedge_w = with_edges ? div_up(a.E, 4) : 0;
+    double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
+    ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
+    if (B.coarse) {
+        int npc = costs ? B.nseg_max : 0;
+        long blocks = div_up((long)npc + nedge_w, 4);
+        if (blocks == 0) return;
+        hipLaunchKernelGGL((cost_edge_kernel<NS, true>), dim3(blocks), dim3(256), 0, c.stream, a, slot0,
+                           spatial ? 1 : 0, npc);
+    } else {
+        constexpr int R = NS == P ? CH_R4 : CH_R12;
+        int npc = costs ? (int)div_up(a.N, 256 * R) : 0;
+        long blocks = npc + div_up(nedge_w, 4);
+        if (blocks == 0) return;
+        hipLaunchKernelGGL((cost_edge_kernel<NS, false>), dim3(blocks), dim3(256), PC_LDS_BYTES, c.stream, a, slot0,
+                           0, npc);
+    }
+}
+
+// level init: prior planes, seeds, first RANSAC candidates, costs of all 12 slots
+static void run_level_init(Ctx &c, BP &b, const SweepArgs &a, int l) {
+    const BPLevel &B = b.lv[l];
+    HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)a.E, c.stream));
+    hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
+    launch_cost_edge<K>(c, a, B, l, 0, false, true, false);
+    hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
+    HBP_CHECK(hipGetLastError());
 }
 
 // one sweep as separate kernels
 static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float ndz, float ndn) {
     int l = sw.level;
+    const BPLevel &B = b.lv[l];
     SweepArgs a = sweep_args(b, l);
+    bool spatial = sw.kind == kSpatial;
     if (sw.first) {
-        HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)a.E, c.stream));
-        hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
-        launch_plane_cost<12, 2>(c, a, l, 0);
-        hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
+        run_level_init(c, b, a, l);
+        launch_cost_edge<P>(c, a, B, l, 2 * P, false, false, a.E > 0);
     } else {
-        if (sw.kind == kSpatial) {
+        if (spatial && !B.coarse) {
             ProfScope pss(c, "spatial_L" + std::to_string(l), (16.0 * P + 28) * a.V);
             hipLaunchKernelGGL(spatial_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
         }
-        launch_plane_cost<4, 8>(c, a, l, 2 * P);
-    }
-    if (a.E > 0) {
-        ProfScope ps(c, "edge_L" + std::to_string(l), edge_bytes(a.E, a.V));
-        hipLaunchKernelGGL(edge_kernel, dim3(div_up(a.E, 16)), dim3(256), 0, c.stream, a);
+        launch_cost_edge<P>(c, a, B, l, 2 * P, spatial && B.coarse, true, a.E > 0);
     }
     {
         ProfScope psn(c, "node_L" + std::to_string(l), node_bytes(a.E, a.V));
-        hipLaunchKernelGGL(node_kernel, dim3(grid_for(c, a.V, 128)), dim3(128), 0, c.stream, a, sw.kind, next_kind,
-                           ndz, ndn);
+        hipLaunchKernelGGL(node_kernel, dim3(div_up((long)a.V * 16, 256)), dim3(256), 0, c.stream, a, sw.kind,
+                           next_kind, ndz, ndn);
     }
     HBP_CHECK(hipGetLastError());
 }
 
 int bp_level_coop_blocks(int num_cu) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_level_kernel, 256, PC_LDS_BYTES) != hipSuccess ||
+    int per_cu = 0, per_cu2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_level_kernel<true>, 256, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bp_level_kernel<false>, 256, PC_LDS_BYTES) !=
+            hipSuccess ||
+        per_cu2 < 1)
+        per_cu2 = 1;
+    per_cu = std::min(per_cu, per_cu2);
     if (per_cu > 4) per_cu = 4;
     return per_cu * num_cu;
 }
@@ -948,6 +1162,7 @@ int bp_level_coop_blocks(int num_cu) {
 static void run_sweeps_fused(Ctx &c, BP &b, size_t i0, size_t i1) {
     const Sweep &s0 = b.sched[i0];
     int l = s0.level;
+    const BPLevel &B = b.lv[l];
     SweepArgs a = sweep_args(b, l);
     // the device copy of the level's schedule
     std::vector<SweepDesc> h(i1 - i0);
@@ -964,23 +1179,22 @@ static void run_sweeps_fused(Ctx &c, BP &b, size_t i0, size_t i1) {
     b.sweep_desc.alloc(c.pool, h.size() * sizeof(SweepDesc));
     HBP_CHECK(hipMemcpyAsync(b.sweep_desc.p, h.data(), h.size() * sizeof(SweepDesc), hipMemcpyHostToDevice,
                              c.stream));
-    if (s0.first) {
-        // level init as separate kernels (its 12-slot plane cost needs more registers)
-        HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)a.E, c.stream));
-        hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
-        launch_plane_cost<12, 2>(c, a, l, 0);
-        hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
-        HBP_CHECK(hipGetLastError());
-    }
-    long work = std::max({(long)div_up(a.N, CH4), (long)div_up(a.E, 16), (long)div_up(a.V, 256), 1L});
+    if (s0.first) run_level_init(c, b, a, l);
+    int npc = B.coarse ? B.nseg_max : (int)div_up(a.N, 256 * CH_R4);
+    long work = B.coarse ? div_up((long)npc + div_up(a.E, 4), 4) : npc + div_up(a.E, 16);
+    work = std::max({work, (long)div_up((long)a.V * 16, 256), 1L});
     int blocks = (int)std::min<long>(work, c.bp_coop_blocks);
     const SweepDesc *dsw = (const SweepDesc *)b.sweep_desc.p;
     int sb = 0, se = (int)(i1 - i0), fc = s0.first ? 1 : 0;
-    void *args[] = {(void *)&a, (void *)&dsw, (void *)&sb, (void *)&se, (void *)&fc};
-    double bytes = (double)(i1 - i0) * (plane_cost_bytes(a.N, a.V, 4) + edge_bytes(a.E, a.V) + node_bytes(a.E, a.V));
+    void *args[] = {(void *)&a, (void *)&dsw, (void *)&sb, (void *)&se, (void *)&fc, (void *)&npc};
+    double bytes = (double)(i1 - i0) * (plane_cost_bytes(a.N, a.V, P) + edge_bytes(a.E, a.V) + node_bytes(a.E, a.V));
     ProfScope ps(c, "bp_level_L" + std::to_string(l), bytes);
-    HBP_CHECK(hipLaunchCooperativeKernel((const void *)bp_level_kernel, dim3(blocks), dim3(256), args, PC_LDS_BYTES,
-                                         c.stream));
+    if (B.coarse)
+        HBP_CHECK(hipLaunchCooperativeKernel((const void *)bp_level_kernel<true>, dim3(blocks), dim3(256), args, 0,
+                                             c.stream));
+    else
+        HBP_CHECK(hipLaunchCooperativeKernel((const void *)bp_level_kernel<false>, dim3(blocks), dim3(256), args,
+                                             PC_LDS_BYTES, c.stream));
 }
 
 int bp_step(Ctx &c, BP &b, int max_sweeps) {
@@ -1024,8 +1238,8 @@ void bp_map(Ctx &c, BP &b) {
     b.best.alloc(c.pool, B.V);
     hipLaunchKernelGGL(map_best_kernel, dim3(grid_for(c, B.V)), dim3(256), 0, c.stream, B.V, B.belief.p, B.plane.p,
                        b.best.p);
-    hipLaunchKernelGGL(map_paint_kernel, dim3(grid_for(c, b.N)), dim3(256), 0, c.stream, B.pix.p, b.N, b.best.p, b.W,
-                       b.D, b.disp.p, b.plane3.p);
+    hipLaunchKernelGGL(map_paint_kernel, dim3(grid_for(c, b.N)), dim3(256), 0, c.stream, B.pxy.p, B.pcl.p, b.N,
+                       b.best.p, b.W, b.D, b.disp.p, b.plane3.p);
     HBP_CHECK(hipGetLastError());
     b.mapped = true;
 }

@@ -174,6 +174,14 @@ int hbp_profile_read(hbp_ctx *ctx, const char *kernel, double *ms, long long *la
     HBP_API_END
 }
 
+int hbp_profile_only(hbp_ctx *ctx, const char *prefix) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx, "ctx is NULL");
+    ctx->c.prof.flush();
+    ctx->c.prof.only = prefix ? prefix : "";
+    HBP_API_END
+}
+
 int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen) {
     if (!ctx) return -1;
     std::string all;

@@ -202,6 +202,7 @@ struct KernelStat {
 };
 struct Profiler {
     bool on = false;
+    std::string only;   // when set, only launches whose name starts with it are timed
     std::vector<std::string> names;
     std::vector<KernelStat> stats;
     struct Pending {
@@ -277,6 +278,7 @@ class ProfScope {
 public:
     ProfScope(Ctx &c, const std::string &name, double bytes) : c_(c) {
         if (!c.prof.on) return;
+        if (!c.prof.only.empty() && name.compare(0, c.prof.only.size(), c.prof.only) != 0) return;
         id_ = c.prof.id_of(name);
         a_ = c.prof.take();
         b_ = c.prof.take();
@@ -284,7 +286,7 @@ public:
         if (a_ && b_) (void)hipEventRecord(a_, c.stream);
     }
     ~ProfScope() {
-        if (!c_.prof.on || !a_ || !b_) return;
+        if (!a_ || !b_) return;
         (void)hipEventRecord(b_, c_.stream);
         c_.prof.pending.push_back({id_, a_, b_, bytes_});
         if (c_.prof.pending.size() > 4096) c_.prof.flush();

@@ -37,8 +37,12 @@ struct BPLevel {
     long E = 0;
     const int *clus = nullptr;     // into the pyramid
     const int2 *bd = nullptr;      // into the pyramid
-    DevBuf<uint2> pix;             // (packed xy, cluster) sorted by (cluster, pixel id) [N]
+    DevBuf<uint32_t> pxy, pcl;     // pixels sorted by (cluster, pixel id): packed x|y<<16, cluster [N]
     DevBuf<int> pix_off;           // [V+1]
+    bool coarse = false;           // plane costs by per-superpixel segments (else by chunks)
+    int nseg_max = 0;              // coarse: upper bound of the segment count (N/512 + V)
+    DevBuf<int4> seg;              // coarse: (first pixel, end pixel, cluster, index in cluster)
+    DevBuf<int> sego;              // coarse: first segment of each superpixel; sego[V] = count
     DevBuf<int> off;               // CSR offsets by source [V+1]
     DevBuf<int> rev;               // reverse edge [E]
     DevBuf<float> c0, dpos;        // per edge: max(dc0, .01) * bdl and max(dc1, 1) [E]
@@ -72,7 +76,7 @@ struct BP {
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;
     int sweeps = 0;
-    bool fused = true;             // one cooperative launch per level, else one kernel per phase
+    bool fused = false;            // one cooperative launch per level, else two kernels per sweep
     DevBuf<uint8_t> sweep_desc;    // device copy of the schedule of the running launch
     bool mapped = false;
     std::vector<Sweep> sched;

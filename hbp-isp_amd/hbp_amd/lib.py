@@ -39,6 +39,7 @@ SIGNATURES = {
     "hbp_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_double)]),
     "hbp_profile_names": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "hbp_profile_only": (C.c_int, [_vp, C.c_char_p]),
     "hbp_cost_volume": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_volume_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_to_right": (C.c_int, [_vp, _vp]),
@@ -133,6 +134,9 @@ class Ctx:
     def profile_reset(self):
         _check(self.lib.hbp_profile_reset(self.h))
 
+    def profile_only(self, prefix=None):
+        _check(self.lib.hbp_profile_only(self.h, prefix.encode() if prefix else None))
+
     def profile_stats(self):
         n = self.lib.hbp_profile_names(self.h, None, 0)
         buf = C.create_string_buffer(max(n, 1))
@@ -153,7 +157,7 @@ class Ctx:
         return Pyramid(self, bgr, num_nb)
 
     # ---- (c)
-    def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1, fused=True):
+    def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1, fused=False):
         return BP(self, pyr, vol, alpha, tau_s, min_level, use_cost_vol, fused)
 
     # ---- (d)
@@ -238,7 +242,7 @@ class Pyramid:
 
 class BP:
     def __init__(self, ctx: Ctx, pyr: Pyramid, vol: Volume, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1,
-                 fused=True):
+                 fused=False):
         self.ctx, self.pyr, self.vol = ctx, pyr, vol
         h = C.c_void_p()
         _check(ctx.lib.hbp_bp_create(ctx.h, pyr.h, vol.h, alpha, tau_s, min_level, use_cost_vol, C.byref(h)))
