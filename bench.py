@@ -42,8 +42,6 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
-    ap.add_argument("--mode", choices=["split", "fused"], default="split",
-                    help="split: two kernels per BP sweep; fused: one cooperative launch per BP level")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stereo path runs one pair per GPU and this build has "
@@ -71,14 +69,13 @@ def main():
 
     info = {}
 
-    def one_pair(fused=args.mode == "fused"):
+    def one_pair():
         vol, pyr, bp = C.c_void_p(), C.c_void_p(), C.c_void_p()
         H._check(lib.hbp_cost_volume_dev(ctx.h, d_l, d_r, W, Hh, D, b"Census", C.byref(vol)))
         H._check(lib.hbp_pyramid_build_dev(ctx.h, d_l, W, Hh, 2, 4, 0.0, C.byref(pyr)))
         L = lib.hbp_pyramid_levels(pyr)
         ml = max(1, L - 1 - wl["bp_levels"]) if wl["bp_levels"] > 0 else 2
         H._check(lib.hbp_bp_create(ctx.h, pyr, vol, 1000.1, 0.6, ml, 1, C.byref(bp)))
-        H._check(lib.hbp_bp_set_fused(bp, int(fused)))
         info.update(levels=L, min_level=ml, sweeps=lib.hbp_bp_total_sweeps(bp))
         H._check(lib.hbp_bp_run(ctx.h, bp, None, None))
         lib.hbp_bp_free(ctx.h, bp)
@@ -101,7 +98,7 @@ def main():
         one_pair()
     ctx.sync()
     # timed region: HIP events around the dominant kernel's launches only
-    dom = "bp_level" if args.mode == "fused" else "plane_cost"
+    dom = "plane_cost"
     ctx.profile(True)
     ctx.profile_only(dom)
     ctx.profile_reset()
@@ -175,12 +172,14 @@ def main():
         "p99_ms": round(p99, 3),
         "config": {"workload": wl["name"], "width": W, "height": Hh, "max_disp": D, "levels": info["levels"],
                    "min_level": info["min_level"], "sweeps": info["sweeps"], "alpha": 1000.1,
-                   "tau_smooth": 0.6, "seed": args.seed, "mode": args.mode},
+                   "tau_smooth": 0.6, "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": round(avg_bytes),
                      "avg_launch_us": round(1e3 * avg_ms, 2), "launches": gd["launches"]},
         "kernels_one_pair": kernels,
+        "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
+                               for n, v in sorted(pair_stats.items()) if "_L" in n},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -26,14 +26,10 @@
 // (the 8 slots the next sweep reads), sweep messages [E][12] f32, pixels sorted by
 // (superpixel, pixel id) as packed x|y<<16 [N] and superpixel id [N], CSR offsets / reverse
 // ids / edge weights per edge, fixed-point sums [V][rrep][12] int64.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
 
 #include "hbp_internal.h"
-
-namespace cg = cooperative_groups;
 
 namespace hbp {
 
@@ -276,12 +272,6 @@ struct SweepArgs {
     uint32_t *s_ransac, *s_neighbor, *s_child;
 };
 
-// the schedule of one sweep as the device reads it
-struct SweepDesc {
-    int kind, next_kind;
-    float ndz, ndn;
-};
-
 constexpr int CH_R12 = 2, CH_R4 = 8;                              // 256-pixel steps per chunk
 constexpr int MAXC12 = 256 * CH_R12 / 2 + 2, MAXC4 = 256 * CH_R4 / 2 + 2;  // superpixels a chunk can touch
 constexpr size_t PC_LDS_BYTES =
@@ -358,19 +348,24 @@ __device__ void spatial_node(const SweepArgs &a, long id) {
     for (int p = 0; p < P; ++p) a.plane[id * K + 2 * P + p] = spatial_candidate(a, id, p);
 }
 
+// Plane-cost modes: the cost volume of the left view, of the right view (cost_to_right), or
+// the WTA indicator of StereoData(use_cost_vol = 0).
+enum CostMode { kCostLeft = 0, kCostRight = 1, kCostWta = 2 };
+
 // Entry (d, y, x) of the cost volume, 0 <= d < D, recomputed from the census words exactly as
 // the Hamming kernel stores it: left view min(popc(CL(x) ^ CR(max(x-d, 0))), 25); right view
 // (cost_to_right) the left entry at x' = min(x+d, W-1).  clx = CL(x) of the pixel's row.
-__device__ __forceinline__ float census_entry(const SweepArgs &a, const uint32_t *clrow, const uint32_t *crrow,
-                                              uint32_t clx, int x, int d) {
+template <int MODE>
+__device__ __forceinline__ float census_entry(int W, const uint32_t *clrow, const uint32_t *crrow, uint32_t clx,
+                                              int x, int d) {
     uint32_t l, r;
-    if (!a.right) {
+    if (MODE == kCostLeft) {
         int xr = x - d;
         l = clx;
         r = crrow[xr < 0 ? 0 : xr];
     } else {
         int xl = x + d;
-        xl = xl > a.W - 1 ? a.W - 1 : xl;
+        xl = xl > W - 1 ? W - 1 : xl;
         l = clrow[xl];
         r = crrow[xl - d];
     }
@@ -378,25 +373,49 @@ __device__ __forceinline__ float census_entry(const SweepArgs &a, const uint32_t
     return (float)(c < 25 ? c : 25);
 }
 
-// read_plane_cost (hbp_isp.cu:177-228) of one pixel and plane, times lambda
-__device__ __forceinline__ float pixel_plane_cost(const SweepArgs &a, float4 pl, int x, int y, float fx, float fy,
-                                                  const uint32_t *clrow, const uint32_t *crrow, uint32_t clx) {
+// read_plane_cost (hbp_isp.cu:177-228) of one pixel and plane, times lambda.  Branch-free: the
+// two entries are read at the disparity clamped into [0, D-2] and replaced by tau when the
+// plane's disparity is outside [0, D-1).  dw = WTA disparity of the pixel (kCostWta).
+template <int MODE>
+__device__ __forceinline__ float pixel_plane_cost(const SweepArgs &a, float4 pl, int x, float fx, float fy,
+                                                  const uint32_t *clrow, const uint32_t *crrow, uint32_t clx,
+                                                  float dw) {
     float df = pl.x * fx + pl.y * fy + pl.z;
     float cost;
-    if (a.use_cost_vol) {
-        int d = floor_to_int(df);
-        if (d < 0 || (long long)d + 1 >= a.D) {
-            cost = a.tau;
-        } else {
-            float v0 = census_entry(a, clrow, crrow, clx, x, d);
-            float v1 = census_entry(a, clrow, crrow, clx, x, d + 1);
-            cost = ((float)(d + 1) - df) * v0 + (df - (float)d) * v1;
-        }
+    if (MODE == kCostWta) {
+        cost = fabsf(dw - df) <= 1.f ? 0.f : 1.f;
     } else {
-        float Dw = a.dwta[(long)y * a.W + x];
-        cost = fabsf(Dw - df) <= 1.f ? 0.f : 1.f;
+        int d = floor_to_int(df);
+        bool in = d >= 0 && d < a.D - 1;
+        int dc = d < a.D - 2 ? d : a.D - 2;
+        dc = dc < 0 ? 0 : dc;
+        float v0 = census_entry<MODE>(a.W, clrow, crrow, clx, x, dc);
+        float v1 = census_entry<MODE>(a.W, clrow, crrow, clx, x, dc + 1);
+        float ci = ((float)(d + 1) - df) * v0 + (df - (float)d) * v1;
+        cost = in ? ci : a.tau;
     }
     return a.lambda * cost;
+}
+
+// per-pixel inputs of pixel_plane_cost
+struct PixelIn {
+    int x;
+    float fx, fy, dw;
+    const uint32_t *clrow, *crrow;
+    uint32_t clx;
+};
+template <int MODE>
+__device__ __forceinline__ PixelIn load_pixel(const SweepArgs &a, uint32_t q) {
+    PixelIn p;
+    p.x = unpack_x(q);
+    int y = unpack_y(q);
+    p.fx = (float)p.x;
+    p.fy = (float)y;
+    p.clrow = a.cl + (long)y * a.W;
+    p.crrow = a.cr + (long)y * a.W;
+    p.clx = MODE == kCostLeft ? p.clrow[p.x] : 0u;
+    p.dw = MODE == kCostWta ? a.dwta[(long)y * a.W + p.x] : 0.f;
+    return p;
 }
 
 // ----------------------------------------------------------------------------------------
@@ -404,9 +423,10 @@ __device__ __forceinline__ float pixel_plane_cost(const SweepArgs &a, float4 pl,
 // Lane l reads pixels first + l + 64 r; the planes are wave-uniform.  The per-lane int64 sums
 // are reduced across the wavefront and added to replica (segment mod rrep) of the
 // superpixel's sums.  On spatial sweeps lanes 0..P-1 resolve the candidates first and the
-// superpixel's first segment stores them.
+// superpixel's first segment stores them.  Per-pixel values are finite or NaN (the shift
+// keeps |v 2^s| far below 2^63), so the fixed-point term comes from the float's bits.
 // ----------------------------------------------------------------------------------------
-template <int NS>
+template <int NS, int MODE>
 __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
     const int lane = threadIdx.x & 63;
     const int4 sg = a.seg[w];
@@ -428,22 +448,20 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) acc[k] = 0;
     unsigned int nanm = 0u;
-    const int end = sg.y;
-#pragma unroll 4
-    for (int r = 0; r < SEG_R; ++r) {
-        const int j = sg.x + r * 64 + lane;
-        if (j < end) {
-            uint32_t q = a.pxy[j];
-            int x = unpack_x(q), y = unpack_y(q);
-            float fx = (float)x, fy = (float)y;
-            const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
-            const uint32_t clx = a.right ? 0u : clrow[x];
+    const int first = __builtin_amdgcn_readfirstlane(sg.x), end = __builtin_amdgcn_readfirstlane(sg.y);
+    const int steps = (end - first + 63) >> 6;
+#pragma unroll 2
+    for (int r = 0; r < steps; ++r) {
+        const int j = first + r * 64 + lane;
+        const bool valid = j < end;
+        const PixelIn px = load_pixel<MODE>(a, a.pxy[valid ? j : end - 1]);
 #pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                float v = pixel_plane_cost(a, pl[k], x, y, fx, fy, clrow, crrow, clx);
-                if (v != v) nanm |= 1u << (slot0 + k);
-                else acc[k] += to_fixed(v, a.shift);
-            }
+        for (int k = 0; k < NS; ++k) {
+            float v = pixel_plane_cost<MODE>(a, pl[k], px.x, px.fx, px.fy, px.clrow, px.crrow, px.clx, px.dw);
+            bool isn = v != v;
+            int64_t q = to_fixed_finite(isn ? 0.f : v, a.shift);
+            acc[k] += (valid && !isn) ? q : 0;
+            nanm |= (valid && isn) ? (1u << (slot0 + k)) : 0u;
         }
     }
 #pragma unroll
@@ -469,7 +487,7 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 // at most 128 R + 1 of them); the block then adds each counter to replica (chunk mod rrep)
 // of the global sum.
 // ----------------------------------------------------------------------------------------
-template <int NS, int R>
+template <int NS, int R, int MODE>
 __device__ void pc_chunk(const SweepArgs &a, long chunk, int slot0, unsigned long long *sacc, unsigned int *snan) {
     constexpr int CH = 256 * R;
     constexpr int MAXC = CH / 2 + 2;
@@ -491,16 +509,13 @@ __device__ void pc_chunk(const SweepArgs &a, long chunk, int slot0, unsigned lon
         for (int k = 0; k < NS; ++k) v[k] = 0;
         if (valid) {
             c = (int)a.pcl[j];
-            uint32_t q = a.pxy[j];
-            int x = unpack_x(q), y = unpack_y(q);
-            float fx = (float)x, fy = (float)y;
-            const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
-            const uint32_t clx = a.right ? 0u : clrow[x];
+            const PixelIn px = load_pixel<MODE>(a, a.pxy[j]);
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
-                float f = pixel_plane_cost(a, a.plane[(long)c * K + slot0 + k], x, y, fx, fy, clrow, crrow, clx);
+                float f = pixel_plane_cost<MODE>(a, a.plane[(long)c * K + slot0 + k], px.x, px.fx, px.fy, px.clrow,
+                                                 px.crrow, px.clx, px.dw);
                 if (f != f) nanm |= 1u << (slot0 + k);
-                else v[k] = to_fixed(f, a.shift);
+                else v[k] = to_fixed_finite(f, a.shift);
             }
         }
         // inclusive scan within runs of equal superpixel (runs are contiguous: pixels sorted)
@@ -529,9 +544,9 @@ __device__ void pc_chunk(const SweepArgs &a, long chunk, int slot0, unsigned lon
     __syncthreads();
     const int rep = (int)(chunk % a.rrep);
     for (int i = threadIdx.x; i < nclus * NS; i += 256) {
-        unsigned long long s = sacc[i];
+        unsigned long long sm = sacc[i];
         int lc = i / NS, k = i % NS;
-        if (s) atomicAdd(&a.acc[((long)(c_first + lc) * a.rrep + rep) * K + slot0 + k], s);
+        if (sm) atomicAdd(&a.acc[((long)(c_first + lc) * a.rrep + rep) * K + slot0 + k], sm);
     }
     for (int i = threadIdx.x; i < nclus; i += 256)
         if (snan[i]) atomicOr(&a.accnan[c_first + i], snan[i]);
@@ -733,18 +748,19 @@ __global__ __launch_bounds__(256) void node_kernel(SweepArgs a, int this_kind, i
 // Plane costs of slots [slot0, slot0+NS) and, for NS == P, the sweep's messages.  Coarse
 // levels: wavefronts [0, npc) take segments and the rest take 4 edges each.  Fine levels:
 // blocks [0, npc) take chunks and the waves of the other blocks take 4 edges each.
-template <int NS, bool COARSE>
+template <int NS, bool COARSE, int MODE>
 __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, int spatial, int npc) {
     extern __shared__ unsigned long long lds_pc[];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (COARSE) {
         const int w = blockIdx.x * 4 + wave;
-        if (w < npc) pc_segment<NS>(a, w, slot0, spatial != 0);
+        if (w < npc) pc_segment<NS, MODE>(a, w, slot0, spatial != 0);
         else if (NS == P) edge_wave(a, w - npc, spatial != 0);
     } else {
         constexpr int R = NS == P ? CH_R4 : CH_R12;
         constexpr int MAXC = 256 * R / 2 + 2;
-        if ((int)blockIdx.x < npc) pc_chunk<NS, R>(a, blockIdx.x, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
+        if ((int)blockIdx.x < npc)
+            pc_chunk<NS, R, MODE>(a, blockIdx.x, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
         else if (NS == P) edge_wave(a, (long)(blockIdx.x - npc) * 4 + wave, false);
     }
 }
@@ -752,50 +768,6 @@ __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, 
 __global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-// ----------------------------------------------------------------------------------------
-// all sweeps [s_begin, s_end) of one level in one cooperative launch: per sweep
-//   (fine levels, spatial sweeps: candidates | grid barrier)
-//   plane costs + messages | barrier | node updates (+ next candidates) | barrier
-// Work items are spread over the wavefronts / blocks in grid-stride order.  With first_costed,
-// the first sweep's candidate costs were computed by the level init and its plane-cost work is
-// skipped.
-// ----------------------------------------------------------------------------------------
-template <bool COARSE>
-__global__ __launch_bounds__(256) void bp_level_kernel(SweepArgs a, const SweepDesc *__restrict__ sw, int s_begin,
-                                                       int s_end, int first_costed, int npc) {
-    extern __shared__ unsigned long long lds_pc[];
-    auto grid = cg::this_grid();
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const long nwaves = (long)gridDim.x * 4;
-    const long nedge = (a.E + 3) / 4;
-    const long ngroups = ((long)a.V * 16 + 255) / 256 * 4;   // node waves
-    for (int i = s_begin; i < s_end; ++i) {
-        const SweepDesc d = sw[i];
-        const bool spatial = d.kind == kSpatial;
-        const int np = (first_costed && i == s_begin) ? 0 : npc;
-        if (COARSE) {
-            for (long w = (long)blockIdx.x * 4 + wave; w < np + nedge; w += nwaves) {
-                if (w < np) pc_segment<P>(a, (int)w, 2 * P, spatial);
-                else edge_wave(a, w - np, spatial);
-            }
-        } else {
-            if (spatial) {
-                for (long id = (long)blockIdx.x * 256 + threadIdx.x; id < a.V; id += nwaves * 64) spatial_node(a, id);
-                grid.sync();
-            }
-            for (long b = blockIdx.x; b < np; b += gridDim.x)
-                pc_chunk<P, CH_R4>(a, b, 2 * P, lds_pc, (unsigned int *)(lds_pc + MAXC4 * P));
-            for (long w = (long)blockIdx.x * 4 + wave; w < nedge; w += nwaves) edge_wave(a, w, false);
-        }
-        grid.sync();
-        for (long w = (long)blockIdx.x * 4 + wave; w < ngroups; w += nwaves) {
-            const long s = w * 4 + ((threadIdx.x & 63) >> 4);
-            if (s < a.V) node_group(a, s, d.kind, d.next_kind, d.ndz, d.ndn);
-        }
-        grid.sync();
-    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1087,26 +1059,39 @@ static double plane_cost_bytes(long N, int V, int NS) { return 12.0 * N + (doubl
 static double edge_bytes(long E, int V) { return 96.0 * E + (16.0 * K + 48) * V; }
 static double node_bytes(long E, int V) { return 84.0 * E + (48.0 * K + 16) * V; }
 
+template <int NS, bool COARSE>
+static void launch_cost_edge_mode(Ctx &c, const SweepArgs &a, int mode, long blocks, size_t lds, int slot0,
+                                  int spatial, int npc) {
+    if (mode == kCostLeft)
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostLeft>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
+                           spatial, npc);
+    else if (mode == kCostRight)
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostRight>), dim3(blocks), dim3(256), lds, c.stream, a,
+                           slot0, spatial, npc);
+    else
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostWta>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
+                           spatial, npc);
+}
+
 // plane costs of slots [slot0, slot0 + NS) (+ the sweep's messages when with_edges)
 template <int NS>
 static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l, int slot0, bool spatial,
                              bool costs, bool with_edges) {
     long nedge_w = with_edges ? div_up(a.E, 4) : 0;
     double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
+    const int mode = !a.use_cost_vol ? kCostWta : (a.right ? kCostRight : kCostLeft);
     ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
     if (B.coarse) {
         int npc = costs ? B.nseg_max : 0;
         long blocks = div_up((long)npc + nedge_w, 4);
         if (blocks == 0) return;
-        hipLaunchKernelGGL((cost_edge_kernel<NS, true>), dim3(blocks), dim3(256), 0, c.stream, a, slot0,
-                           spatial ? 1 : 0, npc);
+        launch_cost_edge_mode<NS, true>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
     } else {
         constexpr int R = NS == P ? CH_R4 : CH_R12;
         int npc = costs ? (int)div_up(a.N, 256 * R) : 0;
         long blocks = npc + div_up(nedge_w, 4);
         if (blocks == 0) return;
-        hipLaunchKernelGGL((cost_edge_kernel<NS, false>), dim3(blocks), dim3(256), PC_LDS_BYTES, c.stream, a, slot0,
-                           0, npc);
+        launch_cost_edge_mode<NS, false>(c, a, mode, blocks, PC_LDS_BYTES, slot0, 0, npc);
     }
 }
 
@@ -1144,77 +1129,10 @@ static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float
     HBP_CHECK(hipGetLastError());
 }
 
-int bp_level_coop_blocks(int num_cu) {
-    int per_cu = 0, per_cu2 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_level_kernel<true>, 256, 0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bp_level_kernel<false>, 256, PC_LDS_BYTES) !=
-            hipSuccess ||
-        per_cu2 < 1)
-        per_cu2 = 1;
-    per_cu = std::min(per_cu, per_cu2);
-    if (per_cu > 4) per_cu = 4;
-    return per_cu * num_cu;
-}
-
-// sweeps [i0, i1) of one level in one cooperative launch
-static void run_sweeps_fused(Ctx &c, BP &b, size_t i0, size_t i1) {
-    const Sweep &s0 = b.sched[i0];
-    int l = s0.level;
-    const BPLevel &B = b.lv[l];
-    SweepArgs a = sweep_args(b, l);
-    // the device copy of the level's schedule
-    std::vector<SweepDesc> h(i1 - i0);
-    for (size_t i = i0; i < i1; ++i) {
-        const Sweep &sw = b.sched[i];
-        SweepDesc d{sw.kind, kNone, 0.f, 0.f};
-        if (!sw.last && i + 1 < b.sched.size()) {
-            d.next_kind = b.sched[i + 1].kind;
-            d.ndz = b.sched[i + 1].dz;
-            d.ndn = b.sched[i + 1].dn;
-        }
-        h[i - i0] = d;
-    }
-    b.sweep_desc.alloc(c.pool, h.size() * sizeof(SweepDesc));
-    HBP_CHECK(hipMemcpyAsync(b.sweep_desc.p, h.data(), h.size() * sizeof(SweepDesc), hipMemcpyHostToDevice,
-                             c.stream));
-    if (s0.first) run_level_init(c, b, a, l);
-    int npc = B.coarse ? B.nseg_max : (int)div_up(a.N, 256 * CH_R4);
-    long work = B.coarse ? div_up((long)npc + div_up(a.E, 4), 4) : npc + div_up(a.E, 16);
-    work = std::max({work, (long)div_up((long)a.V * 16, 256), 1L});
-    int blocks = (int)std::min<long>(work, c.bp_coop_blocks);
-    const SweepDesc *dsw = (const SweepDesc *)b.sweep_desc.p;
-    int sb = 0, se = (int)(i1 - i0), fc = s0.first ? 1 : 0;
-    void *args[] = {(void *)&a, (void *)&dsw, (void *)&sb, (void *)&se, (void *)&fc, (void *)&npc};
-    double bytes = (double)(i1 - i0) * (plane_cost_bytes(a.N, a.V, P) + edge_bytes(a.E, a.V) + node_bytes(a.E, a.V));
-    ProfScope ps(c, "bp_level_L" + std::to_string(l), bytes);
-    if (B.coarse)
-        HBP_CHECK(hipLaunchCooperativeKernel((const void *)bp_level_kernel<true>, dim3(blocks), dim3(256), args, 0,
-                                             c.stream));
-    else
-        HBP_CHECK(hipLaunchCooperativeKernel((const void *)bp_level_kernel<false>, dim3(blocks), dim3(256), args,
-                                             PC_LDS_BYTES, c.stream));
-}
-
 int bp_step(Ctx &c, BP &b, int max_sweeps) {
     int done = 0;
     while (done < max_sweeps && b.next < b.sched.size()) {
         const Sweep &sw = b.sched[b.next];
-        if (b.fused) {
-            // the rest of this level (or as much of it as max_sweeps allows) in one launch
-            size_t i1 = b.next;
-            while (i1 < b.sched.size() && b.sched[i1].level == sw.level && (long)(i1 - b.next) < max_sweeps - done &&
-                   (i1 == b.next || !b.sched[i1].first))
-                ++i1;
-            run_sweeps_fused(c, b, b.next, i1);
-            int n = (int)(i1 - b.next);
-            b.cur = sw.level;
-            b.next = i1;
-            done += n;
-            b.sweeps += n;
-            continue;
-        }
         int next_kind = kNone;
         float ndz = 0.f, ndn = 0.f;
         if (!sw.last && b.next + 1 < b.sched.size()) {

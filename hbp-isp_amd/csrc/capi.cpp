@@ -118,7 +118,6 @@ int hbp_ctx_create(int device, hbp_ctx **out) {
     ctx->c.num_cu = prop.multiProcessorCount;
     HBP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
     ctx->c.coop_blocks = crtrees_coop_blocks(ctx->c.num_cu);
-    ctx->c.bp_coop_blocks = bp_level_coop_blocks(ctx->c.num_cu);
     *out = ctx;
     HBP_API_END
 }
@@ -440,13 +439,6 @@ int hbp_bp_step(hbp_ctx *ctx, hbp_bp *bp, int max_sweeps, int *done) {
     use_device(ctx);
     int d = bp_step(ctx->c, bp->b, max_sweeps);
     if (done) *done = d;
-    HBP_API_END
-}
-
-int hbp_bp_set_fused(hbp_bp *bp, int fused) {
-    HBP_API_BEGIN
-    HBP_REQUIRE(bp, "NULL argument");
-    bp->b.fused = fused != 0;
     HBP_API_END
 }
 

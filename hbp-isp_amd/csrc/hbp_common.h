@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -112,6 +113,17 @@ __host__ __device__ inline float int64_to_float_rne(int64_t q) {
 // int64_to_float_rne(sum) * 2^-s, or the quiet NaN when any term was NaN.
 int fixed_point_shift(float lambda, float tau, int use_cost_vol, long npix);
 __host__ __device__ inline int64_t to_fixed(float v, int s) { return (int64_t)rintf(ldexpf(v, s)); }
+// to_fixed for finite v with |v 2^s| < 2^63, from the bits of the rounded float: an integer
+// valued float is its 24-bit significand shifted by (exponent - 150)
+__host__ __device__ inline int64_t to_fixed_finite(float v, int s) {
+    float f = rintf(ldexpf(v, s));
+    uint32_t b = 0;
+    memcpy(&b, &f, 4);
+    int sh = (int)((b >> 23) & 0xffu) - 150;
+    uint64_t m = (uint64_t)((b & 0x7fffffu) | 0x800000u);
+    uint64_t u = sh >= 0 ? (m << (sh > 40 ? 40 : sh)) : (m >> (-sh > 31 ? 31 : -sh));
+    return (b >> 31) ? -(int64_t)u : (int64_t)u;
+}
 __host__ __device__ inline float from_fixed(int64_t q, bool nan, int s) {
     if (nan) return __int_as_float(0x7FC00000);
     return ldexpf(int64_to_float_rne(q), -s);
@@ -262,11 +274,9 @@ struct Ctx {
     Pool pool;
     int num_cu = 256;
     int coop_blocks = 0;     // co-resident blocks of the CRTrees cooperative kernel
-    int bp_coop_blocks = 0;  // co-resident blocks of the fused BP level kernel
     // scratch reused by every call on this context
     DevBuf<uint8_t> scratch;
     DevBuf<int> counters;  // small device counters (sizes, flags)
-    int *host_counters = nullptr;  // pinned mirror
     void *scratch_bytes(size_t bytes) {
         if (scratch.n < bytes) scratch.alloc(pool, bytes + (bytes >> 3));
         return scratch.p;

@@ -76,8 +76,6 @@ struct BP {
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;
     int sweeps = 0;
-    bool fused = false;            // one cooperative launch per level, else two kernels per sweep
-    DevBuf<uint8_t> sweep_desc;    // device copy of the schedule of the running launch
     bool mapped = false;
     std::vector<Sweep> sched;
     size_t next = 0;
@@ -97,7 +95,6 @@ int compact_labels(Ctx &c, int *d_label, int V);
 
 // bp.hip
 void bp_setup(Ctx &c, BP &b);
-int bp_level_coop_blocks(int num_cu);
 int bp_step(Ctx &c, BP &b, int max_sweeps);
 void bp_map(Ctx &c, BP &b);
 void bp_get_state(Ctx &c, BP &b, float *planes, float *cost, float *belief, float *msg);

@@ -5,11 +5,13 @@
 // the edge reduction (sort of (s', t') keys, run-length dedupe, boundary-length sums), the
 // per-parent sums in a fixed child order and the next level's 1-NN.  The host reads back two
 // counts per level (V', E') to size the next level.
+
 #include <hip/hip_cooperative_groups.h>
 
 #include <cmath>
 
 #include "hbp_internal.h"
+
 
 namespace cg = cooperative_groups;
 
@@ -88,15 +90,30 @@ __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int
 }
 
 // ----------------------------------------------------------------------------------------
-// CRTrees::get_clus (CRTrees.cu:315-354) as one kernel.  Phases are grid-stride loops over
-// the nodes separated by grid barriers; the loop flags live in device memory and every block
-// reads the same value after the barrier, so all blocks leave each loop together.
-//   A = nnc, B = nnn, M = max_d, C = clus; nnw is the working copy of nn.
+// CRTrees::get_clus (CRTrees.cu:315-354).  The loop is the reference's literal iteration
+//   A = nnc, B = nnn, M = max_d, C = clus; nnw is the working copy of nn;
 // re_link reads a snapshot of A (writes go to B), the double-buffered form of the in-place
-// kernel; the oracle checks both forms give the same labels.
+// kernel (the oracle checks both forms give the same labels).  Three executions of it:
+//  * levels up to kCrtSingleMax nodes: one 1024-thread block, phases separated by
+//    __syncthreads;
+//  * larger levels: one kernel per phase, launched in a fixed schedule of kCrtRounds rounds of
+//    kCrtJumps pointer-jumping steps and kCrtMaxJumps max-jumping steps.  The loop flags live
+//    in device memory; a kernel whose loop has already ended returns at once, and the
+//    ping-pong buffer of jumping step j is fixed by the parity of j, so the host never waits
+//    inside the loop.  A loop that needs more steps than scheduled sets the overflow word;
+//  * then (rare) one cooperative kernel with grid barriers recomputes the level with the full
+//    bounds (64 rounds, 64 jumps).
 // ----------------------------------------------------------------------------------------
 struct CRTBuf {
     int *nnw, *A, *B, *M, *C, *mask, *maskc, *indeg, *flags, *status;
+};
+constexpr int kCrtSingleMax = 40000;
+constexpr int kCrtRounds = 5, kCrtJumps = 8, kCrtMaxJumps = 8;
+struct CRTFlags {               // zeroed before a scheduled run
+    int more[kCrtRounds + 1];   // round r continues (some node has indegree > 1)
+    int jl[kCrtRounds][kCrtJumps];
+    int jm[kCrtMaxJumps];
+    int overflow;               // the loop needed more steps than scheduled
 };
 
 template <bool GRID>
@@ -110,16 +127,59 @@ __device__ __forceinline__ void crt_sync() {
 __device__ __forceinline__ void flag_or(int *flags, int slot, int local) {
     if (__syncthreads_or(local) && threadIdx.x == 0) atomicOr(&flags[slot], 1);
 }
-__device__ __forceinline__ int flag_read(int *flags, int slot) {
+__device__ __forceinline__ int flag_read(const int *flags, int slot) {
     return __hip_atomic_load(&flags[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void flag_clear(int *flags, int slot) {
     __hip_atomic_store(&flags[slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---- the phases (every execution form runs these bodies over its share of the nodes)
+__device__ __forceinline__ void crt_delink(const CRTBuf &s, long i) {     // CRTrees.cu:58-74, 304-311
+    int m = s.mask[i];
+    s.maskc[i] = m;
+    int dl = s.indeg[i] > 1;
+    s.mask[i] = dl;
+    if (dl) s.A[i] = (int)i;
+}
+// pointer_jumping (CRTrees.cu:314-333), one step from p to q; returns the continue condition
+__device__ __forceinline__ int crt_jump(const CRTBuf &s, const int *p, int *q, long i) {
+    if (!s.maskc[i]) return 0;
+    int a = p[i], b = p[a];
+    q[i] = b;
+    return (!s.mask[a] && s.mask[b]) ? 1 : 0;
+}
+__device__ __forceinline__ void crt_relink(const CRTBuf &s, long i) {     // CRTrees.cu:178-201
+    int v = s.A[i];
+    if (s.mask[i]) {
+        int a = s.nnw[i];
+        v = a;
+        if (!s.mask[a]) v = s.A[a];
+    } else if (!s.maskc[i]) {
+        int a = s.A[i];
+        if (s.maskc[a]) v = s.A[a];
+    }
+    s.B[i] = v;
+}
+__device__ __forceinline__ void crt_relink_all(const CRTBuf &s, long i) { // CRTrees.cu:203-225
+    int a = s.B[i];
+    if (s.mask[i]) s.nnw[i] = a;
+    else if (s.mask[a]) s.nnw[i] = s.B[a];
+}
+// pointer_jumping_max (CRTrees.cu:274-298), one step; returns whether the label changed
+__device__ __forceinline__ int crt_max_jump(const int *p, int *q, const int *mx, int *cl, long i) {
+    int a = p[i];
+    int m = mx[i];
+    int c = max(m, mx[a]);
+    cl[i] = c;
+    q[i] = p[a];
+    return c != m ? 1 : 0;
+}
+
+// ---- one kernel for the whole loop (single block, or cooperative grid)
 template <bool GRID>
-__global__ __launch_bounds__(256) void crtrees_kernel(const int *__restrict__ nn0, int V, CRTBuf s,
-                                                      int *__restrict__ label) {
+__global__ __launch_bounds__(1024) void crtrees_kernel(const int *__restrict__ nn0, int V, CRTBuf s,
+                                                       int *__restrict__ label) {
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nth = (long)gridDim.x * blockDim.x;
     int *flags = s.flags;
@@ -154,27 +214,15 @@ __global__ __launch_bounds__(256) void crtrees_kernel(const int *__restrict__ nn
             if (tid == 0) *s.status = 1;
             break;
         }
-        // de_link_and_compute_mask (CRTrees.cu:58-74, 304-311)
-        for (long i = tid; i < V; i += nth) {
-            int m = s.mask[i];
-            s.maskc[i] = m;
-            int dl = s.indeg[i] > 1;
-            s.mask[i] = dl;
-            if (dl) s.A[i] = (int)i;
-        }
+        for (long i = tid; i < V; i += nth) crt_delink(s, i);
         crt_sync<GRID>();
-        // pointer_jumping (CRTrees.cu:314-333): ping-pong between A and B; the caller keeps
-        // reading buffer A, which holds the result of the last even iteration.
+        // ping-pong between A and B; the caller keeps reading buffer A, which holds the
+        // result of the last even step
         {
             int *p = s.A, *q = s.B;
             for (int it = 0;; ++it) {
                 int jl = 0;
-                for (long i = tid; i < V; i += nth) {
-                    if (!s.maskc[i]) continue;
-                    int a = p[i], b = p[a];
-                    q[i] = b;
-                    if (!s.mask[a] && s.mask[b]) jl = 1;
-                }
+                for (long i = tid; i < V; i += nth) jl |= crt_jump(s, p, q, i);
                 flag_or(flags, fi, jl);
                 if (tid == 0) flag_clear(flags, (fi + 1) & 3);
                 crt_sync<GRID>();
@@ -190,44 +238,19 @@ __global__ __launch_bounds__(256) void crtrees_kernel(const int *__restrict__ nn
                 }
             }
         }
-        // re_link (CRTrees.cu:178-201), reading A, writing B
-        for (long i = tid; i < V; i += nth) {
-            int v = s.A[i];
-            if (s.mask[i]) {
-                int a = s.nnw[i];
-                v = a;
-                if (!s.mask[a]) v = s.A[a];
-            } else if (!s.maskc[i]) {
-                int a = s.A[i];
-                if (s.maskc[a]) v = s.A[a];
-            }
-            s.B[i] = v;
-        }
+        for (long i = tid; i < V; i += nth) crt_relink(s, i);
         crt_sync<GRID>();
-        // re_link_all (CRTrees.cu:203-225)
-        for (long i = tid; i < V; i += nth) {
-            int a = s.B[i];
-            if (s.mask[i]) s.nnw[i] = a;
-            else if (s.mask[a]) s.nnw[i] = s.B[a];
-        }
+        for (long i = tid; i < V; i += nth) crt_relink_all(s, i);
         crt_sync<GRID>();
         for (long i = tid; i < V; i += nth) s.A[i] = s.nnw[i];   // CRTrees.cu:423
         crt_sync<GRID>();
     }
-    // pointer_jumping_max (CRTrees.cu:274-298)
     for (long i = tid; i < V; i += nth) s.M[i] = s.A[i];
     crt_sync<GRID>();
     int *p = s.A, *q = s.B, *mx = s.M, *cl = s.C;
     for (int it = 0;; ++it) {
         int jl = 0;
-        for (long i = tid; i < V; i += nth) {
-            int a = p[i];
-            int m = mx[i];
-            int c = max(m, mx[a]);
-            cl[i] = c;
-            q[i] = p[a];
-            if (c != m) jl = 1;
-        }
+        for (long i = tid; i < V; i += nth) jl |= crt_max_jump(p, q, mx, cl, i);
         flag_or(flags, fi, jl);
         if (tid == 0) flag_clear(flags, (fi + 1) & 3);
         crt_sync<GRID>();
@@ -255,6 +278,86 @@ int crtrees_coop_blocks(int num_cu) {
         per_cu = 1;
     if (per_cu > 4) per_cu = 4;
     return per_cu * num_cu;
+}
+
+// ---- the scheduled form: one kernel per phase; every kernel first reads its gate
+#define CRT_NODES(i) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (long)gridDim.x * blockDim.x)
+
+__global__ void crt_init_kernel(const int *__restrict__ nn0, int V, CRTBuf s) {
+    CRT_NODES(i) {
+        int v = nn0[i];
+        s.nnw[i] = v;
+        s.A[i] = v;
+        s.mask[i] = 1;
+        s.indeg[i] = 0;
+    }
+}
+// round r runs when round r-1 continued
+__device__ __forceinline__ bool crt_round_open(const CRTFlags *f, int r) {
+    return r == 0 || flag_read(f->more, r - 1) != 0;
+}
+__global__ void crt_count_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
+    if (!crt_round_open(f, r)) return;
+    CRT_NODES(i) if (s.mask[i]) atomicAdd(&s.indeg[s.A[i]], 1);
+}
+__global__ void crt_check_kernel(int V, CRTBuf s, CRTFlags *f, int r) {
+    if (!crt_round_open(f, r)) return;
+    int loc = 0;
+    CRT_NODES(i) if (s.mask[i] && s.indeg[i] > 1) loc = 1;
+    if (__syncthreads_or(loc) && threadIdx.x == 0) {
+        atomicOr(&f->more[r], 1);
+        if (r == kCrtRounds) atomicOr(&f->overflow, 1);
+    }
+}
+__global__ void crt_delink_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
+    if (!flag_read(f->more, r)) return;
+    CRT_NODES(i) crt_delink(s, i);
+}
+__global__ void crt_jump_kernel(int V, CRTBuf s, CRTFlags *f, int r, int j) {
+    if (!flag_read(f->more, r) || (j > 0 && !flag_read(f->jl[r], j - 1))) return;
+    const int *p = (j & 1) ? s.B : s.A;
+    int *q = (j & 1) ? s.A : s.B;
+    int jl = 0;
+    CRT_NODES(i) jl |= crt_jump(s, p, q, i);
+    if (__syncthreads_or(jl) && threadIdx.x == 0) {
+        atomicOr(&f->jl[r][j], 1);
+        if (j == kCrtJumps - 1) atomicOr(&f->overflow, 1);
+    }
+}
+__global__ void crt_relink_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
+    if (!flag_read(f->more, r)) return;
+    CRT_NODES(i) crt_relink(s, i);
+}
+// re_link_all, then the copy A = nnw (CRTrees.cu:423) and the next round's indegree reset
+__global__ void crt_relink_all_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
+    if (!flag_read(f->more, r)) return;
+    CRT_NODES(i) {
+        crt_relink_all(s, i);
+        s.A[i] = s.nnw[i];
+        s.indeg[i] = 0;
+    }
+}
+__global__ void crt_max_init_kernel(int V, CRTBuf s, const CRTFlags *f) {
+    if (flag_read(&f->overflow, 0)) return;
+    CRT_NODES(i) s.M[i] = s.A[i];
+}
+__global__ void crt_max_kernel(int V, CRTBuf s, CRTFlags *f, int j) {
+    if (flag_read(&f->overflow, 0) || (j > 0 && !flag_read(f->jm, j - 1))) return;
+    const int *p = (j & 1) ? s.B : s.A;
+    int *q = (j & 1) ? s.A : s.B;
+    const int *mx = (j & 1) ? s.C : s.M;
+    int *cl = (j & 1) ? s.M : s.C;
+    int jl = 0;
+    CRT_NODES(i) jl |= crt_max_jump(p, q, mx, cl, i);
+    if (__syncthreads_or(jl) && threadIdx.x == 0) {
+        atomicOr(&f->jm[j], 1);
+        if (j == kCrtMaxJumps - 1) atomicOr(&f->overflow, 1);
+    }
+}
+// the last step changes no label, so M and C hold the same labels
+__global__ void crt_final_kernel(int V, const CRTBuf s, const CRTFlags *f, int *__restrict__ label) {
+    if (flag_read(&f->overflow, 0)) return;
+    CRT_NODES(i) label[i] = s.M[i];
 }
 
 // CRTrees::compact_clus_label (CRTrees.cu:387-402): rank of each label among the labels used.
@@ -407,39 +510,96 @@ static void upload_srgb_lut(Ctx &c) {
     HBP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_srgb_lut), lut, sizeof(lut), 0, hipMemcpyHostToDevice, c.stream));
 }
 
-void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label) {
-    DevBuf<int> buf(c.pool, (size_t)V * 8 + 8);
+// device memory of one CRTrees run: 8 node arrays, the loop flags / status words of the
+// one-kernel form and the flags of the scheduled form
+struct CRTRun {
+    DevBuf<int> buf;
     CRTBuf s;
-    s.nnw = buf.p;
-    s.A = buf.p + (size_t)V;
-    s.B = buf.p + 2 * (size_t)V;
-    s.M = buf.p + 3 * (size_t)V;
-    s.C = buf.p + 4 * (size_t)V;
-    s.mask = buf.p + 5 * (size_t)V;
-    s.maskc = buf.p + 6 * (size_t)V;
-    s.indeg = buf.p + 7 * (size_t)V;
-    s.flags = buf.p + 8 * (size_t)V;
-    s.status = buf.p + 8 * (size_t)V + 4;
-    HBP_CHECK(hipMemsetAsync(s.status, 0, sizeof(int), c.stream));
+    CRTFlags *f = nullptr;
+    int V = 0;
+};
+static void crt_setup(Ctx &c, CRTRun &r, int V) {
+    const size_t words = 8 + (sizeof(CRTFlags) + 3) / 4;
+    r.V = V;
+    r.buf.alloc(c.pool, (size_t)V * 8 + words);
+    CRTBuf &s = r.s;
+    s.nnw = r.buf.p;
+    s.A = r.buf.p + (size_t)V;
+    s.B = r.buf.p + 2 * (size_t)V;
+    s.M = r.buf.p + 3 * (size_t)V;
+    s.C = r.buf.p + 4 * (size_t)V;
+    s.mask = r.buf.p + 5 * (size_t)V;
+    s.maskc = r.buf.p + 6 * (size_t)V;
+    s.indeg = r.buf.p + 7 * (size_t)V;
+    s.flags = r.buf.p + 8 * (size_t)V;
+    s.status = s.flags + 4;
+    r.f = (CRTFlags *)(s.flags + 8);
+    HBP_CHECK(hipMemsetAsync(s.flags, 0, sizeof(int) * words, c.stream));
+}
+// enqueues the run without waiting: one block for small levels, else the scheduled kernels
+static void crt_enqueue(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
+    const int V = r.V;
+    const CRTBuf &s = r.s;
     ProfScope ps(c, "crtrees", 16.0 * V);
-    if (V <= 8192) {
-        hipLaunchKernelGGL(crtrees_kernel<false>, dim3(1), dim3(256), 0, c.stream, d_nn, V, s, d_label);
+    if (V <= kCrtSingleMax) {
+        hipLaunchKernelGGL(crtrees_kernel<false>, dim3(1), dim3(1024), 0, c.stream, d_nn, V, s, d_label);
         HBP_CHECK(hipGetLastError());
-    } else {
-        int blocks = c.coop_blocks;
-        long need = ((long)V + 255) / 256;
-        if (need < blocks) blocks = (int)need;
-        void *args[] = {(void *)&d_nn, (void *)&V, (void *)&s, (void *)&d_label};
-        HBP_CHECK(hipLaunchCooperativeKernel((const void *)crtrees_kernel<true>, dim3(blocks), dim3(256), args, 0,
-                                             c.stream));
+        return;
     }
-    int status = 0;
-    HBP_CHECK(hipMemcpyAsync(&status, s.status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipStreamSynchronize(c.stream));
+    const dim3 g(grid_for(c, V)), b(256);
+    CRTFlags *f = r.f;
+    hipLaunchKernelGGL(crt_init_kernel, g, b, 0, c.stream, d_nn, V, s);
+    for (int rd = 0; rd <= kCrtRounds; ++rd) {
+        hipLaunchKernelGGL(crt_count_kernel, g, b, 0, c.stream, V, s, f, rd);
+        hipLaunchKernelGGL(crt_check_kernel, g, b, 0, c.stream, V, s, f, rd);
+        if (rd == kCrtRounds) break;
+        hipLaunchKernelGGL(crt_delink_kernel, g, b, 0, c.stream, V, s, f, rd);
+        for (int j = 0; j < kCrtJumps; ++j)
+            hipLaunchKernelGGL(crt_jump_kernel, g, b, 0, c.stream, V, s, f, rd, j);
+        hipLaunchKernelGGL(crt_relink_kernel, g, b, 0, c.stream, V, s, f, rd);
+        hipLaunchKernelGGL(crt_relink_all_kernel, g, b, 0, c.stream, V, s, f, rd);
+    }
+    hipLaunchKernelGGL(crt_max_init_kernel, g, b, 0, c.stream, V, s, f);
+    for (int j = 0; j < kCrtMaxJumps; ++j) hipLaunchKernelGGL(crt_max_kernel, g, b, 0, c.stream, V, s, f, j);
+    hipLaunchKernelGGL(crt_final_kernel, g, b, 0, c.stream, V, s, f, d_label);
+    HBP_CHECK(hipGetLastError());
+}
+// the one-kernel form over a cooperative grid (a scheduled run that overflowed)
+static void crt_cooperative(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
+    CRTBuf s = r.s;
+    int V = r.V;
+    HBP_CHECK(hipMemsetAsync(s.flags, 0, sizeof(int) * 8, c.stream));
+    ProfScope ps(c, "crtrees_cooperative", 16.0 * V);
+    int blocks = c.coop_blocks;
+    long need = ((long)V + 255) / 256;
+    if (need < blocks) blocks = (int)need;
+    void *args[] = {(void *)&d_nn, (void *)&V, (void *)&s, (void *)&d_label};
+    HBP_CHECK(hipLaunchCooperativeKernel((const void *)crtrees_kernel<true>, dim3(blocks), dim3(256), args, 0,
+                                         c.stream));
+}
+static void crt_require(int status) {
     HBP_REQUIRE(status == 0, "CRTrees iteration did not converge within its bounds");
 }
 
-int compact_labels(Ctx &c, int *d_label, int V) {
+void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label) {
+    CRTRun r;
+    crt_setup(c, r, V);
+    crt_enqueue(c, r, d_nn, d_label);
+    int w[2] = {0, 0};
+    HBP_CHECK(hipMemcpyAsync(&w[0], r.s.status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipMemcpyAsync(&w[1], &r.f->overflow, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    if (w[1]) {
+        crt_cooperative(c, r, d_nn, d_label);
+        HBP_CHECK(hipMemcpyAsync(&w[0], r.s.status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HBP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    crt_require(w[0]);
+}
+
+// compaction; also copies n_extra device words to the host in the same wait
+static int compact_labels_and_read(Ctx &c, int *d_label, int V, const int *const *extra, int *extra_host,
+                                   int n_extra) {
     DevBuf<int> pred(c.pool, (size_t)V + 1), pos(c.pool, (size_t)V + 1);
     HBP_CHECK(hipMemsetAsync(pred.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
     hipLaunchKernelGGL(mark_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pred.p);
@@ -447,7 +607,26 @@ int compact_labels(Ctx &c, int *d_label, int V) {
     hipLaunchKernelGGL(remap_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pos.p);
     int Vn = 0;
     HBP_CHECK(hipMemcpyAsync(&Vn, pos.p + V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    for (int i = 0; i < n_extra; ++i)
+        HBP_CHECK(hipMemcpyAsync(&extra_host[i], extra[i], sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HBP_CHECK(hipStreamSynchronize(c.stream));
+    return Vn;
+}
+int compact_labels(Ctx &c, int *d_label, int V) { return compact_labels_and_read(c, d_label, V, nullptr, nullptr, 0); }
+
+// labels + compaction of one pyramid level with one host wait (two after an overflow)
+static int crtrees_level(Ctx &c, const int *d_nn, int V, int *d_label) {
+    CRTRun r;
+    crt_setup(c, r, V);
+    crt_enqueue(c, r, d_nn, d_label);
+    const int *extra[2] = {r.s.status, &r.f->overflow};
+    int w[2] = {0, 0};
+    int Vn = compact_labels_and_read(c, d_label, V, extra, w, 2);
+    if (w[1]) {
+        crt_cooperative(c, r, d_nn, d_label);
+        Vn = compact_labels_and_read(c, d_label, V, extra, w, 1);
+    }
+    crt_require(w[0]);
     return Vn;
 }
 
@@ -486,8 +665,7 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
     long vo = 0, eo = 0;
     while (V > 1) {
         int *clus = p.clus.p + vo;
-        crtrees_labels(c, nn.p, V, clus);
-        int Vn = compact_labels(c, clus, V);
+        int Vn = crtrees_level(c, nn.p, V, clus);
         p.V.push_back(V);
         p.E.push_back((int)E);
         p.voff.push_back(vo);

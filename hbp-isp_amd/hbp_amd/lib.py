@@ -63,7 +63,6 @@ SIGNATURES = {
     "hbp_crtrees_compact": (C.c_int, [_vp, _vp, C.c_int, _ip]),
     "hbp_bp_create": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int, _pp]),
     "hbp_bp_step": (C.c_int, [_vp, _vp, C.c_int, _ip]),
-    "hbp_bp_set_fused": (C.c_int, [_vp, C.c_int]),
     "hbp_bp_total_sweeps": (C.c_int, [_vp]),
     "hbp_bp_level": (C.c_int, [_vp]),
     "hbp_bp_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
@@ -157,8 +156,8 @@ class Ctx:
         return Pyramid(self, bgr, num_nb)
 
     # ---- (c)
-    def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1, fused=False):
-        return BP(self, pyr, vol, alpha, tau_s, min_level, use_cost_vol, fused)
+    def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
+        return BP(self, pyr, vol, alpha, tau_s, min_level, use_cost_vol)
 
     # ---- (d)
     def lr_check_fill(self, dispL, dispR, planeL):
@@ -241,13 +240,11 @@ class Pyramid:
 
 
 class BP:
-    def __init__(self, ctx: Ctx, pyr: Pyramid, vol: Volume, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1,
-                 fused=False):
+    def __init__(self, ctx: Ctx, pyr: Pyramid, vol: Volume, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
         self.ctx, self.pyr, self.vol = ctx, pyr, vol
         h = C.c_void_p()
         _check(ctx.lib.hbp_bp_create(ctx.h, pyr.h, vol.h, alpha, tau_s, min_level, use_cost_vol, C.byref(h)))
         self.h = h
-        _check(ctx.lib.hbp_bp_set_fused(h, int(fused)))
         self.W, self.H = pyr.W, pyr.H
 
     def free(self):

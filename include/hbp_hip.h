@@ -101,10 +101,6 @@ int hbp_bp_create(hbp_ctx *ctx, const hbp_pyramid *pyr, const hbp_volume *vol, f
                   float tau_smooth, int min_level, int use_cost_vol, hbp_bp **out);
 /* run up to max_sweeps sweeps of HBP_ISP::run_ms (hbp_isp.cu:1313-1346); *done = sweeps run */
 int hbp_bp_step(hbp_ctx *ctx, hbp_bp *bp, int max_sweeps, int *done);
-/* execution form of the sweeps (no counterpart in the reference): 0 (default) launches two
- * kernels per sweep (plane costs + messages, node update); 1 runs all sweeps of a level in one
- * cooperative launch with grid barriers between the phases.  Same results. */
-int hbp_bp_set_fused(hbp_bp *bp, int fused);
 /* total sweeps the schedule holds, the level being worked on (min_level-1 when finished) */
 int hbp_bp_total_sweeps(const hbp_bp *bp);
 int hbp_bp_level(const hbp_bp *bp);

@@ -118,9 +118,19 @@ def test_crtrees_device_api_matches_literal_model(ctx):
     oracle's literal model, on random functional graphs small (one block) and large (grid)."""
     hip = _hip()
     rng = np.random.default_rng(5)
-    for V in (7, 300, 5000, 20000, 200000):
-        nn = rng.integers(0, V, V).astype(np.int32)
-        say(f"crtrees V={V}")
+    cases = [(V, "random") for V in (7, 300, 5000, 20000, 200000)]
+    # above the one-block size: near-grid graphs (the scheduled kernels) and a chain (its
+    # pointer jumping outgrows the schedule: the cooperative kernel redoes it)
+    cases += [(60000, "near"), (150000, "near"), (70000, "chain")]
+    for V, kind in cases:
+        if kind == "random":
+            nn = rng.integers(0, V, V).astype(np.int32)
+        elif kind == "near":
+            nn = np.clip(np.arange(V) + rng.integers(-3, 4, V), 0, V - 1).astype(np.int32)
+        else:
+            nn = np.arange(1, V + 1, dtype=np.int32)
+            nn[-1] = V - 2
+        say(f"crtrees V={V} {kind}")
         ref = np.zeros(V, np.int32)
         O.lib.orc_crtrees_literal(nn, V, 0, ref)
         d_nn, d_cl = C.c_void_p(), C.c_void_p()
@@ -152,17 +162,16 @@ def _bp_pair(seed, W, H, D, planes=(3, 8)):
     return left, right
 
 
-@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2)])
-def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml, fused):
-    say(f"bp state seed={seed} {W}x{H} D={D} min_level={ml} fused={fused}")
+def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml):
+    say(f"bp state seed={seed} {W}x{H} D={D} min_level={ml}")
     left, right = _bp_pair(seed, W, H, D)
     vo, wo = O.cost_volume(left, right, D)
     po = O.Pyramid(left)
     bo = O.BP(po, vo, wo, D, min_level=ml)
     vg = ctx.cost_volume(left, right, D)
     pg = ctx.pyramid(left)
-    bg = ctx.bp(pg, vg, min_level=ml, fused=fused)
+    bg = ctx.bp(pg, vg, min_level=ml)
     assert bg.total_sweeps == O.lib.orc_bp_total_sweeps(po.levels, ml, D)
     for sweep in range(12):
         assert bo.step(1) == 1 and bg.step(1) == 1
@@ -174,11 +183,10 @@ def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml, fused):
         assert same_bits(so["msg"], sg["msg"]), f"messages differ after sweep {sweep}"
 
 
-@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2),
                                            (8, 160, 120, 48, 3)])
-def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml, fused):
-    say(f"bp full seed={seed} {W}x{H} D={D} min_level={ml} fused={fused}")
+def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
+    say(f"bp full seed={seed} {W}x{H} D={D} min_level={ml}")
     left, right = _bp_pair(seed, W, H, D)
     vo, wo = O.cost_volume(left, right, D)
     po = O.Pyramid(left)
@@ -187,7 +195,7 @@ def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml, fused):
     do, plo = bo.map()
     vg = ctx.cost_volume(left, right, D)
     pg = ctx.pyramid(left)
-    bg = ctx.bp(pg, vg, min_level=ml, fused=fused)
+    bg = ctx.bp(pg, vg, min_level=ml)
     dg, plg = bg.run()
     assert same_bits(do, dg), f"{mismatch(do, dg).mean():.4%} of disparities differ"
     assert same_bits(plo, plg)
@@ -224,25 +232,37 @@ def test_plain_path_cfg1_bit_exact(ctx):
     assert same_bits(plo, plg)
 
 
-def test_bp_is_deterministic_and_fused_equals_split(ctx):
+def test_bp_is_deterministic(ctx):
     left, right = _bp_pair(4, 120, 90, 32)
     vg = ctx.cost_volume(left, right, 32)
     pg = ctx.pyramid(left)
     d1, p1 = ctx.bp(pg, vg, min_level=2).run()
     d2, p2 = ctx.bp(pg, vg, min_level=2).run()
-    d3, p3 = ctx.bp(pg, vg, min_level=2, fused=False).run()
     assert same_bits(d1, d2) and same_bits(p1, p2)
-    assert same_bits(d1, d3) and same_bits(p1, p3)
 
 
-def test_large_pair_fused_equals_split(ctx):
-    say("large pair fused vs split")
+def test_large_pair_bp_state_bit_exact(ctx):
+    """1472x984, D=256 (the benchmark pair's shape): BP state after the 530 first sweeps (all of
+    level L-2 and the init of level L-3), and a full run twice (repeatable)."""
+    say("large pair bp state")
     left, right, _ = synth.make_pair(11, 1472, 984, 256, (20, 40))
+    vo, wo = O.cost_volume(left, right, 256)
+    po = O.Pyramid(left)
+    ml = max(1, po.levels - 5)
+    bo = O.BP(po, vo, wo, 256, min_level=ml)
     vg = ctx.cost_volume(left, right, 256)
     pg = ctx.pyramid(left)
-    ml = max(1, pg.levels - 5)
+    bg = ctx.bp(pg, vg, min_level=ml)
+    n = 530
+    assert bo.step(n) == n and bg.step(n) == n
+    so, sg = bo.state(), bg.state()
+    say(f"  level {so['level']} V={so['V']}")
+    assert so["level"] == sg["level"]
+    for key in ("cost", "belief", "msg"):
+        assert same_bits(so[key], sg[key]), key
+    assert same_bits(so["planes"][:8], sg["planes"][:8])
     d1, p1 = ctx.bp(pg, vg, min_level=ml).run()
-    d2, p2 = ctx.bp(pg, vg, min_level=ml, fused=False).run()
+    d2, p2 = ctx.bp(pg, vg, min_level=ml).run()
     assert same_bits(d1, d2) and same_bits(p1, p2)
 
 
