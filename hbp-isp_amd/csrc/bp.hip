@@ -420,60 +420,160 @@ __device__ __forceinline__ PixelIn load_pixel(const SweepArgs &a, uint32_t q) {
 
 // ----------------------------------------------------------------------------------------
 // plane cost, coarse levels: one wavefront per segment (<= SEG pixels of one superpixel).
-// Lane l reads pixels first + l + 64 r; the planes are wave-uniform.  The per-lane int64 sums
-// are reduced across the wavefront and added to replica (segment mod rrep) of the
-// superpixel's sums.  On spatial sweeps lanes 0..P-1 resolve the candidates first and the
-// superpixel's first segment stores them.  Per-pixel values are finite or NaN (the shift
-// keeps |v 2^s| far below 2^63), so the fixed-point term comes from the float's bits.
+// Lane l reads pixels first + l + 64 r; the planes are wave-uniform.  A batch of B steps first
+// issues the B packed positions, then every census word the B x 4 plane costs need (at the
+// disparity clamped into [0, D-2]), pins them with an empty asm so that no load is deferred
+// into a branch, and only then computes: two memory round trips per batch.  The per-lane
+// int64 sums are reduced across the wavefront and added to replica (segment mod rrep) of the
+// superpixel's sums.  Per-pixel values are finite or NaN (the shift keeps |v 2^s| far below
+// 2^63), so the fixed-point term comes from the float's bits.
 // ----------------------------------------------------------------------------------------
+__device__ __forceinline__ void pin(uint32_t &v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(float &v) { asm volatile("" : "+v"(v)); }
+
+template <int MODE, int B>
+__device__ __forceinline__ void pc_batch(const SweepArgs &a, const float4 (&pl)[P], int slot0, int first, int end,
+                                         int r0, int64_t (&acc)[P], unsigned int &nanm) {
+    const int lane = threadIdx.x & 63;
+    const float dmax = (float)(a.D > 2 ? a.D - 2 : 0);
+    uint32_t q[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const int j = first + (r0 + r) * 64 + lane;
+        q[r] = a.pxy[j < end ? j : end - 1];
+    }
+    uint32_t u0[B][P], u1[B][P], w0[B][P], w1[B][P], clx[B];
+    float dw[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const int x = unpack_x(q[r]), y = unpack_y(q[r]);
+        const float fx = (float)x, fy = (float)y;
+        const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
+        if (MODE == kCostLeft) clx[r] = clrow[x];
+        if (MODE == kCostWta) dw[r] = a.dwta[(long)y * a.W + x];
+        if (MODE != kCostWta) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const float df = pl[k].x * fx + pl[k].y * fy + pl[k].z;
+                // disparity clamped into [0, D-2] (NaN -> 0)
+                const int dc = (int)fminf(fmaxf(floorf(df), 0.f), dmax);
+                if (MODE == kCostLeft) {
+                    // CR(max(x-dc, 0)) and CR(max(x-dc-1, 0)) as base[0], base[-1] from base = max(x-dc, 1)
+                    const int xa = x - dc;
+                    const uint32_t *base = crrow + (xa > 1 ? xa : 1);
+                    const uint32_t hi = base[0], lo = base[-1];
+                    u0[r][k] = xa > 0 ? hi : lo;
+                    u1[r][k] = lo;
+                } else {
+                    const int xa = x + dc < a.W - 1 ? x + dc : a.W - 1;
+                    const int xb = x + dc + 1 < a.W - 1 ? x + dc + 1 : a.W - 1;
+                    u0[r][k] = clrow[xa];
+                    w0[r][k] = crrow[xa - dc];
+                    u1[r][k] = clrow[xb];
+                    w1[r][k] = crrow[xb - dc - 1];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        if (MODE == kCostLeft) pin(clx[r]);
+        if (MODE == kCostWta) pin(dw[r]);
+        if (MODE != kCostWta) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                pin(u0[r][k]);
+                pin(u1[r][k]);
+                if (MODE == kCostRight) {
+                    pin(w0[r][k]);
+                    pin(w1[r][k]);
+                }
+            }
+        }
+    }
+    const float dlim = (float)(a.D - 1);
+    const bool d_ok = a.D > 1;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const bool valid = first + (r0 + r) * 64 + lane < end;
+        const float fx = (float)unpack_x(q[r]), fy = (float)unpack_y(q[r]);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const float df = pl[k].x * fx + pl[k].y * fy + pl[k].z;
+            float cost;
+            if (MODE == kCostWta) {
+                cost = fabsf(dw[r] - df) <= 1.f ? 0.f : 1.f;
+            } else {
+                // d = floor(df) in range [0, D-1) (NaN counts as d = 0, in range); the weights
+                // (d+1) - df and df - d are exact float forms of the integer d
+                const float fl = floorf(df);
+                const bool in = d_ok && !(fl < 0.f) && !(fl >= dlim);
+                int c0, c1;
+                if (MODE == kCostLeft) {
+                    c0 = __popc(clx[r] ^ u0[r][k]);
+                    c1 = __popc(clx[r] ^ u1[r][k]);
+                } else {
+                    c0 = __popc(u0[r][k] ^ w0[r][k]);
+                    c1 = __popc(u1[r][k] ^ w1[r][k]);
+                }
+                const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
+                const float ci = ((fl + 1.f) - df) * v0 + (df - fl) * v1;
+                cost = in ? ci : a.tau;
+            }
+            const float v = a.lambda * cost;
+            const bool isn = v != v;
+            // every cost is >= 0, so every term has lambda's sign: the lanes add |q| (rint is
+            // odd) and the segment's sum takes lambda's sign at the end.  Invalid lanes and NaN
+            // terms add zero; NaN sets the slot's flag.
+            const float vz = (valid && !isn) ? fabsf(v) : 0.f;
+            acc[k] += nonneg_int_float_to_i64(rintf(ldexpf(vz, a.shift)));
+            nanm |= (valid && isn) ? (1u << (slot0 + k)) : 0u;
+        }
+    }
+}
+
 template <int NS, int MODE>
 __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
+    static_assert(NS % P == 0, "planes are costed in groups of P");
     const int lane = threadIdx.x & 63;
     const int4 sg = a.seg[w];
     if (sg.y <= sg.x) return;   // past the last segment (the array is padded with empty ones)
     const int c = __builtin_amdgcn_readfirstlane(sg.z);
-    float4 pl[NS];
-    if (spatial) {
-        float4 cand = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < P) cand = spatial_candidate(a, c, lane);
-        if (sg.w == 0 && lane < P) a.plane[(long)c * K + 2 * P + lane] = cand;
-#pragma unroll
-        for (int k = 0; k < NS; ++k)
-            pl[k] = make_float4(__shfl(cand.x, k), __shfl(cand.y, k), __shfl(cand.z, k), 0.f);
-    } else {
-#pragma unroll
-        for (int k = 0; k < NS; ++k) pl[k] = a.plane[(long)c * K + slot0 + k];
-    }
-    int64_t acc[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) acc[k] = 0;
-    unsigned int nanm = 0u;
     const int first = __builtin_amdgcn_readfirstlane(sg.x), end = __builtin_amdgcn_readfirstlane(sg.y);
     const int steps = (end - first + 63) >> 6;
-#pragma unroll 2
-    for (int r = 0; r < steps; ++r) {
-        const int j = first + r * 64 + lane;
-        const bool valid = j < end;
-        const PixelIn px = load_pixel<MODE>(a, a.pxy[valid ? j : end - 1]);
+    unsigned int nanm = 0u;
+    unsigned long long mine = 0ull;
+#pragma unroll 1
+    for (int g0 = 0; g0 < NS; g0 += P) {
+        float4 pl[P];
+        if (spatial) {
+            float4 cand = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane < P) cand = spatial_candidate(a, c, lane);
+            if (sg.w == 0 && lane < P) a.plane[(long)c * K + 2 * P + lane] = cand;
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            float v = pixel_plane_cost<MODE>(a, pl[k], px.x, px.fx, px.fy, px.clrow, px.crrow, px.clx, px.dw);
-            bool isn = v != v;
-            int64_t q = to_fixed_finite(isn ? 0.f : v, a.shift);
-            acc[k] += (valid && !isn) ? q : 0;
-            nanm |= (valid && isn) ? (1u << (slot0 + k)) : 0u;
+            for (int k = 0; k < P; ++k)
+                pl[k] = make_float4(__shfl(cand.x, k), __shfl(cand.y, k), __shfl(cand.z, k), 0.f);
+        } else {
+#pragma unroll
+            for (int k = 0; k < P; ++k) pl[k] = a.plane[(long)c * K + slot0 + g0 + k];
+        }
+        int64_t acc[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc[k] = 0;
+        if (MODE == kCostRight || NS > P || steps <= 4) {
+            for (int r0 = 0; r0 < steps; r0 += 4) pc_batch<MODE, 4>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+        } else {
+            pc_batch<MODE, 8>(a, pl, slot0 + g0, first, end, 0, acc, nanm);
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+            if (lane == g0 + k) mine = (unsigned long long)(a.lambda < 0.f ? -acc[k] : acc[k]);
         }
     }
 #pragma unroll
-    for (int k = 0; k < NS; ++k)
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
-#pragma unroll
     for (int o = 32; o >= 1; o >>= 1) nanm |= __shfl_xor(nanm, o);
-    unsigned long long mine = 0ull;
-#pragma unroll
-    for (int k = 0; k < NS; ++k)
-        if (lane == k) mine = (unsigned long long)acc[k];
     const int rep = w % a.rrep;
     if (lane < NS && mine) atomicAdd(&a.acc[((long)c * a.rrep + rep) * K + slot0 + lane], mine);
     if (lane == 0 && nanm) atomicOr(&a.accnan[c], nanm);

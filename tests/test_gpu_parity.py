@@ -214,6 +214,26 @@ def test_bp_use_cost_vol_false(ctx):
     assert same_bits(do, dg) and same_bits(plo, plg)
 
 
+def test_bp_negative_alpha(ctx):
+    """alpha < 0 (every plane cost term negative): the fixed-point sums keep the sign."""
+    say("bp alpha=-3.7")
+    left, right = _bp_pair(9, 96, 72, 24)
+    vo, wo = O.cost_volume(left, right, 24)
+    po = O.Pyramid(left)
+    bo = O.BP(po, vo, wo, 24, min_level=2, alpha=-3.7)
+    for _ in range(6):
+        bo.step(1)
+    bg = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, 24), alpha=-3.7, min_level=2)
+    for _ in range(6):
+        bg.step(1)
+    so, sg = bo.state(), bg.state()
+    assert same_bits(so["cost"], sg["cost"]) and same_bits(so["belief"], sg["belief"])
+    bo.step()
+    do, plo = bo.map()
+    dg, plg = bg.run()
+    assert same_bits(do, dg) and same_bits(plo, plg)
+
+
 def test_plain_path_cfg1_bit_exact(ctx):
     """450x375, D=64, reference defaults (alpha 1000.1, tau_s 0.6, min_level 2)."""
     say("plain path 450x375 D=64")
