@@ -39,7 +39,7 @@ constexpr int COARSE_MIN_PIX = 32;           // average pixels per superpixel of
 
 // replicas of each fixed-point accumulator: work item i adds into replica i mod rrep, which
 // spreads the atomics of the few large superpixels of the coarse levels over rrep addresses
-static inline int acc_replicas(int V) { return V <= 4096 ? 16 : (V <= 65536 ? 4 : 1); }
+static inline int acc_replicas(int V) { return V <= 4096 ? 16 : (V <= 65536 ? 4 : 1); }   // <= kMaxReplicas
 
 // ----------------------------------------------------------------------------------------
 // prologue kernels
@@ -69,6 +69,31 @@ __global__ void lower_bound_kernel(const unsigned int *__restrict__ keys, long n
 __global__ void count_src_kernel2(const int2 *__restrict__ bd, long E, int *__restrict__ cnt) {
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x)
         atomicAdd(&cnt[bd[e].x], 1);
+}
+__global__ void deg_max_kernel(const int *__restrict__ off, int V, int *__restrict__ dmax) {
+    int m = 0;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
+        m = max(m, off[v + 1] - off[v]);
+    if (m > 0) atomicMax(dmax, m);
+}
+// node-major edge slots: edge e = (s, t) is slot eloc[e] = e - off[s] of s; revl[s][slot] = rev(e)
+__global__ void node_major_kernel(const int2 *__restrict__ bd, const int *__restrict__ off,
+                                  const int *__restrict__ rev, long E, int maxdeg, int *__restrict__ eloc,
+                                  int *__restrict__ revl) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
+        int sn = bd[e].x;
+        int i = (int)(e - off[sn]);
+        eloc[e] = i;
+        revl[(long)sn * maxdeg + i] = rev[e];
+    }
+}
+// WTA disparity of the sorted pixels
+__global__ void pw_kernel(const uint32_t *__restrict__ pxy, long n, const float *__restrict__ dwta, int W,
+                          float *__restrict__ pw) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        uint32_t q = pxy[j];
+        pw[j] = dwta[(long)unpack_y(q) * W + unpack_x(q)];
+    }
 }
 // setup_D4 (hbp_isp.cu:871-891): pixels in (cluster, pixel id) order as packed xy + cluster
 __global__ void pix_kernel(const unsigned int *__restrict__ keys, const int *__restrict__ ids, long n, int W,
@@ -173,7 +198,7 @@ __device__ __forceinline__ float4 fit_plane(float4 A, float4 B, float4 C) {
 // (advanced in place), three distinct pixels of the superpixel, the plane through their WTA
 // disparities.  q returns the packed pixels.
 __device__ __forceinline__ float4 ransac_plane(uint32_t &s, const uint32_t *__restrict__ pxy, int pos, int n,
-                                               const float *__restrict__ dwta, int W, uint4 &q) {
+                                               const float *__restrict__ pw, uint4 &q) {
     s = lcg_next(s);
     int p0 = (int)umod0(s, n);
     s = lcg_next(s);
@@ -184,23 +209,22 @@ __device__ __forceinline__ float4 ransac_plane(uint32_t &s, const uint32_t *__re
     q.x = pxy[pos + p0];
     q.y = pxy[pos + p1];
     q.z = pxy[pos + p2];
-    int x0 = unpack_x(q.x), y0 = unpack_y(q.x), x1 = unpack_x(q.y), y1 = unpack_y(q.y), x2 = unpack_x(q.z),
-        y2 = unpack_y(q.z);
-    float4 A = make_float4((float)x0, (float)y0, dwta[(long)y0 * W + x0], 0.f);
-    float4 B = make_float4((float)x1, (float)y1, dwta[(long)y1 * W + x1], 0.f);
-    float4 C = make_float4((float)x2, (float)y2, dwta[(long)y2 * W + x2], 0.f);
+    const float z0 = pw[pos + p0], z1 = pw[pos + p1], z2 = pw[pos + p2];
+    float4 A = make_float4((float)unpack_x(q.x), (float)unpack_y(q.x), z0, 0.f);
+    float4 B = make_float4((float)unpack_x(q.y), (float)unpack_y(q.y), z1, 0.f);
+    float4 C = make_float4((float)unpack_x(q.z), (float)unpack_y(q.z), z2, 0.f);
     return fit_plane(A, B, C);
 }
 
 // ransac_candidate_plane for the candidate slots 2P..3P-1 of one superpixel
 __device__ void ransac_node(int id, const uint32_t *__restrict__ pxy, const int *__restrict__ pix_off,
-                            const float *__restrict__ dwta, int W, uint32_t *__restrict__ s_ransac,
+                            const float *__restrict__ pw, uint32_t *__restrict__ s_ransac,
                             float4 *__restrict__ plane, uint4 *__restrict__ point) {
     int pos = pix_off[id];
     int n = pix_off[id + 1] - pos;
     uint32_t s = s_ransac[id];
     uint4 q = make_uint4(0u, 0u, 0u, 0u);
-    for (int p = 2 * P; p < 3 * P; ++p) plane[(long)id * K + p] = ransac_plane(s, pxy, pos, n, dwta, W, q);
+    for (int p = 2 * P; p < 3 * P; ++p) plane[(long)id * K + p] = ransac_plane(s, pxy, pos, n, pw, q);
     point[id] = q;
     s_ransac[id] = s;
 }
@@ -243,6 +267,10 @@ struct SweepArgs {
     const int *sego;             // coarse levels: first segment of each superpixel [V+1]
     const int *off;
     const int *rev;
+    int maxdeg;                  // largest out-degree of the level
+    const int *eloc;             // slot of edge e among its source's edges [E]
+    const int *revl;             // reverse edge of each (node, slot) [V][maxdeg]
+    const float *pw;             // WTA disparity of the sorted pixels [N]
     const float *c0;
     const float *dpos;
     const float2 *mxy;
@@ -265,7 +293,7 @@ struct SweepArgs {
     int use_cost_vol, shift;
     // state
     float4 *msg;                 // [E][2] float4
-    float *msg_t;                // [E][12]
+    float *msg_t;                // sweep messages, node-major [V][maxdeg][12]
     unsigned long long *acc;     // [V][rrep][12] fixed-point sums, rrep replicas
     int rrep;
     unsigned int *accnan;        // [V]
@@ -278,13 +306,22 @@ constexpr size_t PC_LDS_BYTES =
     (MAXC12 * 12 * 8 + MAXC12 * 4) > (MAXC4 * 4 * 8 + MAXC4 * 4) ? (MAXC12 * 12 * 8 + MAXC12 * 4)
                                                                  : (MAXC4 * 4 * 8 + MAXC4 * 4);
 
+// sum / clear of the rrep replicas of one accumulator; unrolled to the largest replica count
+// so that all loads are in flight together
+constexpr int kMaxReplicas = 16;
 __device__ __forceinline__ int64_t acc_sum(const SweepArgs &a, long id, int k) {
+    unsigned long long v[kMaxReplicas];
+#pragma unroll
+    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < a.rrep ? a.acc[(id * a.rrep + r) * K + k] : 0ull;
     unsigned long long t = 0ull;
-    for (int r = 0; r < a.rrep; ++r) t += a.acc[(id * a.rrep + r) * K + k];
+#pragma unroll
+    for (int r = 0; r < kMaxReplicas; ++r) t += v[r];
     return (int64_t)t;
 }
 __device__ __forceinline__ void acc_zero(const SweepArgs &a, long id, int k) {
-    for (int r = 0; r < a.rrep; ++r) a.acc[(id * a.rrep + r) * K + k] = 0ull;
+#pragma unroll
+    for (int r = 0; r < kMaxReplicas; ++r)
+        if (r < a.rrep) a.acc[(id * a.rrep + r) * K + k] = 0ull;
 }
 
 // init_bp (hbp_isp.cu:930-950): copy_prior_plane (56-73), init_rand_seed_k (570-591), plus the
@@ -308,7 +345,7 @@ __device__ void init_node(const SweepArgs &a, long id) {
         a.cost[id * K + k] = 0.f;
     }
     a.accnan[id] = 0u;
-    ransac_node((int)id, a.pxy, a.pix_off, a.dwta, a.W, a.s_ransac, a.plane, a.point);
+    ransac_node((int)id, a.pxy, a.pix_off, a.pw, a.s_ransac, a.plane, a.point);
 }
 
 // costs of slots 0..2P-1 after init (compute_plane_cost(0), (P)); belief = cost
@@ -665,9 +702,11 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
     const int y = lane & 15;
     const bool live = e < a.E && y < K;
     float out = 0.f;
+    long mslot = 0;
     if (live) {
         int2 b = a.bd[e];
         int s = b.x, t = b.y;
+        mslot = (long)s * a.maxdeg + a.eloc[e];
         uint4 ps = a.point[s], pt = a.point[t];
         float X[6], Y[6];
         X[0] = (float)unpack_x(ps.x); Y[0] = (float)unpack_y(ps.x);
@@ -708,7 +747,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
 #pragma unroll
     for (int k = 0; k < K; ++k) mean += __shfl(out, g + k, 64);
     mean /= K;
-    if (live) a.msg_t[e * K + y] = out - mean;
+    if (live) a.msg_t[mslot * K + y] = out - mean;
 }
 
 // ----------------------------------------------------------------------------------------
@@ -727,8 +766,7 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     const bool own = k < K;
     const int kk = own ? k : 0;
     // independent loads first
-    const int eb = a.off[s], ee = a.off[s + 1];
-    const int deg = ee - eb;
+    const int deg = a.off[s + 1] - a.off[s];
     const int ppos = a.pix_off[s], pn = a.pix_off[s + 1] - ppos;
     // the NaN bits of the candidate sums and the RANSAC seed: lane 2P reads (and rewrites)
     // them, the others get them by shuffle
@@ -741,10 +779,12 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     float cost = kk < 2 * P ? a.cost[s * K + kk] : 0.f;
     int64_t fsum = kk >= 2 * P ? acc_sum(a, s, kk) : 0;
     const float4 pin = a.plane[s * K + kk];
-    const int rv = (k < deg && k < MAXDEG) ? a.rev[eb + k] : 0;
+    // node-major slots: every address below depends on s only
+    const long nm = (long)s * a.maxdeg;
+    const int rv = k < a.maxdeg ? a.revl[nm + k] : 0;
     float mt[MAXDEG];
 #pragma unroll
-    for (int i = 0; i < MAXDEG; ++i) mt[i] = i < deg ? a.msg_t[(long)(eb + i) * K + kk] : 0.f;
+    for (int i = 0; i < MAXDEG; ++i) mt[i] = i < a.maxdeg ? a.msg_t[(nm + i) * K + kk] : 0.f;
     if (k == 2 * P) a.accnan[s] = 0u;
     nb = __shfl(nb, g + 2 * P);
     if (kk >= 2 * P) {
@@ -756,7 +796,7 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
 #pragma unroll
     for (int i = 0; i < MAXDEG; ++i)
         if (i < deg) bel += mt[i];
-    for (int e = eb + MAXDEG; e < ee; ++e) bel += a.msg_t[(long)e * K + kk];
+    for (int i = MAXDEG; i < deg; ++i) bel += a.msg_t[(nm + i) * K + kk];
     // rank: order[k] = #{j < k: !(bel[k] < bel[j])} + #{j > k: bel[j] < bel[k]}
     int order = 0;
 #pragma unroll
@@ -793,9 +833,9 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
             if (k < 2 * P) msg[(long)r * 8 + k] = o;
         }
     }
-    for (int e = eb + MAXDEG; e < ee; ++e) {
-        const float o = __shfl(a.msg_t[(long)e * K + kk], g + src);
-        if (k < 2 * P) msg[(long)a.rev[e] * 8 + k] = o;
+    for (int i = MAXDEG; i < deg; ++i) {
+        const float o = __shfl(a.msg_t[(nm + i) * K + kk], g + src);
+        if (k < 2 * P) msg[(long)a.revl[nm + i] * 8 + k] = o;
     }
     if (this_kind == kSpatial && s < a.Vh && k == 0) {
         uint32_t sc = a.s_child[s], sn = a.s_neighbor[s];
@@ -817,7 +857,7 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
             for (int i = 0; i < steps * (k - 2 * P); ++i) sk = lcg_next(sk);
             if (next_kind == kRansac) {
                 uint4 q = make_uint4(0u, 0u, 0u, 0u);
-                a.plane[s * K + k] = ransac_plane(sk, a.pxy, ppos, pn, a.dwta, a.W, q);
+                a.plane[s * K + k] = ransac_plane(sk, a.pxy, ppos, pn, a.pw, q);
                 if (k == K - 1) a.point[s] = q;
             } else {
                 float2 mxy = a.mxy[s];
@@ -848,20 +888,34 @@ __global__ __launch_bounds__(256) void node_kernel(SweepArgs a, int this_kind, i
 // Plane costs of slots [slot0, slot0+NS) and, for NS == P, the sweep's messages.  Coarse
 // levels: wavefronts [0, npc) take segments and the rest take 4 edges each.  Fine levels:
 // blocks [0, npc) take chunks and the waves of the other blocks take 4 edges each.
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8); this maps
+// them so that each XCD takes one contiguous range of work items, whose pixels share census
+// rows in that XCD's L2.  A bijection on [0, n) for any n.
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    constexpr int X = 8;
+    const int q = n / X, r = n % X, x = b % X;
+    return x * q + (x < r ? x : r) + b / X;
+}
+
 template <int NS, bool COARSE, int MODE>
 __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, int spatial, int npc) {
     extern __shared__ unsigned long long lds_pc[];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (COARSE) {
-        const int w = blockIdx.x * 4 + wave;
-        if (w < npc) pc_segment<NS, MODE>(a, w, slot0, spatial != 0);
-        else if (NS == P) edge_wave(a, w - npc, spatial != 0);
-    } else {
-        constexpr int R = NS == P ? CH_R4 : CH_R12;
-        constexpr int MAXC = 256 * R / 2 + 2;
-        if ((int)blockIdx.x < npc)
-            pc_chunk<NS, R, MODE>(a, blockIdx.x, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
-        else if (NS == P) edge_wave(a, (long)(blockIdx.x - npc) * 4 + wave, false);
+    const int b = blockIdx.x;
+    // plane-cost blocks first (XCD-contiguous), then the message blocks in dispatch order
+    const int npc_blocks = COARSE ? (npc + 3) / 4 : npc;
+    if (b < npc_blocks) {
+        const int blk = xcd_contiguous(b, npc_blocks);
+        if (COARSE) {
+            const int w = blk * 4 + wave;
+            if (w < npc) pc_segment<NS, MODE>(a, w, slot0, spatial != 0);
+        } else {
+            constexpr int R = NS == P ? CH_R4 : CH_R12;
+            constexpr int MAXC = 256 * R / 2 + 2;
+            pc_chunk<NS, R, MODE>(a, blk, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
+        }
+    } else if (NS == P) {
+        edge_wave(a, (long)(b - npc_blocks) * 4 + wave, COARSE && spatial != 0);
     }
 }
 
@@ -995,6 +1049,10 @@ static SweepArgs sweep_args(BP &b, int l) {
     a.sego = B.sego.p;
     a.off = B.off.p;
     a.rev = B.rev.p;
+    a.maxdeg = B.maxdeg;
+    a.eloc = B.eloc.p;
+    a.revl = B.revl.p;
+    a.pw = B.pw.p;
     a.c0 = B.c0.p;
     a.dpos = B.dpos.p;
     a.mxy = B.mxy.p;
@@ -1047,6 +1105,8 @@ void bp_setup(Ctx &c, BP &b) {
     long N = b.N;
     DevBuf<unsigned int> im_clus(c.pool, N), keys(c.pool, N), keys2(c.pool, N);
     DevBuf<int> ids(c.pool, N), ids2(c.pool, N);
+    DevBuf<int> dmax(c.pool, (size_t)b.L);
+    HBP_CHECK(hipMemsetAsync(dmax.p, 0, sizeof(int) * (size_t)b.L, c.stream));
     hipLaunchKernelGGL(iota_u32_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, im_clus.p, N);
     int maxV = 1;
     long maxE = 1;
@@ -1096,6 +1156,10 @@ void bp_setup(Ctx &c, BP &b) {
             scan_exclusive_int(c, cnt.p, B.off.p, V + 1, nullptr);
             B.rev.alloc(c.pool, E);
             hipLaunchKernelGGL(rev_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, B.bd, B.off.p, E, B.rev.p);
+            hipLaunchKernelGGL(deg_max_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.off.p, V, dmax.p + l);
+            B.pw.alloc(c.pool, N);
+            hipLaunchKernelGGL(pw_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, B.pxy.p, N, v.dwta.p, b.W,
+                               B.pw.p);
             B.c0.alloc(c.pool, E);
             B.dpos.alloc(c.pool, E);
             hipLaunchKernelGGL(edge_weights_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, mean, V, B.bd,
@@ -1129,8 +1193,23 @@ void bp_setup(Ctx &c, BP &b) {
             hipLaunchKernelGGL(compose_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, im_clus.p, B.clus, N);
         HBP_CHECK(hipGetLastError());
     }
+    // node-major message slots: the out-degree bound of every BP level
+    std::vector<int> hmax(b.L, 0);
+    HBP_CHECK(hipMemcpyAsync(hmax.data(), dmax.p, sizeof(int) * (size_t)b.L, hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    size_t slots = 1;
+    for (int l = b.min_level; l < b.L; ++l) {
+        BPLevel &B = b.lv[l];
+        B.maxdeg = std::max(1, hmax[l]);
+        slots = std::max(slots, (size_t)B.V * B.maxdeg);
+        B.eloc.alloc(c.pool, std::max<long>(B.E, 1));
+        B.revl.alloc(c.pool, (size_t)B.V * B.maxdeg);
+        if (B.E > 0)
+            hipLaunchKernelGGL(node_major_kernel, dim3(grid_for(c, B.E)), dim3(256), 0, c.stream, B.bd, B.off.p,
+                               B.rev.p, B.E, B.maxdeg, B.eloc.p, B.revl.p);
+    }
     b.msg.alloc(c.pool, (size_t)maxE * 2);
-    b.msg_t.alloc(c.pool, (size_t)maxE * K);
+    b.msg_t.alloc(c.pool, slots * K);
     size_t acc_n = 0;
     for (int l = b.min_level; l < b.L; ++l)
         acc_n = std::max(acc_n, (size_t)b.lv[l].V * acc_replicas(b.lv[l].V) * K);
@@ -1183,7 +1262,7 @@ static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l
     ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
     if (B.coarse) {
         int npc = costs ? B.nseg_max : 0;
-        long blocks = div_up((long)npc + nedge_w, 4);
+        long blocks = div_up(npc, 4) + div_up(nedge_w, 4);
         if (blocks == 0) return;
         launch_cost_edge_mode<NS, true>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
     } else {

@@ -173,11 +173,13 @@ int hbp_profile_read(hbp_ctx *ctx, const char *kernel, double *ms, long long *la
     HBP_API_END
 }
 
-int hbp_profile_only(hbp_ctx *ctx, const char *prefix) {
+int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx, "ctx is NULL");
     ctx->c.prof.flush();
     ctx->c.prof.only = prefix ? prefix : "";
+    ctx->c.prof.every = every > 1 ? every : 1;
+    ctx->c.prof.seen = 0;
     HBP_API_END
 }
 

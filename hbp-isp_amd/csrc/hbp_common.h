@@ -234,6 +234,8 @@ struct KernelStat {
 struct Profiler {
     bool on = false;
     std::string only;   // when set, only launches whose name starts with it are timed
+    int every = 1;      // of those, one launch in `every` is timed
+    long long seen = 0;
     std::vector<std::string> names;
     std::vector<KernelStat> stats;
     struct Pending {
@@ -308,6 +310,7 @@ public:
     ProfScope(Ctx &c, const std::string &name, double bytes) : c_(c) {
         if (!c.prof.on) return;
         if (!c.prof.only.empty() && name.compare(0, c.prof.only.size(), c.prof.only) != 0) return;
+        if (c.prof.every > 1 && (c.prof.seen++ % c.prof.every) != 0) return;
         id_ = c.prof.id_of(name);
         a_ = c.prof.take();
         b_ = c.prof.take();

@@ -45,6 +45,10 @@ struct BPLevel {
     DevBuf<int> sego;              // coarse: first segment of each superpixel; sego[V] = count
     DevBuf<int> off;               // CSR offsets by source [V+1]
     DevBuf<int> rev;               // reverse edge [E]
+    int maxdeg = 1;                // largest out-degree
+    DevBuf<int> eloc;              // slot of each edge among its source's edges [E]
+    DevBuf<int> revl;              // reverse edge per (node, slot) [V][maxdeg]
+    DevBuf<float> pw;              // WTA disparity of the sorted pixels [N]
     DevBuf<float> c0, dpos;        // per edge: max(dc0, .01) * bdl and max(dc1, 1) [E]
     DevBuf<float2> mxy;            // normalised mean position [V]
     DevBuf<int> child, child_off;  // nodes grouped by parent [V], [Vh+1]
@@ -69,7 +73,7 @@ struct BP {
     int fx_shift = 0;
     std::vector<BPLevel> lv;
     DevBuf<float4> msg;            // [E][8] persistent messages of the current level (2 x float4)
-    DevBuf<float> msg_t;           // [E][12] messages of the current sweep
+    DevBuf<float> msg_t;           // [V][maxdeg][12] messages of the current sweep (node-major)
     DevBuf<unsigned long long> acc;  // [V][12] fixed-point plane-cost sums
     DevBuf<unsigned int> accnan;     // [V] NaN bits per slot
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;

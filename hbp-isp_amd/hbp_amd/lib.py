@@ -39,7 +39,7 @@ SIGNATURES = {
     "hbp_profile_read": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_double)]),
     "hbp_profile_names": (C.c_int, [_vp, C.c_char_p, C.c_int]),
-    "hbp_profile_only": (C.c_int, [_vp, C.c_char_p]),
+    "hbp_profile_only": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "hbp_cost_volume": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_volume_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_to_right": (C.c_int, [_vp, _vp]),
@@ -133,8 +133,8 @@ class Ctx:
     def profile_reset(self):
         _check(self.lib.hbp_profile_reset(self.h))
 
-    def profile_only(self, prefix=None):
-        _check(self.lib.hbp_profile_only(self.h, prefix.encode() if prefix else None))
+    def profile_only(self, prefix=None, every=1):
+        _check(self.lib.hbp_profile_only(self.h, prefix.encode() if prefix else None, every))
 
     def profile_stats(self):
         n = self.lib.hbp_profile_names(self.h, None, 0)

@@ -41,8 +41,9 @@ int hbp_profile_reset(hbp_ctx *ctx);
 int hbp_profile_read(hbp_ctx *ctx, const char *kernel, double *total_ms, long long *launches,
                      double *bytes);
 int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen);
-/* time only launches whose name starts with prefix (NULL or "" times all) */
-int hbp_profile_only(hbp_ctx *ctx, const char *prefix);
+/* time only launches whose name starts with prefix (NULL or "" times all), and of those one
+ * launch in `every` (1 = all) */
+int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every);
 
 /* ---- (a) cost volume: StereoCost(imgL, imgR, labels, vol_dir) + compute_stereo_cost("Census")
  *      (stereo_cost.hpp:90-116, stereo_cost.cu:421-445).  algo must be "Census". */
