@@ -109,6 +109,28 @@ def main():
     ctx.profile_only(None)
     # the same steps without any events (the events' own cost)
     elapsed_plain, _ = timed(args.steps)
+    # the host-buffer boundary (images from host memory, disparity + planes back to host):
+    # PCIe-inclusive time per pair, outside the timed region
+    disp_h = np.zeros((Hh, W), np.float32)
+    plane_h = np.zeros((Hh, W, 3), np.float32)
+
+    def one_pair_host():
+        vol, pyr, bp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        H._check(lib.hbp_cost_volume(ctx.h, left, right, W, Hh, D, b"Census", C.byref(vol)))
+        H._check(lib.hbp_pyramid_build(ctx.h, left, W, Hh, 2, 4, 0.0, C.byref(pyr)))
+        H._check(lib.hbp_bp_create(ctx.h, pyr, vol, 1000.1, 0.6, info["min_level"], 1, C.byref(bp)))
+        H._check(lib.hbp_bp_run(ctx.h, bp, disp_h.ctypes.data_as(C.c_void_p), plane_h.ctypes.data_as(C.c_void_p)))
+        lib.hbp_bp_free(ctx.h, bp)
+        lib.hbp_pyramid_free(ctx.h, pyr)
+        lib.hbp_volume_free(ctx.h, vol)
+
+    one_pair_host()
+    ctx.sync()
+    th = time.perf_counter()
+    for _ in range(3):
+        one_pair_host()
+    ms_host = 1e3 * (time.perf_counter() - th) / 3
+
     # one more pair with every kernel timed, outside the timed region: per-kernel breakdown
     ctx.profile(True)
     ctx.profile_reset()
@@ -163,6 +185,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "ms_per_step_without_events": round(1e3 * elapsed_plain / args.steps, 3),
+        "ms_per_pair_host_buffers": round(ms_host, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
