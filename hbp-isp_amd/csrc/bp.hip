@@ -919,6 +919,77 @@ __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, 
     }
 }
 
+// ----------------------------------------------------------------------------------------
+// batched sweeps: one launch carries one sweep of each of several pairs.  A pair's share of a
+// launch is a BatchItem; its blocks are a contiguous range starting at a multiple of 8 (so
+// that the XCD-contiguous mapping inside the range holds), found by binary search.
+// ----------------------------------------------------------------------------------------
+struct BatchItem {
+    int args;                   // index of the pair-level SweepArgs
+    int kind, next_kind;
+    float ndz, ndn;
+    int coarse, spatial;        // plane-cost form; spatial candidates resolved in the cost kernel
+    int npc;                    // coarse: segment bound; fine: chunks; 0: no plane costs
+    int blk_begin, pc_blocks, edge_blocks;   // cost/message launch
+    int node_begin, node_blocks;             // node launch
+    int sp_begin, sp_blocks;                 // spatial launch (fine levels)
+};
+
+template <int FIELD>
+__device__ __forceinline__ int batch_find(const BatchItem *__restrict__ it, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int v = FIELD == 0 ? it[mid].blk_begin : (FIELD == 1 ? it[mid].node_begin : it[mid].sp_begin);
+        if (v <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void cost_edge_batch_kernel(const SweepArgs *__restrict__ args,
+                                                              const BatchItem *__restrict__ items, int nitems) {
+    extern __shared__ unsigned long long lds_pc[];
+    const int b = blockIdx.x;
+    const BatchItem it = items[batch_find<0>(items, nitems, b)];
+    const int local = b - it.blk_begin;
+    if (local >= it.pc_blocks + it.edge_blocks) return;   // alignment padding
+    const SweepArgs a = args[it.args];
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (local < it.pc_blocks) {
+        const int blk = xcd_contiguous(local, it.pc_blocks);
+        if (it.coarse) {
+            const int w = blk * 4 + wave;
+            if (w < it.npc) pc_segment<P, MODE>(a, w, 2 * P, it.spatial != 0);
+        } else {
+            pc_chunk<P, CH_R4, MODE>(a, blk, 2 * P, lds_pc, (unsigned int *)(lds_pc + MAXC4 * P));
+        }
+    } else {
+        edge_wave(a, (long)(local - it.pc_blocks) * 4 + wave, it.coarse && it.spatial);
+    }
+}
+__global__ __launch_bounds__(256) void node_batch_kernel(const SweepArgs *__restrict__ args,
+                                                         const BatchItem *__restrict__ items, int nitems) {
+    const int b = blockIdx.x;
+    const BatchItem it = items[batch_find<1>(items, nitems, b)];
+    const int local = b - it.node_begin;
+    if (local >= it.node_blocks) return;
+    const SweepArgs a = args[it.args];
+    const long s = ((long)local * 256 + threadIdx.x) >> 4;
+    if (s < a.V) node_group(a, s, it.kind, it.next_kind, it.ndz, it.ndn);
+}
+__global__ __launch_bounds__(256) void spatial_batch_kernel(const SweepArgs *__restrict__ args,
+                                                            const BatchItem *__restrict__ items, int nitems) {
+    const int b = blockIdx.x;
+    const BatchItem it = items[batch_find<2>(items, nitems, b)];
+    const int local = b - it.sp_begin;
+    if (local >= it.sp_blocks) return;
+    const SweepArgs a = args[it.args];
+    const long id = (long)local * 256 + threadIdx.x;
+    if (id < a.V) spatial_node(a, id);
+}
+
 __global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1328,6 +1399,122 @@ int bp_step(Ctx &c, BP &b, int max_sweeps) {
     }
     if (b.next >= b.sched.size()) b.cur = b.min_level - 1;
     return done;
+}
+
+// Runs the remaining sweeps of several BP objects together: launch group i carries sweep
+// next+i of every pair that has one.  Per pair the kernels do exactly what the one-pair path
+// does (same work items, same fixed-point sums), so every pair's results are bit-identical to
+// its own run.
+void bp_run_batch(Ctx &c, const std::vector<BP *> &bps) {
+    const int n = (int)bps.size();
+    if (n == 0) return;
+    const BP &b0 = *bps[0];
+    const int mode = !b0.use_cost_vol ? kCostWta : (b0.vol->right ? kCostRight : kCostLeft);
+    for (BP *bp : bps) {
+        const int m = !bp->use_cost_vol ? kCostWta : (bp->vol->right ? kCostRight : kCostLeft);
+        HBP_REQUIRE(m == mode, "a batch must use one cost form (view and use_cost_vol)");
+    }
+    // SweepArgs of every (pair, level >= min_level)
+    std::vector<SweepArgs> hargs;
+    std::vector<int> args_base(n);
+    for (int p = 0; p < n; ++p) {
+        BP &b = *bps[p];
+        args_base[p] = (int)hargs.size() - b.min_level;
+        for (int l = b.min_level; l < b.L; ++l) hargs.push_back(sweep_args(b, l));
+    }
+    // the items of every launch group
+    size_t rounds = 0;
+    for (BP *bp : bps) rounds = std::max(rounds, bp->sched.size() - bp->next);
+    struct Group {
+        int first, count;                     // items
+        int cost_blocks, node_blocks, sp_blocks;
+    };
+    std::vector<Group> groups(rounds);
+    std::vector<BatchItem> hitems;
+    for (size_t r = 0; r < rounds; ++r) {
+        Group g{(int)hitems.size(), 0, 0, 0, 0};
+        for (int p = 0; p < n; ++p) {
+            BP &b = *bps[p];
+            const size_t i = b.next + r;
+            if (i >= b.sched.size()) continue;
+            const Sweep &sw = b.sched[i];
+            const BPLevel &B = b.lv[sw.level];
+            BatchItem it{};
+            it.args = args_base[p] + sw.level;
+            it.kind = sw.kind;
+            it.next_kind = kNone;
+            if (!sw.last && i + 1 < b.sched.size()) {
+                it.next_kind = b.sched[i + 1].kind;
+                it.ndz = b.sched[i + 1].dz;
+                it.ndn = b.sched[i + 1].dn;
+            }
+            it.coarse = B.coarse ? 1 : 0;
+            it.spatial = sw.kind == kSpatial ? 1 : 0;
+            it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, 256 * CH_R4));
+            it.pc_blocks = B.coarse ? div_up(it.npc, 4) : it.npc;
+            it.edge_blocks = div_up(B.E, 16);
+            it.blk_begin = g.cost_blocks;
+            g.cost_blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
+            it.node_begin = g.node_blocks;
+            it.node_blocks = div_up((long)B.V * 16, 256);
+            g.node_blocks += it.node_blocks;
+            it.sp_begin = g.sp_blocks;
+            it.sp_blocks = (!B.coarse && sw.kind == kSpatial && !sw.first) ? div_up(B.V, 256) : 0;
+            g.sp_blocks += it.sp_blocks;
+            hitems.push_back(it);
+            ++g.count;
+        }
+        groups[r] = g;
+    }
+    DevBuf<SweepArgs> dargs(c.pool, hargs.size());
+    DevBuf<BatchItem> ditems(c.pool, std::max<size_t>(hitems.size(), 1));
+    HBP_CHECK(hipMemcpyAsync(dargs.p, hargs.data(), sizeof(SweepArgs) * hargs.size(), hipMemcpyHostToDevice,
+                             c.stream));
+    HBP_CHECK(hipMemcpyAsync(ditems.p, hitems.data(), sizeof(BatchItem) * hitems.size(), hipMemcpyHostToDevice,
+                             c.stream));
+    for (size_t r = 0; r < rounds; ++r) {
+        const Group &g = groups[r];
+        // level inits of the pairs whose sweep starts a level
+        for (int p = 0; p < n; ++p) {
+            BP &b = *bps[p];
+            const size_t i = b.next + r;
+            if (i < b.sched.size() && b.sched[i].first) {
+                const int l = b.sched[i].level;
+                run_level_init(c, b, hargs[args_base[p] + l], l);
+            }
+        }
+        const BatchItem *it = ditems.p + g.first;
+        if (g.sp_blocks > 0) {
+            ProfScope ps(c, "batch_spatial", 0.0);
+            hipLaunchKernelGGL(spatial_batch_kernel, dim3(g.sp_blocks), dim3(256), 0, c.stream, dargs.p, it, g.count);
+        }
+        if (g.cost_blocks > 0) {
+            ProfScope ps(c, "batch_plane_cost", 0.0);
+            if (mode == kCostLeft)
+                hipLaunchKernelGGL(cost_edge_batch_kernel<kCostLeft>, dim3(g.cost_blocks), dim3(256), PC_LDS_BYTES,
+                                   c.stream, dargs.p, it, g.count);
+            else if (mode == kCostRight)
+                hipLaunchKernelGGL(cost_edge_batch_kernel<kCostRight>, dim3(g.cost_blocks), dim3(256), PC_LDS_BYTES,
+                                   c.stream, dargs.p, it, g.count);
+            else
+                hipLaunchKernelGGL(cost_edge_batch_kernel<kCostWta>, dim3(g.cost_blocks), dim3(256), PC_LDS_BYTES,
+                                   c.stream, dargs.p, it, g.count);
+        }
+        {
+            ProfScope ps(c, "batch_node", 0.0);
+            hipLaunchKernelGGL(node_batch_kernel, dim3(g.node_blocks), dim3(256), 0, c.stream, dargs.p, it, g.count);
+        }
+        HBP_CHECK(hipGetLastError());
+    }
+    for (BP *bp : bps) {
+        BP &b = *bp;
+        b.sweeps += (int)(b.sched.size() - b.next);
+        b.next = b.sched.size();
+        b.cur = b.min_level - 1;
+        bp_map(c, b);
+    }
+    // the tables are read by the launches above
+    HBP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 void bp_map(Ctx &c, BP &b) {

@@ -470,6 +470,20 @@ int hbp_bp_run(hbp_ctx *ctx, hbp_bp *bp, float *disp, float *plane3) {
     HBP_API_END
 }
 
+int hbp_bp_run_batch(hbp_ctx *ctx, int n, hbp_bp *const *bps) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && n >= 0 && (n == 0 || bps), "NULL argument");
+    use_device(ctx);
+    std::vector<BP *> v;
+    for (int i = 0; i < n; ++i) {
+        HBP_REQUIRE(bps[i], "NULL BP object in the batch");
+        for (BP *q : v) HBP_REQUIRE(q != &bps[i]->b, "a BP object appears twice in the batch");
+        v.push_back(&bps[i]->b);
+    }
+    bp_run_batch(ctx->c, v);
+    HBP_API_END
+}
+
 float *hbp_bp_dev_disp(hbp_bp *bp) { return bp ? bp->b.disp.p : nullptr; }
 float *hbp_bp_dev_plane(hbp_bp *bp) { return bp ? bp->b.plane3.p : nullptr; }
 

@@ -101,6 +101,7 @@ int compact_labels(Ctx &c, int *d_label, int V);
 void bp_setup(Ctx &c, BP &b);
 int bp_step(Ctx &c, BP &b, int max_sweeps);
 void bp_map(Ctx &c, BP &b);
+void bp_run_batch(Ctx &c, const std::vector<BP *> &bps);
 void bp_get_state(Ctx &c, BP &b, float *planes, float *cost, float *belief, float *msg);
 int bp_total_sweeps(int levels, int min_level, int D);
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "hbp_bp_level": (C.c_int, [_vp]),
     "hbp_bp_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "hbp_bp_run": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hbp_bp_run_batch": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
     "hbp_bp_dev_plane": (_vp, [_vp]),
     "hbp_bp_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -158,6 +159,12 @@ class Ctx:
     # ---- (c)
     def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
         return BP(self, pyr, vol, alpha, tau_s, min_level, use_cost_vol)
+
+    def bp_run_batch(self, bps):
+        """Runs the remaining sweeps of several BP objects together (hbp_bp_run_batch); each
+        object's results are then read with its run()."""
+        arr = (C.c_void_p * len(bps))(*[b.h for b in bps])
+        _check(self.lib.hbp_bp_run_batch(self.h, len(bps), arr))
 
     # ---- (d)
     def lr_check_fill(self, dispL, dispR, planeL):

@@ -109,6 +109,12 @@ int hbp_bp_level_size(const hbp_bp *bp, int level, int *num_clus, int *num_bd);
 /* map_msg (hbp_isp.cu:1081-1105): disp f32 [H][W] and plane f32 [H][W][3] on the host
  * (either may be NULL); runs the remaining sweeps first */
 int hbp_bp_run(hbp_ctx *ctx, hbp_bp *bp, float *disp, float *plane3);
+/* batched BP (no counterpart in the reference, which runs one pair per HBP_ISP::run_ms): runs
+ * the remaining sweeps of n BP objects of one context together, sweep i of every pair in the
+ * same launches, then the map of each.  Every pair's disparity and planes are bit-identical to
+ * its own hbp_bp_run; read them with hbp_bp_run (which then only maps and copies) or the
+ * device pointers below.  All n objects must use the same view and use_cost_vol. */
+int hbp_bp_run_batch(hbp_ctx *ctx, int n, hbp_bp *const *bps);
 /* device pointers of the last map: disp [H][W], plane [H][W][3] */
 float *hbp_bp_dev_disp(hbp_bp *bp);
 float *hbp_bp_dev_plane(hbp_bp *bp);

@@ -325,3 +325,64 @@ def test_left_right_check_and_fill_bit_exact(ctx, seed, W, H, D):
     assert np.array_equal(fl, flg) and np.array_equal(fr, frg) and np.array_equal(f2, f2g)
     assert same_bits(plo, plg)
     assert same_bits(dispo, ctx.plane_to_disp(plg))
+
+
+# ---------------------------------------------------------------- batched BP
+def _batch_inputs(ctx, specs, right_view=False):
+    out = []
+    for seed, W, H, D, ml in specs:
+        left, right = _bp_pair(seed, W, H, D)
+        vol = ctx.cost_volume(left, right, D, right_view)
+        pyr = ctx.pyramid(right if right_view else left)
+        out.append((pyr, vol, ml))
+    return out
+
+
+def _check_batch(ctx, items, pre_steps=None, **kw):
+    singles = [ctx.bp(p, v, min_level=ml, **kw).run() for p, v, ml in items]
+    bps = [ctx.bp(p, v, min_level=ml, **kw) for p, v, ml in items]
+    if pre_steps:
+        for b, n in zip(bps, pre_steps):
+            b.step(n)
+    ctx.bp_run_batch(bps)
+    for i, (b, (d1, p1)) in enumerate(zip(bps, singles)):
+        d2, p2 = b.run()
+        assert same_bits(d1, d2), f"pair {i}: {mismatch(d1, d2).mean():.4%} of disparities differ"
+        assert same_bits(p1, p2), f"pair {i}: planes differ"
+
+
+def test_batch_equals_single_runs(ctx):
+    """Sweep i of every pair in the same launches: each pair's output is bit-identical to its
+    own run (sizes, disparity ranges and min_level differ, so levels and schedules differ)."""
+    say("batch of 6 mixed pairs")
+    specs = [(21, 64, 48, 16, 2), (22, 96, 72, 24, 1), (23, 120, 90, 32, 2), (24, 160, 120, 48, 3),
+             (25, 80, 200, 32, 2), (26, 200, 64, 40, 1)]
+    _check_batch(ctx, _batch_inputs(ctx, specs))
+
+
+def test_batch_partially_stepped(ctx):
+    say("batch of partially stepped pairs")
+    specs = [(31, 96, 72, 24, 2), (32, 120, 90, 32, 2), (33, 64, 48, 16, 1)]
+    _check_batch(ctx, _batch_inputs(ctx, specs), pre_steps=[0, 7, 30])
+
+
+def test_batch_right_view_and_wta_costs(ctx):
+    say("batch right view / use_cost_vol=0")
+    specs = [(41, 96, 72, 24, 2), (42, 120, 90, 32, 2)]
+    _check_batch(ctx, _batch_inputs(ctx, specs, right_view=True))
+    _check_batch(ctx, _batch_inputs(ctx, specs), use_cost_vol=0, alpha=1.3)
+
+
+def test_batch_larger_pairs(ctx):
+    """Coarse (segment) and fine (chunk) plane-cost forms in the same launches."""
+    say("batch of larger pairs")
+    specs = [(51, 640, 480, 64, 2), (52, 320, 240, 64, 2), (53, 1280, 720, 128, 4)]
+    _check_batch(ctx, _batch_inputs(ctx, specs))
+
+
+def test_batch_rejects_mixed_cost_forms(ctx):
+    left, right = _bp_pair(61, 64, 48, 16)
+    a = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, 16), min_level=2)
+    b = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, 16), min_level=2, use_cost_vol=0)
+    with pytest.raises(H.HbpError):
+        ctx.bp_run_batch([a, b])
