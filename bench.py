@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-pairs", type=int, default=64,
+                    help="pairs of the batched-vs-plain comparison (mixed sizes; 0 skips it)")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
     args = ap.parse_args()
@@ -57,6 +59,8 @@ def main():
     wl = WORKLOADS[args.workload]
     W, Hh, D = wl["width"], wl["height"], wl["ndisp"]
     left, right, _ = synth.make_pair(args.seed, W, Hh, D, wl["n_planes"])
+    # the mixed-size pairs of the batched comparison, generated before the GPU is touched
+    batch_pairs = make_batch_pairs(args.batch_pairs, args.seed) if args.batch_pairs > 0 else None
 
     ctx = H.Ctx(0)
     lib = ctx.lib
@@ -172,6 +176,8 @@ def main():
         except (OSError, ValueError):
             traffic = None
 
+    batched = batched_vs_plain(ctx, hip, batch_pairs) if batch_pairs else None
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(left, right, wl, info, args.cpu_sweeps)
@@ -205,12 +211,134 @@ def main():
         "kernels_one_pair": kernels,
         "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
                                for n, v in sorted(pair_stats.items()) if "_L" in n},
+        "batched_vs_plain": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     hip.hipFree(d_l)
     hip.hipFree(d_r)
     ctx.close()
+
+
+def batch_spec(n, seed):
+    """Mixed sizes: W ~ U[320, 1920], H ~ U[240, 1080] (multiples of 8), D from {64, 128, 192,
+    256}, one generator seed per pair."""
+    from hbp_amd import synth
+    import numpy as np
+    k = synth._key(seed, 77)
+    r = synth._u01(k, np.arange(3 * n, dtype=np.uint64)).reshape(n, 3)
+    out = []
+    for i in range(n):
+        w = int(round((320 + r[i, 0] * 1600) / 8)) * 8
+        h = int(round((240 + r[i, 1] * 840) / 8)) * 8
+        d = (64, 128, 192, 256)[min(3, int(r[i, 2] * 4))]
+        out.append((seed * 1000 + i, w, h, d))
+    return out
+
+
+def _gen_pair(spec):
+    from hbp_amd import synth
+    sd, w, h, d = spec
+    left, right, _ = synth.make_pair(sd, w, h, d, (6, 12))
+    return spec, left, right
+
+
+def make_batch_pairs(n, seed):
+    import multiprocessing as mp
+    specs = batch_spec(n, seed)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_gen_pair, specs)
+
+
+def batched_vs_plain(ctx, hip, pairs, min_level=2):
+    """The same mixed-size pairs through the plain loop (one pair after the other) and through
+    the batched BP (every pair's cost volume and pyramid, then all BP sweeps together); images
+    resident in HBM; outputs checked bit for bit."""
+    import ctypes as C
+    import numpy as np
+    from hbp_amd import lib as H
+    lib = ctx.lib
+    dev = []
+    for spec, left, right in pairs:
+        nb = left.nbytes
+        dl, dr = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(dl), C.c_size_t(nb)) == 0 and hip.hipMalloc(C.byref(dr), C.c_size_t(nb)) == 0
+        assert hip.hipMemcpy(dl, left.ctypes.data_as(C.c_void_p), C.c_size_t(nb), 1) == 0
+        assert hip.hipMemcpy(dr, right.ctypes.data_as(C.c_void_p), C.c_size_t(nb), 1) == 0
+        dev.append((spec, dl, dr))
+
+    def prepare(spec, dl, dr):
+        _, w, h, d = spec
+        vol, pyr, bp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        H._check(lib.hbp_cost_volume_dev(ctx.h, dl, dr, w, h, d, b"Census", C.byref(vol)))
+        H._check(lib.hbp_pyramid_build_dev(ctx.h, dl, w, h, 2, 4, 0.0, C.byref(pyr)))
+        ml = min(min_level, lib.hbp_pyramid_levels(pyr) - 1)
+        H._check(lib.hbp_bp_create(ctx.h, pyr, vol, 1000.1, 0.6, ml, 1, C.byref(bp)))
+        return vol, pyr, bp
+
+    def release(objs):
+        for vol, pyr, bp in objs:
+            lib.hbp_bp_free(ctx.h, bp)
+            lib.hbp_pyramid_free(ctx.h, pyr)
+            lib.hbp_volume_free(ctx.h, vol)
+
+    def download(bp, spec):
+        _, w, h, _ = spec
+        disp = np.zeros((h, w), np.float32)
+        plane = np.zeros((h, w, 3), np.float32)
+        H._check(lib.hbp_bp_run(ctx.h, bp, disp.ctypes.data_as(C.c_void_p), plane.ctypes.data_as(C.c_void_p)))
+        return disp, plane
+
+    def plain(keep):
+        outs = []
+        for spec, dl, dr in dev:
+            o = prepare(spec, dl, dr)
+            H._check(lib.hbp_bp_run(ctx.h, o[2], None, None))
+            if keep:
+                outs.append(download(o[2], spec))
+            release([o])
+        return outs
+
+    def batched(keep):
+        objs = [prepare(spec, dl, dr) for spec, dl, dr in dev]
+        arr = (C.c_void_p * len(objs))(*[o[2] for o in objs])
+        H._check(lib.hbp_bp_run_batch(ctx.h, len(objs), arr))
+        ctx.sync()
+        outs = [download(o[2], spec) for o, (spec, _, _) in zip(objs, dev)] if keep else []
+        release(objs)
+        return outs
+
+    # each form once untimed first (the context's memory pool then holds its buffers)
+    plain(False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    plain(False)
+    ctx.sync()
+    t_plain = time.perf_counter() - t0
+    batched(False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    batched(False)
+    ctx.sync()
+    t_batch = time.perf_counter() - t0
+    # bit-identical outputs (outside the timed runs)
+    ref = plain(True)
+    got = batched(True)
+    same = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
+               np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) for a, b in zip(ref, got))
+    for _, dl, dr in dev:
+        hip.hipFree(dl)
+        hip.hipFree(dr)
+    n = len(dev)
+    mpx = sum(w * h * d for _, w, h, d in (p for p, _, _ in dev)) / 1e6
+    return {"workload": f"{n} pairs, W U[320,1920] x H U[240,1080] (multiples of 8), D from {{64,128,192,256}}, "
+                        f"6-12 planes, min_level 2, reference parameters; images resident in HBM",
+            "pairs": n, "plain_pairs_per_s": round(n / t_plain, 3), "batched_pairs_per_s": round(n / t_batch, 3),
+            "speedup": round(t_plain / t_batch, 3), "plain_mpix_disp_per_s": round(mpx / t_plain, 1),
+            "batched_mpix_disp_per_s": round(mpx / t_batch, 1), "bit_identical": bool(same),
+            "timed": "plain: cost volume + pyramid + BP + map per pair in turn; batched: every pair's cost volume "
+                     "and pyramid, then hbp_bp_run_batch over all pairs"}
 
 
 def cpu_baseline(left, right, wl, info, sweeps):

@@ -300,11 +300,10 @@ struct SweepArgs {
     uint32_t *s_ransac, *s_neighbor, *s_child;
 };
 
-constexpr int CH_R12 = 2, CH_R4 = 8;                              // 256-pixel steps per chunk
-constexpr int MAXC12 = 256 * CH_R12 / 2 + 2, MAXC4 = 256 * CH_R4 / 2 + 2;  // superpixels a chunk can touch
-constexpr size_t PC_LDS_BYTES =
-    (MAXC12 * 12 * 8 + MAXC12 * 4) > (MAXC4 * 4 * 8 + MAXC4 * 4) ? (MAXC12 * 12 * 8 + MAXC12 * 4)
-                                                                 : (MAXC4 * 4 * 8 + MAXC4 * 4);
+// fine levels: a 256-thread block costs CHUNK consecutive sorted pixels, 4 per lane
+constexpr int CHUNK = 1024;
+constexpr int MAXC = CHUNK / 2 + 2;    // superpixels a chunk can touch (each holds >= 2 pixels)
+constexpr size_t PC_LDS_BYTES = (size_t)MAXC * P * 16 + (size_t)MAXC * P * 8 + (size_t)MAXC * 4;
 
 // sum / clear of the rrep replicas of one accumulator; unrolled to the largest replica count
 // so that all loads are in flight together
@@ -597,11 +596,15 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
         int64_t acc[P];
 #pragma unroll
         for (int k = 0; k < P; ++k) acc[k] = 0;
-        if (MODE == kCostRight || NS > P || steps <= 4) {
-            for (int r0 = 0; r0 < steps; r0 += 4) pc_batch<MODE, 4>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
-        } else {
-            pc_batch<MODE, 8>(a, pl, slot0 + g0, first, end, 0, acc, nanm);
+        // batches of 4 steps, then the remainder as one batch of 1 or 2 (+1) steps: no step
+        // past the segment's end is computed
+        int r0 = 0;
+        for (; r0 + 4 <= steps; r0 += 4) pc_batch<MODE, 4>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+        if (steps - r0 >= 2) {
+            pc_batch<MODE, 2>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+            r0 += 2;
         }
+        if (steps - r0 == 1) pc_batch<MODE, 1>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
 #pragma unroll
@@ -617,90 +620,176 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 }
 
 // ----------------------------------------------------------------------------------------
-// plane cost, fine levels: one 256-thread block per chunk of 256 R consecutive sorted pixels
-// (lane-consecutive reads).  Each step's per-pixel values are summed per superpixel run with a
-// segmented wavefront scan; run tails add into LDS int64 counters indexed by superpixel -
-// first superpixel of the chunk (superpixels at BP levels hold >= 2 pixels, so a chunk touches
-// at most 128 R + 1 of them); the block then adds each counter to replica (chunk mod rrep)
-// of the global sum.
+// plane cost, fine levels: one 256-thread block per chunk of CHUNK consecutive sorted pixels,
+// lane l taking pixels base + 4l .. 4l+3 (16-byte loads of positions and superpixel ids).
+// The candidate planes of the chunk's superpixels are staged in LDS; a lane issues the census
+// words of its 4 pixels x 4 planes before using any, sums its terms per superpixel run and
+// adds each run into LDS int64 counters indexed by superpixel - first superpixel of the chunk;
+// the block then adds each counter to replica (chunk mod rrep) of the global sum.  NS = 12
+// runs three groups of 4 planes.
 // ----------------------------------------------------------------------------------------
-template <int NS, int R, int MODE>
-__device__ void pc_chunk(const SweepArgs &a, long chunk, int slot0, unsigned long long *sacc, unsigned int *snan) {
-    constexpr int CH = 256 * R;
-    constexpr int MAXC = CH / 2 + 2;
-    const int lane = threadIdx.x & 63;
-    const long base = chunk * CH;
-    for (int i = threadIdx.x; i < MAXC * NS; i += 256) sacc[i] = 0ull;
-    for (int i = threadIdx.x; i < MAXC; i += 256) snan[i] = 0u;
+template <int NS, int MODE>
+__device__ void pc_runs(const SweepArgs &a, long chunk, int slot0, unsigned char *lds) {
+    float4 *splane = (float4 *)lds;                                       // [MAXC][P]
+    unsigned long long *ssum = (unsigned long long *)(lds + (size_t)MAXC * P * 16);   // [MAXC][P]
+    unsigned int *snan = (unsigned int *)(lds + (size_t)MAXC * P * 24);   // [MAXC]
+    const long base = chunk * CHUNK;
+    const long last = (base + CHUNK < a.N ? base + CHUNK : a.N) - 1;
     const int c_first = (int)a.pcl[base];
-    const long last = (base + CH < a.N ? base + CH : a.N) - 1;
     const int nclus = (int)a.pcl[last] - c_first + 1;
-    __syncthreads();
-    for (int r = 0; r < R; ++r) {
-        const long j = base + (long)r * 256 + threadIdx.x;
-        const bool valid = j < a.N;
-        int c = -1;
-        int64_t v[NS];
+    const long j0 = base + 4L * threadIdx.x;
+    const float dmax = (float)(a.D > 2 ? a.D - 2 : 0);
+    const float dlim = (float)(a.D - 1);
+    const bool d_ok = a.D > 1;
+    const bool neg = a.lambda < 0.f;
+    // this lane's pixels
+    uint32_t q[4], cl[4];
+    bool valid[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        valid[i] = j0 + i <= last;
+        const long j = valid[i] ? j0 + i : last;
+        q[i] = a.pxy[j];
+        cl[i] = a.pcl[j];
+    }
+    for (int g0 = 0; g0 < NS; g0 += P) {
+        for (int i = threadIdx.x; i < nclus * P; i += 256) {
+            splane[i] = a.plane[(long)(c_first + i / P) * K + slot0 + g0 + i % P];
+            ssum[i] = 0ull;
+        }
+        for (int i = threadIdx.x; i < nclus; i += 256) snan[i] = 0u;
+        __syncthreads();
+        uint32_t u0[4][P], u1[4][P], w0[4][P], w1[4][P], clx[4];
+        float dw[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = unpack_x(q[i]), y = unpack_y(q[i]);
+            const float fx = (float)x, fy = (float)y;
+            const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
+            const int lc = (int)cl[i] - c_first;
+            if (MODE == kCostLeft) clx[i] = clrow[x];
+            if (MODE == kCostWta) dw[i] = a.dwta[(long)y * a.W + x];
+            if (MODE != kCostWta) {
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    const float4 pl = splane[lc * P + k];
+                    const float df = pl.x * fx + pl.y * fy + pl.z;
+                    const int dc = (int)fminf(fmaxf(floorf(df), 0.f), dmax);
+                    if (MODE == kCostLeft) {
+                        const int xa = x - dc;
+                        const uint32_t *bp = crrow + (xa > 1 ? xa : 1);
+                        const uint32_t hi = bp[0], lo = bp[-1];
+                        u0[i][k] = xa > 0 ? hi : lo;
+                        u1[i][k] = lo;
+                    } else {
+                        const int xa = x + dc < a.W - 1 ? x + dc : a.W - 1;
+                        const int xb = x + dc + 1 < a.W - 1 ? x + dc + 1 : a.W - 1;
+                        u0[i][k] = clrow[xa];
+                        w0[i][k] = crrow[xa - dc];
+                        u1[i][k] = clrow[xb];
+                        w1[i][k] = crrow[xb - dc - 1];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (MODE == kCostLeft) pin(clx[i]);
+            if (MODE == kCostWta) pin(dw[i]);
+            if (MODE != kCostWta) {
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    pin(u0[i][k]);
+                    pin(u1[i][k]);
+                    if (MODE == kCostRight) {
+                        pin(w0[i][k]);
+                        pin(w1[i][k]);
+                    }
+                }
+            }
+        }
+        // per-run sums (|q|; lambda's sign applied at the flush) and NaN bits
+        int64_t acc[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc[k] = 0;
         unsigned int nanm = 0u;
+        int crun = (int)cl[0];
 #pragma unroll
-        for (int k = 0; k < NS; ++k) v[k] = 0;
-        if (valid) {
-            c = (int)a.pcl[j];
-            const PixelIn px = load_pixel<MODE>(a, a.pxy[j]);
+        for (int i = 0; i < 4; ++i) {
+            if (valid[i] && (int)cl[i] != crun) {
+                const int lr = crun - c_first;
 #pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                float f = pixel_plane_cost<MODE>(a, a.plane[(long)c * K + slot0 + k], px.x, px.fx, px.fy, px.clrow,
-                                                 px.crrow, px.clx, px.dw);
-                if (f != f) nanm |= 1u << (slot0 + k);
-                else v[k] = to_fixed_finite(f, a.shift);
+                for (int k = 0; k < P; ++k)
+                    if (acc[k]) atomicAdd(&ssum[lr * P + k], (unsigned long long)(neg ? -acc[k] : acc[k]));
+                if (nanm) atomicOr(&snan[lr], nanm);
+#pragma unroll
+                for (int k = 0; k < P; ++k) acc[k] = 0;
+                nanm = 0u;
+                crun = (int)cl[i];
+            }
+            const int lc = (int)cl[i] - c_first;
+            const float fx = (float)unpack_x(q[i]), fy = (float)unpack_y(q[i]);
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const float4 pl = splane[lc * P + k];
+                const float df = pl.x * fx + pl.y * fy + pl.z;
+                float cost;
+                if (MODE == kCostWta) {
+                    cost = fabsf(dw[i] - df) <= 1.f ? 0.f : 1.f;
+                } else {
+                    const float fl = floorf(df);
+                    const bool in = d_ok && !(fl < 0.f) && !(fl >= dlim);
+                    int c0, c1;
+                    if (MODE == kCostLeft) {
+                        c0 = __popc(clx[i] ^ u0[i][k]);
+                        c1 = __popc(clx[i] ^ u1[i][k]);
+                    } else {
+                        c0 = __popc(u0[i][k] ^ w0[i][k]);
+                        c1 = __popc(u1[i][k] ^ w1[i][k]);
+                    }
+                    const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
+                    const float ci = ((fl + 1.f) - df) * v0 + (df - fl) * v1;
+                    cost = in ? ci : a.tau;
+                }
+                const float v = a.lambda * cost;
+                const bool isn = v != v;
+                const float vz = (valid[i] && !isn) ? fabsf(v) : 0.f;
+                acc[k] += nonneg_int_float_to_i64(rintf(ldexpf(vz, a.shift)));
+                nanm |= (valid[i] && isn) ? (1u << (slot0 + g0 + k)) : 0u;
             }
         }
-        // inclusive scan within runs of equal superpixel (runs are contiguous: pixels sorted)
+        {
+            const int lr = crun - c_first;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int cu = __shfl_up(c, o);
-            bool same = lane >= o && cu == c;
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                int64_t u = __shfl_up(v[k], o);
-                if (same) v[k] += u;
-            }
-            unsigned int nu = __shfl_up(nanm, o);
-            if (same) nanm |= nu;
+            for (int k = 0; k < P; ++k)
+                if (acc[k]) atomicAdd(&ssum[lr * P + k], (unsigned long long)(neg ? -acc[k] : acc[k]));
+            if (nanm) atomicOr(&snan[lr], nanm);
         }
-        int cd = __shfl_down(c, 1);
-        bool tail = lane == 63 || cd != c;
-        const int lc = c - c_first;
-        if (valid && tail && lc < MAXC) {
-#pragma unroll
-            for (int k = 0; k < NS; ++k)
-                if (v[k]) atomicAdd(&sacc[lc * NS + k], (unsigned long long)v[k]);
-            if (nanm) atomicOr(&snan[lc], nanm);
+        __syncthreads();
+        const int rep = (int)(chunk % a.rrep);
+        for (int i = threadIdx.x; i < nclus * P; i += 256) {
+            unsigned long long sm = ssum[i];
+            if (sm) atomicAdd(&a.acc[((long)(c_first + i / P) * a.rrep + rep) * K + slot0 + g0 + i % P], sm);
         }
+        for (int i = threadIdx.x; i < nclus; i += 256)
+            if (snan[i]) atomicOr(&a.accnan[c_first + i], snan[i]);
+        __syncthreads();
     }
-    __syncthreads();
-    const int rep = (int)(chunk % a.rrep);
-    for (int i = threadIdx.x; i < nclus * NS; i += 256) {
-        unsigned long long sm = sacc[i];
-        int lc = i / NS, k = i % NS;
-        if (sm) atomicAdd(&a.acc[((long)(c_first + lc) * a.rrep + rep) * K + slot0 + k], sm);
-    }
-    for (int i = threadIdx.x; i < nclus; i += 256)
-        if (snan[i]) atomicOr(&a.accnan[c_first + i], snan[i]);
-    __syncthreads();
 }
 
 // ----------------------------------------------------------------------------------------
-// messages for 4 edges (one wavefront): passing_msg_k (hbp_isp.cu:292-304), smooth_msg_k
-// (306-355), normalize_msg_k (273-290).  Lane y < 12 of edge e = (s, t) computes the message
-// into s for s's slot y from t's 8 best + prior particles.  On spatial sweeps whose candidates
-// are resolved in this kernel, s's candidate planes are resolved here too.
+// messages for 5 edges (one wavefront, 12 lanes each): passing_msg_k (hbp_isp.cu:292-304),
+// smooth_msg_k (306-355), normalize_msg_k (273-290).  Lane y of edge e = (s, t) computes the
+// message into s for s's slot y from t's 8 best + prior particles.  On spatial sweeps whose
+// candidates are resolved in this kernel, s's candidate planes are resolved here too.
 // ----------------------------------------------------------------------------------------
+constexpr int EDGES_PER_WAVE = 5;
 __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
     const int lane = threadIdx.x & 63;
-    const long e = wv * 4 + (lane >> 4);
-    const int y = lane & 15;
-    const bool live = e < a.E && y < K;
+    const int grp = lane / K;
+    const long e = wv * EDGES_PER_WAVE + grp;
+    const int y = lane - grp * K;
+    const bool live = grp < EDGES_PER_WAVE && e < a.E;
     float out = 0.f;
     long mslot = 0;
     if (live) {
@@ -735,7 +824,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
             float phi = fabsf(dpx * X[0] + dpy * Y[0] + dpz);
 #pragma unroll
             for (int j = 1; j < 6; ++j) phi = phi + fabsf(dpx * X[j] + dpy * Y[j] + dpz);
-            phi = fminf(phi / 3 / dpos, a.tau_s);
+            phi = fminf(div3_nonneg(phi) / dpos, a.tau_s);
             float v = m + c0 * phi;
             if (v < minimum) minimum = v;
         }
@@ -743,7 +832,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
     }
     // normalize_msg_k: mean over the 12 slots, summed in slot order
     float mean = 0.f;
-    const int g = lane & ~15;
+    const int g = grp * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) mean += __shfl(out, g + k, 64);
     mean /= K;
@@ -910,9 +999,7 @@ __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, 
             const int w = blk * 4 + wave;
             if (w < npc) pc_segment<NS, MODE>(a, w, slot0, spatial != 0);
         } else {
-            constexpr int R = NS == P ? CH_R4 : CH_R12;
-            constexpr int MAXC = 256 * R / 2 + 2;
-            pc_chunk<NS, R, MODE>(a, blk, slot0, lds_pc, (unsigned int *)(lds_pc + MAXC * NS));
+            pc_runs<NS, MODE>(a, blk, slot0, (unsigned char *)lds_pc);
         }
     } else if (NS == P) {
         edge_wave(a, (long)(b - npc_blocks) * 4 + wave, COARSE && spatial != 0);
@@ -935,6 +1022,23 @@ struct BatchItem {
     int sp_begin, sp_blocks;                 // spatial launch (fine levels)
 };
 
+// Every pointer of a SweepArgs, for the relative form of the batch table: the host stores each
+// pointer as its byte offset from the table's own device address and the kernels add the
+// table pointer (a kernel argument) back, so the compiler knows every access is to global
+// memory (a pointer loaded from memory would be generic and compile to flat accesses).
+#define SWEEP_ARGS_POINTERS(X)                                                                   \
+    X(clus) X(bd) X(pxy) X(pcl) X(pix_off) X(seg) X(sego) X(off) X(rev) X(eloc) X(revl) X(pw) X(c0) \
+    X(dpos) X(mxy) X(child) X(child_off) X(plane) X(cost) X(belief) X(point) X(bd_h) X(off_h)         \
+    X(plane_h) X(cl) X(cr) X(dwta) X(msg) X(msg_t) X(acc) X(accnan) X(s_ransac) X(s_neighbor) X(s_child)
+
+__device__ __forceinline__ SweepArgs rebased_args(const SweepArgs *__restrict__ args, int i) {
+    SweepArgs a = args[i];
+#define HBP_REBASE(f) a.f = (decltype(a.f))((const char *)args + (unsigned long long)a.f);
+    SWEEP_ARGS_POINTERS(HBP_REBASE)
+#undef HBP_REBASE
+    return a;
+}
+
 template <int FIELD>
 __device__ __forceinline__ int batch_find(const BatchItem *__restrict__ it, int n, int b) {
     int lo = 0, hi = n - 1;
@@ -947,7 +1051,7 @@ __device__ __forceinline__ int batch_find(const BatchItem *__restrict__ it, int 
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <int MODE>
+template <int MODE, bool COARSE>
 __global__ __launch_bounds__(256) void cost_edge_batch_kernel(const SweepArgs *__restrict__ args,
                                                               const BatchItem *__restrict__ items, int nitems) {
     extern __shared__ unsigned long long lds_pc[];
@@ -955,18 +1059,18 @@ __global__ __launch_bounds__(256) void cost_edge_batch_kernel(const SweepArgs *_
     const BatchItem it = items[batch_find<0>(items, nitems, b)];
     const int local = b - it.blk_begin;
     if (local >= it.pc_blocks + it.edge_blocks) return;   // alignment padding
-    const SweepArgs a = args[it.args];
+    const SweepArgs a = rebased_args(args, it.args);
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (local < it.pc_blocks) {
         const int blk = xcd_contiguous(local, it.pc_blocks);
-        if (it.coarse) {
+        if (COARSE) {
             const int w = blk * 4 + wave;
             if (w < it.npc) pc_segment<P, MODE>(a, w, 2 * P, it.spatial != 0);
         } else {
-            pc_chunk<P, CH_R4, MODE>(a, blk, 2 * P, lds_pc, (unsigned int *)(lds_pc + MAXC4 * P));
+            pc_runs<P, MODE>(a, blk, 2 * P, (unsigned char *)lds_pc);
         }
     } else {
-        edge_wave(a, (long)(local - it.pc_blocks) * 4 + wave, it.coarse && it.spatial);
+        edge_wave(a, (long)(local - it.pc_blocks) * 4 + wave, COARSE && it.spatial);
     }
 }
 __global__ __launch_bounds__(256) void node_batch_kernel(const SweepArgs *__restrict__ args,
@@ -975,7 +1079,7 @@ __global__ __launch_bounds__(256) void node_batch_kernel(const SweepArgs *__rest
     const BatchItem it = items[batch_find<1>(items, nitems, b)];
     const int local = b - it.node_begin;
     if (local >= it.node_blocks) return;
-    const SweepArgs a = args[it.args];
+    const SweepArgs a = rebased_args(args, it.args);
     const long s = ((long)local * 256 + threadIdx.x) >> 4;
     if (s < a.V) node_group(a, s, it.kind, it.next_kind, it.ndz, it.ndn);
 }
@@ -985,7 +1089,7 @@ __global__ __launch_bounds__(256) void spatial_batch_kernel(const SweepArgs *__r
     const BatchItem it = items[batch_find<2>(items, nitems, b)];
     const int local = b - it.sp_begin;
     if (local >= it.sp_blocks) return;
-    const SweepArgs a = args[it.args];
+    const SweepArgs a = rebased_args(args, it.args);
     const long id = (long)local * 256 + threadIdx.x;
     if (id < a.V) spatial_node(a, id);
 }
@@ -1327,7 +1431,7 @@ static void launch_cost_edge_mode(Ctx &c, const SweepArgs &a, int mode, long blo
 template <int NS>
 static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l, int slot0, bool spatial,
                              bool costs, bool with_edges) {
-    long nedge_w = with_edges ? div_up(a.E, 4) : 0;
+    long nedge_w = with_edges ? div_up(a.E, EDGES_PER_WAVE) : 0;
     double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
     const int mode = !a.use_cost_vol ? kCostWta : (a.right ? kCostRight : kCostLeft);
     ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
@@ -1337,8 +1441,7 @@ static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l
         if (blocks == 0) return;
         launch_cost_edge_mode<NS, true>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
     } else {
-        constexpr int R = NS == P ? CH_R4 : CH_R12;
-        int npc = costs ? (int)div_up(a.N, 256 * R) : 0;
+        int npc = costs ? (int)div_up(a.N, CHUNK) : 0;
         long blocks = npc + div_up(nedge_w, 4);
         if (blocks == 0) return;
         launch_cost_edge_mode<NS, false>(c, a, mode, blocks, PC_LDS_BYTES, slot0, 0, npc);
@@ -1425,14 +1528,18 @@ void bp_run_batch(Ctx &c, const std::vector<BP *> &bps) {
     // the items of every launch group
     size_t rounds = 0;
     for (BP *bp : bps) rounds = std::max(rounds, bp->sched.size() - bp->next);
+    // per launch group: the coarse items (segment form, no LDS) first, then the fine ones
+    // (run form); each form is one cost launch with its own block numbering, the node launch
+    // covers both
     struct Group {
-        int first, count;                     // items
-        int cost_blocks, node_blocks, sp_blocks;
+        int first, ncoarse, nfine;
+        int coarse_blocks, fine_blocks, node_blocks, sp_blocks;
     };
     std::vector<Group> groups(rounds);
     std::vector<BatchItem> hitems;
     for (size_t r = 0; r < rounds; ++r) {
-        Group g{(int)hitems.size(), 0, 0, 0, 0};
+        Group g{(int)hitems.size(), 0, 0, 0, 0, 0, 0};
+        std::vector<BatchItem> part[2];
         for (int p = 0; p < n; ++p) {
             BP &b = *bps[p];
             const size_t i = b.next + r;
@@ -1450,24 +1557,36 @@ void bp_run_batch(Ctx &c, const std::vector<BP *> &bps) {
             }
             it.coarse = B.coarse ? 1 : 0;
             it.spatial = sw.kind == kSpatial ? 1 : 0;
-            it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, 256 * CH_R4));
+            it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, CHUNK));
             it.pc_blocks = B.coarse ? div_up(it.npc, 4) : it.npc;
-            it.edge_blocks = div_up(B.E, 16);
-            it.blk_begin = g.cost_blocks;
-            g.cost_blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
-            it.node_begin = g.node_blocks;
+            it.edge_blocks = div_up(div_up(B.E, EDGES_PER_WAVE), 4);
+            int &blocks = B.coarse ? g.coarse_blocks : g.fine_blocks;
+            it.blk_begin = blocks;
+            blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
             it.node_blocks = div_up((long)B.V * 16, 256);
-            g.node_blocks += it.node_blocks;
-            it.sp_begin = g.sp_blocks;
             it.sp_blocks = (!B.coarse && sw.kind == kSpatial && !sw.first) ? div_up(B.V, 256) : 0;
-            g.sp_blocks += it.sp_blocks;
-            hitems.push_back(it);
-            ++g.count;
+            part[B.coarse ? 0 : 1].push_back(it);
         }
+        g.ncoarse = (int)part[0].size();
+        g.nfine = (int)part[1].size();
+        for (int f = 0; f < 2; ++f)
+            for (BatchItem &it : part[f]) {
+                it.node_begin = g.node_blocks;
+                g.node_blocks += it.node_blocks;
+                it.sp_begin = g.sp_blocks;
+                g.sp_blocks += it.sp_blocks;
+                hitems.push_back(it);
+            }
         groups[r] = g;
     }
     DevBuf<SweepArgs> dargs(c.pool, hargs.size());
     DevBuf<BatchItem> ditems(c.pool, std::max<size_t>(hitems.size(), 1));
+    // pointers relative to the table's device address (see rebased_args)
+    for (SweepArgs &a : hargs) {
+#define HBP_RELATIVE(f) a.f = (decltype(a.f))((uintptr_t)a.f - (uintptr_t)dargs.p);
+        SWEEP_ARGS_POINTERS(HBP_RELATIVE)
+#undef HBP_RELATIVE
+    }
     HBP_CHECK(hipMemcpyAsync(dargs.p, hargs.data(), sizeof(SweepArgs) * hargs.size(), hipMemcpyHostToDevice,
                              c.stream));
     HBP_CHECK(hipMemcpyAsync(ditems.p, hitems.data(), sizeof(BatchItem) * hitems.size(), hipMemcpyHostToDevice,
@@ -1480,29 +1599,42 @@ void bp_run_batch(Ctx &c, const std::vector<BP *> &bps) {
             const size_t i = b.next + r;
             if (i < b.sched.size() && b.sched[i].first) {
                 const int l = b.sched[i].level;
-                run_level_init(c, b, hargs[args_base[p] + l], l);
+                run_level_init(c, b, sweep_args(b, l), l);
             }
         }
-        const BatchItem *it = ditems.p + g.first;
+        const BatchItem *itc = ditems.p + g.first, *itf = itc + g.ncoarse;
+        const int count = g.ncoarse + g.nfine;
         if (g.sp_blocks > 0) {
             ProfScope ps(c, "batch_spatial", 0.0);
-            hipLaunchKernelGGL(spatial_batch_kernel, dim3(g.sp_blocks), dim3(256), 0, c.stream, dargs.p, it, g.count);
+            hipLaunchKernelGGL(spatial_batch_kernel, dim3(g.sp_blocks), dim3(256), 0, c.stream, dargs.p, itc, count);
         }
-        if (g.cost_blocks > 0) {
-            ProfScope ps(c, "batch_plane_cost", 0.0);
+        if (g.coarse_blocks > 0) {
+            ProfScope ps(c, "batch_plane_cost_coarse", 0.0);
             if (mode == kCostLeft)
-                hipLaunchKernelGGL(cost_edge_batch_kernel<kCostLeft>, dim3(g.cost_blocks), dim3(256), PC_LDS_BYTES,
-                                   c.stream, dargs.p, it, g.count);
+                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, true>), dim3(g.coarse_blocks), dim3(256), 0,
+                                   c.stream, dargs.p, itc, g.ncoarse);
             else if (mode == kCostRight)
-                hipLaunchKernelGGL(cost_edge_batch_kernel<kCostRight>, dim3(g.cost_blocks), dim3(256), PC_LDS_BYTES,
-                                   c.stream, dargs.p, it, g.count);
+                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, true>), dim3(g.coarse_blocks), dim3(256), 0,
+                                   c.stream, dargs.p, itc, g.ncoarse);
             else
-                hipLaunchKernelGGL(cost_edge_batch_kernel<kCostWta>, dim3(g.cost_blocks), dim3(256), PC_LDS_BYTES,
-                                   c.stream, dargs.p, it, g.count);
+                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, true>), dim3(g.coarse_blocks), dim3(256), 0,
+                                   c.stream, dargs.p, itc, g.ncoarse);
+        }
+        if (g.fine_blocks > 0) {
+            ProfScope ps(c, "batch_plane_cost_fine", 0.0);
+            if (mode == kCostLeft)
+                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, false>), dim3(g.fine_blocks), dim3(256),
+                                   PC_LDS_BYTES, c.stream, dargs.p, itf, g.nfine);
+            else if (mode == kCostRight)
+                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, false>), dim3(g.fine_blocks), dim3(256),
+                                   PC_LDS_BYTES, c.stream, dargs.p, itf, g.nfine);
+            else
+                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), dim3(g.fine_blocks), dim3(256),
+                                   PC_LDS_BYTES, c.stream, dargs.p, itf, g.nfine);
         }
         {
             ProfScope ps(c, "batch_node", 0.0);
-            hipLaunchKernelGGL(node_batch_kernel, dim3(g.node_blocks), dim3(256), 0, c.stream, dargs.p, it, g.count);
+            hipLaunchKernelGGL(node_batch_kernel, dim3(g.node_blocks), dim3(256), 0, c.stream, dargs.p, itc, count);
         }
         HBP_CHECK(hipGetLastError());
     }

@@ -93,6 +93,18 @@ __host__ __device__ inline int floor_to_int_sel(float x) {
     return x != x ? 0 : i;
 }
 
+// rn(x / 3) for x >= +0 from a multiplication by rn(1/3) and one FMA correction.  Checked
+// exhaustively against the IEEE quotient over all 2^32 floats: equal for every finite x
+// except -0 (gives +0); +-inf give NaN.  The caller's x is a sum of absolute values (+0,
+// finite, inf or NaN) and goes through fminf(x / 3 / dpos, tau_s), where inf and NaN both
+// give tau_s, so the result is identical to the division.
+__host__ __device__ inline float div3_nonneg(float x) {
+    const float y = 0.333333343267440796f;
+    const float q0 = x * y;
+    const float r = fmaf(-q0, 3.0f, x);
+    return fmaf(r, y, q0);
+}
+
 // int64 -> float, round to nearest even, from integer operations only.
 __host__ __device__ inline float int64_to_float_rne(int64_t q) {
     if (q == 0) return 0.0f;
