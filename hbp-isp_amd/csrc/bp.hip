@@ -1504,148 +1504,168 @@ int bp_step(Ctx &c, BP &b, int max_sweeps) {
     return done;
 }
 
-// Runs the remaining sweeps of several BP objects together: launch group i carries sweep
-// next+i of every pair that has one.  Per pair the kernels do exactly what the one-pair path
-// does (same work items, same fixed-point sums), so every pair's results are bit-identical to
-// its own run.
+// ----------------------------------------------------------------------------------------
+// Batched sweeps, host side.  A BatchRunner keeps a device table of SweepArgs (one slot of
+// kBatchLevels entries per BP object, pointers stored relative to the table, see
+// rebased_args) and runs rounds: one round is one sweep of every BP object given to it, in
+// one spatial + two plane-cost/message launches (segment form, run form) + one node launch.
+// Per pair the kernels do exactly what the one-pair path does (same work items, same
+// fixed-point sums), so every pair's results are bit-identical to its own run.
+// ----------------------------------------------------------------------------------------
+void BatchRunner::init(Ctx &c, int slots) {
+    nslots = slots;
+    table.alloc(c.pool, (size_t)slots * kBatchLevels * sizeof(SweepArgs));
+    used.assign(slots, 0);
+    mode = -1;
+}
+
+static int cost_mode(const BP &b) { return !b.use_cost_vol ? kCostWta : (b.vol->right ? kCostRight : kCostLeft); }
+
+int BatchRunner::add(Ctx &c, BP &b) {
+    const int m = cost_mode(b);
+    int live = 0;
+    for (int u : used) live += u;
+    HBP_REQUIRE(live == 0 || mode < 0 || m == mode, "a batch must use one cost form (view and use_cost_vol)");
+    mode = m;
+    HBP_REQUIRE(b.L <= kBatchLevels, "pyramid has more levels than a batch slot holds");
+    int slot = -1;
+    for (int i = 0; i < nslots; ++i)
+        if (!used[i]) {
+            slot = i;
+            break;
+        }
+    HBP_REQUIRE(slot >= 0, "no free batch slot");
+    used[slot] = 1;
+    std::vector<SweepArgs> h(kBatchLevels);
+    SweepArgs *dt = (SweepArgs *)table.p;
+    for (int l = b.min_level; l < b.L; ++l) {
+        SweepArgs a = sweep_args(b, l);
+#define HBP_RELATIVE(f) a.f = (decltype(a.f))((uintptr_t)a.f - (uintptr_t)dt);
+        SWEEP_ARGS_POINTERS(HBP_RELATIVE)
+#undef HBP_RELATIVE
+        h[l] = a;
+    }
+    HBP_CHECK(hipMemcpyAsync(dt + (size_t)slot * kBatchLevels, h.data(), sizeof(SweepArgs) * kBatchLevels,
+                             hipMemcpyHostToDevice, c.stream));
+    return slot;
+}
+
+void BatchRunner::remove(int slot) {
+    if (slot >= 0 && slot < nslots) used[slot] = 0;
+}
+
+void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active) {
+    // items: the segment-form ones first, then the run-form ones; each form is one cost launch
+    // with its own block numbering, the node launch covers both
+    std::vector<BatchItem> part[2];
+    int coarse_blocks = 0, fine_blocks = 0, node_blocks = 0, sp_blocks = 0;
+    for (const auto &pr : active) {
+        BP &b = *pr.first;
+        const size_t i = b.next;
+        if (i >= b.sched.size()) continue;
+        const Sweep &sw = b.sched[i];
+        const BPLevel &B = b.lv[sw.level];
+        if (sw.first) run_level_init(c, b, sweep_args(b, sw.level), sw.level);
+        BatchItem it{};
+        it.args = pr.second * kBatchLevels + sw.level;
+        it.kind = sw.kind;
+        it.next_kind = kNone;
+        if (!sw.last && i + 1 < b.sched.size()) {
+            it.next_kind = b.sched[i + 1].kind;
+            it.ndz = b.sched[i + 1].dz;
+            it.ndn = b.sched[i + 1].dn;
+        }
+        it.coarse = B.coarse ? 1 : 0;
+        it.spatial = sw.kind == kSpatial ? 1 : 0;
+        it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, CHUNK));
+        it.pc_blocks = B.coarse ? div_up(it.npc, 4) : it.npc;
+        it.edge_blocks = div_up(div_up(B.E, EDGES_PER_WAVE), 4);
+        int &blocks = B.coarse ? coarse_blocks : fine_blocks;
+        it.blk_begin = blocks;
+        blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
+        it.node_blocks = div_up((long)B.V * 16, 256);
+        it.sp_blocks = (!B.coarse && sw.kind == kSpatial && !sw.first) ? div_up(B.V, 256) : 0;
+        part[B.coarse ? 0 : 1].push_back(it);
+    }
+    const int ncoarse = (int)part[0].size(), nfine = (int)part[1].size(), count = ncoarse + nfine;
+    if (count == 0) return;
+    std::vector<BatchItem> hitems;
+    hitems.reserve(count);
+    for (int f = 0; f < 2; ++f)
+        for (BatchItem &it : part[f]) {
+            it.node_begin = node_blocks;
+            node_blocks += it.node_blocks;
+            it.sp_begin = sp_blocks;
+            sp_blocks += it.sp_blocks;
+            hitems.push_back(it);
+        }
+    if (items.n < (size_t)count) items.alloc(c.pool, std::max<size_t>(count, 64));
+    // pageable source: the copy is complete with respect to hitems when the call returns, and
+    // it is ordered after the previous round's kernels on the stream
+    HBP_CHECK(hipMemcpyAsync(items.p, hitems.data(), sizeof(BatchItem) * count, hipMemcpyHostToDevice, c.stream));
+    const SweepArgs *dargs = (const SweepArgs *)table.p;
+    const BatchItem *itc = items.p, *itf = items.p + ncoarse;
+    if (sp_blocks > 0) {
+        ProfScope ps(c, "batch_spatial", 0.0);
+        hipLaunchKernelGGL(spatial_batch_kernel, dim3(sp_blocks), dim3(256), 0, c.stream, dargs, itc, count);
+    }
+    if (coarse_blocks > 0) {
+        ProfScope ps(c, "batch_plane_cost_coarse", 0.0);
+        if (mode == kCostLeft)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, true>), dim3(coarse_blocks), dim3(256), 0, c.stream,
+                               dargs, itc, ncoarse);
+        else if (mode == kCostRight)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, true>), dim3(coarse_blocks), dim3(256), 0,
+                               c.stream, dargs, itc, ncoarse);
+        else
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, true>), dim3(coarse_blocks), dim3(256), 0, c.stream,
+                               dargs, itc, ncoarse);
+    }
+    if (fine_blocks > 0) {
+        ProfScope ps(c, "batch_plane_cost_fine", 0.0);
+        if (mode == kCostLeft)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, false>), dim3(fine_blocks), dim3(256), PC_LDS_BYTES,
+                               c.stream, dargs, itf, nfine);
+        else if (mode == kCostRight)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, false>), dim3(fine_blocks), dim3(256),
+                               PC_LDS_BYTES, c.stream, dargs, itf, nfine);
+        else
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), dim3(fine_blocks), dim3(256), PC_LDS_BYTES,
+                               c.stream, dargs, itf, nfine);
+    }
+    {
+        ProfScope ps(c, "batch_node", 0.0);
+        hipLaunchKernelGGL(node_batch_kernel, dim3(node_blocks), dim3(256), 0, c.stream, dargs, itc, count);
+    }
+    HBP_CHECK(hipGetLastError());
+    for (const auto &pr : active) {
+        BP &b = *pr.first;
+        if (b.next >= b.sched.size()) continue;
+        b.cur = b.sched[b.next].level;
+        ++b.next;
+        ++b.sweeps;
+        if (b.next >= b.sched.size()) b.cur = b.min_level - 1;
+    }
+}
+
+// Runs the remaining sweeps of several BP objects together (round i carries sweep next+i of
+// every pair that has one), then the map of each.
 void bp_run_batch(Ctx &c, const std::vector<BP *> &bps) {
     const int n = (int)bps.size();
     if (n == 0) return;
-    const BP &b0 = *bps[0];
-    const int mode = !b0.use_cost_vol ? kCostWta : (b0.vol->right ? kCostRight : kCostLeft);
-    for (BP *bp : bps) {
-        const int m = !bp->use_cost_vol ? kCostWta : (bp->vol->right ? kCostRight : kCostLeft);
-        HBP_REQUIRE(m == mode, "a batch must use one cost form (view and use_cost_vol)");
+    for (BP *bp : bps)
+        HBP_REQUIRE(cost_mode(*bp) == cost_mode(*bps[0]), "a batch must use one cost form (view and use_cost_vol)");
+    BatchRunner r;
+    r.init(c, n);
+    std::vector<std::pair<BP *, int>> active;
+    for (BP *bp : bps) active.push_back({bp, r.add(c, *bp)});
+    for (;;) {
+        bool any = false;
+        for (BP *bp : bps) any |= bp->next < bp->sched.size();
+        if (!any) break;
+        r.round(c, active);
     }
-    // SweepArgs of every (pair, level >= min_level)
-    std::vector<SweepArgs> hargs;
-    std::vector<int> args_base(n);
-    for (int p = 0; p < n; ++p) {
-        BP &b = *bps[p];
-        args_base[p] = (int)hargs.size() - b.min_level;
-        for (int l = b.min_level; l < b.L; ++l) hargs.push_back(sweep_args(b, l));
-    }
-    // the items of every launch group
-    size_t rounds = 0;
-    for (BP *bp : bps) rounds = std::max(rounds, bp->sched.size() - bp->next);
-    // per launch group: the coarse items (segment form, no LDS) first, then the fine ones
-    // (run form); each form is one cost launch with its own block numbering, the node launch
-    // covers both
-    struct Group {
-        int first, ncoarse, nfine;
-        int coarse_blocks, fine_blocks, node_blocks, sp_blocks;
-    };
-    std::vector<Group> groups(rounds);
-    std::vector<BatchItem> hitems;
-    for (size_t r = 0; r < rounds; ++r) {
-        Group g{(int)hitems.size(), 0, 0, 0, 0, 0, 0};
-        std::vector<BatchItem> part[2];
-        for (int p = 0; p < n; ++p) {
-            BP &b = *bps[p];
-            const size_t i = b.next + r;
-            if (i >= b.sched.size()) continue;
-            const Sweep &sw = b.sched[i];
-            const BPLevel &B = b.lv[sw.level];
-            BatchItem it{};
-            it.args = args_base[p] + sw.level;
-            it.kind = sw.kind;
-            it.next_kind = kNone;
-            if (!sw.last && i + 1 < b.sched.size()) {
-                it.next_kind = b.sched[i + 1].kind;
-                it.ndz = b.sched[i + 1].dz;
-                it.ndn = b.sched[i + 1].dn;
-            }
-            it.coarse = B.coarse ? 1 : 0;
-            it.spatial = sw.kind == kSpatial ? 1 : 0;
-            it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, CHUNK));
-            it.pc_blocks = B.coarse ? div_up(it.npc, 4) : it.npc;
-            it.edge_blocks = div_up(div_up(B.E, EDGES_PER_WAVE), 4);
-            int &blocks = B.coarse ? g.coarse_blocks : g.fine_blocks;
-            it.blk_begin = blocks;
-            blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
-            it.node_blocks = div_up((long)B.V * 16, 256);
-            it.sp_blocks = (!B.coarse && sw.kind == kSpatial && !sw.first) ? div_up(B.V, 256) : 0;
-            part[B.coarse ? 0 : 1].push_back(it);
-        }
-        g.ncoarse = (int)part[0].size();
-        g.nfine = (int)part[1].size();
-        for (int f = 0; f < 2; ++f)
-            for (BatchItem &it : part[f]) {
-                it.node_begin = g.node_blocks;
-                g.node_blocks += it.node_blocks;
-                it.sp_begin = g.sp_blocks;
-                g.sp_blocks += it.sp_blocks;
-                hitems.push_back(it);
-            }
-        groups[r] = g;
-    }
-    DevBuf<SweepArgs> dargs(c.pool, hargs.size());
-    DevBuf<BatchItem> ditems(c.pool, std::max<size_t>(hitems.size(), 1));
-    // pointers relative to the table's device address (see rebased_args)
-    for (SweepArgs &a : hargs) {
-#define HBP_RELATIVE(f) a.f = (decltype(a.f))((uintptr_t)a.f - (uintptr_t)dargs.p);
-        SWEEP_ARGS_POINTERS(HBP_RELATIVE)
-#undef HBP_RELATIVE
-    }
-    HBP_CHECK(hipMemcpyAsync(dargs.p, hargs.data(), sizeof(SweepArgs) * hargs.size(), hipMemcpyHostToDevice,
-                             c.stream));
-    HBP_CHECK(hipMemcpyAsync(ditems.p, hitems.data(), sizeof(BatchItem) * hitems.size(), hipMemcpyHostToDevice,
-                             c.stream));
-    for (size_t r = 0; r < rounds; ++r) {
-        const Group &g = groups[r];
-        // level inits of the pairs whose sweep starts a level
-        for (int p = 0; p < n; ++p) {
-            BP &b = *bps[p];
-            const size_t i = b.next + r;
-            if (i < b.sched.size() && b.sched[i].first) {
-                const int l = b.sched[i].level;
-                run_level_init(c, b, sweep_args(b, l), l);
-            }
-        }
-        const BatchItem *itc = ditems.p + g.first, *itf = itc + g.ncoarse;
-        const int count = g.ncoarse + g.nfine;
-        if (g.sp_blocks > 0) {
-            ProfScope ps(c, "batch_spatial", 0.0);
-            hipLaunchKernelGGL(spatial_batch_kernel, dim3(g.sp_blocks), dim3(256), 0, c.stream, dargs.p, itc, count);
-        }
-        if (g.coarse_blocks > 0) {
-            ProfScope ps(c, "batch_plane_cost_coarse", 0.0);
-            if (mode == kCostLeft)
-                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, true>), dim3(g.coarse_blocks), dim3(256), 0,
-                                   c.stream, dargs.p, itc, g.ncoarse);
-            else if (mode == kCostRight)
-                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, true>), dim3(g.coarse_blocks), dim3(256), 0,
-                                   c.stream, dargs.p, itc, g.ncoarse);
-            else
-                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, true>), dim3(g.coarse_blocks), dim3(256), 0,
-                                   c.stream, dargs.p, itc, g.ncoarse);
-        }
-        if (g.fine_blocks > 0) {
-            ProfScope ps(c, "batch_plane_cost_fine", 0.0);
-            if (mode == kCostLeft)
-                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, false>), dim3(g.fine_blocks), dim3(256),
-                                   PC_LDS_BYTES, c.stream, dargs.p, itf, g.nfine);
-            else if (mode == kCostRight)
-                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, false>), dim3(g.fine_blocks), dim3(256),
-                                   PC_LDS_BYTES, c.stream, dargs.p, itf, g.nfine);
-            else
-                hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), dim3(g.fine_blocks), dim3(256),
-                                   PC_LDS_BYTES, c.stream, dargs.p, itf, g.nfine);
-        }
-        {
-            ProfScope ps(c, "batch_node", 0.0);
-            hipLaunchKernelGGL(node_batch_kernel, dim3(g.node_blocks), dim3(256), 0, c.stream, dargs.p, itc, count);
-        }
-        HBP_CHECK(hipGetLastError());
-    }
-    for (BP *bp : bps) {
-        BP &b = *bp;
-        b.sweeps += (int)(b.sched.size() - b.next);
-        b.next = b.sched.size();
-        b.cur = b.min_level - 1;
-        bp_map(c, b);
-    }
-    // the tables are read by the launches above
+    for (BP *bp : bps) bp_map(c, *bp);
     HBP_CHECK(hipStreamSynchronize(c.stream));
 }
 

@@ -102,6 +102,22 @@ void bp_setup(Ctx &c, BP &b);
 int bp_step(Ctx &c, BP &b, int max_sweeps);
 void bp_map(Ctx &c, BP &b);
 void bp_run_batch(Ctx &c, const std::vector<BP *> &bps);
+
+// Rounds of batched sweeps over a changing set of BP objects (one sweep of each per round);
+// see bp.hip.  Used by bp_run_batch and by the request server.
+struct SweepArgs;
+struct BatchItem;
+constexpr int kBatchLevels = 24;   // pyramid levels a slot holds (N < 2^24 halves at least per level)
+struct BatchRunner {
+    int nslots = 0, mode = -1;
+    DevBuf<uint8_t> table;         // [slots][kBatchLevels] SweepArgs, pointers relative to the table
+    DevBuf<BatchItem> items;       // one round's items
+    std::vector<int> used;
+    void init(Ctx &c, int slots);
+    int add(Ctx &c, BP &b);        // a free slot holding b's levels (b set up, same cost form)
+    void remove(int slot);
+    void round(Ctx &c, const std::vector<std::pair<BP *, int>> &active);   // (object, slot)
+};
 void bp_get_state(Ctx &c, BP &b, float *planes, float *cost, float *belief, float *msg);
 int bp_total_sweeps(int levels, int min_level, int D);
 

@@ -68,6 +68,11 @@ SIGNATURES = {
     "hbp_bp_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "hbp_bp_run": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hbp_bp_run_batch": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "hbp_server_create": (C.c_int, [_vp, C.c_int, C.c_longlong, C.c_float, C.c_float, _pp]),
+    "hbp_server_destroy": (C.c_int, [_vp]),
+    "hbp_submit": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
+    "hbp_server_step": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hbp_poll": (C.c_int, [_vp, C.c_longlong, _vp, _vp]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
     "hbp_bp_dev_plane": (_vp, [_vp]),
     "hbp_bp_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -301,3 +306,48 @@ def exported_symbols():
     import re
     txt = open(HEADER).read()
     return sorted(set(re.findall(r"\b(hbp_[a-z0-9_]+)\s*\(", txt)))
+
+
+class Server:
+    """Continuous-batching request server on a context (hbp_server_*, hbp_submit, hbp_poll)."""
+
+    def __init__(self, ctx: Ctx, max_active=64, hbm_budget=64 << 30, alpha=1000.1, tau_s=0.6):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.lib.hbp_server_create(ctx.h, max_active, int(hbm_budget), alpha, tau_s, C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def submit(self, left, right, labels, min_level=2):
+        left, right = np.ascontiguousarray(left), np.ascontiguousarray(right)
+        H, W = left.shape[:2]
+        t = C.c_longlong()
+        _check(self.ctx.lib.hbp_submit(self.h, left, right, W, H, labels, min_level, C.byref(t)))
+        self.sizes[t.value] = (H, W)
+        return t.value
+
+    def step(self):
+        a, p = C.c_int(), C.c_int()
+        _check(self.ctx.lib.hbp_server_step(self.h, C.byref(a), C.byref(p)))
+        return a.value, p.value
+
+    def poll(self, ticket):
+        """(disp, plane) when done, else None."""
+        H, W = self.sizes[ticket]
+        disp = np.zeros((H, W), np.float32)
+        plane = np.zeros((H, W, 3), np.float32)
+        r = self.ctx.lib.hbp_poll(self.h, ticket, _np_ptr(disp), _np_ptr(plane))
+        if r == 1:
+            return None
+        if r != 0:
+            raise HbpError(self.ctx.lib.hbp_last_error().decode())
+        del self.sizes[ticket]
+        return disp, plane
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.hbp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

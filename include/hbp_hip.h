@@ -127,6 +127,25 @@ int hbp_bp_free(hbp_ctx *ctx, hbp_bp *bp);
 /* sweeps of the schedule for `levels` pyramid levels, min_level and D labels */
 int hbp_schedule_sweeps(int levels, int min_level, int labels);
 
+/* ---- request server: continuous batching (no counterpart in the reference, which runs one
+ *      pair per call).  One host thread drives a server: hbp_submit copies a pair into the
+ *      queue and returns a ticket; every hbp_server_step admits queued pairs while a batch
+ *      slot is free and their estimated device memory fits hbm_budget_bytes (cost volume,
+ *      pyramid and BP set-up run at admission), runs one BP sweep of every admitted pair in
+ *      shared launches, and maps + copies out the pairs that finished.  hbp_poll returns 0 and
+ *      copies disp [H][W] / plane [H][W][3] (either may be NULL) once the pair is done, 1
+ *      while it is queued or running, < 0 on error (a pair that failed or does not fit).
+ *      Each pair's result equals its own hbp_bp_run with the same parameters (num_nb 4, Census,
+ *      left view, min_level clamped to [1, levels-1]).  Destroy the server before its context. */
+typedef struct hbp_server hbp_server;
+int hbp_server_create(hbp_ctx *ctx, int max_active, long long hbm_budget_bytes, float alpha,
+                      float tau_smooth, hbp_server **out);
+int hbp_server_destroy(hbp_server *srv);
+int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_right, int width,
+               int height, int labels, int min_level, long long *ticket);
+int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending);
+int hbp_poll(hbp_server *srv, long long ticket, float *disp, float *plane3);
+
 /* ---- (d) PostProcess(imL, imR) (post_process.hpp:47-55)
  * left_right_check (post_process.cu:47-110): host buffers; plane_left is filled in place;
  * fail_left / fail_right / fail2_left (u8 [H][W], 255 = failed) may be NULL. */

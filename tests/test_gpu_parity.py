@@ -386,3 +386,64 @@ def test_batch_rejects_mixed_cost_forms(ctx):
     b = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, 16), min_level=2, use_cost_vol=0)
     with pytest.raises(H.HbpError):
         ctx.bp_run_batch([a, b])
+
+
+# ---------------------------------------------------------------- request server
+def _single(ctx, left, right, D, ml):
+    pyr = ctx.pyramid(left)
+    ml = max(1, min(ml, pyr.levels - 1))
+    return ctx.bp(pyr, ctx.cost_volume(left, right, D), min_level=ml).run()
+
+
+def test_server_results_equal_single_runs(ctx):
+    """Pairs of different sizes submitted at different times, at most 3 on the device at once:
+    every result equals the pair's own run (each result the same alone and in the mix)."""
+    say("server: 7 pairs, 3 slots, staggered submits")
+    specs = [(71, 96, 72, 24, 2), (72, 160, 120, 48, 2), (73, 64, 48, 16, 1), (74, 200, 64, 40, 2),
+             (75, 120, 90, 32, 3), (76, 80, 200, 32, 2), (77, 128, 96, 32, 2)]
+    pairs = [(_bp_pair(s, W, H, D), D, ml) for s, W, H, D, ml in specs]
+    srv = H.Server(ctx, max_active=3)
+    tickets = {}
+    steps = 0
+    pending = list(range(len(pairs)))
+    while pending or any(t is not None for t in tickets.values()):
+        # one more submit every 40 steps
+        if pending and steps % 40 == 0:
+            i = pending.pop(0)
+            (left, right), D, ml = pairs[i]
+            tickets[i] = srv.submit(left, right, D, ml)
+        srv.step()
+        steps += 1
+        for i, t in list(tickets.items()):
+            if t is None:
+                continue
+            r = srv.poll(t)
+            if r is not None:
+                (left, right), D, ml = pairs[i]
+                d1, p1 = _single(ctx, left, right, D, ml)
+                assert same_bits(d1, r[0]), f"pair {i}: disparity differs"
+                assert same_bits(p1, r[1]), f"pair {i}: planes differ"
+                tickets[i] = None
+        assert steps < 200000
+    srv.close()
+
+
+def test_server_rejects_what_does_not_fit(ctx):
+    say("server: budget")
+    (left, right) = _bp_pair(81, 96, 72, 24)
+    (l2, r2) = _bp_pair(82, 64, 48, 16)
+    srv = H.Server(ctx, max_active=4, hbm_budget=2 << 20)   # 2 MiB: fits 64x48 only
+    big = srv.submit(left, right, 24)
+    small = srv.submit(l2, r2, 16)
+    res = None
+    for _ in range(100000):
+        srv.step()
+        res = srv.poll(small)
+        if res is not None:
+            break
+    assert res is not None
+    d1, p1 = _single(ctx, l2, r2, 16, 2)
+    assert same_bits(d1, res[0]) and same_bits(p1, res[1])
+    with pytest.raises(H.HbpError):
+        srv.poll(big)
+    srv.close()
