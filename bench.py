@@ -43,6 +43,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=64,
                     help="pairs of the batched-vs-plain comparison (mixed sizes; 0 skips it)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the Poisson arrival window of the served-vs-plain comparison (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=12.0, help="mean arrivals per second")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
     args = ap.parse_args()
@@ -61,6 +64,7 @@ def main():
     left, right, _ = synth.make_pair(args.seed, W, Hh, D, wl["n_planes"])
     # the mixed-size pairs of the batched comparison, generated before the GPU is touched
     batch_pairs = make_batch_pairs(args.batch_pairs, args.seed) if args.batch_pairs > 0 else None
+    serve_pool = make_serve_pool(args.seed) if args.serve_seconds > 0 else None
 
     ctx = H.Ctx(0)
     lib = ctx.lib
@@ -177,6 +181,7 @@ def main():
             traffic = None
 
     batched = batched_vs_plain(ctx, hip, batch_pairs) if batch_pairs else None
+    served = served_vs_plain(ctx, serve_pool, args.serve_rate, args.serve_seconds, args.seed) if serve_pool else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -212,6 +217,7 @@ def main():
         "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
                                for n, v in sorted(pair_stats.items()) if "_L" in n},
         "batched_vs_plain": batched,
+        "served_vs_plain": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -339,6 +345,117 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
             "batched_mpix_disp_per_s": round(mpx / t_batch, 1), "bit_identical": bool(same),
             "timed": "plain: cost volume + pyramid + BP + map per pair in turn; batched: every pair's cost volume "
                      "and pyramid, then hbp_bp_run_batch over all pairs"}
+
+
+SERVE_MIX = ((0.4, 640, 480, 64), (0.4, 1280, 720, 128), (0.2, 1920, 1080, 256))
+
+
+def make_serve_pool(seed, per_class=3):
+    """A few generated pairs per class of the serving mix (requests reuse them in turn)."""
+    import multiprocessing as mp
+    specs = [(seed * 7000 + 100 * c + i, w, h, d) for c, (_, w, h, d) in enumerate(SERVE_MIX) for i in range(per_class)]
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("fork").Pool(workers) as pool:
+        out = pool.map(_gen_pair, specs)
+    return [[p for p in out if p[0][1] == w] for _, w, _, _ in SERVE_MIX]
+
+
+def served_vs_plain(ctx, pool, rate, seconds, seed, min_level=2):
+    """Open-loop Poisson arrivals of the serving mix, through the request server (continuous
+    batching) and, as the baseline, through the plain path one pair after the other in arrival
+    order.  Host images in, disparity and planes back to host.  Latency = completion time -
+    scheduled arrival time."""
+    import ctypes as C
+    import numpy as np
+    from hbp_amd import lib as H
+    from hbp_amd import synth
+    lib = ctx.lib
+    k = synth._key(seed, 91)
+    n_max = int(rate * seconds * 3 + 16)
+    u = synth._u01(k, np.arange(2 * n_max, dtype=np.uint64)).reshape(n_max, 2)
+    arrivals, kinds, t = [], [], 0.0
+    counts = [0, 0, 0]
+    for i in range(n_max):
+        t += -np.log(1.0 - u[i, 0]) / rate
+        if t > seconds:
+            break
+        c = 0 if u[i, 1] < SERVE_MIX[0][0] else (1 if u[i, 1] < SERVE_MIX[0][0] + SERVE_MIX[1][0] else 2)
+        arrivals.append(t)
+        kinds.append((c, counts[c] % len(pool[c])))
+        counts[c] += 1
+    n = len(arrivals)
+
+    def pair(i):
+        c, j = kinds[i]
+        return pool[c][j]
+
+    # served: submit each request at its arrival time, one round per step
+    srv = H.Server(ctx, max_active=64, hbm_budget=96 << 30)
+    done_at = [None] * n
+    tickets = {}
+    outputs = {}
+    nxt = 0
+    t0 = time.perf_counter()
+    while nxt < n or tickets:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            (_, w, h, d), left, right = pair(nxt)
+            tickets[srv.submit(left, right, d, min_level)] = nxt
+            nxt += 1
+        if not tickets:
+            time.sleep(max(0.0, min(0.002, arrivals[nxt] - now)))
+            continue
+        _, _, ndone = srv.step()
+        if ndone:
+            now = time.perf_counter() - t0
+            for tk in list(tickets):
+                r = srv.poll(tk)
+                if r is not None:
+                    i = tickets.pop(tk)
+                    done_at[i] = now
+                    if i < 4:
+                        outputs[i] = r
+    srv.close()
+    lat_s = sorted(d - a for d, a in zip(done_at, arrivals))
+    span_s = max(done_at) - arrivals[0]
+
+    # plain baseline: one pair at a time in arrival order
+    done_p = []
+    ref = {}
+    t0 = time.perf_counter()
+    for i in range(n):
+        now = time.perf_counter() - t0
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        (_, w, h, d), left, right = pair(i)
+        vol, pyr, bp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        H._check(lib.hbp_cost_volume(ctx.h, left, right, w, h, d, b"Census", C.byref(vol)))
+        H._check(lib.hbp_pyramid_build(ctx.h, left, w, h, 2, 4, 0.0, C.byref(pyr)))
+        ml = max(1, min(min_level, lib.hbp_pyramid_levels(pyr) - 1))
+        H._check(lib.hbp_bp_create(ctx.h, pyr, vol, 1000.1, 0.6, ml, 1, C.byref(bp)))
+        disp = np.zeros((h, w), np.float32)
+        plane = np.zeros((h, w, 3), np.float32)
+        H._check(lib.hbp_bp_run(ctx.h, bp, disp.ctypes.data_as(C.c_void_p), plane.ctypes.data_as(C.c_void_p)))
+        lib.hbp_bp_free(ctx.h, bp)
+        lib.hbp_pyramid_free(ctx.h, pyr)
+        lib.hbp_volume_free(ctx.h, vol)
+        done_p.append(time.perf_counter() - t0)
+        if i < 4:
+            ref[i] = (disp, plane)
+    lat_p = sorted(d - a for d, a in zip(done_p, arrivals))
+    span_p = max(done_p) - arrivals[0]
+    same = all(np.array_equal(ref[i][0].view(np.uint32), outputs[i][0].view(np.uint32)) and
+               np.array_equal(ref[i][1].view(np.uint32), outputs[i][1].view(np.uint32)) for i in ref)
+
+    def pct(v, q):
+        return round(1e3 * v[min(len(v) - 1, int(round(q * (len(v) - 1))))], 1)
+
+    return {"workload": f"Poisson arrivals at {rate:g} pairs/s for {seconds:g} s: 40% 640x480/D64, 40% 1280x720/D128, "
+                        f"20% 1920x1080/D256, min_level 2, host images in, disparity + planes out",
+            "requests": n,
+            "served": {"pairs_per_s": round(n / span_s, 3), "p50_ms": pct(lat_s, 0.5), "p99_ms": pct(lat_s, 0.99)},
+            "plain": {"pairs_per_s": round(n / span_p, 3), "p50_ms": pct(lat_p, 0.5), "p99_ms": pct(lat_p, 0.99)},
+            "bit_identical_first_4": bool(same)}
 
 
 def cpu_baseline(left, right, wl, info, sweeps):

@@ -111,7 +111,8 @@ static void fail(Server &s, std::unique_ptr<Request> &rp, const std::string &msg
 }
 
 // admit what fits, one round, complete finished pairs
-static void step(Server &s, int *n_active, int *n_pending) {
+static void step(Server &s, int *n_active, int *n_pending, int *n_done) {
+    int done = 0;
     while (!s.pending.empty() && (int)s.active.size() < s.max_active) {
         std::unique_ptr<Request> &r = s.pending.front();
         if (r->est > s.budget) {
@@ -138,13 +139,18 @@ static void step(Server &s, int *n_active, int *n_pending) {
         s.runner.round(*s.c, pairs);
         std::vector<std::unique_ptr<Request>> still;
         for (auto &r : s.active) {
-            if (r->bp.next >= r->bp.sched.size()) complete(s, r);
-            else still.push_back(std::move(r));
+            if (r->bp.next >= r->bp.sched.size()) {
+                complete(s, r);
+                ++done;
+            } else {
+                still.push_back(std::move(r));
+            }
         }
         s.active.swap(still);
     }
     if (n_active) *n_active = (int)s.active.size();
     if (n_pending) *n_pending = (int)s.pending.size();
+    if (n_done) *n_done = done;
 }
 
 }  // namespace hbp
@@ -204,11 +210,11 @@ int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_righ
     HBP_API_END
 }
 
-int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending) {
+int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending, int *n_done) {
     HBP_API_BEGIN
     HBP_REQUIRE(srv, "NULL argument");
     HBP_CHECK(hipSetDevice(srv->s.c->device));
-    step(srv->s, n_active, n_pending);
+    step(srv->s, n_active, n_pending, n_done);
     HBP_API_END
 }
 

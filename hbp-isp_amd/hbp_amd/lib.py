@@ -71,7 +71,7 @@ SIGNATURES = {
     "hbp_server_create": (C.c_int, [_vp, C.c_int, C.c_longlong, C.c_float, C.c_float, _pp]),
     "hbp_server_destroy": (C.c_int, [_vp]),
     "hbp_submit": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
-    "hbp_server_step": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hbp_server_step": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hbp_poll": (C.c_int, [_vp, C.c_longlong, _vp, _vp]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
     "hbp_bp_dev_plane": (_vp, [_vp]),
@@ -327,9 +327,10 @@ class Server:
         return t.value
 
     def step(self):
-        a, p = C.c_int(), C.c_int()
-        _check(self.ctx.lib.hbp_server_step(self.h, C.byref(a), C.byref(p)))
-        return a.value, p.value
+        """(pairs on the device, queued, completed by this step)"""
+        a, p, d = C.c_int(), C.c_int(), C.c_int()
+        _check(self.ctx.lib.hbp_server_step(self.h, C.byref(a), C.byref(p), C.byref(d)))
+        return a.value, p.value, d.value
 
     def poll(self, ticket):
         """(disp, plane) when done, else None."""

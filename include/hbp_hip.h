@@ -143,7 +143,9 @@ int hbp_server_create(hbp_ctx *ctx, int max_active, long long hbm_budget_bytes, 
 int hbp_server_destroy(hbp_server *srv);
 int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_right, int width,
                int height, int labels, int min_level, long long *ticket);
-int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending);
+/* n_active / n_pending: pairs on the device / queued after the step; n_done: pairs completed
+ * by it (any may be NULL) */
+int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending, int *n_done);
 int hbp_poll(hbp_server *srv, long long ticket, float *disp, float *plane3);
 
 /* ---- (d) PostProcess(imL, imR) (post_process.hpp:47-55)

@@ -405,6 +405,7 @@ def test_server_results_equal_single_runs(ctx):
     srv = H.Server(ctx, max_active=3)
     tickets = {}
     steps = 0
+    n_done = 0
     pending = list(range(len(pairs)))
     while pending or any(t is not None for t in tickets.values()):
         # one more submit every 40 steps
@@ -412,7 +413,9 @@ def test_server_results_equal_single_runs(ctx):
             i = pending.pop(0)
             (left, right), D, ml = pairs[i]
             tickets[i] = srv.submit(left, right, D, ml)
-        srv.step()
+        active, queued, done = srv.step()
+        assert active <= 3 and done >= 0
+        n_done += done
         steps += 1
         for i, t in list(tickets.items()):
             if t is None:
@@ -425,6 +428,7 @@ def test_server_results_equal_single_runs(ctx):
                 assert same_bits(p1, r[1]), f"pair {i}: planes differ"
                 tickets[i] = None
         assert steps < 200000
+    assert n_done == len(pairs)
     srv.close()
 
 
