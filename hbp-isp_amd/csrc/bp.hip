@@ -44,6 +44,18 @@ static inline int acc_replicas(int V) { return V <= 4096 ? 16 : (V <= 65536 ? 4 
 // ----------------------------------------------------------------------------------------
 // prologue kernels
 // ----------------------------------------------------------------------------------------
+// Reversed, padded right census for the left-view plane cost: row y holds CR(y, max(W-1-j, 0))
+// at j = 0 .. S-1, so CR(max(x-d, 0)) and CR(max(x-d-1, 0)) are adjacent words for every
+// 0 <= d <= S-W.
+__global__ void crx_kernel(const uint32_t *__restrict__ cr, int W, int H, int S, uint32_t *__restrict__ out) {
+    const long n = (long)H * S;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(i / S), j = (int)(i % S);
+        const int x = W - 1 - j;
+        out[i] = cr[(long)y * W + (x > 0 ? x : 0)];
+    }
+}
+
 __global__ void iota_u32_kernel(unsigned int *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         p[i] = (unsigned)i;
@@ -287,9 +299,12 @@ struct SweepArgs {
     // census words of the pair [H][W] (the cost volume's entries are recomputed from them),
     // view of the volume, WTA disparities and parameters
     const uint32_t *cl, *cr;
+    const uint32_t *crx;         // reversed, padded right census [H][crx_stride] (left view)
+    int crx_stride;
     int right;
     const float *dwta;
     float lambda, tau, tau_s;
+    float lam_fx;                // |lambda| 2^shift
     int use_cost_vol, shift;
     // state
     float4 *msg;                 // [E][2] float4
@@ -388,184 +403,234 @@ __device__ void spatial_node(const SweepArgs &a, long id) {
 // the WTA indicator of StereoData(use_cost_vol = 0).
 enum CostMode { kCostLeft = 0, kCostRight = 1, kCostWta = 2 };
 
-// Entry (d, y, x) of the cost volume, 0 <= d < D, recomputed from the census words exactly as
-// the Hamming kernel stores it: left view min(popc(CL(x) ^ CR(max(x-d, 0))), 25); right view
-// (cost_to_right) the left entry at x' = min(x+d, W-1).  clx = CL(x) of the pixel's row.
-template <int MODE>
-__device__ __forceinline__ float census_entry(int W, const uint32_t *clrow, const uint32_t *crrow, uint32_t clx,
-                                              int x, int d) {
-    uint32_t l, r;
-    if (MODE == kCostLeft) {
-        int xr = x - d;
-        l = clx;
-        r = crrow[xr < 0 ? 0 : xr];
-    } else {
-        int xl = x + d;
-        xl = xl > W - 1 ? W - 1 : xl;
-        l = clrow[xl];
-        r = crrow[xl - d];
-    }
-    int c = __popc(l ^ r);
-    return (float)(c < 25 ? c : 25);
-}
+// ----------------------------------------------------------------------------------------
+// read_plane_cost (hbp_isp.cu:177-228) of one pixel and plane, times lambda, as the term
+// rint(|lambda * cost| 2^s) of the cluster's fixed-point sum (an integer valued float below
+// 2^52 / N, so per-lane sums of these terms are exact in double).
+//
+// Entry (d, y, x) of the cost volume, 0 <= d < D, is recomputed from the census words exactly
+// as the Hamming kernel stores it: left view min(popc(CL(x) ^ CR(max(x-d, 0))), 25); right
+// view (cost_to_right) the left entry at x' = min(x+d, W-1).  The entries at d = floor(df) and
+// d+1 are read at d clamped into [0, D-2]; the cost is tau when d is outside [0, D-1).
+//
+// Left view: the words come from the reversed right census crx, whose row y holds
+// CR(y, max(W-1-j, 0)) at j = 0 .. S-1, S = W + max(D-1, 1): CR(max(x-d, 0)) and
+// CR(max(x-d-1, 0)) are the adjacent words j = W-1-x+d and j+1, one 8-byte load, no clamp.
+//
+// Two forms of the same arithmetic:
+//  * GEN: any plane.  A NaN disparity counts as d = 0, in range, so its cost is NaN (the
+//    caller flags it and adds zero); lanes past the end of the pixels add zero.
+//  * fast: every plane of the group has |a|, |b|, |c| <= 2^60 (|df| < 2^77: never NaN or
+//    inf), D >= 2 and every lane holds a pixel.  d is in range exactly when floor(df) equals
+//    its clamp into [0, D-2].
+// Both take (d+1) - df as 1 - (df - d): for 0 <= d < 2^23, df - d is exact (Sterbenz) and so
+// is d+1, so both are the one rounding of the same real number (out of range the value is
+// not used).  |lambda * cost| 2^s is computed as (|lambda| 2^s) * cost: scaling by 2^s
+// commutes with rounding while the products are normal floats, and otherwise both forms
+// round to 0 (|lambda| 2^s is a finite normal float, checked on the host).
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ void pin(uint32_t &v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(float &v) { asm volatile("" : "+v"(v)); }
 
-// read_plane_cost (hbp_isp.cu:177-228) of one pixel and plane, times lambda.  Branch-free: the
-// two entries are read at the disparity clamped into [0, D-2] and replaced by tau when the
-// plane's disparity is outside [0, D-1).  dw = WTA disparity of the pixel (kCostWta).
-template <int MODE>
-__device__ __forceinline__ float pixel_plane_cost(const SweepArgs &a, float4 pl, int x, float fx, float fy,
-                                                  const uint32_t *clrow, const uint32_t *crrow, uint32_t clx,
-                                                  float dw) {
-    float df = pl.x * fx + pl.y * fy + pl.z;
-    float cost;
-    if (MODE == kCostWta) {
-        cost = fabsf(dw - df) <= 1.f ? 0.f : 1.f;
-    } else {
-        int d = floor_to_int(df);
-        bool in = d >= 0 && d < a.D - 1;
-        int dc = d < a.D - 2 ? d : a.D - 2;
-        dc = dc < 0 ? 0 : dc;
-        float v0 = census_entry<MODE>(a.W, clrow, crrow, clx, x, dc);
-        float v1 = census_entry<MODE>(a.W, clrow, crrow, clx, x, dc + 1);
-        float ci = ((float)(d + 1) - df) * v0 + (df - (float)d) * v1;
-        cost = in ? ci : a.tau;
-    }
-    return a.lambda * cost;
-}
-
-// per-pixel inputs of pixel_plane_cost
-struct PixelIn {
-    int x;
-    float fx, fy, dw;
-    const uint32_t *clrow, *crrow;
-    uint32_t clx;
+struct __attribute__((aligned(4))) Words2 {
+    uint32_t lo, hi;
 };
+
+__device__ __forceinline__ bool tame_plane(float4 p) {
+    const float lim = 0x1p60f;
+    return fabsf(p.x) <= lim && fabsf(p.y) <= lim && fabsf(p.z) <= lim;
+}
+
+// Loads at a 32-bit byte offset from a kernel-argument base: every image array here is below
+// 2^31 bytes, and the offset form needs no 64-bit address arithmetic per lane.
+template <typename T>
+__device__ __forceinline__ T ld_at(const void *base, uint32_t byte_off) {
+    return *(const T *)((const char *)base + byte_off);
+}
+
+struct PcPix {
+    int x;
+    float fx, fy;
+    uint32_t clx;   // left view: CL(x)
+    uint32_t pb;    // left view: word index of crx (y, W-1-x)
+    uint32_t row;   // right view: y W
+    float dw;       // WTA: WTA disparity
+};
+struct PcWords {
+    uint32_t u0, u1, w0, w1;
+    float tf;   // fast form: df - floor(df) when floor(df) is in [0, D-2], else -1
+};
+
+// x, y < 2^16 and y W + x < 2^24: 24-bit multiplies are exact
 template <int MODE>
-__device__ __forceinline__ PixelIn load_pixel(const SweepArgs &a, uint32_t q) {
-    PixelIn p;
+__device__ __forceinline__ PcPix pc_pixel(const SweepArgs &a, uint32_t q) {
+    PcPix p;
     p.x = unpack_x(q);
-    int y = unpack_y(q);
+    const uint32_t y = (uint32_t)unpack_y(q);
     p.fx = (float)p.x;
     p.fy = (float)y;
-    p.clrow = a.cl + (long)y * a.W;
-    p.crrow = a.cr + (long)y * a.W;
-    p.clx = MODE == kCostLeft ? p.clrow[p.x] : 0u;
-    p.dw = MODE == kCostWta ? a.dwta[(long)y * a.W + p.x] : 0.f;
+    p.row = __umul24(y, (uint32_t)a.W);
+    if (MODE == kCostLeft) {
+        p.clx = ld_at<uint32_t>(a.cl, (p.row + (uint32_t)p.x) << 2);
+        p.pb = __umul24(y, (uint32_t)a.crx_stride) + (uint32_t)(a.W - 1 - p.x);
+    }
+    if (MODE == kCostWta) p.dw = ld_at<float>(a.dwta, (p.row + (uint32_t)p.x) << 2);
     return p;
+}
+template <int MODE>
+__device__ __forceinline__ void pc_pin(PcPix &p) {
+    if (MODE == kCostLeft) pin(p.clx);
+    if (MODE == kCostWta) pin(p.dw);
+}
+
+// issue the census loads of one pixel and plane (the disparity clamped into [0, D-2])
+template <int MODE, bool GEN>
+__device__ __forceinline__ void pc_load(const SweepArgs &a, const PcPix &px, float4 pl, float dmax, PcWords &wd) {
+    if (MODE == kCostWta) return;
+    const float df = pl.x * px.fx + pl.y * px.fy + pl.z;
+    const float fl = floorf(df);
+    // GEN: NaN -> 0, +-inf -> the ends
+    const float dcf = GEN ? fminf(fmaxf(fl, 0.f), dmax) : __builtin_amdgcn_fmed3f(fl, 0.f, dmax);
+    const int dc = (int)dcf;
+    if (!GEN) wd.tf = fl == dcf ? df - fl : -1.f;
+    if (MODE == kCostLeft) {
+        const Words2 w = ld_at<Words2>(a.crx, (px.pb + (uint32_t)dc) << 2);
+        wd.u0 = w.lo;
+        wd.u1 = w.hi;
+    } else {
+        const int xa = px.x + dc < a.W - 1 ? px.x + dc : a.W - 1;
+        const int xb = px.x + dc + 1 < a.W - 1 ? px.x + dc + 1 : a.W - 1;
+        wd.u0 = ld_at<uint32_t>(a.cl, (px.row + (uint32_t)xa) << 2);
+        wd.w0 = ld_at<uint32_t>(a.cr, (px.row + (uint32_t)(xa - dc)) << 2);
+        wd.u1 = ld_at<uint32_t>(a.cl, (px.row + (uint32_t)xb) << 2);
+        wd.w1 = ld_at<uint32_t>(a.cr, (px.row + (uint32_t)(xb - dc - 1)) << 2);
+    }
+}
+template <int MODE, bool GEN>
+__device__ __forceinline__ void pc_pin(PcWords &w) {
+    if (MODE == kCostWta) return;
+    if (!GEN) pin(w.tf);
+    pin(w.u0);
+    pin(w.u1);
+    if (MODE == kCostRight) {
+        pin(w.w0);
+        pin(w.w1);
+    }
+}
+
+// the term of one pixel and plane from its loaded words, lam = |lambda| 2^s (0 for a lane
+// without a pixel); isn: the cost is NaN
+template <int MODE, bool GEN>
+__device__ __forceinline__ float pc_term(const SweepArgs &a, const PcPix &px, float4 pl, const PcWords &wd,
+                                         float dmax, float dlim, float lam, bool &isn) {
+    float cost;
+    if (MODE == kCostWta) {
+        const float df = pl.x * px.fx + pl.y * px.fy + pl.z;
+        cost = fabsf(px.dw - df) <= 1.f ? 0.f : 1.f;
+    } else if (!GEN) {
+        const bool in = wd.tf >= 0.f;
+        int c0, c1;
+        if (MODE == kCostLeft) {
+            c0 = __popc(px.clx ^ wd.u0);
+            c1 = __popc(px.clx ^ wd.u1);
+        } else {
+            c0 = __popc(wd.u0 ^ wd.w0);
+            c1 = __popc(wd.u1 ^ wd.w1);
+        }
+        const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
+        const float t1 = wd.tf;
+        float ci = (1.f - t1) * v0 + t1 * v1;
+        pin(ci);   // a select, not a branch around the arithmetic
+        cost = in ? ci : a.tau;
+    } else {
+        const float df = pl.x * px.fx + pl.y * px.fy + pl.z;
+        const float fl = floorf(df);
+        bool in;
+        if (GEN) in = a.D > 1 && !(fl < 0.f) && !(fl >= dlim);
+        else in = fl == __builtin_amdgcn_fmed3f(fl, 0.f, dmax);
+        int c0, c1;
+        if (MODE == kCostLeft) {
+            c0 = __popc(px.clx ^ wd.u0);
+            c1 = __popc(px.clx ^ wd.u1);
+        } else {
+            c0 = __popc(wd.u0 ^ wd.w0);
+            c1 = __popc(wd.u1 ^ wd.w1);
+        }
+        const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
+        const float t1 = df - fl;
+        float ci = (1.f - t1) * v0 + t1 * v1;
+        pin(ci);   // a select, not a branch around the arithmetic
+        cost = in ? ci : a.tau;
+    }
+    const float v = lam * cost;
+    isn = v != v;
+    return rintf(v);
 }
 
 // ----------------------------------------------------------------------------------------
 // plane cost, coarse levels: one wavefront per segment (<= SEG pixels of one superpixel).
 // Lane l reads pixels first + l + 64 r; the planes are wave-uniform.  A batch of B steps first
-// issues the B packed positions, then every census word the B x 4 plane costs need (at the
-// disparity clamped into [0, D-2]), pins them with an empty asm so that no load is deferred
-// into a branch, and only then computes: two memory round trips per batch.  The per-lane
-// int64 sums are reduced across the wavefront and added to replica (segment mod rrep) of the
-// superpixel's sums.  Per-pixel values are finite or NaN (the shift keeps |v 2^s| far below
-// 2^63), so the fixed-point term comes from the float's bits.
+// issues the B packed positions, then every census word the B x P plane costs need, pins them
+// with an empty asm so that no load is deferred into a branch, and only then computes: two
+// memory round trips per batch.  The per-lane sums (exact doubles) are reduced across the
+// wavefront and added to replica (segment mod rrep) of the superpixel's int64 sums.
 // ----------------------------------------------------------------------------------------
-__device__ __forceinline__ void pin(uint32_t &v) { asm volatile("" : "+v"(v)); }
-__device__ __forceinline__ void pin(float &v) { asm volatile("" : "+v"(v)); }
-
-template <int MODE, int B>
+template <int MODE, int B, bool GEN>
 __device__ __forceinline__ void pc_batch(const SweepArgs &a, const float4 (&pl)[P], int slot0, int first, int end,
-                                         int r0, int64_t (&acc)[P], unsigned int &nanm) {
+                                         int r0, double (&acc)[P], unsigned int &nanm) {
     const int lane = threadIdx.x & 63;
     const float dmax = (float)(a.D > 2 ? a.D - 2 : 0);
-    uint32_t q[B];
+    const float dlim = (float)(a.D - 1);
+    PcPix px[B];
+    PcWords wd[B][P];
 #pragma unroll
     for (int r = 0; r < B; ++r) {
         const int j = first + (r0 + r) * 64 + lane;
-        q[r] = a.pxy[j < end ? j : end - 1];
-    }
-    uint32_t u0[B][P], u1[B][P], w0[B][P], w1[B][P], clx[B];
-    float dw[B];
+        px[r] = pc_pixel<MODE>(a, ld_at<uint32_t>(a.pxy, (uint32_t)(GEN && j >= end ? end - 1 : j) << 2));
 #pragma unroll
-    for (int r = 0; r < B; ++r) {
-        const int x = unpack_x(q[r]), y = unpack_y(q[r]);
-        const float fx = (float)x, fy = (float)y;
-        const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
-        if (MODE == kCostLeft) clx[r] = clrow[x];
-        if (MODE == kCostWta) dw[r] = a.dwta[(long)y * a.W + x];
-        if (MODE != kCostWta) {
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const float df = pl[k].x * fx + pl[k].y * fy + pl[k].z;
-                // disparity clamped into [0, D-2] (NaN -> 0)
-                const int dc = (int)fminf(fmaxf(floorf(df), 0.f), dmax);
-                if (MODE == kCostLeft) {
-                    // CR(max(x-dc, 0)) and CR(max(x-dc-1, 0)) as base[0], base[-1] from base = max(x-dc, 1)
-                    const int xa = x - dc;
-                    const uint32_t *base = crrow + (xa > 1 ? xa : 1);
-                    const uint32_t hi = base[0], lo = base[-1];
-                    u0[r][k] = xa > 0 ? hi : lo;
-                    u1[r][k] = lo;
-                } else {
-                    const int xa = x + dc < a.W - 1 ? x + dc : a.W - 1;
-                    const int xb = x + dc + 1 < a.W - 1 ? x + dc + 1 : a.W - 1;
-                    u0[r][k] = clrow[xa];
-                    w0[r][k] = crrow[xa - dc];
-                    u1[r][k] = clrow[xb];
-                    w1[r][k] = crrow[xb - dc - 1];
-                }
-            }
-        }
+        for (int k = 0; k < P; ++k) pc_load<MODE, GEN>(a, px[r], pl[k], dmax, wd[r][k]);
     }
 #pragma unroll
     for (int r = 0; r < B; ++r) {
-        if (MODE == kCostLeft) pin(clx[r]);
-        if (MODE == kCostWta) pin(dw[r]);
-        if (MODE != kCostWta) {
+        pc_pin<MODE>(px[r]);
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-                pin(u0[r][k]);
-                pin(u1[r][k]);
-                if (MODE == kCostRight) {
-                    pin(w0[r][k]);
-                    pin(w1[r][k]);
-                }
-            }
-        }
+        for (int k = 0; k < P; ++k) pc_pin<MODE, GEN>(wd[r][k]);
     }
-    const float dlim = (float)(a.D - 1);
-    const bool d_ok = a.D > 1;
 #pragma unroll
     for (int r = 0; r < B; ++r) {
-        const bool valid = first + (r0 + r) * 64 + lane < end;
-        const float fx = (float)unpack_x(q[r]), fy = (float)unpack_y(q[r]);
+        const bool valid = !GEN || first + (r0 + r) * 64 + lane < end;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            const float df = pl[k].x * fx + pl[k].y * fy + pl[k].z;
-            float cost;
-            if (MODE == kCostWta) {
-                cost = fabsf(dw[r] - df) <= 1.f ? 0.f : 1.f;
-            } else {
-                // d = floor(df) in range [0, D-1) (NaN counts as d = 0, in range); the weights
-                // (d+1) - df and df - d are exact float forms of the integer d
-                const float fl = floorf(df);
-                const bool in = d_ok && !(fl < 0.f) && !(fl >= dlim);
-                int c0, c1;
-                if (MODE == kCostLeft) {
-                    c0 = __popc(clx[r] ^ u0[r][k]);
-                    c1 = __popc(clx[r] ^ u1[r][k]);
-                } else {
-                    c0 = __popc(u0[r][k] ^ w0[r][k]);
-                    c1 = __popc(u1[r][k] ^ w1[r][k]);
-                }
-                const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
-                const float ci = ((fl + 1.f) - df) * v0 + (df - fl) * v1;
-                cost = in ? ci : a.tau;
+            bool isn;
+            float t = pc_term<MODE, GEN>(a, px[r], pl[k], wd[r][k], dmax, dlim, a.lam_fx, isn);
+            if (GEN) {
+                t = (valid && !isn) ? t : 0.f;
+                nanm |= (valid && isn) ? (1u << (slot0 + k)) : 0u;
             }
-            const float v = a.lambda * cost;
-            const bool isn = v != v;
-            // every cost is >= 0, so every term has lambda's sign: the lanes add |q| (rint is
-            // odd) and the segment's sum takes lambda's sign at the end.  Invalid lanes and NaN
-            // terms add zero; NaN sets the slot's flag.
-            const float vz = (valid && !isn) ? fabsf(v) : 0.f;
-            acc[k] += nonneg_int_float_to_i64(rintf(ldexpf(vz, a.shift)));
-            nanm |= (valid && isn) ? (1u << (slot0 + k)) : 0u;
+            acc[k] += (double)t;
         }
     }
+}
+
+// exact sum over the wavefront of integer valued doubles below 2^53 (any order is exact):
+// xor 1 and xor 2 within quads, rotations by 4 and 8 within rows of 16 (every lane then holds
+// its row's sum), then the four rows
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true),
+                            __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double wave_sum_exact(double v) {
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x124>(v);   // row_ror 4
+    v += dpp_f64<0x128>(v);   // row_ror 8
+    double r = 0.0;
+#pragma unroll
+    for (int row = 0; row < 4; ++row)
+        r += __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * row),
+                              __builtin_amdgcn_readlane(__double2loint(v), 16 * row));
+    return r;
 }
 
 template <int NS, int MODE>
@@ -577,6 +642,7 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
     const int c = __builtin_amdgcn_readfirstlane(sg.z);
     const int first = __builtin_amdgcn_readfirstlane(sg.x), end = __builtin_amdgcn_readfirstlane(sg.y);
     const int steps = (end - first + 63) >> 6;
+    const int full = (end - first) >> 6;
     unsigned int nanm = 0u;
     unsigned long long mine = 0ull;
 #pragma unroll 1
@@ -593,23 +659,29 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 #pragma unroll
             for (int k = 0; k < P; ++k) pl[k] = a.plane[(long)c * K + slot0 + g0 + k];
         }
-        int64_t acc[P];
+        bool tame = a.D >= 2;
 #pragma unroll
-        for (int k = 0; k < P; ++k) acc[k] = 0;
-        // batches of 4 steps, then the remainder as one batch of 1 or 2 (+1) steps: no step
-        // past the segment's end is computed
+        for (int k = 0; k < P; ++k) tame = tame && tame_plane(pl[k]);
+        const bool fast = __builtin_amdgcn_readfirstlane((int)tame) != 0;
+        double acc[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc[k] = 0.0;
+        // full steps in fast batches of 4, then batches of 4, 2 and 1 in the general form: no
+        // step past the segment's end is computed
         int r0 = 0;
-        for (; r0 + 4 <= steps; r0 += 4) pc_batch<MODE, 4>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+        if (fast)
+            for (; r0 + 4 <= full; r0 += 4) pc_batch<MODE, 4, false>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+        for (; r0 + 4 <= steps; r0 += 4) pc_batch<MODE, 4, true>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
         if (steps - r0 >= 2) {
-            pc_batch<MODE, 2>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+            pc_batch<MODE, 2, true>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
             r0 += 2;
         }
-        if (steps - r0 == 1) pc_batch<MODE, 1>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
+        if (steps - r0 == 1) pc_batch<MODE, 1, true>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
-            if (lane == g0 + k) mine = (unsigned long long)(a.lambda < 0.f ? -acc[k] : acc[k]);
+            // every cost is >= 0, so the sum takes lambda's sign
+            const unsigned long long m = (unsigned long long)wave_sum_exact(acc[k]);
+            if (lane == g0 + k) mine = a.lambda < 0.f ? 0ull - m : m;
         }
     }
 #pragma unroll
@@ -623,11 +695,73 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 // plane cost, fine levels: one 256-thread block per chunk of CHUNK consecutive sorted pixels,
 // lane l taking pixels base + 4l .. 4l+3 (16-byte loads of positions and superpixel ids).
 // The candidate planes of the chunk's superpixels are staged in LDS; a lane issues the census
-// words of its 4 pixels x 4 planes before using any, sums its terms per superpixel run and
+// words of its 4 pixels x P planes before using any, sums its terms per superpixel run and
 // adds each run into LDS int64 counters indexed by superpixel - first superpixel of the chunk;
 // the block then adds each counter to replica (chunk mod rrep) of the global sum.  NS = 12
-// runs three groups of 4 planes.
+// runs three groups of P planes.
 // ----------------------------------------------------------------------------------------
+template <int MODE, bool GEN>
+__device__ __forceinline__ void pc_run_group(const SweepArgs &a, const uint32_t (&q)[4], const uint32_t (&cl)[4],
+                                             const bool (&valid)[4], int c_first, int slot, const float4 *splane,
+                                             unsigned long long *ssum, unsigned int *snan) {
+    const float dmax = (float)(a.D > 2 ? a.D - 2 : 0);
+    const float dlim = (float)(a.D - 1);
+    const bool neg = a.lambda < 0.f;
+    PcPix px[4];
+    PcWords wd[4][P];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        px[i] = pc_pixel<MODE>(a, q[i]);
+        const int lc = (int)cl[i] - c_first;
+#pragma unroll
+        for (int k = 0; k < P; ++k) pc_load<MODE, GEN>(a, px[i], splane[lc * P + k], dmax, wd[i][k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        pc_pin<MODE>(px[i]);
+#pragma unroll
+        for (int k = 0; k < P; ++k) pc_pin<MODE, GEN>(wd[i][k]);
+    }
+    // per-run sums (|terms|; lambda's sign applied at the flush) and NaN bits
+    double acc[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) acc[k] = 0.0;
+    unsigned int nanm = 0u;
+    int crun = (int)cl[0];
+    auto flush = [&](int run) {
+        const int lr = run - c_first;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (acc[k] != 0.0) {
+                const unsigned long long m = (unsigned long long)acc[k];
+                atomicAdd(&ssum[lr * P + k], neg ? 0ull - m : m);
+            }
+            acc[k] = 0.0;
+        }
+        if (nanm) atomicOr(&snan[lr], nanm);
+        nanm = 0u;
+    };
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (valid[i] && (int)cl[i] != crun) {
+            flush(crun);
+            crun = (int)cl[i];
+        }
+        const int lc = (int)cl[i] - c_first;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            bool isn;
+            float t = pc_term<MODE, GEN>(a, px[i], splane[lc * P + k], wd[i][k], dmax, dlim, a.lam_fx, isn);
+            if (GEN) {
+                t = (valid[i] && !isn) ? t : 0.f;
+                nanm |= (valid[i] && isn) ? (1u << (slot + k)) : 0u;
+            }
+            acc[k] += (double)t;
+        }
+    }
+    flush(crun);
+}
+
 template <int NS, int MODE>
 __device__ void pc_runs(const SweepArgs &a, long chunk, int slot0, unsigned char *lds) {
     float4 *splane = (float4 *)lds;                                       // [MAXC][P]
@@ -635,13 +769,10 @@ __device__ void pc_runs(const SweepArgs &a, long chunk, int slot0, unsigned char
     unsigned int *snan = (unsigned int *)(lds + (size_t)MAXC * P * 24);   // [MAXC]
     const long base = chunk * CHUNK;
     const long last = (base + CHUNK < a.N ? base + CHUNK : a.N) - 1;
+    const bool whole = base + CHUNK <= a.N;
     const int c_first = (int)a.pcl[base];
     const int nclus = (int)a.pcl[last] - c_first + 1;
     const long j0 = base + 4L * threadIdx.x;
-    const float dmax = (float)(a.D > 2 ? a.D - 2 : 0);
-    const float dlim = (float)(a.D - 1);
-    const bool d_ok = a.D > 1;
-    const bool neg = a.lambda < 0.f;
     // this lane's pixels
     uint32_t q[4], cl[4];
     bool valid[4];
@@ -653,118 +784,17 @@ __device__ void pc_runs(const SweepArgs &a, long chunk, int slot0, unsigned char
         cl[i] = a.pcl[j];
     }
     for (int g0 = 0; g0 < NS; g0 += P) {
+        int wild = 0;
         for (int i = threadIdx.x; i < nclus * P; i += 256) {
-            splane[i] = a.plane[(long)(c_first + i / P) * K + slot0 + g0 + i % P];
+            const float4 p = a.plane[(long)(c_first + i / P) * K + slot0 + g0 + i % P];
+            splane[i] = p;
             ssum[i] = 0ull;
+            wild |= tame_plane(p) ? 0 : 1;
         }
         for (int i = threadIdx.x; i < nclus; i += 256) snan[i] = 0u;
-        __syncthreads();
-        uint32_t u0[4][P], u1[4][P], w0[4][P], w1[4][P], clx[4];
-        float dw[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = unpack_x(q[i]), y = unpack_y(q[i]);
-            const float fx = (float)x, fy = (float)y;
-            const uint32_t *clrow = a.cl + (long)y * a.W, *crrow = a.cr + (long)y * a.W;
-            const int lc = (int)cl[i] - c_first;
-            if (MODE == kCostLeft) clx[i] = clrow[x];
-            if (MODE == kCostWta) dw[i] = a.dwta[(long)y * a.W + x];
-            if (MODE != kCostWta) {
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    const float4 pl = splane[lc * P + k];
-                    const float df = pl.x * fx + pl.y * fy + pl.z;
-                    const int dc = (int)fminf(fmaxf(floorf(df), 0.f), dmax);
-                    if (MODE == kCostLeft) {
-                        const int xa = x - dc;
-                        const uint32_t *bp = crrow + (xa > 1 ? xa : 1);
-                        const uint32_t hi = bp[0], lo = bp[-1];
-                        u0[i][k] = xa > 0 ? hi : lo;
-                        u1[i][k] = lo;
-                    } else {
-                        const int xa = x + dc < a.W - 1 ? x + dc : a.W - 1;
-                        const int xb = x + dc + 1 < a.W - 1 ? x + dc + 1 : a.W - 1;
-                        u0[i][k] = clrow[xa];
-                        w0[i][k] = crrow[xa - dc];
-                        u1[i][k] = clrow[xb];
-                        w1[i][k] = crrow[xb - dc - 1];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (MODE == kCostLeft) pin(clx[i]);
-            if (MODE == kCostWta) pin(dw[i]);
-            if (MODE != kCostWta) {
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    pin(u0[i][k]);
-                    pin(u1[i][k]);
-                    if (MODE == kCostRight) {
-                        pin(w0[i][k]);
-                        pin(w1[i][k]);
-                    }
-                }
-            }
-        }
-        // per-run sums (|q|; lambda's sign applied at the flush) and NaN bits
-        int64_t acc[P];
-#pragma unroll
-        for (int k = 0; k < P; ++k) acc[k] = 0;
-        unsigned int nanm = 0u;
-        int crun = (int)cl[0];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (valid[i] && (int)cl[i] != crun) {
-                const int lr = crun - c_first;
-#pragma unroll
-                for (int k = 0; k < P; ++k)
-                    if (acc[k]) atomicAdd(&ssum[lr * P + k], (unsigned long long)(neg ? -acc[k] : acc[k]));
-                if (nanm) atomicOr(&snan[lr], nanm);
-#pragma unroll
-                for (int k = 0; k < P; ++k) acc[k] = 0;
-                nanm = 0u;
-                crun = (int)cl[i];
-            }
-            const int lc = (int)cl[i] - c_first;
-            const float fx = (float)unpack_x(q[i]), fy = (float)unpack_y(q[i]);
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const float4 pl = splane[lc * P + k];
-                const float df = pl.x * fx + pl.y * fy + pl.z;
-                float cost;
-                if (MODE == kCostWta) {
-                    cost = fabsf(dw[i] - df) <= 1.f ? 0.f : 1.f;
-                } else {
-                    const float fl = floorf(df);
-                    const bool in = d_ok && !(fl < 0.f) && !(fl >= dlim);
-                    int c0, c1;
-                    if (MODE == kCostLeft) {
-                        c0 = __popc(clx[i] ^ u0[i][k]);
-                        c1 = __popc(clx[i] ^ u1[i][k]);
-                    } else {
-                        c0 = __popc(u0[i][k] ^ w0[i][k]);
-                        c1 = __popc(u1[i][k] ^ w1[i][k]);
-                    }
-                    const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
-                    const float ci = ((fl + 1.f) - df) * v0 + (df - fl) * v1;
-                    cost = in ? ci : a.tau;
-                }
-                const float v = a.lambda * cost;
-                const bool isn = v != v;
-                const float vz = (valid[i] && !isn) ? fabsf(v) : 0.f;
-                acc[k] += nonneg_int_float_to_i64(rintf(ldexpf(vz, a.shift)));
-                nanm |= (valid[i] && isn) ? (1u << (slot0 + g0 + k)) : 0u;
-            }
-        }
-        {
-            const int lr = crun - c_first;
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-                if (acc[k]) atomicAdd(&ssum[lr * P + k], (unsigned long long)(neg ? -acc[k] : acc[k]));
-            if (nanm) atomicOr(&snan[lr], nanm);
-        }
+        const bool fast = __syncthreads_or(wild) == 0 && whole && a.D >= 2;
+        if (fast) pc_run_group<MODE, false>(a, q, cl, valid, c_first, slot0 + g0, splane, ssum, snan);
+        else pc_run_group<MODE, true>(a, q, cl, valid, c_first, slot0 + g0, splane, ssum, snan);
         __syncthreads();
         const int rep = (int)(chunk % a.rrep);
         for (int i = threadIdx.x; i < nclus * P; i += 256) {
@@ -986,6 +1016,13 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n) {
     return x * q + (x < r ? x : r) + b / X;
 }
 
+// the sweep's messages alone, 4 waves of EDGES_PER_WAVE edges per block (fine levels: its
+// register budget is the message code's alone, for more resident waves)
+__global__ __launch_bounds__(256) void edge_kernel(SweepArgs a) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    edge_wave(a, (long)blockIdx.x * 4 + wave, false);
+}
+
 template <int NS, bool COARSE, int MODE>
 __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, int spatial, int npc) {
     extern __shared__ unsigned long long lds_pc[];
@@ -1029,7 +1066,7 @@ struct BatchItem {
 #define SWEEP_ARGS_POINTERS(X)                                                                   \
     X(clus) X(bd) X(pxy) X(pcl) X(pix_off) X(seg) X(sego) X(off) X(rev) X(eloc) X(revl) X(pw) X(c0) \
     X(dpos) X(mxy) X(child) X(child_off) X(plane) X(cost) X(belief) X(point) X(bd_h) X(off_h)         \
-    X(plane_h) X(cl) X(cr) X(dwta) X(msg) X(msg_t) X(acc) X(accnan) X(s_ransac) X(s_neighbor) X(s_child)
+    X(plane_h) X(cl) X(cr) X(crx) X(dwta) X(msg) X(msg_t) X(acc) X(accnan) X(s_ransac) X(s_neighbor) X(s_child)
 
 __device__ __forceinline__ SweepArgs rebased_args(const SweepArgs *__restrict__ args, int i) {
     SweepArgs a = args[i];
@@ -1173,7 +1210,7 @@ int fixed_point_shift(float lambda, float tau, int use_cost_vol, long npix) {
     if (vmax > 0.0) std::frexp(vmax, &ev);
     int en = 0;
     std::frexp((double)(npix > 0 ? npix : 1), &en);
-    int s = 62 - ev - en;
+    int s = 52 - ev - en;
     if (s > 120) s = 120;
     if (s < -60) s = -60;
     return s;
@@ -1245,6 +1282,8 @@ static SweepArgs sweep_args(BP &b, int l) {
     }
     a.cl = v.cl.p;
     a.cr = v.cr.p;
+    a.crx = b.crx.p;
+    a.crx_stride = b.crx_stride;
     a.right = v.right ? 1 : 0;
     a.dwta = v.dwta.p;
     a.lambda = b.lambda;
@@ -1252,6 +1291,7 @@ static SweepArgs sweep_args(BP &b, int l) {
     a.tau_s = b.tau_s;
     a.use_cost_vol = b.use_cost_vol;
     a.shift = b.fx_shift;
+    a.lam_fx = std::ldexp(std::fabs(b.lambda), b.fx_shift);
     a.msg = b.msg.p;
     a.msg_t = b.msg_t.p;
     a.acc = b.acc.p;
@@ -1274,7 +1314,20 @@ void bp_setup(Ctx &c, BP &b) {
     b.L = p.levels();
     b.tau = v.tau;
     HBP_REQUIRE(b.min_level >= 1 && b.min_level <= b.L - 1, "min_level must be in [1, levels-1]");
+    HBP_REQUIRE(std::isfinite(b.lambda) && std::isfinite(b.tau_s), "alpha and tau_smooth must be finite");
     b.fx_shift = fixed_point_shift(b.lambda, b.tau, b.use_cost_vol, b.N);
+    HBP_REQUIRE(b.fx_shift > -60, "alpha too large for the fixed-point plane-cost sums");
+    {
+        const float lf = std::ldexp(std::fabs(b.lambda), b.fx_shift);
+        HBP_REQUIRE(std::isfinite(lf) && (lf == 0.f || std::isnormal(lf)),
+                    "alpha and the cost truncation give no usable fixed-point scale");
+    }
+    // reversed, padded right census of the left-view plane cost
+    b.crx_stride = b.W + std::max(b.D - 1, 1);
+    b.crx.alloc(c.pool, (size_t)b.H * b.crx_stride);
+    hipLaunchKernelGGL(crx_kernel, dim3(grid_for(c, (long)b.H * b.crx_stride)), dim3(256), 0, c.stream, v.cr.p, b.W,
+                       b.H, b.crx_stride, b.crx.p);
+    HBP_CHECK(hipGetLastError());
     b.lv.clear();
     b.lv.resize(b.L);
     long N = b.N;
@@ -1432,19 +1485,28 @@ template <int NS>
 static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l, int slot0, bool spatial,
                              bool costs, bool with_edges) {
     long nedge_w = with_edges ? div_up(a.E, EDGES_PER_WAVE) : 0;
-    double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
     const int mode = !a.use_cost_vol ? kCostWta : (a.right ? kCostRight : kCostLeft);
-    ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
-    if (B.coarse) {
+    // Coarse levels: one launch carries the segments' waves and the message waves (the
+    // segments' latency chains and the messages' arithmetic overlap).  Fine levels and
+    // message-only launches: the messages run in their own kernel, whose smaller register
+    // budget keeps twice the waves resident.
+    if (B.coarse && costs) {
+        double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
+        ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
         int npc = costs ? B.nseg_max : 0;
         long blocks = div_up(npc, 4) + div_up(nedge_w, 4);
         if (blocks == 0) return;
         launch_cost_edge_mode<NS, true>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
     } else {
-        int npc = costs ? (int)div_up(a.N, CHUNK) : 0;
-        long blocks = npc + div_up(nedge_w, 4);
-        if (blocks == 0) return;
-        launch_cost_edge_mode<NS, false>(c, a, mode, blocks, PC_LDS_BYTES, slot0, 0, npc);
+        if (costs) {
+            ProfScope ps(c, "plane_cost_L" + std::to_string(l), plane_cost_bytes(a.N, a.V, NS));
+            int npc = (int)div_up(a.N, CHUNK);
+            launch_cost_edge_mode<NS, false>(c, a, mode, npc, PC_LDS_BYTES, slot0, 0, npc);
+        }
+        if (nedge_w > 0) {
+            ProfScope ps(c, "edge_L" + std::to_string(l), edge_bytes(a.E, a.V));
+            hipLaunchKernelGGL(edge_kernel, dim3(div_up(nedge_w, 4)), dim3(256), 0, c.stream, a);
+        }
     }
 }
 

@@ -129,32 +129,10 @@ __host__ __device__ inline float int64_to_float_rne(int64_t q) {
     return neg ? -f : f;
 }
 
-// Cluster sums of per-pixel plane costs are exact int64 sums of q = rint(v * 2^s); the cost is
+// Cluster sums of per-pixel plane costs are exact sums of q = rint(|v| 2^s) (below 2^52, so
+// exact in double per lane and in int64 across lanes), with lambda's sign; the cost is
 // int64_to_float_rne(sum) * 2^-s, or the quiet NaN when any term was NaN.
 int fixed_point_shift(float lambda, float tau, int use_cost_vol, long npix);
-__host__ __device__ inline int64_t to_fixed(float v, int s) { return (int64_t)rintf(ldexpf(v, s)); }
-// to_fixed for finite v with |v 2^s| < 2^63, from the bits of the rounded float: a nonzero
-// integer valued float f has exponent field e >= 127 and f = significand * 2^(e - 150), i.e.
-// (significand << 40) >> (190 - e) with a shift in [0, 63].  Selects only, no branches.
-// an integer valued float 0 <= f < 2^62 as int64: f = hi 2^32 + lo with hi = floor(f 2^-32)
-// and lo = f - hi 2^32 in [0, 2^32), both exact (lo keeps at most f's 24 significant bits);
-// they are the two 32-bit words of the result
-__host__ __device__ inline int64_t nonneg_int_float_to_i64(float f) {
-    float hi = floorf(f * 2.3283064365386963e-10f);
-    float lo = fmaf(hi, -4294967296.0f, f);
-    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
-}
-__host__ __device__ inline int64_t to_fixed_finite(float v, int s) {
-    float f = rintf(ldexpf(v, s));
-    uint32_t b = 0;
-    memcpy(&b, &f, 4);
-    int e = (int)((b >> 23) & 0xffu);
-    uint64_t m = (b & 0x7fffffffu) ? (uint64_t)((b & 0x7fffffu) | 0x800000u) : 0ull;
-    int sh = 190 - e;
-    sh = sh < 0 ? 0 : (sh > 63 ? 63 : sh);
-    uint64_t u = (m << 40) >> sh;
-    return (b >> 31) ? -(int64_t)u : (int64_t)u;
-}
 __host__ __device__ inline float from_fixed(int64_t q, bool nan, int s) {
     if (nan) return __int_as_float(0x7FC00000);
     return ldexpf(int64_to_float_rne(q), -s);

@@ -74,6 +74,8 @@ struct BP {
     std::vector<BPLevel> lv;
     DevBuf<float4> msg;            // [E][8] persistent messages of the current level (2 x float4)
     DevBuf<float> msg_t;           // [V][maxdeg][12] messages of the current sweep (node-major)
+    DevBuf<uint32_t> crx;          // reversed, padded right census [H][crx_stride]
+    int crx_stride = 0;
     DevBuf<unsigned long long> acc;  // [V][12] fixed-point plane-cost sums
     DevBuf<unsigned int> accnan;     // [V] NaN bits per slot
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;

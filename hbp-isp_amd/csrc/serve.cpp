@@ -49,11 +49,11 @@ struct Server {
 };
 
 // device bytes of one pair: u8 volume, census/WTA, the pyramid's packed arrays (2N nodes and
-// 2*4*2N edges of capacity, segment.hpp:276-285) and the BP levels (sorted pixels, segments,
-// per-level structures and messages), rounded up
+// 2*4*2N edges of capacity, segment.hpp:276-285), the BP levels (sorted pixels, segments,
+// per-level structures and messages) and the padded right census, rounded up
 static size_t estimate_bytes(int W, int H, int D) {
     const size_t N = (size_t)W * H;
-    return (size_t)D * H * round_up(W, 64) + 16 * N + 152 * N + 160 * N;
+    return (size_t)D * H * round_up(W, 64) + 16 * N + 152 * N + 160 * N + 4 * (size_t)H * (W + D);
 }
 
 static void admit(Server &s, Request &r) {

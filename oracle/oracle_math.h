@@ -83,15 +83,16 @@ inline float i64_to_f32(int64_t q) {
 // Fixed-point cluster sums (pinned rule for the plane-cost reduction, whose order the
 // reference leaves to thrust::reduce_by_key, hbp_isp.cu:1167-1173): each per-pixel value v
 // becomes q = rint(v * 2^s) as int64; a cluster's cost is i64_to_f32(sum q) * 2^-s, or NaN
-// if any of its values is NaN.  s is chosen so that no sum can overflow:
-//   s = 62 - e(vmax) - e(N),  x < 2^e(x) from frexp,  vmax = |lambda| * (tau or 1) * 1.0001.
+// if any of its values is NaN.  s is chosen so that every partial sum stays below 2^52 (exact
+// in int64 and in double, in any order):
+//   s = 52 - e(vmax) - e(N),  x < 2^e(x) from frexp,  vmax = |lambda| * (tau or 1) * 1.0001.
 inline int fixed_shift(float lambda, float tau, int use_cost_vol, int npix) {
     double vmax = std::fabs((double)lambda) * (use_cost_vol ? std::fabs((double)tau) : 1.0) * 1.0001;
     int ev = 0;
     if (vmax > 0.0) std::frexp(vmax, &ev);
     int en = 0;
     std::frexp((double)(npix > 0 ? npix : 1), &en);
-    int s = 62 - ev - en;
+    int s = 52 - ev - en;
     if (s > 120) s = 120;
     if (s < -60) s = -60;
     return s;
