@@ -29,6 +29,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <utility>
+
 #include "hbp_internal.h"
 
 namespace hbp {
@@ -209,15 +211,19 @@ __device__ __forceinline__ float4 fit_plane(float4 A, float4 B, float4 C) {
 // one plane of ransac_candidate_plane (hbp_isp.cu:592-640): three LCG draws from seed s
 // (advanced in place), three distinct pixels of the superpixel, the plane through their WTA
 // disparities.  q returns the packed pixels.
+__device__ __forceinline__ void ransac_draw(uint32_t &s, int n, int &p0, int &p1, int &p2) {
+    s = lcg_next(s);
+    p0 = (int)umod0(s, n);
+    s = lcg_next(s);
+    p1 = (int)umod0(s, n - 1);
+    s = lcg_next(s);
+    p2 = (int)umod0(s, n - 2);
+    three_diff(p0, p1, p2, n);
+}
 __device__ __forceinline__ float4 ransac_plane(uint32_t &s, const uint32_t *__restrict__ pxy, int pos, int n,
                                                const float *__restrict__ pw, uint4 &q) {
-    s = lcg_next(s);
-    int p0 = (int)umod0(s, n);
-    s = lcg_next(s);
-    int p1 = (int)umod0(s, n - 1);
-    s = lcg_next(s);
-    int p2 = (int)umod0(s, n - 2);
-    three_diff(p0, p1, p2, n);
+    int p0, p1, p2;
+    ransac_draw(s, n, p0, p1, p2);
     q.x = pxy[pos + p0];
     q.y = pxy[pos + p1];
     q.z = pxy[pos + p2];
@@ -877,6 +883,28 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
 // that same lane or reaches it through a cross-lane shuffle, so no lane overwrites data
 // another lane has still to read.
 // ----------------------------------------------------------------------------------------
+// lane J of each 16-lane row, to every lane of the row (DPP row_newbcast)
+template <int J>
+__device__ __forceinline__ int row_bcast(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + J, 0xF, 0xF, false);
+}
+template <int J>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __int_as_float(row_bcast<J>(__float_as_int(v)));
+}
+// order[k] = #{j < k: !(bel[k] < bel[j])} + #{j > k: bel[j] < bel[k]} over the K slots
+template <int... J>
+__device__ __forceinline__ int belief_rank(float bel, int k, std::integer_sequence<int, J...>) {
+    int order = 0;
+    ((order += ((J < k) & !(bel < row_bcast<J>(bel))) | ((J > k) & (row_bcast<J>(bel) < bel))), ...);
+    return order;
+}
+// the incoming slots i < MAXDEG: their reverse (node, slot) index, from lane i of the row
+template <int... I>
+__device__ __forceinline__ void row_revl(int rv, int (&r)[sizeof...(I)], std::integer_sequence<int, I...>) {
+    ((r[I] = row_bcast<I>(rv)), ...);
+}
+
 __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn) {
     constexpr int MAXDEG = 16;   // incoming messages held in registers (the rest are re-read)
     const int lane = threadIdx.x & 63;
@@ -884,28 +912,43 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     const int g = lane & ~15;
     const bool own = k < K;
     const int kk = own ? k : 0;
-    // independent loads first
+    // ---- everything that depends on s alone, in one round trip
     const int deg = a.off[s + 1] - a.off[s];
     const int ppos = a.pix_off[s], pn = a.pix_off[s + 1] - ppos;
-    // the NaN bits of the candidate sums and the RANSAC seed: lane 2P reads (and rewrites)
-    // them, the others get them by shuffle
-    unsigned int nb = 0u;
-    uint32_t s0 = 0u;
-    if (k == 2 * P) {
-        nb = a.accnan[s];
-        s0 = a.s_ransac[s];
-    }
-    float cost = kk < 2 * P ? a.cost[s * K + kk] : 0.f;
-    int64_t fsum = kk >= 2 * P ? acc_sum(a, s, kk) : 0;
+    const unsigned int nb = a.accnan[s];
+    const uint32_t s0 = a.s_ransac[s];
+    const float cost_in = a.cost[s * K + kk];
+    const int64_t fsum = acc_sum(a, s, kk >= 2 * P ? kk : 2 * P);
     const float4 pin = a.plane[s * K + kk];
-    // node-major slots: every address below depends on s only
+    const float2 mxy = a.mxy[s];
     const long nm = (long)s * a.maxdeg;
-    const int rv = k < a.maxdeg ? a.revl[nm + k] : 0;
+    const int dmax = a.maxdeg - 1;
+    const int rv = a.revl[nm + (k < dmax ? k : dmax)];
     float mt[MAXDEG];
 #pragma unroll
-    for (int i = 0; i < MAXDEG; ++i) mt[i] = i < a.maxdeg ? a.msg_t[(nm + i) * K + kk] : 0.f;
+    for (int i = 0; i < MAXDEG; ++i) mt[i] = a.msg_t[(nm + (i < dmax ? i : dmax)) * K + kk];
+    const bool seeds = this_kind == kSpatial && s < a.Vh && k == 0;
+    const long sh = s < a.Vh ? s : a.Vh - 1;
+    uint32_t sc = a.s_child[sh], sn = a.s_neighbor[sh];
+    // the next sweep's RANSAC samples depend on the seed alone: lane p in [2P, 3P) draws
+    // candidate p from the seed after the draws of candidates 2P..p-1 and issues its loads now
+    const bool cand = own && k >= 2 * P;
+    uint32_t sk = s0;
+    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+    float z0 = 0.f, z1 = 0.f, z2 = 0.f;
+    if (next_kind == kRansac) {
+        const int adv = cand ? 3 * (k - 2 * P) : 0;
+        for (int i = 0; i < adv; ++i) sk = lcg_next(sk);
+        int p0, p1, p2;
+        ransac_draw(sk, pn, p0, p1, p2);
+        q = make_uint4(a.pxy[ppos + p0], a.pxy[ppos + p1], a.pxy[ppos + p2], 0u);
+        z0 = a.pw[ppos + p0];
+        z1 = a.pw[ppos + p1];
+        z2 = a.pw[ppos + p2];
+    }
+    // ---- sums and the selection
     if (k == 2 * P) a.accnan[s] = 0u;
-    nb = __shfl(nb, g + 2 * P);
+    float cost = cost_in;
     if (kk >= 2 * P) {
         cost = from_fixed(fsum, (nb >> kk) & 1u, a.shift);
         acc_zero(a, s, kk);
@@ -913,17 +956,9 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     // belief = cost + incoming messages in edge order
     float bel = cost;
 #pragma unroll
-    for (int i = 0; i < MAXDEG; ++i)
-        if (i < deg) bel += mt[i];
+    for (int i = 0; i < MAXDEG; ++i) bel = i < deg ? bel + mt[i] : bel;
     for (int i = MAXDEG; i < deg; ++i) bel += a.msg_t[(nm + i) * K + kk];
-    // rank: order[k] = #{j < k: !(bel[k] < bel[j])} + #{j > k: bel[j] < bel[k]}
-    int order = 0;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        float bj = __shfl(bel, g + j);
-        if (j < k) order += (bel < bj) ? 0 : 1;
-        else if (j > k) order += (bj < bel) ? 1 : 0;
-    }
+    const int order = belief_rank(bel, k, std::make_integer_sequence<int, K>{});
     const bool sel = own && order < P;
     const uint32_t m = (uint32_t)(__ballot(sel) >> g) & 0xfffu;
     const int nsel = __popc(m);
@@ -937,27 +972,28 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     const float cost_n = __shfl(cost, g + src), bel_n = __shfl(bel, g + src);
     const float4 pl_n = make_float4(__shfl(pin.x, g + src), __shfl(pin.y, g + src), __shfl(pin.z, g + src),
                                     __shfl(pin.w, g + src));
+    // messages over s's particles -> reverse edge, compacted by s's order
+    float o[MAXDEG];
+    int r[MAXDEG];
+#pragma unroll
+    for (int i = 0; i < MAXDEG; ++i) o[i] = __shfl(mt[i], g + src);
+    row_revl(rv, r, std::make_integer_sequence<int, MAXDEG>{});
     if (own) {
         a.plane[s * K + k] = pl_n;
         a.cost[s * K + k] = cost_n;
         a.belief[s * K + k] = bel_n;
     }
-    // messages over s's particles -> reverse edge, compacted by s's order
     float *msg = (float *)a.msg;
+    if (k < 2 * P) {
 #pragma unroll
-    for (int i = 0; i < MAXDEG; ++i) {
-        if (i < deg) {
-            const float o = __shfl(mt[i], g + src);
-            const int r = __shfl(rv, g + i);
-            if (k < 2 * P) msg[(long)r * 8 + k] = o;
-        }
+        for (int i = 0; i < MAXDEG; ++i)
+            if (i < deg) msg[(long)r[i] * 8 + k] = o[i];
     }
     for (int i = MAXDEG; i < deg; ++i) {
-        const float o = __shfl(a.msg_t[(nm + i) * K + kk], g + src);
-        if (k < 2 * P) msg[(long)a.revl[nm + i] * 8 + k] = o;
+        const float oi = __shfl(a.msg_t[(nm + i) * K + kk], g + src);
+        if (k < 2 * P) msg[(long)a.revl[nm + i] * 8 + k] = oi;
     }
-    if (this_kind == kSpatial && s < a.Vh && k == 0) {
-        uint32_t sc = a.s_child[s], sn = a.s_neighbor[s];
+    if (seeds) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             sc = lcg_next(sc);
@@ -966,23 +1002,22 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
         a.s_child[s] = sc;
         a.s_neighbor[s] = sn;
     }
-    // next candidates: lane p in [2P, 3P) makes candidate p from the seed after the draws of
-    // candidates 2P..p-1; lane 2P also stores the seed after all of them
+    // next candidates; lane 2P also stores the seed after all of them
     if (next_kind == kRansac || next_kind == kRandom) {
-        s0 = __shfl(s0, g + 2 * P);
-        if (k >= 2 * P && own) {
-            const int steps = next_kind == kRansac ? 3 : 4;
-            uint32_t sk = s0;
-            for (int i = 0; i < steps * (k - 2 * P); ++i) sk = lcg_next(sk);
+        if (cand) {
             if (next_kind == kRansac) {
-                uint4 q = make_uint4(0u, 0u, 0u, 0u);
-                a.plane[s * K + k] = ransac_plane(sk, a.pxy, ppos, pn, a.pw, q);
+                const float4 A = make_float4((float)unpack_x(q.x), (float)unpack_y(q.x), z0, 0.f);
+                const float4 B = make_float4((float)unpack_x(q.y), (float)unpack_y(q.y), z1, 0.f);
+                const float4 C = make_float4((float)unpack_x(q.z), (float)unpack_y(q.z), z2, 0.f);
+                a.plane[s * K + k] = fit_plane(A, B, C);
                 if (k == K - 1) a.point[s] = q;
             } else {
-                float2 mxy = a.mxy[s];
-                a.plane[s * K + k] = random_plane(sk, pl_n, mxy.x, mxy.y, max_dz, max_dn);
+                uint32_t sr = s0;
+                for (int i = 0; i < 4 * (k - 2 * P); ++i) sr = lcg_next(sr);
+                a.plane[s * K + k] = random_plane(sr, pl_n, mxy.x, mxy.y, max_dz, max_dn);
             }
             if (k == 2 * P) {
+                const int steps = next_kind == kRansac ? 3 : 4;
                 uint32_t sf = s0;
                 for (int i = 0; i < steps * P; ++i) sf = lcg_next(sf);
                 a.s_ransac[s] = sf;
