@@ -436,6 +436,7 @@ enum CostMode { kCostLeft = 0, kCostRight = 1, kCostWta = 2 };
 // round to 0 (|lambda| 2^s is a finite normal float, checked on the host).
 // ----------------------------------------------------------------------------------------
 __device__ __forceinline__ void pin(uint32_t &v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(int &v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void pin(float &v) { asm volatile("" : "+v"(v)); }
 
 struct __attribute__((aligned(4))) Words2 {
@@ -581,9 +582,10 @@ __device__ __forceinline__ float pc_term(const SweepArgs &a, const PcPix &px, fl
 // memory round trips per batch.  The per-lane sums (exact doubles) are reduced across the
 // wavefront and added to replica (segment mod rrep) of the superpixel's int64 sums.
 // ----------------------------------------------------------------------------------------
+// q: the packed positions of steps r0 .. r0+B-1 (loaded by the caller)
 template <int MODE, int B, bool GEN>
 __device__ __forceinline__ void pc_batch(const SweepArgs &a, const float4 (&pl)[P], int slot0, int first, int end,
-                                         int r0, double (&acc)[P], unsigned int &nanm) {
+                                         int r0, const uint32_t *q, double (&acc)[P], unsigned int &nanm) {
     const int lane = threadIdx.x & 63;
     const float dmax = (float)(a.D > 2 ? a.D - 2 : 0);
     const float dlim = (float)(a.D - 1);
@@ -591,8 +593,7 @@ __device__ __forceinline__ void pc_batch(const SweepArgs &a, const float4 (&pl)[
     PcWords wd[B][P];
 #pragma unroll
     for (int r = 0; r < B; ++r) {
-        const int j = first + (r0 + r) * 64 + lane;
-        px[r] = pc_pixel<MODE>(a, ld_at<uint32_t>(a.pxy, (uint32_t)(GEN && j >= end ? end - 1 : j) << 2));
+        px[r] = pc_pixel<MODE>(a, q[r]);
 #pragma unroll
         for (int k = 0; k < P; ++k) pc_load<MODE, GEN>(a, px[r], pl[k], dmax, wd[r][k]);
     }
@@ -643,17 +644,30 @@ template <int NS, int MODE>
 __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
     static_assert(NS % P == 0, "planes are costed in groups of P");
     const int lane = threadIdx.x & 63;
-    const int4 sg = a.seg[w];
+    int4 sg = a.seg[w];
+    pin(sg.x); pin(sg.y); pin(sg.z); pin(sg.w);   // one 16-byte load, not two around the test
     if (sg.y <= sg.x) return;   // past the last segment (the array is padded with empty ones)
     const int c = __builtin_amdgcn_readfirstlane(sg.z);
     const int first = __builtin_amdgcn_readfirstlane(sg.x), end = __builtin_amdgcn_readfirstlane(sg.y);
     const int steps = (end - first + 63) >> 6;
     const int full = (end - first) >> 6;
+    // the positions of all SEG_R steps (lanes past the end re-read the last pixel), then the
+    // first group's planes: one round trip for both, in straight-line code
+    uint32_t q[SEG_R];
+#pragma unroll
+    for (int r = 0; r < SEG_R; ++r) {
+        const int j = first + r * 64 + lane;
+        q[r] = ld_at<uint32_t>(a.pxy, (uint32_t)(j < end ? j : end - 1) << 2);
+    }
+    float4 pl[P];
+    if (!spatial) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) pl[k] = a.plane[(long)c * K + slot0 + k];
+    }
     unsigned int nanm = 0u;
     unsigned long long mine = 0ull;
 #pragma unroll 1
     for (int g0 = 0; g0 < NS; g0 += P) {
-        float4 pl[P];
         if (spatial) {
             float4 cand = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane < P) cand = spatial_candidate(a, c, lane);
@@ -661,7 +675,7 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 #pragma unroll
             for (int k = 0; k < P; ++k)
                 pl[k] = make_float4(__shfl(cand.x, k), __shfl(cand.y, k), __shfl(cand.z, k), 0.f);
-        } else {
+        } else if (g0 > 0) {
 #pragma unroll
             for (int k = 0; k < P; ++k) pl[k] = a.plane[(long)c * K + slot0 + g0 + k];
         }
@@ -672,17 +686,27 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
         double acc[P];
 #pragma unroll
         for (int k = 0; k < P; ++k) acc[k] = 0.0;
-        // full steps in fast batches of 4, then batches of 4, 2 and 1 in the general form: no
+        // the two halves of SEG_R / 2 steps: a full half in one batch (fast form when the
+        // planes allow), a partial one in batches of 2 and 1 in the general form, so that no
         // step past the segment's end is computed
-        int r0 = 0;
-        if (fast)
-            for (; r0 + 4 <= full; r0 += 4) pc_batch<MODE, 4, false>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
-        for (; r0 + 4 <= steps; r0 += 4) pc_batch<MODE, 4, true>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
-        if (steps - r0 >= 2) {
-            pc_batch<MODE, 2, true>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
-            r0 += 2;
+        static_assert(SEG_R == 8, "two halves of 4 steps");
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            constexpr int HB = 4;
+            const int r0 = h * HB;
+            const int rem = steps - r0;
+            if (rem <= 0) break;
+            const int sl = slot0 + g0;
+            if (rem >= HB) {
+                if (fast && r0 + HB <= full) pc_batch<MODE, HB, false>(a, pl, sl, first, end, r0, q + r0, acc, nanm);
+                else pc_batch<MODE, HB, true>(a, pl, sl, first, end, r0, q + r0, acc, nanm);
+            } else if (rem == 1) {
+                pc_batch<MODE, 1, true>(a, pl, sl, first, end, r0, q + r0, acc, nanm);
+            } else {
+                pc_batch<MODE, 2, true>(a, pl, sl, first, end, r0, q + r0, acc, nanm);
+                if (rem == 3) pc_batch<MODE, 1, true>(a, pl, sl, first, end, r0 + 2, q + r0 + 2, acc, nanm);
+            }
         }
-        if (steps - r0 == 1) pc_batch<MODE, 1, true>(a, pl, slot0 + g0, first, end, r0, acc, nanm);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             // every cost is >= 0, so the sum takes lambda's sign
