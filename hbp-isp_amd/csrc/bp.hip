@@ -1116,6 +1116,7 @@ struct BatchItem {
     int blk_begin, pc_blocks, edge_blocks;   // cost/message launch
     int node_begin, node_blocks;             // node launch
     int sp_begin, sp_blocks;                 // spatial launch (fine levels)
+    int msg_begin, msg_blocks;               // message launch (fine levels)
 };
 
 // Every pointer of a SweepArgs, for the relative form of the batch table: the host stores each
@@ -1140,7 +1141,10 @@ __device__ __forceinline__ int batch_find(const BatchItem *__restrict__ it, int 
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int v = FIELD == 0 ? it[mid].blk_begin : (FIELD == 1 ? it[mid].node_begin : it[mid].sp_begin);
+        const int v = FIELD == 0   ? it[mid].blk_begin
+                      : FIELD == 1 ? it[mid].node_begin
+                      : FIELD == 2 ? it[mid].sp_begin
+                                   : it[mid].msg_begin;
         if (v <= b) lo = mid;
         else hi = mid - 1;
     }
@@ -1178,6 +1182,16 @@ __global__ __launch_bounds__(256) void node_batch_kernel(const SweepArgs *__rest
     const SweepArgs a = rebased_args(args, it.args);
     const long s = ((long)local * 256 + threadIdx.x) >> 4;
     if (s < a.V) node_group(a, s, it.kind, it.next_kind, it.ndz, it.ndn);
+}
+__global__ __launch_bounds__(256) void edge_batch_kernel(const SweepArgs *__restrict__ args,
+                                                         const BatchItem *__restrict__ items, int nitems) {
+    const int b = blockIdx.x;
+    const BatchItem it = items[batch_find<3>(items, nitems, b)];
+    const int local = b - it.msg_begin;
+    if (local >= it.msg_blocks) return;
+    const SweepArgs a = rebased_args(args, it.args);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    edge_wave(a, (long)local * 4 + wave, false);
 }
 __global__ __launch_bounds__(256) void spatial_batch_kernel(const SweepArgs *__restrict__ args,
                                                             const BatchItem *__restrict__ items, int nitems) {
@@ -1679,7 +1693,7 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
     // items: the segment-form ones first, then the run-form ones; each form is one cost launch
     // with its own block numbering, the node launch covers both
     std::vector<BatchItem> part[2];
-    int coarse_blocks = 0, fine_blocks = 0, node_blocks = 0, sp_blocks = 0;
+    int coarse_blocks = 0, fine_blocks = 0, node_blocks = 0, sp_blocks = 0, msg_blocks = 0;
     for (const auto &pr : active) {
         BP &b = *pr.first;
         const size_t i = b.next;
@@ -1700,7 +1714,10 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
         it.spatial = sw.kind == kSpatial ? 1 : 0;
         it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, CHUNK));
         it.pc_blocks = B.coarse ? div_up(it.npc, 4) : it.npc;
-        it.edge_blocks = div_up(div_up(B.E, EDGES_PER_WAVE), 4);
+        // coarse levels: the messages share the segments' launch; fine levels: their own
+        const int eb = (int)div_up(div_up(B.E, EDGES_PER_WAVE), 4);
+        it.edge_blocks = B.coarse ? eb : 0;
+        it.msg_blocks = B.coarse ? 0 : eb;
         int &blocks = B.coarse ? coarse_blocks : fine_blocks;
         it.blk_begin = blocks;
         blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
@@ -1716,6 +1733,8 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
         for (BatchItem &it : part[f]) {
             it.node_begin = node_blocks;
             node_blocks += it.node_blocks;
+            it.msg_begin = msg_blocks;
+            msg_blocks += it.msg_blocks;
             it.sp_begin = sp_blocks;
             sp_blocks += it.sp_blocks;
             hitems.push_back(it);
@@ -1753,6 +1772,10 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
         else
             hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), dim3(fine_blocks), dim3(256), PC_LDS_BYTES,
                                c.stream, dargs, itf, nfine);
+    }
+    if (msg_blocks > 0) {
+        ProfScope ps(c, "batch_edge", 0.0);
+        hipLaunchKernelGGL(edge_batch_kernel, dim3(msg_blocks), dim3(256), 0, c.stream, dargs, itc, count);
     }
     {
         ProfScope ps(c, "batch_node", 0.0);
