@@ -222,10 +222,22 @@ void PostProcess::plane_label_to_disp(std::pair<cv::Mat, cv::Mat> &L, std::pair<
           "PostProcess::plane_label_to_disp");
 }
 
-void PostProcess::median_filter(std::pair<cv::Mat, cv::Mat> &, std::pair<cv::Mat, cv::Mat> &) {
-    throw std::logic_error("PostProcess::median_filter is not provided");
+// post_process.cu:127-185: the left view's failed pixels (i < 1 in the reference's loop)
+void PostProcess::median_filter(std::pair<cv::Mat, cv::Mat> &L, std::pair<cv::Mat, cv::Mat> &) {
+    if (fail[0].empty()) throw std::logic_error("PostProcess::median_filter: run left_right_check first");
+    if (L.second.type() != CV_32FC3 || L.second.cols != width || L.second.rows != height)
+        throw std::invalid_argument("median_filter: planes must be CV_32FC3 of the image size");
+    if (I[0].type() != CV_8UC3) throw std::invalid_argument("median_filter: the left image must be CV_8UC3");
+    check(hbp_weighted_median(default_ctx(), I[0].data, fail[0].data, (float *)L.second.data, width, height),
+          "PostProcess::median_filter");
 }
 
-void PostProcess::median_filter_gpu(std::pair<cv::Mat, cv::Mat> &, std::pair<cv::Mat, cv::Mat> &) {
-    throw std::logic_error("PostProcess::median_filter_gpu is not provided");
+void medianFilter(cv::Mat &D) {
+    if (D.type() != CV_32FC1) throw std::invalid_argument("medianFilter: CV_32FC1 expected");
+    check(hbp_median7(default_ctx(), (float *)D.data, D.cols, D.rows), "medianFilter");
+}
+
+// post_process.cu:187-371, the device twin of median_filter: the same computation
+void PostProcess::median_filter_gpu(std::pair<cv::Mat, cv::Mat> &L, std::pair<cv::Mat, cv::Mat> &R) {
+    median_filter(L, R);
 }

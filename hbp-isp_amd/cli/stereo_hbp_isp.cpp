@@ -1,9 +1,10 @@
 // Single-pair driver with the argument order and call sequence of main_stereo_hbp_isp.cu
 // (lines 8-118): SegHAC -> run_ms -> ISPData -> StereoCost -> StereoData -> HBP_ISP::run_ms ->
 // medianFilter -> PFM.  Images are binary PPM (P6) files; the output is
-// <out_name>_hbp_isp_disp.pfm.  With a trailing "lr" argument it also runs the right view and
-// the left-right check + fill (the MidV3_LR sequence, main_bench_middv3.cu:132-238) and writes
-// <out_name>_hbp_isp_disp_lr.pfm.
+// <out_name>_hbp_isp_disp.pfm.  With a trailing "lr" argument it also runs the right view, the
+// left-right check + fill and the weighted-median refinement (the sequence the reference keeps
+// commented out at main_stereo_hbp_isp.cu:84-113, and MidV3_LR, main_bench_middv3.cu:132-238)
+// and writes <out_name>_hbp_isp_disp_lr.pfm.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -57,37 +58,6 @@ static void save_pfm(const std::string &path, const cv::Mat &img) {
     std::fclose(s);
 }
 
-// medianFilter (main_stereo_hbp_isp.cu:120-193): 7-tap horizontal then vertical median over
-// pixels with D >= 0, on the interior only; the vertical pass reads the horizontal result,
-// whose 3-pixel border stays 0.
-static void median_filter7(cv::Mat &D) {
-    const int w = D.cols, h = D.rows, r = 3;
-    cv::Mat T(h, w, CV_32FC1);
-    std::memset(T.data, 0, sizeof(float) * (size_t)w * h);
-    float vals[7];
-    auto median = [&](auto get) {
-        int j = 0;
-        for (int k = -r; k <= r; ++k) {
-            float t = get(k);
-            int i = j - 1;
-            while (i >= 0 && vals[i] > t) {
-                vals[i + 1] = vals[i];
-                i--;
-            }
-            vals[i + 1] = t;
-            j++;
-        }
-        return vals[r];
-    };
-    for (int u = r; u < w - r; ++u)
-        for (int v = r; v < h - r; ++v)
-            T.at<float>(v, u) = D.at<float>(v, u) >= 0 ? median([&](int k) { return D.at<float>(v, u + k); })
-                                                       : D.at<float>(v, u);
-    for (int u = r; u < w - r; ++u)
-        for (int v = r; v < h - r; ++v)
-            if (D.at<float>(v, u) >= 0) D.at<float>(v, u) = median([&](int k) { return T.at<float>(v + k, u); });
-}
-
 int main(int argc, char **argv) {
     if (argc != 13 && argc != 14 && argc != 15) {
         std::cerr << "usage: " << argv[0]
@@ -135,7 +105,7 @@ int main(int argc, char **argv) {
         t1 = clk::now();
         std::cout << "time for belief propagation on ISP: " << ms(t0, t1) << " ms" << std::endl;
         cv::Mat disp = left.first.clone();
-        median_filter7(disp);
+        medianFilter(disp);
         save_pfm(out + "_hbp_isp_disp.pfm", disp);
         delete hbp;
         delete sd;
@@ -156,8 +126,11 @@ int main(int argc, char **argv) {
             delete hbp_r;
             delete sd_r;
             delete isp_r;
+            // main_stereo_hbp_isp.cu:107-113: check + fill, weighted-median refinement
             PostProcess pp(L, R);
             pp.left_right_check(left, right);
+            pp.plane_label_to_disp(left, right);
+            pp.median_filter(left, right);
             pp.plane_label_to_disp(left, right);
             save_pfm(out + "_hbp_isp_disp_lr.pfm", left.first);
         }

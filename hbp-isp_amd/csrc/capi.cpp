@@ -511,6 +511,56 @@ int hbp_lr_check_fill(hbp_ctx *ctx, const float *dl, const float *dr, float *pl,
     HBP_API_END
 }
 
+int hbp_weighted_median_dev(hbp_ctx *ctx, const uint8_t *d_bgr, const uint8_t *d_fail, float *d_plane3, int W,
+                            int H) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && d_bgr && d_fail && d_plane3 && W > 0 && H > 0, "bad argument");
+    HBP_REQUIRE((long)W * H < (1L << 31), "image too large");
+    use_device(ctx);
+    weighted_median(ctx->c, d_bgr, d_fail, d_plane3, W, H);
+    HBP_API_END
+}
+
+int hbp_weighted_median(hbp_ctx *ctx, const uint8_t *bgr, const uint8_t *fail, float *plane3, int W, int H) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && bgr && fail && plane3 && W > 0 && H > 0, "bad argument");
+    HBP_REQUIRE((long)W * H < (1L << 31), "image too large");
+    use_device(ctx);
+    size_t n = (size_t)W * H;
+    hipStream_t s = ctx->c.stream;
+    DevBuf<float> d(ctx->c.pool, 3 * n);
+    DevBuf<uint8_t> b(ctx->c.pool, 4 * n);
+    HBP_CHECK(hipMemcpyAsync(b.p, bgr, 3 * n, hipMemcpyHostToDevice, s));
+    HBP_CHECK(hipMemcpyAsync(b.p + 3 * n, fail, n, hipMemcpyHostToDevice, s));
+    HBP_CHECK(hipMemcpyAsync(d.p, plane3, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    weighted_median(ctx->c, b.p, b.p + 3 * n, d.p, W, H);
+    HBP_CHECK(hipMemcpyAsync(plane3, d.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    HBP_CHECK(hipStreamSynchronize(s));
+    HBP_API_END
+}
+
+int hbp_median7_dev(hbp_ctx *ctx, float *d_disp, int W, int H) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && d_disp && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    median7_filter(ctx->c, d_disp, W, H);
+    HBP_API_END
+}
+
+int hbp_median7(hbp_ctx *ctx, float *disp, int W, int H) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && disp && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    size_t n = (size_t)W * H;
+    hipStream_t s = ctx->c.stream;
+    DevBuf<float> d(ctx->c.pool, n);
+    HBP_CHECK(hipMemcpyAsync(d.p, disp, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    median7_filter(ctx->c, d.p, W, H);
+    HBP_CHECK(hipMemcpyAsync(disp, d.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HBP_CHECK(hipStreamSynchronize(s));
+    HBP_API_END
+}
+
 int hbp_plane_to_disp(hbp_ctx *ctx, const float *plane3, int W, int H, float *disp) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && plane3 && disp && W > 0 && H > 0, "bad argument");

@@ -127,6 +127,8 @@ int bp_total_sweeps(int levels, int min_level, int D);
 void lr_check_fill(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
                    uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2);
 void plane_to_disp(Ctx &c, const float *d_plane3, int W, int H, float *d_disp);
+void weighted_median(Ctx &c, const uint8_t *d_bgr, const uint8_t *d_fail, float *d_plane3, int W, int H);
+void median7_filter(Ctx &c, float *d_disp, int W, int H);
 
 // primitives.hip
 void scan_exclusive_int(Ctx &c, const int *d_in, int *d_out, long n, int *d_total);

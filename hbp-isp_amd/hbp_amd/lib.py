@@ -78,6 +78,10 @@ SIGNATURES = {
     "hbp_bp_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "hbp_bp_free": (C.c_int, [_vp, _vp]),
     "hbp_schedule_sweeps": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "hbp_weighted_median": (C.c_int, [_vp, _u8, _u8, _f32, C.c_int, C.c_int]),
+    "hbp_weighted_median_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int]),
+    "hbp_median7": (C.c_int, [_vp, _f32, C.c_int, C.c_int]),
+    "hbp_median7_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int]),
     "hbp_lr_check_fill": (C.c_int, [_vp, _f32, _f32, _f32, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hbp_lr_check_fill_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hbp_plane_to_disp": (C.c_int, [_vp, _f32, C.c_int, C.c_int, _f32]),
@@ -182,6 +186,21 @@ class Ctx:
                                           np.ascontiguousarray(dispR, np.float32), plane, W, H,
                                           _np_ptr(fl), _np_ptr(fr), _np_ptr(f2)))
         return plane, fl, fr, f2
+
+    def weighted_median(self, bgr, fail, plane3):
+        """PostProcess::median_filter: the failed pixels' planes (a new array)"""
+        H, W = fail.shape
+        out = np.ascontiguousarray(plane3, dtype=np.float32).copy()
+        _check(self.lib.hbp_weighted_median(self.h, np.ascontiguousarray(bgr, np.uint8),
+                                            np.ascontiguousarray(fail, np.uint8), out, W, H))
+        return out
+
+    def median7(self, disp):
+        """the CLI's 7-tap separable median (a new array)"""
+        H, W = disp.shape
+        out = np.ascontiguousarray(disp, dtype=np.float32).copy()
+        _check(self.lib.hbp_median7(self.h, out, W, H))
+        return out
 
     def plane_to_disp(self, plane3):
         H, W = plane3.shape[:2]

@@ -160,6 +160,21 @@ int hbp_lr_check_fill_dev(hbp_ctx *ctx, const float *d_disp_left, const float *d
                           uint8_t *d_fail_right, uint8_t *d_fail2_left);
 /* plane_label_to_disp (post_process.cu:373-390), host buffers */
 int hbp_plane_to_disp(hbp_ctx *ctx, const float *plane3, int width, int height, float *disp);
+/* median_filter / median_filter_gpu (post_process.cu:127-185, 187-371): every pixel with
+ * fail != 0 takes the weighted median of the planes in its 41x41 window, in the order of their
+ * disparities at the pixel, weights exp(-|I(p) - I(q)|_1 / 10) on the BGR image; planes are
+ * read as they were before the call.  Host buffers: bgr u8 [H][W][3], fail u8 [H][W],
+ * plane3 f32 [H][W][3] in place. */
+int hbp_weighted_median(hbp_ctx *ctx, const uint8_t *bgr, const uint8_t *fail, float *plane3, int width,
+                        int height);
+/* same on device buffers */
+int hbp_weighted_median_dev(hbp_ctx *ctx, const uint8_t *d_bgr, const uint8_t *d_fail, float *d_plane3,
+                            int width, int height);
+/* medianFilter of the CLI (main_stereo_hbp_isp.cu:120-193): horizontal then vertical 7-tap
+ * median of a disparity image f32 [H][W], in place (host buffer) */
+int hbp_median7(hbp_ctx *ctx, float *disp, int width, int height);
+/* same on a device buffer */
+int hbp_median7_dev(hbp_ctx *ctx, float *d_disp, int width, int height);
 
 #ifdef __cplusplus
 }

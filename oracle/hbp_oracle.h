@@ -69,6 +69,12 @@ void orc_plane_to_disp(const float *plane3, int W, int H, float *disp);
 void orc_left_right_check(const float *dispL, const float *dispR, float *planeL, float *planeR,
                           int W, int H, uint8_t *failL, uint8_t *failR, uint8_t *fail2L);
 
+/* refinements after the LR check: weighted median of the failed pixels' planes (in place) and
+ * the CLI's 7-tap separable median of a disparity image (in place) */
+float orc_wm_weight(float absdiff);
+void orc_weighted_median(const uint8_t *bgr, const uint8_t *fail, float *plane3, int W, int H);
+void orc_median7(float *disp, int W, int H);
+
 /* whole plain path (left view) for the CPU baseline */
 int orc_plain_pair(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D, float alpha,
                    float tau_s, int min_level, int bp_levels, float *disp, float *plane3,

@@ -14,6 +14,8 @@ slanted planes); with it:
   * disp.npy / plane3.npy            the final MAP disparity and planes of the left view
   * disp_right.npy / plane3_right.npy  the same for the right view
   * lr_{fail_l,fail_r,fail2_l}.npy / lr_plane3.npy   left-right check masks and filled planes
+  * wm_plane3.npy                    the filled planes after the weighted-median refinement
+  * disp_median7.npy                 the final left disparity after the CLI's 7-tap median
 meta.json holds the sizes and counts; SHA256SUMS the digest of every file.
 """
 import hashlib
@@ -97,6 +99,12 @@ def compute():
     fl, fr, f2 = (np.zeros((H, W), np.uint8) for _ in range(3))
     O.lib.orc_left_right_check(out["disp"], out["disp_right"], pl, pr, W, H, fl, fr, f2)
     out["lr_fail_l"], out["lr_fail_r"], out["lr_fail2_l"], out["lr_plane3"] = fl, fr, f2, pl
+    wm = pl.copy()
+    O.lib.orc_weighted_median(left, fl, wm, W, H)
+    out["wm_plane3"] = wm
+    d7 = out["disp"].copy()
+    O.lib.orc_median7(d7, W, H)
+    out["disp_median7"] = d7
     return out, meta
 
 

@@ -58,6 +58,9 @@ def _load():
         "orc_lr_fill": (None, [_u8p, _u8p, _f32p, C.c_int, C.c_int]),
         "orc_plane_to_disp": (None, [_f32p, C.c_int, C.c_int, _f32p]),
         "orc_left_right_check": (None, [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, _u8p, _u8p, _u8p]),
+        "orc_wm_weight": (C.c_float, [C.c_float]),
+        "orc_weighted_median": (None, [_u8p, _u8p, _f32p, C.c_int, C.c_int]),
+        "orc_median7": (None, [_f32p, C.c_int, C.c_int]),
         "orc_plain_pair": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                      C.c_int, C.c_int, _f32p, _f32p, C.POINTER(C.c_int)]),
     }

@@ -96,3 +96,5 @@ def test_golden_bp_and_lr(ctx):
     assert same_bits(fl, load("lr_fail_l")) and same_bits(fr, load("lr_fail_r"))
     assert same_bits(f2, load("lr_fail2_l"))
     assert same_bits(filled, load("lr_plane3"))
+    assert same_bits(ctx.weighted_median(left, fl, filled), load("wm_plane3"))
+    assert same_bits(ctx.median7(disp), load("disp_median7"))
