@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the Poisson arrival window of the served-vs-plain comparison (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=12.0, help="mean arrivals per second")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the refinements after BP (right view, LR check, weighted median, 7-tap median)")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
     args = ap.parse_args()
@@ -180,12 +182,15 @@ def main():
         except (OSError, ValueError):
             traffic = None
 
+    post, post_io = (None, None) if args.no_post else post_process(ctx, left, right, D, info["min_level"])
     batched = batched_vs_plain(ctx, hip, batch_pairs) if batch_pairs else None
     served = served_vs_plain(ctx, serve_pool, args.serve_rate, args.serve_seconds, args.seed) if serve_pool else None
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(left, right, wl, info, args.cpu_sweeps)
+        if post is not None:
+            post.update(cpu_weighted_median(*post_io))
 
     out = {
         "metric": METRIC,
@@ -218,6 +223,7 @@ def main():
                                for n, v in sorted(pair_stats.items()) if "_L" in n},
         "batched_vs_plain": batched,
         "served_vs_plain": served,
+        "post_process": post,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -456,6 +462,58 @@ def served_vs_plain(ctx, pool, rate, seconds, seed, min_level=2):
             "served": {"pairs_per_s": round(n / span_s, 3), "p50_ms": pct(lat_s, 0.5), "p99_ms": pct(lat_s, 0.99)},
             "plain": {"pairs_per_s": round(n / span_p, 3), "p50_ms": pct(lat_p, 0.5), "p99_ms": pct(lat_p, 0.99)},
             "bit_identical_first_4": bool(same)}
+
+
+def post_process(ctx, left, right, D, min_level):
+    """The refinements after BP on the same pair (the sequence of main_stereo_hbp_isp.cu:84-113):
+    the right view's BP, the left-right check + fill, the weighted-median refinement of the
+    failed pixels' planes, plane_label_to_disp, and the CLI's 7-tap median of the left
+    disparity.  Device time of each stage (HIP events), host buffers in between."""
+    Hh, W = left.shape[:2]
+    dl, pl = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, D), min_level=min_level).run()
+    dr, _ = ctx.bp(ctx.pyramid(right), ctx.cost_volume(left, right, D, right_view=True), min_level=min_level).run()
+    runs = []
+    for _ in range(3):
+        ctx.profile(True)
+        ctx.profile_reset()
+        filled, fl, _, _ = ctx.lr_check_fill(dl, dr, pl)
+        wm = ctx.weighted_median(left, fl, filled)
+        ctx.plane_to_disp(wm)
+        ctx.median7(dl)
+        ctx.sync()
+        runs.append(ctx.profile_stats())
+        ctx.profile(False)
+    st = runs[-1]
+
+    def ms(name):
+        return round(min(r[name]["ms"] for r in runs), 3) if name in st else None
+
+    nfail = int((fl != 0).sum())
+    out = {"workload": f"{W}x{Hh} D{D}: right-view BP, LR check + fill, weighted median (41x41, failed pixels), "
+                       f"7-tap median",
+           "failed_pixels": nfail, "lr_check_ms": ms("lr_check"), "weighted_median_ms": ms("weighted_median"),
+           "median7_ms": ms("median7")}
+    wmm = out["weighted_median_ms"]
+    if wmm:
+        out["weighted_median_mpix_window_per_s"] = round(nfail * 41 * 41 / (wmm * 1e-3) / 1e6, 1)
+    return out, (left, fl, filled, wm)
+
+
+def cpu_weighted_median(left, fl, filled, wm_gpu):
+    """The oracle's weighted median on the same failed pixels (host cores), and the GPU result
+    checked against it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_ffi as O
+    Hh, W = fl.shape
+    ref = np.ascontiguousarray(filled, np.float32).copy()
+    t0 = time.perf_counter()
+    O.lib.orc_weighted_median(np.ascontiguousarray(left), np.ascontiguousarray(fl), ref, W, Hh)
+    t = time.perf_counter() - t0
+    same = bool(np.array_equal(np.isnan(ref), np.isnan(wm_gpu)) and
+                np.array_equal(ref.view(np.uint32)[~np.isnan(ref)], wm_gpu.view(np.uint32)[~np.isnan(wm_gpu)]))
+    return {"cpu_weighted_median_ms": round(1e3 * t, 1), "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", "1")),
+            "weighted_median_bit_identical": same}
 
 
 def cpu_baseline(left, right, wl, info, sweeps):

@@ -104,6 +104,8 @@ static int grid_for(const Ctx &c, long n, int block = 256) {
 void lr_check_fill(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
                    uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2) {
     long n = (long)W * H;
+    // algorithmic bytes: both disparities in, three masks out, the left planes in and out
+    ProfScope ps(c, "lr_check", (8.0 + 3.0 + 24.0) * n);
     hipLaunchKernelGGL(lr_consistency_kernel, dim3(grid_for(c, 2 * n)), dim3(256), 0, c.stream, d_dispL, d_dispR, W,
                        H, d_failL, d_failR);
     hipLaunchKernelGGL(dilate3_kernel, dim3(grid_for(c, n)), dim3(256), 0, c.stream, d_failL, W, H, d_fail2);
