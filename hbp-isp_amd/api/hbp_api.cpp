@@ -1,8 +1,10 @@
 // Implementation of the C++ drop-in classes over the C ABI of libhbp_hip.so.
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <mutex>
 #include <stdexcept>
+#include <vector>
 
 #include "CRTrees.hpp"
 #include "hbp_isp.hpp"
@@ -106,12 +108,29 @@ StereoCost::StereoCost(const cv::Mat &img_left, const cv::Mat &img_right, const 
 }
 
 void StereoCost::compute_stereo_cost(const std::string algo) {
-    if (algo != "Census") throw std::invalid_argument("StereoCost: only the \"Census\" cost is implemented");
     auto h = std::make_shared<hbp_api::VolumeHandle>();
-    check(hbp_cost_volume(default_ctx(), imgL.data, imgR.data, width, height, m_labels, "Census", &h->v),
-          "StereoCost::compute_stereo_cost");
+    if (algo == "Census") {
+        check(hbp_cost_volume(default_ctx(), imgL.data, imgR.data, width, height, m_labels, "Census", &h->v),
+              "StereoCost::compute_stereo_cost");
+        m_tau = 25.f;
+    } else if (algo == "MC-CNN") {
+        // read_volume_from_file (stereo_cost.cu:399-419): vol_dir names a headerless f32
+        // [labels][height][width] file (loadMatBinary(..., readHeader = false))
+        const size_t n = (size_t)m_labels * height * width;
+        std::vector<float> raw(n);
+        std::ifstream f(vol_dir, std::ios::binary);
+        if (!f) throw std::runtime_error("StereoCost: cost volume file " + vol_dir + " not found");
+        f.read((char *)raw.data(), (std::streamsize)(n * sizeof(float)));
+        if ((size_t)f.gcount() != n * sizeof(float))
+            throw std::runtime_error("StereoCost: " + vol_dir + " holds fewer than labels*height*width floats");
+        check(hbp_cost_volume_f32(default_ctx(), raw.data(), width, height, m_labels, "MC-CNN", &h->v),
+              "StereoCost::compute_stereo_cost");
+        m_tau = 0.5f;
+    } else {
+        throw std::invalid_argument("StereoCost: the \"Census\" and \"MC-CNN\" costs are implemented (not \"" +
+                                    algo + "\")");
+    }
     vol = h;
-    m_tau = 25.f;
 }
 
 void StereoCost::cost_to_right() {

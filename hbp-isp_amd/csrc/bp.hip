@@ -307,6 +307,8 @@ struct SweepArgs {
     const uint32_t *cl, *cr;
     const uint32_t *crx;         // reversed, padded right census [H][crx_stride] (left view)
     int crx_stride;
+    const float *volf;           // a given f32 volume [D][H][W] (MC-CNN), else null
+    long vplane;                 // H W
     int right;
     const float *dwta;
     float lambda, tau, tau_s;
@@ -405,9 +407,10 @@ __device__ void spatial_node(const SweepArgs &a, long id) {
     for (int p = 0; p < P; ++p) a.plane[id * K + 2 * P + p] = spatial_candidate(a, id, p);
 }
 
-// Plane-cost modes: the cost volume of the left view, of the right view (cost_to_right), or
-// the WTA indicator of StereoData(use_cost_vol = 0).
-enum CostMode { kCostLeft = 0, kCostRight = 1, kCostWta = 2 };
+// Plane-cost modes: the census cost volume of the left view, of the right view
+// (cost_to_right), the WTA indicator of StereoData(use_cost_vol = 0), or a given f32 volume
+// (the "MC-CNN" cost, either view: the library holds it in the view's own orientation).
+enum CostMode { kCostLeft = 0, kCostRight = 1, kCostWta = 2, kCostF32 = 3 };
 
 // ----------------------------------------------------------------------------------------
 // read_plane_cost (hbp_isp.cu:177-228) of one pixel and plane, times lambda, as the term
@@ -504,6 +507,11 @@ __device__ __forceinline__ void pc_load(const SweepArgs &a, const PcPix &px, flo
         const Words2 w = ld_at<Words2>(a.crx, (px.pb + (uint32_t)dc) << 2);
         wd.u0 = w.lo;
         wd.u1 = w.hi;
+    } else if (MODE == kCostF32) {
+        // entries (dc, y, x) and (dc+1, y, x) of the f32 volume, as bits
+        const float *e = a.volf + (long)dc * a.vplane + px.row + px.x;
+        wd.u0 = __float_as_uint(e[0]);
+        wd.u1 = __float_as_uint(e[a.vplane]);
     } else {
         const int xa = px.x + dc < a.W - 1 ? px.x + dc : a.W - 1;
         const int xb = px.x + dc + 1 < a.W - 1 ? px.x + dc + 1 : a.W - 1;
@@ -536,15 +544,22 @@ __device__ __forceinline__ float pc_term(const SweepArgs &a, const PcPix &px, fl
         cost = fabsf(px.dw - df) <= 1.f ? 0.f : 1.f;
     } else if (!GEN) {
         const bool in = wd.tf >= 0.f;
-        int c0, c1;
-        if (MODE == kCostLeft) {
-            c0 = __popc(px.clx ^ wd.u0);
-            c1 = __popc(px.clx ^ wd.u1);
+        float v0, v1;
+        if (MODE == kCostF32) {
+            v0 = __uint_as_float(wd.u0);
+            v1 = __uint_as_float(wd.u1);
         } else {
-            c0 = __popc(wd.u0 ^ wd.w0);
-            c1 = __popc(wd.u1 ^ wd.w1);
+            int c0, c1;
+            if (MODE == kCostLeft) {
+                c0 = __popc(px.clx ^ wd.u0);
+                c1 = __popc(px.clx ^ wd.u1);
+            } else {
+                c0 = __popc(wd.u0 ^ wd.w0);
+                c1 = __popc(wd.u1 ^ wd.w1);
+            }
+            v0 = (float)(c0 < 25 ? c0 : 25);
+            v1 = (float)(c1 < 25 ? c1 : 25);
         }
-        const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
         const float t1 = wd.tf;
         float ci = (1.f - t1) * v0 + t1 * v1;
         pin(ci);   // a select, not a branch around the arithmetic
@@ -555,15 +570,22 @@ __device__ __forceinline__ float pc_term(const SweepArgs &a, const PcPix &px, fl
         bool in;
         if (GEN) in = a.D > 1 && !(fl < 0.f) && !(fl >= dlim);
         else in = fl == __builtin_amdgcn_fmed3f(fl, 0.f, dmax);
-        int c0, c1;
-        if (MODE == kCostLeft) {
-            c0 = __popc(px.clx ^ wd.u0);
-            c1 = __popc(px.clx ^ wd.u1);
+        float v0, v1;
+        if (MODE == kCostF32) {
+            v0 = __uint_as_float(wd.u0);
+            v1 = __uint_as_float(wd.u1);
         } else {
-            c0 = __popc(wd.u0 ^ wd.w0);
-            c1 = __popc(wd.u1 ^ wd.w1);
+            int c0, c1;
+            if (MODE == kCostLeft) {
+                c0 = __popc(px.clx ^ wd.u0);
+                c1 = __popc(px.clx ^ wd.u1);
+            } else {
+                c0 = __popc(wd.u0 ^ wd.w0);
+                c1 = __popc(wd.u1 ^ wd.w1);
+            }
+            v0 = (float)(c0 < 25 ? c0 : 25);
+            v1 = (float)(c1 < 25 ? c1 : 25);
         }
-        const float v0 = (float)(c0 < 25 ? c0 : 25), v1 = (float)(c1 < 25 ? c1 : 25);
         const float t1 = df - fl;
         float ci = (1.f - t1) * v0 + t1 * v1;
         pin(ci);   // a select, not a branch around the arithmetic
@@ -710,8 +732,9 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             // every cost is >= 0, so the sum takes lambda's sign
-            const unsigned long long m = (unsigned long long)wave_sum_exact(acc[k]);
-            if (lane == g0 + k) mine = a.lambda < 0.f ? 0ull - m : m;
+            // the terms carry the costs' signs and |lambda|; the sum takes lambda's sign
+            const long long m = (long long)wave_sum_exact(acc[k]);
+            if (lane == g0 + k) mine = (unsigned long long)(a.lambda < 0.f ? -m : m);
         }
     }
 #pragma unroll
@@ -763,8 +786,8 @@ __device__ __forceinline__ void pc_run_group(const SweepArgs &a, const uint32_t 
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             if (acc[k] != 0.0) {
-                const unsigned long long m = (unsigned long long)acc[k];
-                atomicAdd(&ssum[lr * P + k], neg ? 0ull - m : m);
+                const long long m = (long long)acc[k];
+                atomicAdd(&ssum[lr * P + k], (unsigned long long)(neg ? -m : m));
             }
             acc[k] = 0.0;
         }
@@ -1126,7 +1149,7 @@ struct BatchItem {
 #define SWEEP_ARGS_POINTERS(X)                                                                   \
     X(clus) X(bd) X(pxy) X(pcl) X(pix_off) X(seg) X(sego) X(off) X(rev) X(eloc) X(revl) X(pw) X(c0) \
     X(dpos) X(mxy) X(child) X(child_off) X(plane) X(cost) X(belief) X(point) X(bd_h) X(off_h)         \
-    X(plane_h) X(cl) X(cr) X(crx) X(dwta) X(msg) X(msg_t) X(acc) X(accnan) X(s_ransac) X(s_neighbor) X(s_child)
+    X(plane_h) X(cl) X(cr) X(crx) X(volf) X(dwta) X(msg) X(msg_t) X(acc) X(accnan) X(s_ransac) X(s_neighbor) X(s_child)
 
 __device__ __forceinline__ SweepArgs rebased_args(const SweepArgs *__restrict__ args, int i) {
     SweepArgs a = args[i];
@@ -1357,6 +1380,8 @@ static SweepArgs sweep_args(BP &b, int l) {
     a.cr = v.cr.p;
     a.crx = b.crx.p;
     a.crx_stride = b.crx_stride;
+    a.volf = v.f32src ? v.f32.p : nullptr;
+    a.vplane = (long)v.W * v.H;
     a.right = v.right ? 1 : 0;
     a.dwta = v.dwta.p;
     a.lambda = b.lambda;
@@ -1388,19 +1413,25 @@ void bp_setup(Ctx &c, BP &b) {
     b.tau = v.tau;
     HBP_REQUIRE(b.min_level >= 1 && b.min_level <= b.L - 1, "min_level must be in [1, levels-1]");
     HBP_REQUIRE(std::isfinite(b.lambda) && std::isfinite(b.tau_s), "alpha and tau_smooth must be finite");
-    b.fx_shift = fixed_point_shift(b.lambda, b.tau, b.use_cost_vol, b.N);
+    // the bound of the per-pixel costs: tau, or the largest |entry| of a given f32 volume
+    // (its entries are truncated at tau from above only)
+    const float cmax = std::max(std::fabs(b.tau), v.f32src ? v.absmax : 0.f);
+    HBP_REQUIRE(std::isfinite(cmax), "the cost volume holds infinite entries");
+    b.fx_shift = fixed_point_shift(b.lambda, cmax, b.use_cost_vol, b.N);
     HBP_REQUIRE(b.fx_shift > -60, "alpha too large for the fixed-point plane-cost sums");
     {
         const float lf = std::ldexp(std::fabs(b.lambda), b.fx_shift);
         HBP_REQUIRE(std::isfinite(lf) && (lf == 0.f || std::isnormal(lf)),
                     "alpha and the cost truncation give no usable fixed-point scale");
     }
-    // reversed, padded right census of the left-view plane cost
-    b.crx_stride = b.W + std::max(b.D - 1, 1);
-    b.crx.alloc(c.pool, (size_t)b.H * b.crx_stride);
-    hipLaunchKernelGGL(crx_kernel, dim3(grid_for(c, (long)b.H * b.crx_stride)), dim3(256), 0, c.stream, v.cr.p, b.W,
-                       b.H, b.crx_stride, b.crx.p);
-    HBP_CHECK(hipGetLastError());
+    // reversed, padded right census of the left-view plane cost (census volumes)
+    if (!v.f32src) {
+        b.crx_stride = b.W + std::max(b.D - 1, 1);
+        b.crx.alloc(c.pool, (size_t)b.H * b.crx_stride);
+        hipLaunchKernelGGL(crx_kernel, dim3(grid_for(c, (long)b.H * b.crx_stride)), dim3(256), 0, c.stream, v.cr.p,
+                           b.W, b.H, b.crx_stride, b.crx.p);
+        HBP_CHECK(hipGetLastError());
+    }
     b.lv.clear();
     b.lv.resize(b.L);
     long N = b.N;
@@ -1548,6 +1579,9 @@ static void launch_cost_edge_mode(Ctx &c, const SweepArgs &a, int mode, long blo
     else if (mode == kCostRight)
         hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostRight>), dim3(blocks), dim3(256), lds, c.stream, a,
                            slot0, spatial, npc);
+    else if (mode == kCostF32)
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostF32>), dim3(blocks), dim3(256), lds, c.stream, a,
+                           slot0, spatial, npc);
     else
         hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostWta>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
                            spatial, npc);
@@ -1558,7 +1592,7 @@ template <int NS>
 static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l, int slot0, bool spatial,
                              bool costs, bool with_edges) {
     long nedge_w = with_edges ? div_up(a.E, EDGES_PER_WAVE) : 0;
-    const int mode = !a.use_cost_vol ? kCostWta : (a.right ? kCostRight : kCostLeft);
+    const int mode = !a.use_cost_vol ? kCostWta : (a.volf ? kCostF32 : (a.right ? kCostRight : kCostLeft));
     // Coarse levels: one launch carries the segments' waves and the message waves (the
     // segments' latency chains and the messages' arithmetic overlap).  Fine levels and
     // message-only launches: the messages run in their own kernel, whose smaller register
@@ -1654,7 +1688,9 @@ void BatchRunner::init(Ctx &c, int slots) {
     mode = -1;
 }
 
-static int cost_mode(const BP &b) { return !b.use_cost_vol ? kCostWta : (b.vol->right ? kCostRight : kCostLeft); }
+static int cost_mode(const BP &b) {
+    return !b.use_cost_vol ? kCostWta : (b.vol->f32src ? kCostF32 : (b.vol->right ? kCostRight : kCostLeft));
+}
 
 int BatchRunner::add(Ctx &c, BP &b) {
     const int m = cost_mode(b);
@@ -1757,6 +1793,9 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
         else if (mode == kCostRight)
             hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, true>), dim3(coarse_blocks), dim3(256), 0,
                                c.stream, dargs, itc, ncoarse);
+        else if (mode == kCostF32)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostF32, true>), dim3(coarse_blocks), dim3(256), 0,
+                               c.stream, dargs, itc, ncoarse);
         else
             hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, true>), dim3(coarse_blocks), dim3(256), 0, c.stream,
                                dargs, itc, ncoarse);
@@ -1768,6 +1807,9 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
                                c.stream, dargs, itf, nfine);
         else if (mode == kCostRight)
             hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, false>), dim3(fine_blocks), dim3(256),
+                               PC_LDS_BYTES, c.stream, dargs, itf, nfine);
+        else if (mode == kCostF32)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostF32, false>), dim3(fine_blocks), dim3(256),
                                PC_LDS_BYTES, c.stream, dargs, itf, nfine);
         else
             hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), dim3(fine_blocks), dim3(256), PC_LDS_BYTES,

@@ -212,6 +212,69 @@ int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *l, const uint8_t *r, int W, int
     HBP_API_END
 }
 
+static float f32_algo_tau(const char *algo) {
+    HBP_REQUIRE(algo && std::strcmp(algo, "MC-CNN") == 0, "a given f32 volume is the \"MC-CNN\" cost");
+    return kMcCnnTau;
+}
+
+int hbp_cost_volume_f32_dev(hbp_ctx *ctx, const float *d_vol, int W, int H, int D, const char *algo,
+                            hbp_volume **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && out && d_vol, "NULL argument");
+    *out = nullptr;
+    const float tau = f32_algo_tau(algo);
+    use_device(ctx);
+    auto *v = new hbp_volume();
+    try {
+        volume_build_f32(ctx->c, v->v, d_vol, W, H, D, tau);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    } catch (...) {
+        delete v;
+        throw;
+    }
+    *out = v;
+    HBP_API_END
+}
+
+int hbp_cost_volume_f32(hbp_ctx *ctx, const float *vol, int W, int H, int D, const char *algo, hbp_volume **out) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && out && vol, "NULL argument");
+    *out = nullptr;
+    const float tau = f32_algo_tau(algo);
+    HBP_REQUIRE(W > 0 && H > 0 && D > 0 && (long)W * H < (1L << 24), "bad volume size");
+    use_device(ctx);
+    const size_t n = (size_t)W * H * D;
+    DevBuf<float> raw(ctx->c.pool, n);
+    HBP_CHECK(hipMemcpyAsync(raw.p, vol, sizeof(float) * n, hipMemcpyHostToDevice, ctx->c.stream));
+    auto *v = new hbp_volume();
+    try {
+        volume_build_f32(ctx->c, v->v, raw.p, W, H, D, tau);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    } catch (...) {
+        delete v;
+        throw;
+    }
+    *out = v;
+    HBP_API_END
+}
+
+int hbp_volume_download_f32(hbp_ctx *ctx, hbp_volume *vol, float *vh, float *wh) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && vol, "NULL argument");
+    use_device(ctx);
+    Volume &v = vol->v;
+    if (vh) {
+        const float *p = volume_f32(ctx->c, v);
+        HBP_CHECK(hipMemcpyAsync(vh, p, sizeof(float) * (size_t)v.W * v.H * v.D, hipMemcpyDeviceToHost,
+                                 ctx->c.stream));
+    }
+    if (wh)
+        HBP_CHECK(hipMemcpyAsync(wh, v.dwta.p, sizeof(float) * (size_t)v.W * v.H, hipMemcpyDeviceToHost,
+                                 ctx->c.stream));
+    HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    HBP_API_END
+}
+
 int hbp_cost_to_right(hbp_ctx *ctx, hbp_volume *vol) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && vol, "NULL argument");
@@ -252,6 +315,7 @@ int hbp_volume_download(hbp_ctx *ctx, const hbp_volume *vol, uint8_t *vh, float 
     HBP_REQUIRE(ctx && vol, "NULL argument");
     use_device(ctx);
     const Volume &v = vol->v;
+    HBP_REQUIRE(!(vh && v.f32src), "an f32 (MC-CNN) volume has no u8 form: use hbp_volume_download_f32");
     if (vh)
         HBP_CHECK(hipMemcpy2DAsync(vh, (size_t)v.W, v.vol.p, (size_t)v.pitch, (size_t)v.W, (size_t)v.H * v.D,
                                    hipMemcpyDeviceToHost, ctx->c.stream));

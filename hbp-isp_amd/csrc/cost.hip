@@ -4,6 +4,8 @@
 // every H*pitch bytes.  Costs are min(Hamming, 25) <= 25, so u8 holds the reference's f32
 // values exactly at a quarter of the bytes; the plane-cost gathers of BP read two bytes per
 // pixel and plane from this layout.
+#include <cstring>
+
 #include "hbp_internal.h"
 
 namespace hbp {
@@ -150,6 +152,59 @@ __global__ __launch_bounds__(HM_THREADS) void hamming_wta_kernel(const uint32_t 
     }
 }
 
+// ----------------------------------------------------------------------------------------
+// a given f32 volume [D][H][W] (the "MC-CNN" cost: read_volume_from_file, stereo_cost.cu:
+// 399-419): the left view's out-of-view columns x <= d take column d's value
+// (fill_out_of_view_k, 366-395), then every entry is truncated at tau with fminf
+// (truncate_data_k, 230-246); one pass from the given volume.
+// ----------------------------------------------------------------------------------------
+__global__ void f32_volume_kernel(const float *__restrict__ raw, int W, int H, int D, float tau,
+                                  float *__restrict__ out) {
+    const long hw = (long)W * H, total = hw * D;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long d = i / hw, r = i - d * hw;
+        const long y = r / W, x = r - y * W;
+        const float v = x <= d ? raw[d * hw + y * W + d] : raw[i];
+        out[i] = fminf(v, tau);
+    }
+}
+// the right view (convert_vol_L2R_k + fill_out_of_view_k(false), stereo_cost.cu:471-516):
+// C_R[d][y][x] = C_L[d][y][min(x+d, W-1)]
+__global__ void f32_to_right_kernel(const float *__restrict__ in, int W, int H, int D, float *__restrict__ out) {
+    const long hw = (long)W * H, total = hw * D;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long d = i / hw, r = i - d * hw;
+        const long y = r / W, x = r - y * W;
+        const long xs = x + d < W - 1 ? x + d : W - 1;
+        out[i] = in[d * hw + y * W + xs];
+    }
+}
+// the largest |entry| (as float bits: non-negative floats order as integers; +inf included)
+__global__ void f32_absmax_kernel(const float *__restrict__ vol, long total, unsigned int *__restrict__ out) {
+    unsigned int m = 0u;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(fabsf(vol[i])));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+// WTA_D_k (stereo_cost.cu:248-266): the first d of the minimum under <, starting from FLT_MAX
+__global__ void f32_wta_kernel(const float *__restrict__ vol, int W, int H, int D, float *__restrict__ dwta) {
+    const long hw = (long)W * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < hw; i += (long)gridDim.x * blockDim.x) {
+        float mc = 3.402823466e+38f;
+        int mi = 0;
+        for (int d = 0; d < D; ++d) {
+            const float c = vol[(long)d * hw + i];
+            if (c < mc) {
+                mc = c;
+                mi = d;
+            }
+        }
+        dwta[i] = (float)mi;
+    }
+}
+
 // f32 [D][H][W] copy of the volume for the reference's float* getter (stereo_cost.hpp:42-46).
 __global__ void volume_to_f32_kernel(const uint8_t *__restrict__ vol, int W, int H, int D, int pitch, long plane,
                                      float *__restrict__ out) {
@@ -209,7 +264,55 @@ void volume_build(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgr
     hamming_launch(c, v, false);
 }
 
+static int grid_of(const Ctx &c, long n) {
+    const long b = (n + 255) / 256, cap = (long)c.num_cu * 16;
+    return (int)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D, float tau) {
+    HBP_REQUIRE(W >= 9 && H >= 7, "image must be at least 9x7");
+    HBP_REQUIRE(D >= 1 && D <= 1024, "max disparity must be in [1, 1024]");
+    HBP_REQUIRE(D <= W, "max disparity must not exceed the image width");
+    HBP_REQUIRE((long)W * H < (1L << 24), "images are limited to 2^24 pixels (ids are held in floats)");
+    HBP_REQUIRE(W < 65536 && H < 65536, "image side must be < 65536");
+    v.W = W;
+    v.H = H;
+    v.D = D;
+    v.pitch = W;
+    v.plane = (long)W * H;
+    v.tau = tau;
+    v.right = false;
+    v.f32src = true;
+    const long n = (long)W * H;
+    v.f32.alloc(c.pool, (size_t)n * D);
+    v.dwta.alloc(c.pool, (size_t)n);
+    ProfScope ps(c, "volume_f32", 8.0 * n * D + 4.0 * n);
+    hipLaunchKernelGGL(f32_volume_kernel, dim3(grid_of(c, n * D)), dim3(256), 0, c.stream, d_raw, W, H, D, tau,
+                       v.f32.p);
+    hipLaunchKernelGGL(f32_wta_kernel, dim3(grid_of(c, n)), dim3(256), 0, c.stream, v.f32.p, W, H, D, v.dwta.p);
+    DevBuf<unsigned int> am(c.pool, 1);
+    HBP_CHECK(hipMemsetAsync(am.p, 0, sizeof(unsigned int), c.stream));
+    hipLaunchKernelGGL(f32_absmax_kernel, dim3(grid_of(c, n * D)), dim3(256), 0, c.stream, v.f32.p, n * D, am.p);
+    HBP_CHECK(hipGetLastError());
+    unsigned int bits = 0;
+    HBP_CHECK(hipMemcpyAsync(&bits, am.p, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    std::memcpy(&v.absmax, &bits, 4);
+}
+
 void volume_to_right(Ctx &c, Volume &v) {
+    if (v.f32src) {
+        const long n = (long)v.W * v.H;
+        DevBuf<float> r(c.pool, (size_t)n * v.D);
+        hipLaunchKernelGGL(f32_to_right_kernel, dim3(grid_of(c, n * v.D)), dim3(256), 0, c.stream, v.f32.p, v.W,
+                           v.H, v.D, r.p);
+        hipLaunchKernelGGL(f32_wta_kernel, dim3(grid_of(c, n)), dim3(256), 0, c.stream, r.p, v.W, v.H, v.D,
+                           v.dwta.p);
+        HBP_CHECK(hipGetLastError());
+        std::swap(v.f32, r);
+        v.right = true;
+        return;
+    }
     hamming_launch(c, v, true);
     v.right = true;
     v.f32.release();

@@ -10,10 +10,12 @@ struct Volume {
     long plane = 0;          // bytes between disparity planes (H * pitch)
     float tau = kCensusTau;  // truncation (stereo_cost.cu:433)
     bool right = false;      // cost_to_right applied
+    bool f32src = false;     // a given f32 volume (MC-CNN): f32 holds it, there is no u8 volume
+    float absmax = 0.f;      // f32src: the largest |entry| after truncation (bounds the plane costs)
     DevBuf<uint32_t> cl, cr; // census words of the left / right image [H][W]
     DevBuf<uint8_t> vol;     // u8 [D][H][pitch]
     DevBuf<float> dwta;      // WTA disparity [H][W]
-    DevBuf<float> f32;       // f32 [D][H][W], materialised only for the float* getter
+    DevBuf<float> f32;       // f32 [D][H][W]: the given volume (f32src), else made for the float* getter
 };
 
 // Superpixel pyramid in the reference's packed ISP format (segment.cu:884-905): per level l,
@@ -91,6 +93,7 @@ struct BP {
 
 // cost.hip
 void volume_build(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D);
+void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D, float tau);
 void volume_to_right(Ctx &c, Volume &v);
 float *volume_f32(Ctx &c, Volume &v);
 

@@ -48,6 +48,9 @@ SIGNATURES = {
     "hbp_volume_dev_f32": (_vp, [_vp, _vp]),
     "hbp_volume_dev_u8": (_vp, [_vp]),
     "hbp_volume_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "hbp_cost_volume_f32": (C.c_int, [_vp, _f32, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
+    "hbp_cost_volume_f32_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
+    "hbp_volume_download_f32": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hbp_volume_free": (C.c_int, [_vp, _vp]),
     "hbp_pyramid_build": (C.c_int, [_vp, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _pp]),
     "hbp_pyramid_build_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _pp]),
@@ -161,6 +164,10 @@ class Ctx:
     def cost_volume(self, left, right, D, right_view=False):
         return Volume(self, left, right, D, right_view)
 
+    def cost_volume_f32(self, vol, right_view=False):
+        """the "MC-CNN" cost: a given f32 volume [D][H][W]"""
+        return Volume(self, None, None, vol.shape[0], right_view, f32=vol)
+
     # ---- (b)
     def pyramid(self, bgr, num_nb=4):
         return Pyramid(self, bgr, num_nb)
@@ -210,12 +217,18 @@ class Ctx:
 
 
 class Volume:
-    def __init__(self, ctx: Ctx, left, right, D, right_view=False):
+    def __init__(self, ctx: Ctx, left, right, D, right_view=False, f32=None):
         self.ctx = ctx
-        H, W = left.shape[:2]
         h = C.c_void_p()
-        _check(ctx.lib.hbp_cost_volume(ctx.h, np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
-                                       b"Census", C.byref(h)))
+        if f32 is not None:
+            D, H, W = f32.shape
+            _check(ctx.lib.hbp_cost_volume_f32(ctx.h, np.ascontiguousarray(f32, np.float32), W, H, D, b"MC-CNN",
+                                               C.byref(h)))
+        else:
+            H, W = left.shape[:2]
+            _check(ctx.lib.hbp_cost_volume(ctx.h, np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
+                                           b"Census", C.byref(h)))
+        self.f32 = f32 is not None
         self.h = h
         self.W, self.H, self.D = W, H, D
         if right_view:
@@ -228,6 +241,12 @@ class Volume:
 
     def __del__(self):
         self.free()
+
+    def download_f32(self):
+        vol = np.zeros((self.D, self.H, self.W), np.float32)
+        wta = np.zeros((self.H, self.W), np.float32)
+        _check(self.ctx.lib.hbp_volume_download_f32(self.ctx.h, self.h, _np_ptr(vol), _np_ptr(wta)))
+        return vol, wta
 
     def download(self):
         vol = np.zeros((self.D, self.H, self.W), np.uint8)

@@ -54,6 +54,17 @@ int hbp_cost_volume_dev(hbp_ctx *ctx, const uint8_t *d_bgr_left, const uint8_t *
                         int width, int height, int labels, const char *algo, hbp_volume **out);
 /* StereoCost::cost_to_right (stereo_cost.cu:488-516): the volume and WTA of the right view */
 int hbp_cost_to_right(hbp_ctx *ctx, hbp_volume *vol);
+/* compute_stereo_cost("MC-CNN") (stereo_cost.cu:399-445): a given f32 volume [D][H][W] (the
+ * .acrt file of read_volume_from_file, headerless), its left out-of-view columns filled, every
+ * entry truncated at tau = 0.5, then WTA; host buffer (algo must be "MC-CNN") */
+int hbp_cost_volume_f32(hbp_ctx *ctx, const float *vol, int width, int height, int labels, const char *algo,
+                        hbp_volume **out);
+/* same from a device buffer (not kept: the library holds its own truncated copy) */
+int hbp_cost_volume_f32_dev(hbp_ctx *ctx, const float *d_vol, int width, int height, int labels,
+                            const char *algo, hbp_volume **out);
+/* the volume as f32 [D][H][W] (the reference's float getter layout) and the WTA disparity;
+ * either pointer may be NULL */
+int hbp_volume_download_f32(hbp_ctx *ctx, hbp_volume *vol, float *vol_host, float *wta_host);
 /* getters g_labels / g_im_size / g_tau (stereo_cost.hpp:27-41) and the layout */
 int hbp_volume_info(const hbp_volume *vol, int *width, int *height, int *labels, float *tau,
                     int *pitch);

@@ -31,6 +31,8 @@ void orc_hamming(const uint32_t *cl, const uint32_t *cr, int W, int H, int D, in
                  uint8_t *vol);
 void orc_wta(const uint8_t *vol, int W, int H, int D, float *dwta);
 void orc_vol_to_right(uint8_t *vol, int W, int H, int D);
+int orc_cost_volume_f32(const float *raw, int W, int H, int D, float tau, int right_view, float *vol,
+                        float *dwta);
 int orc_cost_volume(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D,
                     int right_view, uint8_t *vol, float *dwta);
 
@@ -50,6 +52,8 @@ void orc_pyr_free(void *p);
 /* (c) hierarchical BP on the pyramid */
 void *orc_bp_new(void *pyr, const uint8_t *vol, const float *dwta, int W, int H, int D,
                  float tau, float alpha, float tau_s, int min_level, int use_cost_vol);
+void *orc_bp_new_f32(void *pyr, const float *vol, const float *dwta, int W, int H, int D, float tau,
+                     float alpha, float tau_s, int min_level, int use_cost_vol);
 int orc_bp_step(void *bp, int max_sweeps);
 int orc_bp_level(void *bp);
 int orc_bp_sweeps_done(void *bp);

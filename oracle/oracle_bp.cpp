@@ -47,6 +47,7 @@ struct BPLevel {
 struct BP {
     const Pyramid *pyr;
     const uint8_t *vol;
+    const float *volf;   // a given f32 volume (MC-CNN), used instead of vol when set
     const float *dwta;
     int W, H, D, L, min_level, use_cost_vol;
     long N;
@@ -153,8 +154,8 @@ static inline float plane_px_cost(const BP &b, const F4 &pl, int pxid) {
     if (d < 0 || (long long)d + 1 >= b.D) {
         c = b.tau;
     } else {
-        float c0 = (float)b.vol[(long)d * b.N + pxid];
-        float c1 = (float)b.vol[(long)(d + 1) * b.N + pxid];
+        float c0 = b.volf ? b.volf[(long)d * b.N + pxid] : (float)b.vol[(long)d * b.N + pxid];
+        float c1 = b.volf ? b.volf[(long)(d + 1) * b.N + pxid] : (float)b.vol[(long)(d + 1) * b.N + pxid];
         c = ((float)(d + 1) - df) * c0 + (df - (float)d) * c1;
     }
     return b.lambda * c;
@@ -574,6 +575,7 @@ void *orc_bp_new(void *pyr, const uint8_t *vol, const float *dwta, int W, int H,
     auto *b = new orc::BP();
     b->pyr = (const orc::Pyramid *)pyr;
     b->vol = vol;
+    b->volf = nullptr;
     b->dwta = dwta;
     b->W = W;
     b->H = H;
@@ -592,6 +594,23 @@ void *orc_bp_new(void *pyr, const uint8_t *vol, const float *dwta, int W, int H,
     b->fx_shift = orc::fixed_shift(alpha, tau, use_cost_vol, (int)b->N);
     orc::prologue(*b);
     b->cur = b->L - 2;
+    return b;
+}
+// the same on a given f32 volume [D][H][W] (MC-CNN, tau 0.5)
+void *orc_bp_new_f32(void *pyr, const float *vol, const float *dwta, int W, int H, int D, float tau, float alpha,
+                     float tau_s, int min_level, int use_cost_vol) {
+    auto *b = (orc::BP *)orc_bp_new(pyr, nullptr, dwta, W, H, D, tau, alpha, tau_s, min_level, use_cost_vol);
+    if (!b) return nullptr;
+    b->volf = vol;
+    // the bound of the per-pixel costs: tau, or the largest |entry| (truncated from above only)
+    float amax = 0.f;
+    for (long i = 0; i < (long)D * W * H; ++i) amax = std::fmax(amax, std::fabs(vol[i]));
+    const float cmax = std::fmax(std::fabs(tau), amax);
+    if (!std::isfinite(cmax)) {
+        orc_bp_free(b);
+        return nullptr;
+    }
+    b->fx_shift = orc::fixed_shift(alpha, cmax, use_cost_vol, (int)b->N);
     return b;
 }
 int orc_bp_step(void *bp, int max_sweeps) { return orc::step(*(orc::BP *)bp, max_sweeps); }

@@ -1,5 +1,7 @@
 // Oracle (test infrastructure only): stage (a), census transform + Hamming cost volume.
 // Restates stereo_cost.cu and imutil.cu of the reference (citations per function).
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -7,6 +9,42 @@
 #include "oracle_math.h"
 
 extern "C" {
+
+// compute_stereo_cost("MC-CNN") (stereo_cost.cu:399-445): the given f32 volume [D][H][W], the
+// left view's columns x <= d set to column d (fill_out_of_view_k, 366-395), every entry
+// truncated with fminf at tau (truncate_data_k, 230-246), then WTA_D_k (248-266: the first d
+// of the minimum under <, starting from FLT_MAX).  right_view: cost_to_right (471-516),
+// C_R[d][y][x] = C_L[d][y][min(x+d, W-1)], and its WTA.
+int orc_cost_volume_f32(const float *raw, int W, int H, int D, float tau, int right_view, float *vol,
+                        float *dwta) {
+    if (W <= 0 || H <= 0 || D < 1 || D > W) return -1;
+    const long hw = (long)W * H;
+    std::vector<float> left((size_t)hw * D);
+    for (int d = 0; d < D; ++d)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const long i = (long)d * hw + (long)y * W + x;
+                const float v = x <= d ? raw[(long)d * hw + (long)y * W + d] : raw[i];
+                left[i] = std::fmin(v, tau);
+            }
+    for (int d = 0; d < D; ++d)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const int xs = right_view ? std::min(x + d, W - 1) : x;
+                vol[(long)d * hw + (long)y * W + x] = left[(long)d * hw + (long)y * W + xs];
+            }
+    for (long i = 0; i < hw; ++i) {
+        float mc = 3.402823466e+38f;
+        int mi = 0;
+        for (int d = 0; d < D; ++d)
+            if (vol[(long)d * hw + i] < mc) {
+                mc = vol[(long)d * hw + i];
+                mi = d;
+            }
+        dwta[i] = (float)mi;
+    }
+    return 0;
+}
 
 // imutil.cu:116-130 (grey = B*0.114 + G*0.587 + R*0.299 in FP64, stored as f32) followed by
 // imutil.cu:154-168 (u8 = f < 255 ? trunc(f) : 255).

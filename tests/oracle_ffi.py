@@ -45,6 +45,9 @@ def _load():
         "orc_pyr_free": (None, [C.c_void_p]),
         "orc_bp_new": (C.c_void_p, [C.c_void_p, _u8p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float,
                                     C.c_float, C.c_float, C.c_int, C.c_int]),
+        "orc_bp_new_f32": (C.c_void_p, [C.c_void_p, _f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                        C.c_float, C.c_float, C.c_int, C.c_int]),
+        "orc_cost_volume_f32": (C.c_int, [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _f32p, _f32p]),
         "orc_bp_step": (C.c_int, [C.c_void_p, C.c_int]),
         "orc_bp_level": (C.c_int, [C.c_void_p]),
         "orc_bp_sweeps_done": (C.c_int, [C.c_void_p]),
@@ -111,13 +114,26 @@ class Pyramid:
         return dict(V=V, E=E, clus=clus, mean=mean.reshape(6, V), bd=bd.reshape(E, 2), bdl=bdl)
 
 
+def cost_volume_f32(raw, tau=0.5, right_view=False):
+    D, H, W = raw.shape
+    vol = np.zeros((D, H, W), np.float32)
+    dwta = np.zeros((H, W), np.float32)
+    assert lib.orc_cost_volume_f32(np.ascontiguousarray(raw, np.float32), W, H, D, tau, int(right_view), vol,
+                                   dwta) == 0
+    return vol, dwta
+
+
 class BP:
     def __init__(self, pyr: Pyramid, vol, dwta, D, alpha=1000.1, tau_s=0.6, min_level=2,
-                 use_cost_vol=1, tau=25.0):
+                 use_cost_vol=1, tau=None):
         self.pyr, self.vol, self.dwta = pyr, np.ascontiguousarray(vol), np.ascontiguousarray(dwta)
         self.W, self.H, self.D = pyr.W, pyr.H, D
-        self.h = lib.orc_bp_new(pyr.h, self.vol, self.dwta, pyr.W, pyr.H, D, tau, alpha, tau_s,
-                                min_level, use_cost_vol)
+        if self.vol.dtype == np.float32:
+            self.h = lib.orc_bp_new_f32(pyr.h, self.vol, self.dwta, pyr.W, pyr.H, D, 0.5 if tau is None else tau,
+                                        alpha, tau_s, min_level, use_cost_vol)
+        else:
+            self.h = lib.orc_bp_new(pyr.h, self.vol, self.dwta, pyr.W, pyr.H, D, 25.0 if tau is None else tau,
+                                    alpha, tau_s, min_level, use_cost_vol)
         assert self.h, "invalid min_level"
 
     def __del__(self):

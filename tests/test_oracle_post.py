@@ -1,7 +1,9 @@
 """Oracle pins for the refinements after the left-right check: the weighted median of planes
-(post_process.cu:127-185) and the CLI's 7-tap separable median (main_stereo_hbp_isp.cu:120-193).
-Each oracle function is checked against an independent statement of the reference's own
-arithmetic (double sums in the reference's orders, libm exp), written here in numpy."""
+(post_process.cu:127-185) and the CLI's 7-tap separable median (main_stereo_hbp_isp.cu:120-193);
+and for the "MC-CNN" cost (a given f32 volume, stereo_cost.cu:399-445).  Each oracle function is
+checked against an independent statement of the reference's own arithmetic (double sums in the
+reference's orders, libm exp, the volume rules), written here in numpy, or against the pinned
+census path."""
 import math
 
 import numpy as np
@@ -145,3 +147,39 @@ def test_median7_is_the_separable_median_for_finite_values():
     ref = D.copy()
     ref[3:H - 3, 3:W - 3] = np.sort(win2, axis=-1)[:, 3:W - 3, 3]
     assert np.array_equal(got, ref)
+
+
+# ---- the "MC-CNN" cost: a given f32 volume (stereo_cost.cu:399-445, 471-516) ----
+
+def test_f32_volume_fill_truncate_wta_from_definition():
+    rng = np.random.default_rng(11)
+    D, H, W = 12, 9, 20
+    raw = rng.random((D, H, W)).astype(np.float32) * 2 - 0.5
+    raw[rng.random(raw.shape) < 0.02] = np.float32(np.nan)
+    vol, wta = O.cost_volume_f32(raw, 0.5)
+    ref = raw.copy()
+    for d in range(D):
+        ref[d, :, :d + 1] = raw[d, :, d:d + 1]           # columns x <= d take column d
+    ref = np.fmin(ref, np.float32(0.5))                  # NaN -> 0.5
+    assert np.array_equal(vol, ref)
+    assert np.array_equal(wta, np.argmin(ref, axis=0).astype(np.float32))
+    volr, wtar = O.cost_volume_f32(raw, 0.5, right_view=True)
+    xs = np.arange(W)
+    for d in range(D):
+        assert np.array_equal(volr[d], ref[d][:, np.minimum(xs + d, W - 1)])
+    assert np.array_equal(wtar, np.argmin(volr, axis=0).astype(np.float32))
+
+
+def test_bp_on_an_f32_copy_of_the_census_volume_is_the_census_bp():
+    from hbp_amd import synth
+    left, right, _ = synth.make_pair(9, 80, 60, 24, (3, 6))
+    vol, wta = O.cost_volume(left, right, 24)
+    pyr = O.Pyramid(left)
+    a = O.BP(pyr, vol, wta, 24, min_level=2)
+    b = O.BP(pyr, vol.astype(np.float32), wta, 24, min_level=2, tau=25.0)
+    a.step()
+    b.step()
+    da, pa = a.map()
+    db, pb = b.map()
+    assert np.array_equal(da.view(np.uint32), db.view(np.uint32))
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
