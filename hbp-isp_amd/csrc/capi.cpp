@@ -603,6 +603,22 @@ int hbp_weighted_median(hbp_ctx *ctx, const uint8_t *bgr, const uint8_t *fail, f
     HBP_API_END
 }
 
+int hbp_eval_bad_pixels(hbp_ctx *ctx, const float *disp, const float *gt, const uint8_t *nonocc, int W, int H,
+                        double threshold, long long *counts) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && disp && gt && nonocc && counts && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    size_t n = (size_t)W * H;
+    hipStream_t s = ctx->c.stream;
+    DevBuf<float> d(ctx->c.pool, 2 * n);
+    DevBuf<uint8_t> m(ctx->c.pool, n);
+    HBP_CHECK(hipMemcpyAsync(d.p, disp, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    HBP_CHECK(hipMemcpyAsync(d.p + n, gt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    HBP_CHECK(hipMemcpyAsync(m.p, nonocc, n, hipMemcpyHostToDevice, s));
+    eval_bad_pixels(ctx->c, d.p, d.p + n, m.p, W, H, threshold, counts);
+    HBP_API_END
+}
+
 int hbp_median7_dev(hbp_ctx *ctx, float *d_disp, int W, int H) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && d_disp && W > 0 && H > 0, "bad argument");

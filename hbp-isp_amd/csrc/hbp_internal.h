@@ -132,6 +132,8 @@ void lr_check_fill(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_
 void plane_to_disp(Ctx &c, const float *d_plane3, int W, int H, float *d_disp);
 void weighted_median(Ctx &c, const uint8_t *d_bgr, const uint8_t *d_fail, float *d_plane3, int W, int H);
 void median7_filter(Ctx &c, float *d_disp, int W, int H);
+void eval_bad_pixels(Ctx &c, const float *d_disp, const float *d_gt, const uint8_t *d_nonocc, int W, int H,
+                     double thr, long long *counts);
 
 // primitives.hip
 void scan_exclusive_int(Ctx &c, const int *d_in, int *d_out, long n, int *d_total);

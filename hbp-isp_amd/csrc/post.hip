@@ -199,6 +199,36 @@ __global__ void med7_v_kernel(float *__restrict__ D, const float *__restrict__ T
     }
 }
 
+// Evaluator::evaluate (Evaluator.h:46-52, 66-78): valid = GT > 0 and GT != +inf, nonocc as
+// given, occ = valid and not nonocc; a pixel is good when |disp - GT| <= threshold, all in
+// float (cv::compare converts the double threshold to the float Mat's depth; NaN is never
+// good).  counts: valid, good & valid, nonocc, good & nonocc, occ, good & occ.
+__global__ void eval_kernel(const float *__restrict__ disp, const float *__restrict__ gt,
+                            const uint8_t *__restrict__ nonocc, long n, float thr,
+                            unsigned long long *__restrict__ counts) {
+    unsigned int c[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float g = gt[i];
+        const bool valid = g > 0.0f && g != INFINITY;
+        const bool no = nonocc[i] != 0;
+        const bool occ = valid && !no;
+        const bool good = fabsf(disp[i] - g) <= thr;
+        c[0] += valid;
+        c[1] += valid && good;
+        c[2] += no;
+        c[3] += no && good;
+        c[4] += occ;
+        c[5] += occ && good;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        unsigned int v = c[k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v += (unsigned int)__shfl_xor((int)v, o);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counts[k], (unsigned long long)v);
+    }
+}
+
 int grid_of(const Ctx &c, long n) {
     const long b = (n + 255) / 256, cap = (long)c.num_cu * 16;
     return (int)(b < 1 ? 1 : (b > cap ? cap : b));
@@ -226,6 +256,20 @@ void weighted_median(Ctx &c, const uint8_t *d_bgr, const uint8_t *d_fail, float 
                            count.p, W, H);
     }
     HBP_CHECK(hipGetLastError());
+}
+
+void eval_bad_pixels(Ctx &c, const float *d_disp, const float *d_gt, const uint8_t *d_nonocc, int W, int H,
+                     double thr, long long *counts) {
+    const long n = (long)W * H;
+    DevBuf<unsigned long long> cnt(c.pool, 6);
+    HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 6, c.stream));
+    hipLaunchKernelGGL(eval_kernel, dim3(grid_of(c, n)), dim3(256), 0, c.stream, d_disp, d_gt, d_nonocc, n, (float)thr,
+                       cnt.p);
+    HBP_CHECK(hipGetLastError());
+    unsigned long long h[6];
+    HBP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 6; ++k) counts[k] = (long long)h[k];
 }
 
 // disparity [H][W] in place

@@ -85,6 +85,8 @@ SIGNATURES = {
     "hbp_weighted_median_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int]),
     "hbp_median7": (C.c_int, [_vp, _f32, C.c_int, C.c_int]),
     "hbp_median7_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int]),
+    "hbp_eval_bad_pixels": (C.c_int, [_vp, _f32, _f32, _u8, C.c_int, C.c_int, C.c_double,
+                                      C.POINTER(C.c_longlong)]),
     "hbp_lr_check_fill": (C.c_int, [_vp, _f32, _f32, _f32, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hbp_lr_check_fill_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hbp_plane_to_disp": (C.c_int, [_vp, _f32, C.c_int, C.c_int, _f32]),
@@ -201,6 +203,17 @@ class Ctx:
         _check(self.lib.hbp_weighted_median(self.h, np.ascontiguousarray(bgr, np.uint8),
                                             np.ascontiguousarray(fail, np.uint8), out, W, H))
         return out
+
+    def eval_bad_pixels(self, disp, gt, nonocc, threshold):
+        """Evaluator::evaluate's counts: (valid, good), (nonocc, good), (occ, good)"""
+        H, W = gt.shape
+        if np.shape(disp) != (H, W) or np.shape(nonocc) != (H, W):
+            raise ValueError("eval_bad_pixels: disp, gt and nonocc must all be [H][W]")
+        cnt = (C.c_longlong * 6)()
+        _check(self.lib.hbp_eval_bad_pixels(self.h, np.ascontiguousarray(disp, np.float32),
+                                            np.ascontiguousarray(gt, np.float32),
+                                            np.ascontiguousarray(nonocc, np.uint8), W, H, float(threshold), cnt))
+        return [(cnt[0], cnt[1]), (cnt[2], cnt[3]), (cnt[4], cnt[5])]
 
     def median7(self, disp):
         """the CLI's 7-tap separable median (a new array)"""

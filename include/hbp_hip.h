@@ -187,6 +187,14 @@ int hbp_median7(hbp_ctx *ctx, float *disp, int width, int height);
 /* same on a device buffer */
 int hbp_median7_dev(hbp_ctx *ctx, float *d_disp, int width, int height);
 
+/* ---- Middlebury evaluation: Evaluator::evaluate (Evaluator.h:46-52, 66-78).  Over
+ * valid = (GT > 0 and GT != +inf), nonocc = (mask != 0) and occ = (valid and not nonocc):
+ * counts[6] = {valid, valid and good, nonocc, nonocc and good, occ, occ and good}, a pixel
+ * being good when |disp - GT| <= (float)threshold, in float.  Host buffers f32 [H][W] and
+ * u8 [H][W]; the bad-pixel rates are 100 (1 - good / pixels) per mask. */
+int hbp_eval_bad_pixels(hbp_ctx *ctx, const float *disp, const float *gt, const uint8_t *nonocc, int width,
+                        int height, double threshold, long long *counts);
+
 #ifdef __cplusplus
 }
 #endif

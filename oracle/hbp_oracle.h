@@ -78,6 +78,9 @@ void orc_left_right_check(const float *dispL, const float *dispR, float *planeL,
 float orc_wm_weight(float absdiff);
 void orc_weighted_median(const uint8_t *bgr, const uint8_t *fail, float *plane3, int W, int H);
 void orc_median7(float *disp, int W, int H);
+/* Evaluator.h:49-76: counts {valid, good & valid, nonocc, good & nonocc, occ, good & occ} */
+void orc_eval_bad_pixels(const float *disp, const float *gt, const uint8_t *nonocc, int W, int H, double thr,
+                         long long *counts);
 
 /* whole plain path (left view) for the CPU baseline */
 int orc_plain_pair(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D, float alpha,

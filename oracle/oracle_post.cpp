@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): the refinements after the left-right check, restating
 // PostProcess::median_filter (post_process.cu:127-185, weighted median of planes) and the
-// CLI's 7-tap separable median (main_stereo_hbp_isp.cu:120-193).
+// CLI's 7-tap separable median (main_stereo_hbp_isp.cu:120-193); and the Middlebury
+// bad-pixel counts of Evaluator (Evaluator.h:49-76).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -130,6 +131,29 @@ void orc_median7(float *D, int W, int H) {
                 D[(long)v * W + u] = median7_sorted(w);
             }
         }
+}
+
+// Evaluator.h:49-76 with OpenCV's element rules: validMask = (GT > 0) & (GT != inf),
+// occMask = ~nonocc & validMask, errorMap = |disp - GT| <= threshold where cv::compare takes the
+// double threshold in the float Mat's depth (a NaN difference is never good).  The nonocc counts
+// are over the whole mask, valid or not, as the reference counts them.
+void orc_eval_bad_pixels(const float *disp, const float *gt, const uint8_t *nonocc, int W, int H, double thr,
+                         long long *counts) {
+    const float t = (float)thr;
+    for (int k = 0; k < 6; ++k) counts[k] = 0;
+    for (long i = 0; i < (long)W * H; ++i) {
+        const float g = gt[i];
+        const bool valid = g > 0.0f && g != INFINITY;
+        const bool no = nonocc[i] != 0;
+        const bool occ = !no && valid;
+        const bool good = std::fabs(disp[i] - g) <= t;
+        counts[0] += valid;
+        counts[1] += good && valid;
+        counts[2] += no;
+        counts[3] += good && no;
+        counts[4] += occ;
+        counts[5] += good && occ;
+    }
 }
 
 }  // extern "C"

@@ -14,6 +14,7 @@ _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 
 def _load():
@@ -64,6 +65,7 @@ def _load():
         "orc_wm_weight": (C.c_float, [C.c_float]),
         "orc_weighted_median": (None, [_u8p, _u8p, _f32p, C.c_int, C.c_int]),
         "orc_median7": (None, [_f32p, C.c_int, C.c_int]),
+        "orc_eval_bad_pixels": (None, [_f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_double, _i64p]),
         "orc_plain_pair": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                      C.c_int, C.c_int, _f32p, _f32p, C.POINTER(C.c_int)]),
     }
@@ -176,3 +178,12 @@ def plain_pair(left, right, D, alpha=1000.1, tau_s=0.6, min_level=2, bp_levels=0
     ml = lib.orc_plain_pair(np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D, alpha,
                             tau_s, min_level, bp_levels, disp, plane3, C.byref(L))
     return disp, plane3, L.value, ml
+
+
+def eval_bad_pixels(disp, gt, nonocc, threshold):
+    """[(valid, good), (nonocc, good), (occ, good)] as the oracle counts them."""
+    H, W = gt.shape
+    c = np.zeros(6, np.int64)
+    lib.orc_eval_bad_pixels(np.ascontiguousarray(disp, np.float32), np.ascontiguousarray(gt, np.float32),
+                            np.ascontiguousarray(nonocc, np.uint8), W, H, float(threshold), c)
+    return [(int(c[0]), int(c[1])), (int(c[2]), int(c[3])), (int(c[4]), int(c[5]))]
