@@ -56,9 +56,9 @@ int CRTrees::compact_clus_label(int *clus_d, const int N) {
 // ------------------------------------------------------------------ SegHAC
 SegHAC::SegHAC(const cv::Mat &src, const Linkage link, const int num_nb_, const double sigma,
                const SegFormat seg_format, const int target_clus, const bool show_cycle)
-    : num_nb(num_nb_), width(src.cols), height(src.rows) {
+    : num_nb(num_nb_), m_link(link), width(src.cols), height(src.rows) {
     hbp_api::require_bgr(src);
-    if (link != CentoridLink) throw std::invalid_argument("SegHAC: only CentoridLink is implemented");
+    if (link < MinLink || link > WardLink) throw std::invalid_argument("SegHAC: unknown linkage");
     if (seg_format != LabelISPFormat) throw std::invalid_argument("SegHAC: only LabelISPFormat is implemented");
     if (target_clus != 1) throw std::invalid_argument("SegHAC: only target_clus = 1 is implemented");
     if (sigma > 1.17549435e-38) throw std::invalid_argument("SegHAC: only sigma = 0 is implemented");
@@ -84,7 +84,8 @@ void SegHAC::set_seq_img(const cv::Mat &m) {
 
 void SegHAC::run_ms(std::vector<cv::Mat> *) {
     auto h = std::make_shared<hbp_api::PyramidHandle>();
-    check(hbp_pyramid_build(default_ctx(), img.data, width, height, 2, num_nb, 0.0, &h->p), "SegHAC::run_ms");
+    check(hbp_pyramid_build(default_ctx(), img.data, width, height, (int)m_link, num_nb, 0.0, &h->p),
+          "SegHAC::run_ms");
     pyr = h;
     int L = hbp_pyramid_levels(pyr->p);
     num_clus_isp.assign(L, 0);
@@ -109,10 +110,10 @@ StereoCost::StereoCost(const cv::Mat &img_left, const cv::Mat &img_right, const 
 
 void StereoCost::compute_stereo_cost(const std::string algo) {
     auto h = std::make_shared<hbp_api::VolumeHandle>();
-    if (algo == "Census") {
-        check(hbp_cost_volume(default_ctx(), imgL.data, imgR.data, width, height, m_labels, "Census", &h->v),
+    if (algo == "Census" || algo == "AD") {
+        check(hbp_cost_volume(default_ctx(), imgL.data, imgR.data, width, height, m_labels, algo.c_str(), &h->v),
               "StereoCost::compute_stereo_cost");
-        m_tau = 25.f;
+        m_tau = algo == "AD" ? 30.f : 25.f;
     } else if (algo == "MC-CNN") {
         // read_volume_from_file (stereo_cost.cu:399-419): vol_dir names a headerless f32
         // [labels][height][width] file (loadMatBinary(..., readHeader = false))
@@ -127,8 +128,8 @@ void StereoCost::compute_stereo_cost(const std::string algo) {
               "StereoCost::compute_stereo_cost");
         m_tau = 0.5f;
     } else {
-        throw std::invalid_argument("StereoCost: the \"Census\" and \"MC-CNN\" costs are implemented (not \"" +
-                                    algo + "\")");
+        throw std::invalid_argument("StereoCost: the costs are \"Census\", \"AD\" and \"MC-CNN\" (not \"" + algo +
+                                    "\")");
     }
     vol = h;
 }

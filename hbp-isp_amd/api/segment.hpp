@@ -1,7 +1,7 @@
 // Drop-in for segment.hpp of HBP-ISP (segment.hpp:8-340): the CRTrees superpixel pyramid,
-// backed by hbp_pyramid_build.  Implemented configuration: CentroidLink, LabelISPFormat,
-// target_clus = 1, sigma = 0, num_nb 4 or 8 (every script of the reference uses it); other
-// values throw std::invalid_argument.
+// backed by hbp_pyramid_build.  Implemented configuration: the four linkages (CentroidLink,
+// the one every script of the reference uses, MinLink, MaxLink, WardLink), LabelISPFormat,
+// target_clus = 1, sigma = 0, num_nb 4 or 8; other values throw std::invalid_argument.
 #ifndef _SEGMENT_HPP_
 #define _SEGMENT_HPP_
 
@@ -46,6 +46,7 @@ protected:
 private:
     cv::Mat img;
     const int num_nb;
+    const Linkage m_link;
     int width, height;
     const int mean_channels = 6;
     std::shared_ptr<hbp_api::PyramidHandle> pyr;

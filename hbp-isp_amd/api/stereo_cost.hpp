@@ -1,6 +1,7 @@
 // Drop-in for stereo_cost.hpp of HBP-ISP (stereo_cost.hpp:8-132), backed by hbp_cost_volume.
-// compute_stereo_cost implements "Census" (tau 25) and "MC-CNN" (tau 0.5: the headerless f32
-// volume file named by vol_dir); "AD" throws std::invalid_argument.  g_img_cost_d() returns
+// compute_stereo_cost implements "Census" (tau 25), "AD" (tau 30) and "MC-CNN" (tau 0.5: the
+// headerless f32 volume file named by vol_dir); another name throws std::invalid_argument (the
+// reference leaves the volume unset).  g_img_cost_d() returns
 // the f32 [D][H][W] volume of the reference (for Census built on first use from the u8 volume
 // the library computes with).
 #ifndef _STEREO_COST_HPP_

@@ -165,9 +165,11 @@ int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen) {
 }
 
 // ---------------------------------------------------------------- (a)
-static void check_algo(const char *algo) {
-    HBP_REQUIRE(algo && std::strcmp(algo, "Census") == 0,
-                "only the \"Census\" cost is implemented (\"AD\" and \"MC-CNN\" are not)");
+// compute_stereo_cost (stereo_cost.cu:421-445) from images: "Census" (u8 volume) or "AD" (f32)
+static bool image_algo_is_ad(const char *algo) {
+    HBP_REQUIRE(algo && (std::strcmp(algo, "Census") == 0 || std::strcmp(algo, "AD") == 0),
+                "the costs from images are \"Census\" and \"AD\" (\"MC-CNN\" takes a given volume)");
+    return std::strcmp(algo, "AD") == 0;
 }
 
 int hbp_cost_volume_dev(hbp_ctx *ctx, const uint8_t *d_l, const uint8_t *d_r, int W, int H, int D, const char *algo,
@@ -175,11 +177,12 @@ int hbp_cost_volume_dev(hbp_ctx *ctx, const uint8_t *d_l, const uint8_t *d_r, in
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && out && d_l && d_r, "NULL argument");
     *out = nullptr;
-    check_algo(algo);
+    const bool ad = image_algo_is_ad(algo);
     use_device(ctx);
     auto *v = new hbp_volume();
     try {
-        volume_build(ctx->c, v->v, d_l, d_r, W, H, D);
+        if (ad) volume_build_ad(ctx->c, v->v, d_l, d_r, W, H, D);
+        else volume_build(ctx->c, v->v, d_l, d_r, W, H, D);
     } catch (...) {
         delete v;
         throw;
@@ -193,7 +196,7 @@ int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *l, const uint8_t *r, int W, int
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && out && l && r, "NULL argument");
     *out = nullptr;
-    check_algo(algo);
+    const bool ad = image_algo_is_ad(algo);
     HBP_REQUIRE(W > 0 && H > 0, "bad image size");
     use_device(ctx);
     size_t bytes = (size_t)W * H * 3;
@@ -202,7 +205,8 @@ int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *l, const uint8_t *r, int W, int
     HBP_CHECK(hipMemcpyAsync(dr.p, r, bytes, hipMemcpyHostToDevice, ctx->c.stream));
     auto *v = new hbp_volume();
     try {
-        volume_build(ctx->c, v->v, dl.p, dr.p, W, H, D);
+        if (ad) volume_build_ad(ctx->c, v->v, dl.p, dr.p, W, H, D);
+        else volume_build(ctx->c, v->v, dl.p, dr.p, W, H, D);
         HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
     } catch (...) {
         delete v;
@@ -339,7 +343,7 @@ int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol) {
 
 // ---------------------------------------------------------------- (b)
 static void check_seg_args(int link, int num_nb, double sigma) {
-    HBP_REQUIRE(link == 2, "only CentroidLink (2) is implemented");
+    HBP_REQUIRE(link >= 0 && link <= 3, "link must be 0 (MinLink), 1 (MaxLink), 2 (CentroidLink) or 3 (WardLink)");
     HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
     HBP_REQUIRE(sigma <= 1.17549435e-38, "only sigma = 0 (no Gaussian pre-blur) is implemented");
 }
@@ -353,7 +357,7 @@ int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int 
     use_device(ctx);
     auto *p = new hbp_pyramid();
     try {
-        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb);
+        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb, link);
     } catch (...) {
         delete p;
         throw;
@@ -375,7 +379,7 @@ int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int link, 
     HBP_CHECK(hipMemcpyAsync(d.p, bgr, bytes, hipMemcpyHostToDevice, ctx->c.stream));
     auto *p = new hbp_pyramid();
     try {
-        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb);
+        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb, link);
     } catch (...) {
         delete p;
         throw;

@@ -205,6 +205,29 @@ __global__ void f32_wta_kernel(const float *__restrict__ vol, int W, int H, int 
     }
 }
 
+// ----------------------------------------------------------------------------------------
+// the "AD" cost (ad_cost, stereo_cost.cu:9-43): grey in float (img_BGR_to_grey, imutil.cu:
+// 116-130, the double sum rounded once), then C[d][y][x] = min(|gL(x) - gR(x-d)|, tau) for
+// x >= D-1 and 0 for every d in the columns x < D-1.
+// ----------------------------------------------------------------------------------------
+__global__ void grey_f32_kernel(const uint8_t *__restrict__ bgrL, const uint8_t *__restrict__ bgrR, long n,
+                                float *__restrict__ g) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint8_t *a = bgrL + 3 * i, *b = bgrR + 3 * i;
+        g[i] = (float)((double)a[0] * 0.114 + (double)a[1] * 0.587 + (double)a[2] * 0.299);
+        g[n + i] = (float)((double)b[0] * 0.114 + (double)b[1] * 0.587 + (double)b[2] * 0.299);
+    }
+}
+__global__ void ad_volume_kernel(const float *__restrict__ g, int W, int H, int D, float tau,
+                                 float *__restrict__ out) {
+    const long hw = (long)W * H, total = hw * D;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long d = i / hw, r = i - d * hw;
+        const long x = r % W;
+        out[i] = x < D - 1 ? 0.f : fminf(fabsf(g[r] - g[hw + r - d]), tau);
+    }
+}
+
 // f32 [D][H][W] copy of the volume for the reference's float* getter (stereo_cost.hpp:42-46).
 __global__ void volume_to_f32_kernel(const uint8_t *__restrict__ vol, int W, int H, int D, int pitch, long plane,
                                      float *__restrict__ out) {
@@ -269,7 +292,7 @@ static int grid_of(const Ctx &c, long n) {
     return (int)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D, float tau) {
+static void f32_shape(Volume &v, int W, int H, int D, float tau) {
     HBP_REQUIRE(W >= 9 && H >= 7, "image must be at least 9x7");
     HBP_REQUIRE(D >= 1 && D <= 1024, "max disparity must be in [1, 1024]");
     HBP_REQUIRE(D <= W, "max disparity must not exceed the image width");
@@ -283,21 +306,46 @@ void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D
     v.tau = tau;
     v.right = false;
     v.f32src = true;
+}
+
+// WTA of the f32 volume and the largest |entry| (which bounds the plane costs' fixed point)
+static void f32_finish(Ctx &c, Volume &v) {
+    const long n = (long)v.W * v.H;
+    hipLaunchKernelGGL(f32_wta_kernel, dim3(grid_of(c, n)), dim3(256), 0, c.stream, v.f32.p, v.W, v.H, v.D,
+                       v.dwta.p);
+    DevBuf<unsigned int> am(c.pool, 1);
+    HBP_CHECK(hipMemsetAsync(am.p, 0, sizeof(unsigned int), c.stream));
+    hipLaunchKernelGGL(f32_absmax_kernel, dim3(grid_of(c, n * v.D)), dim3(256), 0, c.stream, v.f32.p, n * v.D,
+                       am.p);
+    HBP_CHECK(hipGetLastError());
+    unsigned int bits = 0;
+    HBP_CHECK(hipMemcpyAsync(&bits, am.p, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    std::memcpy(&v.absmax, &bits, 4);
+}
+
+void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D, float tau) {
+    f32_shape(v, W, H, D, tau);
     const long n = (long)W * H;
     v.f32.alloc(c.pool, (size_t)n * D);
     v.dwta.alloc(c.pool, (size_t)n);
     ProfScope ps(c, "volume_f32", 8.0 * n * D + 4.0 * n);
     hipLaunchKernelGGL(f32_volume_kernel, dim3(grid_of(c, n * D)), dim3(256), 0, c.stream, d_raw, W, H, D, tau,
                        v.f32.p);
-    hipLaunchKernelGGL(f32_wta_kernel, dim3(grid_of(c, n)), dim3(256), 0, c.stream, v.f32.p, W, H, D, v.dwta.p);
-    DevBuf<unsigned int> am(c.pool, 1);
-    HBP_CHECK(hipMemsetAsync(am.p, 0, sizeof(unsigned int), c.stream));
-    hipLaunchKernelGGL(f32_absmax_kernel, dim3(grid_of(c, n * D)), dim3(256), 0, c.stream, v.f32.p, n * D, am.p);
-    HBP_CHECK(hipGetLastError());
-    unsigned int bits = 0;
-    HBP_CHECK(hipMemcpyAsync(&bits, am.p, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipStreamSynchronize(c.stream));
-    std::memcpy(&v.absmax, &bits, 4);
+    f32_finish(c, v);
+}
+
+void volume_build_ad(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D) {
+    f32_shape(v, W, H, D, kAdTau);
+    const long n = (long)W * H;
+    v.f32.alloc(c.pool, (size_t)n * D);
+    v.dwta.alloc(c.pool, (size_t)n);
+    DevBuf<float> g(c.pool, 2 * (size_t)n);
+    ProfScope ps(c, "volume_ad", 6.0 * n + 4.0 * n * D + 4.0 * n);
+    hipLaunchKernelGGL(grey_f32_kernel, dim3(grid_of(c, n)), dim3(256), 0, c.stream, d_bgrL, d_bgrR, n, g.p);
+    hipLaunchKernelGGL(ad_volume_kernel, dim3(grid_of(c, n * D)), dim3(256), 0, c.stream, g.p, W, H, D, kAdTau,
+                       v.f32.p);
+    f32_finish(c, v);
 }
 
 void volume_to_right(Ctx &c, Volume &v) {

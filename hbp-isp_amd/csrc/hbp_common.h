@@ -153,7 +153,10 @@ __host__ __device__ inline uint32_t umod0(uint32_t s, int m) { return (uint32_t)
 constexpr int kP = 4;         // particles kept per superpixel
 constexpr int kK = 3 * kP;    // best, prior and candidate slots
 constexpr float kCensusTau = 25.0f;
+// SegHAC linkages (segment.hpp:13)
+constexpr int kMinLink = 0, kMaxLink = 1, kCentroidLink = 2, kWardLink = 3;
 constexpr float kMcCnnTau = 0.5f;   // compute_stereo_cost("MC-CNN"), stereo_cost.cu:437-438
+constexpr float kAdTau = 30.0f;     // compute_stereo_cost("AD"), stereo_cost.cu:426-427
 
 // packed pixel coordinate (x | y << 16)
 __host__ __device__ inline uint32_t pack_xy(int x, int y) { return (uint32_t)x | ((uint32_t)y << 16); }

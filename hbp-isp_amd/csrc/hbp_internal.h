@@ -10,7 +10,7 @@ struct Volume {
     long plane = 0;          // bytes between disparity planes (H * pitch)
     float tau = kCensusTau;  // truncation (stereo_cost.cu:433)
     bool right = false;      // cost_to_right applied
-    bool f32src = false;     // a given f32 volume (MC-CNN): f32 holds it, there is no u8 volume
+    bool f32src = false;     // an f32 volume (MC-CNN given, or AD): f32 holds it, there is no u8 volume
     float absmax = 0.f;      // f32src: the largest |entry| after truncation (bounds the plane costs)
     DevBuf<uint32_t> cl, cr; // census words of the left / right image [H][W]
     DevBuf<uint8_t> vol;     // u8 [D][H][pitch]
@@ -94,11 +94,12 @@ struct BP {
 // cost.hip
 void volume_build(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D);
 void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D, float tau);
+void volume_build_ad(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D);
 void volume_to_right(Ctx &c, Volume &v);
 float *volume_f32(Ctx &c, Volume &v);
 
 // pyramid.hip
-void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb);
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link = kCentroidLink);
 void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label);
 int compact_labels(Ctx &c, int *d_label, int V);
 
@@ -137,8 +138,9 @@ void eval_bad_pixels(Ctx &c, const float *d_disp, const float *d_gt, const uint8
 
 // primitives.hip
 void scan_exclusive_int(Ctx &c, const int *d_in, int *d_out, long n, int *d_total);
-void sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, int *vals, int *vals_tmp,
-                    long n, int end_bit);
+void sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, unsigned int *vals,
+                    unsigned int *vals_tmp, long n, int end_bit);
+void sort_keys_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, long n, int end_bit);
 void sort_pairs_u32(Ctx &c, unsigned int *keys, unsigned int *keys_tmp, int *vals, int *vals_tmp, long n,
                     int end_bit);
 

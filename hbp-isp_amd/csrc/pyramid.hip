@@ -61,8 +61,11 @@ __device__ __forceinline__ float dist_px(float4 s, float4 t) {
 
 // compute_1nn_grid_k (segment.cu:57-195) with bdl = 1 (segment.cu:846-850): neighbours in the
 // order u, d, l, r (+ ul, ur, dr, dl), first strict minimum, border slots are self-loops.
+// edist (Min/Max linkages only, else null): each slot's distance, 0 for a self-loop slot (the
+// reduction drops those).
 __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int num_nb, int *__restrict__ nn,
-                               float *__restrict__ dist_min, int2 *__restrict__ bd, int *__restrict__ bdl) {
+                               float *__restrict__ dist_min, int2 *__restrict__ bd, int *__restrict__ bdl,
+                               float *__restrict__ edist) {
     long n = (long)W * H;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         int x = (int)(i % W), y = (int)(i / W);
@@ -74,8 +77,9 @@ __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int
         long nb[8] = {i - W, i + W, i - 1, i + 1, i - W - 1, i - W + 1, i + W + 1, i + W - 1};
         for (int k = 0; k < num_nb; ++k) {
             long j = ok[k] ? nb[k] : i;
+            float d = 0.f;
             if (ok[k]) {
-                float d = dist_px(v, lab[j]);
+                d = dist_px(v, lab[j]);
                 if (d < best) {
                     best = d;
                     arg = (int)j;
@@ -83,6 +87,7 @@ __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int
             }
             bd[num_nb * i + k] = make_int2((int)i, (int)j);
             bdl[num_nb * i + k] = 1;
+            if (edist) edist[num_nb * i + k] = d;
         }
         nn[i] = arg;
         dist_min[i] = best;
@@ -373,14 +378,16 @@ __global__ void remap_labels_kernel(int *__restrict__ label, int V, const int *_
 // ----------------------------------------------------------------------------------------
 // Edge reduction (reduce_boundary_dist, segment.cu:1050-1091)
 // ----------------------------------------------------------------------------------------
-__global__ void map_edges_kernel(const int2 *__restrict__ bd, const int *__restrict__ bdl,
+// map_boundary (303-314): the edge's (clus[s], clus[t]) as one 64-bit key; with Min/Max the
+// carried distance travels with it (sort_by_key, 1056-1058), as its float bits
+__global__ void map_edges_kernel(const int2 *__restrict__ bd, const float *__restrict__ edist,
                                  const int *__restrict__ clus, long E, int bits,
-                                 unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+                                 unsigned long long *__restrict__ keys, unsigned int *__restrict__ dbits) {
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
         int2 b = bd[e];
         unsigned long long s = (unsigned)clus[b.x], t = (unsigned)clus[b.y];
         keys[e] = (s << bits) | t;
-        vals[e] = bdl[e];
+        if (edist) dbits[e] = __float_as_uint(edist[e]);
     }
 }
 // last element of a run of equal keys with s' != t' (is_diff_bd, segment.cu:320-334)
@@ -393,17 +400,27 @@ __global__ void run_flags_kernel(const unsigned long long *__restrict__ keys, lo
         flag[e] = (!self && (e == E - 1 || keys[e + 1] != k)) ? 1 : 0;
     }
 }
-// reduce_bdl (segment.cu:368-386) + scatter_if (1085): every non-self element adds its bdl to
-// its run's slot (integer sums); the run's last element writes the edge.
-__global__ void scatter_edges_kernel(const unsigned long long *__restrict__ keys, const int *__restrict__ vals,
+// reduce_bdl (segment.cu:368-386) + scatter_if (1085): every non-self element at sorted position
+// e adds bdl_old[e] -- the bdl stored at that position, which the sort did not move -- to its
+// run's slot (integer sums); the run's last element writes the edge.  Min/Max (reduce_dist,
+// 338-364): the run's minimum / maximum of the carried distances, which are >= +0 and not NaN
+// (squares, a max(0, .) term and non-negative penalties), so their float bits order as
+// unsigned integers.
+__global__ void scatter_edges_kernel(const unsigned long long *__restrict__ keys, const int *__restrict__ bdl_old,
+                                     const unsigned int *__restrict__ dbits, int minlink,
                                      const int *__restrict__ flag, const int *__restrict__ pos, long E, int bits,
-                                     int2 *__restrict__ bd_out, int *__restrict__ bdl_out) {
+                                     int2 *__restrict__ bd_out, int *__restrict__ bdl_out,
+                                     unsigned int *__restrict__ dist_out) {
     unsigned long long tmask = (1ull << bits) - 1ull;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
         unsigned long long k = keys[e];
         int s = (int)(k >> bits), t = (int)(k & tmask);
         if (s == t) continue;
-        atomicAdd(&bdl_out[pos[e]], vals[e]);
+        atomicAdd(&bdl_out[pos[e]], bdl_old[e]);
+        if (dbits) {
+            if (minlink) atomicMin(&dist_out[pos[e]], dbits[e]);
+            else atomicMax(&dist_out[pos[e]], dbits[e]);
+        }
         if (flag[e]) bd_out[pos[e]] = make_int2(s, t);
     }
 }
@@ -456,20 +473,35 @@ __device__ __forceinline__ float dist_ch(float s, float n, int c) {
     return d;
 }
 
-// compute_dist_mean_k (508-534), pernalize_dist_k (582-612), compute_1nn (1211-1231):
-// per node over its out-edges (sorted by t): minimum distance, ties -> largest t.
-__global__ void level_nn_kernel(const float *__restrict__ mu, int Vn, const int2 *__restrict__ bd,
-                                const int *__restrict__ off, const float *__restrict__ dist_max,
+// compute_dist_mean_k (508-534) [Ward: ward_weight (561-577), s_w t_w / (s_w + t_w) on the
+// pixel counts; Min/Max: the reduced carried distance instead], pernalize_dist_k (582-612),
+// compute_1nn (1211-1231): per node over its out-edges (sorted by t): minimum distance,
+// ties -> largest t.  Min/Max keep the penalised distance for the next level's reduction.
+__global__ void level_nn_kernel(const float *__restrict__ mu, const float *__restrict__ wsum, int link, int Vn,
+                                const int2 *__restrict__ bd, const int *__restrict__ off,
+                                const float *__restrict__ dist_max, float *__restrict__ edist,
                                 int *__restrict__ nn, float *__restrict__ dist_min) {
+    const bool use_mean = link == kCentroidLink || link == kWardLink;
     for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < Vn; s += (long)gridDim.x * blockDim.x) {
         float L = mu[s], A = mu[Vn + s], B = mu[2L * Vn + s];
+        const float sw = wsum[s];
         float best = FLT_MAX;
         int arg = 0;
         for (int e = off[s]; e < off[s + 1]; ++e) {
             int t = bd[e].y;
-            float d = dist_ch(L, mu[t], 0) + dist_ch(A, mu[Vn + t], 1) + dist_ch(B, mu[2L * Vn + t], 2);
+            float d;
+            if (use_mean) {
+                d = dist_ch(L, mu[t], 0) + dist_ch(A, mu[Vn + t], 1) + dist_ch(B, mu[2L * Vn + t], 2);
+                if (link == kWardLink) {
+                    const float tw = wsum[t];
+                    d = sw * tw / (sw + tw) * d;
+                }
+            } else {
+                d = edist[e];
+            }
             float delta = d - dist_max[t];
             if (delta > 0) d += delta;
+            if (!use_mean) edist[e] = d;
             if (d < best) {
                 best = d;
                 arg = t;
@@ -630,9 +662,11 @@ static int crtrees_level(Ctx &c, const int *d_nn, int V, int *d_label) {
     return Vn;
 }
 
-// SegHAC::run_ms (segment.cu:1279-1321) for CentroidLink, LabelISPFormat, target_clus = 1.
-void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb) {
+// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, target_clus = 1, any linkage.
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link) {
     HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
+    HBP_REQUIRE(link >= kMinLink && link <= kWardLink, "linkage must be 0 (Min), 1 (Max), 2 (Centroid) or 3 (Ward)");
+    const bool use_mean = link == kCentroidLink || link == kWardLink;
     HBP_REQUIRE(W >= 2 && H >= 2, "image must be at least 2x2");
     long N = (long)W * H;
     HBP_REQUIRE(N < (1L << 24), "images are limited to 2^24 pixels");
@@ -654,9 +688,12 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
     DevBuf<float4> lab(c.pool, N);
     DevBuf<int> nn(c.pool, N);
     DevBuf<float> dist_min(c.pool, N);
+    // Min/Max: the per-edge distances carried from level to level (dist_d, segment.cu:838-844)
+    DevBuf<float> edist;
+    if (!use_mean) edist.alloc(c.pool, (size_t)num_nb * N);
     hipLaunchKernelGGL(lab_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, d_bgr, W, H, lab.p, p.mean.p);
     hipLaunchKernelGGL(nn_grid_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, lab.p, W, H, num_nb, nn.p,
-                       dist_min.p, p.bd.p, p.bdl.p);
+                       dist_min.p, p.bd.p, p.bdl.p, edist.p);
     HBP_CHECK(hipGetLastError());
     lab.release();
 
@@ -676,10 +713,16 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
         // --- edge reduction
         int bits = bits_for(Vn);
         DevBuf<unsigned long long> keys(c.pool, E), keys2(c.pool, E);
-        DevBuf<int> vals(c.pool, E), vals2(c.pool, E), flag(c.pool, E + 1), pos(c.pool, E + 1);
-        hipLaunchKernelGGL(map_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, p.bd.p + eo,
-                           p.bdl.p + eo, clus, E, bits, keys.p, vals.p);
-        sort_pairs_u64(c, keys.p, keys2.p, vals.p, vals2.p, E, 2 * bits);
+        DevBuf<unsigned int> dbits, dbits2;
+        DevBuf<int> flag(c.pool, E + 1), pos(c.pool, E + 1);
+        if (!use_mean) {
+            dbits.alloc(c.pool, E);
+            dbits2.alloc(c.pool, E);
+        }
+        hipLaunchKernelGGL(map_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, p.bd.p + eo, edist.p,
+                           clus, E, bits, keys.p, dbits.p);
+        if (use_mean) sort_keys_u64(c, keys.p, keys2.p, E, 2 * bits);
+        else sort_pairs_u64(c, keys.p, keys2.p, dbits.p, dbits2.p, E, 2 * bits);
         HBP_CHECK(hipMemsetAsync(flag.p + E, 0, sizeof(int), c.stream));
         hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p, E, bits, flag.p);
         scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);
@@ -688,14 +731,24 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
         HBP_CHECK(hipStreamSynchronize(c.stream));
         HBP_REQUIRE(eo2 + En <= ecap, "pyramid edge capacity exceeded");
         HBP_CHECK(hipMemsetAsync(p.bdl.p + eo2, 0, sizeof(int) * (size_t)En, c.stream));
-        hipLaunchKernelGGL(scatter_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p, vals.p,
-                           flag.p, pos.p, E, bits, p.bd.p + eo2, p.bdl.p + eo2);
+        DevBuf<float> edist2;
+        if (!use_mean) {
+            // reduce_dist's start values (segment.cu:1070-1071): FLT_MAX for Min, 0 for Max
+            edist2.alloc(c.pool, (size_t)En + 1);
+            hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c, En)), dim3(256), 0, c.stream, (int *)edist2.p,
+                               (long)En, link == kMinLink ? 0x7f7fffff : 0);
+        }
+        hipLaunchKernelGGL(scatter_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p,
+                           p.bdl.p + eo, dbits.p, link == kMinLink ? 1 : 0, flag.p, pos.p, E, bits, p.bd.p + eo2,
+                           p.bdl.p + eo2, (unsigned int *)edist2.p);
         keys.release();
         keys2.release();
-        vals.release();
-        vals2.release();
+        dbits.release();
+        dbits2.release();
         flag.release();
         pos.release();
+        if (!use_mean) std::swap(edist, edist2);
+        edist2.release();
         // --- children grouped by parent, ascending child index (stable sort by parent)
         DevBuf<unsigned int> ck(c.pool, V), ck2(c.pool, V);
         DevBuf<int> cv(c.pool, V), cv2(c.pool, V), cnt(c.pool, (size_t)Vn + 1), coff(c.pool, (size_t)Vn + 1);
@@ -714,8 +767,9 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
         hipLaunchKernelGGL(count_src_kernel, dim3(grid_for(c, En)), dim3(256), 0, c.stream, p.bd.p + eo2, (long)En,
                            cnt.p);
         scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
-        hipLaunchKernelGGL(level_nn_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, mu.p, Vn, p.bd.p + eo2,
-                           coff.p, dist_max.p, nn.p, dist_min.p);
+        hipLaunchKernelGGL(level_nn_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, mu.p,
+                           p.mean.p + 6 * vo2 + 5L * Vn, link, Vn, p.bd.p + eo2, coff.p, dist_max.p, edist.p, nn.p,
+                           dist_min.p);
         HBP_CHECK(hipGetLastError());
         V = Vn;
         E = En;

@@ -163,16 +163,19 @@ class Ctx:
         return out
 
     # ---- (a)
-    def cost_volume(self, left, right, D, right_view=False):
-        return Volume(self, left, right, D, right_view)
+    def cost_volume(self, left, right, D, right_view=False, cost="Census"):
+        """compute_stereo_cost(cost) on two BGR images: "Census" (u8 volume) or "AD" (f32)"""
+        return Volume(self, left, right, D, right_view, cost=cost)
 
     def cost_volume_f32(self, vol, right_view=False):
         """the "MC-CNN" cost: a given f32 volume [D][H][W]"""
         return Volume(self, None, None, vol.shape[0], right_view, f32=vol)
 
     # ---- (b)
-    def pyramid(self, bgr, num_nb=4):
-        return Pyramid(self, bgr, num_nb)
+    def pyramid(self, bgr, num_nb=4, link=2):
+        """SegHAC(bgr, link, num_nb, 0, LabelISPFormat, 1).run_ms(); link 0 Min, 1 Max,
+        2 Centroid (default), 3 Ward"""
+        return Pyramid(self, bgr, num_nb, link)
 
     # ---- (c)
     def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
@@ -230,7 +233,7 @@ class Ctx:
 
 
 class Volume:
-    def __init__(self, ctx: Ctx, left, right, D, right_view=False, f32=None):
+    def __init__(self, ctx: Ctx, left, right, D, right_view=False, f32=None, cost="Census"):
         self.ctx = ctx
         h = C.c_void_p()
         if f32 is not None:
@@ -240,8 +243,8 @@ class Volume:
         else:
             H, W = left.shape[:2]
             _check(ctx.lib.hbp_cost_volume(ctx.h, np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
-                                           b"Census", C.byref(h)))
-        self.f32 = f32 is not None
+                                           cost.encode(), C.byref(h)))
+        self.f32 = f32 is not None or cost == "AD"
         self.h = h
         self.W, self.H, self.D = W, H, D
         if right_view:
@@ -269,11 +272,11 @@ class Volume:
 
 
 class Pyramid:
-    def __init__(self, ctx: Ctx, bgr, num_nb=4):
+    def __init__(self, ctx: Ctx, bgr, num_nb=4, link=2):
         self.ctx = ctx
         H, W = bgr.shape[:2]
         h = C.c_void_p()
-        _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, 2, num_nb, 0.0, C.byref(h)))
+        _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, 0.0, C.byref(h)))
         self.h = h
         self.W, self.H = W, H
 

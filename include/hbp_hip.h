@@ -45,8 +45,10 @@ int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen);
  * launch in `every` (1 = all) */
 int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every);
 
-/* ---- (a) cost volume: StereoCost(imgL, imgR, labels, vol_dir) + compute_stereo_cost("Census")
- *      (stereo_cost.hpp:90-116, stereo_cost.cu:421-445).  algo must be "Census". */
+/* ---- (a) cost volume: StereoCost(imgL, imgR, labels, vol_dir) + compute_stereo_cost(algo)
+ *      (stereo_cost.hpp:90-116, stereo_cost.cu:421-445).  algo "Census" (u8 volume, tau 25)
+ *      or "AD" (ad_cost, stereo_cost.cu:9-43: f32 volume of |grey difference| truncated at
+ *      tau 30, zero in the columns x < D-1; read it with hbp_volume_download_f32). */
 int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *bgr_left, const uint8_t *bgr_right, int width,
                     int height, int labels, const char *algo, hbp_volume **out);
 /* same, images already in device memory */
@@ -79,8 +81,8 @@ int hbp_volume_download(hbp_ctx *ctx, const hbp_volume *vol, uint8_t *vol_host, 
 int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol);
 
 /* ---- (b) superpixel pyramid: SegHAC(src, link, num_nb, sigma, LabelISPFormat, 1, false) +
- *      run_ms() (segment.hpp:248-309, segment.cu:1279-1321).  link must be 2 (CentroidLink),
- *      sigma 0, num_nb 4 or 8. */
+ *      run_ms() (segment.hpp:248-309, segment.cu:1279-1321).  link 0 MinLink, 1 MaxLink,
+ *      2 CentroidLink (the reference's default), 3 WardLink; sigma 0; num_nb 4 or 8. */
 int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int width, int height, int link,
                       int num_nb, double sigma, hbp_pyramid **out);
 int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int width, int height, int link,

@@ -31,6 +31,9 @@ void orc_hamming(const uint32_t *cl, const uint32_t *cr, int W, int H, int D, in
                  uint8_t *vol);
 void orc_wta(const uint8_t *vol, int W, int H, int D, float *dwta);
 void orc_vol_to_right(uint8_t *vol, int W, int H, int D);
+/* the "AD" cost (stereo_cost.cu:9-43): f32 volume, tau 30; right_view applies cost_to_right */
+int orc_cost_volume_ad(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D, int right_view, float *vol,
+                       float *dwta);
 int orc_cost_volume_f32(const float *raw, int W, int H, int D, float tau, int right_view, float *vol,
                         float *dwta);
 int orc_cost_volume(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D,
@@ -44,6 +47,8 @@ void orc_cc_max(const int *nn, int V, int *label);
 void orc_crtrees_literal(const int *nn, int V, int order, int *label);
 int orc_compact(int *label, int V);
 void *orc_pyramid_build(const uint8_t *bgr, int W, int H, int num_nb);
+/* the same with a linkage: 0 MinLink, 1 MaxLink, 2 CentroidLink, 3 WardLink (segment.hpp:13) */
+void *orc_pyramid_build_link(const uint8_t *bgr, int W, int H, int num_nb, int link);
 int orc_pyr_levels(void *p);
 void orc_pyr_level_size(void *p, int level, int *V, int *E);
 void orc_pyr_get_level(void *p, int level, int *clus, float *mean, int *bd, int *bdl);

@@ -46,6 +46,48 @@ int orc_cost_volume_f32(const float *raw, int W, int H, int D, float tau, int ri
     return 0;
 }
 
+// the "AD" cost (ad_cost, stereo_cost.cu:9-43, tau = 30 from compute_stereo_cost 423-428): the
+// float greys of imutil.cu:116-130; in the columns x < D-1 every disparity costs 0, elsewhere
+// C[d][y][x] = fminf(|gL(y, x) - gR(y, x-d)|, tau).  Then cost_to_right for the right view
+// (C_R[d][y][x] = C_L[d][y][min(x+d, W-1)], 471-516) and WTA_D_k (248-266).
+int orc_cost_volume_ad(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D, int right_view, float *vol,
+                       float *dwta) {
+    if (W <= 0 || H <= 0 || D < 1 || D > W) return -1;
+    const long hw = (long)W * H;
+    std::vector<float> gl(hw), gr(hw);
+    for (long i = 0; i < hw; ++i) {
+        gl[i] = (float)((double)bgrL[3 * i] * 0.114 + (double)bgrL[3 * i + 1] * 0.587 +
+                        (double)bgrL[3 * i + 2] * 0.299);
+        gr[i] = (float)((double)bgrR[3 * i] * 0.114 + (double)bgrR[3 * i + 1] * 0.587 +
+                        (double)bgrR[3 * i + 2] * 0.299);
+    }
+    const float tau = 30.0f;
+    std::vector<float> left((size_t)hw * D);
+    for (int d = 0; d < D; ++d)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const long i = (long)y * W + x;
+                left[(long)d * hw + i] = x < D - 1 ? 0.0f : std::fmin(std::fabs(gl[i] - gr[i - d]), tau);
+            }
+    for (int d = 0; d < D; ++d)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const int xs = right_view ? std::min(x + d, W - 1) : x;
+                vol[(long)d * hw + (long)y * W + x] = left[(long)d * hw + (long)y * W + xs];
+            }
+    for (long i = 0; i < hw; ++i) {
+        float mc = 3.402823466e+38f;
+        int mi = 0;
+        for (int d = 0; d < D; ++d)
+            if (vol[(long)d * hw + i] < mc) {
+                mc = vol[(long)d * hw + i];
+                mi = d;
+            }
+        dwta[i] = (float)mi;
+    }
+    return 0;
+}
+
 // imutil.cu:116-130 (grey = B*0.114 + G*0.587 + R*0.299 in FP64, stored as f32) followed by
 // imutil.cu:154-168 (u8 = f < 255 ? trunc(f) : 255).
 void orc_grey(const uint8_t *bgr, int W, int H, uint8_t *grey) {

@@ -73,7 +73,9 @@ void lab_image(const uint8_t *bgr, int W, int H, float *lab4) {
 // segment.cu:57-195 (compute_1nn_grid_k): neighbours in the order up, down, left, right
 // (then ul, ur, dr, dl for num_nb = 8); first strict minimum wins; border slots become
 // self-loops (i, i) that take no part in the minimum.  nn starts at 0, dist_min at FLT_MAX.
-void nn_grid(const float *lab4, int W, int H, int num_nb, int *nn, float *dist_min, int *bd) {
+// edist (optional, the per-edge distances the Min/Max linkages carry): the distance of each
+// in-image slot; a self-loop slot is never read (0 here, unwritten in the reference).
+void nn_grid(const float *lab4, int W, int H, int num_nb, int *nn, float *dist_min, int *bd, float *edist) {
     long n = (long)W * H;
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) {
@@ -91,13 +93,15 @@ void nn_grid(const float *lab4, int W, int H, int num_nb, int *nn, float *dist_m
                      {y < H - 1 && x > 0, i + W - 1}};
         for (int k = 0; k < num_nb; ++k) {
             long j = nbs[k].ok ? nbs[k].j : i;
+            float d = 0.0f;
             if (nbs[k].ok) {
-                float d = dist_px(lab4 + 4 * i, lab4 + 4 * j);
+                d = dist_px(lab4 + 4 * i, lab4 + 4 * j);
                 if (d < best) {
                     best = d;
                     arg = (int)j;
                 }
             }
+            if (edist) edist[num_nb * i + k] = d;
             if (bd) {
                 bd[2 * (num_nb * i + k)] = (int)i;
                 bd[2 * (num_nb * i + k) + 1] = (int)j;
@@ -249,9 +253,11 @@ void crtrees_literal(const int *nn_in, int V, int order, int *out) {
     for (int i = 0; i < V; ++i) out[i] = C[i];
 }
 
-// SegHAC::run_ms (segment.cu:1279-1321) for CentroidLink, LabelISPFormat, target_clus = 1,
-// sigma = 0 (no blur), pernalize on.
-void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P) {
+// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, target_clus = 1, sigma = 0 (no
+// blur), pernalize on, with any of the four linkages (use_mean = Centroid or Ward,
+// segment.hpp:248).
+void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link) {
+    const bool use_mean = link == kCentroidLink || link == kWardLink;
     long N = (long)W * H;
     P.W = W;
     P.H = H;
@@ -261,7 +267,9 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P) {
     std::vector<int> nn(N);
     std::vector<float> dist_min(N);
     std::vector<int> bd(2 * num_nb * N);
-    nn_grid(lab.data(), W, H, num_nb, nn.data(), dist_min.data(), bd.data());
+    // the per-edge distances carried from level to level by Min/Max (dist_d, segment.cu:838-844)
+    std::vector<float> edist(use_mean ? 0 : num_nb * N);
+    nn_grid(lab.data(), W, H, num_nb, nn.data(), dist_min.data(), bd.data(), use_mean ? nullptr : edist.data());
     // init_data_mean (segment.cu:856-861, imutil.cu:92-114): SoA (L, a, b, x, y, w)
     int V = (int)N;
     long E = num_nb * N;
@@ -290,27 +298,39 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P) {
         L.bdl = bdl;
         P.levels.push_back(std::move(L));
         if (Vn == 1) break;
-        // reduce_boundary_dist (segment.cu:1050-1091): map, sort (int2 lexicographic order of
-        // util.h:118-127), keep the last of each run with s != t, sum bdl over the run.
-        std::vector<std::pair<long long, int>> keys(E);
+        // reduce_boundary_dist (segment.cu:1050-1091): map the edges, sort them (int2
+        // lexicographic order, util.h:118-127; with sort_by_key the carried distances move with
+        // their edges, 1056-1062), keep the last of each run with s != t.  bdl is not moved by
+        // the sort: reduce_bdl (368-386) adds bdl[i] of the *position* i of each run element, so
+        // a new edge's bdl is the sum of the previous level's bdl over its run's positions (the
+        // boundary length only while the previous bdl are all 1, i.e. for level 1).  Min/Max
+        // reduce the carried distances over the run (reduce_dist 338-364, from FLT_MAX / 0).
+        std::vector<std::pair<long long, float>> keys(E);
         for (long e = 0; e < E; ++e) {
             int s = clus[bd[2 * e]], t = clus[bd[2 * e + 1]];
-            keys[e] = {((long long)s << 32) | (unsigned)t, bdl[e]};
+            keys[e] = {((long long)s << 32) | (unsigned)t, use_mean ? 0.0f : edist[e]};
         }
         std::stable_sort(keys.begin(), keys.end(),
-                         [](const std::pair<long long, int> &a, const std::pair<long long, int> &b) {
+                         [](const std::pair<long long, float> &a, const std::pair<long long, float> &b) {
                              return a.first < b.first;
                          });
         std::vector<int> nbd, nbdl;
+        std::vector<float> ndist;
         for (long e = 0; e < E;) {
             long f = e;
             int sum = 0;
-            while (f < E && keys[f].first == keys[e].first) sum += keys[f++].second;
+            float red = link == kMinLink ? FLT_MAX : 0.0f;
+            for (; f < E && keys[f].first == keys[e].first; ++f) {
+                sum += bdl[f];
+                if (link == kMinLink) red = std::min(red, keys[f].second);
+                if (link == kMaxLink) red = std::max(red, keys[f].second);
+            }
             int s = (int)(keys[e].first >> 32), t = (int)(keys[e].first & 0xffffffff);
             if (s != t) {
                 nbd.push_back(s);
                 nbd.push_back(t);
                 nbdl.push_back(sum);
+                ndist.push_back(red);
             }
             e = f;
         }
@@ -326,12 +346,23 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P) {
         std::vector<float> mu(5 * (long)Vn);
         for (int c = 0; c < 5; ++c)
             for (int p = 0; p < Vn; ++p) mu[(long)c * Vn + p] = sums[(long)c * Vn + p] / sums[5L * Vn + p];
-        // compute_dist_mean_k (segment.cu:508-534) then pernalize_dist_k (582-612)
+        // Centroid / Ward: compute_dist_mean_k (segment.cu:508-534), Ward's weight
+        // s_w t_w / (s_w + t_w) on the pixel counts (ward_weight 561-577); Min / Max: the
+        // reduced carried distance.  Then pernalize_dist_k (582-612), which Min / Max carry on.
         std::vector<float> dist(En);
         for (long e = 0; e < En; ++e) {
             int s = nbd[2 * e], t = nbd[2 * e + 1];
-            float d = dist_ch(mu[s], mu[t], 0) + dist_ch(mu[Vn + s], mu[Vn + t], 1) +
-                      dist_ch(mu[2L * Vn + s], mu[2L * Vn + t], 2);
+            float d;
+            if (use_mean) {
+                d = dist_ch(mu[s], mu[t], 0) + dist_ch(mu[Vn + s], mu[Vn + t], 1) +
+                    dist_ch(mu[2L * Vn + s], mu[2L * Vn + t], 2);
+                if (link == kWardLink) {
+                    const float sw = sums[5L * Vn + s], tw = sums[5L * Vn + t];
+                    d = sw * tw / (sw + tw) * d;
+                }
+            } else {
+                d = ndist[e];
+            }
             float delta = d - dist_max[t];
             if (delta > 0) d += delta;
             dist[e] = d;
@@ -347,6 +378,7 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P) {
         mean.swap(sums);
         bd.swap(nbd);
         bdl.swap(nbdl);
+        if (!use_mean) edist.swap(dist);
         V = Vn;
         E = En;
     }
@@ -369,6 +401,12 @@ int orc_compact(int *label, int V) { return orc::compact(label, V); }
 void *orc_pyramid_build(const uint8_t *bgr, int W, int H, int num_nb) {
     auto *p = new orc::Pyramid();
     orc::pyramid_build(bgr, W, H, num_nb, *p);
+    return p;
+}
+void *orc_pyramid_build_link(const uint8_t *bgr, int W, int H, int num_nb, int link) {
+    if (link < 0 || link > 3) return nullptr;
+    auto *p = new orc::Pyramid();
+    orc::pyramid_build(bgr, W, H, num_nb, *p, link);
     return p;
 }
 int orc_pyr_levels(void *p) { return (int)((orc::Pyramid *)p)->levels.size(); }

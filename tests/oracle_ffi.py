@@ -39,7 +39,9 @@ def _load():
         "orc_cc_max": (None, [_i32p, C.c_int, _i32p]),
         "orc_crtrees_literal": (None, [_i32p, C.c_int, C.c_int, _i32p]),
         "orc_compact": (C.c_int, [_i32p, C.c_int]),
+        "orc_cost_volume_ad": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, _f32p]),
         "orc_pyramid_build": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int]),
+        "orc_pyramid_build_link": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
         "orc_pyr_levels": (C.c_int, [C.c_void_p]),
         "orc_pyr_level_size": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "orc_pyr_get_level": (None, [C.c_void_p, C.c_int, _i32p, _f32p, _i32p, _i32p]),
@@ -90,10 +92,11 @@ def cost_volume(left, right, D, right_view=False):
 
 
 class Pyramid:
-    def __init__(self, bgr, num_nb=4):
+    def __init__(self, bgr, num_nb=4, link=2):
         H, W = bgr.shape[:2]
         self.W, self.H = W, H
-        self.h = lib.orc_pyramid_build(np.ascontiguousarray(bgr), W, H, num_nb)
+        self.h = lib.orc_pyramid_build_link(np.ascontiguousarray(bgr), W, H, num_nb, link)
+        assert self.h, "bad linkage"
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -114,6 +117,15 @@ class Pyramid:
         bdl = np.zeros(E, np.int32)
         lib.orc_pyr_get_level(self.h, l, clus, mean, bd, bdl)
         return dict(V=V, E=E, clus=clus, mean=mean.reshape(6, V), bd=bd.reshape(E, 2), bdl=bdl)
+
+
+def cost_volume_ad(left, right, D, right_view=False):
+    H, W = left.shape[:2]
+    vol = np.zeros((D, H, W), np.float32)
+    dwta = np.zeros((H, W), np.float32)
+    assert lib.orc_cost_volume_ad(np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
+                                  int(right_view), vol, dwta) == 0
+    return vol, dwta
 
 
 def cost_volume_f32(raw, tau=0.5, right_view=False):

@@ -103,20 +103,48 @@ def test_right_view_from_left_volume():
     assert np.array_equal(np.argmin(vr, axis=0).astype(np.float32), load("wta_right_d16"))
 
 
+def literal_reduce(clus, bd, bdl, Vn):
+    """reduce_boundary_dist (segment.cu:1050-1091) as the reference executes it: the mapped edges
+    are sorted (thrust::sort moves only the edges), each run of equal (s, t) with s != t becomes
+    one edge, and reduce_bdl adds bdl[i] for every *position* i of the run -- the bdl array is
+    never permuted, so the sum is over the previous level's bdl at the run's positions."""
+    m = np.stack([clus[bd[:, 0]], clus[bd[:, 1]]], axis=1).astype(np.int64)
+    keys = np.sort(m[:, 0] * Vn + m[:, 1])
+    uniq, start = np.unique(keys, return_index=True)
+    end = np.r_[start[1:], len(keys)]
+    csum = np.r_[0, np.cumsum(bdl.astype(np.int64))]
+    keep = (uniq // Vn) != (uniq % Vn)
+    edges = np.stack([uniq // Vn, uniq % Vn], axis=1)[keep].astype(np.int32)
+    return edges, (csum[end] - csum[start])[keep].astype(np.int32)
+
+
 @pytest.mark.parametrize("l", [0, 1])
 def test_coarse_edges_from_fine_level(l):
-    # the edges of level l+1: the sorted set of mapped (clus[s], clus[t]) with s != t, and the
-    # boundary length of each the sum of the multiplicities
+    # the edges of level l+1: the sorted set of mapped (clus[s], clus[t]) with s != t; bdl by the
+    # literal position rule, which for l = 0 (all bdl 1) is the multiplicity of each edge
     clus, bd, bdl = load(f"pyr_l{l}_clus"), load(f"pyr_l{l}_bd"), load(f"pyr_l{l}_bdl")
-    m = np.stack([clus[bd[:, 0]], clus[bd[:, 1]]], axis=1)
-    keep = m[:, 0] != m[:, 1]
-    agg = {}
-    for (s, t), w in zip(m[keep].tolist(), bdl[keep].tolist()):
-        agg[(s, t)] = agg.get((s, t), 0) + w
-    keys = sorted(agg)
-    assert np.array_equal(load(f"pyr_l{l + 1}_bd"), np.array(keys, np.int32).reshape(-1, 2))
-    assert np.array_equal(load(f"pyr_l{l + 1}_bdl"), np.array([agg[k] for k in keys], np.int32))
+    edges, lens = literal_reduce(clus, bd, bdl, META["V"][l + 1])
+    assert np.array_equal(load(f"pyr_l{l + 1}_bd"), edges)
+    assert np.array_equal(load(f"pyr_l{l + 1}_bdl"), lens)
     assert len(clus) == META["V"][l] and clus.max() + 1 == META["V"][l + 1]
+    if l == 0:
+        m = clus[bd]
+        m = m[m[:, 0] != m[:, 1]]
+        _, mult = np.unique(m[:, 0].astype(np.int64) * META["V"][1] + m[:, 1], return_counts=True)
+        assert np.array_equal(lens, mult)
+
+
+def test_level2_bdl_differs_from_boundary_length():
+    # the position rule is not the boundary length once the bdl differ (level 1 -> 2): the
+    # fixture holds the reference's values, not the physically summed ones
+    clus, bd, bdl = load("pyr_l1_clus"), load("pyr_l1_bd"), load("pyr_l1_bdl")
+    m = np.stack([clus[bd[:, 0]], clus[bd[:, 1]]], axis=1)
+    agg = {}
+    for (s, t), w in zip(m.tolist(), bdl.tolist()):
+        if s != t:
+            agg[(s, t)] = agg.get((s, t), 0) + w
+    physical = np.array([agg[k] for k in sorted(agg)], np.int32)
+    assert not np.array_equal(load("pyr_l2_bdl"), physical)
 
 
 def test_level0_edges_are_the_4_neighbourhood():

@@ -1,6 +1,6 @@
-"""GPU parity of the "MC-CNN" cost (a given f32 volume: compute_stereo_cost, stereo_cost.cu:
-399-445; cost_to_right, 471-516) and of BP on it, through the C ABI against the oracle, bit for
-bit."""
+"""GPU parity of the f32 costs, through the C ABI against the oracle, bit for bit: the "MC-CNN"
+cost (a given f32 volume: compute_stereo_cost, stereo_cost.cu:399-445; cost_to_right, 471-516),
+the "AD" cost (ad_cost, stereo_cost.cu:9-43), and BP on both."""
 import sys
 
 import numpy as np
@@ -81,3 +81,35 @@ def test_batch_on_f32_volumes_equals_single_runs(ctx):
     for (d1, p1), bp in zip(singles, bps):
         d2, p2 = bp.run()
         assert same_bits(d1, d2) and same_bits(p1, p2)
+
+
+# ---- the "AD" cost ----
+
+@pytest.mark.parametrize("seed,W,H,D,right_view", [(1, 96, 64, 32, False), (2, 37, 23, 5, True),
+                                                   (3, 450, 375, 64, False), (4, 1030, 40, 256, True)])
+def test_ad_volume_and_wta(ctx, seed, W, H, D, right_view):
+    say(f"ad volume seed={seed} {W}x{H} D={D} right={right_view}")
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 6))
+    vo, wo = O.cost_volume_ad(left, right, D, right_view=right_view)
+    vg, wg = ctx.cost_volume(left, right, D, right_view=right_view, cost="AD").download_f32()
+    assert same_bits(vg, vo) and same_bits(wg, wo)
+
+
+@pytest.mark.parametrize("seed,W,H,D,ml,right_view", [(5, 96, 72, 24, 2, False), (6, 120, 90, 32, 1, True)])
+def test_bp_on_ad_volume_bit_exact(ctx, seed, W, H, D, ml, right_view):
+    say(f"ad bp seed={seed} {W}x{H} D={D} min_level={ml} right={right_view}")
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
+    img = right if right_view else left
+    vo, wo = O.cost_volume_ad(left, right, D, right_view=right_view)
+    bo = O.BP(O.Pyramid(img), vo, wo, D, min_level=ml, tau=30.0)
+    bo.step()
+    do, plo = bo.map()
+    vg = ctx.cost_volume(left, right, D, right_view=right_view, cost="AD")
+    dg, plg = ctx.bp(ctx.pyramid(img), vg, min_level=ml).run()
+    assert same_bits(do, dg) and same_bits(plo, plg)
+
+
+def test_unknown_cost_name_is_an_error(ctx):
+    left, right, _ = synth.make_pair(7, 32, 24, 8, (3, 6))
+    with pytest.raises(Exception, match="Census"):
+        ctx.cost_volume(left, right, 8, cost="SAD")

@@ -80,11 +80,12 @@ PYR_CASES = [(1, 64, 48), (2, 37, 23), (6, 120, 90), (3, 450, 375)]
 
 
 @pytest.mark.parametrize("seed,W,H", PYR_CASES)
-def test_pyramid_levels_bit_exact(ctx, seed, W, H):
-    say(f"pyramid seed={seed} {W}x{H}")
+@pytest.mark.parametrize("link", [2, 0, 1, 3])
+def test_pyramid_levels_bit_exact(ctx, seed, W, H, link):
+    say(f"pyramid seed={seed} {W}x{H} link={link}")
     left, _, _ = synth.make_pair(seed, W, H, 16, (3, 8))
-    po = O.Pyramid(left)
-    pg = ctx.pyramid(left)
+    po = O.Pyramid(left, link=link)
+    pg = ctx.pyramid(left, link=link)
     assert pg.levels == po.levels
     for l in range(po.levels):
         a, b = po.level(l), pg.level(l)
@@ -106,7 +107,37 @@ def test_pyramid_large_pair(ctx):
         assert (a["V"], a["E"]) == (b["V"], b["E"]), l
         assert np.array_equal(a["clus"], b["clus"]), l
         assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
         assert same_bits(a["mean"], b["mean"]), l
+
+
+@pytest.mark.parametrize("link", [0, 1, 3])
+def test_pyramid_linkages_large_image(ctx, link):
+    say(f"pyramid 1280x720 link={link}")
+    left, _, _ = synth.make_pair(12, 1280, 720, 128, (12, 24))
+    po = O.Pyramid(left, link=link)
+    pg = ctx.pyramid(left, link=link)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+
+
+@pytest.mark.parametrize("link", [0, 1, 3])
+def test_bp_on_other_linkages_bit_exact(ctx, link):
+    say(f"bp on link={link}")
+    left, right = _bp_pair(4, 120, 90, 24)
+    vo, wo = O.cost_volume(left, right, 24)
+    bo = O.BP(O.Pyramid(left, link=link), vo, wo, 24, min_level=2)
+    bo.step()
+    do, plo = bo.map()
+    bg = ctx.bp(ctx.pyramid(left, link=link), ctx.cost_volume(left, right, 24), min_level=2)
+    dg, plg = bg.run()
+    assert same_bits(do, dg) and same_bits(plo, plg)
 
 
 def _hip():

@@ -1,0 +1,134 @@
+"""Pins of the oracle's pyramid under every linkage of SegHAC (segment.hpp:13; MinLink and
+MaxLink carry per-edge distances reduced by reduce_dist, segment.cu:338-364; WardLink weights
+the centroid distance by ward_weight, 561-577) and of the reference's position rule for the
+boundary lengths: an independent numpy statement of SegHAC::run_ms (segment.cu:1279-1321) for
+LabelISPFormat, built on the oracle's Lab, CRTrees and compaction (each pinned elsewhere),
+compared level by level."""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from hbp_amd import synth
+
+L = O.lib
+MIN, MAX, CENTROID, WARD = 0, 1, 2, 3
+F32 = np.float32
+
+
+def dist_px(s, n):
+    """segment.cu:14-22 in float32, the reference's operation order"""
+    dx, dy, dz = s[..., 0] - n[..., 0], s[..., 1] - n[..., 1], s[..., 2] - n[..., 2]
+    d = (dx * dx + dy * dy) + dz * dz
+    return d + np.maximum(dx * F32(1.0), F32(0.0))
+
+
+def dist_ch(s, n, c):
+    """segment.cu:27-37"""
+    d = (s - n) * (s - n)
+    if c == 0:
+        d = d + np.maximum((s - n) * F32(1.0), F32(0.0))
+    return d
+
+
+def numpy_pyramid(bgr, link, num_nb=4):
+    H, W = bgr.shape[:2]
+    N = H * W
+    lab = np.zeros(N * 4, F32)
+    L.orc_lab(np.ascontiguousarray(bgr), W, H, lab)
+    lab = lab.reshape(N, 4)
+    ys, xs = np.divmod(np.arange(N), W)
+    offs = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (-1, 1), (1, 1), (1, -1)][:num_nb]
+    bd = np.zeros((N, num_nb, 2), np.int64)
+    edist = np.zeros((N, num_nb), F32)
+    nn = np.zeros(N, np.int64)
+    dmin = np.full(N, np.finfo(F32).max, F32)
+    for k, (dy, dx) in enumerate(offs):             # compute_1nn_grid_k, slots in this order
+        ok = (ys + dy >= 0) & (ys + dy < H) & (xs + dx >= 0) & (xs + dx < W)
+        j = np.where(ok, np.arange(N) + dy * W + dx, np.arange(N))
+        d = dist_px(lab[:, :3], lab[j, :3])
+        bd[:, k, 0], bd[:, k, 1] = np.arange(N), j
+        edist[:, k] = np.where(ok, d, F32(0))
+        better = ok & (d < dmin)                    # first strict minimum
+        nn[better], dmin[better] = j[better], d[better]
+    bd, edist = bd.reshape(-1, 2), edist.reshape(-1)
+    bdl = np.ones(len(bd), np.int64)
+    mean = np.stack([lab[:, 0], lab[:, 1], lab[:, 2], xs.astype(F32), ys.astype(F32), lab[:, 3]]).astype(F32)
+    V, levels = N, []
+    while V > 1:
+        clus = np.zeros(V, np.int32)
+        L.orc_crtrees_literal(nn.astype(np.int32), V, 0, clus)
+        Vn = L.orc_compact(clus, V)
+        levels.append(dict(V=V, clus=clus.copy(), mean=mean.copy(), bd=bd.astype(np.int32), bdl=bdl.astype(np.int32)))
+        if Vn == 1:
+            break
+        # reduce_boundary_dist: sorted edges; bdl summed by position; min / max of carried dists
+        m = clus[bd].astype(np.int64)
+        keys = m[:, 0] * Vn + m[:, 1]
+        order = np.argsort(keys, kind="stable")
+        sk, sd = keys[order], edist[order]
+        uniq, start = np.unique(sk, return_index=True)
+        end = np.r_[start[1:], len(sk)]
+        cb = np.r_[0, np.cumsum(bdl)]
+        keep = (uniq // Vn) != (uniq % Vn)
+        nbd = np.stack([uniq // Vn, uniq % Vn], axis=1)[keep]
+        nbdl = (cb[end] - cb[start])[keep]
+        red = (np.minimum.reduceat(sd, start) if link == MIN else np.maximum.reduceat(sd, start))[keep]
+        # compute_dist_inner_max, reduce_data_mean (ascending children, float32 left folds)
+        dmax = np.zeros(Vn, F32)
+        np.maximum.at(dmax, clus, dmin)
+        sums = np.zeros((6, Vn), F32)
+        for c in range(6):
+            np.add.at(sums[c], clus, mean[c])
+        mu = sums[:5] / sums[5]
+        s, t = nbd[:, 0], nbd[:, 1]
+        if link in (CENTROID, WARD):
+            d = (dist_ch(mu[0, s], mu[0, t], 0) + dist_ch(mu[1, s], mu[1, t], 1)) + dist_ch(mu[2, s], mu[2, t], 2)
+            if link == WARD:
+                sw, tw = sums[5, s], sums[5, t]
+                d = ((sw * tw) / (sw + tw)) * d
+        else:
+            d = red
+        delta = d - dmax[t]
+        d = np.where(delta > 0, d + delta, d).astype(F32)        # pernalize_dist_k
+        # compute_1nn: minimum over out-edges, ties to the largest t
+        dmin = np.full(Vn, np.finfo(F32).max, F32)
+        np.minimum.at(dmin, s, d)
+        nn = np.zeros(Vn, np.int64)
+        at_min = d == dmin[s]
+        np.maximum.at(nn, s[at_min], t[at_min])
+        bd, bdl, mean, V = nbd, nbdl, sums, Vn
+        if link in (MIN, MAX):
+            edist = d
+    return levels
+
+
+@pytest.mark.parametrize("link", [MIN, MAX, CENTROID, WARD])
+@pytest.mark.parametrize("seed,W,H,num_nb", [(3, 40, 30, 4), (4, 57, 23, 8), (5, 96, 64, 4)])
+def test_oracle_pyramid_matches_numpy_statement(link, seed, W, H, num_nb):
+    left, _, _ = synth.make_pair(seed, W, H, 8, (3, 6))
+    ref = numpy_pyramid(left, link, num_nb)
+    p = O.Pyramid(left, num_nb=num_nb, link=link)
+    assert p.levels == len(ref)
+    for l, r in enumerate(ref):
+        a = p.level(l)
+        assert a["V"] == r["V"], l
+        assert np.array_equal(a["clus"], r["clus"]), l
+        assert np.array_equal(a["bd"], r["bd"]), l
+        assert np.array_equal(a["bdl"], r["bdl"]), l
+        assert np.array_equal(a["mean"].view(np.uint32), r["mean"].view(np.uint32)), l
+
+
+def test_linkages_give_different_pyramids():
+    left, _, _ = synth.make_pair(6, 64, 48, 8, (3, 6))
+    sizes = {}
+    for link in (MIN, MAX, CENTROID, WARD):
+        p = O.Pyramid(left, link=link)
+        sizes[link] = tuple(p.level(l)["V"] for l in range(p.levels))
+    # level 0 and 1 (the pixel 1-NN) are shared; above them the distances differ
+    assert len({s[:2] for s in sizes.values()}) == 1
+    assert len(set(sizes.values())) >= 3, sizes
+
+
+def test_unknown_linkage_is_rejected():
+    left, _, _ = synth.make_pair(6, 32, 24, 8, (3, 6))
+    assert L.orc_pyramid_build_link(left, 32, 24, 4, 7) is None
