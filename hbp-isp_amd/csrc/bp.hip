@@ -1446,7 +1446,9 @@ void bp_setup(Ctx &c, BP &b) {
         BPLevel &B = b.lv[l];
         B.V = p.V[l];
         B.E = p.E[l];
-        B.Vh = (l < b.L - 1) ? p.V[l + 1] : 1;
+        // the top level's parents: one cluster, or target_clus of them (unused by BP, which does
+        // not run on the top level, but its children are grouped like every level's)
+        B.Vh = (l < b.L - 1) ? p.V[l + 1] : p.top;
         B.clus = p.clus.p + p.voff[l];
         B.bd = p.bd.p + p.eoff[l];
         if (l >= b.min_level) {

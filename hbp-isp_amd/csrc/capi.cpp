@@ -348,8 +348,8 @@ static void check_seg_args(int link, int num_nb, double sigma) {
     HBP_REQUIRE(sigma <= 1.17549435e-38, "only sigma = 0 (no Gaussian pre-blur) is implemented");
 }
 
-int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int link, int num_nb, double sigma,
-                          hbp_pyramid **out) {
+int hbp_pyramid_build_target_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int link, int num_nb,
+                                 double sigma, int target_clus, hbp_pyramid **out) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && out && d_bgr, "NULL argument");
     *out = nullptr;
@@ -357,7 +357,7 @@ int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int 
     use_device(ctx);
     auto *p = new hbp_pyramid();
     try {
-        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb, link);
+        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb, link, target_clus);
     } catch (...) {
         delete p;
         throw;
@@ -366,8 +366,8 @@ int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int 
     HBP_API_END
 }
 
-int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int link, int num_nb, double sigma,
-                      hbp_pyramid **out) {
+int hbp_pyramid_build_target(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int link, int num_nb, double sigma,
+                             int target_clus, hbp_pyramid **out) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && out && bgr, "NULL argument");
     *out = nullptr;
@@ -379,13 +379,23 @@ int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int link, 
     HBP_CHECK(hipMemcpyAsync(d.p, bgr, bytes, hipMemcpyHostToDevice, ctx->c.stream));
     auto *p = new hbp_pyramid();
     try {
-        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb, link);
+        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb, link, target_clus);
     } catch (...) {
         delete p;
         throw;
     }
     *out = p;
     HBP_API_END
+}
+
+int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int link, int num_nb, double sigma,
+                          hbp_pyramid **out) {
+    return hbp_pyramid_build_target_dev(ctx, d_bgr, W, H, link, num_nb, sigma, 1, out);
+}
+
+int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int link, int num_nb, double sigma,
+                      hbp_pyramid **out) {
+    return hbp_pyramid_build_target(ctx, bgr, W, H, link, num_nb, sigma, 1, out);
 }
 
 int hbp_pyramid_levels(const hbp_pyramid *pyr) { return pyr ? pyr->p.levels() : -1; }

@@ -23,6 +23,7 @@ struct Volume {
 // bdl[E_l]; the levels follow each other in the four packed arrays.
 struct Pyramid {
     int W = 0, H = 0, num_nb = 4;
+    int top = 1;                    // clusters the last level maps to: 1, or target_clus
     std::vector<int> V, E;          // per level
     std::vector<long> voff, eoff;   // element offsets of each level in the packed arrays
     DevBuf<int> clus;
@@ -99,7 +100,8 @@ void volume_to_right(Ctx &c, Volume &v);
 float *volume_f32(Ctx &c, Volume &v);
 
 // pyramid.hip
-void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link = kCentroidLink);
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link = kCentroidLink,
+                   int target_clus = 1);
 void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label);
 int compact_labels(Ctx &c, int *d_label, int V);
 

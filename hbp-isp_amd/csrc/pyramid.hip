@@ -515,6 +515,67 @@ __global__ void level_nn_kernel(const float *__restrict__ mu, const float *__res
 }
 
 // ----------------------------------------------------------------------------------------
+// get_target_clus (segment.cu:954-1041): re-label a level of Vn < target clusters (whose
+// children number V > target) into exactly `target` clusters.  Children are grouped by cluster
+// in ascending child index (child / child_off); the pinned orders: each cluster's sum of its
+// children's dist_min is a left fold in that order; clusters rank by that sum descending, ties
+// by ascending id (a stable sort on the complemented float bits, the sums being >= +0); the
+// children of a broken cluster take its values downwards in ascending child index.
+// ----------------------------------------------------------------------------------------
+__global__ void target_sums_kernel(const int *__restrict__ child, const int *__restrict__ child_off, int Vn,
+                                   const float *__restrict__ dist_min, unsigned int *__restrict__ key,
+                                   int *__restrict__ id) {
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < Vn; p += (long)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+        for (int j = child_off[p]; j < child_off[p + 1]; ++j) acc += dist_min[child[j]];
+        key[p] = ~__float_as_uint(acc);
+        id[p] = (int)p;
+    }
+}
+// the child count of the cluster at each rank (the scan's input; element Vn is 0)
+__global__ void target_counts_kernel(const int *__restrict__ id, const int *__restrict__ child_off, int Vn,
+                                     int *__restrict__ cnt) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i <= Vn; i += (long)gridDim.x * blockDim.x)
+        cnt[i] = i < Vn ? child_off[id[i] + 1] - child_off[id[i]] : 0;
+}
+// is_break_cluster (segment.cu:647-671): i* = the first rank with cum[i] + Vn - (i+1) >= target,
+// cum[i] = excl[i+1]; the condition only grows with i
+__global__ void target_istar_kernel(const int *__restrict__ excl, int Vn, int target, int *__restrict__ istar) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < Vn; i += (long)gridDim.x * blockDim.x)
+        if ((long)excl[i + 1] + Vn - (i + 1) >= target) atomicMin(istar, (int)i);
+}
+// pre_clus_label + scatter (segment.cu:675-690, 727-728): the value of each cluster, by id
+__global__ void target_pos_kernel(const int *__restrict__ id, const int *__restrict__ excl, int Vn,
+                                  const int *__restrict__ istar_p, int *__restrict__ pos) {
+    const int is = min(*istar_p, Vn - 1);
+    const int nb = excl[is + 1];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < Vn; i += (long)gridDim.x * blockDim.x)
+        pos[id[i]] = i <= is ? excl[i + 1] - 1 : (int)i - is + nb - 1;
+}
+// reset_clus_label (segment.cu:695-718) over the children grouped by cluster: the k-th child of a
+// broken cluster takes value pos - k, clamped at `exact`
+__global__ void target_label_kernel(const unsigned int *__restrict__ parent_sorted, const int *__restrict__ child,
+                                    const int *__restrict__ child_off, long V, int Vn, int target,
+                                    const int *__restrict__ excl, const int *__restrict__ istar_p,
+                                    const int *__restrict__ pos, int *__restrict__ clus) {
+    const int is = min(*istar_p, Vn - 1);
+    const int nb = excl[is + 1];
+    const int exact = target - Vn + is;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < V; j += (long)gridDim.x * blockDim.x) {
+        const int cc = (int)parent_sorted[j];
+        const int pre = pos[cc];
+        int label;
+        if (pre < nb) {
+            const int v = pre - (int)(j - child_off[cc]);
+            label = v >= exact ? exact : v;
+        } else {
+            label = pre + exact - nb + 1;
+        }
+        clus[child[j]] = label;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------------------
 static int grid_for(const Ctx &c, long n, int block = 256) {
@@ -662,17 +723,53 @@ static int crtrees_level(Ctx &c, const int *d_nn, int V, int *d_label) {
     return Vn;
 }
 
-// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, target_clus = 1, any linkage.
-void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link) {
+// get_target_clus on the device; clus holds the compacted labels of V children into Vn < target
+// clusters and receives labels in [0, target), every one used (so the reference's following
+// compaction is the identity)
+static void target_clus_relabel(Ctx &c, int *clus, int V, int Vn, const float *dist_min, int target) {
+    const int bits = bits_for(Vn);
+    DevBuf<unsigned int> ck(c.pool, V), ck2(c.pool, V);
+    DevBuf<int> cv(c.pool, V), cv2(c.pool, V), cnt(c.pool, (size_t)Vn + 1), coff(c.pool, (size_t)Vn + 1);
+    HBP_CHECK(hipMemcpyAsync(ck.p, clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, cv.p, (long)V);
+    sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits);
+    HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
+    hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, clus, (long)V, cnt.p);
+    scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
+    DevBuf<unsigned int> key(c.pool, Vn), key2(c.pool, Vn);
+    DevBuf<int> id(c.pool, Vn), id2(c.pool, Vn), rc(c.pool, (size_t)Vn + 1), excl(c.pool, (size_t)Vn + 1);
+    DevBuf<int> istar(c.pool, 1), pos(c.pool, Vn);
+    hipLaunchKernelGGL(target_sums_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, cv.p, coff.p, Vn, dist_min,
+                       key.p, id.p);
+    sort_pairs_u32(c, key.p, key2.p, id.p, id2.p, Vn, 32);
+    hipLaunchKernelGGL(target_counts_kernel, dim3(grid_for(c, (long)Vn + 1)), dim3(256), 0, c.stream, id.p, coff.p,
+                       Vn, rc.p);
+    scan_exclusive_int(c, rc.p, excl.p, Vn + 1, nullptr);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, c.stream, istar.p, 1L, Vn);
+    hipLaunchKernelGGL(target_istar_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, excl.p, Vn, target,
+                       istar.p);
+    hipLaunchKernelGGL(target_pos_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, id.p, excl.p, Vn, istar.p,
+                       pos.p);
+    hipLaunchKernelGGL(target_label_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, ck.p, cv.p, coff.p,
+                       (long)V, Vn, target, excl.p, istar.p, pos.p, clus);
+    HBP_CHECK(hipGetLastError());
+}
+
+// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, any linkage; target_clus > 1 ends
+// the pyramid at the first level of at most that many clusters, re-labelled to exactly
+// target_clus (get_target_clus, then save_output's stop test, segment.cu:1103-1110, 1299-1306).
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link, int target_clus) {
     HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
     HBP_REQUIRE(link >= kMinLink && link <= kWardLink, "linkage must be 0 (Min), 1 (Max), 2 (Centroid) or 3 (Ward)");
     const bool use_mean = link == kCentroidLink || link == kWardLink;
     HBP_REQUIRE(W >= 2 && H >= 2, "image must be at least 2x2");
     long N = (long)W * H;
     HBP_REQUIRE(N < (1L << 24), "images are limited to 2^24 pixels");
+    HBP_REQUIRE(target_clus >= 1 && target_clus < N, "target_clus must be in [1, width*height)");
     p.W = W;
     p.H = H;
     p.num_nb = num_nb;
+    p.top = target_clus;
     p.V.clear();
     p.E.clear();
     p.voff.clear();
@@ -703,11 +800,15 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
     while (V > 1) {
         int *clus = p.clus.p + vo;
         int Vn = crtrees_level(c, nn.p, V, clus);
+        if (Vn < target_clus) {
+            target_clus_relabel(c, clus, V, Vn, dist_min.p, target_clus);
+            Vn = target_clus;
+        }
         p.V.push_back(V);
         p.E.push_back((int)E);
         p.voff.push_back(vo);
         p.eoff.push_back(eo);
-        if (Vn == 1) break;
+        if (Vn == target_clus) break;
         long vo2 = vo + V, eo2 = eo + E;
         HBP_REQUIRE(vo2 + Vn <= vcap && eo2 + E <= ecap, "pyramid capacity exceeded");
         // --- edge reduction

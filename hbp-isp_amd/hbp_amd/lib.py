@@ -54,6 +54,10 @@ SIGNATURES = {
     "hbp_volume_free": (C.c_int, [_vp, _vp]),
     "hbp_pyramid_build": (C.c_int, [_vp, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _pp]),
     "hbp_pyramid_build_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _pp]),
+    "hbp_pyramid_build_target": (C.c_int, [_vp, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                           _pp]),
+    "hbp_pyramid_build_target_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                               C.c_int, _pp]),
     "hbp_pyramid_levels": (C.c_int, [_vp]),
     "hbp_pyramid_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "hbp_pyramid_dev_clus": (_vp, [_vp]),
@@ -172,10 +176,10 @@ class Ctx:
         return Volume(self, None, None, vol.shape[0], right_view, f32=vol)
 
     # ---- (b)
-    def pyramid(self, bgr, num_nb=4, link=2):
-        """SegHAC(bgr, link, num_nb, 0, LabelISPFormat, 1).run_ms(); link 0 Min, 1 Max,
+    def pyramid(self, bgr, num_nb=4, link=2, target_clus=1):
+        """SegHAC(bgr, link, num_nb, 0, LabelISPFormat, target_clus).run_ms(); link 0 Min, 1 Max,
         2 Centroid (default), 3 Ward"""
-        return Pyramid(self, bgr, num_nb, link)
+        return Pyramid(self, bgr, num_nb, link, target_clus)
 
     # ---- (c)
     def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
@@ -272,11 +276,15 @@ class Volume:
 
 
 class Pyramid:
-    def __init__(self, ctx: Ctx, bgr, num_nb=4, link=2):
+    def __init__(self, ctx: Ctx, bgr, num_nb=4, link=2, target_clus=1):
         self.ctx = ctx
         H, W = bgr.shape[:2]
         h = C.c_void_p()
-        _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, 0.0, C.byref(h)))
+        if target_clus == 1:
+            _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, 0.0, C.byref(h)))
+        else:
+            _check(ctx.lib.hbp_pyramid_build_target(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, 0.0,
+                                                    target_clus, C.byref(h)))
         self.h = h
         self.W, self.H = W, H
 

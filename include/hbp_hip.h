@@ -87,6 +87,14 @@ int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int width, int height, i
                       int num_nb, double sigma, hbp_pyramid **out);
 int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int width, int height, int link,
                           int num_nb, double sigma, hbp_pyramid **out);
+/* the same with SegHAC's target_clus (segment.cu:954-1041, 1299-1306): 1 <= target_clus <
+ * width*height; above 1 the pyramid ends at the first level of at most target_clus clusters,
+ * re-labelled to exactly target_clus (the clusters whose children have the largest summed
+ * 1-NN distances are broken into their children). */
+int hbp_pyramid_build_target(hbp_ctx *ctx, const uint8_t *bgr, int width, int height, int link, int num_nb,
+                             double sigma, int target_clus, hbp_pyramid **out);
+int hbp_pyramid_build_target_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int width, int height, int link,
+                                 int num_nb, double sigma, int target_clus, hbp_pyramid **out);
 /* g_num_clus_isp().size() */
 int hbp_pyramid_levels(const hbp_pyramid *pyr);
 /* g_num_clus_isp()[level], g_num_bd_isp()[level] (segment.hpp:127-136) */

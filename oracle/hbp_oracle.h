@@ -49,6 +49,8 @@ int orc_compact(int *label, int V);
 void *orc_pyramid_build(const uint8_t *bgr, int W, int H, int num_nb);
 /* the same with a linkage: 0 MinLink, 1 MaxLink, 2 CentroidLink, 3 WardLink (segment.hpp:13) */
 void *orc_pyramid_build_link(const uint8_t *bgr, int W, int H, int num_nb, int link);
+/* ... and target_clus (1 <= target_clus < W*H; get_target_clus, segment.cu:954-1041) */
+void *orc_pyramid_build_target(const uint8_t *bgr, int W, int H, int num_nb, int link, int target_clus);
 int orc_pyr_levels(void *p);
 void orc_pyr_level_size(void *p, int level, int *V, int *E);
 void orc_pyr_get_level(void *p, int level, int *clus, float *mean, int *bd, int *bdl);

@@ -78,7 +78,9 @@ static void prologue(BP &b) {
         B.lev = &Lv;
         B.V = Lv.V;
         B.E = Lv.E;
-        B.Vh = (l < b.L - 1) ? P_.levels[l + 1].V : 1;
+        // the top level's parents: one cluster, or target_clus of them (unused by BP, which
+        // does not run on the top level, but its children are grouped like every level's)
+        B.Vh = (l < b.L - 1) ? P_.levels[l + 1].V : P_.top;
         if (l >= b.min_level) {
             int V = B.V;
             long E = B.E;

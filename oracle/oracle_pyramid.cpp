@@ -253,14 +253,61 @@ void crtrees_literal(const int *nn_in, int V, int order, int *out) {
     for (int i = 0; i < V; ++i) out[i] = C[i];
 }
 
-// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, target_clus = 1, sigma = 0 (no
-// blur), pernalize on, with any of the four linkages (use_mean = Centroid or Ward,
-// segment.hpp:248).
-void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link) {
+// get_target_clus (segment.cu:954-1041) for num_clus < target < num_clus_p: re-labels the level
+// so that it has exactly `target` clusters, breaking the clusters with the largest sums of their
+// children's 1-NN distances into their children.
+//  * per cluster: the child count and the sum of the children's dist_min (reduce_dist_pos_atomic,
+//    a float atomicAdd; pinned: ascending child index, left fold from 0);
+//  * clusters ranked by that sum, descending (thrust::sort_by_key with greater; pinned stable:
+//    ties keep ascending cluster id); cum = inclusive scan of the counts in rank order;
+//  * i* = the first rank with cum[i] + num_clus - (i+1) >= target (is_break_cluster; its
+//    write of i* into pos_scan[num_clus-1] races with that element's read in the reference:
+//    pinned as read before write); nb = cum[i*], exact = target - num_clus + i*;
+//  * rank i <= i*: its children take the values cum[i]-1, cum[i]-2, ... (the atomicSub of
+//    reset_clus_label; pinned: ascending child index takes them in that order), each label
+//    min(value, exact); rank i > i*: one cluster labelled (i - i* + nb - 1) + exact - nb + 1.
+static void target_clus_labels(int *clus, int V, int Vn, const float *dist_min, int target) {
+    std::vector<float> dsum(Vn, 0.0f);
+    std::vector<int> cnt(Vn, 0);
+    for (int i = 0; i < V; ++i) {
+        dsum[clus[i]] += dist_min[i];
+        cnt[clus[i]] += 1;
+    }
+    std::vector<int> rank_id(Vn);
+    std::iota(rank_id.begin(), rank_id.end(), 0);
+    std::stable_sort(rank_id.begin(), rank_id.end(), [&](int a, int b) { return dsum[a] > dsum[b]; });
+    std::vector<long> cum(Vn);
+    long run = 0;
+    for (int i = 0; i < Vn; ++i) cum[i] = run += cnt[rank_id[i]];
+    int istar = -1;
+    for (int i = 0; i < Vn && istar < 0; ++i)
+        if (cum[i] + Vn - (i + 1) >= target) istar = i;
+    const long nb = cum[istar];
+    const long exact = (long)target - Vn + istar;
+    std::vector<long> pos(Vn);
+    for (int i = 0; i < Vn; ++i) pos[rank_id[i]] = i <= istar ? cum[i] - 1 : i - istar + nb - 1;
+    for (int i = 0; i < V; ++i) {
+        const int cc = clus[i];
+        const long pre = pos[cc];
+        if (pre < nb) {
+            const long v = pos[cc]--;
+            clus[i] = (int)(v >= exact ? exact : v);
+        } else {
+            clus[i] = (int)(pre + exact - nb + 1);
+        }
+    }
+}
+
+// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, sigma = 0 (no blur), pernalize on,
+// with any of the four linkages (use_mean = Centroid or Ward, segment.hpp:248).  target_clus > 1
+// ends the pyramid at the first level with at most that many clusters, re-labelled to exactly
+// target_clus (get_target_clus, then save_output's stop test, segment.cu:1103-1110, 1299-1306).
+void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link, int target_clus) {
     const bool use_mean = link == kCentroidLink || link == kWardLink;
     long N = (long)W * H;
     P.W = W;
     P.H = H;
+    P.top = target_clus;
     P.levels.clear();
     std::vector<float> lab(4 * N);
     lab_image(bgr, W, H, lab.data());
@@ -288,6 +335,10 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int
         std::vector<int> clus(V);
         crtrees_literal(nn.data(), V, 0, clus.data());
         int Vn = compact(clus.data(), V);
+        if (Vn <= target_clus && Vn != target_clus) {
+            target_clus_labels(clus.data(), V, Vn, dist_min.data(), target_clus);
+            Vn = compact(clus.data(), V);
+        }
         // save_output -> push_label_isp (segment.cu:1096-1111, 884-905)
         Level L;
         L.V = V;
@@ -297,7 +348,7 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int
         L.bd = bd;
         L.bdl = bdl;
         P.levels.push_back(std::move(L));
-        if (Vn == 1) break;
+        if (Vn == target_clus) break;
         // reduce_boundary_dist (segment.cu:1050-1091): map the edges, sort them (int2
         // lexicographic order, util.h:118-127; with sort_by_key the carried distances move with
         // their edges, 1056-1062), keep the last of each run with s != t.  bdl is not moved by
@@ -407,6 +458,12 @@ void *orc_pyramid_build_link(const uint8_t *bgr, int W, int H, int num_nb, int l
     if (link < 0 || link > 3) return nullptr;
     auto *p = new orc::Pyramid();
     orc::pyramid_build(bgr, W, H, num_nb, *p, link);
+    return p;
+}
+void *orc_pyramid_build_target(const uint8_t *bgr, int W, int H, int num_nb, int link, int target_clus) {
+    if (link < 0 || link > 3 || target_clus < 1 || (long)target_clus >= (long)W * H) return nullptr;
+    auto *p = new orc::Pyramid();
+    orc::pyramid_build(bgr, W, H, num_nb, *p, link, target_clus);
     return p;
 }
 int orc_pyr_levels(void *p) { return (int)((orc::Pyramid *)p)->levels.size(); }

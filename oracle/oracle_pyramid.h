@@ -19,6 +19,7 @@ struct Level {
 
 struct Pyramid {
     int W = 0, H = 0;
+    int top = 1;   // clusters the last level maps to: 1, or target_clus
     std::vector<Level> levels;
 };
 
@@ -30,6 +31,7 @@ int compact(int *label, int V);
 void crtrees_literal(const int *nn, int V, int order, int *out);
 // linkages as in segment.hpp:13: 0 MinLink, 1 MaxLink, 2 CentroidLink, 3 WardLink
 enum { kMinLink = 0, kMaxLink = 1, kCentroidLink = 2, kWardLink = 3 };
-void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link = kCentroidLink);
+void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link = kCentroidLink,
+                   int target_clus = 1);
 
 }  // namespace orc

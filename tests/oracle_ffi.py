@@ -42,6 +42,7 @@ def _load():
         "orc_cost_volume_ad": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, _f32p]),
         "orc_pyramid_build": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int]),
         "orc_pyramid_build_link": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "orc_pyramid_build_target": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "orc_pyr_levels": (C.c_int, [C.c_void_p]),
         "orc_pyr_level_size": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "orc_pyr_get_level": (None, [C.c_void_p, C.c_int, _i32p, _f32p, _i32p, _i32p]),
@@ -92,11 +93,11 @@ def cost_volume(left, right, D, right_view=False):
 
 
 class Pyramid:
-    def __init__(self, bgr, num_nb=4, link=2):
+    def __init__(self, bgr, num_nb=4, link=2, target_clus=1):
         H, W = bgr.shape[:2]
         self.W, self.H = W, H
-        self.h = lib.orc_pyramid_build_link(np.ascontiguousarray(bgr), W, H, num_nb, link)
-        assert self.h, "bad linkage"
+        self.h = lib.orc_pyramid_build_target(np.ascontiguousarray(bgr), W, H, num_nb, link, target_clus)
+        assert self.h, "bad linkage or target_clus"
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -127,6 +127,44 @@ def test_pyramid_linkages_large_image(ctx, link):
         assert same_bits(a["mean"], b["mean"]), l
 
 
+@pytest.mark.parametrize("target,link,W,H", [(2, 2, 120, 90), (50, 2, 120, 90), (300, 0, 160, 120),
+                                              (5000, 2, 450, 375), (777, 3, 450, 375)])
+def test_pyramid_target_clus_bit_exact(ctx, target, link, W, H):
+    say(f"pyramid target_clus={target} link={link} {W}x{H}")
+    left, _, _ = synth.make_pair(14, W, H, 16, (3, 8))
+    po = O.Pyramid(left, link=link, target_clus=target)
+    pg = ctx.pyramid(left, link=link, target_clus=target)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+    top = pg.level(pg.levels - 1)["clus"]
+    assert top.max() + 1 == target and len(np.unique(top)) == target
+
+
+def test_bp_on_target_clus_pyramid_bit_exact(ctx):
+    say("bp on target_clus=40")
+    left, right = _bp_pair(15, 120, 90, 24)
+    vo, wo = O.cost_volume(left, right, 24)
+    bo = O.BP(O.Pyramid(left, target_clus=40), vo, wo, 24, min_level=1)
+    bo.step()
+    do, plo = bo.map()
+    bg = ctx.bp(ctx.pyramid(left, target_clus=40), ctx.cost_volume(left, right, 24), min_level=1)
+    dg, plg = bg.run()
+    assert same_bits(do, dg) and same_bits(plo, plg)
+
+
+def test_target_clus_out_of_range_is_an_error(ctx):
+    left, _, _ = synth.make_pair(16, 32, 24, 8, (3, 6))
+    for bad in (0, 32 * 24):
+        with pytest.raises(H.HbpError, match="target_clus"):
+            ctx.pyramid(left, target_clus=bad)
+
+
 @pytest.mark.parametrize("link", [0, 1, 3])
 def test_bp_on_other_linkages_bit_exact(ctx, link):
     say(f"bp on link={link}")

@@ -30,7 +30,30 @@ def dist_ch(s, n, c):
     return d
 
 
-def numpy_pyramid(bgr, link, num_nb=4):
+def numpy_target_labels(clus, dmin, Vn, target):
+    """get_target_clus (segment.cu:954-1041) with the pinned orders: per-cluster sums of the
+    children's dist_min in ascending child order, clusters ranked by sum descending (ties by id),
+    children of a broken cluster taking its values downwards in ascending child order."""
+    dsum = np.zeros(Vn, F32)
+    np.add.at(dsum, clus, dmin)
+    cnt = np.bincount(clus, minlength=Vn)
+    rank_id = np.argsort(-dsum, kind="stable")
+    cum = np.cumsum(cnt[rank_id])
+    ranks = np.arange(Vn)
+    istar = int(np.argmax(cum + Vn - (ranks + 1) >= target))
+    nb, exact = int(cum[istar]), target - Vn + istar
+    pos = np.empty(Vn, np.int64)
+    pos[rank_id] = np.where(ranks <= istar, cum - 1, ranks - istar + nb - 1)
+    out = np.empty(len(clus), np.int64)
+    broken = pos[clus] < nb
+    out[~broken] = pos[clus[~broken]] + exact - nb + 1
+    for cc in np.unique(clus[broken]):
+        kids = np.flatnonzero(clus == cc)                  # ascending child index
+        out[kids] = np.minimum(pos[cc] - np.arange(len(kids)), exact)
+    return out.astype(np.int32)
+
+
+def numpy_pyramid(bgr, link, num_nb=4, target=1):
     H, W = bgr.shape[:2]
     N = H * W
     lab = np.zeros(N * 4, F32)
@@ -58,8 +81,12 @@ def numpy_pyramid(bgr, link, num_nb=4):
         clus = np.zeros(V, np.int32)
         L.orc_crtrees_literal(nn.astype(np.int32), V, 0, clus)
         Vn = L.orc_compact(clus, V)
+        if Vn < target:
+            clus = numpy_target_labels(clus, dmin, Vn, target)
+            assert set(np.unique(clus)) == set(range(target))
+            Vn = target
         levels.append(dict(V=V, clus=clus.copy(), mean=mean.copy(), bd=bd.astype(np.int32), bdl=bdl.astype(np.int32)))
-        if Vn == 1:
+        if Vn == target:
             break
         # reduce_boundary_dist: sorted edges; bdl summed by position; min / max of carried dists
         m = clus[bd].astype(np.int64)
@@ -118,6 +145,36 @@ def test_oracle_pyramid_matches_numpy_statement(link, seed, W, H, num_nb):
         assert np.array_equal(a["mean"].view(np.uint32), r["mean"].view(np.uint32)), l
 
 
+@pytest.mark.parametrize("target", [2, 7, 40, 300])
+@pytest.mark.parametrize("link", [CENTROID, MIN])
+def test_oracle_target_clus_matches_numpy_statement(target, link):
+    left, _, _ = synth.make_pair(8, 64, 48, 8, (3, 6))
+    ref = numpy_pyramid(left, link, 4, target)
+    p = O.Pyramid(left, link=link, target_clus=target)
+    assert p.levels == len(ref)
+    for l, r in enumerate(ref):
+        a = p.level(l)
+        assert np.array_equal(a["clus"], r["clus"]), l
+        assert np.array_equal(a["bd"], r["bd"]), l
+        assert np.array_equal(a["bdl"], r["bdl"]), l
+        assert np.array_equal(a["mean"].view(np.uint32), r["mean"].view(np.uint32)), l
+    top = p.level(p.levels - 1)["clus"]
+    assert top.max() + 1 == target and len(np.unique(top)) == target
+
+
+def test_target_clus_prefix_equals_the_full_pyramid():
+    # the levels before the stop are those of target_clus = 1
+    left, _, _ = synth.make_pair(9, 64, 48, 8, (3, 6))
+    full = O.Pyramid(left)
+    cut = O.Pyramid(left, target_clus=50)
+    sizes = [full.level(l)["V"] for l in range(full.levels)]
+    assert cut.levels < full.levels
+    for l in range(cut.levels - 1):
+        assert np.array_equal(cut.level(l)["clus"], full.level(l)["clus"])
+    assert sizes[cut.levels - 1] == cut.level(cut.levels - 1)["V"]
+    assert (sizes[cut.levels] if cut.levels < len(sizes) else 1) <= 50
+
+
 def test_linkages_give_different_pyramids():
     left, _, _ = synth.make_pair(6, 64, 48, 8, (3, 6))
     sizes = {}
@@ -129,6 +186,8 @@ def test_linkages_give_different_pyramids():
     assert len(set(sizes.values())) >= 3, sizes
 
 
-def test_unknown_linkage_is_rejected():
+def test_unknown_linkage_and_bad_target_are_rejected():
     left, _, _ = synth.make_pair(6, 32, 24, 8, (3, 6))
     assert L.orc_pyramid_build_link(left, 32, 24, 4, 7) is None
+    assert L.orc_pyramid_build_target(left, 32, 24, 4, 2, 0) is None
+    assert L.orc_pyramid_build_target(left, 32, 24, 4, 2, 32 * 24) is None
