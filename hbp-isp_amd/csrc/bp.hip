@@ -37,7 +37,10 @@ namespace hbp {
 
 constexpr int P = kP, K = kK;
 constexpr int SEG_R = 8, SEG = 64 * SEG_R;   // pixels per plane-cost segment (coarse levels)
-constexpr int COARSE_MIN_PIX = 32;           // average pixels per superpixel of a coarse level
+// average pixels per superpixel from which a level takes the coarse (segment) form: below it a
+// wavefront's segment would hold one small superpixel and leave most lanes idle (measured
+// crossover between 64 and 128 on 1280x720 and mixed-size batches)
+constexpr int COARSE_MIN_PIX = 96;
 
 // replicas of each fixed-point accumulator: work item i adds into replica i mod rrep, which
 // spreads the atomics of the few large superpixels of the coarse levels over rrep addresses
@@ -662,9 +665,13 @@ __device__ __forceinline__ double wave_sum_exact(double v) {
     return r;
 }
 
-template <int NS, int MODE>
+// HB: steps per batch.  4 keeps 16 census gathers in flight per wavefront (a single pair, whose
+// coarse launches have fewer wavefronts than the chip holds); 2 halves the registers the
+// gathers hold, for more resident wavefronts when many pairs share a launch.
+template <int NS, int MODE, int HB = 4>
 __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
     static_assert(NS % P == 0, "planes are costed in groups of P");
+    static_assert(HB == 4 || HB == 2, "batches of 4 or 2 steps");
     const int lane = threadIdx.x & 63;
     int4 sg = a.seg[w];
     pin(sg.x); pin(sg.y); pin(sg.z); pin(sg.w);   // one 16-byte load, not two around the test
@@ -708,13 +715,12 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
         double acc[P];
 #pragma unroll
         for (int k = 0; k < P; ++k) acc[k] = 0.0;
-        // the two halves of SEG_R / 2 steps: a full half in one batch (fast form when the
-        // planes allow), a partial one in batches of 2 and 1 in the general form, so that no
-        // step past the segment's end is computed
-        static_assert(SEG_R == 8, "two halves of 4 steps");
+        // SEG_R / HB batches of HB steps: a full batch at once (fast form when the planes
+        // allow), a partial one in batches of 2 and 1 in the general form, so that no step past
+        // the segment's end is computed
+        static_assert(SEG_R % HB == 0, "whole batches");
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            constexpr int HB = 4;
+        for (int h = 0; h < SEG_R / HB; ++h) {
             const int r0 = h * HB;
             const int rem = steps - r0;
             if (rem <= 0) break;
@@ -1188,7 +1194,7 @@ __global__ __launch_bounds__(256) void cost_edge_batch_kernel(const SweepArgs *_
         const int blk = xcd_contiguous(local, it.pc_blocks);
         if (COARSE) {
             const int w = blk * 4 + wave;
-            if (w < it.npc) pc_segment<P, MODE>(a, w, 2 * P, it.spatial != 0);
+            if (w < it.npc) pc_segment<P, MODE, 2>(a, w, 2 * P, it.spatial != 0);
         } else {
             pc_runs<P, MODE>(a, blk, 2 * P, (unsigned char *)lds_pc);
         }
