@@ -43,9 +43,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=64,
                     help="pairs of the batched-vs-plain comparison (mixed sizes; 0 skips it)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="length of the Poisson arrival window of the served-vs-plain comparison (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=12.0, help="mean arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=30.0,
+                    help="mean arrivals per second (above both paths' capacity, so that pairs/s measures "
+                         "capacity rather than the offered load)")
     ap.add_argument("--no-post", action="store_true",
                     help="skip the refinements after BP (right view, LR check, weighted median, 7-tap median)")
     ap.add_argument("--cpu-sweeps", type=int, default=0,

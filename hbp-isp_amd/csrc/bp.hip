@@ -1694,6 +1694,17 @@ void BatchRunner::init(Ctx &c, int slots) {
     table.alloc(c.pool, (size_t)slots * kBatchLevels * sizeof(SweepArgs));
     used.assign(slots, 0);
     mode = -1;
+    HBP_CHECK(hipHostMalloc((void **)&ring, sizeof(BatchItem) * (size_t)kRing * slots, hipHostMallocDefault));
+    for (int i = 0; i < kRing; ++i) HBP_CHECK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+}
+
+BatchRunner::~BatchRunner() {
+    for (int i = 0; i < kRing; ++i)
+        if (ev[i]) {
+            if (ev_live[i]) (void)hipEventSynchronize(ev[i]);
+            (void)hipEventDestroy(ev[i]);
+        }
+    if (ring) (void)hipHostFree(ring);
 }
 
 static int cost_mode(const BP &b) {
@@ -1784,9 +1795,17 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
             hitems.push_back(it);
         }
     if (items.n < (size_t)count) items.alloc(c.pool, std::max<size_t>(count, 64));
-    // pageable source: the copy is complete with respect to hitems when the call returns, and
-    // it is ordered after the previous round's kernels on the stream
-    HBP_CHECK(hipMemcpyAsync(items.p, hitems.data(), sizeof(BatchItem) * count, hipMemcpyHostToDevice, c.stream));
+    // staged in the pinned ring (the copy runs in stream order, after the previous round's
+    // kernels have read the device items); the slot's last copy must have run before reuse
+    HBP_REQUIRE(count <= nslots, "more batch items than slots");
+    const int slot = ring_pos;
+    ring_pos = (ring_pos + 1) % kRing;
+    if (ev_live[slot]) HBP_CHECK(hipEventSynchronize(ev[slot]));
+    BatchItem *src = ring + (size_t)slot * nslots;
+    std::memcpy(src, hitems.data(), sizeof(BatchItem) * count);
+    HBP_CHECK(hipMemcpyAsync(items.p, src, sizeof(BatchItem) * count, hipMemcpyHostToDevice, c.stream));
+    HBP_CHECK(hipEventRecord(ev[slot], c.stream));
+    ev_live[slot] = true;
     const SweepArgs *dargs = (const SweepArgs *)table.p;
     const BatchItem *itc = items.p, *itf = items.p + ncoarse;
     if (sp_blocks > 0) {

@@ -289,6 +289,10 @@ struct Ctx {
     Pool pool;
     int num_cu = 256;
     int coop_blocks = 0;     // co-resident blocks of the CRTrees cooperative kernel
+    // a stream whose queued work must finish before a cooperative launch on this context (the
+    // server's admission context, whose work runs beside the BP rounds of the main stream:
+    // a cooperative grid needs every one of its blocks resident at once)
+    hipStream_t drain_before_coop = nullptr;
     // scratch reused by every call on this context
     DevBuf<uint8_t> scratch;
     DevBuf<int> counters;  // small device counters (sizes, flags)

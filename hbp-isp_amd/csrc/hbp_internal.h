@@ -119,8 +119,20 @@ constexpr int kBatchLevels = 24;   // pyramid levels a slot holds (N < 2^24 halv
 struct BatchRunner {
     int nslots = 0, mode = -1;
     DevBuf<uint8_t> table;         // [slots][kBatchLevels] SweepArgs, pointers relative to the table
-    DevBuf<BatchItem> items;       // one round's items
+    DevBuf<BatchItem> items;       // one round's items (rounds are ordered on the stream)
     std::vector<int> used;
+    // the rounds' items staged in pinned host memory, so that queuing a round never waits for
+    // the device: a ring of kRing slots of nslots items, a slot reused once the event recorded
+    // behind its copy has completed
+    static constexpr int kRing = 256;
+    BatchItem *ring = nullptr;
+    hipEvent_t ev[kRing] = {};
+    bool ev_live[kRing] = {};
+    int ring_pos = 0;
+    BatchRunner() = default;
+    BatchRunner(const BatchRunner &) = delete;
+    BatchRunner &operator=(const BatchRunner &) = delete;
+    ~BatchRunner();
     void init(Ctx &c, int slots);
     int add(Ctx &c, BP &b);        // a free slot holding b's levels (b set up, same cost form)
     void remove(int slot);

@@ -667,6 +667,7 @@ static void crt_cooperative(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
     long need = ((long)V + 255) / 256;
     if (need < blocks) blocks = (int)need;
     void *args[] = {(void *)&d_nn, (void *)&V, (void *)&s, (void *)&d_label};
+    if (c.drain_before_coop) HBP_CHECK(hipStreamSynchronize(c.drain_before_coop));
     HBP_CHECK(hipLaunchCooperativeKernel((const void *)crtrees_kernel<true>, dim3(blocks), dim3(256), args, 0,
                                          c.stream));
 }

@@ -9,6 +9,12 @@
 // and leave between rounds, so pairs of different sizes and arrival times overlap on the
 // device; each pair's result is the one its own hbp_bp_run gives (same kernels, same sums).
 //
+// Admission runs on a second stream with its own memory pool (the admission context), so that
+// a new pair's cost volume and pyramid overlap the BP rounds already queued on the main stream;
+// admit() returns once that work is done, and the pair's buffers return to the admission pool
+// only after complete() has synchronised the main stream.  Rounds stage their items in pinned
+// memory and never wait for the device, so the host queues rounds ahead of it.
+//
 // The reference has no serving layer (one pair per HBP_ISP::run_ms, main_stereo_hbp_isp.cu:
 // 33-76); this is new.  A server is driven by one host thread: hbp_submit queues, and every
 // hbp_server_step admits, runs one round and completes finished pairs.
@@ -38,6 +44,7 @@ struct Request {
 
 struct Server {
     Ctx *c = nullptr;
+    Ctx admit_ctx;   // own stream and pool; see above
     int max_active = 0;
     size_t budget = 0, in_use = 0;
     float alpha = 1000.1f, tau_s = 0.6f;
@@ -46,6 +53,16 @@ struct Server {
     std::vector<std::unique_ptr<Request>> active;
     std::unordered_map<long long, std::unique_ptr<Request>> finished;
     long long next_id = 1;
+    ~Server() {
+        // the requests' buffers back to their pools, then the admission context's memory
+        finished.clear();
+        active.clear();
+        pending.clear();
+        admit_ctx.scratch.release();
+        admit_ctx.counters.release();
+        admit_ctx.pool.release_all();
+        if (admit_ctx.stream) (void)hipStreamDestroy(admit_ctx.stream);
+    }
 };
 
 // device bytes of one pair: u8 volume, census/WTA, the pyramid's packed arrays (2N nodes and
@@ -57,7 +74,7 @@ static size_t estimate_bytes(int W, int H, int D) {
 }
 
 static void admit(Server &s, Request &r) {
-    Ctx &c = *s.c;
+    Ctx &c = s.admit_ctx;
     const size_t n3 = (size_t)r.W * r.H * 3;
     DevBuf<uint8_t> dl(c.pool, n3), dr(c.pool, n3);
     HBP_CHECK(hipMemcpyAsync(dl.p, r.left.data(), n3, hipMemcpyHostToDevice, c.stream));
@@ -73,7 +90,8 @@ static void admit(Server &s, Request &r) {
     HBP_REQUIRE(L >= 2, "image too small for a pyramid of two levels");
     r.bp.min_level = std::max(1, std::min(r.min_level, L - 1));
     bp_setup(c, r.bp);
-    r.slot = s.runner.add(c, r.bp);
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    r.slot = s.runner.add(*s.c, r.bp);
     std::vector<uint8_t>().swap(r.left);
     std::vector<uint8_t>().swap(r.right);
 }
@@ -175,7 +193,18 @@ int hbp_server_create(hbp_ctx *ctx, int max_active, long long hbm_budget_bytes, 
     srv->s.budget = (size_t)hbm_budget_bytes;
     srv->s.alpha = alpha;
     srv->s.tau_s = tau_smooth;
-    srv->s.runner.init(ctx->c, max_active);
+    Ctx &ac = srv->s.admit_ctx;
+    ac.device = ctx->c.device;
+    ac.num_cu = ctx->c.num_cu;
+    ac.coop_blocks = ctx->c.coop_blocks;
+    ac.drain_before_coop = ctx->c.stream;
+    try {
+        HBP_CHECK(hipStreamCreateWithFlags(&ac.stream, hipStreamNonBlocking));
+        srv->s.runner.init(ctx->c, max_active);
+    } catch (...) {
+        delete srv;
+        throw;
+    }
     *out = srv;
     HBP_API_END
 }
@@ -185,6 +214,7 @@ int hbp_server_destroy(hbp_server *srv) {
     if (!srv) return 0;
     HBP_CHECK(hipSetDevice(srv->s.c->device));
     HBP_CHECK(hipStreamSynchronize(srv->s.c->stream));
+    if (srv->s.admit_ctx.stream) HBP_CHECK(hipStreamSynchronize(srv->s.admit_ctx.stream));
     delete srv;
     HBP_API_END
 }
