@@ -1466,7 +1466,10 @@ void bp_setup(Ctx &c, BP &b) {
             // pixels by (cluster, pixel id)
             HBP_CHECK(hipMemcpyAsync(keys.p, im_clus.p, sizeof(unsigned) * N, hipMemcpyDeviceToDevice, c.stream));
             hipLaunchKernelGGL(iota_i32_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, ids.p, N);
-            sort_pairs_u32(c, keys.p, keys2.p, ids.p, ids2.p, N, bits_for(V));
+            if (sort_pairs_u32(c, keys.p, keys2.p, ids.p, ids2.p, N, bits_for(V))) {
+                std::swap(keys, keys2);
+                std::swap(ids, ids2);
+            }
             B.pxy.alloc(c.pool, N);
             B.pcl.alloc(c.pool, N);
             hipLaunchKernelGGL(pix_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, keys.p, ids.p, N, b.W,
@@ -1514,7 +1517,10 @@ void bp_setup(Ctx &c, BP &b) {
                 B.child_off.alloc(c.pool, (size_t)B.Vh + 1);
                 HBP_CHECK(hipMemcpyAsync(ck.p, B.clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
                 hipLaunchKernelGGL(iota_i32_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.child.p, (long)V);
-                sort_pairs_u32(c, ck.p, ck2.p, B.child.p, cv2.p, V, bits_for(B.Vh));
+                if (sort_pairs_u32(c, ck.p, ck2.p, B.child.p, cv2.p, V, bits_for(B.Vh))) {
+                    std::swap(ck, ck2);
+                    std::swap(B.child, cv2);
+                }
                 hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, ck.p, (long)V,
                                    B.Vh, B.child_off.p);
             }

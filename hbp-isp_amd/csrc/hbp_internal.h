@@ -152,10 +152,11 @@ void eval_bad_pixels(Ctx &c, const float *d_disp, const float *d_gt, const uint8
 
 // primitives.hip
 void scan_exclusive_int(Ctx &c, const int *d_in, int *d_out, long n, int *d_total);
-void sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, unsigned int *vals,
-                    unsigned int *vals_tmp, long n, int end_bit);
-void sort_keys_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, long n, int end_bit);
-void sort_pairs_u32(Ctx &c, unsigned int *keys, unsigned int *keys_tmp, int *vals, int *vals_tmp, long n,
-                    int end_bit);
+// stable radix sorts; true: the sorted data are in the *_tmp buffers (swap them in)
+[[nodiscard]] bool sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, unsigned int *vals,
+                                  unsigned int *vals_tmp, long n, int end_bit);
+[[nodiscard]] bool sort_keys_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, long n, int end_bit);
+[[nodiscard]] bool sort_pairs_u32(Ctx &c, unsigned int *keys, unsigned int *keys_tmp, int *vals, int *vals_tmp,
+                                  long n, int end_bit);
 
 }  // namespace hbp

@@ -16,38 +16,40 @@ void scan_exclusive_int(Ctx &c, const int *d_in, int *d_out, long n, int *d_tota
     HBP_CHECK(rocprim::exclusive_scan(tmp, bytes, d_in, d_out, 0, (size_t)n, rocprim::plus<int>(), c.stream));
 }
 
+// Stable LSD radix sorts on double buffers: the sorted data end in either buffer, and the
+// functions return true when they ended in the second (the caller swaps its buffers), which
+// saves the copy back.
 template <class K, class V>
-static void sort_pairs(Ctx &c, K *keys, K *keys_tmp, V *vals, V *vals_tmp, long n, int end_bit) {
-    if (n <= 1) return;
+static bool sort_pairs(Ctx &c, K *keys, K *keys_tmp, V *vals, V *vals_tmp, long n, int end_bit) {
+    if (n <= 1) return false;
+    rocprim::double_buffer<K> kb(keys, keys_tmp);
+    rocprim::double_buffer<V> vb(vals, vals_tmp);
     size_t bytes = 0;
-    HBP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys, keys_tmp, vals, vals_tmp, (size_t)n, 0u,
-                                        (unsigned)end_bit, c.stream));
+    HBP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, (size_t)n, 0u, (unsigned)end_bit, c.stream));
     void *tmp = c.scratch_bytes(bytes);
-    HBP_CHECK(rocprim::radix_sort_pairs(tmp, bytes, keys, keys_tmp, vals, vals_tmp, (size_t)n, 0u,
-                                        (unsigned)end_bit, c.stream));
-    HBP_CHECK(hipMemcpyAsync(keys, keys_tmp, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
-    HBP_CHECK(hipMemcpyAsync(vals, vals_tmp, sizeof(V) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+    HBP_CHECK(rocprim::radix_sort_pairs(tmp, bytes, kb, vb, (size_t)n, 0u, (unsigned)end_bit, c.stream));
+    HBP_REQUIRE((kb.current() == keys_tmp) == (vb.current() == vals_tmp), "sort buffers out of step");
+    return kb.current() == keys_tmp;
 }
 
-void sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, unsigned int *vals,
+bool sort_pairs_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, unsigned int *vals,
                     unsigned int *vals_tmp, long n, int end_bit) {
-    sort_pairs(c, keys, keys_tmp, vals, vals_tmp, n, end_bit);
+    return sort_pairs(c, keys, keys_tmp, vals, vals_tmp, n, end_bit);
 }
 
-void sort_pairs_u32(Ctx &c, unsigned int *keys, unsigned int *keys_tmp, int *vals, int *vals_tmp, long n,
+bool sort_pairs_u32(Ctx &c, unsigned int *keys, unsigned int *keys_tmp, int *vals, int *vals_tmp, long n,
                     int end_bit) {
-    sort_pairs(c, keys, keys_tmp, vals, vals_tmp, n, end_bit);
+    return sort_pairs(c, keys, keys_tmp, vals, vals_tmp, n, end_bit);
 }
 
-// keys only; the sorted keys end in keys
-void sort_keys_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, long n, int end_bit) {
-    if (n <= 1) return;
+bool sort_keys_u64(Ctx &c, unsigned long long *keys, unsigned long long *keys_tmp, long n, int end_bit) {
+    if (n <= 1) return false;
+    rocprim::double_buffer<unsigned long long> kb(keys, keys_tmp);
     size_t bytes = 0;
-    HBP_CHECK(rocprim::radix_sort_keys(nullptr, bytes, keys, keys_tmp, (size_t)n, 0u, (unsigned)end_bit, c.stream));
+    HBP_CHECK(rocprim::radix_sort_keys(nullptr, bytes, kb, (size_t)n, 0u, (unsigned)end_bit, c.stream));
     void *tmp = c.scratch_bytes(bytes);
-    HBP_CHECK(rocprim::radix_sort_keys(tmp, bytes, keys, keys_tmp, (size_t)n, 0u, (unsigned)end_bit, c.stream));
-    HBP_CHECK(hipMemcpyAsync(keys, keys_tmp, sizeof(unsigned long long) * (size_t)n, hipMemcpyDeviceToDevice,
-                             c.stream));
+    HBP_CHECK(rocprim::radix_sort_keys(tmp, bytes, kb, (size_t)n, 0u, (unsigned)end_bit, c.stream));
+    return kb.current() == keys_tmp;
 }
 
 }  // namespace hbp

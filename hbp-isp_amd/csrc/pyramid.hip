@@ -733,7 +733,10 @@ static void target_clus_relabel(Ctx &c, int *clus, int V, int Vn, const float *d
     DevBuf<int> cv(c.pool, V), cv2(c.pool, V), cnt(c.pool, (size_t)Vn + 1), coff(c.pool, (size_t)Vn + 1);
     HBP_CHECK(hipMemcpyAsync(ck.p, clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
     hipLaunchKernelGGL(iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, cv.p, (long)V);
-    sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits);
+    if (sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits)) {
+        std::swap(ck, ck2);
+        std::swap(cv, cv2);
+    }
     HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
     hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, clus, (long)V, cnt.p);
     scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
@@ -742,7 +745,10 @@ static void target_clus_relabel(Ctx &c, int *clus, int V, int Vn, const float *d
     DevBuf<int> istar(c.pool, 1), pos(c.pool, Vn);
     hipLaunchKernelGGL(target_sums_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, cv.p, coff.p, Vn, dist_min,
                        key.p, id.p);
-    sort_pairs_u32(c, key.p, key2.p, id.p, id2.p, Vn, 32);
+    if (sort_pairs_u32(c, key.p, key2.p, id.p, id2.p, Vn, 32)) {
+        std::swap(key, key2);
+        std::swap(id, id2);
+    }
     hipLaunchKernelGGL(target_counts_kernel, dim3(grid_for(c, (long)Vn + 1)), dim3(256), 0, c.stream, id.p, coff.p,
                        Vn, rc.p);
     scan_exclusive_int(c, rc.p, excl.p, Vn + 1, nullptr);
@@ -823,8 +829,12 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
         }
         hipLaunchKernelGGL(map_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, p.bd.p + eo, edist.p,
                            clus, E, bits, keys.p, dbits.p);
-        if (use_mean) sort_keys_u64(c, keys.p, keys2.p, E, 2 * bits);
-        else sort_pairs_u64(c, keys.p, keys2.p, dbits.p, dbits2.p, E, 2 * bits);
+        const bool swapped = use_mean ? sort_keys_u64(c, keys.p, keys2.p, E, 2 * bits)
+                                      : sort_pairs_u64(c, keys.p, keys2.p, dbits.p, dbits2.p, E, 2 * bits);
+        if (swapped) {
+            std::swap(keys, keys2);
+            std::swap(dbits, dbits2);
+        }
         HBP_CHECK(hipMemsetAsync(flag.p + E, 0, sizeof(int), c.stream));
         hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p, E, bits, flag.p);
         scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);
@@ -856,7 +866,10 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
         DevBuf<int> cv(c.pool, V), cv2(c.pool, V), cnt(c.pool, (size_t)Vn + 1), coff(c.pool, (size_t)Vn + 1);
         HBP_CHECK(hipMemcpyAsync(ck.p, clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
         hipLaunchKernelGGL(iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, cv.p, (long)V);
-        sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits);
+        if (sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits)) {
+            std::swap(ck, ck2);
+            std::swap(cv, cv2);
+        }
         HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
         hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, clus, (long)V, cnt.p);
         scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
