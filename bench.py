@@ -52,6 +52,9 @@ def main():
                     help="skip the refinements after BP (right view, LR check, weighted median, 7-tap median)")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default kernel forms instead of timing the candidates first (the forms "
+                         "change speed only, never the results)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the stereo path runs one pair per GPU and this build has "
@@ -107,6 +110,13 @@ def main():
         ctx.sync()
         return time.perf_counter() - t0, lat
 
+    one_pair()   # fills info (levels, min_level) and the context's memory pool
+    tuning = {}
+    if not args.no_autotune:
+        # single_hb for this pair's shape; coarse_min_pix matters for the mixed sizes below (the
+        # coarse levels of this pair all hold thousands of pixels per superpixel)
+        from hbp_amd import tune
+        tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2)})
     for _ in range(args.warmup):
         one_pair()
     ctx.sync()
@@ -185,6 +195,10 @@ def main():
             traffic = None
 
     post, post_io = (None, None) if args.no_post else post_process(ctx, left, right, D, info["min_level"])
+    if batch_pairs and not args.no_autotune:
+        from hbp_amd import tune
+        sub = [(left_b, right_b, spec[3]) for spec, left_b, right_b in batch_pairs[:8]]
+        tuning["batch"] = tune.tune_batch(ctx, sub, 2)
     batched = batched_vs_plain(ctx, hip, batch_pairs) if batch_pairs else None
     served = served_vs_plain(ctx, serve_pool, args.serve_rate, args.serve_seconds, args.seed) if serve_pool else None
 
@@ -223,6 +237,7 @@ def main():
         "kernels_one_pair": kernels,
         "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
                                for n, v in sorted(pair_stats.items()) if "_L" in n},
+        "tuning": dict(tuning, final=ctx.get_tuning()),
         "batched_vs_plain": batched,
         "served_vs_plain": served,
         "post_process": post,

@@ -37,10 +37,6 @@ namespace hbp {
 
 constexpr int P = kP, K = kK;
 constexpr int SEG_R = 8, SEG = 64 * SEG_R;   // pixels per plane-cost segment (coarse levels)
-// average pixels per superpixel from which a level takes the coarse (segment) form: below it a
-// wavefront's segment would hold one small superpixel and leave most lanes idle (measured
-// crossover between 64 and 128 on 1280x720 and mixed-size batches)
-constexpr int COARSE_MIN_PIX = 96;
 
 // replicas of each fixed-point accumulator: work item i adds into replica i mod rrep, which
 // spreads the atomics of the few large superpixels of the coarse levels over rrep addresses
@@ -1111,7 +1107,7 @@ __global__ __launch_bounds__(256) void edge_kernel(SweepArgs a) {
     edge_wave(a, (long)blockIdx.x * 4 + wave, false);
 }
 
-template <int NS, bool COARSE, int MODE>
+template <int NS, bool COARSE, int MODE, int HB = 4>
 __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, int spatial, int npc) {
     extern __shared__ unsigned long long lds_pc[];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1122,7 +1118,7 @@ __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, 
         const int blk = xcd_contiguous(b, npc_blocks);
         if (COARSE) {
             const int w = blk * 4 + wave;
-            if (w < npc) pc_segment<NS, MODE>(a, w, slot0, spatial != 0);
+            if (w < npc) pc_segment<NS, MODE, HB>(a, w, slot0, spatial != 0);
         } else {
             pc_runs<NS, MODE>(a, blk, slot0, (unsigned char *)lds_pc);
         }
@@ -1180,7 +1176,7 @@ __device__ __forceinline__ int batch_find(const BatchItem *__restrict__ it, int 
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <int MODE, bool COARSE>
+template <int MODE, bool COARSE, int HB = 2>
 __global__ __launch_bounds__(256) void cost_edge_batch_kernel(const SweepArgs *__restrict__ args,
                                                               const BatchItem *__restrict__ items, int nitems) {
     extern __shared__ unsigned long long lds_pc[];
@@ -1194,7 +1190,7 @@ __global__ __launch_bounds__(256) void cost_edge_batch_kernel(const SweepArgs *_
         const int blk = xcd_contiguous(local, it.pc_blocks);
         if (COARSE) {
             const int w = blk * 4 + wave;
-            if (w < it.npc) pc_segment<P, MODE, 2>(a, w, 2 * P, it.spatial != 0);
+            if (w < it.npc) pc_segment<P, MODE, HB>(a, w, 2 * P, it.spatial != 0);
         } else {
             pc_runs<P, MODE>(a, blk, 2 * P, (unsigned char *)lds_pc);
         }
@@ -1478,7 +1474,7 @@ void bp_setup(Ctx &c, BP &b) {
             hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, keys.p, N, V,
                                B.pix_off.p);
             // coarse levels: segments of <= SEG pixels per superpixel
-            B.coarse = N >= (long)COARSE_MIN_PIX * V;
+            B.coarse = N >= (long)c.coarse_min_pix * V;
             if (B.coarse) {
                 DevBuf<int> scnt(c.pool, (size_t)V + 1);
                 B.sego.alloc(c.pool, (size_t)V + 1);
@@ -1584,21 +1580,21 @@ static double plane_cost_bytes(long N, int V, int NS) { return 12.0 * N + (doubl
 static double edge_bytes(long E, int V) { return 96.0 * E + (16.0 * K + 48) * V; }
 static double node_bytes(long E, int V) { return 84.0 * E + (48.0 * K + 16) * V; }
 
-template <int NS, bool COARSE>
+template <int NS, bool COARSE, int HB = 4>
 static void launch_cost_edge_mode(Ctx &c, const SweepArgs &a, int mode, long blocks, size_t lds, int slot0,
                                   int spatial, int npc) {
     if (mode == kCostLeft)
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostLeft>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
-                           spatial, npc);
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostLeft, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
+                           slot0, spatial, npc);
     else if (mode == kCostRight)
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostRight>), dim3(blocks), dim3(256), lds, c.stream, a,
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostRight, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
                            slot0, spatial, npc);
     else if (mode == kCostF32)
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostF32>), dim3(blocks), dim3(256), lds, c.stream, a,
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostF32, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
                            slot0, spatial, npc);
     else
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostWta>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
-                           spatial, npc);
+        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostWta, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
+                           slot0, spatial, npc);
 }
 
 // plane costs of slots [slot0, slot0 + NS) (+ the sweep's messages when with_edges)
@@ -1617,7 +1613,10 @@ static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l
         int npc = costs ? B.nseg_max : 0;
         long blocks = div_up(npc, 4) + div_up(nedge_w, 4);
         if (blocks == 0) return;
-        launch_cost_edge_mode<NS, true>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
+        if (NS == P && c.single_hb == 2)
+            launch_cost_edge_mode<NS, true, 2>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
+        else
+            launch_cost_edge_mode<NS, true, 4>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
     } else {
         if (costs) {
             ProfScope ps(c, "plane_cost_L" + std::to_string(l), plane_cost_bytes(a.N, a.V, NS));
@@ -1750,6 +1749,22 @@ void BatchRunner::remove(int slot) {
     if (slot >= 0 && slot < nslots) used[slot] = 0;
 }
 
+template <int HB>
+static void launch_batch_coarse(Ctx &c, int mode, int blocks, const SweepArgs *dargs, const BatchItem *items, int n) {
+    if (mode == kCostLeft)
+        hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, true, HB>), dim3(blocks), dim3(256), 0, c.stream, dargs,
+                           items, n);
+    else if (mode == kCostRight)
+        hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, true, HB>), dim3(blocks), dim3(256), 0, c.stream,
+                           dargs, items, n);
+    else if (mode == kCostF32)
+        hipLaunchKernelGGL((cost_edge_batch_kernel<kCostF32, true, HB>), dim3(blocks), dim3(256), 0, c.stream, dargs,
+                           items, n);
+    else
+        hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, true, HB>), dim3(blocks), dim3(256), 0, c.stream, dargs,
+                           items, n);
+}
+
 void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active) {
     // items: the segment-form ones first, then the run-form ones; each form is one cost launch
     // with its own block numbering, the node launch covers both
@@ -1820,18 +1835,8 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
     }
     if (coarse_blocks > 0) {
         ProfScope ps(c, "batch_plane_cost_coarse", 0.0);
-        if (mode == kCostLeft)
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, true>), dim3(coarse_blocks), dim3(256), 0, c.stream,
-                               dargs, itc, ncoarse);
-        else if (mode == kCostRight)
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, true>), dim3(coarse_blocks), dim3(256), 0,
-                               c.stream, dargs, itc, ncoarse);
-        else if (mode == kCostF32)
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostF32, true>), dim3(coarse_blocks), dim3(256), 0,
-                               c.stream, dargs, itc, ncoarse);
-        else
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, true>), dim3(coarse_blocks), dim3(256), 0, c.stream,
-                               dargs, itc, ncoarse);
+        if (c.batch_hb == 4) launch_batch_coarse<4>(c, mode, coarse_blocks, dargs, itc, ncoarse);
+        else launch_batch_coarse<2>(c, mode, coarse_blocks, dargs, itc, ncoarse);
     }
     if (fine_blocks > 0) {
         ProfScope ps(c, "batch_plane_cost_fine", 0.0);

@@ -153,6 +153,33 @@ int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every) {
     HBP_API_END
 }
 
+int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && name, "NULL argument");
+    const std::string n = name;
+    if (n == "coarse_min_pix") {
+        HBP_REQUIRE(value >= 1 && value <= (1 << 24), "coarse_min_pix must be in [1, 2^24]");
+        ctx->c.coarse_min_pix = value;
+    } else if (n == "single_hb" || n == "batch_hb") {
+        HBP_REQUIRE(value == 2 || value == 4, "steps per batch must be 2 or 4");
+        (n == "single_hb" ? ctx->c.single_hb : ctx->c.batch_hb) = value;
+    } else {
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb)"};
+    }
+    HBP_API_END
+}
+
+int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && name && value, "NULL argument");
+    const std::string n = name;
+    if (n == "coarse_min_pix") *value = ctx->c.coarse_min_pix;
+    else if (n == "single_hb") *value = ctx->c.single_hb;
+    else if (n == "batch_hb") *value = ctx->c.batch_hb;
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb)"};
+    HBP_API_END
+}
+
 int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen) {
     if (!ctx) return -1;
     std::string all;

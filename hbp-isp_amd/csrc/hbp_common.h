@@ -289,6 +289,13 @@ struct Ctx {
     Pool pool;
     int num_cu = 256;
     int coop_blocks = 0;     // co-resident blocks of the CRTrees cooperative kernel
+    // kernel forms, chosen by a tuner; every choice gives the same bits (the plane-cost sums
+    // are exact in any order): the average pixels per superpixel from which a BP level takes
+    // the segment form of the plane cost (below it a segment wavefront holds one small
+    // superpixel; measured crossover 64-128), and the steps per batch of its wavefronts in
+    // single-pair launches (4: more gathers in flight for a latency-bound launch) and in
+    // batched ones (2: fewer registers, more resident wavefronts for a throughput-bound one)
+    int coarse_min_pix = 96, single_hb = 4, batch_hb = 2;
     // a stream whose queued work must finish before a cooperative launch on this context (the
     // server's admission context, whose work runs beside the BP rounds of the main stream:
     // a cooperative grid needs every one of its blocks resident at once)

@@ -75,6 +75,9 @@ static size_t estimate_bytes(int W, int H, int D) {
 
 static void admit(Server &s, Request &r) {
     Ctx &c = s.admit_ctx;
+    c.coarse_min_pix = s.c->coarse_min_pix;   // the forms the main context is tuned to
+    c.single_hb = s.c->single_hb;
+    c.batch_hb = s.c->batch_hb;
     const size_t n3 = (size_t)r.W * r.H * 3;
     DevBuf<uint8_t> dl(c.pool, n3), dr(c.pool, n3);
     HBP_CHECK(hipMemcpyAsync(dl.p, r.left.data(), n3, hipMemcpyHostToDevice, c.stream));

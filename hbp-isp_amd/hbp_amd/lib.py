@@ -40,6 +40,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_double)]),
     "hbp_profile_names": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "hbp_profile_only": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "hbp_ctx_set_tuning": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "hbp_ctx_get_tuning": (C.c_int, [_vp, C.c_char_p, _ip]),
     "hbp_cost_volume": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_volume_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
     "hbp_cost_to_right": (C.c_int, [_vp, _vp]),
@@ -144,6 +146,19 @@ class Ctx:
 
     def sync(self):
         _check(self.lib.hbp_ctx_sync(self.h))
+
+    # ---- kernel-form tuning (results are the same for every value)
+    TUNING = ("coarse_min_pix", "single_hb", "batch_hb")
+
+    def set_tuning(self, name, value):
+        _check(self.lib.hbp_ctx_set_tuning(self.h, name.encode(), int(value)))
+
+    def get_tuning(self, name=None):
+        if name is None:
+            return {n: self.get_tuning(n) for n in self.TUNING}
+        v = C.c_int()
+        _check(self.lib.hbp_ctx_get_tuning(self.h, name.encode(), C.byref(v)))
+        return v.value
 
     # ---- kernel timing (HIP events on the context stream)
     def profile(self, on=True):

@@ -1,0 +1,77 @@
+"""Kernel-form tuner: picks a context's coarse_min_pix, single_hb and batch_hb for a workload by
+timing it on the device.
+
+Each value is a choice between forms that give the same results bit for bit (the plane-cost
+cluster sums are exact integers in any order), so tuning only trades speed:
+  * coarse_min_pix: average pixels per superpixel from which a BP level costs planes in the
+    per-superpixel segment form instead of chunks of sorted pixels;
+  * single_hb / batch_hb: steps per batch of a segment wavefront (2 or 4) in one-pair and in
+    batched launches: 4 keeps more gathers in flight, 2 fewer registers and more wavefronts.
+The reference's kernels have one form each; this module has no counterpart there.
+"""
+import itertools
+import time
+
+SINGLE_GRID = {"coarse_min_pix": (32, 64, 96, 128, 256), "single_hb": (4, 2)}
+BATCH_GRID = {"coarse_min_pix": (64, 96, 128), "batch_hb": (2, 4)}
+
+
+def _timed(ctx, fn, reps):
+    """best of reps timed calls after one untimed call (seconds)"""
+    fn()
+    ctx.sync()
+    best = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ctx.sync()
+        best = min(best, time.perf_counter() - t)
+    return best
+
+
+def _search(ctx, grid, run, reps):
+    keys = list(grid)
+    before = ctx.get_tuning()
+    times = {}
+    try:
+        for values in itertools.product(*(grid[k] for k in keys)):
+            for k, v in zip(keys, values):
+                ctx.set_tuning(k, v)
+            times[values] = _timed(ctx, run, reps)
+    finally:
+        for k, v in before.items():
+            ctx.set_tuning(k, v)
+    best = min(times, key=times.get)
+    for k, v in zip(keys, best):
+        ctx.set_tuning(k, v)
+    return {"best": dict(zip(keys, best)),
+            "ms": {"/".join(f"{k}={v}" for k, v in zip(keys, vals)): round(1e3 * t, 3) for vals, t in times.items()}}
+
+
+def tune_single(ctx, left, right, D, min_level=2, grid=None, reps=2):
+    """Times one pair's BP (plane costs, messages, node updates, MAP) for every combination of
+    the grid and leaves the fastest set on ctx."""
+    vol = ctx.cost_volume(left, right, D)
+    pyr = ctx.pyramid(left)
+    ml = max(1, min(min_level, pyr.levels - 1))
+
+    def run():
+        ctx.bp(pyr, vol, min_level=ml).run()
+
+    return _search(ctx, grid or SINGLE_GRID, run, reps)
+
+
+def tune_batch(ctx, pairs, min_level=2, grid=None, reps=1):
+    """Times the batched BP of pairs [(left, right, D), ...] (one sweep of every pair per round)
+    for every combination of the grid and leaves the fastest set on ctx."""
+    prepared = []
+    for left, right, D in pairs:
+        vol = ctx.cost_volume(left, right, D)
+        pyr = ctx.pyramid(left)
+        prepared.append((vol, pyr, max(1, min(min_level, pyr.levels - 1))))
+
+    def run():
+        bps = [ctx.bp(pyr, vol, min_level=ml) for vol, pyr, ml in prepared]
+        ctx.bp_run_batch(bps)
+
+    return _search(ctx, grid or BATCH_GRID, run, reps)

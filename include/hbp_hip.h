@@ -45,6 +45,16 @@ int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen);
  * launch in `every` (1 = all) */
 int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every);
 
+/* ---- kernel-form tuning of a context (no counterpart in the reference: its kernels have one
+ *      form each).  Every value gives the same results bit for bit; only the speed changes.
+ *      "coarse_min_pix" (>= 1, default 96): average pixels per superpixel from which a BP level
+ *      costs planes in the per-superpixel segment form rather than in chunks; "single_hb",
+ *      "batch_hb" (2 or 4; defaults 4, 2): steps per batch of the segment wavefronts in one-pair
+ *      and in batched launches.  Applies to BP objects created afterwards (and to the batched
+ *      rounds run afterwards).  hbp_amd.tune picks them by timing the caller's workload. */
+int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value);
+int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value);
+
 /* ---- (a) cost volume: StereoCost(imgL, imgR, labels, vol_dir) + compute_stereo_cost(algo)
  *      (stereo_cost.hpp:90-116, stereo_cost.cu:421-445).  algo "Census" (u8 volume, tau 25)
  *      or "AD" (ad_cost, stereo_cost.cu:9-43: f32 volume of |grey difference| truncated at

@@ -1,0 +1,107 @@
+"""The kernel-form tuning of a context (hbp_ctx_set_tuning, hbp_amd.tune) changes speed only:
+one-pair and batched BP give the same bits for every value, including the extremes that put
+every level in one form, and the oracle agrees."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from hbp_amd import lib as H
+from hbp_amd import synth, tune
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same_bits(a, b):
+    a, b = np.ascontiguousarray(a, np.float32), np.ascontiguousarray(b, np.float32)
+    na, nb = np.isnan(a), np.isnan(b)
+    return a.shape == b.shape and np.array_equal(na, nb) and np.array_equal(a.view(np.uint32)[~na],
+                                                                            b.view(np.uint32)[~nb])
+
+
+@pytest.fixture
+def tuned(ctx):
+    before = ctx.get_tuning()
+    yield ctx
+    for k, v in before.items():
+        ctx.set_tuning(k, v)
+
+
+SETTINGS = [dict(coarse_min_pix=96, single_hb=4, batch_hb=2), dict(coarse_min_pix=1, single_hb=2, batch_hb=4),
+            dict(coarse_min_pix=1 << 24, single_hb=4, batch_hb=4), dict(coarse_min_pix=40, single_hb=2, batch_hb=2)]
+
+
+def run_pair(ctx, left, right, D, ml):
+    return ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, D), min_level=ml).run()
+
+
+def test_one_pair_results_do_not_depend_on_tuning(tuned):
+    left, right, _ = synth.make_pair(61, 640, 480, 64, (6, 12))
+    outs = []
+    for st in SETTINGS:
+        say("tuning", st)
+        for k, v in st.items():
+            tuned.set_tuning(k, v)
+        outs.append(run_pair(tuned, left, right, 64, 1))
+    for d, p in outs[1:]:
+        assert same_bits(d, outs[0][0]) and same_bits(p, outs[0][1])
+
+
+def test_extreme_forms_match_the_oracle(tuned):
+    left, right, _ = synth.make_pair(62, 160, 120, 24, (3, 8))
+    vo, wo = O.cost_volume(left, right, 24)
+    bo = O.BP(O.Pyramid(left), vo, wo, 24, min_level=1)
+    bo.step()
+    do, plo = bo.map()
+    for cmp in (1, 1 << 24):
+        for hb in (2, 4):
+            tuned.set_tuning("coarse_min_pix", cmp)
+            tuned.set_tuning("single_hb", hb)
+            d, p = run_pair(tuned, left, right, 24, 1)
+            assert same_bits(d, do) and same_bits(p, plo), (cmp, hb)
+
+
+def test_batched_results_do_not_depend_on_tuning(tuned):
+    specs = [(63, 320, 240, 32), (64, 480, 272, 48), (65, 640, 480, 64)]
+    pairs = [synth.make_pair(s, w, h, d, (3, 8)) for s, w, h, d in specs]
+    ref = [run_pair(tuned, l, r, d, 2) for (l, r, _), (_, _, _, d) in zip(pairs, specs)]
+    for st in SETTINGS:
+        say("batched tuning", st)
+        for k, v in st.items():
+            tuned.set_tuning(k, v)
+        keep = []
+        bps = []
+        for (l, r, _), (_, _, _, d) in zip(pairs, specs):
+            v, p = tuned.cost_volume(l, r, d), tuned.pyramid(l)
+            keep += [v, p]
+            bps.append(tuned.bp(p, v, min_level=2))
+        tuned.bp_run_batch(bps)
+        for bp, (dr, pr) in zip(bps, ref):
+            d, p = bp.run()
+            assert same_bits(d, dr) and same_bits(p, pr)
+
+
+def test_tuning_parameters_are_checked(tuned):
+    with pytest.raises(H.HbpError, match="unknown tuning"):
+        tuned.set_tuning("tile", 3)
+    with pytest.raises(H.HbpError, match="2 or 4"):
+        tuned.set_tuning("batch_hb", 3)
+    with pytest.raises(H.HbpError, match="coarse_min_pix"):
+        tuned.set_tuning("coarse_min_pix", 0)
+    assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb"}
+
+
+def test_tuner_sets_the_fastest_grid_point(tuned):
+    left, right, _ = synth.make_pair(66, 320, 240, 32, (3, 8))
+    r = tune.tune_single(tuned, left, right, 32, 2, grid={"coarse_min_pix": (32, 256), "single_hb": (4, 2)}, reps=1)
+    assert len(r["ms"]) == 4 and r["best"]["coarse_min_pix"] in (32, 256) and r["best"]["single_hb"] in (2, 4)
+    assert tuned.get_tuning("coarse_min_pix") == r["best"]["coarse_min_pix"]
+    pairs = [synth.make_pair(67 + i, 320, 240, 32, (3, 8))[:2] + (32,) for i in range(2)]
+    rb = tune.tune_batch(tuned, pairs, 2, grid={"batch_hb": (2, 4)}, reps=1)
+    assert tuned.get_tuning("batch_hb") == rb["best"]["batch_hb"] and len(rb["ms"]) == 2
