@@ -21,7 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hbp-isp_amd"))
 
 HBM_PEAK_GBS = 8000.0
-EVENT_EVERY = 8   # timed region: HIP events around one in 8 launches of the dominant kernel
+# timed region: one launch in 7 of the dominant kernel carries start/stop events stamped by its
+# dispatch (kernel execution time, as a kernel trace measures it); 7 is coprime with the sweep
+# kinds' period (4 or 5 sweeps per BP iteration), so the sample covers every kind evenly
+EVENT_EVERY = 7
 METRIC = "stereo pairs/s + Mpix·disp/s; p50/p99 latency, batched/served vs plain"
 WORKLOADS = {
     "large": dict(width=1472, height=984, ndisp=256, n_planes=(20, 40), bp_levels=4,

@@ -1581,20 +1581,20 @@ static double edge_bytes(long E, int V) { return 96.0 * E + (16.0 * K + 48) * V;
 static double node_bytes(long E, int V) { return 84.0 * E + (48.0 * K + 16) * V; }
 
 template <int NS, bool COARSE, int HB = 4>
-static void launch_cost_edge_mode(Ctx &c, const SweepArgs &a, int mode, long blocks, size_t lds, int slot0,
-                                  int spatial, int npc) {
+static void launch_cost_edge_mode(Ctx &c, const ProfScope &ps, const SweepArgs &a, int mode, long blocks, size_t lds,
+                                  int slot0, int spatial, int npc) {
     if (mode == kCostLeft)
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostLeft, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
-                           slot0, spatial, npc);
+        HBP_LAUNCH(ps, (cost_edge_kernel<NS, COARSE, kCostLeft, HB>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
+                   spatial, npc);
     else if (mode == kCostRight)
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostRight, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
-                           slot0, spatial, npc);
+        HBP_LAUNCH(ps, (cost_edge_kernel<NS, COARSE, kCostRight, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
+                   slot0, spatial, npc);
     else if (mode == kCostF32)
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostF32, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
-                           slot0, spatial, npc);
+        HBP_LAUNCH(ps, (cost_edge_kernel<NS, COARSE, kCostF32, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
+                   slot0, spatial, npc);
     else
-        hipLaunchKernelGGL((cost_edge_kernel<NS, COARSE, kCostWta, HB>), dim3(blocks), dim3(256), lds, c.stream, a,
-                           slot0, spatial, npc);
+        HBP_LAUNCH(ps, (cost_edge_kernel<NS, COARSE, kCostWta, HB>), dim3(blocks), dim3(256), lds, c.stream, a, slot0,
+                   spatial, npc);
 }
 
 // plane costs of slots [slot0, slot0 + NS) (+ the sweep's messages when with_edges)
@@ -1609,23 +1609,23 @@ static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l
     // budget keeps twice the waves resident.
     if (B.coarse && costs) {
         double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
-        ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes);
         int npc = costs ? B.nseg_max : 0;
         long blocks = div_up(npc, 4) + div_up(nedge_w, 4);
         if (blocks == 0) return;
+        ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes, true);
         if (NS == P && c.single_hb == 2)
-            launch_cost_edge_mode<NS, true, 2>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
+            launch_cost_edge_mode<NS, true, 2>(c, ps, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
         else
-            launch_cost_edge_mode<NS, true, 4>(c, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
+            launch_cost_edge_mode<NS, true, 4>(c, ps, a, mode, blocks, 0, slot0, spatial ? 1 : 0, npc);
     } else {
         if (costs) {
-            ProfScope ps(c, "plane_cost_L" + std::to_string(l), plane_cost_bytes(a.N, a.V, NS));
+            ProfScope ps(c, "plane_cost_L" + std::to_string(l), plane_cost_bytes(a.N, a.V, NS), true);
             int npc = (int)div_up(a.N, CHUNK);
-            launch_cost_edge_mode<NS, false>(c, a, mode, npc, PC_LDS_BYTES, slot0, 0, npc);
+            launch_cost_edge_mode<NS, false>(c, ps, a, mode, npc, PC_LDS_BYTES, slot0, 0, npc);
         }
         if (nedge_w > 0) {
-            ProfScope ps(c, "edge_L" + std::to_string(l), edge_bytes(a.E, a.V));
-            hipLaunchKernelGGL(edge_kernel, dim3(div_up(nedge_w, 4)), dim3(256), 0, c.stream, a);
+            ProfScope ps(c, "edge_L" + std::to_string(l), edge_bytes(a.E, a.V), true);
+            HBP_LAUNCH(ps, edge_kernel, dim3(div_up(nedge_w, 4)), dim3(256), 0, c.stream, a);
         }
     }
 }
@@ -1657,9 +1657,9 @@ static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float
         launch_cost_edge<P>(c, a, B, l, 2 * P, spatial && B.coarse, true, a.E > 0);
     }
     {
-        ProfScope psn(c, "node_L" + std::to_string(l), node_bytes(a.E, a.V));
-        hipLaunchKernelGGL(node_kernel, dim3(div_up((long)a.V * 16, 256)), dim3(256), 0, c.stream, a, sw.kind,
-                           next_kind, ndz, ndn);
+        ProfScope psn(c, "node_L" + std::to_string(l), node_bytes(a.E, a.V), true);
+        HBP_LAUNCH(psn, node_kernel, dim3(div_up((long)a.V * 16, 256)), dim3(256), 0, c.stream, a, sw.kind, next_kind,
+                   ndz, ndn);
     }
     HBP_CHECK(hipGetLastError());
 }

@@ -5,6 +5,7 @@
 // below (cube root, exp, floor-to-int, fixed-point sums, LCG) are the pinned forms of the
 // reference's FP64 libm calls and unordered atomics.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -309,10 +310,13 @@ struct Ctx {
     }
 };
 
-// Times the launches issued while it is alive (when profiling is on).
+// Times the launches issued while it is alive (when profiling is on).  With kernel_events the
+// scope records nothing itself: its one launch takes start_event() / stop_event(), which the
+// dispatch stamps with the kernel's own start and end (hipExtLaunchKernelGGL; see HBP_LAUNCH),
+// so the time excludes the queue's dispatch latency, as a kernel trace does.
 class ProfScope {
 public:
-    ProfScope(Ctx &c, const std::string &name, double bytes) : c_(c) {
+    ProfScope(Ctx &c, const std::string &name, double bytes, bool kernel_events = false) : c_(c), kev_(kernel_events) {
         if (!c.prof.on) return;
         if (!c.prof.only.empty() && name.compare(0, c.prof.only.size(), c.prof.only) != 0) return;
         if (c.prof.every > 1 && (c.prof.seen++ % c.prof.every) != 0) return;
@@ -320,21 +324,35 @@ public:
         a_ = c.prof.take();
         b_ = c.prof.take();
         bytes_ = bytes;
-        if (a_ && b_) (void)hipEventRecord(a_, c.stream);
+        if (a_ && b_ && !kev_) (void)hipEventRecord(a_, c.stream);
     }
     ~ProfScope() {
         if (!a_ || !b_) return;
-        (void)hipEventRecord(b_, c_.stream);
+        if (!kev_) (void)hipEventRecord(b_, c_.stream);
         c_.prof.pending.push_back({id_, a_, b_, bytes_});
         if (c_.prof.pending.size() > 4096) c_.prof.flush();
     }
+    hipEvent_t start_event() const { return kev_ && a_ && b_ ? a_ : nullptr; }
+    hipEvent_t stop_event() const { return kev_ && a_ && b_ ? b_ : nullptr; }
 
 private:
     Ctx &c_;
+    bool kev_ = false;
     int id_ = -1;
     hipEvent_t a_ = nullptr, b_ = nullptr;
     double bytes_ = 0.0;
 };
+
+// one kernel launch under a kernel_events ProfScope: stamped by the dispatch when the scope
+// times it, a plain launch otherwise
+#define HBP_LAUNCH(ps, kernel, grid, block, lds, stream, ...)                                                 \
+    do {                                                                                                     \
+        if ((ps).start_event())                                                                              \
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, (ps).start_event(), (ps).stop_event(), 0, \
+                                  __VA_ARGS__);                                                              \
+        else                                                                                                 \
+            hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                               \
+    } while (0)
 
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 inline long round_up(long a, long b) { return (a + b - 1) / b * b; }
