@@ -56,13 +56,14 @@ int CRTrees::compact_clus_label(int *clus_d, const int N) {
 // ------------------------------------------------------------------ SegHAC
 SegHAC::SegHAC(const cv::Mat &src, const Linkage link, const int num_nb_, const double sigma,
                const SegFormat seg_format, const int target_clus, const bool show_cycle)
-    : num_nb(num_nb_), m_link(link), m_target_clus(target_clus), width(src.cols), height(src.rows) {
+    : num_nb(num_nb_), m_link(link), m_target_clus(target_clus), m_sigma(sigma), width(src.cols), height(src.rows) {
     hbp_api::require_bgr(src);
     if (link < MinLink || link > WardLink) throw std::invalid_argument("SegHAC: unknown linkage");
     if (seg_format != LabelISPFormat) throw std::invalid_argument("SegHAC: only LabelISPFormat is implemented");
     if (target_clus < 1 || (long)target_clus >= (long)src.cols * src.rows)
         throw std::invalid_argument("SegHAC: target_clus must be in [1, width*height)");
-    if (sigma > 1.17549435e-38) throw std::invalid_argument("SegHAC: only sigma = 0 is implemented");
+    if (sigma > 1.17549435e-38 && std::round(3 * sigma) > 8)
+        throw std::invalid_argument("SegHAC: round(3*sigma) must be at most 8 (the Gaussian pre-blur's window)");
     if (show_cycle) throw std::invalid_argument("SegHAC: show_cycle is not implemented");
     if (num_nb != 4 && num_nb != 8) throw std::invalid_argument("SegHAC: num_nb must be 4 or 8");
     img = src.clone();
@@ -85,7 +86,7 @@ void SegHAC::set_seq_img(const cv::Mat &m) {
 
 void SegHAC::run_ms(std::vector<cv::Mat> *) {
     auto h = std::make_shared<hbp_api::PyramidHandle>();
-    check(hbp_pyramid_build_target(default_ctx(), img.data, width, height, (int)m_link, num_nb, 0.0, m_target_clus,
+    check(hbp_pyramid_build_target(default_ctx(), img.data, width, height, (int)m_link, num_nb, m_sigma, m_target_clus,
                                    &h->p),
           "SegHAC::run_ms");
     pyr = h;

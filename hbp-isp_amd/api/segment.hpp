@@ -1,7 +1,8 @@
 // Drop-in for segment.hpp of HBP-ISP (segment.hpp:8-340): the CRTrees superpixel pyramid,
 // backed by hbp_pyramid_build_target.  Implemented configuration: the four linkages
 // (CentroidLink, the one every script of the reference uses, MinLink, MaxLink, WardLink),
-// LabelISPFormat, any target_clus in [1, width*height) (1 in every script), sigma = 0, num_nb 4
+// LabelISPFormat, any target_clus in [1, width*height) (1 in every script), sigma with
+// round(3*sigma) <= 8 (0 in every script; above FLT_MIN the Gaussian pre-blur runs), num_nb 4
 // or 8; other values throw std::invalid_argument.
 #ifndef _SEGMENT_HPP_
 #define _SEGMENT_HPP_
@@ -49,6 +50,7 @@ private:
     const int num_nb;
     const Linkage m_link;
     const int m_target_clus;
+    const double m_sigma;
     int width, height;
     const int mean_channels = 6;
     std::shared_ptr<hbp_api::PyramidHandle> pyr;

@@ -1,4 +1,5 @@
 // extern "C" entry points of libhbp_hip.so (declared in include/hbp_hip.h).
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -372,7 +373,8 @@ int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol) {
 static void check_seg_args(int link, int num_nb, double sigma) {
     HBP_REQUIRE(link >= 0 && link <= 3, "link must be 0 (MinLink), 1 (MaxLink), 2 (CentroidLink) or 3 (WardLink)");
     HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
-    HBP_REQUIRE(sigma <= 1.17549435e-38, "only sigma = 0 (no Gaussian pre-blur) is implemented");
+    HBP_REQUIRE(!(sigma > 1.17549435e-38) || std::round(3 * sigma) <= 8,
+                "sigma too large for the Gaussian pre-blur: round(3*sigma) must be at most 8");
 }
 
 int hbp_pyramid_build_target_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int H, int link, int num_nb,
@@ -384,7 +386,7 @@ int hbp_pyramid_build_target_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int 
     use_device(ctx);
     auto *p = new hbp_pyramid();
     try {
-        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb, link, target_clus);
+        pyramid_build(ctx->c, p->p, d_bgr, W, H, num_nb, link, target_clus, sigma);
     } catch (...) {
         delete p;
         throw;
@@ -406,7 +408,7 @@ int hbp_pyramid_build_target(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int
     HBP_CHECK(hipMemcpyAsync(d.p, bgr, bytes, hipMemcpyHostToDevice, ctx->c.stream));
     auto *p = new hbp_pyramid();
     try {
-        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb, link, target_clus);
+        pyramid_build(ctx->c, p->p, d.p, W, H, num_nb, link, target_clus, sigma);
     } catch (...) {
         delete p;
         throw;

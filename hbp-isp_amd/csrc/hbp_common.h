@@ -49,6 +49,20 @@ __host__ __device__ inline double cbrt_newton(double x) {
     return y;
 }
 
+// t^2.4 on the sRGB linearisation domain ((0.04045 + 0.055)/1.055, 1 + 2^-20]: t^2 times the
+// fifth root of t^2, the root by Newton's step y <- y + (a/y^4 - y)/5 from 0.6 + 0.4a, eight
+// steps.
+__host__ __device__ inline double pow24_newton(double t) {
+    double a = t * t;
+    double y = 0.6 + 0.4 * a;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        double y2 = y * y;
+        y = y + (a / (y2 * y2) - y) / 5.0;
+    }
+    return a * y;
+}
+
 // exp(x), x <= 0: Cody-Waite split of ln2 (high part with 32 trailing zero bits), degree-13
 // Taylor polynomial on |r| <= ln2/2 (Horner), scaled by 2^n.
 __host__ __device__ inline double exp_cody_waite(double x) {

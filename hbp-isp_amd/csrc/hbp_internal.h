@@ -101,7 +101,7 @@ float *volume_f32(Ctx &c, Volume &v);
 
 // pyramid.hip
 void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link = kCentroidLink,
-                   int target_clus = 1);
+                   int target_clus = 1, double sigma = 0.0);
 void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label);
 int compact_labels(Ctx &c, int *d_label, int V);
 

@@ -50,6 +50,77 @@ __global__ void lab_kernel(const uint8_t *__restrict__ bgr, int W, int H, float4
     }
 }
 
+// The Gaussian pre-blur for sigma > FLT_MIN (img_CPU_to_GPU, segment.cu:795-809): gauss_blur
+// (imutil.cu:399-441) of img_uchar3_to_float4's (B, G, R, 1), then BGR_to_LAB and the level-0
+// mean array as in lab_kernel.  A block takes a 32x8 tile of pixels and stages its clamped
+// (32+2S)x(8+2S) neighbourhood as floats in LDS; each pixel's channels are left folds in f32
+// over r, then c, in [-S, S] of pixel * w[r][c] (the reference's order).  The blurred w channel
+// is the same fold of the weights for every pixel (every input w is 1), so the host passes it.
+// Where the reference's edge blocks race (several threads write the last column / row, from
+// windows centred beyond the image), each pixel here takes the window centred on itself.
+constexpr int kBlurTX = 32, kBlurTY = 8, kBlurMaxS = 8;
+struct GaussTaps {
+    float w[(2 * kBlurMaxS + 1) * (2 * kBlurMaxS + 1)];   // (2S+1)^2 used, row-major
+};
+
+__global__ __launch_bounds__(256) void blur_lab_kernel(const uint8_t *__restrict__ bgr, int W, int H, int S,
+                                                       GaussTaps taps, float wsum, float4 *__restrict__ lab,
+                                                       float *__restrict__ mean) {
+    constexpr int kTile = (kBlurTX + 2 * kBlurMaxS) * (kBlurTY + 2 * kBlurMaxS);
+    __shared__ float tb[kTile], tg[kTile], tr[kTile];
+    __shared__ float sw[(2 * kBlurMaxS + 1) * (2 * kBlurMaxS + 1)];
+    const int F = 2 * S + 1;
+    const int tw = kBlurTX + 2 * S, th = kBlurTY + 2 * S;
+    const int x0 = (int)blockIdx.x * kBlurTX - S, y0 = (int)blockIdx.y * kBlurTY - S;
+    for (int k = threadIdx.x; k < F * F; k += blockDim.x) sw[k] = taps.w[k];
+    for (int k = threadIdx.x; k < tw * th; k += blockDim.x) {
+        int ty = k / tw, tx = k - ty * tw;
+        int yy = min(max(y0 + ty, 0), H - 1), xx = min(max(x0 + tx, 0), W - 1);
+        const uint8_t *px = bgr + 3 * ((long)yy * W + xx);
+        tb[k] = (float)px[0];
+        tg[k] = (float)px[1];
+        tr[k] = (float)px[2];
+    }
+    __syncthreads();
+    const int lx = (int)threadIdx.x % kBlurTX, ly = (int)threadIdx.x / kBlurTX;
+    const int x = (int)blockIdx.x * kBlurTX + lx, y = (int)blockIdx.y * kBlurTY + ly;
+    if (x >= W || y >= H) return;
+    float fb = 0.f, fg = 0.f, fr = 0.f;
+    for (int r = 0; r < F; ++r) {
+        const int row = (ly + r) * tw + lx;
+        for (int c = 0; c < F; ++c) {
+            const float w = sw[r * F + c];
+            fb = fb + tb[row + c] * w;
+            fg = fg + tg[row + c] * w;
+            fr = fr + tr[row + c] * w;
+        }
+    }
+    double b = fb / 255.0, g = fg / 255.0, rr = fr / 255.0;
+    rr = (rr > 0.04045) ? pow24_newton((rr + 0.055) / 1.055) : rr / 12.92;
+    g = (g > 0.04045) ? pow24_newton((g + 0.055) / 1.055) : g / 12.92;
+    b = (b > 0.04045) ? pow24_newton((b + 0.055) / 1.055) : b / 12.92;
+    double X = (rr * 0.412453 + g * 0.357580 + b * 0.180423) / 0.950456;
+    double Y = (rr * 0.212671 + g * 0.715160 + b * 0.072169) / 1.0;
+    double Z = (rr * 0.019334 + g * 0.119193 + b * 0.950227) / 1.088754;
+    const double epsilon = 216 / 24389.0;
+    const double k = 841 / 108.0;
+    const double kb = 16 / 116.0;
+    X = (X > epsilon) ? cbrt_newton(X) : (k * X) + kb;
+    Y = (Y > epsilon) ? cbrt_newton(Y) : (k * Y) + kb;
+    Z = (Z > epsilon) ? cbrt_newton(Z) : (k * Z) + kb;
+    const float L = (float)((116 * Y) - 16);
+    const float A = (float)(500 * (X - Y));
+    const float B = (float)(200 * (Y - Z));
+    const long n = (long)W * H, i = (long)y * W + x;
+    lab[i] = make_float4(L, A, B, wsum);
+    mean[i] = L;
+    mean[n + i] = A;
+    mean[2 * n + i] = B;
+    mean[3 * n + i] = (float)x;
+    mean[4 * n + i] = (float)y;
+    mean[5 * n + i] = wsum;
+}
+
 // segment.cu:14-22
 __device__ __forceinline__ float dist_px(float4 s, float4 t) {
     float dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
@@ -590,6 +661,33 @@ static int bits_for(long n) {
     return b;
 }
 
+// compute_gauss_kernel (imutil.cu:443-471), host code in the reference too: S = round(3 sigma),
+// w[r][c] = exp(-(r^2 + c^2)/(2 sigma^2)) in FP64 (libm), r then c over [-S, S], each times
+// 1/sum, rounded to f32; *wsum = the f32 left fold of the weights (the blurred w channel).
+static int gauss_taps(double sigma, GaussTaps &t, float *wsum) {
+    const int S = (int)std::round(3 * sigma);
+    HBP_REQUIRE(S <= kBlurMaxS, "sigma too large for the Gaussian pre-blur: round(3*sigma) must be at most 8 "
+                                "(the reference's blur launch needs (16 + 2S)^2 <= 1024 threads)");
+    const int F = 2 * S + 1;
+    double f[(2 * kBlurMaxS + 1) * (2 * kBlurMaxS + 1)];
+    double sum = 0;
+    for (int r = -S; r <= S; ++r)
+        for (int c = -S; c <= S; ++c) {
+            double val = std::exp(-(double)(r * r + c * c) / (2 * sigma * sigma));
+            f[(r + S) * F + c + S] = val;
+            sum += val;
+        }
+    sum = 1. / sum;
+    float acc = 0.f;
+    for (int i = 0; i < F * F; ++i) {
+        t.w[i] = (float)(f[i] * sum);
+        acc = acc + 1.0f * t.w[i];
+    }
+    for (int i = F * F; i < (2 * kBlurMaxS + 1) * (2 * kBlurMaxS + 1); ++i) t.w[i] = 0.f;
+    *wsum = acc;
+    return S;
+}
+
 static void upload_srgb_lut(Ctx &c) {
     static double lut[256];
     static bool init = false;
@@ -765,7 +863,8 @@ static void target_clus_relabel(Ctx &c, int *clus, int V, int Vn, const float *d
 // SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, any linkage; target_clus > 1 ends
 // the pyramid at the first level of at most that many clusters, re-labelled to exactly
 // target_clus (get_target_clus, then save_output's stop test, segment.cu:1103-1110, 1299-1306).
-void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link, int target_clus) {
+void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link, int target_clus,
+                   double sigma) {
     HBP_REQUIRE(num_nb == 4 || num_nb == 8, "num_nb must be 4 or 8");
     HBP_REQUIRE(link >= kMinLink && link <= kWardLink, "linkage must be 0 (Min), 1 (Max), 2 (Centroid) or 3 (Ward)");
     const bool use_mean = link == kCentroidLink || link == kWardLink;
@@ -773,6 +872,12 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
     long N = (long)W * H;
     HBP_REQUIRE(N < (1L << 24), "images are limited to 2^24 pixels");
     HBP_REQUIRE(target_clus >= 1 && target_clus < N, "target_clus must be in [1, width*height)");
+    // the blur only for sigma > FLT_MIN (segment.cu:805)
+    const bool blur = sigma > 1.17549435e-38;
+    GaussTaps taps;
+    float wsum = 1.f;
+    const int S = blur ? gauss_taps(sigma, taps, &wsum) : 0;
+    HBP_REQUIRE(!blur || (H + kBlurTY - 1) / kBlurTY <= 65535, "image too tall for the pre-blur's grid");
     p.W = W;
     p.H = H;
     p.num_nb = num_nb;
@@ -795,7 +900,11 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
     // Min/Max: the per-edge distances carried from level to level (dist_d, segment.cu:838-844)
     DevBuf<float> edist;
     if (!use_mean) edist.alloc(c.pool, (size_t)num_nb * N);
-    hipLaunchKernelGGL(lab_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, d_bgr, W, H, lab.p, p.mean.p);
+    if (blur)
+        hipLaunchKernelGGL(blur_lab_kernel, dim3((W + kBlurTX - 1) / kBlurTX, (H + kBlurTY - 1) / kBlurTY), dim3(256), 0,
+                           c.stream, d_bgr, W, H, S, taps, wsum, lab.p, p.mean.p);
+    else
+        hipLaunchKernelGGL(lab_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, d_bgr, W, H, lab.p, p.mean.p);
     hipLaunchKernelGGL(nn_grid_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, lab.p, W, H, num_nb, nn.p,
                        dist_min.p, p.bd.p, p.bdl.p, edist.p);
     HBP_CHECK(hipGetLastError());

@@ -191,10 +191,10 @@ class Ctx:
         return Volume(self, None, None, vol.shape[0], right_view, f32=vol)
 
     # ---- (b)
-    def pyramid(self, bgr, num_nb=4, link=2, target_clus=1):
-        """SegHAC(bgr, link, num_nb, 0, LabelISPFormat, target_clus).run_ms(); link 0 Min, 1 Max,
-        2 Centroid (default), 3 Ward"""
-        return Pyramid(self, bgr, num_nb, link, target_clus)
+    def pyramid(self, bgr, num_nb=4, link=2, target_clus=1, sigma=0.0):
+        """SegHAC(bgr, link, num_nb, sigma, LabelISPFormat, target_clus).run_ms(); link 0 Min,
+        1 Max, 2 Centroid (default), 3 Ward; sigma > 0 blurs first (round(3 sigma) <= 8)"""
+        return Pyramid(self, bgr, num_nb, link, target_clus, sigma)
 
     # ---- (c)
     def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
@@ -291,15 +291,16 @@ class Volume:
 
 
 class Pyramid:
-    def __init__(self, ctx: Ctx, bgr, num_nb=4, link=2, target_clus=1):
+    def __init__(self, ctx: Ctx, bgr, num_nb=4, link=2, target_clus=1, sigma=0.0):
         self.ctx = ctx
         H, W = bgr.shape[:2]
         h = C.c_void_p()
         if target_clus == 1:
-            _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, 0.0, C.byref(h)))
+            _check(ctx.lib.hbp_pyramid_build(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, float(sigma),
+                                             C.byref(h)))
         else:
-            _check(ctx.lib.hbp_pyramid_build_target(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb, 0.0,
-                                                    target_clus, C.byref(h)))
+            _check(ctx.lib.hbp_pyramid_build_target(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb,
+                                                    float(sigma), target_clus, C.byref(h)))
         self.h = h
         self.W, self.H = W, H
 

@@ -92,7 +92,10 @@ int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol);
 
 /* ---- (b) superpixel pyramid: SegHAC(src, link, num_nb, sigma, LabelISPFormat, 1, false) +
  *      run_ms() (segment.hpp:248-309, segment.cu:1279-1321).  link 0 MinLink, 1 MaxLink,
- *      2 CentroidLink (the reference's default), 3 WardLink; sigma 0; num_nb 4 or 8. */
+ *      2 CentroidLink (the reference's default), 3 WardLink; num_nb 4 or 8.  sigma > FLT_MIN
+ *      blurs the image first (img_CPU_to_GPU, segment.cu:795-809; imutil.cu:399-498): a
+ *      (2S+1)^2 Gaussian, S = round(3 sigma) <= 8 (larger is an error, as the reference's launch
+ *      of (16+2S)^2 threads fails), clamped borders; 0 (every script) skips it. */
 int hbp_pyramid_build(hbp_ctx *ctx, const uint8_t *bgr, int width, int height, int link,
                       int num_nb, double sigma, hbp_pyramid **out);
 int hbp_pyramid_build_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int width, int height, int link,

@@ -19,6 +19,7 @@ extern "C" {
 /* pinned scalar rules (exported so tests can pin them against libm) */
 double orc_pin_cbrt(double x);
 double orc_pin_exp(double x);
+double orc_pin_pow24(double t);
 float orc_i64_to_f32(int64_t q);
 int orc_f2i_rd(float x);
 int orc_fixed_shift(float lambda, float tau, int use_cost_vol, int npix);
@@ -51,6 +52,16 @@ void *orc_pyramid_build(const uint8_t *bgr, int W, int H, int num_nb);
 void *orc_pyramid_build_link(const uint8_t *bgr, int W, int H, int num_nb, int link);
 /* ... and target_clus (1 <= target_clus < W*H; get_target_clus, segment.cu:954-1041) */
 void *orc_pyramid_build_target(const uint8_t *bgr, int W, int H, int num_nb, int link, int target_clus);
+/* ... and the Gaussian pre-blur (sigma > FLT_MIN; imutil.cu:399-498, segment.cu:795-809);
+ * NULL for round(3 sigma) > 8 */
+void *orc_pyramid_build_sigma(const uint8_t *bgr, int W, int H, int num_nb, int link, int target_clus,
+                              double sigma);
+/* the blur's weights ((2S+1)^2 floats, row-major) and the blurred float4 (B, G, R, w) image;
+ * both return S = round(3 sigma), or -1 above 8 */
+int orc_gauss_weights(double sigma, float *w);
+int orc_gauss_blur(const uint8_t *bgr, int W, int H, double sigma, float *out4);
+/* BGR_to_LAB on float4 pixels (w passes through) */
+void orc_lab_f(const float *in4, long n, float *lab4);
 int orc_pyr_levels(void *p);
 void orc_pyr_level_size(void *p, int level, int *V, int *E);
 void orc_pyr_get_level(void *p, int level, int *clus, float *mean, int *bd, int *bdl);

@@ -70,6 +70,85 @@ void lab_image(const uint8_t *bgr, int W, int H, float *lab4) {
     for (long i = 0; i < n; ++i) lab_pixel(bgr[3 * i], bgr[3 * i + 1], bgr[3 * i + 2], lab4 + 4 * i);
 }
 
+// compute_gauss_kernel (imutil.cu:443-471) with S = round(3 sigma) (img_gauss_blur 481):
+// w[r][c] = exp(-(r^2 + c^2)/(2 sigma^2)) in FP64 (libm exp, host code in the reference too),
+// r then c over [-S, S], each times 1/sum, rounded to f32.  Returns S.
+int gauss_weights(double sigma, std::vector<float> &w) {
+    const int S = (int)std::round(3 * sigma);
+    const int F = 2 * S + 1;
+    std::vector<double> f((size_t)F * F);
+    double sum = 0;
+    for (int r = -S; r <= S; ++r)
+        for (int c = -S; c <= S; ++c) {
+            double val = std::exp(-(double)(r * r + c * c) / (2 * sigma * sigma));
+            f[(size_t)(r + S) * F + c + S] = val;
+            sum += val;
+        }
+    sum = 1. / sum;
+    w.assign((size_t)F * F, 0.f);
+    for (size_t i = 0; i < f.size(); ++i) w[i] = (float)(f[i] * sum);
+    return S;
+}
+
+// gauss_blur (imutil.cu:399-441) of img_uchar3_to_float4's (B, G, R, 1) (imutil.cu:170-185):
+// out(y, x) = sum over r, then c, in [-S, S] of in(clamp(y + r), clamp(x + c)) * w[r][c], each
+// channel a left fold in f32 from 0.  The reference's edge thread blocks also write the last
+// column / row from threads whose windows are centred beyond the image (a write race); the
+// pinned value is the window centred on the pixel itself.  The reference's launch needs
+// (16 + 2S)^2 <= 1024 threads, so S <= 8; returns S, or -1 (nothing written) above that.
+int gauss_blur(const uint8_t *bgr, int W, int H, double sigma, float *out4) {
+    std::vector<float> w;
+    const int S = gauss_weights(sigma, w);
+    if (S > 8) return -1;
+    const int F = 2 * S + 1;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int r = -S; r <= S; ++r) {
+                int yy = std::min(std::max(0, y + r), H - 1);
+                for (int c = -S; c <= S; ++c) {
+                    int xx = std::min(std::max(0, x + c), W - 1);
+                    const uint8_t *px = bgr + 3 * ((long)yy * W + xx);
+                    float wt = w[(size_t)(r + S) * F + c + S];
+                    acc[0] = acc[0] + (float)px[0] * wt;
+                    acc[1] = acc[1] + (float)px[1] * wt;
+                    acc[2] = acc[2] + (float)px[2] * wt;
+                    acc[3] = acc[3] + 1.0f * wt;
+                }
+            }
+            std::memcpy(out4 + 4 * ((long)y * W + x), acc, sizeof(acc));
+        }
+    return S;
+}
+
+// imutil.cu:4-40 (BGR_to_LAB) on a float pixel (the blurred image): the same arithmetic as
+// lab_pixel with pow((c + 0.055)/1.055, 2.4) by pin_pow24; w passes through.
+static void lab_pixel_f(const float *in, float out[4]) {
+    double b = in[0] / 255.0, g = in[1] / 255.0, r = in[2] / 255.0;
+    r = (r > 0.04045) ? pin_pow24((r + 0.055) / 1.055) : r / 12.92;
+    g = (g > 0.04045) ? pin_pow24((g + 0.055) / 1.055) : g / 12.92;
+    b = (b > 0.04045) ? pin_pow24((b + 0.055) / 1.055) : b / 12.92;
+    double x = (r * 0.412453 + g * 0.357580 + b * 0.180423) / 0.950456;
+    double y = (r * 0.212671 + g * 0.715160 + b * 0.072169) / 1.0;
+    double z = (r * 0.019334 + g * 0.119193 + b * 0.950227) / 1.088754;
+    const double epsilon = 216 / 24389.0;
+    const double k = 841 / 108.0;
+    const double kb = 16 / 116.0;
+    x = (x > epsilon) ? pin_cbrt(x) : (k * x) + kb;
+    y = (y > epsilon) ? pin_cbrt(y) : (k * y) + kb;
+    z = (z > epsilon) ? pin_cbrt(z) : (k * z) + kb;
+    out[0] = (float)((116 * y) - 16);
+    out[1] = (float)(500 * (x - y));
+    out[2] = (float)(200 * (y - z));
+    out[3] = in[3];
+}
+
+void lab_image_f(const float *in4, long n, float *lab4) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) lab_pixel_f(in4 + 4 * i, lab4 + 4 * i);
+}
+
 // segment.cu:57-195 (compute_1nn_grid_k): neighbours in the order up, down, left, right
 // (then ul, ur, dr, dl for num_nb = 8); first strict minimum wins; border slots become
 // self-loops (i, i) that take no part in the minimum.  nn starts at 0, dist_min at FLT_MAX.
@@ -298,19 +377,28 @@ static void target_clus_labels(int *clus, int V, int Vn, const float *dist_min, 
     }
 }
 
-// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, sigma = 0 (no blur), pernalize on,
-// with any of the four linkages (use_mean = Centroid or Ward, segment.hpp:248).  target_clus > 1
-// ends the pyramid at the first level with at most that many clusters, re-labelled to exactly
-// target_clus (get_target_clus, then save_output's stop test, segment.cu:1103-1110, 1299-1306).
-void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link, int target_clus) {
+// SegHAC::run_ms (segment.cu:1279-1321) for LabelISPFormat, pernalize on, with any of the four
+// linkages (use_mean = Centroid or Ward, segment.hpp:248).  sigma > FLT_MIN blurs the image
+// before the Lab conversion (img_CPU_to_GPU, segment.cu:795-809); the blurred w channel becomes
+// the level-0 weight.  target_clus > 1 ends the pyramid at the first level with at most that
+// many clusters, re-labelled to exactly target_clus (get_target_clus, then save_output's stop
+// test, segment.cu:1103-1110, 1299-1306).  Returns false (P untouched) for S = round(3 sigma) > 8.
+bool pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link, int target_clus,
+                   double sigma) {
     const bool use_mean = link == kCentroidLink || link == kWardLink;
     long N = (long)W * H;
+    std::vector<float> lab(4 * N);
+    if (sigma > FLT_MIN) {
+        std::vector<float> blur(4 * N);
+        if (gauss_blur(bgr, W, H, sigma, blur.data()) < 0) return false;
+        lab_image_f(blur.data(), N, lab.data());
+    } else {
+        lab_image(bgr, W, H, lab.data());
+    }
     P.W = W;
     P.H = H;
     P.top = target_clus;
     P.levels.clear();
-    std::vector<float> lab(4 * N);
-    lab_image(bgr, W, H, lab.data());
     std::vector<int> nn(N);
     std::vector<float> dist_min(N);
     std::vector<int> bd(2 * num_nb * N);
@@ -433,6 +521,7 @@ void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int
         V = Vn;
         E = En;
     }
+    return true;
 }
 
 }  // namespace orc
@@ -461,11 +550,29 @@ void *orc_pyramid_build_link(const uint8_t *bgr, int W, int H, int num_nb, int l
     return p;
 }
 void *orc_pyramid_build_target(const uint8_t *bgr, int W, int H, int num_nb, int link, int target_clus) {
+    return orc_pyramid_build_sigma(bgr, W, H, num_nb, link, target_clus, 0.0);
+}
+void *orc_pyramid_build_sigma(const uint8_t *bgr, int W, int H, int num_nb, int link, int target_clus, double sigma) {
     if (link < 0 || link > 3 || target_clus < 1 || (long)target_clus >= (long)W * H) return nullptr;
     auto *p = new orc::Pyramid();
-    orc::pyramid_build(bgr, W, H, num_nb, *p, link, target_clus);
+    if (!orc::pyramid_build(bgr, W, H, num_nb, *p, link, target_clus, sigma)) {
+        delete p;
+        return nullptr;
+    }
     return p;
 }
+int orc_gauss_weights(double sigma, float *w) {
+    std::vector<float> v;
+    int S = orc::gauss_weights(sigma, v);
+    if (S > 8) return -1;
+    std::memcpy(w, v.data(), sizeof(float) * v.size());
+    return S;
+}
+int orc_gauss_blur(const uint8_t *bgr, int W, int H, double sigma, float *out4) {
+    return orc::gauss_blur(bgr, W, H, sigma, out4);
+}
+void orc_lab_f(const float *in4, long n, float *lab4) { orc::lab_image_f(in4, n, lab4); }
+double orc_pin_pow24(double t) { return orc::pin_pow24(t); }
 int orc_pyr_levels(void *p) { return (int)((orc::Pyramid *)p)->levels.size(); }
 void orc_pyr_level_size(void *p, int l, int *V, int *E) {
     auto &L = ((orc::Pyramid *)p)->levels[l];

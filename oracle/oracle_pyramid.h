@@ -24,6 +24,9 @@ struct Pyramid {
 };
 
 void lab_image(const uint8_t *bgr, int W, int H, float *lab4);
+void lab_image_f(const float *in4, long n, float *lab4);
+int gauss_weights(double sigma, std::vector<float> &w);
+int gauss_blur(const uint8_t *bgr, int W, int H, double sigma, float *out4);
 void nn_grid(const float *lab4, int W, int H, int num_nb, int *nn, float *dist_min, int *bd,
              float *edist = nullptr);
 void cc_max(const int *nn, int V, int *label);
@@ -31,7 +34,7 @@ int compact(int *label, int V);
 void crtrees_literal(const int *nn, int V, int order, int *out);
 // linkages as in segment.hpp:13: 0 MinLink, 1 MaxLink, 2 CentroidLink, 3 WardLink
 enum { kMinLink = 0, kMaxLink = 1, kCentroidLink = 2, kWardLink = 3 };
-void pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link = kCentroidLink,
-                   int target_clus = 1);
+bool pyramid_build(const uint8_t *bgr, int W, int H, int num_nb, Pyramid &P, int link = kCentroidLink,
+                   int target_clus = 1, double sigma = 0.0);
 
 }  // namespace orc

@@ -24,6 +24,11 @@ def _load():
     sig = {
         "orc_pin_cbrt": (C.c_double, [C.c_double]),
         "orc_pin_exp": (C.c_double, [C.c_double]),
+        "orc_pin_pow24": (C.c_double, [C.c_double]),
+        "orc_gauss_weights": (C.c_int, [C.c_double, _f32p]),
+        "orc_gauss_blur": (C.c_int, [_u8p, C.c_int, C.c_int, C.c_double, _f32p]),
+        "orc_lab_f": (None, [_f32p, C.c_long, _f32p]),
+        "orc_pyramid_build_sigma": (C.c_void_p, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
         "orc_i64_to_f32": (C.c_float, [C.c_int64]),
         "orc_f2i_rd": (C.c_int, [C.c_float]),
         "orc_fixed_shift": (C.c_int, [C.c_float, C.c_float, C.c_int, C.c_int]),
@@ -93,11 +98,11 @@ def cost_volume(left, right, D, right_view=False):
 
 
 class Pyramid:
-    def __init__(self, bgr, num_nb=4, link=2, target_clus=1):
+    def __init__(self, bgr, num_nb=4, link=2, target_clus=1, sigma=0.0):
         H, W = bgr.shape[:2]
         self.W, self.H = W, H
-        self.h = lib.orc_pyramid_build_target(np.ascontiguousarray(bgr), W, H, num_nb, link, target_clus)
-        assert self.h, "bad linkage or target_clus"
+        self.h = lib.orc_pyramid_build_sigma(np.ascontiguousarray(bgr), W, H, num_nb, link, target_clus, sigma)
+        assert self.h, "bad linkage, target_clus or sigma"
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -200,3 +205,25 @@ def eval_bad_pixels(disp, gt, nonocc, threshold):
     lib.orc_eval_bad_pixels(np.ascontiguousarray(disp, np.float32), np.ascontiguousarray(gt, np.float32),
                             np.ascontiguousarray(nonocc, np.uint8), W, H, float(threshold), c)
     return [(int(c[0]), int(c[1])), (int(c[2]), int(c[3])), (int(c[4]), int(c[5]))]
+
+
+def gauss_weights(sigma):
+    w = np.zeros(289, np.float32)
+    S = lib.orc_gauss_weights(sigma, w)
+    assert S >= 0, "round(3 sigma) > 8"
+    F = 2 * S + 1
+    return S, w[:F * F].reshape(F, F)
+
+
+def gauss_blur(bgr, sigma):
+    H, W = bgr.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    S = lib.orc_gauss_blur(np.ascontiguousarray(bgr), W, H, sigma, out)
+    return S, out
+
+
+def lab_f(img4):
+    img4 = np.ascontiguousarray(img4, np.float32)
+    out = np.zeros_like(img4)
+    lib.orc_lab_f(img4, img4.size // 4, out)
+    return out

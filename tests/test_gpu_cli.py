@@ -39,13 +39,13 @@ def write_ppm(path, bgr):
         f.write(np.ascontiguousarray(bgr[..., ::-1]).tobytes())
 
 
-def run_cli(tmp_path, left, right, D, cost="Census", lr=False):
+def run_cli(tmp_path, left, right, D, cost="Census", lr=False, sigma="0"):
     if not os.path.exists(CLI):
         pytest.fail("bin/stereo_hbp_isp is not built (make -C hbp-isp_amd)")
     lp, rp, out = tmp_path / "l.ppm", tmp_path / "r.ppm", str(tmp_path / "res")
     write_ppm(lp, left)
     write_ppm(rp, right)
-    args = [CLI, str(lp), str(rp), str(D), "2", "4", "0", str(ALPHA), str(TAU_S), out, "1", str(MIN_LEVEL), cost]
+    args = [CLI, str(lp), str(rp), str(D), "2", "4", sigma, str(ALPHA), str(TAU_S), out, "1", str(MIN_LEVEL), cost]
     args += ["lr"] if lr else []
     say("cli:", " ".join(args[3:]))
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
@@ -53,8 +53,8 @@ def run_cli(tmp_path, left, right, D, cost="Census", lr=False):
     return out
 
 
-def oracle_view(img, vol, wta, D, tau=None):
-    bp = O.BP(O.Pyramid(img), vol, wta, D, alpha=ALPHA, tau_s=TAU_S, min_level=MIN_LEVEL, tau=tau)
+def oracle_view(img, vol, wta, D, tau=None, sigma=0.0):
+    bp = O.BP(O.Pyramid(img, sigma=sigma), vol, wta, D, alpha=ALPHA, tau_s=TAU_S, min_level=MIN_LEVEL, tau=tau)
     bp.step()
     return bp.map()
 
@@ -65,6 +65,16 @@ def test_cli_census_with_median7(tmp_path):
     out = run_cli(tmp_path, left, right, D)
     vol, wta = O.cost_volume(left, right, D)
     disp, _ = oracle_view(left, vol, wta, D)
+    O.lib.orc_median7(disp, W, H)
+    assert same_bits(M.read_pfm(out + "_hbp_isp_disp.pfm"), disp)
+
+
+def test_cli_census_with_gaussian_preblur(tmp_path):
+    W, H, D = 90, 70, 16
+    left, right, _ = synth.make_pair(74, W, H, D, (3, 6))
+    out = run_cli(tmp_path, left, right, D, sigma="0.8")
+    vol, wta = O.cost_volume(left, right, D)
+    disp, _ = oracle_view(left, vol, wta, D, sigma=0.8)
     O.lib.orc_median7(disp, W, H)
     assert same_bits(M.read_pfm(out + "_hbp_isp_disp.pfm"), disp)
 

@@ -158,6 +158,41 @@ def test_bp_on_target_clus_pyramid_bit_exact(ctx):
     assert same_bits(do, dg) and same_bits(plo, plg)
 
 
+@pytest.mark.parametrize("sigma,W,H", [(0.1, 64, 48), (0.7, 157, 93), (1.0, 33, 9), (2.83, 120, 90),
+                                         (1.0, 1472, 984)])
+def test_pyramid_blurred_levels_bit_exact(ctx, sigma, W, H):
+    say(f"pyramid sigma={sigma} {W}x{H}")
+    left, _, _ = synth.make_pair(21, W, H, 16, (3, 8))
+    po = O.Pyramid(left, sigma=sigma)
+    pg = ctx.pyramid(left, sigma=sigma)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+
+
+def test_bp_on_blurred_pyramid_bit_exact(ctx):
+    say("bp on sigma=1.2")
+    left, right = _bp_pair(17, 120, 90, 24)
+    vo, wo = O.cost_volume(left, right, 24)
+    bo = O.BP(O.Pyramid(left, sigma=1.2), vo, wo, 24, min_level=2)
+    bo.step()
+    do, plo = bo.map()
+    bg = ctx.bp(ctx.pyramid(left, sigma=1.2), ctx.cost_volume(left, right, 24), min_level=2)
+    dg, plg = bg.run()
+    assert same_bits(do, dg) and same_bits(plo, plg)
+
+
+def test_sigma_beyond_the_blur_window_is_an_error(ctx):
+    left, _, _ = synth.make_pair(16, 32, 24, 8, (3, 6))
+    with pytest.raises(H.HbpError, match="sigma"):
+        ctx.pyramid(left, sigma=3.0)
+
+
 def test_target_clus_out_of_range_is_an_error(ctx):
     left, _, _ = synth.make_pair(16, 32, 24, 8, (3, 6))
     for bad in (0, 32 * 24):
