@@ -36,6 +36,9 @@
 namespace hbp {
 
 constexpr int P = kP, K = kK;
+// incoming messages a node update keeps in registers; the node-major message and reverse-slot
+// arrays carry this many padding rows so that a node's 16 row reads stay in bounds
+constexpr int kNodeMaxDeg = 16;
 constexpr int SEG_R = 8, SEG = 64 * SEG_R;   // pixels per plane-cost segment (coarse levels)
 
 // replicas of each fixed-point accumulator: work item i adds into replica i mod rrep, which
@@ -331,18 +334,20 @@ constexpr size_t PC_LDS_BYTES = (size_t)MAXC * P * 16 + (size_t)MAXC * P * 8 + (
 // so that all loads are in flight together
 constexpr int kMaxReplicas = 16;
 __device__ __forceinline__ int64_t acc_sum(const SweepArgs &a, long id, int k) {
+    const unsigned long long *__restrict__ p = a.acc + (size_t)id * a.rrep * K + k;
     unsigned long long v[kMaxReplicas];
 #pragma unroll
-    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < a.rrep ? a.acc[(id * a.rrep + r) * K + k] : 0ull;
+    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < a.rrep ? p[r * K] : 0ull;
     unsigned long long t = 0ull;
 #pragma unroll
     for (int r = 0; r < kMaxReplicas; ++r) t += v[r];
     return (int64_t)t;
 }
 __device__ __forceinline__ void acc_zero(const SweepArgs &a, long id, int k) {
+    unsigned long long *__restrict__ p = a.acc + (size_t)id * a.rrep * K + k;
 #pragma unroll
     for (int r = 0; r < kMaxReplicas; ++r)
-        if (r < a.rrep) a.acc[(id * a.rrep + r) * K + k] = 0ull;
+        if (r < a.rrep) p[r * K] = 0ull;
 }
 
 // init_bp (hbp_isp.cu:930-950): copy_prior_plane (56-73), init_rand_seed_k (570-591), plus the
@@ -955,7 +960,7 @@ __device__ __forceinline__ void row_revl(int rv, int (&r)[sizeof...(I)], std::in
 }
 
 __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn) {
-    constexpr int MAXDEG = 16;   // incoming messages held in registers (the rest are re-read)
+    constexpr int MAXDEG = kNodeMaxDeg;   // incoming messages held in registers (the rest are re-read)
     const int lane = threadIdx.x & 63;
     const int k = lane & 15;
     const int g = lane & ~15;
@@ -971,11 +976,13 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     const float4 pin = a.plane[s * K + kk];
     const float2 mxy = a.mxy[s];
     const long nm = (long)s * a.maxdeg;
-    const int dmax = a.maxdeg - 1;
-    const int rv = a.revl[nm + (k < dmax ? k : dmax)];
+    // rows nm .. nm+15, past deg into the next node's rows or the arrays' kNodeMaxDeg padding
+    // rows (never used), at constant offsets from one address
+    const int rv = a.revl[nm + k];
+    const float *__restrict__ mtp = a.msg_t + nm * K + kk;
     float mt[MAXDEG];
 #pragma unroll
-    for (int i = 0; i < MAXDEG; ++i) mt[i] = a.msg_t[(nm + (i < dmax ? i : dmax)) * K + kk];
+    for (int i = 0; i < MAXDEG; ++i) mt[i] = mtp[i * K];
     const bool seeds = this_kind == kSpatial && s < a.Vh && k == 0;
     const long sh = s < a.Vh ? s : a.Vh - 1;
     uint32_t sc = a.s_child[sh], sn = a.s_neighbor[sh];
@@ -986,8 +993,7 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     uint4 q = make_uint4(0u, 0u, 0u, 0u);
     float z0 = 0.f, z1 = 0.f, z2 = 0.f;
     if (next_kind == kRansac) {
-        const int adv = cand ? 3 * (k - 2 * P) : 0;
-        for (int i = 0; i < adv; ++i) sk = lcg_next(sk);
+        sk = lcg_skip<3>(s0, cand ? k - 2 * P : 0);
         int p0, p1, p2;
         ransac_draw(sk, pn, p0, p1, p2);
         q = make_uint4(a.pxy[ppos + p0], a.pxy[ppos + p1], a.pxy[ppos + p2], 0u);
@@ -1043,11 +1049,9 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
         if (k < 2 * P) msg[(long)a.revl[nm + i] * 8 + k] = oi;
     }
     if (seeds) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            sc = lcg_next(sc);
-            sn = lcg_next(sn);
-        }
+        constexpr LcgJump jp = lcg_jump_coef(P);
+        sc = (jp.a * sc + jp.c) % 2147483648u;
+        sn = (jp.a * sn + jp.c) % 2147483648u;
         a.s_child[s] = sc;
         a.s_neighbor[s] = sn;
     }
@@ -1061,15 +1065,13 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
                 a.plane[s * K + k] = fit_plane(A, B, C);
                 if (k == K - 1) a.point[s] = q;
             } else {
-                uint32_t sr = s0;
-                for (int i = 0; i < 4 * (k - 2 * P); ++i) sr = lcg_next(sr);
+                uint32_t sr = lcg_skip<4>(s0, k - 2 * P);
                 a.plane[s * K + k] = random_plane(sr, pl_n, mxy.x, mxy.y, max_dz, max_dn);
             }
             if (k == 2 * P) {
-                const int steps = next_kind == kRansac ? 3 : 4;
-                uint32_t sf = s0;
-                for (int i = 0; i < steps * P; ++i) sf = lcg_next(sf);
-                a.s_ransac[s] = sf;
+                constexpr LcgJump j3 = lcg_jump_coef(3 * P), j4 = lcg_jump_coef(4 * P);
+                const LcgJump jf = next_kind == kRansac ? j3 : j4;
+                a.s_ransac[s] = (jf.a * s0 + jf.c) % 2147483648u;
             }
         }
     }
@@ -1545,13 +1547,13 @@ void bp_setup(Ctx &c, BP &b) {
         B.maxdeg = std::max(1, hmax[l]);
         slots = std::max(slots, (size_t)B.V * B.maxdeg);
         B.eloc.alloc(c.pool, std::max<long>(B.E, 1));
-        B.revl.alloc(c.pool, (size_t)B.V * B.maxdeg);
+        B.revl.alloc(c.pool, (size_t)B.V * B.maxdeg + kNodeMaxDeg);
         if (B.E > 0)
             hipLaunchKernelGGL(node_major_kernel, dim3(grid_for(c, B.E)), dim3(256), 0, c.stream, B.bd, B.off.p,
                                B.rev.p, B.E, B.maxdeg, B.eloc.p, B.revl.p);
     }
     b.msg.alloc(c.pool, (size_t)maxE * 2);
-    b.msg_t.alloc(c.pool, slots * K);
+    b.msg_t.alloc(c.pool, (slots + kNodeMaxDeg) * K);
     size_t acc_n = 0;
     for (int l = b.min_level; l < b.L; ++l)
         acc_n = std::max(acc_n, (size_t)b.lv[l].V * acc_replicas(b.lv[l].V) * K);

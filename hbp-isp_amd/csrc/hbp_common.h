@@ -155,6 +155,28 @@ __host__ __device__ inline float from_fixed(int64_t q, bool nan, int s) {
 
 // BSD LCG: s' = (1103515245 s + 12345) mod 2^31 in uint32 arithmetic.
 __host__ __device__ inline uint32_t lcg_next(uint32_t s) { return (1103515245u * s + 12345u) % 2147483648u; }
+// n steps of lcg_next are one affine map mod 2^31: s -> (a s + c) mod 2^31 with a = A^n and
+// c = C (1 + A + ... + A^(n-1)), both taken mod 2^32 (2^31 divides 2^32, so reducing once at the
+// end gives the stepwise value)
+struct LcgJump {
+    uint32_t a, c;
+};
+__host__ __device__ constexpr LcgJump lcg_jump_coef(int n) {
+    uint32_t a = 1u, c = 0u;
+    for (int i = 0; i < n; ++i) {
+        c = 1103515245u * c + 12345u;
+        a = 1103515245u * a;
+    }
+    return LcgJump{a, c};
+}
+// STEP * j steps of lcg_next from s, j in [0, 3] (per lane), without a loop
+template <int STEP>
+__host__ __device__ inline uint32_t lcg_skip(uint32_t s, int j) {
+    constexpr LcgJump j1 = lcg_jump_coef(STEP), j2 = lcg_jump_coef(2 * STEP), j3 = lcg_jump_coef(3 * STEP);
+    const uint32_t a = j == 1 ? j1.a : (j == 2 ? j2.a : j3.a);
+    const uint32_t c = j == 1 ? j1.c : (j == 2 ? j2.c : j3.c);
+    return j == 0 ? s : (a * s + c) % 2147483648u;
+}
 __host__ __device__ inline float lcg_uniform(uint32_t &s, float f) {
     s = lcg_next(s);
     return (float)s / 2147483648.0f * f;
