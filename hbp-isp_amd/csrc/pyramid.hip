@@ -183,7 +183,7 @@ __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int
 struct CRTBuf {
     int *nnw, *A, *B, *M, *C, *mask, *maskc, *indeg, *flags, *status;
 };
-constexpr int kCrtSingleMax = 40000;
+constexpr int kCrtSingleMax = 16000;
 constexpr int kCrtRounds = 5, kCrtJumps = 8, kCrtMaxJumps = 8;
 struct CRTFlags {               // zeroed before a scheduled run
     int more[kCrtRounds + 1];   // round r continues (some node has indegree > 1)
