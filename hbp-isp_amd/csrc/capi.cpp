@@ -100,6 +100,7 @@ int hbp_ctx_destroy(hbp_ctx *ctx) {
     HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
     ctx->c.scratch.release();
     ctx->c.counters.release();
+    ctx->c.free_host_box();
     ctx->c.pool.release_all();
     HBP_CHECK(hipStreamDestroy(ctx->c.stream));
     delete ctx;

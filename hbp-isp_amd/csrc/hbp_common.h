@@ -337,6 +337,16 @@ struct Ctx {
     // server's admission context, whose work runs beside the BP rounds of the main stream:
     // a cooperative grid needs every one of its blocks resident at once)
     hipStream_t drain_before_coop = nullptr;
+    // pinned host words that kernels write counts into, read after one stream wait (no copies)
+    int *hbox = nullptr;
+    int *host_box() {
+        if (!hbox) HBP_CHECK(hipHostMalloc((void **)&hbox, 64 * sizeof(int), hipHostMallocDefault));
+        return hbox;
+    }
+    void free_host_box() {
+        if (hbox) (void)hipHostFree(hbox);
+        hbox = nullptr;
+    }
     // scratch reused by every call on this context
     DevBuf<uint8_t> scratch;
     DevBuf<int> counters;  // small device counters (sizes, flags)

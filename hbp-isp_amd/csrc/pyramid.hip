@@ -437,6 +437,13 @@ __global__ void crt_final_kernel(int V, const CRTBuf s, const CRTFlags *f, int *
 }
 
 // CRTrees::compact_clus_label (CRTrees.cu:387-402): rank of each label among the labels used.
+// Labels from CRTrees name each component by its largest node, so label j is used exactly when
+// label[j] == j: the flags need neither a cleared array nor a scatter.  pred[V] = 0.
+__global__ void root_flags_kernel(const int *__restrict__ label, int V, int *__restrict__ pred) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i <= V; i += (long)gridDim.x * blockDim.x)
+        pred[i] = (i < V && label[i] == (int)i) ? 1 : 0;
+}
+// any labels in [0, V): pred (cleared, V + 1 entries) marks those used
 __global__ void mark_labels_kernel(const int *__restrict__ label, int V, int *__restrict__ pred) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (long)gridDim.x * blockDim.x)
         pred[label[i]] = 1;
@@ -462,10 +469,15 @@ __global__ void map_edges_kernel(const int2 *__restrict__ bd, const float *__res
     }
 }
 // last element of a run of equal keys with s' != t' (is_diff_bd, segment.cu:320-334)
+// (flag[E] = 0, so the exclusive scan's element E is the new edge count)
 __global__ void run_flags_kernel(const unsigned long long *__restrict__ keys, long E, int bits,
                                  int *__restrict__ flag) {
     unsigned long long tmask = (1ull << bits) - 1ull;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e <= E; e += (long)gridDim.x * blockDim.x) {
+        if (e == E) {
+            flag[e] = 0;
+            continue;
+        }
         unsigned long long k = keys[e];
         bool self = (k >> bits) == (k & tmask);
         flag[e] = (!self && (e == E - 1 || keys[e + 1] != k)) ? 1 : 0;
@@ -502,13 +514,44 @@ __global__ void fill_int_kernel(int *__restrict__ p, long n, int v) {
 __global__ void iota_kernel(int *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = (int)i;
 }
+// the sort's input for grouping children by parent: (parent, child) = (clus[i], i)
+__global__ void key_iota_kernel(const int *__restrict__ clus, long n, unsigned int *__restrict__ key,
+                                int *__restrict__ val) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        key[i] = (unsigned int)clus[i];
+        val[i] = (int)i;
+    }
+}
+// offsets of each key's run in a sorted array whose keys cover [0, nk) with every key present
+// (compacted labels): the run starting at j writes off[key[j]] = j; off[nk] = n
+__global__ void run_starts_kernel(const unsigned int *__restrict__ key, long n, int nk, int *__restrict__ off) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        if (j == 0 || key[j] != key[j - 1]) off[key[j]] = (int)j;
+        if (j == 0) off[nk] = (int)n;
+    }
+}
+// CSR offsets by source of the edges [0, *en) sorted by (s, t) over nodes [0, nv], where a node
+// may have no edge; also stores the edge count in *en_out
+__global__ void edge_csr_kernel(const int2 *__restrict__ bd, const int *__restrict__ en, long emax, int nv,
+                                int *__restrict__ off, int *__restrict__ en_out) {
+    const long n = *en < emax ? *en : emax;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (n > 0 ? n : 1); e += (long)gridDim.x * blockDim.x) {
+        if (n == 0) {
+            for (int v = 0; v <= nv; ++v) off[v] = 0;
+            *en_out = 0;
+            continue;
+        }
+        const int s = bd[e].x;
+        const int sp = e > 0 ? bd[e - 1].x : -1;
+        for (int v = sp + 1; v <= s; ++v) off[v] = (int)e;
+        if (e == n - 1)
+            for (int v = s + 1; v <= nv; ++v) off[v] = (int)n;
+        if (e == 0) *en_out = (int)n;
+    }
+}
 __global__ void count_keys_kernel(const int *__restrict__ key, long n, int *__restrict__ cnt) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         atomicAdd(&cnt[key[i]], 1);
-}
-__global__ void count_src_kernel(const int2 *__restrict__ bd, long E, int *__restrict__ cnt) {
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[bd[e].x], 1);
 }
 
 // reduce_data_mean (segment.cu:1134-1153) in the pinned order (children ascending, left fold
@@ -773,50 +816,76 @@ static void crt_require(int status) {
     HBP_REQUIRE(status == 0, "CRTrees iteration did not converge within its bounds");
 }
 
+// Device words to the host with one wait and no copies: a one-lane kernel queued after the work
+// that produces them stores them into the context's pinned host words.
+struct WordPtrs {
+    const int *p[8];
+};
+__global__ void publish_kernel(WordPtrs w, int n, int *__restrict__ out) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (threadIdx.x == 0 && i < n) out[i] = *w.p[i];
+}
+static void read_words(Ctx &c, const WordPtrs &w, int n, int *out) {
+    int *hb = c.host_box();
+    hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, c.stream, w, n, hb);
+    HBP_CHECK(hipGetLastError());
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < n; ++i) out[i] = ((volatile int *)hb)[i];
+}
+
 void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label) {
     CRTRun r;
     crt_setup(c, r, V);
     crt_enqueue(c, r, d_nn, d_label);
     int w[2] = {0, 0};
-    HBP_CHECK(hipMemcpyAsync(&w[0], r.s.status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipMemcpyAsync(&w[1], &r.f->overflow, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipStreamSynchronize(c.stream));
+    read_words(c, WordPtrs{{r.s.status, &r.f->overflow}}, 2, w);
     if (w[1]) {
         crt_cooperative(c, r, d_nn, d_label);
-        HBP_CHECK(hipMemcpyAsync(&w[0], r.s.status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        HBP_CHECK(hipStreamSynchronize(c.stream));
+        read_words(c, WordPtrs{{r.s.status}}, 1, w);
     }
     crt_require(w[0]);
 }
 
-// compaction; also copies n_extra device words to the host in the same wait
+// compaction; also reads n_extra (<= 7) device words in the same wait.  roots: the labels come
+// from CRTrees (a used label j has label[j] == j)
 static int compact_labels_and_read(Ctx &c, int *d_label, int V, const int *const *extra, int *extra_host,
-                                   int n_extra) {
+                                   int n_extra, bool roots) {
     DevBuf<int> pred(c.pool, (size_t)V + 1), pos(c.pool, (size_t)V + 1);
-    HBP_CHECK(hipMemsetAsync(pred.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
-    hipLaunchKernelGGL(mark_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pred.p);
+    if (roots) {
+        hipLaunchKernelGGL(root_flags_kernel, dim3(grid_for(c, (long)V + 1)), dim3(256), 0, c.stream, d_label, V,
+                           pred.p);
+    } else {
+        HBP_CHECK(hipMemsetAsync(pred.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
+        hipLaunchKernelGGL(mark_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pred.p);
+    }
     scan_exclusive_int(c, pred.p, pos.p, V + 1, nullptr);   // pred[V] = 0, so pos[V] = count
     hipLaunchKernelGGL(remap_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pos.p);
-    int Vn = 0;
-    HBP_CHECK(hipMemcpyAsync(&Vn, pos.p + V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    for (int i = 0; i < n_extra; ++i)
-        HBP_CHECK(hipMemcpyAsync(&extra_host[i], extra[i], sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipStreamSynchronize(c.stream));
-    return Vn;
+    WordPtrs w{};
+    w.p[0] = pos.p + V;
+    for (int i = 0; i < n_extra; ++i) w.p[1 + i] = extra[i];
+    int got[8] = {0};
+    read_words(c, w, 1 + n_extra, got);
+    for (int i = 0; i < n_extra; ++i) extra_host[i] = got[1 + i];
+    return got[0];
 }
-int compact_labels(Ctx &c, int *d_label, int V) { return compact_labels_and_read(c, d_label, V, nullptr, nullptr, 0); }
+int compact_labels(Ctx &c, int *d_label, int V) {
+    return compact_labels_and_read(c, d_label, V, nullptr, nullptr, 0, false);
+}
 
-// labels + compaction of one pyramid level with one host wait (two after an overflow)
-static int crtrees_level(Ctx &c, const int *d_nn, int V, int *d_label) {
+// labels + compaction of one pyramid level with one host wait (two after an overflow); en, when
+// not null, is a device word (the previous edge reduction's edge count) read in the same wait
+static int crtrees_level(Ctx &c, const int *d_nn, int V, int *d_label, const int *en, int *en_host) {
     CRTRun r;
     crt_setup(c, r, V);
     crt_enqueue(c, r, d_nn, d_label);
-    const int *extra[2] = {r.s.status, &r.f->overflow};
-    int w[2] = {0, 0};
-    int Vn = compact_labels_and_read(c, d_label, V, extra, w, 2);
+    const int *extra[3] = {r.s.status, &r.f->overflow, en};
+    int w[3] = {0, 0, 0};
+    int Vn = compact_labels_and_read(c, d_label, V, extra, w, en ? 3 : 2, true);
+    if (en) *en_host = w[2];
     if (w[1]) {
         crt_cooperative(c, r, d_nn, d_label);
-        Vn = compact_labels_and_read(c, d_label, V, extra, w, 1);
+        Vn = compact_labels_and_read(c, d_label, V, extra, w, 1, true);
     }
     crt_require(w[0]);
     return Vn;
@@ -913,9 +982,15 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
     int V = (int)N;
     long E = num_nb * N;
     long vo = 0, eo = 0;
+    // en: the edge count of the last edge reduction, read in the next level's wait (E bounds it,
+    // so the launches after the reduction are sized by E and need no wait of their own)
+    DevBuf<int> en(c.pool, 1);
+    bool en_pending = false;
     while (V > 1) {
         int *clus = p.clus.p + vo;
-        int Vn = crtrees_level(c, nn.p, V, clus);
+        int En_prev = 0;
+        int Vn = crtrees_level(c, nn.p, V, clus, en_pending ? en.p : nullptr, &En_prev);
+        if (en_pending) E = En_prev;
         if (Vn < target_clus) {
             target_clus_relabel(c, clus, V, Vn, dist_min.p, target_clus);
             Vn = target_clus;
@@ -944,20 +1019,18 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
             std::swap(keys, keys2);
             std::swap(dbits, dbits2);
         }
-        HBP_CHECK(hipMemsetAsync(flag.p + E, 0, sizeof(int), c.stream));
-        hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p, E, bits, flag.p);
-        scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);
-        int En = 0;
-        HBP_CHECK(hipMemcpyAsync(&En, pos.p + E, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        HBP_CHECK(hipStreamSynchronize(c.stream));
-        HBP_REQUIRE(eo2 + En <= ecap, "pyramid edge capacity exceeded");
-        HBP_CHECK(hipMemsetAsync(p.bdl.p + eo2, 0, sizeof(int) * (size_t)En, c.stream));
+        hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E + 1)), dim3(256), 0, c.stream, keys.p, E, bits,
+                           flag.p);
+        scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);   // pos[E]: the new level's edge count
+        // the new level's boundary-length sums from 0, Min/Max's reduced distances from
+        // reduce_dist's start values (segment.cu:1070-1071: FLT_MAX for Min, 0 for Max); both
+        // over E entries, which bound the new level's edges
+        HBP_CHECK(hipMemsetAsync(p.bdl.p + eo2, 0, sizeof(int) * (size_t)E, c.stream));
         DevBuf<float> edist2;
         if (!use_mean) {
-            // reduce_dist's start values (segment.cu:1070-1071): FLT_MAX for Min, 0 for Max
-            edist2.alloc(c.pool, (size_t)En + 1);
-            hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c, En)), dim3(256), 0, c.stream, (int *)edist2.p,
-                               (long)En, link == kMinLink ? 0x7f7fffff : 0);
+            edist2.alloc(c.pool, (size_t)E + 1);
+            hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, (int *)edist2.p, E,
+                               link == kMinLink ? 0x7f7fffff : 0);
         }
         hipLaunchKernelGGL(scatter_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p,
                            p.bdl.p + eo, dbits.p, link == kMinLink ? 1 : 0, flag.p, pos.p, E, bits, p.bd.p + eo2,
@@ -967,36 +1040,33 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
         dbits.release();
         dbits2.release();
         flag.release();
-        pos.release();
         if (!use_mean) std::swap(edist, edist2);
         edist2.release();
-        // --- children grouped by parent, ascending child index (stable sort by parent)
+        // --- children grouped by parent, ascending child index (stable sort by parent); every
+        // parent has a child (the labels are compacted), so the runs give the offsets
         DevBuf<unsigned int> ck(c.pool, V), ck2(c.pool, V);
-        DevBuf<int> cv(c.pool, V), cv2(c.pool, V), cnt(c.pool, (size_t)Vn + 1), coff(c.pool, (size_t)Vn + 1);
-        HBP_CHECK(hipMemcpyAsync(ck.p, clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
-        hipLaunchKernelGGL(iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, cv.p, (long)V);
+        DevBuf<int> cv(c.pool, V), cv2(c.pool, V), coff(c.pool, (size_t)Vn + 1);
+        hipLaunchKernelGGL(key_iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, clus, (long)V, ck.p, cv.p);
         if (sort_pairs_u32(c, ck.p, ck2.p, cv.p, cv2.p, V, bits)) {
             std::swap(ck, ck2);
             std::swap(cv, cv2);
         }
-        HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
-        hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, clus, (long)V, cnt.p);
-        scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
+        hipLaunchKernelGGL(run_starts_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, ck.p, (long)V, Vn,
+                           coff.p);
         // --- sums, means, dist_max
         DevBuf<float> mu(c.pool, 5L * Vn), dist_max(c.pool, Vn);
         hipLaunchKernelGGL(parent_sums_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, p.mean.p + 6 * vo, V,
                            cv.p, coff.p, Vn, dist_min.p, p.mean.p + 6 * vo2, mu.p, dist_max.p);
-        // --- CSR of the new edges, then the next level's 1-NN
-        HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)Vn + 1), c.stream));
-        hipLaunchKernelGGL(count_src_kernel, dim3(grid_for(c, En)), dim3(256), 0, c.stream, p.bd.p + eo2, (long)En,
-                           cnt.p);
-        scan_exclusive_int(c, cnt.p, coff.p, Vn + 1, nullptr);
+        // --- CSR of the new edges (sorted by source), then the next level's 1-NN
+        hipLaunchKernelGGL(edge_csr_kernel, dim3(grid_for(c, E > 0 ? E : 1)), dim3(256), 0, c.stream, p.bd.p + eo2,
+                           pos.p + E, E, Vn, coff.p, en.p);
+        pos.release();
         hipLaunchKernelGGL(level_nn_kernel, dim3(grid_for(c, Vn)), dim3(256), 0, c.stream, mu.p,
                            p.mean.p + 6 * vo2 + 5L * Vn, link, Vn, p.bd.p + eo2, coff.p, dist_max.p, edist.p, nn.p,
                            dist_min.p);
         HBP_CHECK(hipGetLastError());
+        en_pending = true;
         V = Vn;
-        E = En;
         vo = vo2;
         eo = eo2;
     }
