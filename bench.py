@@ -21,10 +21,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hbp-isp_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# timed region: one launch in 7 of the dominant kernel carries start/stop events stamped by its
-# dispatch (kernel execution time, as a kernel trace measures it); 7 is coprime with the sweep
-# kinds' period (4 or 5 sweeps per BP iteration), so the sample covers every kind evenly
-EVENT_EVERY = 7
+# timed region: one launch in 29 of the dominant kernel carries start/stop events stamped by its
+# dispatch (kernel execution time, as a kernel trace measures it); 29 is coprime with the sweep
+# kinds' period (4 or 5 sweeps per BP iteration), so the sample covers every kind evenly, and
+# with 1754 launches per pair ten steps still time about 600 launches while the events' own
+# cost (about 8 us per sampled launch) stays near 1 % of the step
+EVENT_EVERY = 29
 METRIC = "stereo pairs/s + Mpix·disp/s; p50/p99 latency, batched/served vs plain"
 WORKLOADS = {
     "large": dict(width=1472, height=984, ndisp=256, n_planes=(20, 40), bp_levels=4,
