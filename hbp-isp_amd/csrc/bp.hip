@@ -66,9 +66,6 @@ __global__ void iota_u32_kernel(unsigned int *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         p[i] = (unsigned)i;
 }
-__global__ void iota_i32_kernel(int *__restrict__ p, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = (int)i;
-}
 __global__ void compose_kernel(unsigned int *__restrict__ im_clus, const int *__restrict__ clus, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         im_clus[i] = (unsigned)clus[im_clus[i]];
@@ -84,9 +81,29 @@ __global__ void lower_bound_kernel(const unsigned int *__restrict__ keys, long n
             for (long k = k1 + 1; k <= nkeys; ++k) off[k] = (int)n;
     }
 }
-__global__ void count_src_kernel2(const int2 *__restrict__ bd, long E, int *__restrict__ cnt) {
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[bd[e].x], 1);
+// the same over the sources of edges sorted by (s, t) (every pyramid level's edge list is);
+// no edges (a top level of one cluster): every offset is 0
+__global__ void lower_bound_src_kernel(const int2 *__restrict__ bd, long n, int nkeys, int *__restrict__ off) {
+    if (n == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (int k = 0; k <= nkeys; ++k) off[k] = 0;
+        return;
+    }
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        long k1 = bd[j].x;
+        long k0 = j == 0 ? 0 : (long)bd[j - 1].x + 1;
+        for (long k = k0; k <= k1 && k <= nkeys; ++k) off[k] = (int)j;
+        if (j == n - 1)
+            for (long k = k1 + 1; k <= nkeys; ++k) off[k] = (int)n;
+    }
+}
+// a sort's input: keys copied, values 0 .. n-1
+__global__ void copy_iota_kernel(const unsigned int *__restrict__ src, long n, unsigned int *__restrict__ keys,
+                                 int *__restrict__ ids) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        keys[i] = src[i];
+        ids[i] = (int)i;
+    }
 }
 __global__ void deg_max_kernel(const int *__restrict__ off, int V, int *__restrict__ dmax) {
     int m = 0;
@@ -1233,9 +1250,15 @@ __global__ __launch_bounds__(256) void spatial_batch_kernel(const SweepArgs *__r
     if (id < a.V) spatial_node(a, id);
 }
 
-__global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+// a level's particle state from zero: planes, costs, beliefs [V][12], sample points [V]
+__global__ void zero_state_kernel(float4 *__restrict__ plane, float *__restrict__ cost, float *__restrict__ belief,
+                                  uint4 *__restrict__ point, long nk, long v) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
+        plane[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        cost[i] = 0.f;
+        belief[i] = 0.f;
+        if (i < v) point[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1464,8 +1487,8 @@ void bp_setup(Ctx &c, BP &b) {
             if (E > maxE) maxE = E;
             const float *mean = p.mean.p + 6 * p.voff[l];
             // pixels by (cluster, pixel id)
-            HBP_CHECK(hipMemcpyAsync(keys.p, im_clus.p, sizeof(unsigned) * N, hipMemcpyDeviceToDevice, c.stream));
-            hipLaunchKernelGGL(iota_i32_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, ids.p, N);
+            hipLaunchKernelGGL(copy_iota_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, im_clus.p, N, keys.p,
+                               ids.p);
             if (sort_pairs_u32(c, keys.p, keys2.p, ids.p, ids2.p, N, bits_for(V))) {
                 std::swap(keys, keys2);
                 std::swap(ids, ids2);
@@ -1492,11 +1515,9 @@ void bp_setup(Ctx &c, BP &b) {
                                    B.sego.p, V, B.seg.p);
             }
             // CSR by source, reverse edges, edge weights, mean positions
-            DevBuf<int> cnt(c.pool, (size_t)V + 1);
             B.off.alloc(c.pool, (size_t)V + 1);
-            HBP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)V + 1), c.stream));
-            hipLaunchKernelGGL(count_src_kernel2, dim3(grid_for(c, E)), dim3(256), 0, c.stream, B.bd, E, cnt.p);
-            scan_exclusive_int(c, cnt.p, B.off.p, V + 1, nullptr);
+            hipLaunchKernelGGL(lower_bound_src_kernel, dim3(grid_for(c, E > 0 ? E : 1)), dim3(256), 0, c.stream, B.bd,
+                               E, V, B.off.p);
             B.rev.alloc(c.pool, E);
             hipLaunchKernelGGL(rev_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, B.bd, B.off.p, E, B.rev.p);
             hipLaunchKernelGGL(deg_max_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.off.p, V, dmax.p + l);
@@ -1515,8 +1536,8 @@ void bp_setup(Ctx &c, BP &b) {
                 DevBuf<int> cv2(c.pool, V);
                 B.child.alloc(c.pool, V);
                 B.child_off.alloc(c.pool, (size_t)B.Vh + 1);
-                HBP_CHECK(hipMemcpyAsync(ck.p, B.clus, sizeof(int) * (size_t)V, hipMemcpyDeviceToDevice, c.stream));
-                hipLaunchKernelGGL(iota_i32_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.child.p, (long)V);
+                hipLaunchKernelGGL(copy_iota_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream,
+                                   (const unsigned int *)B.clus, (long)V, ck.p, B.child.p);
                 if (sort_pairs_u32(c, ck.p, ck2.p, B.child.p, cv2.p, V, bits_for(B.Vh))) {
                     std::swap(ck, ck2);
                     std::swap(B.child, cv2);
@@ -1529,11 +1550,8 @@ void bp_setup(Ctx &c, BP &b) {
             B.belief.alloc(c.pool, (size_t)V * K);
             B.point.alloc(c.pool, V);
             // the top level's planes are never set by the method; they read as zero
-            hipLaunchKernelGGL(zero_f4_kernel, dim3(grid_for(c, (long)V * K)), dim3(256), 0, c.stream, B.plane.p,
-                               (long)V * K);
-            HBP_CHECK(hipMemsetAsync(B.cost.p, 0, sizeof(float) * (size_t)V * K, c.stream));
-            HBP_CHECK(hipMemsetAsync(B.belief.p, 0, sizeof(float) * (size_t)V * K, c.stream));
-            HBP_CHECK(hipMemsetAsync(B.point.p, 0, sizeof(uint4) * (size_t)V, c.stream));
+            hipLaunchKernelGGL(zero_state_kernel, dim3(grid_for(c, (long)V * K)), dim3(256), 0, c.stream, B.plane.p,
+                               B.cost.p, B.belief.p, B.point.p, (long)V * K, (long)V);
         }
         if (l < b.L - 1)
             hipLaunchKernelGGL(compose_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, im_clus.p, B.clus, N);
