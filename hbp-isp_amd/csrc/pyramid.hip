@@ -469,15 +469,20 @@ __global__ void map_edges_kernel(const int2 *__restrict__ bd, const float *__res
     }
 }
 // last element of a run of equal keys with s' != t' (is_diff_bd, segment.cu:320-334)
-// (flag[E] = 0, so the exclusive scan's element E is the new edge count)
+// (flag[E] = 0, so the exclusive scan's element E is the new edge count).  Also starts the new
+// level's E-entry bound: boundary-length sums at 0 and, with Min/Max (dist_out not null), the
+// reduced distances at reduce_dist's start value (segment.cu:1070-1071)
 __global__ void run_flags_kernel(const unsigned long long *__restrict__ keys, long E, int bits,
-                                 int *__restrict__ flag) {
+                                 int *__restrict__ flag, int *__restrict__ bdl_out, unsigned int *__restrict__ dist_out,
+                                 unsigned int dist_start) {
     unsigned long long tmask = (1ull << bits) - 1ull;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e <= E; e += (long)gridDim.x * blockDim.x) {
         if (e == E) {
             flag[e] = 0;
             continue;
         }
+        bdl_out[e] = 0;
+        if (dist_out) dist_out[e] = dist_start;
         unsigned long long k = keys[e];
         bool self = (k >> bits) == (k & tmask);
         flag[e] = (!self && (e == E - 1 || keys[e + 1] != k)) ? 1 : 0;
@@ -1019,19 +1024,13 @@ void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int n
             std::swap(keys, keys2);
             std::swap(dbits, dbits2);
         }
-        hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E + 1)), dim3(256), 0, c.stream, keys.p, E, bits,
-                           flag.p);
-        scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);   // pos[E]: the new level's edge count
-        // the new level's boundary-length sums from 0, Min/Max's reduced distances from
-        // reduce_dist's start values (segment.cu:1070-1071: FLT_MAX for Min, 0 for Max); both
-        // over E entries, which bound the new level's edges
-        HBP_CHECK(hipMemsetAsync(p.bdl.p + eo2, 0, sizeof(int) * (size_t)E, c.stream));
+        // the new level's sums and (Min/Max) reduced distances start over E entries, which bound
+        // its edges: FLT_MAX for Min, 0 for Max
         DevBuf<float> edist2;
-        if (!use_mean) {
-            edist2.alloc(c.pool, (size_t)E + 1);
-            hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, (int *)edist2.p, E,
-                               link == kMinLink ? 0x7f7fffff : 0);
-        }
+        if (!use_mean) edist2.alloc(c.pool, (size_t)E + 1);
+        hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(c, E + 1)), dim3(256), 0, c.stream, keys.p, E, bits,
+                           flag.p, p.bdl.p + eo2, (unsigned int *)edist2.p, link == kMinLink ? 0x7f7fffffu : 0u);
+        scan_exclusive_int(c, flag.p, pos.p, E + 1, nullptr);   // pos[E]: the new level's edge count
         hipLaunchKernelGGL(scatter_edges_kernel, dim3(grid_for(c, E)), dim3(256), 0, c.stream, keys.p,
                            p.bdl.p + eo, dbits.p, link == kMinLink ? 1 : 0, flag.p, pos.p, E, bits, p.bd.p + eo2,
                            p.bdl.p + eo2, (unsigned int *)edist2.p);
