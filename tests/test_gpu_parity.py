@@ -175,6 +175,24 @@ def test_pyramid_blurred_levels_bit_exact(ctx, sigma, W, H):
         assert same_bits(a["mean"], b["mean"]), l
 
 
+@pytest.mark.parametrize("sigma,link", [(1.0, 0), (0.7, 1), (1.0, 3), (2.0, 3)])
+def test_pyramid_blurred_other_linkages_bit_exact(ctx, sigma, link):
+    # the blurred weight channel (the fold of the weights, not 1) enters Ward's weights and
+    # every level's sums
+    say(f"pyramid sigma={sigma} link={link} 157x93")
+    left, _, _ = synth.make_pair(22, 157, 93, 16, (3, 8))
+    po = O.Pyramid(left, link=link, sigma=sigma)
+    pg = ctx.pyramid(left, link=link, sigma=sigma)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+
+
 def test_bp_on_blurred_pyramid_bit_exact(ctx):
     say("bp on sigma=1.2")
     left, right = _bp_pair(17, 120, 90, 24)
