@@ -832,6 +832,7 @@ __global__ void publish_kernel(WordPtrs w, int n, int *__restrict__ out) {
         if (threadIdx.x == 0 && i < n) out[i] = *w.p[i];
 }
 static void read_words(Ctx &c, const WordPtrs &w, int n, int *out) {
+    HBP_REQUIRE(n >= 1 && n <= 8, "one to eight words per read");
     int *hb = c.host_box();
     hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, c.stream, w, n, hb);
     HBP_CHECK(hipGetLastError());
@@ -866,6 +867,7 @@ static int compact_labels_and_read(Ctx &c, int *d_label, int V, const int *const
     }
     scan_exclusive_int(c, pred.p, pos.p, V + 1, nullptr);   // pred[V] = 0, so pos[V] = count
     hipLaunchKernelGGL(remap_labels_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, d_label, V, pos.p);
+    HBP_REQUIRE(n_extra >= 0 && n_extra <= 7, "at most 7 extra words per read");
     WordPtrs w{};
     w.p[0] = pos.p + V;
     for (int i = 0; i < n_extra; ++i) w.p[1 + i] = extra[i];
