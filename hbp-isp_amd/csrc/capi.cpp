@@ -1,4 +1,5 @@
 // extern "C" entry points of libhbp_hip.so (declared in include/hbp_hip.h).
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -12,7 +13,21 @@ namespace hbp {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+static std::atomic<long long> g_fault[kFaultKinds];
+void fault_arm(int kind, long long after) {
+    HBP_REQUIRE(kind >= 0 && kind < kFaultKinds, "unknown fault kind");
+    g_fault[kind].store(after > 0 ? after : 0);
+}
+void fault_point(int kind, const char *what) {
+    long long v = g_fault[kind].load(std::memory_order_relaxed);
+    while (v > 0 && !g_fault[kind].compare_exchange_weak(v, v - 1)) {
+    }
+    if (v == 1) throw HipError{std::string("injected failure of a ") + what};
+}
+
 void *Pool::get(size_t bytes) {
+    fault_point(kFaultAlloc, "device allocation");
+    std::lock_guard<std::mutex> lk(mu_);
     size_t n = (bytes + 255) & ~(size_t)255;
     size_t best = (size_t)-1, bi = 0;
     for (size_t i = 0; i < free_.size(); ++i)
@@ -45,6 +60,7 @@ void *Pool::get(size_t bytes) {
 }
 
 void Pool::put(void *p) {
+    std::lock_guard<std::mutex> lk(mu_);
     for (size_t i = 0; i < used_.size(); ++i)
         if (used_[i].p == p) {
             in_use_ -= used_[i].n;
@@ -55,6 +71,7 @@ void Pool::put(void *p) {
 }
 
 void Pool::release_all() {
+    std::lock_guard<std::mutex> lk(mu_);
     for (auto &f : free_) (void)hipFree(f.p);
     for (auto &u : used_) (void)hipFree(u.p);
     free_.clear();
@@ -73,6 +90,15 @@ static void use_device(hbp_ctx *ctx) { HBP_CHECK(hipSetDevice(ctx->c.device)); }
 extern "C" {
 
 const char *hbp_last_error(void) { return hbp::g_last_error.c_str(); }
+
+int hbp_inject_fault(const char *what, long long after) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(what, "NULL argument");
+    if (std::strcmp(what, "alloc") == 0) fault_arm(kFaultAlloc, after);
+    else if (std::strcmp(what, "round") == 0) fault_arm(kFaultRound, after);
+    else throw HipError{"fault kinds are \"alloc\" and \"round\""};
+    HBP_API_END
+}
 
 int hbp_ctx_create(int device, hbp_ctx **out) {
     HBP_API_BEGIN
@@ -96,6 +122,9 @@ int hbp_ctx_create(int device, hbp_ctx **out) {
 int hbp_ctx_destroy(hbp_ctx *ctx) {
     HBP_API_BEGIN
     if (!ctx) return 0;
+    HBP_REQUIRE(ctx->live == 0, std::to_string(ctx->live) +
+                                    " volume, pyramid, BP or server handles of this context are still alive: "
+                                    "free them before the context");
     use_device(ctx);
     HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
     ctx->c.scratch.release();
@@ -216,7 +245,7 @@ int hbp_cost_volume_dev(hbp_ctx *ctx, const uint8_t *d_l, const uint8_t *d_r, in
         delete v;
         throw;
     }
-    *out = v;
+    *out = adopt_handle(ctx, v);
     HBP_API_END
 }
 
@@ -241,7 +270,7 @@ int hbp_cost_volume(hbp_ctx *ctx, const uint8_t *l, const uint8_t *r, int W, int
         delete v;
         throw;
     }
-    *out = v;
+    *out = adopt_handle(ctx, v);
     HBP_API_END
 }
 
@@ -265,7 +294,7 @@ int hbp_cost_volume_f32_dev(hbp_ctx *ctx, const float *d_vol, int W, int H, int 
         delete v;
         throw;
     }
-    *out = v;
+    *out = adopt_handle(ctx, v);
     HBP_API_END
 }
 
@@ -287,7 +316,7 @@ int hbp_cost_volume_f32(hbp_ctx *ctx, const float *vol, int W, int H, int D, con
         delete v;
         throw;
     }
-    *out = v;
+    *out = adopt_handle(ctx, v);
     HBP_API_END
 }
 
@@ -362,11 +391,7 @@ int hbp_volume_download(hbp_ctx *ctx, const hbp_volume *vol, uint8_t *vh, float 
 int hbp_volume_free(hbp_ctx *ctx, hbp_volume *vol) {
     HBP_API_BEGIN
     if (!vol) return 0;
-    if (ctx) {
-        use_device(ctx);
-        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
-    }
-    delete vol;
+    release_handle(ctx, vol);
     HBP_API_END
 }
 
@@ -392,7 +417,7 @@ int hbp_pyramid_build_target_dev(hbp_ctx *ctx, const uint8_t *d_bgr, int W, int 
         delete p;
         throw;
     }
-    *out = p;
+    *out = adopt_handle(ctx, p);
     HBP_API_END
 }
 
@@ -414,7 +439,7 @@ int hbp_pyramid_build_target(hbp_ctx *ctx, const uint8_t *bgr, int W, int H, int
         delete p;
         throw;
     }
-    *out = p;
+    *out = adopt_handle(ctx, p);
     HBP_API_END
 }
 
@@ -463,11 +488,7 @@ int hbp_pyramid_download_level(hbp_ctx *ctx, const hbp_pyramid *pyr, int level, 
 int hbp_pyramid_free(hbp_ctx *ctx, hbp_pyramid *pyr) {
     HBP_API_BEGIN
     if (!pyr) return 0;
-    if (ctx) {
-        use_device(ctx);
-        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
-    }
-    delete pyr;
+    release_handle(ctx, pyr);
     HBP_API_END
 }
 
@@ -509,7 +530,7 @@ int hbp_bp_create(hbp_ctx *ctx, const hbp_pyramid *pyr, const hbp_volume *vol, f
         delete b;
         throw;
     }
-    *out = b;
+    *out = adopt_handle(ctx, b);
     HBP_API_END
 }
 
@@ -576,11 +597,7 @@ int hbp_bp_get_state(hbp_ctx *ctx, hbp_bp *bp, float *planes, float *cost, float
 int hbp_bp_free(hbp_ctx *ctx, hbp_bp *bp) {
     HBP_API_BEGIN
     if (!bp) return 0;
-    if (ctx) {
-        use_device(ctx);
-        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
-    }
-    delete bp;
+    release_handle(ctx, bp);
     HBP_API_END
 }
 

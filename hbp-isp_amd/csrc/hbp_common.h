@@ -12,6 +12,7 @@
 #include <climits>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -203,6 +204,8 @@ __host__ __device__ inline int unpack_y(uint32_t p) { return (int)(p >> 16); }
 // ---------------------------------------------------------------------------------------
 // device memory: a per-context caching pool (blocks are reused by exact size)
 // ---------------------------------------------------------------------------------------
+// (thread-safe: a server's admission thread allocates from its pool while the serving thread
+// returns finished pairs' blocks to it)
 class Pool {
 public:
     void *get(size_t bytes);
@@ -217,9 +220,17 @@ private:
         void *p;
         size_t n;
     };
+    std::mutex mu_;
     std::vector<Blk> free_, used_;
     size_t in_use_ = 0, reserved_ = 0;
 };
+
+// Host-forced failures for tests of the error paths (hbp_inject_fault): the n-th next device
+// allocation from any pool, or the n-th next batched round, throws instead of running.  No
+// kernel is ever started in a failing state.
+enum FaultKind { kFaultAlloc = 0, kFaultRound = 1, kFaultKinds = 2 };
+void fault_arm(int kind, long long after);
+void fault_point(int kind, const char *what);   // throws when the armed count reaches this call
 
 template <class T>
 struct DevBuf {

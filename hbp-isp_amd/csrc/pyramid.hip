@@ -430,10 +430,12 @@ __global__ void crt_max_kernel(int V, CRTBuf s, CRTFlags *f, int j) {
         if (j == kCrtMaxJumps - 1) atomicOr(&f->overflow, 1);
     }
 }
-// the last step changes no label, so M and C hold the same labels
+// the last step changes no label, so M and C hold the same labels.  After an overflow the
+// labels are the identity: in range for the compaction that is queued behind this kernel before
+// the host sees the overflow word (the cooperative rerun then overwrites them)
 __global__ void crt_final_kernel(int V, const CRTBuf s, const CRTFlags *f, int *__restrict__ label) {
-    if (flag_read(&f->overflow, 0)) return;
-    CRT_NODES(i) label[i] = s.M[i];
+    const bool ov = flag_read(&f->overflow, 0) != 0;
+    CRT_NODES(i) label[i] = ov ? (int)i : s.M[i];
 }
 
 // CRTrees::compact_clus_label (CRTrees.cu:387-402): rank of each label among the labels used.

@@ -1,27 +1,33 @@
 // Request server: continuous batching of stereo pairs on one context.
 //
-// A request (two BGR images, their size and disparity range) waits in a FIFO until there is
-// a free batch slot and its estimated device memory fits the HBM budget.  Admission runs the
-// pair's cost volume, superpixel pyramid and BP set-up on the context's stream; the pair then
-// joins the rounds of the BatchRunner, which run one BP sweep of every admitted pair per round
-// in shared launches.  A pair whose schedule is done is mapped, its disparity and planes are
-// copied to host memory, and its device memory returns to the context's pool.  Pairs enter
-// and leave between rounds, so pairs of different sizes and arrival times overlap on the
-// device; each pair's result is the one its own hbp_bp_run gives (same kernels, same sums).
+// A request (two BGR images, their size and disparity range) waits in a FIFO until there is a
+// free batch slot and its device memory fits the HBM budget.  It then goes to the admission
+// thread, which builds the pair's cost volume, superpixel pyramid and BP set-up on a stream and
+// memory pool of its own (the admission context) and drops the u8 cost volume, which BP does not
+// read (it recomputes the costs it needs from the census words).  Admitted pairs join the rounds
+// of the BatchRunner on the serving thread, which run one BP sweep of every admitted pair per
+// round in shared launches.  When a pair's schedule is done its map and the copies of its
+// disparity and planes into pinned host memory are queued behind its last round, with an event;
+// the pair's device memory returns to the pool once that event has completed.  So neither
+// admission nor completion makes the serving thread wait for the device, pairs of different
+// sizes and arrival times overlap on the device, and each pair's result is the one its own
+// hbp_bp_run gives (same kernels, same sums).
 //
-// Admission runs on a second stream with its own memory pool (the admission context), so that
-// a new pair's cost volume and pyramid overlap the BP rounds already queued on the main stream;
-// admit() returns once that work is done, and the pair's buffers return to the admission pool
-// only after complete() has synchronised the main stream.  Rounds stage their items in pinned
-// memory and never wait for the device, so the host queues rounds ahead of it.
+// Failures are per request: an error while admitting, mapping or copying out one pair fails that
+// pair only; an error in a shared round fails the pairs of that round, after the device has
+// finished the work already queued.  A failed, cancelled or finished pair never leaves a null or
+// half-released entry behind.
 //
 // The reference has no serving layer (one pair per HBP_ISP::run_ms, main_stereo_hbp_isp.cu:
-// 33-76); this is new.  A server is driven by one host thread: hbp_submit queues, and every
-// hbp_server_step admits, runs one round and completes finished pairs.
+// 33-76); this is new.  hbp_submit, hbp_server_step, hbp_poll and hbp_cancel are called from
+// one host thread; the admission thread is internal.
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <thread>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "../../include/hbp_hip.h"
 #include "handles.h"
@@ -31,154 +37,406 @@ namespace hbp {
 struct Request {
     long long id = 0;
     int W = 0, H = 0, D = 0, min_level = 2;
+    int coarse_min_pix = 96, single_hb = 4, batch_hb = 2;   // the serving context's forms
     std::vector<uint8_t> left, right;
-    std::vector<float> disp, plane;
-    size_t est = 0;
+    size_t est = 0;           // device bytes held while on the device (without the u8 volume)
+    size_t est_admit = 0;     // peak during admission (with it)
+    bool counted = false;     // est is in the server's in_use
     int slot = -1;
-    int status = 0;         // 0 ok, < 0 failed (message in error)
+    int status = 0;           // 0 ok, < 0 failed (message in error)
+    bool cancelled = false;
     std::string error;
+    hipEvent_t done = nullptr;    // recorded behind the copies out (or behind the last round that read it)
+    float *out = nullptr;         // pinned: disp [N], then plane [3N]
+    size_t out_bytes = 0;
     Volume vol;
     Pyramid pyr;
     BP bp;
+    void release_device() {
+        bp = BP();
+        pyr = Pyramid();
+        vol = Volume();
+    }
+};
+
+// pinned host blocks for the results, reused by size
+struct PinnedCache {
+    struct Blk {
+        void *p;
+        size_t n;
+    };
+    std::vector<Blk> free_;
+    void *get(size_t n) {
+        for (size_t i = 0; i < free_.size(); ++i)
+            if (free_[i].n >= n && free_[i].n <= 2 * n) {
+                void *p = free_[i].p;
+                free_.erase(free_.begin() + (long)i);
+                return p;
+            }
+        void *p = nullptr;
+        HBP_CHECK(hipHostMalloc(&p, n, hipHostMallocDefault));
+        return p;
+    }
+    void put(void *p, size_t n) {
+        if (p) free_.push_back({p, n});
+    }
+    ~PinnedCache() {
+        for (auto &b : free_) (void)hipHostFree(b.p);
+    }
 };
 
 struct Server {
     Ctx *c = nullptr;
-    Ctx admit_ctx;   // own stream and pool; see above
+    Ctx admit_ctx;   // own stream and pool, used only by the admission thread
     int max_active = 0;
     size_t budget = 0, in_use = 0;
     float alpha = 1000.1f, tau_s = 0.6f;
     BatchRunner runner;
-    std::deque<std::unique_ptr<Request>> pending;
-    std::vector<std::unique_ptr<Request>> active;
+    PinnedCache pinned;
+    std::deque<std::unique_ptr<Request>> pending;                 // not yet handed to admission
+    std::vector<std::unique_ptr<Request>> active;                 // in the rounds
+    std::vector<std::unique_ptr<Request>> draining;               // copies out or release queued
     std::unordered_map<long long, std::unique_ptr<Request>> finished;
+    std::unordered_set<long long> in_admission, cancel_ids;   // serving thread only
     long long next_id = 1;
-    ~Server() {
-        // the requests' buffers back to their pools, then the admission context's memory
-        finished.clear();
-        active.clear();
-        pending.clear();
-        admit_ctx.scratch.release();
-        admit_ctx.counters.release();
-        admit_ctx.free_host_box();
-        admit_ctx.pool.release_all();
-        if (admit_ctx.stream) (void)hipStreamDestroy(admit_ctx.stream);
-    }
+    // the admission thread's queues (guarded by mu)
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::unique_ptr<Request>> to_admit, admitted;
+    int admitting = 0;    // requests handed over and not yet back
+    bool stop = false;
+    std::thread worker;
+    ~Server();
 };
 
-// device bytes of one pair: u8 volume, census/WTA, the pyramid's packed arrays (2N nodes and
-// 2*4*2N edges of capacity, segment.hpp:276-285), the BP levels (sorted pixels, segments,
-// per-level structures and messages) and the padded right census, rounded up
+// device bytes of one pair while it is on the device: census/WTA, the pyramid's packed arrays
+// (2N nodes and 2*4*2N edges of capacity, segment.hpp:276-285), the BP levels (sorted pixels,
+// segments, per-level structures and messages) and the padded right census, rounded up; plus
+// the u8 volume during admission
 static size_t estimate_bytes(int W, int H, int D) {
     const size_t N = (size_t)W * H;
-    return (size_t)D * H * round_up(W, 64) + 16 * N + 152 * N + 160 * N + 4 * (size_t)H * (W + D);
+    return 16 * N + 152 * N + 160 * N + 4 * (size_t)H * (W + D);
 }
+static size_t volume_bytes(int W, int H, int D) { return (size_t)D * H * round_up(W, 64); }
 
-static void admit(Server &s, Request &r) {
+// cost volume, pyramid and BP set-up of one pair on the admission context (admission thread)
+static void admit_work(Server &s, Request &r) {
     Ctx &c = s.admit_ctx;
-    c.coarse_min_pix = s.c->coarse_min_pix;   // the forms the main context is tuned to
-    c.single_hb = s.c->single_hb;
-    c.batch_hb = s.c->batch_hb;
+    c.coarse_min_pix = r.coarse_min_pix;
+    c.single_hb = r.single_hb;
+    c.batch_hb = r.batch_hb;
     const size_t n3 = (size_t)r.W * r.H * 3;
-    DevBuf<uint8_t> dl(c.pool, n3), dr(c.pool, n3);
-    HBP_CHECK(hipMemcpyAsync(dl.p, r.left.data(), n3, hipMemcpyHostToDevice, c.stream));
-    HBP_CHECK(hipMemcpyAsync(dr.p, r.right.data(), n3, hipMemcpyHostToDevice, c.stream));
-    volume_build(c, r.vol, dl.p, dr.p, r.W, r.H, r.D);
-    pyramid_build(c, r.pyr, dl.p, r.W, r.H, 4);
-    r.bp.pyr = &r.pyr;
-    r.bp.vol = &r.vol;
-    r.bp.lambda = s.alpha;
-    r.bp.tau_s = s.tau_s;
-    r.bp.use_cost_vol = 1;
-    const int L = r.pyr.levels();
-    HBP_REQUIRE(L >= 2, "image too small for a pyramid of two levels");
-    r.bp.min_level = std::max(1, std::min(r.min_level, L - 1));
-    bp_setup(c, r.bp);
-    HBP_CHECK(hipStreamSynchronize(c.stream));
-    r.slot = s.runner.add(*s.c, r.bp);
+    {
+        DevBuf<uint8_t> dl(c.pool, n3), dr(c.pool, n3);
+        HBP_CHECK(hipMemcpyAsync(dl.p, r.left.data(), n3, hipMemcpyHostToDevice, c.stream));
+        HBP_CHECK(hipMemcpyAsync(dr.p, r.right.data(), n3, hipMemcpyHostToDevice, c.stream));
+        volume_build(c, r.vol, dl.p, dr.p, r.W, r.H, r.D);
+        pyramid_build(c, r.pyr, dl.p, r.W, r.H, 4);
+        r.bp.pyr = &r.pyr;
+        r.bp.vol = &r.vol;
+        r.bp.lambda = s.alpha;
+        r.bp.tau_s = s.tau_s;
+        r.bp.use_cost_vol = 1;
+        const int L = r.pyr.levels();
+        HBP_REQUIRE(L >= 2, "image too small for a pyramid of two levels");
+        r.bp.min_level = std::max(1, std::min(r.min_level, L - 1));
+        bp_setup(c, r.bp);
+        HBP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    r.vol.vol.release();   // BP reads the census words and the WTA only
     std::vector<uint8_t>().swap(r.left);
     std::vector<uint8_t>().swap(r.right);
 }
 
-static void complete(Server &s, std::unique_ptr<Request> &rp) {
-    Ctx &c = *s.c;
-    Request &r = *rp;
-    bp_map(c, r.bp);
-    const size_t n = (size_t)r.W * r.H;
-    r.disp.resize(n);
-    r.plane.resize(3 * n);
-    HBP_CHECK(hipMemcpyAsync(r.disp.data(), r.bp.disp.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipMemcpyAsync(r.plane.data(), r.bp.plane3.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c.stream));
-    HBP_CHECK(hipStreamSynchronize(c.stream));
-    s.runner.remove(r.slot);
-    r.slot = -1;
-    // device memory back to the pool (stream-ordered: later work runs after this pair's)
-    r.bp = BP();
-    r.pyr = Pyramid();
-    r.vol = Volume();
-    s.in_use -= r.est;
-    s.finished[r.id] = std::move(rp);
+static void admission_loop(Server *sp) {
+    Server &s = *sp;
+    (void)hipSetDevice(s.admit_ctx.device);
+    for (;;) {
+        std::unique_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(s.mu);
+            s.cv_work.wait(lk, [&] { return s.stop || !s.to_admit.empty(); });
+            if (s.to_admit.empty()) return;   // stop, nothing left
+            r = std::move(s.to_admit.front());
+            s.to_admit.pop_front();
+        }
+        try {
+            admit_work(s, *r);
+        } catch (const HipError &e) {
+            (void)hipStreamSynchronize(s.admit_ctx.stream);
+            r->status = -1;
+            r->error = e.msg;
+        } catch (const std::bad_alloc &) {
+            (void)hipStreamSynchronize(s.admit_ctx.stream);
+            r->status = -1;
+            r->error = "host allocation failed";
+        }
+        {
+            std::lock_guard<std::mutex> lk(s.mu);
+            s.admitted.push_back(std::move(r));
+        }
+        s.cv_done.notify_all();
+    }
 }
 
-static void fail(Server &s, std::unique_ptr<Request> &rp, const std::string &msg) {
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+        for (auto &r : to_admit) {   // never started: nothing on the device
+            r->status = -1;
+            r->error = "server destroyed";
+        }
+        to_admit.clear();
+    }
+    cv_work.notify_all();
+    if (worker.joinable()) worker.join();
+    if (c && c->stream) (void)hipStreamSynchronize(c->stream);
+    if (admit_ctx.stream) (void)hipStreamSynchronize(admit_ctx.stream);
+    auto drop = [&](std::unique_ptr<Request> &r) {
+        if (!r) return;
+        if (r->done) (void)hipEventDestroy(r->done);
+        if (r->out) pinned.put(r->out, r->out_bytes);
+        r->release_device();
+    };
+    for (auto &r : admitted) drop(r);
+    for (auto &r : active) drop(r);
+    for (auto &r : draining) drop(r);
+    for (auto &kv : finished) drop(kv.second);
+    admitted.clear();
+    active.clear();
+    draining.clear();
+    finished.clear();
+    pending.clear();
+    admit_ctx.scratch.release();
+    admit_ctx.counters.release();
+    admit_ctx.free_host_box();
+    admit_ctx.pool.release_all();
+    if (admit_ctx.stream) (void)hipStreamDestroy(admit_ctx.stream);
+}
+
+// a request leaves for `finished`, failed; the device no longer reads its buffers (callers
+// make sure of that: the stream that read them was synchronised, or nothing ran)
+static void finish_failed(Server &s, std::unique_ptr<Request> rp, const std::string &msg) {
     Request &r = *rp;
     if (r.slot >= 0) s.runner.remove(r.slot);
     r.slot = -1;
     r.status = -1;
     r.error = msg;
-    r.bp = BP();
-    r.pyr = Pyramid();
-    r.vol = Volume();
+    r.release_device();
+    if (r.done) (void)hipEventDestroy(r.done);
+    r.done = nullptr;
+    if (r.out) s.pinned.put(r.out, r.out_bytes);
+    r.out = nullptr;
+    if (r.counted) s.in_use -= r.est;
+    r.counted = false;
     s.finished[r.id] = std::move(rp);
 }
 
-// admit what fits, one round, complete finished pairs
+// queue the map and the copies out behind the pair's last round (serving thread)
+static void queue_completion(Server &s, Request &r) {
+    Ctx &c = *s.c;
+    bp_map(c, r.bp);
+    const size_t n = (size_t)r.W * r.H;
+    r.out_bytes = sizeof(float) * 4 * n;
+    r.out = (float *)s.pinned.get(r.out_bytes);
+    HBP_CHECK(hipMemcpyAsync(r.out, r.bp.disp.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipMemcpyAsync(r.out + n, r.bp.plane3.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+    HBP_CHECK(hipEventRecord(r.done, c.stream));
+}
+
+// admit what fits, collect admitted pairs, one round, queue the completions, retire what the
+// device has finished
 static void step(Server &s, int *n_active, int *n_pending, int *n_done) {
     int done = 0;
-    while (!s.pending.empty() && (int)s.active.size() < s.max_active) {
-        std::unique_ptr<Request> &r = s.pending.front();
-        if (r->est > s.budget) {
+    // 1. retire: pairs whose copies (or last reads) the device has finished
+    {
+        std::vector<std::unique_ptr<Request>> keep;
+        for (auto &rp : s.draining) {
+            hipError_t q = hipEventQuery(rp->done);
+            if (q == hipErrorNotReady) {
+                keep.push_back(std::move(rp));
+                continue;
+            }
+            (void)hipEventDestroy(rp->done);
+            rp->done = nullptr;
+            if (q != hipSuccess) {
+                (void)hipStreamSynchronize(s.c->stream);
+                finish_failed(s, std::move(rp), std::string("copying the result out failed: ") + hipGetErrorString(q));
+                continue;
+            }
+            rp->release_device();
+            if (rp->counted) s.in_use -= rp->est;
+            rp->counted = false;
+            if (rp->cancelled) {
+                finish_failed(s, std::move(rp), "cancelled");
+                continue;
+            }
+            ++done;
+            s.finished[rp->id] = std::move(rp);
+        }
+        s.draining.swap(keep);
+    }
+    // 2. hand pending pairs to the admission thread while slots and budget allow
+    {
+        int in_flight;
+        {
+            std::lock_guard<std::mutex> lk(s.mu);
+            in_flight = s.admitting;
+        }
+        while (!s.pending.empty() && (int)s.active.size() + in_flight < s.max_active) {
+            std::unique_ptr<Request> &r = s.pending.front();
+            if (r->est_admit > s.budget) {
+                std::unique_ptr<Request> rp = std::move(r);
+                s.pending.pop_front();
+                finish_failed(s, std::move(rp), "pair does not fit the server's HBM budget");
+                continue;
+            }
+            // the volume is held only during admission, one admission at a time
+            if (s.in_use + r->est_admit > s.budget) break;
             std::unique_ptr<Request> rp = std::move(r);
             s.pending.pop_front();
-            fail(s, rp, "pair does not fit the server's HBM budget");
-            continue;
-        }
-        if (s.in_use + r->est > s.budget) break;
-        std::unique_ptr<Request> rp = std::move(r);
-        s.pending.pop_front();
-        s.in_use += rp->est;
-        try {
-            admit(s, *rp);
-            s.active.push_back(std::move(rp));
-        } catch (const HipError &e) {
-            s.in_use -= rp->est;
-            fail(s, rp, e.msg);
+            s.in_use += rp->est;
+            rp->counted = true;
+            s.in_admission.insert(rp->id);
+            rp->coarse_min_pix = s.c->coarse_min_pix;
+            rp->single_hb = s.c->single_hb;
+            rp->batch_hb = s.c->batch_hb;
+            {
+                std::lock_guard<std::mutex> lk(s.mu);
+                s.to_admit.push_back(std::move(rp));
+                ++s.admitting;
+                ++in_flight;
+            }
+            s.cv_work.notify_one();
         }
     }
+    // 3. admitted pairs join the rounds
+    {
+        std::deque<std::unique_ptr<Request>> got;
+        {
+            std::unique_lock<std::mutex> lk(s.mu);
+            // nothing else to do: wait a little for an admission instead of spinning
+            if (s.admitted.empty() && s.active.empty() && s.draining.empty() && s.admitting > 0)
+                s.cv_done.wait_for(lk, std::chrono::milliseconds(2), [&] { return !s.admitted.empty(); });
+            got.swap(s.admitted);
+            s.admitting -= (int)got.size();
+        }
+        for (auto &rp : got) {
+            s.in_admission.erase(rp->id);
+            if (rp->status != 0) {
+                std::string msg = rp->error;
+                finish_failed(s, std::move(rp), msg);
+                continue;
+            }
+            if (s.cancel_ids.erase(rp->id)) {
+                finish_failed(s, std::move(rp), "cancelled");
+                continue;
+            }
+            try {
+                rp->slot = s.runner.add(*s.c, rp->bp);
+                s.active.push_back(std::move(rp));
+            } catch (const HipError &e) {
+                (void)hipStreamSynchronize(s.c->stream);
+                finish_failed(s, std::move(rp), e.msg);
+            }
+        }
+    }
+    // 4. one round of every active pair
     if (!s.active.empty()) {
         std::vector<std::pair<BP *, int>> pairs;
         for (auto &r : s.active) pairs.push_back({&r->bp, r->slot});
-        s.runner.round(*s.c, pairs);
+        bool round_ok = true;
+        std::string msg;
+        try {
+            fault_point(kFaultRound, "batched round");
+            s.runner.round(*s.c, pairs);
+        } catch (const HipError &e) {
+            round_ok = false;
+            msg = e.msg;
+        }
         std::vector<std::unique_ptr<Request>> still;
-        for (auto &r : s.active) {
-            if (r->bp.next >= r->bp.sched.size()) {
-                complete(s, r);
-                ++done;
-            } else {
-                still.push_back(std::move(r));
+        if (!round_ok) {
+            // the pairs' states are undefined after a partial round: fail them all once the
+            // device has stopped reading their buffers
+            (void)hipStreamSynchronize(s.c->stream);
+            for (auto &rp : s.active) finish_failed(s, std::move(rp), "batched round failed: " + msg);
+        } else {
+            for (auto &rp : s.active) {
+                if (rp->bp.next < rp->bp.sched.size()) {
+                    still.push_back(std::move(rp));
+                    continue;
+                }
+                s.runner.remove(rp->slot);
+                rp->slot = -1;
+                try {
+                    queue_completion(s, *rp);
+                    s.draining.push_back(std::move(rp));
+                } catch (const HipError &e) {
+                    (void)hipStreamSynchronize(s.c->stream);
+                    finish_failed(s, std::move(rp), e.msg);
+                }
             }
         }
         s.active.swap(still);
     }
     if (n_active) *n_active = (int)s.active.size();
-    if (n_pending) *n_pending = (int)s.pending.size();
+    if (n_pending) *n_pending = (int)(s.pending.size() + s.in_admission.size());
     if (n_done) *n_done = done;
+}
+
+// hbp_cancel: 0 if the ticket was queued, running or finished (it then polls as "cancelled",
+// or is gone if it had finished), 1 if it is unknown
+static int cancel(Server &s, long long id) {
+    for (auto it = s.pending.begin(); it != s.pending.end(); ++it)
+        if ((*it)->id == id) {
+            std::unique_ptr<Request> rp = std::move(*it);
+            s.pending.erase(it);
+            finish_failed(s, std::move(rp), "cancelled");
+            return 0;
+        }
+    if (s.in_admission.count(id)) {   // leaves as cancelled when admission hands it back
+        s.cancel_ids.insert(id);
+        return 0;
+    }
+    for (auto it = s.active.begin(); it != s.active.end(); ++it)
+        if ((*it)->id == id) {
+            // leaves the rounds now; its buffers return once the queued rounds have run
+            std::unique_ptr<Request> rp = std::move(*it);
+            s.active.erase(it);
+            s.runner.remove(rp->slot);
+            rp->slot = -1;
+            rp->cancelled = true;
+            try {
+                HBP_CHECK(hipEventCreateWithFlags(&rp->done, hipEventDisableTiming));
+                HBP_CHECK(hipEventRecord(rp->done, s.c->stream));
+                s.draining.push_back(std::move(rp));
+            } catch (const HipError &) {
+                (void)hipStreamSynchronize(s.c->stream);
+                finish_failed(s, std::move(rp), "cancelled");
+            }
+            return 0;
+        }
+    for (auto &r : s.draining)
+        if (r->id == id) {
+            r->cancelled = true;
+            return 0;
+        }
+    auto f = s.finished.find(id);
+    if (f != s.finished.end()) {
+        if (f->second->out) s.pinned.put(f->second->out, f->second->out_bytes);
+        s.finished.erase(f);
+        return 0;
+    }
+    return 1;
 }
 
 }  // namespace hbp
 
 struct hbp_server {
     hbp::Server s;
+    hbp_ctx *owner = nullptr;
 };
 
 extern "C" {
@@ -205,10 +463,13 @@ int hbp_server_create(hbp_ctx *ctx, int max_active, long long hbm_budget_bytes, 
     try {
         HBP_CHECK(hipStreamCreateWithFlags(&ac.stream, hipStreamNonBlocking));
         srv->s.runner.init(ctx->c, max_active);
+        srv->s.worker = std::thread(admission_loop, &srv->s);
     } catch (...) {
         delete srv;
         throw;
     }
+    srv->owner = ctx;
+    ++ctx->live;
     *out = srv;
     HBP_API_END
 }
@@ -217,9 +478,9 @@ int hbp_server_destroy(hbp_server *srv) {
     HBP_API_BEGIN
     if (!srv) return 0;
     HBP_CHECK(hipSetDevice(srv->s.c->device));
-    HBP_CHECK(hipStreamSynchronize(srv->s.c->stream));
-    if (srv->s.admit_ctx.stream) HBP_CHECK(hipStreamSynchronize(srv->s.admit_ctx.stream));
-    delete srv;
+    hbp_ctx *o = srv->owner;
+    delete srv;   // stops the admission thread, waits for both streams, returns the memory
+    if (o) --o->live;
     HBP_API_END
 }
 
@@ -229,6 +490,7 @@ int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_righ
     HBP_REQUIRE(srv && bgr_left && bgr_right && ticket, "NULL argument");
     HBP_REQUIRE(width >= 9 && height >= 7 && labels >= 1 && labels <= width && labels <= 1024,
                 "bad image size or disparity range");
+    HBP_REQUIRE((long)width * height < (1L << 24) && width < 65536 && height < 65536, "image too large");
     auto r = std::make_unique<Request>();
     r->id = srv->s.next_id++;
     r->W = width;
@@ -239,6 +501,7 @@ int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_righ
     r->left.assign(bgr_left, bgr_left + n3);
     r->right.assign(bgr_right, bgr_right + n3);
     r->est = estimate_bytes(width, height, labels);
+    r->est_admit = r->est + volume_bytes(width, height, labels);
     *ticket = r->id;
     srv->s.pending.push_back(std::move(r));
     HBP_API_END
@@ -255,28 +518,46 @@ int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending, int *n_done)
 int hbp_poll(hbp_server *srv, long long ticket, float *disp, float *plane3) {
     try {
         HBP_REQUIRE(srv, "NULL argument");
-        auto it = srv->s.finished.find(ticket);
-        if (it == srv->s.finished.end()) {
-            for (auto &r : srv->s.pending)
-                if (r->id == ticket) return 1;
-            for (auto &r : srv->s.active)
-                if (r->id == ticket) return 1;
+        Server &s = srv->s;
+        auto it = s.finished.find(ticket);
+        if (it == s.finished.end()) {
+            if (ticket >= 1 && ticket < s.next_id) {
+                for (auto &r : s.pending)
+                    if (r->id == ticket) return 1;
+                for (auto &r : s.active)
+                    if (r->id == ticket) return 1;
+                for (auto &r : s.draining)
+                    if (r->id == ticket) return 1;
+                if (s.in_admission.count(ticket)) return 1;
+            }
             throw HipError{"unknown ticket"};
         }
         Request &r = *it->second;
         if (r.status != 0) {
             std::string msg = r.error;
-            srv->s.finished.erase(it);
+            s.finished.erase(it);
             throw HipError{msg};
         }
-        if (disp) std::memcpy(disp, r.disp.data(), sizeof(float) * r.disp.size());
-        if (plane3) std::memcpy(plane3, r.plane.data(), sizeof(float) * r.plane.size());
-        srv->s.finished.erase(it);
+        const size_t n = (size_t)r.W * r.H;
+        if (disp) std::memcpy(disp, r.out, sizeof(float) * n);
+        if (plane3) std::memcpy(plane3, r.out + n, sizeof(float) * 3 * n);
+        s.pinned.put(r.out, r.out_bytes);
+        r.out = nullptr;
+        s.finished.erase(it);
         return 0;
     } catch (const HipError &e) {
         set_error(e.msg);
         return -1;
     }
+}
+
+int hbp_cancel(hbp_server *srv, long long ticket) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(srv, "NULL argument");
+    HBP_CHECK(hipSetDevice(srv->s.c->device));
+    int r = cancel(srv->s, ticket);
+    HBP_REQUIRE(r == 0, "unknown ticket");
+    HBP_API_END
 }
 
 }  // extern "C"

@@ -5,8 +5,11 @@ a missing library, a missing symbol or a failing call raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -82,6 +85,8 @@ SIGNATURES = {
     "hbp_submit": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
     "hbp_server_step": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hbp_poll": (C.c_int, [_vp, C.c_longlong, _vp, _vp]),
+    "hbp_cancel": (C.c_int, [_vp, C.c_longlong]),
+    "hbp_inject_fault": (C.c_int, [C.c_char_p, C.c_longlong]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
     "hbp_bp_dev_plane": (_vp, [_vp]),
     "hbp_bp_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -126,6 +131,58 @@ def _check(rc):
         raise HbpError(load().hbp_last_error().decode())
 
 
+# Handles are freed explicitly (free()/close()), by their context's close(), or at interpreter
+# exit by the atexit hook below, which runs while the HIP runtime is still whole.  Finalisers
+# that run after it (or during interpreter shutdown) make no C call at all.
+_shutting_down = False
+_contexts = weakref.WeakSet()
+
+
+def _alive():
+    return not _shutting_down and not sys.is_finalizing()
+
+
+@atexit.register
+def _close_all():
+    global _shutting_down
+    for c in list(_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+    _shutting_down = True
+
+
+def inject_fault(what, after):
+    """Host-forced failure of the after-th next device allocation ("alloc") or batched round
+    ("round"); 0 disarms (hbp_inject_fault)."""
+    _check(load().hbp_inject_fault(what.encode(), int(after)))
+
+
+class _Handle:
+    """A library object owned by a context: freed before the context is."""
+
+    _free_fn = None
+
+    def _adopt(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        ctx._children.add(self)
+
+    def free(self):
+        h = getattr(self, "h", None)
+        if not h:
+            return
+        self.h = None
+        ctx = self.ctx
+        ctx._children.discard(self)
+        if ctx.h and _alive():
+            getattr(ctx.lib, self._free_fn)(ctx.h, h)
+
+    def __del__(self):
+        if _alive():
+            self.free()
+
+
 class Ctx:
     """One device context (one HIP stream, one device memory pool)."""
 
@@ -135,14 +192,24 @@ class Ctx:
         _check(lib.hbp_ctx_create(device, C.byref(h)))
         self.h = h
         self.lib = lib
+        self._children = weakref.WeakSet()
+        _contexts.add(self)
 
     def close(self):
-        if getattr(self, "h", None):
-            self.lib.hbp_ctx_destroy(self.h)
-            self.h = None
+        """Frees the context's servers, BP objects, pyramids and volumes, then the context."""
+        if not getattr(self, "h", None):
+            return
+        kids = list(self._children)
+        order = {"Server": 0, "BP": 1, "Pyramid": 2, "Volume": 3}
+        for k in sorted(kids, key=lambda o: order.get(type(o).__name__, 4)):
+            k.free()
+        rc = self.lib.hbp_ctx_destroy(self.h)
+        self.h = None
+        _check(rc)
 
     def __del__(self):
-        self.close()
+        if _alive():
+            self.close()
 
     def sync(self):
         _check(self.lib.hbp_ctx_sync(self.h))
@@ -251,7 +318,9 @@ class Ctx:
         return d
 
 
-class Volume:
+class Volume(_Handle):
+    _free_fn = "hbp_volume_free"
+
     def __init__(self, ctx: Ctx, left, right, D, right_view=False, f32=None, cost="Census"):
         self.ctx = ctx
         h = C.c_void_p()
@@ -264,18 +333,10 @@ class Volume:
             _check(ctx.lib.hbp_cost_volume(ctx.h, np.ascontiguousarray(left), np.ascontiguousarray(right), W, H, D,
                                            cost.encode(), C.byref(h)))
         self.f32 = f32 is not None or cost == "AD"
-        self.h = h
+        self._adopt(ctx, h)
         self.W, self.H, self.D = W, H, D
         if right_view:
             _check(ctx.lib.hbp_cost_to_right(ctx.h, self.h))
-
-    def free(self):
-        if getattr(self, "h", None):
-            self.ctx.lib.hbp_volume_free(self.ctx.h, self.h)
-            self.h = None
-
-    def __del__(self):
-        self.free()
 
     def download_f32(self):
         vol = np.zeros((self.D, self.H, self.W), np.float32)
@@ -290,7 +351,9 @@ class Volume:
         return vol, wta
 
 
-class Pyramid:
+class Pyramid(_Handle):
+    _free_fn = "hbp_pyramid_free"
+
     def __init__(self, ctx: Ctx, bgr, num_nb=4, link=2, target_clus=1, sigma=0.0):
         self.ctx = ctx
         H, W = bgr.shape[:2]
@@ -301,16 +364,8 @@ class Pyramid:
         else:
             _check(ctx.lib.hbp_pyramid_build_target(ctx.h, np.ascontiguousarray(bgr), W, H, link, num_nb,
                                                     float(sigma), target_clus, C.byref(h)))
-        self.h = h
+        self._adopt(ctx, h)
         self.W, self.H = W, H
-
-    def free(self):
-        if getattr(self, "h", None):
-            self.ctx.lib.hbp_pyramid_free(self.ctx.h, self.h)
-            self.h = None
-
-    def __del__(self):
-        self.free()
 
     @property
     def levels(self):
@@ -329,21 +384,15 @@ class Pyramid:
         return dict(V=V, E=E, clus=clus, mean=mean.reshape(6, V), bd=bd.reshape(E, 2), bdl=bdl)
 
 
-class BP:
+class BP(_Handle):
+    _free_fn = "hbp_bp_free"
+
     def __init__(self, ctx: Ctx, pyr: Pyramid, vol: Volume, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
         self.ctx, self.pyr, self.vol = ctx, pyr, vol
         h = C.c_void_p()
         _check(ctx.lib.hbp_bp_create(ctx.h, pyr.h, vol.h, alpha, tau_s, min_level, use_cost_vol, C.byref(h)))
-        self.h = h
+        self._adopt(ctx, h)
         self.W, self.H = pyr.W, pyr.H
-
-    def free(self):
-        if getattr(self, "h", None):
-            self.ctx.lib.hbp_bp_free(self.ctx.h, self.h)
-            self.h = None
-
-    def __del__(self):
-        self.free()
 
     @property
     def total_sweeps(self):
@@ -386,14 +435,14 @@ def exported_symbols():
     return sorted(set(re.findall(r"\b(hbp_[a-z0-9_]+)\s*\(", txt)))
 
 
-class Server:
+class Server(_Handle):
     """Continuous-batching request server on a context (hbp_server_*, hbp_submit, hbp_poll)."""
 
     def __init__(self, ctx: Ctx, max_active=64, hbm_budget=64 << 30, alpha=1000.1, tau_s=0.6):
         self.ctx = ctx
         h = C.c_void_p()
         _check(ctx.lib.hbp_server_create(ctx.h, max_active, int(hbm_budget), alpha, tau_s, C.byref(h)))
-        self.h = h
+        self._adopt(ctx, h)
         self.sizes = {}
 
     def submit(self, left, right, labels, min_level=2):
@@ -411,8 +460,8 @@ class Server:
         return a.value, p.value, d.value
 
     def poll(self, ticket):
-        """(disp, plane) when done, else None."""
-        H, W = self.sizes[ticket]
+        """(disp, plane) when done, else None; raises for a failed or cancelled request."""
+        H, W = self.sizes.get(ticket, (0, 0))
         disp = np.zeros((H, W), np.float32)
         plane = np.zeros((H, W, 3), np.float32)
         r = self.ctx.lib.hbp_poll(self.h, ticket, _np_ptr(disp), _np_ptr(plane))
@@ -423,10 +472,17 @@ class Server:
         del self.sizes[ticket]
         return disp, plane
 
-    def close(self):
-        if getattr(self, "h", None):
-            self.ctx.lib.hbp_server_destroy(self.h)
-            self.h = None
+    def cancel(self, ticket):
+        _check(self.ctx.lib.hbp_cancel(self.h, ticket))
+        self.sizes.pop(ticket, None)
 
-    def __del__(self):
-        self.close()
+    def free(self):
+        h = getattr(self, "h", None)
+        if not h:
+            return
+        self.h = None
+        self.ctx._children.discard(self)
+        if self.ctx.h and _alive():
+            self.ctx.lib.hbp_server_destroy(h)
+
+    close = free

@@ -27,6 +27,8 @@ typedef struct hbp_bp hbp_bp;
 /* ---- context: replaces the implicit device/legacy stream of the reference
  *      (cudaSetDevice(0), main_stereo_hbp_isp.cu:31) */
 int hbp_ctx_create(int device, hbp_ctx **out);
+/* fails (and destroys nothing) while volumes, pyramids, BP objects or servers made on the
+ * context are still alive: free those first */
 int hbp_ctx_destroy(hbp_ctx *ctx);
 int hbp_ctx_sync(hbp_ctx *ctx);
 void *hbp_ctx_stream(hbp_ctx *ctx); /* hipStream_t of the context */
@@ -163,10 +165,12 @@ int hbp_schedule_sweeps(int levels, int min_level, int labels);
 
 /* ---- request server: continuous batching (no counterpart in the reference, which runs one
  *      pair per call).  One host thread drives a server: hbp_submit copies a pair into the
- *      queue and returns a ticket; every hbp_server_step admits queued pairs while a batch
- *      slot is free and their estimated device memory fits hbm_budget_bytes (cost volume,
- *      pyramid and BP set-up run at admission), runs one BP sweep of every admitted pair in
- *      shared launches, and maps + copies out the pairs that finished.  hbp_poll returns 0 and
+ *      queue and returns a ticket; every hbp_server_step hands queued pairs to the server's
+ *      admission thread while a batch slot is free and their estimated device memory fits
+ *      hbm_budget_bytes (cost volume, pyramid and BP set-up run there, on a stream of their
+ *      own, without blocking the step), lets admitted pairs join, runs one BP sweep of every
+ *      pair on the device in shared launches, and queues the map and copies out of the pairs
+ *      that finished, behind an event (no step waits for the device).  hbp_poll returns 0 and
  *      copies disp [H][W] / plane [H][W][3] (either may be NULL) once the pair is done, 1
  *      while it is queued or running, < 0 on error (a pair that failed or does not fit).
  *      Each pair's result equals its own hbp_bp_run with the same parameters (num_nb 4, Census,
@@ -177,10 +181,20 @@ int hbp_server_create(hbp_ctx *ctx, int max_active, long long hbm_budget_bytes, 
 int hbp_server_destroy(hbp_server *srv);
 int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_right, int width,
                int height, int labels, int min_level, long long *ticket);
-/* n_active / n_pending: pairs on the device / queued after the step; n_done: pairs completed
- * by it (any may be NULL) */
+/* n_active: pairs in the BP rounds; n_pending: pairs queued or being admitted; n_done: pairs
+ * whose results became ready in this step (any may be NULL) */
 int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending, int *n_done);
 int hbp_poll(hbp_server *srv, long long ticket, float *disp, float *plane3);
+/* withdraw a request: a queued one leaves at once, one being admitted or on the device leaves
+ * the rounds and its memory returns once the queued work has run; it then polls as an error
+ * ("cancelled").  A finished result is discarded.  Unknown tickets are an error. */
+int hbp_cancel(hbp_server *srv, long long ticket);
+
+/* ---- host-forced failures for tests of the error paths (no counterpart in the reference,
+ *      which checks no errors).  what = "alloc": the after-th next device allocation of any
+ *      context fails; "round": the after-th next batched round of a server fails before it
+ *      launches anything.  after = 0 disarms.  No kernel runs in a failing state. */
+int hbp_inject_fault(const char *what, long long after);
 
 /* ---- (d) PostProcess(imL, imR) (post_process.hpp:47-55)
  * left_right_check (post_process.cu:47-110): host buffers; plane_left is filled in place;

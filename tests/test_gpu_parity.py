@@ -111,6 +111,30 @@ def test_pyramid_large_pair(ctx):
         assert same_bits(a["mean"], b["mean"]), l
 
 
+@pytest.mark.parametrize("W,H,kind", [(20, 1000, "flat"), (24, 900, "columns")])
+def test_pyramid_level_whose_crtrees_outgrows_the_schedule(ctx, W, H, kind):
+    """A level above the one-block size (> 16000 nodes) whose 1-NN chains are longer than the
+    scheduled pointer jumping covers (2^8): in a flat image every pixel's first minimum is the
+    pixel above it, so each column is one chain of H nodes.  The scheduled kernels overflow and
+    the cooperative kernel redoes the level; the labels the compaction reads in between must
+    stay in range and the level must equal the oracle's."""
+    say(f"pyramid CRTrees overflow {W}x{H} {kind}")
+    left = np.full((H, W, 3), 90, np.uint8)
+    if kind == "columns":
+        left[:, :, 1] = (np.arange(W) * 9 % 256).astype(np.uint8)[None, :]
+    assert W * H > 16000 and H > 2 ** 8
+    po = O.Pyramid(left)
+    pg = ctx.pyramid(left)
+    assert pg.levels == po.levels
+    for l in range(po.levels):
+        a, b = po.level(l), pg.level(l)
+        assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+        assert np.array_equal(a["clus"], b["clus"]), l
+        assert np.array_equal(a["bd"], b["bd"]), l
+        assert np.array_equal(a["bdl"], b["bdl"]), l
+        assert same_bits(a["mean"], b["mean"]), l
+
+
 @pytest.mark.parametrize("link", [0, 1, 3])
 def test_pyramid_linkages_large_image(ctx, link):
     say(f"pyramid 1280x720 link={link}")
