@@ -115,6 +115,7 @@ int hbp_ctx_create(int device, hbp_ctx **out) {
     ctx->c.num_cu = prop.multiProcessorCount;
     HBP_CHECK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
     ctx->c.coop_blocks = crtrees_coop_blocks(ctx->c.num_cu);
+    lr_scratch_setup(device);
     *out = ctx;
     HBP_API_END
 }
@@ -612,6 +613,15 @@ int hbp_lr_check_fill_dev(hbp_ctx *ctx, const float *dl, const float *dr, float 
     size_t n = (size_t)W * H;
     DevBuf<uint8_t> tmp(ctx->c.pool, 3 * n);
     lr_check_fill(ctx->c, dl, dr, pl, W, H, fl ? fl : tmp.p, fr ? fr : tmp.p + n, f2 ? f2 : tmp.p + 2 * n);
+    HBP_API_END
+}
+
+int hbp_lr_check_fill_dev_noalloc(hbp_ctx *ctx, const float *dl, const float *dr, float *pl, int W, int H,
+                                  uint8_t *fl, uint8_t *fr, uint8_t *f2) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && dl && dr && pl && W > 0 && H > 0, "bad argument");
+    use_device(ctx);
+    lr_check_fill_noalloc(ctx->c, dl, dr, pl, W, H, fl, fr, f2);
     HBP_API_END
 }
 

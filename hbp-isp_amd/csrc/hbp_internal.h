@@ -145,6 +145,11 @@ int bp_total_sweeps(int levels, int min_level, int D);
 void lr_check_fill(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
                    uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2);
 void plane_to_disp(Ctx &c, const float *d_plane3, int W, int H, float *d_disp);
+// the allocation-free form: scratch in a static device array, set up once per device
+void lr_scratch_setup(int device);
+long lr_noalloc_max_pixels();
+void lr_check_fill_noalloc(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
+                           uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2);
 void weighted_median(Ctx &c, const uint8_t *d_bgr, const uint8_t *d_fail, float *d_plane3, int W, int H);
 void median7_filter(Ctx &c, float *d_disp, int W, int H);
 void eval_bad_pixels(Ctx &c, const float *d_disp, const float *d_gt, const uint8_t *d_nonocc, int W, int H,

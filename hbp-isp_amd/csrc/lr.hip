@@ -1,5 +1,7 @@
 // Stage (d): left-right consistency check and fill (post_process.cu of the reference, which
 // runs it on the host with OpenCV; here it is four small kernels on the device).
+#include <mutex>
+
 #include "hbp_internal.h"
 
 namespace hbp {
@@ -101,20 +103,70 @@ static int grid_for(const Ctx &c, long n, int block = 256) {
     return (int)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
+// the four kernels; lft / rgt: 2 * W * H ints of scratch, every entry written before it is read
+static void lr_launch(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
+                      uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2, int *lft, int *rgt) {
+    const long n = (long)W * H;
+    hipLaunchKernelGGL(lr_consistency_kernel, dim3(grid_for(c, 2 * n)), dim3(256), 0, c.stream, d_dispL, d_dispR, W,
+                       H, d_failL, d_failR);
+    hipLaunchKernelGGL(dilate3_kernel, dim3(grid_for(c, n)), dim3(256), 0, c.stream, d_failL, W, H, d_fail2);
+    hipLaunchKernelGGL(nearest_valid_kernel, dim3(grid_for(c, H, 64)), dim3(64), 0, c.stream, d_fail2, W, H, lft, rgt);
+    hipLaunchKernelGGL(lr_fill_kernel, dim3(grid_for(c, n)), dim3(256), 0, c.stream, d_failL, lft, rgt, W, H,
+                       d_planeL);
+    HBP_CHECK(hipGetLastError());
+}
+
 void lr_check_fill(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
                    uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2) {
     long n = (long)W * H;
     // algorithmic bytes: both disparities in, three masks out, the left planes in and out
     ProfScope ps(c, "lr_check", (8.0 + 3.0 + 24.0) * n);
-    hipLaunchKernelGGL(lr_consistency_kernel, dim3(grid_for(c, 2 * n)), dim3(256), 0, c.stream, d_dispL, d_dispR, W,
-                       H, d_failL, d_failR);
-    hipLaunchKernelGGL(dilate3_kernel, dim3(grid_for(c, n)), dim3(256), 0, c.stream, d_failL, W, H, d_fail2);
     DevBuf<int> lr(c.pool, 2 * n);
-    hipLaunchKernelGGL(nearest_valid_kernel, dim3(grid_for(c, H, 64)), dim3(64), 0, c.stream, d_fail2, W, H, lr.p,
-                       lr.p + n);
-    hipLaunchKernelGGL(lr_fill_kernel, dim3(grid_for(c, n)), dim3(256), 0, c.stream, d_failL, lr.p, lr.p + n, W, H,
-                       d_planeL);
-    HBP_CHECK(hipGetLastError());
+    lr_launch(c, d_dispL, d_dispR, d_planeL, W, H, d_failL, d_failR, d_fail2, lr.p, lr.p + n);
+}
+
+// ---- the allocation-free form (hbp_lr_check_fill_dev_noalloc).  Its scratch is this
+// file-scope device array: the row scans' nearest valid columns (2 ints per pixel), then the
+// three masks for callers that pass none (3 bytes per pixel), for images of up to
+// kNoAllocMaxPixels pixels: 88 MiB that each GPU the library runs on holds until the process
+// ends.  Host code never uses the array's name as an address: its device address is looked up
+// once per device (lr_scratch_setup, from hbp_ctx_create) and passed to the kernels.  Every call
+// on one GPU shares the array, so calls that overlap on one GPU race; no part of it needs
+// clearing (each entry a call reads, the call has written first).
+constexpr long kNoAllocMaxPixels = 1L << 23;
+constexpr size_t kLrScratchBytes = (size_t)kNoAllocMaxPixels * (2 * sizeof(int) + 3);
+static __device__ __align__(256) uint8_t scratch[kLrScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+void lr_scratch_setup(int device) {   // with `device` current
+    HBP_REQUIRE(device >= 0 && device < kMaxDevices, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    HBP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HBP_REQUIRE(p, "no device address for the scratch array");
+    g_scratch_ptr[device] = (uint8_t *)p;
+}
+
+long lr_noalloc_max_pixels() { return kNoAllocMaxPixels; }
+
+void lr_check_fill_noalloc(Ctx &c, const float *d_dispL, const float *d_dispR, float *d_planeL, int W, int H,
+                           uint8_t *d_failL, uint8_t *d_failR, uint8_t *d_fail2) {
+    const long n = (long)W * H;
+    HBP_REQUIRE(n <= kNoAllocMaxPixels, "image larger than the allocation-free scratch holds (" +
+                                            std::to_string(kNoAllocMaxPixels) + " pixels)");
+    uint8_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = c.device >= 0 && c.device < kMaxDevices ? g_scratch_ptr[c.device] : nullptr;
+    }
+    HBP_REQUIRE(base, "the scratch array of this device is not set up");
+    int *lft = (int *)base, *rgt = lft + n;
+    uint8_t *masks = base + sizeof(int) * 2 * (size_t)kNoAllocMaxPixels;
+    lr_launch(c, d_dispL, d_dispR, d_planeL, W, H, d_failL ? d_failL : masks, d_failR ? d_failR : masks + n,
+              d_fail2 ? d_fail2 : masks + 2 * n, lft, rgt);
 }
 
 void plane_to_disp(Ctx &c, const float *d_plane3, int W, int H, float *d_disp) {

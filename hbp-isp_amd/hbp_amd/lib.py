@@ -100,6 +100,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_longlong)]),
     "hbp_lr_check_fill": (C.c_int, [_vp, _f32, _f32, _f32, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hbp_lr_check_fill_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "hbp_lr_check_fill_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hbp_plane_to_disp": (C.c_int, [_vp, _f32, C.c_int, C.c_int, _f32]),
 }
 

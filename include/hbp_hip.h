@@ -206,6 +206,14 @@ int hbp_lr_check_fill(hbp_ctx *ctx, const float *disp_left, const float *disp_ri
 int hbp_lr_check_fill_dev(hbp_ctx *ctx, const float *d_disp_left, const float *d_disp_right,
                           float *d_plane_left, int width, int height, uint8_t *d_fail_left,
                           uint8_t *d_fail_right, uint8_t *d_fail2_left);
+/* the same as hbp_lr_check_fill_dev, allocating nothing (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops).  Its scratch is a static device array of the library, 88 MiB per GPU, held until the
+ * process ends; images of more than 2^23 pixels are an error.  Calls that overlap on one GPU
+ * share that array and race: order them (one stream, or events). */
+int hbp_lr_check_fill_dev_noalloc(hbp_ctx *ctx, const float *d_disp_left, const float *d_disp_right,
+                                  float *d_plane_left, int width, int height, uint8_t *d_fail_left,
+                                  uint8_t *d_fail_right, uint8_t *d_fail2_left);
 /* plane_label_to_disp (post_process.cu:373-390), host buffers */
 int hbp_plane_to_disp(hbp_ctx *ctx, const float *plane3, int width, int height, float *disp);
 /* median_filter / median_filter_gpu (post_process.cu:127-185, 187-371): every pixel with
