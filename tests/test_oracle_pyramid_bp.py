@@ -243,17 +243,6 @@ def test_bp_select_keeps_the_best_beliefs(bp_case):
                 assert (v in kept) or v >= kept.max()
 
 
-def test_bp_messages_have_zero_mean_before_compaction(bp_case):
-    left, right, gt, vol, wta, p = bp_case
-    bp = O.BP(p, vol, wta, 16, min_level=2)
-    bp.step(3)
-    s = bp.state()
-    m = s["msg"]
-    finite = np.isfinite(m).all(1)
-    # the 8 kept slots are a selection of a zero-mean 12-vector: bounded by its spread
-    assert np.all(np.abs(m[finite]).max(1) < 1e7)
-
-
 def test_bp_full_run_beats_wta_at_low_alpha(bp_case):
     left, right, gt, vol, wta, p = bp_case
     bp = O.BP(p, vol, wta, 16, min_level=2, alpha=0.05)
