@@ -139,7 +139,9 @@ static void admit_work(Server &s, Request &r) {
         r.bp.use_cost_vol = 1;
         const int L = r.pyr.levels();
         HBP_REQUIRE(L >= 2, "image too small for a pyramid of two levels");
-        r.bp.min_level = std::max(1, std::min(r.min_level, L - 1));
+        // min_level < 0: BP on -min_level levels below the top (min_level = L - 1 + min_level)
+        const int ml = r.min_level < 0 ? L - 1 + r.min_level : r.min_level;
+        r.bp.min_level = std::max(1, std::min(ml, L - 1));
         bp_setup(c, r.bp);
         HBP_CHECK(hipStreamSynchronize(c.stream));
     }

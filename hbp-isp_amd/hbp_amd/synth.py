@@ -64,14 +64,19 @@ def _value_noise(key: np.uint64, x: np.ndarray, y: np.ndarray, pitch: float) -> 
     ty = ty * ty * (3.0 - 2.0 * ty)
     gx = gx.astype(np.int64) + 4096
     gy = gy.astype(np.int64) + 4096
-
-    def lat(ix, iy):
-        return _u01(key, (iy.astype(np.uint64) << np.uint64(20)) | ix.astype(np.uint64)) * 2.0 - 1.0
-
-    v00 = lat(gx, gy)
-    v10 = lat(gx + 1, gy)
-    v01 = lat(gx, gy + 1)
-    v11 = lat(gx + 1, gy + 1)
+    if gx.size == 0:
+        return np.zeros(gx.shape)
+    # the lattice values once per lattice point of the covering box, then gathered per pixel
+    x0, y0 = int(gx.min()), int(gy.min())
+    ix = np.arange(x0, int(gx.max()) + 2, dtype=np.uint64)
+    iy = np.arange(y0, int(gy.max()) + 2, dtype=np.uint64)
+    lat = _u01(key, (iy[:, None] << np.uint64(20)) | ix[None, :]) * 2.0 - 1.0
+    gx -= x0
+    gy -= y0
+    v00 = lat[gy, gx]
+    v10 = lat[gy, gx + 1]
+    v01 = lat[gy + 1, gx]
+    v11 = lat[gy + 1, gx + 1]
     top = v00 + (v10 - v00) * tx
     bot = v01 + (v11 - v01) * tx
     return top + (bot - top) * ty
@@ -127,29 +132,32 @@ class Scene:
 
 
 def make_pair(seed: int, width: int, height: int, ndisp: int,
-              n_planes: tuple[int, int] = (6, 12)):
-    """Return (left_bgr u8 [H,W,3], right_bgr u8 [H,W,3], gt_disp f32 [H,W])."""
+              n_planes: tuple[int, int] = (6, 12), rows: tuple[int, int] | None = None):
+    """Return (left_bgr u8 [H,W,3], right_bgr u8 [H,W,3], gt_disp f32 [H,W]); with rows = (y0, y1)
+    only those rows of the same pair (every pixel depends on its own coordinates alone)."""
     sc = Scene(seed, width, height, ndisp, n_planes)
-    ys, xs = np.mgrid[0:height, 0:width].astype(np.float64)
+    y0, y1 = (0, height) if rows is None else rows
+    ys, xs = np.mgrid[y0:y1, 0:width].astype(np.float64)
+    hh = y1 - y0
 
     # left view: z-buffer on disparity
-    best_d = np.full((height, width), -np.inf)
-    best_k = np.zeros((height, width), dtype=np.int32)
+    best_d = np.full((hh, width), -np.inf)
+    best_k = np.zeros((hh, width), dtype=np.int32)
     for k in range(sc.n):
         d = sc.disparity(k, xs, ys)
         m = sc.covers(k, xs, ys) & (d > best_d)
         best_d[m] = d[m]
         best_k[m] = k
-    left = np.empty((height, width, 3), dtype=np.float64)
+    left = np.empty((hh, width, 3), dtype=np.float64)
     for k in range(sc.n):
         m = best_k == k
         if m.any():
             left[m] = sc.texture(k, xs[m], ys[m])
 
     # right view: x_l = (x_r + a*cx - b*(y-cy) ... ) solved per plane
-    rbest_d = np.full((height, width), -np.inf)
-    rbest_k = np.zeros((height, width), dtype=np.int32)
-    rbest_xl = np.zeros((height, width))
+    rbest_d = np.full((hh, width), -np.inf)
+    rbest_k = np.zeros((hh, width), dtype=np.int32)
+    rbest_xl = np.zeros((hh, width))
     for k in range(sc.n):
         xl = (xs - sc.a[k] * sc.cx[k] + sc.b[k] * (ys - sc.cy[k]) + sc.c[k]) / (1.0 - sc.a[k])
         d = sc.disparity(k, xl, ys)
@@ -157,7 +165,7 @@ def make_pair(seed: int, width: int, height: int, ndisp: int,
         rbest_d[m] = d[m]
         rbest_k[m] = k
         rbest_xl[m] = xl[m]
-    right = np.empty((height, width, 3), dtype=np.float64)
+    right = np.empty((hh, width, 3), dtype=np.float64)
     for k in range(sc.n):
         m = rbest_k == k
         if m.any():
@@ -165,8 +173,9 @@ def make_pair(seed: int, width: int, height: int, ndisp: int,
 
     kn = _key(seed, 3)
     npx = height * width * 3
-    left += _u01(kn, np.arange(npx, dtype=np.uint64)).reshape(height, width, 3) * 4.0 - 2.0
-    right += _u01(kn, np.arange(npx, 2 * npx, dtype=np.uint64)).reshape(height, width, 3) * 4.0 - 2.0
+    a0, a1 = y0 * width * 3, y1 * width * 3
+    left += _u01(kn, np.arange(a0, a1, dtype=np.uint64)).reshape(hh, width, 3) * 4.0 - 2.0
+    right += _u01(kn, np.arange(npx + a0, npx + a1, dtype=np.uint64)).reshape(hh, width, 3) * 4.0 - 2.0
     kg = _key(seed, 4)
     gain = 0.95 + 0.10 * float(_u01(kg, 0))
     bias = -5.0 + 10.0 * float(_u01(kg, 1))
@@ -176,6 +185,45 @@ def make_pair(seed: int, width: int, height: int, ndisp: int,
     right_u8 = np.clip(np.rint(right), 0, 255).astype(np.uint8)
     gt = np.clip(best_d, 0.0, ndisp - 1).astype(np.float32)
     return left_u8, right_u8, gt
+
+
+def _rows_job(a):
+    return make_pair(*a)
+
+
+def make_pair_parallel(seed: int, width: int, height: int, ndisp: int,
+                       n_planes: tuple[int, int] = (6, 12), workers: int = 16, pool=None):
+    """make_pair's result, its rows generated in chunks by a process pool (`pool`, or one of
+    `workers` processes made for the call)."""
+    import multiprocessing as mp
+    n = max(1, height // 64)
+    cuts = [height * i // n for i in range(n + 1)]
+    jobs = [(seed, width, height, ndisp, n_planes, (cuts[i], cuts[i + 1])) for i in range(n)]
+    if pool is not None:
+        parts = pool.map(_rows_job, jobs)
+    else:
+        with mp.get_context("fork").Pool(max(1, min(workers, n))) as own:
+            parts = own.map(_rows_job, jobs)
+    return tuple(np.concatenate([p[i] for p in parts], axis=0) for i in range(3))
+
+
+def make_pairs_parallel(specs, n_planes: tuple[int, int] = (6, 12), workers: int = 16, pool=None):
+    """Pairs (seed, width, height, ndisp) generated by a process pool: [(left, right), ...]."""
+    import multiprocessing as mp
+    jobs = []
+    for k, (sd, w, h, d) in enumerate(specs):
+        n = max(1, h // 64)
+        jobs += [(k, (sd, w, h, d, n_planes, (h * i // n, h * (i + 1) // n))) for i in range(n)]
+    if pool is not None:
+        parts = pool.map(_rows_job, [j for _, j in jobs])
+    else:
+        with mp.get_context("fork").Pool(max(1, min(workers, len(jobs)))) as own:
+            parts = own.map(_rows_job, [j for _, j in jobs])
+    out = []
+    for k in range(len(specs)):
+        mine = [p for (kk, _), p in zip(jobs, parts) if kk == k]
+        out.append(tuple(np.concatenate([p[i] for p in mine], axis=0) for i in range(2)))
+    return out
 
 
 # Workloads named by what they are (width, height, max disparity, plane count range).

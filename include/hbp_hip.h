@@ -174,7 +174,8 @@ int hbp_schedule_sweeps(int levels, int min_level, int labels);
  *      copies disp [H][W] / plane [H][W][3] (either may be NULL) once the pair is done, 1
  *      while it is queued or running, < 0 on error (a pair that failed or does not fit).
  *      Each pair's result equals its own hbp_bp_run with the same parameters (num_nb 4, Census,
- *      left view, min_level clamped to [1, levels-1]).  Destroy the server before its context. */
+ *      left view, min_level clamped to [1, levels-1]; a negative min_level -n asks for BP on n
+ *      levels, min_level = levels - 1 - n).  Destroy the server before its context. */
 typedef struct hbp_server hbp_server;
 int hbp_server_create(hbp_ctx *ctx, int max_active, long long hbm_budget_bytes, float alpha,
                       float tau_smooth, hbp_server **out);

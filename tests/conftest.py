@@ -26,3 +26,16 @@ def ctx():
     c = lib.Ctx(0)
     yield c
     c.close()
+
+
+def pytest_collection_modifyitems(session, config, items):
+    # the pair generator's worker processes are forked now, before any test initialises HIP
+    if any(item.fspath.basename == "test_gpu_workloads.py" for item in items):
+        import pairgen
+        pairgen.start()
+
+
+def pytest_sessionfinish(session, exitstatus):
+    import sys as _sys
+    if "pairgen" in _sys.modules:
+        _sys.modules["pairgen"].stop()
