@@ -57,6 +57,12 @@ def main():
                     help="skip the refinements after BP (right view, LR check, weighted median, 7-tap median)")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
                     help="sweeps the CPU baseline runs (0: scale to about 20 s)")
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="run only the library's 256 MiB float4 copy kernel three times (the known traffic that "
+                         "rocprofv3's FETCH_SIZE / WRITE_SIZE counts are scaled by) and exit")
+    ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV"),
+                    help="no GPU: the dominant kernel's HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and "
+                         "WRITE_SIZE counter CSVs of a bench run and of a --traffic-calibration run; prints JSON")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel forms instead of timing the candidates first (the forms "
                          "change speed only, never the results)")
@@ -66,10 +72,21 @@ def main():
               "no multi-GPU runner; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.traffic_summary:
+        print(json.dumps(traffic_summary(*args.traffic_summary), indent=1))
+        return
+
     import ctypes as C
     import numpy as np
     from hbp_amd import lib as H
     from hbp_amd import synth
+
+    if args.traffic_calibration:
+        ctx = H.Ctx(0)
+        H._check(ctx.lib.hbp_profile_copy(ctx.h, 256 << 20, 3))
+        ctx.close()
+        print(json.dumps({"calibration": "calib_copy_kernel: 3 launches, each reading and writing 262144 KiB"}))
+        return
 
     wl = WORKLOADS[args.workload]
     W, Hh, D = wl["width"], wl["height"], wl["ndisp"]
@@ -252,6 +269,44 @@ def main():
     hip.hipFree(d_l)
     hip.hipFree(d_r)
     ctx.close()
+
+
+DOMINANT_KERNEL = "cost_edge_kernel<4, true, 0, 4>"   # plane cost, coarse form, left view, 4-step batches
+CALIB_KIB = 262144.0                                  # the calibration copy reads and writes 256 MiB
+
+
+def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=DOMINANT_KERNEL):
+    """HBM bytes per launch of `kernel`: the mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
+    over its launches, each scaled by the ratio of the known bytes to the counter's reading on
+    the calibration copy (calib_copy_kernel)."""
+    import csv
+
+    def per_launch(path, name):
+        vals, grids = [], {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if name in row["Kernel_Name"]:
+                    v = float(row["Counter_Value"])
+                    vals.append(v)
+                    grids.setdefault(row["Grid_Size"], []).append(v)
+        return vals, grids
+
+    cf, _ = per_launch(cal_fetch_csv, "calib_copy_kernel")
+    cw, _ = per_launch(cal_write_csv, "calib_copy_kernel")
+    sf = CALIB_KIB / (sum(cf) / len(cf))
+    sw = CALIB_KIB / (sum(cw) / len(cw))
+    f, fg = per_launch(fetch_csv, kernel)
+    w, wg = per_launch(write_csv, kernel)
+    fk, wk = sum(f) / len(f), sum(w) / len(w)
+    return {"kernel": kernel, "bytes_per_launch": round((fk * sf + wk * sw) * 1024),
+            "fetch_kib_per_launch_raw": round(fk, 1), "write_kib_per_launch_raw": round(wk, 1),
+            "scale_fetch": round(sf, 4), "scale_write": round(sw, 4),
+            "calibration": f"calib_copy_kernel ({CALIB_KIB:.0f} KiB read + written per launch): FETCH_SIZE reads "
+                           f"{sum(cf) / len(cf):.2f} KiB, WRITE_SIZE {sum(cw) / len(cw):.2f} KiB per dispatch",
+            "launches": len(f),
+            "by_grid_size": {g: {"launches": len(v), "fetch_kib": round(sum(v) / len(v), 2),
+                                 "write_kib": round(sum(wg.get(g, [0])) / max(len(wg.get(g, [])), 1), 2)}
+                             for g, v in sorted(fg.items())}}
 
 
 def batch_spec(n, seed):

@@ -87,6 +87,13 @@ int crtrees_coop_blocks(int num_cu);
 
 static void use_device(hbp_ctx *ctx) { HBP_CHECK(hipSetDevice(ctx->c.device)); }
 
+// a copy of known size: each of its launches reads and writes exactly `n` float4 once, which is
+// what the hardware byte counters of a profiler are scaled by
+__global__ void calib_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 extern "C" {
 
 const char *hbp_last_error(void) { return hbp::g_last_error.c_str(); }
@@ -172,6 +179,20 @@ int hbp_profile_read(hbp_ctx *ctx, const char *kernel, double *ms, long long *la
     if (ms) *ms = s.ms;
     if (launches) *launches = s.launches;
     if (bytes) *bytes = s.bytes;
+    HBP_API_END
+}
+
+int hbp_profile_copy(hbp_ctx *ctx, long long bytes, int launches) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && bytes >= 16 && bytes % 16 == 0 && launches >= 1, "bad argument");
+    use_device(ctx);
+    const long n = (long)(bytes / 16);
+    DevBuf<float4> a(ctx->c.pool, (size_t)n), b(ctx->c.pool, (size_t)n);
+    HBP_CHECK(hipMemsetAsync(a.p, 0, (size_t)bytes, ctx->c.stream));
+    for (int i = 0; i < launches; ++i)
+        hipLaunchKernelGGL(calib_copy_kernel, dim3(4096), dim3(256), 0, ctx->c.stream, a.p, b.p, n);
+    HBP_CHECK(hipGetLastError());
+    HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
     HBP_API_END
 }
 

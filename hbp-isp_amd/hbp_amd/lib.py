@@ -86,6 +86,7 @@ SIGNATURES = {
     "hbp_server_step": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hbp_poll": (C.c_int, [_vp, C.c_longlong, _vp, _vp]),
     "hbp_cancel": (C.c_int, [_vp, C.c_longlong]),
+    "hbp_profile_copy": (C.c_int, [_vp, C.c_longlong, C.c_int]),
     "hbp_inject_fault": (C.c_int, [C.c_char_p, C.c_longlong]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
     "hbp_bp_dev_plane": (_vp, [_vp]),

@@ -46,6 +46,10 @@ int hbp_profile_names(hbp_ctx *ctx, char *buf, int buflen);
 /* time only launches whose name starts with prefix (NULL or "" times all), and of those one
  * launch in `every` (1 = all) */
 int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every);
+/* a float4 copy of `bytes` (a multiple of 16), `launches` times (no counterpart in the
+ * reference): the kernel of known HBM traffic that a profiler's FETCH_SIZE / WRITE_SIZE
+ * counters are calibrated on (bench.py --traffic-calibration) */
+int hbp_profile_copy(hbp_ctx *ctx, long long bytes, int launches);
 
 /* ---- kernel-form tuning of a context (no counterpart in the reference: its kernels have one
  *      form each).  Every value gives the same results bit for bit; only the speed changes.
