@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--no-post", action="store_true",
                     help="skip the refinements after BP (right view, LR check, weighted median, 7-tap median)")
     ap.add_argument("--cpu-sweeps", type=int, default=0,
-                    help="sweeps the CPU baseline runs (0: scale to about 20 s)")
+                    help="sweeps of the one-thread CPU sample (0: about 12 s worth)")
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="run only the library's 256 MiB float4 copy kernel three times (the known traffic that "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE counts are scaled by) and exit")
@@ -226,7 +226,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(left, right, wl, info, args.cpu_sweeps)
+        cpu = cpu_baseline(left, right, wl, info, args.cpu_sweeps, disp_h, plane_h)
         if post is not None:
             post.update(cpu_weighted_median(*post_io))
 
@@ -593,44 +593,68 @@ def cpu_weighted_median(left, fl, filled, wm_gpu):
             "weighted_median_bit_identical": same}
 
 
-def cpu_baseline(left, right, wl, info, sweeps):
-    """The oracle on the box's host cores: cost volume + pyramid + the first `sweeps` BP sweeps
-    of the same pair, scaled to a whole pair by the sweep count."""
-    import ctypes as C
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(left, right, wl, info, sweeps, gpu_disp, gpu_plane):
+    """The oracle (host C++17, -O3, OpenMP over pixels, clusters and edges) on the GPU host's
+    cores.  All cores: the whole pair (cost volume, pyramid, every BP sweep, map), its disparity
+    and planes compared with the GPU's.  One thread: cost volume, pyramid and a bounded number of
+    sweeps, scaled to the whole pair by the sweep count."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_ffi as O
 
-    cores = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     D = wl["ndisp"]
+    ml, total = info["min_level"], info["sweeps"]
+    cores = os.cpu_count() or 1
+
+    def prefix(threads):
+        O.lib.orc_set_threads(threads)
+        t0 = time.perf_counter()
+        vol, wta = O.cost_volume(left, right, D)
+        pyr = O.Pyramid(left)
+        return time.perf_counter() - t0, O.BP(pyr, vol, wta, D, min_level=ml)
+
+    # all cores, the whole pair
+    t_pre, bp = prefix(cores)
     t0 = time.perf_counter()
-    vol, wta = O.cost_volume(left, right, D)
-    pyr = O.Pyramid(left)
-    t1 = time.perf_counter()
-    ml = info["min_level"]
-    bp = O.BP(pyr, vol, wta, D, min_level=ml)
-    total = info["sweeps"]
-    if sweeps <= 0:
-        # calibrate on 8 sweeps, then run about 20 s worth
-        tc = time.perf_counter()
-        done = bp.step(8)
-        per = (time.perf_counter() - tc) / max(done, 1)
-        sweeps = int(max(8, min(total, 20.0 / max(per, 1e-6))))
-        rest = sweeps - done
-    else:
-        done, rest = 0, sweeps
-    t2 = time.perf_counter()
-    done2 = bp.step(rest) if rest > 0 else 0
-    t3 = time.perf_counter()
-    ran = done + done2
-    per_sweep = (t3 - t2) / max(done2, 1) if done2 else (t3 - t2)
-    pair_s = (t1 - t0) + per_sweep * total
-    return {"value": round(1.0 / pair_s, 5), "unit": "pairs/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port",
-            "sample": f"oracle (host C++, OpenMP) on the same pair: cost volume + pyramid ({t1 - t0:.2f} s) and "
-                      f"{ran} of {total} BP sweeps ({per_sweep * 1e3:.1f} ms/sweep at the top levels), "
-                      f"scaled to all sweeps"}
+    ran = bp.step()
+    disp, plane = bp.map()
+    t_all = t_pre + time.perf_counter() - t0
+    same_d = disp.view(np.uint32) == gpu_disp.view(np.uint32)
+    same_p = (plane.view(np.uint32) == gpu_plane.view(np.uint32)).all(axis=2)
+    differ = float(1.0 - (same_d & same_p).mean())
+    del bp
+    # one thread, a sample: the prefix and `n1` sweeps
+    t_pre1, bp1 = prefix(1)
+    n1 = sweeps if sweeps > 0 else 0
+    tc = time.perf_counter()
+    done = bp1.step(4)
+    per = (time.perf_counter() - tc) / max(done, 1)
+    if n1 <= 0:
+        n1 = int(max(8, min(total, 12.0 / max(per, 1e-6))))
+    rest = max(0, n1 - done)
+    tc = time.perf_counter()
+    done2 = bp1.step(rest) if rest else 0
+    per1 = (time.perf_counter() - tc) / done2 if done2 else per
+    t_one = t_pre1 + per1 * total
+    O.lib.orc_set_threads(cores)
+    return {"value": round(1.0 / t_all, 5), "unit": "pairs/s", "cores": cores, "kind": "port",
+            "cpu": cpu_model(), "nproc": cores,
+            "sample": f"oracle (host C++17 -O3 -march=x86-64-v3, OpenMP) on {cores} threads, the whole pair: cost "
+                      f"volume + pyramid {t_pre:.2f} s, all {ran} BP sweeps and the map, {t_all:.2f} s in all",
+            "matches_gpu": {"bit_identical": differ == 0.0, "pixels_differing_fraction": differ},
+            "one_thread": {"value": round(1.0 / t_one, 6), "unit": "pairs/s", "cores": 1,
+                           "sample": f"cost volume + pyramid {t_pre1:.2f} s and {done + done2} of {total} BP sweeps "
+                                     f"({per1 * 1e3:.1f} ms/sweep), scaled to all sweeps"}}
 
 
 if __name__ == "__main__":

@@ -16,6 +16,10 @@
 extern "C" {
 #endif
 
+/* OpenMP threads of the oracle's parallel loops (results do not depend on it) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* pinned scalar rules (exported so tests can pin them against libm) */
 double orc_pin_cbrt(double x);
 double orc_pin_exp(double x);

@@ -232,6 +232,7 @@ static void init_level(BP &b, int l) {
     BPLevel &Hh = b.lv[l + 1];
     b.msg.assign((long)K * B.E, 0.f);
     b.msg_t.assign((long)K * B.E, 0.f);
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < B.V; ++id) {
         int cc = B.lev->clus[id];
         for (int p = 0; p < P; ++p) {
@@ -247,6 +248,7 @@ static void init_level(BP &b, int l) {
     b.s_ransac.assign(B.V, 0);
     b.s_neighbor.assign(B.V, 0);
     b.s_child.assign(B.Vh, 0);
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < B.V; ++id) {
         b.s_neighbor[id] = (uint32_t)B.off[id];
         b.s_ransac[id] = (uint32_t)B.pix_off[id];
@@ -297,6 +299,7 @@ static inline uint32_t umod(uint32_t s, int m) { return (uint32_t)m == 0u ? 0u :
 // ransac_candidate_plane (hbp_isp.cu:592-640)
 static void ransac(BP &b, int l) {
     BPLevel &B = b.lv[l];
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < B.V; ++id) {
         int n = B.pix_off[id + 1] - B.pix_off[id];
         int pos = B.pix_off[id];
@@ -331,6 +334,7 @@ static void spatial(BP &b, int l) {
     BPLevel &B = b.lv[l];
     BPLevel &Hh = b.lv[l + 1];
     int Vh = Hh.V;
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < Vh; ++id) {
         int cnt = B.child_off[id + 1] - B.child_off[id];
         int pos = B.child_off[id];
@@ -344,6 +348,7 @@ static void spatial(BP &b, int l) {
         b.s_child[id] = s;
     }
     std::vector<F4> buf((long)P * Vh);
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < Vh; ++id) {
         int nb_num = Hh.off[id + 1] - Hh.off[id];
         int nb_pos = Hh.off[id];
@@ -358,6 +363,7 @@ static void spatial(BP &b, int l) {
     }
     for (int p = 0; p < P; ++p)
         for (int id = 0; id < Vh; ++id) Hh.plane[(long)(p + 2 * P) * Vh + id] = buf[(long)p * Vh + id];
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < B.V; ++id) {
         int c = B.lev->clus[id];
         for (int p = 2 * P; p < 3 * P; ++p) B.plane[(long)p * B.V + id] = Hh.plane[(long)p * Vh + c];
@@ -367,6 +373,7 @@ static void spatial(BP &b, int l) {
 // random_candidate_plane (hbp_isp.cu:641-674) with normalize_plane (495-503)
 static void random_adjust(BP &b, int l) {
     BPLevel &B = b.lv[l];
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < B.V; ++id) {
         uint32_t s = b.s_ransac[id];
         float x = B.mu[3L * B.V + id];
@@ -406,6 +413,7 @@ static void bp_step(BP &b, int l) {
     long E = B.E;
     std::vector<float> &msg = b.msg, &msg_t = b.msg_t;
     // passing_msg_k
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < E; ++e) {
         int t = Lv.bd[2 * e + 1];
         for (int p = 0; p < 2 * P; ++p) msg[(long)p * E + e] = B.belief[(long)p * V + t] - msg[(long)p * E + e];
@@ -456,6 +464,7 @@ static void bp_step(BP &b, int l) {
         for (int k = 0; k < K; ++k) msg_t[(long)k * E + e] -= mean;
     }
     // aggregate_belief: cost + messages in edge order
+#pragma omp parallel for schedule(static)
     for (int s = 0; s < V; ++s)
         for (int k = 0; k < K; ++k) {
             float acc = B.cost[(long)k * V + s];
@@ -463,10 +472,12 @@ static void bp_step(BP &b, int l) {
             B.belief[(long)k * V + s] = acc;
         }
     // ping_pang_msg_k
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < E; ++e)
         for (int k = 0; k < K; ++k) msg[(long)k * E + B.rev[e]] = msg_t[(long)k * E + e];
     // get_belief_order, select_belief
     std::vector<int> order((long)K * V, 0);
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < V; ++id) {
         for (int i = 0; i < K - 1; ++i)
             for (int j = i + 1; j < K; ++j) {
@@ -483,6 +494,7 @@ static void bp_step(BP &b, int l) {
             }
     }
     // select_msg
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < E; ++e) {
         int t = Lv.bd[2 * e + 1];
         int nb = 0;

@@ -1,5 +1,8 @@
 // Oracle (test infrastructure only): stage (a), census transform + Hamming cost volume.
 // Restates stereo_cost.cu and imutil.cu of the reference (citations per function).
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -172,6 +175,21 @@ void orc_vol_to_right(uint8_t *vol, int W, int H, int D) {
 
 // compute_stereo_cost("Census") (stereo_cost.cu:421-445, 299-331): tau = 25, then WTA.
 // right_view applies cost_to_right (stereo_cost.cu:488-516) before its WTA.
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 int orc_cost_volume(const uint8_t *bgrL, const uint8_t *bgrR, int W, int H, int D,
                     int right_view, uint8_t *vol, float *dwta) {
     if (W < 9 || H < 7 || D < 1 || D > 1024 || D > W) return -1;

@@ -42,6 +42,8 @@ def _load():
         "orc_lab": (None, [_u8p, C.c_int, C.c_int, _f32p]),
         "orc_nn_grid": (None, [_f32p, C.c_int, C.c_int, C.c_int, _i32p, _f32p, _i32p]),
         "orc_cc_max": (None, [_i32p, C.c_int, _i32p]),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_get_threads": (C.c_int, []),
         "orc_crtrees_literal": (None, [_i32p, C.c_int, C.c_int, _i32p]),
         "orc_compact": (C.c_int, [_i32p, C.c_int]),
         "orc_cost_volume_ad": (C.c_int, [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, _f32p]),
