@@ -183,6 +183,28 @@ def main():
     pair_stats = ctx.profile_stats()
     ctx.profile(False)
 
+    # the cost-volume kernels: census, the WTA pass of the plain path, and the u8 volume a caller
+    # of the volume getters gets (written on first request; BP reads only census words and WTA)
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(3):
+        vol = C.c_void_p()
+        H._check(lib.hbp_cost_volume_dev(ctx.h, d_l, d_r, W, Hh, D, b"Census", C.byref(vol)))
+        if not lib.hbp_volume_dev_u8(vol):
+            raise H.HbpError(lib.hbp_last_error().decode())
+        lib.hbp_volume_free(ctx.h, vol)
+    ctx.sync()
+    vol_stats = ctx.profile_stats()
+    ctx.profile(False)
+    cost_kernels = {}
+    for name in ("census", "hamming_wta", "hamming_volume"):
+        v = vol_stats.get(name)
+        if v and v["ms"] > 0:
+            gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+            cost_kernels[name] = dict(us=round(1e3 * v["ms"] / v["launches"], 1), gbs=round(gbs, 1),
+                                      frac=round(gbs / HBM_PEAK_GBS, 4),
+                                      bytes_per_launch=round(v["bytes"] / v["launches"]))
+
     pairs_per_s = args.steps / elapsed
     mpix_disp = W * Hh * D / 1e6 * pairs_per_s
     lat_ms = sorted(1e3 * x for x in lat)
@@ -257,6 +279,7 @@ def main():
                      "avg_launch_us": round(1e3 * avg_ms, 2), "launches_timed": gd["launches"],
                      "event_sampling": f"1 in {EVENT_EVERY} launches in the timed region"},
         "kernels_one_pair": kernels,
+        "cost_volume_kernels": cost_kernels,
         "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
                                for n, v in sorted(pair_stats.items()) if "_L" in n},
         "tuning": dict(tuning, final=ctx.get_tuning()),

@@ -380,7 +380,19 @@ int hbp_volume_info(const hbp_volume *vol, int *W, int *H, int *D, float *tau, i
 }
 
 float *hbp_volume_dev_wta(hbp_volume *vol) { return vol ? vol->v.dwta.p : nullptr; }
-const uint8_t *hbp_volume_dev_u8(const hbp_volume *vol) { return vol ? vol->v.vol.p : nullptr; }
+const uint8_t *hbp_volume_dev_u8(const hbp_volume *vol) {
+    try {
+        if (!vol || !vol->owner || vol->v.f32src) return nullptr;
+        hbp_ctx *ctx = vol->owner;
+        use_device(ctx);
+        const uint8_t *p = volume_u8(ctx->c, const_cast<hbp_volume *>(vol)->v);
+        HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
+        return p;
+    } catch (const HipError &e) {
+        set_error(e.msg);
+        return nullptr;
+    }
+}
 float *hbp_volume_dev_f32(hbp_ctx *ctx, hbp_volume *vol) {
     try {
         if (!ctx || !vol) return nullptr;
@@ -398,11 +410,11 @@ int hbp_volume_download(hbp_ctx *ctx, const hbp_volume *vol, uint8_t *vh, float 
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && vol, "NULL argument");
     use_device(ctx);
-    const Volume &v = vol->v;
+    Volume &v = const_cast<hbp_volume *>(vol)->v;
     HBP_REQUIRE(!(vh && v.f32src), "an f32 (MC-CNN) volume has no u8 form: use hbp_volume_download_f32");
     if (vh)
-        HBP_CHECK(hipMemcpy2DAsync(vh, (size_t)v.W, v.vol.p, (size_t)v.pitch, (size_t)v.W, (size_t)v.H * v.D,
-                                   hipMemcpyDeviceToHost, ctx->c.stream));
+        HBP_CHECK(hipMemcpy2DAsync(vh, (size_t)v.W, volume_u8(ctx->c, v), (size_t)v.pitch, (size_t)v.W,
+                                   (size_t)v.H * v.D, hipMemcpyDeviceToHost, ctx->c.stream));
     if (wh)
         HBP_CHECK(hipMemcpyAsync(wh, v.dwta.p, sizeof(float) * (size_t)v.W * v.H, hipMemcpyDeviceToHost,
                                  ctx->c.stream));

@@ -74,82 +74,112 @@ __global__ __launch_bounds__(256) void census_kernel(const uint8_t *__restrict__
 // popc(CL(x) ^ CR(max(x-d,0))); cost_u1_to_f1_transpose (209-228): min(cost, tau), [D][H][W];
 // WTA_D_k (248-266): argmin_d, smallest d on ties.  RIGHT = cost_to_right
 // (stereo_cost.cu:488-516): C_R[d][y][x] = C_L[d][y][min(x+d, W-1)].
-// Block: 128 threads x 8 columns = 1024 columns of one row; the right census (and, for the
-// right view, the left census) of the row segment plus a D-wide halo sit in LDS.  For every d a
-// wave writes 512 contiguous bytes of the disparity plane.
+// One 64-lane workgroup per HS = 256 columns of one row, lane l on columns 4l .. 4l+3.  The
+// census row segment with its D-wide halo (left view: CR(x - d); right view: CL(x + d) and the
+// words CR(W-1-d) of the clamped columns) is staged in LDS once; disparities then go in blocks
+// of HC = 4, the block's census words in a window of 2 HC - 1 registers read from LDS, so each
+// cost is an xor, a popcount and a min.  The WTA keeps min(cost << 10 | d) per column in u16
+// halves (cost <= 25, d < 1024: 15 bits), columns (0, 2) and (1, 3) paired, so one packed min
+// serves two columns.  STORE: also write the four costs of each d as one dword of the u8 volume
+// (a wavefront writes 256 contiguous bytes of a disparity plane); without it only the WTA.
 // ----------------------------------------------------------------------------------------
-constexpr int HM_THREADS = 128, HM_XT = 8, HM_SEG = HM_THREADS * HM_XT;
+constexpr int HC = 4, HS = 64 * HC;
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-template <bool RIGHT>
-__global__ __launch_bounds__(HM_THREADS) void hamming_wta_kernel(const uint32_t *__restrict__ cl,
-                                                                 const uint32_t *__restrict__ cr, int W, int H,
-                                                                 int D, int pitch, long plane,
-                                                                 uint8_t *__restrict__ vol,
-                                                                 float *__restrict__ dwta) {
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+    u16x2 x = __builtin_bit_cast(u16x2, a), y = __builtin_bit_cast(u16x2, b);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(x, y));
+}
+
+// min(popcount(a ^ b), 25) in unsigned arithmetic
+__device__ __forceinline__ uint32_t ham25(uint32_t a, uint32_t b) {
+    const uint32_t p = (uint32_t)__popc(a ^ b);
+    return p < 25u ? p : 25u;
+}
+
+template <bool RIGHT, bool STORE>
+__global__ __launch_bounds__(64) void hamming_wta_kernel(const uint32_t *__restrict__ cl,
+                                                         const uint32_t *__restrict__ cr, int W, int H, int D,
+                                                         int pitch, long plane, uint8_t *__restrict__ vol,
+                                                         float *__restrict__ dwta) {
     extern __shared__ uint32_t lds[];
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * HM_SEG;
+    const int y = blockIdx.y, seg0 = blockIdx.x * HS, lane = threadIdx.x;
     const uint32_t *clr = cl + (long)y * W;
     const uint32_t *crr = cr + (long)y * W;
-    // right-census window: LDS[i] = CR(clamp(x0 - D + 1 + i, 0, W-1)), i < HM_SEG + D - 1
-    const int base = x0 - D + 1;
-    const int nwin = HM_SEG + D - 1;
-    uint32_t *crw = lds;
-    uint32_t *clw = lds + nwin;  // RIGHT only: LDS[i] = CL(clamp(x0 + i, 0, W-1)), i < HM_SEG + D - 1
-    for (int i = threadIdx.x; i < nwin; i += HM_THREADS) {
-        int xi = base + i;
-        xi = xi < 0 ? 0 : (xi > W - 1 ? W - 1 : xi);
-        crw[i] = crr[xi];
-        if (RIGHT) {
-            int xl = x0 + i;
-            xl = xl > W - 1 ? W - 1 : xl;
-            clw[i] = clr[xl];
+    const int nwin = HS + D + 6;
+    // left: lds[i] = CR(clamp(seg0 - D - 3 + i, 0, W-1)); right: lds[i] = CL(min(seg0 + i, W-1)),
+    // then edge[d] = CR(max(W-1-d, 0))
+    uint32_t *edge = lds + nwin;
+    for (int i = lane; i < nwin; i += 64)
+        lds[i] = RIGHT ? clr[min(seg0 + i, W - 1)] : crr[min(max(seg0 - D - 3 + i, 0), W - 1)];
+    if (RIGHT)
+        for (int d = lane; d < D; d += 64) edge[d] = crr[max(W - 1 - d, 0)];
+    __syncthreads();
+    const int x0 = seg0 + lane * HC;
+    if (x0 >= W) return;   // padding columns are never read
+    uint32_t fix[HC];      // left view: CL(x); right view: CR(x)
+#pragma unroll
+    for (int j = 0; j < HC; ++j) fix[j] = (RIGHT ? crr : clr)[min(x0 + j, W - 1)];
+    const uint32_t cl_last = RIGHT ? clr[W - 1] : 0u;
+    uint32_t best[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};   // u16 keys of columns (0, 2) and (1, 3)
+    uint8_t *out = vol + (long)y * pitch + x0;
+    const uint32_t *win_base = lds + lane * HC + (RIGHT ? 0 : D);
+    int d0 = 0;
+    for (; d0 + HC <= D; d0 += HC) {
+        uint32_t win[2 * HC - 1];
+#pragma unroll
+        for (int m = 0; m < 2 * HC - 1; ++m) win[m] = RIGHT ? win_base[d0 + m] : win_base[m - d0];
+#pragma unroll
+        for (int dd = 0; dd < HC; ++dd) {
+            const int d = d0 + dd;
+            uint32_t c[HC];
+#pragma unroll
+            for (int j = 0; j < HC; ++j) {
+                if (!RIGHT) {
+                    c[j] = ham25(fix[j], win[j - dd + HC - 1]);   // CR(x + j - d)
+                } else {
+                    c[j] = ham25(win[j + dd], fix[j]);            // CL(x + j + d)
+                }
+            }
+            if (RIGHT && x0 + HC - 1 + d > W - 1) {
+                // columns with x + d > W - 1 take the entry of column W - 1 - d
+                const uint32_t ce = ham25(cl_last, edge[d]);
+#pragma unroll
+                for (int j = 0; j < HC; ++j) c[j] = x0 + j + d > W - 1 ? ce : c[j];
+            }
+            const uint32_t dp = (uint32_t)d | ((uint32_t)d << 16);
+            const uint32_t t02 = c[0] | (c[2] << 16), t13 = c[1] | (c[3] << 16);
+            best[0] = pk_min_u16(best[0], (t02 << 10) | dp);
+            best[1] = pk_min_u16(best[1], (t13 << 10) | dp);
+            if (STORE) *(uint32_t *)(out + (long)d * plane) = t02 | (t13 << 8);
         }
     }
-    __syncthreads();
-    const int xs = x0 + threadIdx.x * HM_XT;
-    if (xs >= pitch) return;
-    uint32_t c_l[HM_XT];
-    int best[HM_XT], arg[HM_XT];
+    for (int d = d0; d < D; ++d) {   // D not a multiple of HC: the last disparities one by one
+        uint32_t c[HC];
 #pragma unroll
-    for (int j = 0; j < HM_XT; ++j) {
-        int x = xs + j;
-        c_l[j] = clr[x < W ? x : W - 1];
-        best[j] = 1 << 30;
-        arg[j] = 0;
-    }
-    uint8_t *out = vol + (long)y * pitch + xs;
-    for (int d = 0; d < D; ++d) {
-        uint32_t packed[2] = {0u, 0u};
-#pragma unroll
-        for (int j = 0; j < HM_XT; ++j) {
-            int x = xs + j;
+        for (int j = 0; j < HC; ++j) {
+            const int x = x0 + j;
             uint32_t a, b;
             if (!RIGHT) {
-                a = c_l[j];
-                b = crw[x - d - base];  // x - d >= base always; clamped to CR(0) below 0
+                a = fix[j];
+                b = win_base[j - d + HC - 1];                // CR(x - d)
             } else {
-                int xl = x + d;
-                xl = xl > W - 1 ? W - 1 : xl;
-                a = clw[xl - x0 < nwin ? xl - x0 : nwin - 1];
-                int xr = xl - d;
-                b = crw[xr - base];
+                const bool clamped = x + d > W - 1;
+                a = clamped ? cl_last : win_base[j + d];      // CL(min(x + d, W - 1))
+                b = clamped ? edge[d] : fix[j];               // CR(max(min(x + d, W - 1) - d, 0))
             }
-            int c = __popc(a ^ b);
-            c = c < 25 ? c : 25;
-            if (c < best[j]) {
-                best[j] = c;
-                arg[j] = d;
-            }
-            packed[j >> 2] |= (uint32_t)c << (8 * (j & 3));
+            c[j] = ham25(a, b);
         }
-        *(uint2 *)(out + (long)d * plane) = make_uint2(packed[0], packed[1]);
+        const uint32_t dp = (uint32_t)d | ((uint32_t)d << 16);
+        const uint32_t t02 = c[0] | (c[2] << 16), t13 = c[1] | (c[3] << 16);
+        best[0] = pk_min_u16(best[0], (t02 << 10) | dp);
+        best[1] = pk_min_u16(best[1], (t13 << 10) | dp);
+        if (STORE) *(uint32_t *)(out + (long)d * plane) = t02 | (t13 << 8);
     }
+    const uint32_t arg[HC] = {best[0] & 1023u, best[1] & 1023u, (best[0] >> 16) & 1023u, (best[1] >> 16) & 1023u};
 #pragma unroll
-    for (int j = 0; j < HM_XT; ++j) {
-        int x = xs + j;
-        if (x < W) dwta[(long)y * W + x] = (float)arg[j];
-    }
+    for (int j = 0; j < HC; ++j)
+        if (x0 + j < W) dwta[(long)y * W + x0 + j] = (float)arg[j];
 }
 
 // ----------------------------------------------------------------------------------------
@@ -250,19 +280,39 @@ void census_launch(Ctx &c, const uint8_t *bgrL, const uint8_t *bgrR, int W, int 
     HBP_CHECK(hipGetLastError());
 }
 
-void hamming_launch(Ctx &c, Volume &v, bool right) {
-    dim3 grid(div_up(v.pitch, HM_SEG), v.H);
-    int nwin = HM_SEG + v.D - 1;
-    size_t lds = sizeof(uint32_t) * (size_t)nwin * (right ? 2 : 1);
-    // algorithmic bytes: census words in (8N), the u8 volume out (D*N), WTA out (4N)
-    ProfScope ps(c, right ? "hamming_wta_right" : "hamming_wta", (double)(8 + v.D + 4) * v.W * v.H);
-    if (right)
-        hipLaunchKernelGGL(hamming_wta_kernel<true>, grid, dim3(HM_THREADS), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
-                           v.D, v.pitch, v.plane, v.vol.p, v.dwta.p);
+// store: also write the u8 volume (else the WTA alone)
+static void hamming_launch(Ctx &c, Volume &v, bool right, bool store) {
+    dim3 grid((unsigned)div_up(v.W, HS), (unsigned)v.H);
+    const size_t lds = sizeof(uint32_t) * ((size_t)HS + v.D + 6 + (right ? v.D : 0));
+    // algorithmic bytes: census words in (8N), the u8 volume out (D*N) when stored, WTA out (4N)
+    const double n = (double)v.W * v.H;
+    ProfScope ps(c, std::string(store ? "hamming_volume" : "hamming_wta") + (right ? "_right" : ""),
+                 (8.0 + (store ? v.D : 0) + 4.0) * n);
+    uint8_t *out = store ? v.vol.p : nullptr;
+    if (right && store)
+        hipLaunchKernelGGL((hamming_wta_kernel<true, true>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
+                           v.D, v.pitch, v.plane, out, v.dwta.p);
+    else if (right)
+        hipLaunchKernelGGL((hamming_wta_kernel<true, false>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
+                           v.D, v.pitch, v.plane, out, v.dwta.p);
+    else if (store)
+        hipLaunchKernelGGL((hamming_wta_kernel<false, true>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
+                           v.D, v.pitch, v.plane, out, v.dwta.p);
     else
-        hipLaunchKernelGGL(hamming_wta_kernel<false>, grid, dim3(HM_THREADS), lds, c.stream, v.cl.p, v.cr.p, v.W,
-                           v.H, v.D, v.pitch, v.plane, v.vol.p, v.dwta.p);
+        hipLaunchKernelGGL((hamming_wta_kernel<false, false>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W,
+                           v.H, v.D, v.pitch, v.plane, out, v.dwta.p);
     HBP_CHECK(hipGetLastError());
+}
+
+// the u8 volume of a census cost, written on first request: BP reads only the census words and
+// the WTA, so the plain, batched and served paths never need it
+uint8_t *volume_u8(Ctx &c, Volume &v) {
+    HBP_REQUIRE(!v.f32src, "an f32 (AD or MC-CNN) volume has no u8 form");
+    if (!v.vol.p) {
+        v.vol.alloc(c.pool, (size_t)v.plane * v.D);
+        hamming_launch(c, v, v.right, true);
+    }
+    return v.vol.p;
 }
 
 void volume_build(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D) {
@@ -281,10 +331,10 @@ void volume_build(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgr
     long n = (long)W * H;
     v.cl.alloc(c.pool, n);
     v.cr.alloc(c.pool, n);
-    v.vol.alloc(c.pool, (size_t)v.plane * D);
+    v.vol.release();
     v.dwta.alloc(c.pool, n);
     census_launch(c, d_bgrL, d_bgrR, W, H, v.cl.p, v.cr.p);
-    hamming_launch(c, v, false);
+    hamming_launch(c, v, false, false);
 }
 
 static int grid_of(const Ctx &c, long n) {
@@ -361,16 +411,18 @@ void volume_to_right(Ctx &c, Volume &v) {
         v.right = true;
         return;
     }
-    hamming_launch(c, v, true);
+    hamming_launch(c, v, true, false);
     v.right = true;
+    v.vol.release();
     v.f32.release();
 }
 
 float *volume_f32(Ctx &c, Volume &v) {
     if (!v.f32.p) {
+        const uint8_t *u8 = volume_u8(c, v);
         v.f32.alloc(c.pool, (size_t)v.W * v.H * v.D);
         int blocks = c.num_cu * 8;
-        hipLaunchKernelGGL(volume_to_f32_kernel, dim3(blocks), dim3(256), 0, c.stream, v.vol.p, v.W, v.H, v.D,
+        hipLaunchKernelGGL(volume_to_f32_kernel, dim3(blocks), dim3(256), 0, c.stream, u8, v.W, v.H, v.D,
                            v.pitch, v.plane, v.f32.p);
         HBP_CHECK(hipGetLastError());
     }

@@ -13,7 +13,7 @@ struct Volume {
     bool f32src = false;     // an f32 volume (MC-CNN given, or AD): f32 holds it, there is no u8 volume
     float absmax = 0.f;      // f32src: the largest |entry| after truncation (bounds the plane costs)
     DevBuf<uint32_t> cl, cr; // census words of the left / right image [H][W]
-    DevBuf<uint8_t> vol;     // u8 [D][H][pitch]
+    DevBuf<uint8_t> vol;     // u8 [D][H][pitch], written on first request (volume_u8)
     DevBuf<float> dwta;      // WTA disparity [H][W]
     DevBuf<float> f32;       // f32 [D][H][W]: the given volume (f32src), else made for the float* getter
 };
@@ -98,6 +98,7 @@ void volume_build_f32(Ctx &c, Volume &v, const float *d_raw, int W, int H, int D
 void volume_build_ad(Ctx &c, Volume &v, const uint8_t *d_bgrL, const uint8_t *d_bgrR, int W, int H, int D);
 void volume_to_right(Ctx &c, Volume &v);
 float *volume_f32(Ctx &c, Volume &v);
+uint8_t *volume_u8(Ctx &c, Volume &v);
 
 // pyramid.hip
 void pyramid_build(Ctx &c, Pyramid &p, const uint8_t *d_bgr, int W, int H, int num_nb, int link = kCentroidLink,
