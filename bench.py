@@ -139,6 +139,8 @@ def main():
         # coarse levels of this pair all hold thousands of pixels per superpixel)
         from hbp_amd import tune
         tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2)})
+        tuning["levels"] = tune.tune_levels(ctx, left, right, D, info["min_level"])
+        tuning["hamming"] = tune.tune_hamming(ctx, left, right, D)
     for _ in range(args.warmup):
         one_pair()
     ctx.sync()
@@ -282,7 +284,8 @@ def main():
         "cost_volume_kernels": cost_kernels,
         "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
                                for n, v in sorted(pair_stats.items()) if "_L" in n},
-        "tuning": dict(tuning, final=ctx.get_tuning()),
+        "tuning": dict(tuning, final=dict(ctx.get_tuning(), **{
+            f"level_form.{l}": ctx.get_tuning(f"level_form.{l}") for l in range(info["min_level"], info["levels"] - 1)})),
         "batched_vs_plain": batched,
         "served_vs_plain": served,
         "post_process": post,

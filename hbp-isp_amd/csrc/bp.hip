@@ -1501,7 +1501,9 @@ void bp_setup(Ctx &c, BP &b) {
             hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, keys.p, N, V,
                                B.pix_off.p);
             // coarse levels: segments of <= SEG pixels per superpixel
-            B.coarse = N >= (long)c.coarse_min_pix * V;
+            // (a per-level choice of the context overrides the threshold)
+            const int form = l < Ctx::kFormLevels ? c.level_form[l] : -1;
+            B.coarse = form < 0 ? N >= (long)c.coarse_min_pix * V : form == 1;
             if (B.coarse) {
                 DevBuf<int> scnt(c.pool, (size_t)V + 1);
                 B.sego.alloc(c.pool, (size_t)V + 1);

@@ -206,6 +206,16 @@ int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every) {
     HBP_API_END
 }
 
+// "level_form.<l>", l in [0, kFormLevels)
+static int level_form_index(const std::string &n) {
+    const std::string t = n.substr(11);
+    HBP_REQUIRE(!t.empty() && t.size() <= 2 && t.find_first_not_of("0123456789") == std::string::npos,
+                "level forms are named level_form.<level>");
+    const int l = std::stoi(t);
+    HBP_REQUIRE(l >= 0 && l < Ctx::kFormLevels, "level form index out of range");
+    return l;
+}
+
 int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx && name, "NULL argument");
@@ -216,8 +226,15 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
     } else if (n == "single_hb" || n == "batch_hb") {
         HBP_REQUIRE(value == 2 || value == 4, "steps per batch must be 2 or 4");
         (n == "single_hb" ? ctx->c.single_hb : ctx->c.batch_hb) = value;
+    } else if (n == "hamming_cols") {
+        HBP_REQUIRE(value == 4 || value == 8, "hamming_cols must be 4 or 8");
+        ctx->c.hamming_cols = value;
+    } else if (n.rfind("level_form.", 0) == 0) {
+        const int l = level_form_index(n);
+        HBP_REQUIRE(value >= -1 && value <= 1, "a level form is -1 (by coarse_min_pix), 0 (chunks) or 1 (segments)");
+        ctx->c.level_form[l] = value;
     } else {
-        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb)"};
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, level_form.<l>)"};
     }
     HBP_API_END
 }
@@ -229,7 +246,9 @@ int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
     if (n == "coarse_min_pix") *value = ctx->c.coarse_min_pix;
     else if (n == "single_hb") *value = ctx->c.single_hb;
     else if (n == "batch_hb") *value = ctx->c.batch_hb;
-    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb)"};
+    else if (n == "hamming_cols") *value = ctx->c.hamming_cols;
+    else if (n.rfind("level_form.", 0) == 0) *value = ctx->c.level_form[level_form_index(n)];
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, level_form.<l>)"};
     HBP_API_END
 }
 

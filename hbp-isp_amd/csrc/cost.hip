@@ -74,16 +74,17 @@ __global__ __launch_bounds__(256) void census_kernel(const uint8_t *__restrict__
 // popc(CL(x) ^ CR(max(x-d,0))); cost_u1_to_f1_transpose (209-228): min(cost, tau), [D][H][W];
 // WTA_D_k (248-266): argmin_d, smallest d on ties.  RIGHT = cost_to_right
 // (stereo_cost.cu:488-516): C_R[d][y][x] = C_L[d][y][min(x+d, W-1)].
-// One 64-lane workgroup per HS = 256 columns of one row, lane l on columns 4l .. 4l+3.  The
-// census row segment with its D-wide halo (left view: CR(x - d); right view: CL(x + d) and the
-// words CR(W-1-d) of the clamped columns) is staged in LDS once; disparities then go in blocks
-// of HC = 4, the block's census words in a window of 2 HC - 1 registers read from LDS, so each
-// cost is an xor, a popcount and a min.  The WTA keeps min(cost << 10 | d) per column in u16
-// halves (cost <= 25, d < 1024: 15 bits), columns (0, 2) and (1, 3) paired, so one packed min
-// serves two columns.  STORE: also write the four costs of each d as one dword of the u8 volume
-// (a wavefront writes 256 contiguous bytes of a disparity plane); without it only the WTA.
+// One 64-lane workgroup per 64 HC columns of one row, lane l on columns HC l .. HC l + HC - 1
+// (HC = 4 or 8, a tuning choice).  The census row segment with its D-wide halo (left view:
+// CR(x - d); right view: CL(x + d) and the words CR(W-1-d) of the clamped columns) is staged in
+// LDS once; disparities then go in blocks of DB = 4, the block's census words in a window of
+// HC + DB - 1 registers read from LDS, so each cost is an xor, a popcount and a min.  The WTA
+// keeps min(cost << 10 | d) per column in u16 halves (cost <= 25, d < 1024: 15 bits), columns
+// (4i, 4i+2) and (4i+1, 4i+3) paired, so one packed min serves two columns.  STORE: also write the
+// HC costs of each d (one dword per 4 columns) into the u8 volume (a wavefront writes 64 HC
+// contiguous bytes of a disparity plane); without it only the WTA.
 // ----------------------------------------------------------------------------------------
-constexpr int HC = 4, HS = 64 * HC;
+constexpr int DB = 4;
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
@@ -97,21 +98,42 @@ __device__ __forceinline__ uint32_t ham25(uint32_t a, uint32_t b) {
     return p < 25u ? p : 25u;
 }
 
-template <bool RIGHT, bool STORE>
+// the costs c[HC] of one disparity d: WTA keys, and the volume's dwords when storing
+template <int HC, bool STORE>
+__device__ __forceinline__ void ham_emit(const uint32_t (&c)[HC], int d, uint32_t (&best)[HC / 2], uint8_t *out,
+                                         long plane) {
+    const uint32_t dp = (uint32_t)d | ((uint32_t)d << 16);
+    uint32_t w[HC / 4];
+#pragma unroll
+    for (int q = 0; q < HC / 4; ++q) {
+        const uint32_t t02 = c[4 * q] | (c[4 * q + 2] << 16), t13 = c[4 * q + 1] | (c[4 * q + 3] << 16);
+        best[2 * q] = pk_min_u16(best[2 * q], (t02 << 10) | dp);
+        best[2 * q + 1] = pk_min_u16(best[2 * q + 1], (t13 << 10) | dp);
+        w[q] = t02 | (t13 << 8);
+    }
+    if (STORE) {
+        if (HC == 4) *(uint32_t *)(out + (long)d * plane) = w[0];
+        else *(uint2 *)(out + (long)d * plane) = make_uint2(w[0], w[HC / 4 - 1]);
+    }
+}
+
+template <int HC, bool RIGHT, bool STORE>
 __global__ __launch_bounds__(64) void hamming_wta_kernel(const uint32_t *__restrict__ cl,
                                                          const uint32_t *__restrict__ cr, int W, int H, int D,
                                                          int pitch, long plane, uint8_t *__restrict__ vol,
                                                          float *__restrict__ dwta) {
+    static_assert(HC == 4 || HC == 8, "4 or 8 columns per lane");
+    constexpr int HS = 64 * HC;
     extern __shared__ uint32_t lds[];
     const int y = blockIdx.y, seg0 = blockIdx.x * HS, lane = threadIdx.x;
     const uint32_t *clr = cl + (long)y * W;
     const uint32_t *crr = cr + (long)y * W;
-    const int nwin = HS + D + 6;
-    // left: lds[i] = CR(clamp(seg0 - D - 3 + i, 0, W-1)); right: lds[i] = CL(min(seg0 + i, W-1)),
-    // then edge[d] = CR(max(W-1-d, 0))
+    const int nwin = HS + D + DB - 1;
+    // left: lds[i] = CR(clamp(seg0 - D - (DB-1) + i, 0, W-1)); right: lds[i] = CL(min(seg0 + i,
+    // W-1)), then edge[d] = CR(max(W-1-d, 0))
     uint32_t *edge = lds + nwin;
     for (int i = lane; i < nwin; i += 64)
-        lds[i] = RIGHT ? clr[min(seg0 + i, W - 1)] : crr[min(max(seg0 - D - 3 + i, 0), W - 1)];
+        lds[i] = RIGHT ? clr[min(seg0 + i, W - 1)] : crr[min(max(seg0 - D - (DB - 1) + i, 0), W - 1)];
     if (RIGHT)
         for (int d = lane; d < D; d += 64) edge[d] = crr[max(W - 1 - d, 0)];
     __syncthreads();
@@ -121,40 +143,34 @@ __global__ __launch_bounds__(64) void hamming_wta_kernel(const uint32_t *__restr
 #pragma unroll
     for (int j = 0; j < HC; ++j) fix[j] = (RIGHT ? crr : clr)[min(x0 + j, W - 1)];
     const uint32_t cl_last = RIGHT ? clr[W - 1] : 0u;
-    uint32_t best[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};   // u16 keys of columns (0, 2) and (1, 3)
+    uint32_t best[HC / 2];   // u16 keys of columns (4q, 4q+2) and (4q+1, 4q+3)
+#pragma unroll
+    for (int q = 0; q < HC / 2; ++q) best[q] = 0xFFFFFFFFu;
     uint8_t *out = vol + (long)y * pitch + x0;
+    // left: CR(x0 + j - d) = win_base[j - d + DB - 1]; right: CL(x0 + j + d) = win_base[j + d]
     const uint32_t *win_base = lds + lane * HC + (RIGHT ? 0 : D);
     int d0 = 0;
-    for (; d0 + HC <= D; d0 += HC) {
-        uint32_t win[2 * HC - 1];
+    for (; d0 + DB <= D; d0 += DB) {
+        uint32_t win[HC + DB - 1];
 #pragma unroll
-        for (int m = 0; m < 2 * HC - 1; ++m) win[m] = RIGHT ? win_base[d0 + m] : win_base[m - d0];
+        for (int m = 0; m < HC + DB - 1; ++m) win[m] = RIGHT ? win_base[d0 + m] : win_base[m - d0];
 #pragma unroll
-        for (int dd = 0; dd < HC; ++dd) {
+        for (int dd = 0; dd < DB; ++dd) {
             const int d = d0 + dd;
             uint32_t c[HC];
 #pragma unroll
-            for (int j = 0; j < HC; ++j) {
-                if (!RIGHT) {
-                    c[j] = ham25(fix[j], win[j - dd + HC - 1]);   // CR(x + j - d)
-                } else {
-                    c[j] = ham25(win[j + dd], fix[j]);            // CL(x + j + d)
-                }
-            }
+            for (int j = 0; j < HC; ++j)
+                c[j] = RIGHT ? ham25(win[j + dd], fix[j]) : ham25(fix[j], win[j - dd + DB - 1]);
             if (RIGHT && x0 + HC - 1 + d > W - 1) {
                 // columns with x + d > W - 1 take the entry of column W - 1 - d
                 const uint32_t ce = ham25(cl_last, edge[d]);
 #pragma unroll
                 for (int j = 0; j < HC; ++j) c[j] = x0 + j + d > W - 1 ? ce : c[j];
             }
-            const uint32_t dp = (uint32_t)d | ((uint32_t)d << 16);
-            const uint32_t t02 = c[0] | (c[2] << 16), t13 = c[1] | (c[3] << 16);
-            best[0] = pk_min_u16(best[0], (t02 << 10) | dp);
-            best[1] = pk_min_u16(best[1], (t13 << 10) | dp);
-            if (STORE) *(uint32_t *)(out + (long)d * plane) = t02 | (t13 << 8);
+            ham_emit<HC, STORE>(c, d, best, out, plane);
         }
     }
-    for (int d = d0; d < D; ++d) {   // D not a multiple of HC: the last disparities one by one
+    for (int d = d0; d < D; ++d) {   // D not a multiple of DB: the last disparities one by one
         uint32_t c[HC];
 #pragma unroll
         for (int j = 0; j < HC; ++j) {
@@ -162,7 +178,7 @@ __global__ __launch_bounds__(64) void hamming_wta_kernel(const uint32_t *__restr
             uint32_t a, b;
             if (!RIGHT) {
                 a = fix[j];
-                b = win_base[j - d + HC - 1];                // CR(x - d)
+                b = win_base[j - d + DB - 1];                 // CR(x - d)
             } else {
                 const bool clamped = x + d > W - 1;
                 a = clamped ? cl_last : win_base[j + d];      // CL(min(x + d, W - 1))
@@ -170,16 +186,16 @@ __global__ __launch_bounds__(64) void hamming_wta_kernel(const uint32_t *__restr
             }
             c[j] = ham25(a, b);
         }
-        const uint32_t dp = (uint32_t)d | ((uint32_t)d << 16);
-        const uint32_t t02 = c[0] | (c[2] << 16), t13 = c[1] | (c[3] << 16);
-        best[0] = pk_min_u16(best[0], (t02 << 10) | dp);
-        best[1] = pk_min_u16(best[1], (t13 << 10) | dp);
-        if (STORE) *(uint32_t *)(out + (long)d * plane) = t02 | (t13 << 8);
+        ham_emit<HC, STORE>(c, d, best, out, plane);
     }
-    const uint32_t arg[HC] = {best[0] & 1023u, best[1] & 1023u, (best[0] >> 16) & 1023u, (best[1] >> 16) & 1023u};
 #pragma unroll
-    for (int j = 0; j < HC; ++j)
-        if (x0 + j < W) dwta[(long)y * W + x0 + j] = (float)arg[j];
+    for (int q = 0; q < HC / 4; ++q) {
+        const uint32_t arg[4] = {best[2 * q] & 1023u, best[2 * q + 1] & 1023u, (best[2 * q] >> 16) & 1023u,
+                                 (best[2 * q + 1] >> 16) & 1023u};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (x0 + 4 * q + k < W) dwta[(long)y * W + x0 + 4 * q + k] = (float)arg[k];
+    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -281,27 +297,35 @@ void census_launch(Ctx &c, const uint8_t *bgrL, const uint8_t *bgrR, int W, int 
 }
 
 // store: also write the u8 volume (else the WTA alone)
+template <int HC>
+static void hamming_launch_hc(Ctx &c, Volume &v, bool right, bool store) {
+    dim3 grid((unsigned)div_up(v.W, 64 * HC), (unsigned)v.H);
+    const size_t lds = sizeof(uint32_t) * ((size_t)64 * HC + v.D + DB - 1 + (right ? v.D : 0));
+    uint8_t *out = store ? v.vol.p : nullptr;
+    if (right && store)
+        hipLaunchKernelGGL((hamming_wta_kernel<HC, true, true>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W,
+                           v.H, v.D, v.pitch, v.plane, out, v.dwta.p);
+    else if (right)
+        hipLaunchKernelGGL((hamming_wta_kernel<HC, true, false>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W,
+                           v.H, v.D, v.pitch, v.plane, out, v.dwta.p);
+    else if (store)
+        hipLaunchKernelGGL((hamming_wta_kernel<HC, false, true>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W,
+                           v.H, v.D, v.pitch, v.plane, out, v.dwta.p);
+    else
+        hipLaunchKernelGGL((hamming_wta_kernel<HC, false, false>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p,
+                           v.W, v.H, v.D, v.pitch, v.plane, out, v.dwta.p);
+    HBP_CHECK(hipGetLastError());
+}
+
+// store: also write the u8 volume (else the WTA alone); the context's hamming_cols picks the
+// columns per lane (same results either way)
 static void hamming_launch(Ctx &c, Volume &v, bool right, bool store) {
-    dim3 grid((unsigned)div_up(v.W, HS), (unsigned)v.H);
-    const size_t lds = sizeof(uint32_t) * ((size_t)HS + v.D + 6 + (right ? v.D : 0));
     // algorithmic bytes: census words in (8N), the u8 volume out (D*N) when stored, WTA out (4N)
     const double n = (double)v.W * v.H;
     ProfScope ps(c, std::string(store ? "hamming_volume" : "hamming_wta") + (right ? "_right" : ""),
                  (8.0 + (store ? v.D : 0) + 4.0) * n);
-    uint8_t *out = store ? v.vol.p : nullptr;
-    if (right && store)
-        hipLaunchKernelGGL((hamming_wta_kernel<true, true>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
-                           v.D, v.pitch, v.plane, out, v.dwta.p);
-    else if (right)
-        hipLaunchKernelGGL((hamming_wta_kernel<true, false>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
-                           v.D, v.pitch, v.plane, out, v.dwta.p);
-    else if (store)
-        hipLaunchKernelGGL((hamming_wta_kernel<false, true>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W, v.H,
-                           v.D, v.pitch, v.plane, out, v.dwta.p);
-    else
-        hipLaunchKernelGGL((hamming_wta_kernel<false, false>), grid, dim3(64), lds, c.stream, v.cl.p, v.cr.p, v.W,
-                           v.H, v.D, v.pitch, v.plane, out, v.dwta.p);
-    HBP_CHECK(hipGetLastError());
+    if (c.hamming_cols == 8) hamming_launch_hc<8>(c, v, right, store);
+    else hamming_launch_hc<4>(c, v, right, store);
 }
 
 // the u8 volume of a census cost, written on first request: BP reads only the census words and

@@ -344,6 +344,14 @@ struct Ctx {
     // single-pair launches (4: more gathers in flight for a latency-bound launch) and in
     // batched ones (2: fewer registers, more resident wavefronts for a throughput-bound one)
     int coarse_min_pix = 96, single_hb = 4, batch_hb = 2;
+    // columns per lane of the Hamming cost kernel (4 or 8; the wider form writes 512 instead of
+    // 256 contiguous bytes per wavefront store)
+    int hamming_cols = 4;
+    // per BP level: -1 the threshold above decides, 0 chunks of sorted pixels, 1 per-superpixel
+    // segments (a schedule of forms the tuner can set level by level; the bits are the same)
+    static constexpr int kFormLevels = 24;
+    int level_form[kFormLevels] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                                   -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // a stream whose queued work must finish before a cooperative launch on this context (the
     // server's admission context, whose work runs beside the BP rounds of the main stream:
     // a cooperative grid needs every one of its blocks resident at once)

@@ -217,7 +217,7 @@ class Ctx:
         _check(self.lib.hbp_ctx_sync(self.h))
 
     # ---- kernel-form tuning (results are the same for every value)
-    TUNING = ("coarse_min_pix", "single_hb", "batch_hb")
+    TUNING = ("coarse_min_pix", "single_hb", "batch_hb", "hamming_cols")
 
     def set_tuning(self, name, value):
         _check(self.lib.hbp_ctx_set_tuning(self.h, name.encode(), int(value)))
@@ -372,6 +372,12 @@ class Pyramid(_Handle):
     @property
     def levels(self):
         return self.ctx.lib.hbp_pyramid_levels(self.h)
+
+    def level_size(self, l):
+        """(nodes, directed edges) of level l"""
+        V, E = C.c_int(), C.c_int()
+        _check(self.ctx.lib.hbp_pyramid_level_size(self.h, l, C.byref(V), C.byref(E)))
+        return V.value, E.value
 
     def level(self, l):
         V, E = C.c_int(), C.c_int()

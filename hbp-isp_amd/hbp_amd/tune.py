@@ -1,12 +1,14 @@
-"""Kernel-form tuner: picks a context's coarse_min_pix, single_hb and batch_hb for a workload by
-timing it on the device.
+"""Kernel-form tuner: picks a context's kernel forms for a workload by timing it on the device.
 
 Each value is a choice between forms that give the same results bit for bit (the plane-cost
 cluster sums are exact integers in any order), so tuning only trades speed:
   * coarse_min_pix: average pixels per superpixel from which a BP level costs planes in the
     per-superpixel segment form instead of chunks of sorted pixels;
+  * level_form.<l>: that choice for one BP level, overriding coarse_min_pix (tune_levels sets a
+    schedule of forms level by level);
   * single_hb / batch_hb: steps per batch of a segment wavefront (2 or 4) in one-pair and in
-    batched launches: 4 keeps more gathers in flight, 2 fewer registers and more wavefronts.
+    batched launches: 4 keeps more gathers in flight, 2 fewer registers and more wavefronts;
+  * hamming_cols: columns per lane of the Hamming cost kernel (4 or 8).
 The reference's kernels have one form each; this module has no counterpart there.
 """
 import itertools
@@ -75,3 +77,51 @@ def tune_batch(ctx, pairs, min_level=2, grid=None, reps=1):
         ctx.bp_run_batch(bps)
 
     return _search(ctx, grid or BATCH_GRID, run, reps)
+
+
+def tune_levels(ctx, left, right, D, min_level=2, reps=3):
+    """A schedule of plane-cost forms, level by level, for one pair's BP: starting from the
+    current rule, each BP level in turn takes the other form if that makes the whole BP faster
+    (best of reps runs).  Leaves the schedule on ctx; returns it with the times."""
+    vol = ctx.cost_volume(left, right, D)
+    pyr = ctx.pyramid(left)
+    L = pyr.levels
+    ml = max(1, min(min_level, L - 1))
+
+    def run():
+        ctx.bp(pyr, vol, min_level=ml).run()
+
+    # the forms the current rule gives, read back from a BP of this pair: level l is coarse when
+    # its average superpixel holds at least coarse_min_pix pixels
+    N = pyr.W * pyr.H
+    cmp_ = ctx.get_tuning("coarse_min_pix")
+    levels = [l for l in range(ml, L - 1) if l < 24]
+    form = {l: int(N >= cmp_ * pyr.level_size(l)[0]) for l in levels}
+    for l in levels:
+        ctx.set_tuning(f"level_form.{l}", form[l])
+    base = _timed(ctx, run, reps)
+    trace = {"rule_ms": round(1e3 * base, 3)}
+    for l in levels:
+        ctx.set_tuning(f"level_form.{l}", 1 - form[l])
+        t = _timed(ctx, run, reps)
+        trace[f"L{l}:{'segments' if 1 - form[l] else 'chunks'}_ms"] = round(1e3 * t, 3)
+        if t < base:
+            base = t
+            form[l] = 1 - form[l]
+        else:
+            ctx.set_tuning(f"level_form.{l}", form[l])
+    return {"schedule": {f"L{l}": "segments" if form[l] else "chunks" for l in levels},
+            "ms": round(1e3 * base, 3), "trace": trace}
+
+
+def tune_hamming(ctx, left, right, D, reps=3):
+    """Columns per lane of the Hamming cost kernel: times the cost volume (census, WTA and the
+    u8 volume) for 4 and 8; leaves the faster on ctx."""
+    lib = ctx.lib
+
+    def run():
+        v = ctx.cost_volume(left, right, D)
+        lib.hbp_volume_dev_u8(v.h)
+        v.free()
+
+    return _search(ctx, {"hamming_cols": (4, 8)}, run, reps)

@@ -56,7 +56,10 @@ int hbp_profile_copy(hbp_ctx *ctx, long long bytes, int launches);
  *      "coarse_min_pix" (>= 1, default 96): average pixels per superpixel from which a BP level
  *      costs planes in the per-superpixel segment form rather than in chunks; "single_hb",
  *      "batch_hb" (2 or 4; defaults 4, 2): steps per batch of the segment wavefronts in one-pair
- *      and in batched launches.  Applies to BP objects created afterwards (and to the batched
+ *      and in batched launches; "hamming_cols" (4 or 8, default 4): columns per lane of the
+ *      Hamming cost kernel; "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
+ *      planes by coarse_min_pix's rule, in chunks of sorted pixels, or in per-superpixel
+ *      segments.  Applies to BP objects created afterwards (and to the batched
  *      rounds run afterwards).  hbp_amd.tune picks them by timing the caller's workload. */
 int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value);
 int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value);

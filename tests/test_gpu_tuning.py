@@ -31,6 +31,8 @@ def tuned(ctx):
     yield ctx
     for k, v in before.items():
         ctx.set_tuning(k, v)
+    for l in range(24):
+        ctx.set_tuning(f"level_form.{l}", -1)
 
 
 SETTINGS = [dict(coarse_min_pix=96, single_hb=4, batch_hb=2), dict(coarse_min_pix=1, single_hb=2, batch_hb=4),
@@ -94,7 +96,13 @@ def test_tuning_parameters_are_checked(tuned):
         tuned.set_tuning("batch_hb", 3)
     with pytest.raises(H.HbpError, match="coarse_min_pix"):
         tuned.set_tuning("coarse_min_pix", 0)
-    assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb"}
+    with pytest.raises(H.HbpError, match="4 or 8"):
+        tuned.set_tuning("hamming_cols", 6)
+    with pytest.raises(H.HbpError, match="level form"):
+        tuned.set_tuning("level_form.3", 2)
+    with pytest.raises(H.HbpError, match="out of range"):
+        tuned.set_tuning("level_form.24", 0)
+    assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb", "hamming_cols"}
 
 
 def test_tuner_sets_the_fastest_grid_point(tuned):
@@ -105,3 +113,52 @@ def test_tuner_sets_the_fastest_grid_point(tuned):
     pairs = [synth.make_pair(67 + i, 320, 240, 32, (3, 8))[:2] + (32,) for i in range(2)]
     rb = tune.tune_batch(tuned, pairs, 2, grid={"batch_hb": (2, 4)}, reps=1)
     assert tuned.get_tuning("batch_hb") == rb["best"]["batch_hb"] and len(rb["ms"]) == 2
+
+
+def test_per_level_form_schedules_give_the_same_bits(tuned):
+    """Any schedule of plane-cost forms over the BP levels (one pair and batched) gives the bits
+    of the default rule."""
+    rng = np.random.default_rng(7)
+    left, right, _ = synth.make_pair(68, 640, 480, 64, (6, 12))
+    d0, p0 = run_pair(tuned, left, right, 64, 1)
+    specs = [(69, 320, 240, 32), (70, 480, 272, 48)]
+    pairs = [synth.make_pair(s, w, h, d, (3, 8)) for s, w, h, d in specs]
+    ref = [run_pair(tuned, l, r, d, 1) for (l, r, _), (_, _, _, d) in zip(pairs, specs)]
+    for trial in range(3):
+        forms = rng.integers(0, 2, 24)
+        say("level forms", forms[:10])
+        for l in range(24):
+            tuned.set_tuning(f"level_form.{l}", int(forms[l]))
+        d, p = run_pair(tuned, left, right, 64, 1)
+        assert same_bits(d, d0) and same_bits(p, p0)
+        keep, bps = [], []
+        for (l, r, _), (_, _, _, dd) in zip(pairs, specs):
+            v, py = tuned.cost_volume(l, r, dd), tuned.pyramid(l)
+            keep += [v, py]
+            bps.append(tuned.bp(py, v, min_level=1))
+        tuned.bp_run_batch(bps)
+        for bp, (dr, pr) in zip(bps, ref):
+            d, p = bp.run()
+            assert same_bits(d, dr) and same_bits(p, pr)
+
+
+@pytest.mark.parametrize("right_view", [False, True])
+def test_hamming_widths_give_the_oracle_volume(tuned, right_view):
+    for W, Hh, D in ((333, 201, 37), (1472, 64, 256)):
+        left, right, _ = synth.make_pair(71, W, Hh, D, (3, 8))
+        vo, wo = O.cost_volume(left, right, D, right_view=right_view)
+        for cols in (4, 8):
+            tuned.set_tuning("hamming_cols", cols)
+            v = tuned.cost_volume(left, right, D, right_view=right_view)
+            vg, wg = v.download()
+            assert np.array_equal(vg, vo) and np.array_equal(wg, wo), (W, D, cols)
+            v.free()
+
+
+def test_level_and_hamming_tuners_leave_their_choice(tuned):
+    left, right, _ = synth.make_pair(72, 640, 480, 64, (6, 12))
+    r = tune.tune_levels(tuned, left, right, 64, 2, reps=1)
+    for name, f in r["schedule"].items():
+        assert tuned.get_tuning(f"level_form.{name[1:]}") == (1 if f == "segments" else 0)
+    rh = tune.tune_hamming(tuned, left, right, 64, reps=1)
+    assert tuned.get_tuning("hamming_cols") == rh["best"]["hamming_cols"]
