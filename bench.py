@@ -63,6 +63,13 @@ def main():
     ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV"),
                     help="no GPU: the dominant kernel's HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and "
                          "WRITE_SIZE counter CSVs of a bench run and of a --traffic-calibration run; prints JSON")
+    ap.add_argument("--serve-spec", action="store_true",
+                    help="only the serving sweep: open-loop Poisson arrivals of the serving mix at 10 and 15 pairs/s "
+                         "(below capacity, >= 1000 requests each) and 50, 100, 200, 400 pairs/s for 60 s with "
+                         "backpressure; prints one JSON line")
+    ap.add_argument("--stress-spec", action="store_true",
+                    help="only the memory-pressure run: 3840x2160 D384 pairs (BP on 5 levels) interleaved with the "
+                         "64-pair mixed batch, served under a 16 GB and a 256 GB budget; prints one JSON line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel forms instead of timing the candidates first (the forms "
                          "change speed only, never the results)")
@@ -80,6 +87,14 @@ def main():
     import numpy as np
     from hbp_amd import lib as H
     from hbp_amd import synth
+
+    if args.serve_spec:
+        pool = make_serve_pool(args.seed, per_class=4)
+        print(json.dumps(serve_spec(pool, args.seed)), flush=True)
+        return
+    if args.stress_spec:
+        print(json.dumps(stress_spec(args.seed)), flush=True)
+        return
 
     if args.traffic_calibration:
         ctx = H.Ctx(0)
@@ -565,6 +580,192 @@ def served_vs_plain(ctx, pool, rate, seconds, seed, min_level=2):
             "served": {"pairs_per_s": round(n / span_s, 3), "p50_ms": pct(lat_s, 0.5), "p99_ms": pct(lat_s, 0.99)},
             "plain": {"pairs_per_s": round(n / span_p, 3), "p50_ms": pct(lat_p, 0.5), "p99_ms": pct(lat_p, 0.99)},
             "bit_identical_first_4": bool(same)}
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _same(a, b):
+    import numpy as np
+    return bool(np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32)))
+
+
+def serve_spec(pool, seed, min_level=2, max_pending=64,
+               plan=((10.0, 100.0), (15.0, 70.0), (50.0, 60.0), (100.0, 60.0), (200.0, 60.0), (400.0, 60.0))):
+    """The serving mix (40% 640x480/D64, 40% 1280x720/D128, 20% 1920x1080/D256) as open-loop
+    Poisson arrivals at each (rate, seconds) of plan, through the request server with a queue
+    bound of max_pending (an arrival that finds it full is refused: backpressure).  Per rate:
+    offered, accepted, refused, completed, sustained pairs/s (completions inside the window per
+    second), p50/p99 of completion - arrival over the completed requests; every result checked
+    against the plain path.  Also the plain per-pair loop's capacity on the same mix."""
+    import ctypes as C
+    import numpy as np
+    from hbp_amd import lib as H
+    from hbp_amd import synth
+    ctx = H.Ctx(0)
+    flat = [(c, j) for c in range(len(pool)) for j in range(len(pool[c]))]
+    ref = {}
+    for c, j in flat:
+        (_, w, h, d), left, right = pool[c][j]
+        ref[(c, j)] = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, d), min_level=min_level).run()
+    # plain loop capacity: 60 requests of the mix back to back, host images in, results out
+    k = synth._key(seed, 92)
+    u = synth._u01(k, np.arange(120, dtype=np.uint64)).reshape(60, 2)
+    seq = []
+    for i in range(60):
+        c = 0 if u[i, 0] < 0.4 else (1 if u[i, 0] < 0.8 else 2)
+        seq.append((c, int(u[i, 1] * len(pool[c]))))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for c, j in seq:
+        (_, w, h, d), left, right = pool[c][j]
+        ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, d), min_level=min_level).run()
+    plain_rate = len(seq) / (time.perf_counter() - t0)
+    _progress(f"plain loop capacity {plain_rate:.2f} pairs/s")
+    runs = []
+    for rate, seconds in plan:
+        k = synth._key(seed, 93, int(rate))
+        n_max = int(rate * seconds * 1.5 + 64)
+        u = synth._u01(k, np.arange(3 * n_max, dtype=np.uint64)).reshape(n_max, 3)
+        arrivals, kinds, t = [], [], 0.0
+        for i in range(n_max):
+            t += -np.log(1.0 - u[i, 0]) / rate
+            if t > seconds:
+                break
+            c = 0 if u[i, 1] < 0.4 else (1 if u[i, 1] < 0.8 else 2)
+            arrivals.append(t)
+            kinds.append((c, int(u[i, 2] * len(pool[c]))))
+        n = len(arrivals)
+        srv = H.Server(ctx, max_active=64, hbm_budget=96 << 30)
+        srv.set_max_pending(max_pending)
+        tickets, lat, refused, done_in_window, mismatched = {}, [], 0, 0, 0
+        nxt, last_note = 0, 0.0
+        t0 = time.perf_counter()
+        while nxt < n or tickets:
+            now = time.perf_counter() - t0
+            while nxt < n and arrivals[nxt] <= now:
+                c, j = kinds[nxt]
+                (_, w, h, d), left, right = pool[c][j]
+                try:
+                    tickets[srv.submit(left, right, d, min_level)] = nxt
+                except H.HbpError:
+                    refused += 1
+                nxt += 1
+            if not tickets:
+                time.sleep(max(0.0, min(0.002, arrivals[nxt] - now)) if nxt < n else 0.0)
+                continue
+            _, _, ndone = srv.step()
+            if not ndone:
+                continue
+            now = time.perf_counter() - t0
+            for tk in list(tickets):
+                r = srv.poll(tk)
+                if r is None:
+                    continue
+                i = tickets.pop(tk)
+                lat.append(now - arrivals[i])
+                done_in_window += now <= seconds
+                rd, rp = ref[kinds[i]]
+                mismatched += not (_same(r[0], rd) and _same(r[1], rp))
+            if now - last_note >= 10.0:
+                last_note = now
+                _progress(f"rate {rate:g}/s: t={now:.0f} s, {len(lat)} completed, {refused} refused")
+        span = time.perf_counter() - t0
+        srv.close()
+        lat.sort()
+
+        def pct(q):
+            return round(1e3 * lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))], 1) if lat else None
+
+        runs.append({"offered_per_s": rate, "window_s": seconds, "offered": n, "accepted": n - refused,
+                     "refused": refused, "completed": len(lat),
+                     "sustained_pairs_per_s": round(done_in_window / seconds, 3),
+                     "completed_per_s_incl_drain": round(len(lat) / span, 3),
+                     "p50_ms": pct(0.5), "p99_ms": pct(0.99), "results_not_identical": mismatched})
+        _progress(f"rate {rate:g}/s done: {runs[-1]}")
+    ctx.close()
+    return {"serve_spec": {
+        "workload": "open-loop Poisson arrivals: 40% 640x480/D64, 40% 1280x720/D128, 20% 1920x1080/D256 "
+                    f"(4 generated pairs per class), min_level {min_level}, host images in, disparity + planes out; "
+                    f"server: 64 slots, 96 GB budget, queue bound {max_pending} (backpressure)",
+        "plain_loop_pairs_per_s": round(plain_rate, 3), "runs": runs}}
+
+
+def stress_spec(seed, n4k=8, budgets=(16 << 30, 256 << 30)):
+    """Memory pressure: the 64-pair mixed batch with a 3840x2160 D384 pair (BP on 5 levels)
+    after every 8 of them, all submitted at once, served under each HBM budget; pairs/s, the
+    accounted device bytes at most, how long the 4K pairs waited for admission; every result
+    checked against the plain path; the 4K pair's BP message bytes per level."""
+    import numpy as np
+    from hbp_amd import lib as H
+    from hbp_amd import synth
+    specs = synth.mixed_batch(seed, 64)
+    mix = synth.make_pairs_parallel(specs)
+    big = [synth.make_pair_parallel(seed * 100 + i, 3840, 2160, 384, (20, 40))[:2] for i in range(2)]
+    ctx = H.Ctx(0)
+    jobs = []
+    for i in range(64):
+        jobs.append((mix[i][0], mix[i][1], specs[i][3], 2, "mix"))
+        if i % 8 == 7:
+            b = big[(i // 8) % 2]
+            jobs.append((b[0], b[1], 384, -5, "4k"))
+    # plain references (and the 4K pair's levels)
+    ref = []
+    for l, r, d, ml, kind in jobs:
+        pyr = ctx.pyramid(l)
+        L = pyr.levels
+        m = max(1, min(L - 1 + ml if ml < 0 else ml, L - 1))
+        ref.append(ctx.bp(pyr, ctx.cost_volume(l, r, d), min_level=m).run())
+    pyr4 = ctx.pyramid(big[0][0])
+    L4 = pyr4.levels
+    ml4 = L4 - 6
+    levels = []
+    max_e = max_slots = 0
+    for lv in range(ml4, L4 - 1):
+        info = pyr4.level(lv)
+        deg = np.bincount(info["bd"][:, 0], minlength=info["V"]).max() if info["E"] else 1
+        max_e = max(max_e, info["E"])
+        max_slots = max(max_slots, info["V"] * int(deg))
+        levels.append({"level": lv, "nodes": info["V"], "edges": info["E"], "max_degree": int(deg),
+                       "msg_bytes": 32 * info["E"], "msg_t_bytes": 48 * info["V"] * int(deg)})
+    E0 = pyr4.level(0)["E"]
+    runs = []
+    for budget in budgets:
+        srv = H.Server(ctx, max_active=128, hbm_budget=budget)
+        t0 = time.perf_counter()
+        tickets = {srv.submit(l, r, d, ml): i for i, (l, r, d, ml, _) in enumerate(jobs)}
+        first_active, done_at, out_ok = {}, {}, 0
+        while tickets:
+            a, p, nd = srv.step()
+            if not nd:
+                continue
+            now = time.perf_counter() - t0
+            for tk in list(tickets):
+                r = srv.poll(tk)
+                if r is not None:
+                    i = tickets.pop(tk)
+                    done_at[i] = now
+                    out_ok += _same(r[0], ref[i][0]) and _same(r[1], ref[i][1])
+        span = time.perf_counter() - t0
+        _, peak, _ = srv.stats()
+        srv.close()
+        big_done = sorted(done_at[i] for i, j in enumerate(jobs) if j[4] == "4k")
+        runs.append({"budget_gb": budget / (1 << 30), "pairs": len(jobs), "seconds": round(span, 3),
+                     "pairs_per_s": round(len(jobs) / span, 3), "peak_accounted_gb": round(peak / (1 << 30), 2),
+                     "results_identical": out_ok, "first_4k_done_s": round(big_done[0], 3),
+                     "last_4k_done_s": round(big_done[-1], 3)})
+        _progress(f"budget {budget >> 30} GB: {runs[-1]}")
+    ctx.close()
+    return {"stress_spec": {
+        "workload": "64 pairs of the mixed batch (W U[320,1920], H U[240,1080], D 64-256, min_level 2) with a "
+                    "3840x2160 D384 pair (20-40 planes, BP on 5 levels) after every 8, all submitted at once",
+        "runs": runs,
+        "bp_messages_4k": {"levels": levels, "allocated_bytes": 32 * max_e + 48 * (max_slots + 16),
+                           "reference_bytes": 2 * 4 * 12 * E0,
+                           "note": "messages [E][8] f32 and node-major sweep messages [V][maxdeg][12] f32 per BP "
+                                   "level, allocated at the largest BP level; the reference sizes msg and msg_t "
+                                   "by level 0 (12 E_0 floats each, hbp_isp.hpp:367-368)"}}}
 
 
 def post_process(ctx, left, right, D, min_level):

@@ -90,7 +90,10 @@ struct Server {
     Ctx *c = nullptr;
     Ctx admit_ctx;   // own stream and pool, used only by the admission thread
     int max_active = 0;
+    int max_pending = 0;      // queued requests beyond which hbp_submit refuses (0: no limit)
     size_t budget = 0, in_use = 0;
+    size_t peak_in_use = 0;   // the largest in_use reached (admission accounting)
+    long long rejected = 0;   // submissions refused by max_pending
     float alpha = 1000.1f, tau_s = 0.6f;
     BatchRunner runner;
     PinnedCache pinned;
@@ -303,6 +306,7 @@ static void step(Server &s, int *n_active, int *n_pending, int *n_done) {
             std::unique_ptr<Request> rp = std::move(r);
             s.pending.pop_front();
             s.in_use += rp->est;
+            s.peak_in_use = std::max(s.peak_in_use, s.in_use + rp->est_admit - rp->est);
             rp->counted = true;
             s.in_admission.insert(rp->id);
             rp->coarse_min_pix = s.c->coarse_min_pix;
@@ -498,6 +502,11 @@ int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_righ
     HBP_REQUIRE(width >= 9 && height >= 7 && labels >= 1 && labels <= width && labels <= 1024,
                 "bad image size or disparity range");
     HBP_REQUIRE((long)width * height < (1L << 24) && width < 65536 && height < 65536, "image too large");
+    if (srv->s.max_pending > 0 &&
+        (long)(srv->s.pending.size() + srv->s.in_admission.size()) >= srv->s.max_pending) {
+        ++srv->s.rejected;
+        throw HipError{"server queue full: request refused"};
+    }
     auto r = std::make_unique<Request>();
     r->id = srv->s.next_id++;
     r->W = width;
@@ -511,6 +520,22 @@ int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_righ
     r->est_admit = r->est + volume_bytes(width, height, labels);
     *ticket = r->id;
     srv->s.pending.push_back(std::move(r));
+    HBP_API_END
+}
+
+int hbp_server_set_max_pending(hbp_server *srv, int max_pending) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(srv && max_pending >= 0, "bad argument");
+    srv->s.max_pending = max_pending;
+    HBP_API_END
+}
+
+int hbp_server_stats(hbp_server *srv, long long *bytes_in_use, long long *peak_bytes, long long *rejected) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(srv, "NULL argument");
+    if (bytes_in_use) *bytes_in_use = (long long)srv->s.in_use;
+    if (peak_bytes) *peak_bytes = (long long)srv->s.peak_in_use;
+    if (rejected) *rejected = srv->s.rejected;
     HBP_API_END
 }
 

@@ -86,6 +86,8 @@ SIGNATURES = {
     "hbp_server_step": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hbp_poll": (C.c_int, [_vp, C.c_longlong, _vp, _vp]),
     "hbp_cancel": (C.c_int, [_vp, C.c_longlong]),
+    "hbp_server_set_max_pending": (C.c_int, [_vp, C.c_int]),
+    "hbp_server_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "hbp_profile_copy": (C.c_int, [_vp, C.c_longlong, C.c_int]),
     "hbp_inject_fault": (C.c_int, [C.c_char_p, C.c_longlong]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
@@ -470,8 +472,8 @@ class Server(_Handle):
     def poll(self, ticket):
         """(disp, plane) when done, else None; raises for a failed or cancelled request."""
         H, W = self.sizes.get(ticket, (0, 0))
-        disp = np.zeros((H, W), np.float32)
-        plane = np.zeros((H, W, 3), np.float32)
+        disp = np.empty((H, W), np.float32)          # written only once the request is done
+        plane = np.empty((H, W, 3), np.float32)
         r = self.ctx.lib.hbp_poll(self.h, ticket, _np_ptr(disp), _np_ptr(plane))
         if r == 1:
             return None
@@ -479,6 +481,15 @@ class Server(_Handle):
             raise HbpError(self.ctx.lib.hbp_last_error().decode())
         del self.sizes[ticket]
         return disp, plane
+
+    def set_max_pending(self, n):
+        _check(self.ctx.lib.hbp_server_set_max_pending(self.h, int(n)))
+
+    def stats(self):
+        """(device bytes accounted now, at most, submissions refused)"""
+        a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        _check(self.ctx.lib.hbp_server_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def cancel(self, ticket):
         _check(self.ctx.lib.hbp_cancel(self.h, ticket))

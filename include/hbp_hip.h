@@ -193,6 +193,12 @@ int hbp_submit(hbp_server *srv, const uint8_t *bgr_left, const uint8_t *bgr_righ
  * whose results became ready in this step (any may be NULL) */
 int hbp_server_step(hbp_server *srv, int *n_active, int *n_pending, int *n_done);
 int hbp_poll(hbp_server *srv, long long ticket, float *disp, float *plane3);
+/* backpressure: with max_pending > 0, hbp_submit refuses (error "server queue full") while that
+ * many requests wait for admission; 0 (the default) queues without limit */
+int hbp_server_set_max_pending(hbp_server *srv, int max_pending);
+/* the device bytes the server accounts to its pairs now and at most (admission included), and
+ * the submissions refused by max_pending (any may be NULL) */
+int hbp_server_stats(hbp_server *srv, long long *bytes_in_use, long long *peak_bytes, long long *rejected);
 /* withdraw a request: a queued one leaves at once, one being admitted or on the device leaves
  * the rounds and its memory returns once the queued work has run; it then polls as an error
  * ("cancelled").  A finished result is discarded.  Unknown tickets are an error. */

@@ -169,3 +169,30 @@ def test_raw_handles_then_destroy():
     assert lib.hbp_ctx_destroy(h) != 0
     assert lib.hbp_volume_free(None, v) == 0      # NULL context: the handle's own is used
     assert lib.hbp_ctx_destroy(h) == 0
+
+
+def test_backpressure_refuses_beyond_the_queue_bound(ctx):
+    say("server: backpressure")
+    pairs = [(_pair(151 + i, 96, 72, 24), 24, 2) for i in range(5)]
+    srv = H.Server(ctx, max_active=1)
+    srv.set_max_pending(2)
+    tickets = []
+    refused = 0
+    for (l, r), d, ml in pairs:
+        try:
+            tickets.append((srv.submit(l, r, d, ml), (l, r, d, ml)))
+        except H.HbpError as e:
+            assert "queue full" in str(e)
+            refused += 1
+    assert refused == 3 and len(tickets) == 2
+    assert srv.stats()[2] == 3
+    out = _drive(srv, [t for t, _ in tickets])
+    for t, (l, r, d, ml) in tickets:
+        d1, p1 = _single(ctx, l, r, d, ml)
+        assert same_bits(d1, out[t][0]) and same_bits(p1, out[t][1])
+    (l, r), d, ml = pairs[4]
+    t = srv.submit(l, r, d, ml)          # room again
+    assert _drive(srv, [t])[t] is not None
+    used, peak, _ = srv.stats()
+    assert used == 0 and peak > 0
+    srv.close()
