@@ -70,6 +70,12 @@ def main():
     ap.add_argument("--stress-spec", action="store_true",
                     help="only the memory-pressure run: 3840x2160 D384 pairs (BP on 5 levels) interleaved with the "
                          "64-pair mixed batch, served under a 16 GB and a 256 GB budget; prints one JSON line")
+    ap.add_argument("--mix-levels", action="store_true",
+                    help="only the per-level split of the 64-pair mixed batch's plain BP time (each level's time as "
+                         "the difference of runs ending one level apart); prints one JSON line")
+    ap.add_argument("--traffic-kernel", default=None,
+                    help="with --traffic-summary: the kernel (a name prefix) to summarise instead of the plain path's "
+                         "plane-cost kernel")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernel forms instead of timing the candidates first (the forms "
                          "change speed only, never the results)")
@@ -80,7 +86,7 @@ def main():
         sys.exit(2)
 
     if args.traffic_summary:
-        print(json.dumps(traffic_summary(*args.traffic_summary), indent=1))
+        print(json.dumps(traffic_summary(*args.traffic_summary, kernel=args.traffic_kernel), indent=1))
         return
 
     import ctypes as C
@@ -91,6 +97,10 @@ def main():
     if args.serve_spec:
         pool = make_serve_pool(args.seed, per_class=4)
         print(json.dumps(serve_spec(pool, args.seed)), flush=True)
+        return
+    if args.mix_levels:
+        pairs = make_batch_pairs(64, args.seed)
+        print(json.dumps(mix_levels(pairs)), flush=True)
         return
     if args.stress_spec:
         print(json.dumps(stress_spec(args.seed)), flush=True)
@@ -316,7 +326,7 @@ DOMINANT_KERNEL = "cost_edge_kernel<4, true, 0, 4>"   # plane cost, coarse form,
 CALIB_KIB = 262144.0                                  # the calibration copy reads and writes 256 MiB
 
 
-def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=DOMINANT_KERNEL):
+def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=None):
     """HBM bytes per launch of `kernel`: the mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
     over its launches, each scaled by the ratio of the known bytes to the counter's reading on
     the calibration copy (calib_copy_kernel)."""
@@ -332,6 +342,7 @@ def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=D
                     grids.setdefault(row["Grid_Size"], []).append(v)
         return vals, grids
 
+    kernel = kernel or DOMINANT_KERNEL
     cf, _ = per_launch(cal_fetch_csv, "calib_copy_kernel")
     cw, _ = per_launch(cal_write_csv, "calib_copy_kernel")
     sf = CALIB_KIB / (sum(cf) / len(cf))
@@ -690,6 +701,74 @@ def serve_spec(pool, seed, min_level=2, max_pending=64,
                     f"(4 generated pairs per class), min_level {min_level}, host images in, disparity + planes out; "
                     f"server: 64 slots, 96 GB budget, queue bound {max_pending} (backpressure)",
         "plain_loop_pairs_per_s": round(plain_rate, 3), "runs": runs}}
+
+
+def mix_levels(pairs, min_level=2, latency_us=25.0):
+    """Per-level split of the plain path's time over the mixed batch: for each pair, the cost
+    volume, the pyramid and the BP ending at each level from the top down to min_level (best of
+    two runs each); a level's time is the difference of the runs ending at it and one level
+    above.  A level whose time per sweep is below latency_us is counted as latency-bound (its
+    sweeps are chains of small launches that batching amortises); the others as throughput-bound
+    (their launches already fill the GPU with one pair)."""
+    from hbp_amd import lib as H
+    ctx = H.Ctx(0)
+    lib = ctx.lib
+
+    def best(fn, reps=2):
+        fn()
+        ctx.sync()
+        b = float("inf")
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ctx.sync()
+            b = min(b, time.perf_counter() - t)
+        return b
+
+    rows = []
+    t_cost = t_pyr = 0.0
+    for spec, left, right in pairs:
+        _, w, h, d = spec
+        t_cost += best(lambda: ctx.cost_volume(left, right, d).free())
+        t_pyr += best(lambda: ctx.pyramid(left).free())
+        vol, pyr = ctx.cost_volume(left, right, d), ctx.pyramid(left)
+        L = pyr.levels
+        prev = best(lambda: ctx.bp(pyr, vol, min_level=L - 1).run())   # no BP level: the map alone
+        for l in range(L - 2, min_level - 1, -1):
+            t = best(lambda: ctx.bp(pyr, vol, min_level=l).run())
+            sweeps = lib.hbp_schedule_sweeps(L, l, d) - lib.hbp_schedule_sweeps(L, l + 1, d) if l + 1 < L - 1 \
+                else lib.hbp_schedule_sweeps(L, l, d)
+            V, E = pyr.level_size(l)
+            rows.append(dict(w=w, h=h, d=d, level=l, nodes=V, edges=E, sweeps=sweeps, ms=1e3 * max(t - prev, 0.0)))
+            prev = t
+        vol.free()
+        pyr.free()
+    ctx.close()
+    lat = [r for r in rows if r["ms"] * 1e3 / max(r["sweeps"], 1) < latency_us]
+    thr = [r for r in rows if r not in lat]
+    t_lat = sum(r["ms"] for r in lat)
+    t_thr = sum(r["ms"] for r in thr)
+    total = 1e3 * (t_cost + t_pyr) + t_lat + t_thr
+    by_pixels = {}
+    for r in rows:
+        k = "superpixels >= 1000" if r["nodes"] >= 1000 else "superpixels < 1000"
+        a = by_pixels.setdefault(k, dict(ms=0.0, sweeps=0))
+        a["ms"] += r["ms"]
+        a["sweeps"] += r["sweeps"]
+    return {"mix_levels": {
+        "workload": f"{len(pairs)} pairs of the mixed batch, plain path, min_level {min_level}",
+        "cost_volume_ms": round(1e3 * t_cost, 1), "pyramid_ms": round(1e3 * t_pyr, 1),
+        "bp_latency_bound_ms": round(t_lat, 1), "bp_throughput_bound_ms": round(t_thr, 1),
+        "latency_rule_us_per_sweep": latency_us,
+        "levels_latency_bound": len(lat), "levels_throughput_bound": len(thr),
+        "sweeps_latency_bound": sum(r["sweeps"] for r in lat), "sweeps_throughput_bound": sum(r["sweeps"] for r in thr),
+        "us_per_sweep_latency_bound": round(1e3 * t_lat / max(sum(r["sweeps"] for r in lat), 1), 2),
+        "by_level_size": {k: dict(ms=round(v["ms"], 1), sweeps=v["sweeps"],
+                                  us_per_sweep=round(1e3 * v["ms"] / max(v["sweeps"], 1), 2))
+                          for k, v in by_pixels.items()},
+        "total_ms": round(total, 1),
+        "ceiling_if_latency_bound_work_were_free": round(total / max(total - t_lat, 1e-9), 3),
+        "ceiling_if_it_and_the_pyramid_were_free": round(total / max(total - t_lat - 1e3 * t_pyr, 1e-9), 3)}}
 
 
 def stress_spec(seed, n4k=8, budgets=(16 << 30, 256 << 30)):
