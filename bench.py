@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--mix-levels", action="store_true",
                     help="only the per-level split of the 64-pair mixed batch's plain BP time (each level's time as "
                          "the difference of runs ending one level apart); prints one JSON line")
+    ap.add_argument("--batch-kernels", action="store_true",
+                    help="only the batched BP kernels over the 64-pair mixed batch: each kernel's time "
+                         "(HIP events around every launch), launches and algorithmic GB/s against the HBM peak")
     ap.add_argument("--traffic-kernel", default=None,
                     help="with --traffic-summary: the kernel (a name prefix) to summarise instead of the plain path's "
                          "plane-cost kernel")
@@ -104,6 +107,10 @@ def main():
         return
     if args.stress_spec:
         print(json.dumps(stress_spec(args.seed)), flush=True)
+        return
+    if args.batch_kernels:
+        pairs = make_batch_pairs(64, args.seed)
+        print(json.dumps(batch_kernels(pairs)), flush=True)
         return
 
     if args.traffic_calibration:
@@ -163,7 +170,7 @@ def main():
         # single_hb for this pair's shape; coarse_min_pix matters for the mixed sizes below (the
         # coarse levels of this pair all hold thousands of pixels per superpixel)
         from hbp_amd import tune
-        tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2)})
+        tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2), "seg_len": (512, 384, 256)})
         tuning["levels"] = tune.tune_levels(ctx, left, right, D, info["min_level"])
         tuning["hamming"] = tune.tune_hamming(ctx, left, right, D)
     for _ in range(args.warmup):
@@ -441,11 +448,18 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
             release([o])
         return outs
 
+    split = {}
+
     def batched(keep):
+        t0 = time.perf_counter()
         objs = [prepare(spec, dl, dr) for spec, dl, dr in dev]
+        ctx.sync()
+        t1 = time.perf_counter()
         arr = (C.c_void_p * len(objs))(*[o[2] for o in objs])
         H._check(lib.hbp_bp_run_batch(ctx.h, len(objs), arr))
         ctx.sync()
+        split["prepare_ms"] = round((t1 - t0) * 1e3, 1)
+        split["bp_rounds_ms"] = round((time.perf_counter() - t1) * 1e3, 1)
         outs = [download(o[2], spec) for o, (spec, _, _) in zip(objs, dev)] if keep else []
         release(objs)
         return outs
@@ -457,17 +471,28 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
     plain(False)
     ctx.sync()
     t_plain = time.perf_counter() - t0
-    batched(False)
-    ctx.sync()
-    t0 = time.perf_counter()
-    batched(False)
-    ctx.sync()
-    t_batch = time.perf_counter() - t0
+    # the batched rounds queued from the host, then captured in one HIP graph and replayed
+    graph0 = ctx.get_tuning("batch_graph")
+    t_form, split_form = {}, {}
+    for g in (0, 1):
+        ctx.set_tuning("batch_graph", g)
+        batched(False)
+        ctx.sync()
+        t0 = time.perf_counter()
+        batched(False)
+        ctx.sync()
+        t_form[g] = time.perf_counter() - t0
+        split_form[g] = dict(split)
     # bit-identical outputs (outside the timed runs)
     ref = plain(True)
-    got = batched(True)
-    same = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
-               np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) for a, b in zip(ref, got))
+    same = {}
+    for g in (0, 1):
+        ctx.set_tuning("batch_graph", g)
+        got = batched(True)
+        same[g] = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
+                      np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) for a, b in zip(ref, got))
+    ctx.set_tuning("batch_graph", graph0)
+    t_batch = t_form[graph0]
     for _, dl, dr in dev:
         hip.hipFree(dl)
         hip.hipFree(dr)
@@ -477,7 +502,13 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
                         f"6-12 planes, min_level 2, reference parameters; images resident in HBM",
             "pairs": n, "plain_pairs_per_s": round(n / t_plain, 3), "batched_pairs_per_s": round(n / t_batch, 3),
             "speedup": round(t_plain / t_batch, 3), "plain_mpix_disp_per_s": round(mpx / t_plain, 1),
-            "batched_mpix_disp_per_s": round(mpx / t_batch, 1), "bit_identical": bool(same),
+            "batched_mpix_disp_per_s": round(mpx / t_batch, 1), "bit_identical": bool(same[0] and same[1]),
+            "rounds": "in one HIP graph" if graph0 else "queued from the host",
+            "rounds_ab": {"queued_pairs_per_s": round(n / t_form[0], 3), "graph_pairs_per_s": round(n / t_form[1], 3),
+                          "graph_vs_queued": round(t_form[0] / t_form[1], 3),
+                          "split_ms": {"queued": split_form[0], "graph": split_form[1]},
+                          "bit_identical": {"queued": bool(same[0]), "graph": bool(same[1])},
+                          "graph_timed": "capture of every round, instantiation, one replay, synchronisation"},
             "timed": "plain: cost volume + pyramid + BP + map per pair in turn; batched: every pair's cost volume "
                      "and pyramid, then hbp_bp_run_batch over all pairs"}
 
@@ -769,6 +800,64 @@ def mix_levels(pairs, min_level=2, latency_us=25.0):
         "total_ms": round(total, 1),
         "ceiling_if_latency_bound_work_were_free": round(total / max(total - t_lat, 1e-9), 3),
         "ceiling_if_it_and_the_pyramid_were_free": round(total / max(total - t_lat - 1e3 * t_pyr, 1e-9), 3)}}
+
+
+def batch_kernels(pairs, min_level=2):
+    """The batched BP rounds of the mixed batch with HIP events around every launch: per kernel
+    the time, the launches and the algorithmic bytes (per launch the sum of the one-pair
+    launches' figures: 12 B per pixel and 24 B per plane and superpixel for the plane costs, 96 B
+    per edge and 16 K + 48 B per superpixel for the messages, 84 B per edge and 48 K + 16 B per
+    superpixel for the node update), hence GB/s against the HBM peak.  The rounds' wall time
+    with and without the events is reported beside them."""
+    from hbp_amd import lib as H
+    ctx = H.Ctx(0)
+
+    def prepare():
+        keep, bps = [], []
+        for (sd, w, h, d), left, right in pairs:
+            v, p = ctx.cost_volume(left, right, d), ctx.pyramid(left)
+            keep += [v, p]
+            bps.append(ctx.bp(p, v, min_level=min(min_level, p.levels - 1)))
+        ctx.sync()
+        return keep, bps
+
+    def rounds(profiled):
+        keep, bps = prepare()
+        if profiled:
+            ctx.profile(True)
+            ctx.profile_reset()
+            ctx.profile_only(None, 1)
+        t0 = time.perf_counter()
+        ctx.bp_run_batch(bps)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        st = ctx.profile_stats() if profiled else None
+        ctx.profile(False)
+        sweeps = sum(b.total_sweeps for b in bps)
+        for b in bps:
+            b.free()
+        for o in keep:
+            o.free()
+        return el, st, sweeps
+
+    rounds(False)
+    t_plain_ev, _, _ = rounds(False)
+    t_ev, st, sweeps = rounds(True)
+    kern = {}
+    for name, v in sorted(st.items(), key=lambda kv: -kv[1]["ms"]):
+        if not name.startswith("batch_") or v["launches"] == 0:
+            continue
+        gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
+        kern[name] = {"ms": round(v["ms"], 1), "launches": v["launches"],
+                      "us_per_launch": round(1e3 * v["ms"] / v["launches"], 2),
+                      "mb_per_launch": round(v["bytes"] / v["launches"] / 1e6, 2),
+                      "gbs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return {"batch_kernels": {
+        "workload": f"{len(pairs)} pairs of the mixed batch, min_level {min_level}, hbp_bp_run_batch (rounds queued)",
+        "sweeps": sweeps, "rounds_s": round(t_plain_ev, 3), "rounds_s_with_events": round(t_ev, 3),
+        "kernels": kern,
+        "bound": "the messages (batch_edge and the coarse launches' message waves) are VALU-bound; the HBM "
+                 "fractions are of the algorithmic bytes, not of measured traffic"}}
 
 
 def stress_spec(seed, n4k=8, budgets=(16 << 30, 256 << 30)):

@@ -62,6 +62,10 @@ __global__ void crx_kernel(const uint32_t *__restrict__ cr, int W, int H, int S,
     }
 }
 
+__global__ void zero_f4_kernel(float4 *__restrict__ p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
 __global__ void iota_u32_kernel(unsigned int *__restrict__ p, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         p[i] = (unsigned)i;
@@ -139,18 +143,18 @@ __global__ void pix_kernel(const unsigned int *__restrict__ keys, const int *__r
         pcl[j] = keys[j];
     }
 }
-// segments of at most SEG pixels of one superpixel: count, then (first, end, cluster, index)
-__global__ void seg_count_kernel(const int *__restrict__ pix_off, int V, int *__restrict__ cnt) {
+// segments of at most sl <= SEG pixels of one superpixel: count, then (first, end, cluster, index)
+__global__ void seg_count_kernel(const int *__restrict__ pix_off, int V, int sl, int *__restrict__ cnt) {
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c <= V; c += (long)gridDim.x * blockDim.x)
-        cnt[c] = c < V ? (pix_off[c + 1] - pix_off[c] + SEG - 1) / SEG : 0;
+        cnt[c] = c < V ? (pix_off[c + 1] - pix_off[c] + sl - 1) / sl : 0;
 }
-__global__ void seg_fill_kernel(const int *__restrict__ pix_off, const int *__restrict__ sego, int V,
+__global__ void seg_fill_kernel(const int *__restrict__ pix_off, const int *__restrict__ sego, int V, int sl,
                                 int4 *__restrict__ seg) {
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < V; c += (long)gridDim.x * blockDim.x) {
         int b = pix_off[c], e = pix_off[c + 1];
         int o = sego[c];
-        for (int i = 0; b + i * SEG < e; ++i)
-            seg[o + i] = make_int4(b + i * SEG, min(b + (i + 1) * SEG, e), (int)c, i);
+        for (int i = 0; b + i * sl < e; ++i)
+            seg[o + i] = make_int4(b + i * sl, min(b + (i + 1) * sl, e), (int)c, i);
     }
 }
 // reverse_bd_id_k (hbp_isp.cu:154-175): first r in t's CSR range with bd[r].y == s
@@ -620,7 +624,7 @@ __device__ __forceinline__ float pc_term(const SweepArgs &a, const PcPix &px, fl
 }
 
 // ----------------------------------------------------------------------------------------
-// plane cost, coarse levels: one wavefront per segment (<= SEG pixels of one superpixel).
+// plane cost, coarse levels: one wavefront per segment (<= seg_len <= SEG pixels of one superpixel).
 // Lane l reads pixels first + l + 64 r; the planes are wave-uniform.  A batch of B steps first
 // issues the B packed positions, then every census word the B x P plane costs need, pins them
 // with an empty asm so that no load is deferred into a branch, and only then computes: two
@@ -915,6 +919,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
         X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
         float c0 = a.c0[e];
         float dpos = a.dpos[e];
+        const double rdpos = 1.0 / (double)dpos;
         float4 psl = (spatial && y >= 2 * P) ? spatial_candidate(a, s, y - 2 * P) : a.plane[(long)s * K + y];
         float4 m0 = a.msg[2 * e], m1 = a.msg[2 * e + 1];
         float mh[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
@@ -933,7 +938,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
             float phi = fabsf(dpx * X[0] + dpy * Y[0] + dpz);
 #pragma unroll
             for (int j = 1; j < 6; ++j) phi = phi + fabsf(dpx * X[j] + dpy * Y[j] + dpz);
-            phi = fminf(div3_nonneg(phi) / dpos, a.tau_s);
+            phi = fminf(div_recip(div3_nonneg(phi), dpos, rdpos), a.tau_s);
             float v = m + c0 * phi;
             if (v < minimum) minimum = v;
         }
@@ -944,7 +949,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
     const int g = grp * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) mean += __shfl(out, g + k, 64);
-    mean /= K;
+    mean = div_recip(mean, (float)K, 1.0 / K);
     if (live) a.msg_t[mslot * K + y] = out - mean;
 }
 
@@ -1500,21 +1505,23 @@ void bp_setup(Ctx &c, BP &b) {
             B.pix_off.alloc(c.pool, (size_t)V + 1);
             hipLaunchKernelGGL(lower_bound_kernel, dim3(grid_for(c, N)), dim3(256), 0, c.stream, keys.p, N, V,
                                B.pix_off.p);
-            // coarse levels: segments of <= SEG pixels per superpixel
+            // coarse levels: segments of <= seg_len pixels per superpixel
             // (a per-level choice of the context overrides the threshold)
             const int form = l < Ctx::kFormLevels ? c.level_form[l] : -1;
             B.coarse = form < 0 ? N >= (long)c.coarse_min_pix * V : form == 1;
             if (B.coarse) {
+                const int sl = c.seg_len;
+                HBP_REQUIRE(sl >= 64 && sl <= SEG && sl % 64 == 0, "segment length must be a multiple of 64 in [64, 512]");
                 DevBuf<int> scnt(c.pool, (size_t)V + 1);
                 B.sego.alloc(c.pool, (size_t)V + 1);
-                B.nseg_max = (int)(div_up(N, SEG) + V);
+                B.nseg_max = (int)(div_up(N, sl) + V);
                 B.seg.alloc(c.pool, (size_t)B.nseg_max);
                 HBP_CHECK(hipMemsetAsync(B.seg.p, 0, sizeof(int4) * (size_t)B.nseg_max, c.stream));
                 hipLaunchKernelGGL(seg_count_kernel, dim3(grid_for(c, (long)V + 1)), dim3(256), 0, c.stream,
-                                   B.pix_off.p, V, scnt.p);
+                                   B.pix_off.p, V, sl, scnt.p);
                 scan_exclusive_int(c, scnt.p, B.sego.p, V + 1, nullptr);
                 hipLaunchKernelGGL(seg_fill_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.pix_off.p,
-                                   B.sego.p, V, B.seg.p);
+                                   B.sego.p, V, sl, B.seg.p);
             }
             // CSR by source, reverse edges, edge weights, mean positions
             B.off.alloc(c.pool, (size_t)V + 1);
@@ -1657,7 +1664,8 @@ static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l
 // level init: prior planes, seeds, first RANSAC candidates, costs of all 12 slots
 static void run_level_init(Ctx &c, BP &b, const SweepArgs &a, int l) {
     const BPLevel &B = b.lv[l];
-    HBP_CHECK(hipMemsetAsync(b.msg.p, 0, sizeof(float4) * 2 * (size_t)a.E, c.stream));
+    if (a.E > 0)
+        hipLaunchKernelGGL(zero_f4_kernel, dim3(grid_for(c, 2 * a.E)), dim3(256), 0, c.stream, b.msg.p, 2 * a.E);
     hipLaunchKernelGGL(init_level_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
     launch_cost_edge<K>(c, a, B, l, 0, false, true, false);
     hipLaunchKernelGGL(init_cost_kernel, dim3(grid_for(c, a.V)), dim3(256), 0, c.stream, a);
@@ -1789,18 +1797,17 @@ static void launch_batch_coarse(Ctx &c, int mode, int blocks, const SweepArgs *d
                            items, n);
 }
 
-void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active) {
+bool BatchRunner::plan(const std::vector<std::pair<BP *, int>> &active, RoundPlan &rp) const {
     // items: the segment-form ones first, then the run-form ones; each form is one cost launch
     // with its own block numbering, the node launch covers both
     std::vector<BatchItem> part[2];
-    int coarse_blocks = 0, fine_blocks = 0, node_blocks = 0, sp_blocks = 0, msg_blocks = 0;
     for (const auto &pr : active) {
         BP &b = *pr.first;
         const size_t i = b.next;
         if (i >= b.sched.size()) continue;
         const Sweep &sw = b.sched[i];
         const BPLevel &B = b.lv[sw.level];
-        if (sw.first) run_level_init(c, b, sweep_args(b, sw.level), sw.level);
+        if (sw.first) rp.inits.push_back({&b, sw.level});
         BatchItem it{};
         it.args = pr.second * kBatchLevels + sw.level;
         it.kind = sw.kind;
@@ -1818,82 +1825,152 @@ void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active)
         const int eb = (int)div_up(div_up(B.E, EDGES_PER_WAVE), 4);
         it.edge_blocks = B.coarse ? eb : 0;
         it.msg_blocks = B.coarse ? 0 : eb;
-        int &blocks = B.coarse ? coarse_blocks : fine_blocks;
+        int &blocks = B.coarse ? rp.coarse_blocks : rp.fine_blocks;
         it.blk_begin = blocks;
         blocks += (int)round_up(it.pc_blocks + it.edge_blocks, 8);
         it.node_blocks = div_up((long)B.V * 16, 256);
         it.sp_blocks = (!B.coarse && sw.kind == kSpatial && !sw.first) ? div_up(B.V, 256) : 0;
-        part[B.coarse ? 0 : 1].push_back(it);
-    }
-    const int ncoarse = (int)part[0].size(), nfine = (int)part[1].size(), count = ncoarse + nfine;
-    if (count == 0) return;
-    std::vector<BatchItem> hitems;
-    hitems.reserve(count);
-    for (int f = 0; f < 2; ++f)
-        for (BatchItem &it : part[f]) {
-            it.node_begin = node_blocks;
-            node_blocks += it.node_blocks;
-            it.msg_begin = msg_blocks;
-            msg_blocks += it.msg_blocks;
-            it.sp_begin = sp_blocks;
-            sp_blocks += it.sp_blocks;
-            hitems.push_back(it);
+        const double pcb = it.npc > 0 ? plane_cost_bytes(b.N, B.V, P) : 0.0;
+        const double eb_bytes = B.E > 0 ? edge_bytes(B.E, B.V) : 0.0;
+        if (B.coarse) {
+            rp.coarse_bytes += pcb + eb_bytes;
+        } else {
+            rp.fine_bytes += pcb;
+            rp.msg_bytes += eb_bytes;
         }
-    if (items.n < (size_t)count) items.alloc(c.pool, std::max<size_t>(count, 64));
-    // staged in the pinned ring (the copy runs in stream order, after the previous round's
-    // kernels have read the device items); the slot's last copy must have run before reuse
-    HBP_REQUIRE(count <= nslots, "more batch items than slots");
-    const int slot = ring_pos;
-    ring_pos = (ring_pos + 1) % kRing;
-    if (ev_live[slot]) HBP_CHECK(hipEventSynchronize(ev[slot]));
-    BatchItem *src = ring + (size_t)slot * nslots;
-    std::memcpy(src, hitems.data(), sizeof(BatchItem) * count);
-    HBP_CHECK(hipMemcpyAsync(items.p, src, sizeof(BatchItem) * count, hipMemcpyHostToDevice, c.stream));
-    HBP_CHECK(hipEventRecord(ev[slot], c.stream));
-    ev_live[slot] = true;
-    const SweepArgs *dargs = (const SweepArgs *)table.p;
-    const BatchItem *itc = items.p, *itf = items.p + ncoarse;
-    if (sp_blocks > 0) {
-        ProfScope ps(c, "batch_spatial", 0.0);
-        hipLaunchKernelGGL(spatial_batch_kernel, dim3(sp_blocks), dim3(256), 0, c.stream, dargs, itc, count);
-    }
-    if (coarse_blocks > 0) {
-        ProfScope ps(c, "batch_plane_cost_coarse", 0.0);
-        if (c.batch_hb == 4) launch_batch_coarse<4>(c, mode, coarse_blocks, dargs, itc, ncoarse);
-        else launch_batch_coarse<2>(c, mode, coarse_blocks, dargs, itc, ncoarse);
-    }
-    if (fine_blocks > 0) {
-        ProfScope ps(c, "batch_plane_cost_fine", 0.0);
-        if (mode == kCostLeft)
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, false>), dim3(fine_blocks), dim3(256), PC_LDS_BYTES,
-                               c.stream, dargs, itf, nfine);
-        else if (mode == kCostRight)
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, false>), dim3(fine_blocks), dim3(256),
-                               PC_LDS_BYTES, c.stream, dargs, itf, nfine);
-        else if (mode == kCostF32)
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostF32, false>), dim3(fine_blocks), dim3(256),
-                               PC_LDS_BYTES, c.stream, dargs, itf, nfine);
-        else
-            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), dim3(fine_blocks), dim3(256), PC_LDS_BYTES,
-                               c.stream, dargs, itf, nfine);
-    }
-    if (msg_blocks > 0) {
-        ProfScope ps(c, "batch_edge", 0.0);
-        hipLaunchKernelGGL(edge_batch_kernel, dim3(msg_blocks), dim3(256), 0, c.stream, dargs, itc, count);
-    }
-    {
-        ProfScope ps(c, "batch_node", 0.0);
-        hipLaunchKernelGGL(node_batch_kernel, dim3(node_blocks), dim3(256), 0, c.stream, dargs, itc, count);
-    }
-    HBP_CHECK(hipGetLastError());
-    for (const auto &pr : active) {
-        BP &b = *pr.first;
-        if (b.next >= b.sched.size()) continue;
-        b.cur = b.sched[b.next].level;
+        rp.node_bytes += node_bytes(B.E, B.V);
+        if (it.sp_blocks > 0) rp.sp_bytes += (16.0 * P + 28) * B.V;
+        part[B.coarse ? 0 : 1].push_back(it);
+        b.cur = sw.level;
         ++b.next;
         ++b.sweeps;
         if (b.next >= b.sched.size()) b.cur = b.min_level - 1;
     }
+    rp.ncoarse = (int)part[0].size();
+    rp.nfine = (int)part[1].size();
+    const int count = rp.ncoarse + rp.nfine;
+    if (count == 0) return false;
+    HBP_REQUIRE(count <= nslots, "more batch items than slots");
+    rp.items.reserve(count);
+    for (int f = 0; f < 2; ++f)
+        for (BatchItem &it : part[f]) {
+            it.node_begin = rp.node_blocks;
+            rp.node_blocks += it.node_blocks;
+            it.msg_begin = rp.msg_blocks;
+            rp.msg_blocks += it.msg_blocks;
+            it.sp_begin = rp.sp_blocks;
+            rp.sp_blocks += it.sp_blocks;
+            rp.items.push_back(it);
+        }
+    return true;
+}
+
+void BatchRunner::launch(Ctx &c, const RoundPlan &rp, const BatchItem *d_items) const {
+    for (const auto &in : rp.inits) run_level_init(c, *in.first, sweep_args(*in.first, in.second), in.second);
+    const int count = rp.ncoarse + rp.nfine;
+    const SweepArgs *dargs = (const SweepArgs *)table.p;
+    const BatchItem *itc = d_items, *itf = d_items + rp.ncoarse;
+    if (rp.sp_blocks > 0) {
+        ProfScope ps(c, "batch_spatial", rp.sp_bytes);
+        hipLaunchKernelGGL(spatial_batch_kernel, dim3(rp.sp_blocks), dim3(256), 0, c.stream, dargs, itc, count);
+    }
+    if (rp.coarse_blocks > 0) {
+        ProfScope ps(c, "batch_plane_cost_coarse", rp.coarse_bytes);
+        if (c.batch_hb == 4) launch_batch_coarse<4>(c, mode, rp.coarse_blocks, dargs, itc, rp.ncoarse);
+        else launch_batch_coarse<2>(c, mode, rp.coarse_blocks, dargs, itc, rp.ncoarse);
+    }
+    if (rp.fine_blocks > 0) {
+        ProfScope ps(c, "batch_plane_cost_fine", rp.fine_bytes);
+        const dim3 g(rp.fine_blocks), blk(256);
+        const int nf = rp.nfine;
+        if (mode == kCostLeft)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostLeft, false>), g, blk, PC_LDS_BYTES, c.stream, dargs, itf, nf);
+        else if (mode == kCostRight)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostRight, false>), g, blk, PC_LDS_BYTES, c.stream, dargs, itf, nf);
+        else if (mode == kCostF32)
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostF32, false>), g, blk, PC_LDS_BYTES, c.stream, dargs, itf, nf);
+        else
+            hipLaunchKernelGGL((cost_edge_batch_kernel<kCostWta, false>), g, blk, PC_LDS_BYTES, c.stream, dargs, itf, nf);
+    }
+    if (rp.msg_blocks > 0) {
+        ProfScope ps(c, "batch_edge", rp.msg_bytes);
+        hipLaunchKernelGGL(edge_batch_kernel, dim3(rp.msg_blocks), dim3(256), 0, c.stream, dargs, itc, count);
+    }
+    {
+        ProfScope ps(c, "batch_node", rp.node_bytes);
+        hipLaunchKernelGGL(node_batch_kernel, dim3(rp.node_blocks), dim3(256), 0, c.stream, dargs, itc, count);
+    }
+    HBP_CHECK(hipGetLastError());
+}
+
+void BatchRunner::round(Ctx &c, const std::vector<std::pair<BP *, int>> &active) {
+    RoundPlan rp;
+    if (!plan(active, rp)) return;
+    const int count = rp.ncoarse + rp.nfine;
+    if (items.n < (size_t)count) items.alloc(c.pool, std::max<size_t>(count, 64));
+    // staged in the pinned ring (the copy runs in stream order, after the previous round's
+    // kernels have read the device items); the slot's last copy must have run before reuse
+    const int slot = ring_pos;
+    ring_pos = (ring_pos + 1) % kRing;
+    if (ev_live[slot]) HBP_CHECK(hipEventSynchronize(ev[slot]));
+    BatchItem *src = ring + (size_t)slot * nslots;
+    std::memcpy(src, rp.items.data(), sizeof(BatchItem) * count);
+    HBP_CHECK(hipMemcpyAsync(items.p, src, sizeof(BatchItem) * count, hipMemcpyHostToDevice, c.stream));
+    HBP_CHECK(hipEventRecord(ev[slot], c.stream));
+    ev_live[slot] = true;
+    launch(c, rp, items.p);
+}
+
+// The whole schedule of rounds captured in one HIP graph and replayed once: every round's items
+// go to the device in one copy before the capture, and the graph holds kernel launches only
+// (the level inits clear their messages with a kernel, not a memset).  The same launches in
+// the same order as round by round, so the same bits.
+static void bp_run_batch_graph(Ctx &c, BatchRunner &r, const std::vector<std::pair<BP *, int>> &active) {
+    std::vector<RoundPlan> rounds;
+    size_t total = 0;
+    for (;;) {
+        RoundPlan rp;
+        if (!r.plan(active, rp)) break;
+        total += rp.items.size();
+        rounds.push_back(std::move(rp));
+    }
+    if (rounds.empty()) return;
+    std::vector<BatchItem> all;
+    all.reserve(total);
+    for (const RoundPlan &rp : rounds) all.insert(all.end(), rp.items.begin(), rp.items.end());
+    DevBuf<BatchItem> d_all(c.pool, total);
+    HBP_CHECK(hipMemcpyAsync(d_all.p, all.data(), sizeof(BatchItem) * total, hipMemcpyHostToDevice, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    // the profiler's events stay out of the graph
+    const bool prof = c.prof.on;
+    c.prof.on = false;
+    hipGraph_t g = nullptr;
+    HBP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        size_t off = 0;
+        for (const RoundPlan &rp : rounds) {
+            r.launch(c, rp, d_all.p + off);
+            off += rp.items.size();
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c.prof.on = prof;
+        throw;
+    }
+    c.prof.on = prof;
+    HBP_CHECK(hipStreamEndCapture(c.stream, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        HBP_CHECK(ie);
+    }
+    const hipError_t le = hipGraphLaunch(ge, c.stream);
+    const hipError_t se = le == hipSuccess ? hipStreamSynchronize(c.stream) : le;
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    HBP_CHECK(se);
 }
 
 // Runs the remaining sweeps of several BP objects together (round i carries sweep next+i of
@@ -1907,11 +1984,15 @@ void bp_run_batch(Ctx &c, const std::vector<BP *> &bps) {
     r.init(c, n);
     std::vector<std::pair<BP *, int>> active;
     for (BP *bp : bps) active.push_back({bp, r.add(c, *bp)});
-    for (;;) {
-        bool any = false;
-        for (BP *bp : bps) any |= bp->next < bp->sched.size();
-        if (!any) break;
-        r.round(c, active);
+    if (c.batch_graph) {
+        bp_run_batch_graph(c, r, active);
+    } else {
+        for (;;) {
+            bool any = false;
+            for (BP *bp : bps) any |= bp->next < bp->sched.size();
+            if (!any) break;
+            r.round(c, active);
+        }
     }
     for (BP *bp : bps) bp_map(c, *bp);
     HBP_CHECK(hipStreamSynchronize(c.stream));

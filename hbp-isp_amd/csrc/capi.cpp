@@ -229,12 +229,18 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
     } else if (n == "hamming_cols") {
         HBP_REQUIRE(value == 4 || value == 8, "hamming_cols must be 4 or 8");
         ctx->c.hamming_cols = value;
+    } else if (n == "seg_len") {
+        HBP_REQUIRE(value >= 64 && value <= 512 && value % 64 == 0, "seg_len must be a multiple of 64 in [64, 512]");
+        ctx->c.seg_len = value;
+    } else if (n == "batch_graph") {
+        HBP_REQUIRE(value == 0 || value == 1, "batch_graph must be 0 or 1");
+        ctx->c.batch_graph = value;
     } else if (n.rfind("level_form.", 0) == 0) {
         const int l = level_form_index(n);
         HBP_REQUIRE(value >= -1 && value <= 1, "a level form is -1 (by coarse_min_pix), 0 (chunks) or 1 (segments)");
         ctx->c.level_form[l] = value;
     } else {
-        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, level_form.<l>)"};
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, level_form.<l>)"};
     }
     HBP_API_END
 }
@@ -247,8 +253,10 @@ int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
     else if (n == "single_hb") *value = ctx->c.single_hb;
     else if (n == "batch_hb") *value = ctx->c.batch_hb;
     else if (n == "hamming_cols") *value = ctx->c.hamming_cols;
+    else if (n == "batch_graph") *value = ctx->c.batch_graph;
+    else if (n == "seg_len") *value = ctx->c.seg_len;
     else if (n.rfind("level_form.", 0) == 0) *value = ctx->c.level_form[level_form_index(n)];
-    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, level_form.<l>)"};
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, level_form.<l>)"};
     HBP_API_END
 }
 

@@ -121,6 +121,24 @@ __host__ __device__ inline float div3_nonneg(float x) {
     return fmaf(r, y, q0);
 }
 
+// rn(x / d) from r = rn64(1 / (double)d): the double product x r lies within 2^-52 (relative)
+// of x / d, and a quotient of two floats that is a normal float lies at least 2^-49 (relative)
+// from every midpoint of two consecutive floats (a midpoint has a 25-bit odd significand, and
+// x - m d is then a nonzero multiple of m's last bit times d's), so rounding the product gives
+// the quotient's own rounding; overflow rounds alike (the threshold is such a midpoint).
+// Quotients of magnitude at most 2^-125 take the division itself.  Zero, infinite and NaN
+// operands give what the division gives (NaN payloads aside).  One conversion, one double
+// multiplication and one rounding instead of the division's scale, reciprocal and fixup
+// sequence: for a divisor shared by many quotients.
+__host__ __device__ inline float div_recip(float x, float d, double r) {
+    float q = (float)((double)x * r);
+    if (fabsf(q) <= 0x1p-125f && x != 0.0f) {
+        asm volatile("");   // a branch taken by almost no lane, not a division for every lane
+        q = x / d;
+    }
+    return q;
+}
+
 // int64 -> float, round to nearest even, from integer operations only.
 __host__ __device__ inline float int64_to_float_rne(int64_t q) {
     if (q == 0) return 0.0f;
@@ -347,6 +365,12 @@ struct Ctx {
     // columns per lane of the Hamming cost kernel (4 or 8; the wider form writes 512 instead of
     // 256 contiguous bytes per wavefront store)
     int hamming_cols = 4;
+    // pixels per segment wavefront of the plane-cost segment form (a multiple of 64 up to 512):
+    // shorter segments mean more, shorter wavefronts per launch
+    int seg_len = 512;
+    // batched BP: 1 captures the whole schedule of rounds in one HIP graph and replays it, 0
+    // queues the rounds one by one from the host (the same launches, the same bits)
+    int batch_graph = 0;
     // per BP level: -1 the threshold above decides, 0 chunks of sorted pixels, 1 per-superpixel
     // segments (a schedule of forms the tuner can set level by level; the bits are the same)
     static constexpr int kFormLevels = 24;

@@ -117,6 +117,16 @@ void bp_run_batch(Ctx &c, const std::vector<BP *> &bps);
 struct SweepArgs;
 struct BatchItem;
 constexpr int kBatchLevels = 24;   // pyramid levels a slot holds (N < 2^24 halves at least per level)
+// one round's launches: its items (segment-form ones first), the launches' block counts and
+// the level inits that open it
+struct RoundPlan {
+    std::vector<BatchItem> items;
+    int ncoarse = 0, nfine = 0;
+    int coarse_blocks = 0, fine_blocks = 0, node_blocks = 0, sp_blocks = 0, msg_blocks = 0;
+    // algorithmic HBM bytes of each launch (the sums of the one-pair launches' figures)
+    double coarse_bytes = 0.0, fine_bytes = 0.0, node_bytes = 0.0, sp_bytes = 0.0, msg_bytes = 0.0;
+    std::vector<std::pair<BP *, int>> inits;   // (object, level)
+};
 struct BatchRunner {
     int nslots = 0, mode = -1;
     DevBuf<uint8_t> table;         // [slots][kBatchLevels] SweepArgs, pointers relative to the table
@@ -138,6 +148,10 @@ struct BatchRunner {
     int add(Ctx &c, BP &b);        // a free slot holding b's levels (b set up, same cost form)
     void remove(int slot);
     void round(Ctx &c, const std::vector<std::pair<BP *, int>> &active);   // (object, slot)
+    // the next round of the objects (advancing their sweep positions); false when none is left
+    bool plan(const std::vector<std::pair<BP *, int>> &active, RoundPlan &rp) const;
+    // its launches, with the items already on the device
+    void launch(Ctx &c, const RoundPlan &rp, const BatchItem *d_items) const;
 };
 void bp_get_state(Ctx &c, BP &b, float *planes, float *cost, float *belief, float *msg);
 int bp_total_sweeps(int levels, int min_level, int D);

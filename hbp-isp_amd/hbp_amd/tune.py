@@ -8,7 +8,9 @@ cluster sums are exact integers in any order), so tuning only trades speed:
     schedule of forms level by level);
   * single_hb / batch_hb: steps per batch of a segment wavefront (2 or 4) in one-pair and in
     batched launches: 4 keeps more gathers in flight, 2 fewer registers and more wavefronts;
-  * hamming_cols: columns per lane of the Hamming cost kernel (4 or 8).
+  * seg_len: pixels per segment wavefront of the segment form (a multiple of 64 up to 512);
+  * hamming_cols: columns per lane of the Hamming cost kernel (4 or 8);
+  * batch_graph: batched rounds queued one by one or replayed from one captured HIP graph.
 The reference's kernels have one form each; this module has no counterpart there.
 """
 import itertools

@@ -57,7 +57,10 @@ int hbp_profile_copy(hbp_ctx *ctx, long long bytes, int launches);
  *      costs planes in the per-superpixel segment form rather than in chunks; "single_hb",
  *      "batch_hb" (2 or 4; defaults 4, 2): steps per batch of the segment wavefronts in one-pair
  *      and in batched launches; "hamming_cols" (4 or 8, default 4): columns per lane of the
- *      Hamming cost kernel; "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
+ *      Hamming cost kernel; "seg_len" (a multiple of 64 in [64, 512], default 512): pixels per
+ *      segment wavefront of the segment form; "batch_graph" (0 or 1, default 0): hbp_bp_run_batch queues its
+ *      rounds one by one from the host (0) or captures the whole schedule of rounds in one HIP
+ *      graph and replays it (1); "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
  *      planes by coarse_min_pix's rule, in chunks of sorted pixels, or in per-superpixel
  *      segments.  Applies to BP objects created afterwards (and to the batched
  *      rounds run afterwards).  hbp_amd.tune picks them by timing the caller's workload. */

@@ -1,8 +1,8 @@
 """GPU parity at the benchmark workloads' own sizes:
   * the 1472x984, D=256 pair through all 1754 sweeps of its BP (levels 6..9), against the oracle;
   * the batch of 64 pairs of mixed sizes (W U[320,1920], H U[240,1080], D from {64,128,192,256})
-    through hbp_bp_run_batch, every pair bit-identical to its own plain run, the two smallest
-    also against the oracle;
+    through hbp_bp_run_batch, with its rounds queued and replayed from one HIP graph, every pair
+    bit-identical to its own plain run, the two smallest also against the oracle;
   * the request server under Poisson arrivals of the serving mix (40% 640x480/D64, 40%
     1280x720/D128, 20% 1920x1080/D256), 120 requests, every result checked against the plain
     path;
@@ -64,15 +64,20 @@ def test_batch_of_64_mixed_pairs(ctx):
     for (sd, w, h, d), (left, right) in zip(specs, pairs):
         (dg, pg), ml, _ = _plain(ctx, left, right, d, 2)
         ref.append((dg, pg, ml))
-    say("batch of 64 mixed pairs: batched run")
-    bps = []
-    for (sd, w, h, d), (left, right), (_, _, ml) in zip(specs, pairs, ref):
-        bps.append(ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, d), min_level=ml))
-    ctx.bp_run_batch(bps)
-    for i, bp in enumerate(bps):
-        dg, pg = bp.run()
-        assert same_bits(dg, ref[i][0]) and same_bits(pg, ref[i][1]), f"pair {i} {specs[i]}"
-        bp.free()
+    for graph in (0, 1):
+        say(f"batch of 64 mixed pairs: batched run, rounds {'in one HIP graph' if graph else 'queued'}")
+        ctx.set_tuning("batch_graph", graph)
+        try:
+            bps = []
+            for (sd, w, h, d), (left, right), (_, _, ml) in zip(specs, pairs, ref):
+                bps.append(ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, d), min_level=ml))
+            ctx.bp_run_batch(bps)
+        finally:
+            ctx.set_tuning("batch_graph", 0)
+        for i, bp in enumerate(bps):
+            dg, pg = bp.run()
+            assert same_bits(dg, ref[i][0]) and same_bits(pg, ref[i][1]), f"pair {i} {specs[i]} graph {graph}"
+            bp.free()
     small = sorted(range(64), key=lambda i: specs[i][1] * specs[i][2])[:2]
     for i in small:
         sd, w, h, d = specs[i]
