@@ -919,7 +919,6 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
         X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
         float c0 = a.c0[e];
         float dpos = a.dpos[e];
-        const double rdpos = 1.0 / (double)dpos;
         float4 psl = (spatial && y >= 2 * P) ? spatial_candidate(a, s, y - 2 * P) : a.plane[(long)s * K + y];
         float4 m0 = a.msg[2 * e], m1 = a.msg[2 * e + 1];
         float mh[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
@@ -938,7 +937,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
             float phi = fabsf(dpx * X[0] + dpy * Y[0] + dpz);
 #pragma unroll
             for (int j = 1; j < 6; ++j) phi = phi + fabsf(dpx * X[j] + dpy * Y[j] + dpz);
-            phi = fminf(div_recip(div3_nonneg(phi), dpos, rdpos), a.tau_s);
+            phi = fminf(div3_nonneg(phi) / dpos, a.tau_s);
             float v = m + c0 * phi;
             if (v < minimum) minimum = v;
         }
@@ -949,7 +948,7 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
     const int g = grp * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) mean += __shfl(out, g + k, 64);
-    mean = div_recip(mean, (float)K, 1.0 / K);
+    mean /= K;
     if (live) a.msg_t[mslot * K + y] = out - mean;
 }
 

@@ -121,24 +121,6 @@ __host__ __device__ inline float div3_nonneg(float x) {
     return fmaf(r, y, q0);
 }
 
-// rn(x / d) from r = rn64(1 / (double)d): the double product x r lies within 2^-52 (relative)
-// of x / d, and a quotient of two floats that is a normal float lies at least 2^-49 (relative)
-// from every midpoint of two consecutive floats (a midpoint has a 25-bit odd significand, and
-// x - m d is then a nonzero multiple of m's last bit times d's), so rounding the product gives
-// the quotient's own rounding; overflow rounds alike (the threshold is such a midpoint).
-// Quotients of magnitude at most 2^-125 take the division itself.  Zero, infinite and NaN
-// operands give what the division gives (NaN payloads aside).  One conversion, one double
-// multiplication and one rounding instead of the division's scale, reciprocal and fixup
-// sequence: for a divisor shared by many quotients.
-__host__ __device__ inline float div_recip(float x, float d, double r) {
-    float q = (float)((double)x * r);
-    if (fabsf(q) <= 0x1p-125f && x != 0.0f) {
-        asm volatile("");   // a branch taken by almost no lane, not a division for every lane
-        q = x / d;
-    }
-    return q;
-}
-
 // int64 -> float, round to nearest even, from integer operations only.
 __host__ __device__ inline float int64_to_float_rne(int64_t q) {
     if (q == 0) return 0.0f;

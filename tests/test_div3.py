@@ -1,7 +1,5 @@
-"""The message kernel's divisions (hbp_common.h): div3_nonneg equals x / 3 for every
-non-negative float (exhaustive), and div_recip(x, d, 1 / (double)d) equals x / d for every
-non-negative x over a set of divisors and for 2^28 random pairs of arbitrary floats; on the
-host (both functions are __host__ __device__)."""
+"""The message kernel's division by 3 (hbp_common.h div3_nonneg) equals x / 3 for every
+non-negative float: exhaustive, on the host (the function is __host__ __device__)."""
 import os
 import shutil
 import subprocess
@@ -25,16 +23,3 @@ def test_div3_nonneg_exhaustive():
         r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
         assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
 
-
-@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
-                    reason="hipcc (HIP headers) not available")
-def test_div_recip_matches_the_division():
-    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    with tempfile.TemporaryDirectory() as d:
-        exe = os.path.join(d, "div_recip_check")
-        subprocess.run([hipcc, "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off",
-                        "-I", os.path.join(ROOT, "hbp-isp_amd", "csrc"),
-                        os.path.join(ROOT, "tests", "div_recip_check.cpp"), "-o", exe], check=True,
-                       capture_output=True)
-        r = subprocess.run([exe], capture_output=True, text=True, timeout=900)
-        assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
