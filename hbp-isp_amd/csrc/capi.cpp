@@ -196,6 +196,56 @@ int hbp_profile_copy(hbp_ctx *ctx, long long bytes, int launches) {
     HBP_API_END
 }
 
+int hbp_primitive_sort(hbp_ctx *ctx, void *keys, int key_bytes, uint32_t *vals, long long n, int end_bit) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && keys && n >= 0, "bad argument");
+    HBP_REQUIRE(key_bytes == 4 || key_bytes == 8, "key_bytes must be 4 or 8");
+    HBP_REQUIRE(end_bit >= 0 && end_bit <= 8 * key_bytes, "end_bit out of range");
+    HBP_REQUIRE(n < (1LL << 31), "sort too long");
+    if (n == 0) return 0;
+    const int kb = key_bytes;
+    const uint64_t high = end_bit >= 64 ? 0ull : ~((1ull << end_bit) - 1ull);
+    for (long long i = 0; i < n; ++i) {
+        const uint64_t k = kb == 8 ? ((const uint64_t *)keys)[i] : ((const uint32_t *)keys)[i];
+        HBP_REQUIRE((k & high) == 0, "keys hold bits at or above end_bit");
+    }
+    use_device(ctx);
+    Ctx &c = ctx->c;
+    DevBuf<uint8_t> ka(c.pool, (size_t)n * kb), kt(c.pool, (size_t)n * kb);
+    DevBuf<uint32_t> va(c.pool, (size_t)n), vt(c.pool, (size_t)n);
+    HBP_CHECK(hipMemcpyAsync(ka.p, keys, (size_t)n * kb, hipMemcpyHostToDevice, c.stream));
+    if (vals) HBP_CHECK(hipMemcpyAsync(va.p, vals, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, c.stream));
+    bool flipped;
+    if (kb == 4)
+        flipped = sort_pairs_u32(c, (unsigned int *)ka.p, (unsigned int *)kt.p, (int *)va.p, (int *)vt.p, (long)n,
+                                 end_bit);
+    else if (vals)
+        flipped = sort_pairs_u64(c, (unsigned long long *)ka.p, (unsigned long long *)kt.p, va.p, vt.p, (long)n,
+                                 end_bit);
+    else
+        flipped = sort_keys_u64(c, (unsigned long long *)ka.p, (unsigned long long *)kt.p, (long)n, end_bit);
+    HBP_CHECK(hipMemcpyAsync(keys, flipped ? kt.p : ka.p, (size_t)n * kb, hipMemcpyDeviceToHost, c.stream));
+    if (vals)
+        HBP_CHECK(hipMemcpyAsync(vals, flipped ? vt.p : va.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                                 c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    HBP_API_END
+}
+
+int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long n) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && in && out && n >= 0, "bad argument");
+    if (n == 0) return 0;
+    use_device(ctx);
+    Ctx &c = ctx->c;
+    DevBuf<int> a(c.pool, (size_t)n), b(c.pool, (size_t)n);
+    HBP_CHECK(hipMemcpyAsync(a.p, in, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, c.stream));
+    scan_exclusive_int(c, a.p, b.p, (long)n, nullptr);
+    HBP_CHECK(hipMemcpyAsync(out, b.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c.stream));
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    HBP_API_END
+}
+
 int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every) {
     HBP_API_BEGIN
     HBP_REQUIRE(ctx, "ctx is NULL");

@@ -37,7 +37,7 @@ namespace hbp {
 struct Request {
     long long id = 0;
     int W = 0, H = 0, D = 0, min_level = 2;
-    int coarse_min_pix = 96, single_hb = 4, batch_hb = 2, hamming_cols = 4;   // the serving context's forms
+    int coarse_min_pix = 96, single_hb = 4, batch_hb = 2, hamming_cols = 4, seg_len = 512;   // the serving context's forms
     int level_form[Ctx::kFormLevels] = {};
     std::vector<uint8_t> left, right;
     size_t est = 0;           // device bytes held while on the device (without the u8 volume)
@@ -130,6 +130,7 @@ static void admit_work(Server &s, Request &r) {
     c.single_hb = r.single_hb;
     c.batch_hb = r.batch_hb;
     c.hamming_cols = r.hamming_cols;
+    c.seg_len = r.seg_len;
     std::memcpy(c.level_form, r.level_form, sizeof(c.level_form));
     const size_t n3 = (size_t)r.W * r.H * 3;
     {
@@ -313,6 +314,7 @@ static void step(Server &s, int *n_active, int *n_pending, int *n_done) {
             rp->single_hb = s.c->single_hb;
             rp->batch_hb = s.c->batch_hb;
             rp->hamming_cols = s.c->hamming_cols;
+            rp->seg_len = s.c->seg_len;
             std::memcpy(rp->level_form, s.c->level_form, sizeof(rp->level_form));
             {
                 std::lock_guard<std::mutex> lk(s.mu);

@@ -89,6 +89,8 @@ SIGNATURES = {
     "hbp_server_set_max_pending": (C.c_int, [_vp, C.c_int]),
     "hbp_server_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "hbp_profile_copy": (C.c_int, [_vp, C.c_longlong, C.c_int]),
+    "hbp_primitive_sort": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_longlong, C.c_int]),
+    "hbp_primitive_scan": (C.c_int, [_vp, _vp, _vp, C.c_longlong]),
     "hbp_inject_fault": (C.c_int, [C.c_char_p, C.c_longlong]),
     "hbp_bp_dev_disp": (_vp, [_vp]),
     "hbp_bp_dev_plane": (_vp, [_vp]),

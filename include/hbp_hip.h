@@ -51,6 +51,15 @@ int hbp_profile_only(hbp_ctx *ctx, const char *prefix, int every);
  * counters are calibrated on (bench.py --traffic-calibration) */
 int hbp_profile_copy(hbp_ctx *ctx, long long bytes, int launches);
 
+/* ---- the device primitives on host arrays, for their tests (no counterpart in the reference,
+ *      whose edge sorts and scans are Thrust calls: segment.cu:1058-1085, CRTrees.cu:395).
+ *      hbp_primitive_sort: the project's stable LSD radix sort of n keys (key_bytes 4 or 8) on
+ *      bits [0, end_bit), in place, vals (n u32, may be NULL for 8-byte keys) permuted with them;
+ *      keys with bits at or above end_bit are refused.  hbp_primitive_scan: out[i] = in[0] + ...
+ *      + in[i-1] (int32; the sums must fit). */
+int hbp_primitive_sort(hbp_ctx *ctx, void *keys, int key_bytes, uint32_t *vals, long long n, int end_bit);
+int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long n);
+
 /* ---- kernel-form tuning of a context (no counterpart in the reference: its kernels have one
  *      form each).  Every value gives the same results bit for bit; only the speed changes.
  *      "coarse_min_pix" (>= 1, default 96): average pixels per superpixel from which a BP level
