@@ -579,8 +579,7 @@ def served_vs_plain(ctx, pool, rate, seconds, seed, min_level=2):
                 if r is not None:
                     i = tickets.pop(tk)
                     done_at[i] = now
-                    if i < 4:
-                        outputs[i] = r
+                    outputs[i] = _digest(r)
     srv.close()
     lat_s = sorted(d - a for d, a in zip(done_at, arrivals))
     span_s = max(done_at) - arrivals[0]
@@ -606,12 +605,10 @@ def served_vs_plain(ctx, pool, rate, seconds, seed, min_level=2):
         lib.hbp_pyramid_free(ctx.h, pyr)
         lib.hbp_volume_free(ctx.h, vol)
         done_p.append(time.perf_counter() - t0)
-        if i < 4:
-            ref[i] = (disp, plane)
+        ref[i] = _digest((disp, plane))
     lat_p = sorted(d - a for d, a in zip(done_p, arrivals))
     span_p = max(done_p) - arrivals[0]
-    same = all(np.array_equal(ref[i][0].view(np.uint32), outputs[i][0].view(np.uint32)) and
-               np.array_equal(ref[i][1].view(np.uint32), outputs[i][1].view(np.uint32)) for i in ref)
+    differ = sum(1 for i in range(n) if outputs.get(i) != ref[i])
 
     def pct(v, q):
         return round(1e3 * v[min(len(v) - 1, int(round(q * (len(v) - 1))))], 1)
@@ -621,7 +618,17 @@ def served_vs_plain(ctx, pool, rate, seconds, seed, min_level=2):
             "requests": n,
             "served": {"pairs_per_s": round(n / span_s, 3), "p50_ms": pct(lat_s, 0.5), "p99_ms": pct(lat_s, 0.99)},
             "plain": {"pairs_per_s": round(n / span_p, 3), "p50_ms": pct(lat_p, 0.5), "p99_ms": pct(lat_p, 0.99)},
-            "bit_identical_first_4": bool(same)}
+            "results_not_identical": differ,
+            "checked": "every request: a 64-bit xxh3 digest of its disparity and planes, served vs plain"}
+
+
+def _digest(out):
+    """64-bit digest of a (disparity, planes) result's bytes"""
+    import xxhash
+    h = xxhash.xxh3_64()
+    for a in out:
+        h.update(memoryview(a).cast("B"))
+    return h.intdigest()
 
 
 def _progress(msg):
