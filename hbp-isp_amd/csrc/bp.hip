@@ -495,7 +495,8 @@ struct PcPix {
 };
 struct PcWords {
     uint32_t u0, u1, w0, w1;
-    float tf;   // fast form: df - floor(df) when floor(df) is in [0, D-2], else -1
+    float tf;   // fast form: df - floor(df)
+    bool in;    // fast form: floor(df) is in [0, D-2]
 };
 
 // x, y < 2^16 and y W + x < 2^24: 24-bit multiplies are exact
@@ -529,7 +530,10 @@ __device__ __forceinline__ void pc_load(const SweepArgs &a, const PcPix &px, flo
     // GEN: NaN -> 0, +-inf -> the ends
     const float dcf = GEN ? fminf(fmaxf(fl, 0.f), dmax) : __builtin_amdgcn_fmed3f(fl, 0.f, dmax);
     const int dc = (int)dcf;
-    if (!GEN) wd.tf = fl == dcf ? df - fl : -1.f;
+    if (!GEN) {
+        wd.tf = df - fl;
+        wd.in = fl == dcf;
+    }
     if (MODE == kCostLeft) {
         const Words2 w = ld_at<Words2>(a.crx, (px.pb + (uint32_t)dc) << 2);
         wd.u0 = w.lo;
@@ -570,7 +574,7 @@ __device__ __forceinline__ float pc_term(const SweepArgs &a, const PcPix &px, fl
         const float df = pl.x * px.fx + pl.y * px.fy + pl.z;
         cost = fabsf(px.dw - df) <= 1.f ? 0.f : 1.f;
     } else if (!GEN) {
-        const bool in = wd.tf >= 0.f;
+        const bool in = wd.in;
         float v0, v1;
         if (MODE == kCostF32) {
             v0 = __uint_as_float(wd.u0);
@@ -1566,13 +1570,23 @@ void bp_setup(Ctx &c, BP &b) {
         HBP_CHECK(hipGetLastError());
     }
     // node-major message slots: the out-degree bound of every BP level
-    std::vector<int> hmax(b.L, 0);
+    std::vector<int> hmax(b.L, 0), hseg(b.L, 0);
     HBP_CHECK(hipMemcpyAsync(hmax.data(), dmax.p, sizeof(int) * (size_t)b.L, hipMemcpyDeviceToHost, c.stream));
+    // coarse levels: the segment count (the scan's last entry), so that a launch covers the
+    // segments alone and no padding lands on one XCD
+    for (int l = b.min_level; l < b.L; ++l)
+        if (b.lv[l].coarse)
+            HBP_CHECK(hipMemcpyAsync(&hseg[l], b.lv[l].sego.p + b.lv[l].V, sizeof(int), hipMemcpyDeviceToHost,
+                                     c.stream));
     HBP_CHECK(hipStreamSynchronize(c.stream));
     size_t slots = 1;
     for (int l = b.min_level; l < b.L; ++l) {
         BPLevel &B = b.lv[l];
         B.maxdeg = std::max(1, hmax[l]);
+        if (B.coarse) {
+            HBP_REQUIRE(hseg[l] >= 1 && hseg[l] <= B.nseg_max, "segment count out of range");
+            B.nseg = hseg[l];
+        }
         slots = std::max(slots, (size_t)B.V * B.maxdeg);
         B.eloc.alloc(c.pool, std::max<long>(B.E, 1));
         B.revl.alloc(c.pool, (size_t)B.V * B.maxdeg + kNodeMaxDeg);
@@ -1639,7 +1653,7 @@ static void launch_cost_edge(Ctx &c, const SweepArgs &a, const BPLevel &B, int l
     // budget keeps twice the waves resident.
     if (B.coarse && costs) {
         double bytes = (costs ? plane_cost_bytes(a.N, a.V, NS) : 0.0) + (with_edges ? edge_bytes(a.E, a.V) : 0.0);
-        int npc = costs ? B.nseg_max : 0;
+        int npc = costs ? B.nseg : 0;
         long blocks = div_up(npc, 4) + div_up(nedge_w, 4);
         if (blocks == 0) return;
         ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes, true);
@@ -1818,7 +1832,7 @@ bool BatchRunner::plan(const std::vector<std::pair<BP *, int>> &active, RoundPla
         }
         it.coarse = B.coarse ? 1 : 0;
         it.spatial = sw.kind == kSpatial ? 1 : 0;
-        it.npc = sw.first ? 0 : (B.coarse ? B.nseg_max : div_up(b.N, CHUNK));
+        it.npc = sw.first ? 0 : (B.coarse ? B.nseg : div_up(b.N, CHUNK));
         it.pc_blocks = B.coarse ? div_up(it.npc, 4) : it.npc;
         // coarse levels: the messages share the segments' launch; fine levels: their own
         const int eb = (int)div_up(div_up(B.E, EDGES_PER_WAVE), 4);

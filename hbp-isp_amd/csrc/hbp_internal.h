@@ -43,7 +43,8 @@ struct BPLevel {
     DevBuf<uint32_t> pxy, pcl;     // pixels sorted by (cluster, pixel id): packed x|y<<16, cluster [N]
     DevBuf<int> pix_off;           // [V+1]
     bool coarse = false;           // plane costs by per-superpixel segments (else by chunks)
-    int nseg_max = 0;              // coarse: upper bound of the segment count (N/512 + V)
+    int nseg_max = 0;              // coarse: upper bound of the segment count (N/seg_len + V)
+    int nseg = 0;                  // coarse: the segment count (read back at set-up)
     DevBuf<int4> seg;              // coarse: (first pixel, end pixel, cluster, index in cluster)
     DevBuf<int> sego;              // coarse: first segment of each superpixel; sego[V] = count
     DevBuf<int> off;               // CSR offsets by source [V+1]
