@@ -27,6 +27,8 @@ HBM_PEAK_GBS = 8000.0
 # with 1754 launches per pair ten steps still time about 600 launches while the events' own
 # cost (about 8 us per sampled launch) stays near 1 % of the step
 EVENT_EVERY = 29
+# host threads (each with a stream of its own) that build the batched leg's pairs
+PREP_WORKERS = 8
 METRIC = "stereo pairs/s + Mpix·disp/s; p50/p99 latency, batched/served vs plain"
 WORKLOADS = {
     "large": dict(width=1472, height=984, ndisp=256, n_planes=(20, 40), bp_levels=4,
@@ -450,18 +452,30 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
 
     split = {}
 
-    def batched(keep):
+    def batched(keep, workers):
         t0 = time.perf_counter()
-        objs = [prepare(spec, dl, dr) for spec, dl, dr in dev]
+        if workers > 1:   # the pairs built by several host threads on streams of their own
+            objs = ctx.prepare_pairs_dev([(dl.value, dr.value, spec[1], spec[2], spec[3], min_level)
+                                          for spec, dl, dr in dev], workers=workers)
+            objs = [(v, p, b) for v, p, b in objs]
+        else:
+            objs = [prepare(spec, dl, dr) for spec, dl, dr in dev]
         ctx.sync()
         t1 = time.perf_counter()
-        arr = (C.c_void_p * len(objs))(*[o[2] for o in objs])
+        hs = [o[2].h if workers > 1 else o[2] for o in objs]
+        arr = (C.c_void_p * len(objs))(*hs)
         H._check(lib.hbp_bp_run_batch(ctx.h, len(objs), arr))
         ctx.sync()
         split["prepare_ms"] = round((t1 - t0) * 1e3, 1)
         split["bp_rounds_ms"] = round((time.perf_counter() - t1) * 1e3, 1)
-        outs = [download(o[2], spec) for o, (spec, _, _) in zip(objs, dev)] if keep else []
-        release(objs)
+        outs = [download(h, spec) for h, (spec, _, _) in zip(hs, dev)] if keep else []
+        if workers > 1:
+            for v, p, b in objs:
+                b.free()
+                p.free()
+                v.free()
+        else:
+            release(objs)
         return outs
 
     # each form once untimed first (the context's memory pool then holds its buffers)
@@ -471,28 +485,30 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
     plain(False)
     ctx.sync()
     t_plain = time.perf_counter() - t0
-    # the batched rounds queued from the host, then captured in one HIP graph and replayed
+    # the pairs prepared one after the other on the context's stream or by PREP_WORKERS host
+    # threads; the batched rounds queued from the host or captured in one HIP graph
     graph0 = ctx.get_tuning("batch_graph")
+    forms = [(1, 0), (1, 1), (PREP_WORKERS, graph0)]
     t_form, split_form = {}, {}
-    for g in (0, 1):
+    for wk, g in forms:
         ctx.set_tuning("batch_graph", g)
-        batched(False)
+        batched(False, wk)
         ctx.sync()
         t0 = time.perf_counter()
-        batched(False)
+        batched(False, wk)
         ctx.sync()
-        t_form[g] = time.perf_counter() - t0
-        split_form[g] = dict(split)
+        t_form[(wk, g)] = time.perf_counter() - t0
+        split_form[(wk, g)] = dict(split)
     # bit-identical outputs (outside the timed runs)
     ref = plain(True)
     same = {}
-    for g in (0, 1):
+    for wk, g in forms:
         ctx.set_tuning("batch_graph", g)
-        got = batched(True)
-        same[g] = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
-                      np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) for a, b in zip(ref, got))
+        got = batched(True, wk)
+        same[(wk, g)] = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
+                            np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) for a, b in zip(ref, got))
     ctx.set_tuning("batch_graph", graph0)
-    t_batch = t_form[graph0]
+    t_batch = t_form[(PREP_WORKERS, graph0)]
     for _, dl, dr in dev:
         hip.hipFree(dl)
         hip.hipFree(dr)
@@ -502,15 +518,19 @@ def batched_vs_plain(ctx, hip, pairs, min_level=2):
                         f"6-12 planes, min_level 2, reference parameters; images resident in HBM",
             "pairs": n, "plain_pairs_per_s": round(n / t_plain, 3), "batched_pairs_per_s": round(n / t_batch, 3),
             "speedup": round(t_plain / t_batch, 3), "plain_mpix_disp_per_s": round(mpx / t_plain, 1),
-            "batched_mpix_disp_per_s": round(mpx / t_batch, 1), "bit_identical": bool(same[0] and same[1]),
+            "batched_mpix_disp_per_s": round(mpx / t_batch, 1), "bit_identical": bool(all(same.values())),
+            "prepare": f"hbp_pairs_prepare_dev, {PREP_WORKERS} host threads on streams of their own",
             "rounds": "in one HIP graph" if graph0 else "queued from the host",
-            "rounds_ab": {"queued_pairs_per_s": round(n / t_form[0], 3), "graph_pairs_per_s": round(n / t_form[1], 3),
-                          "graph_vs_queued": round(t_form[0] / t_form[1], 3),
-                          "split_ms": {"queued": split_form[0], "graph": split_form[1]},
-                          "bit_identical": {"queued": bool(same[0]), "graph": bool(same[1])},
-                          "graph_timed": "capture of every round, instantiation, one replay, synchronisation"},
-            "timed": "plain: cost volume + pyramid + BP + map per pair in turn; batched: every pair's cost volume "
-                     "and pyramid, then hbp_bp_run_batch over all pairs"}
+            "split_ms": split_form[(PREP_WORKERS, graph0)],
+            "ab": {"one_stream_prepare_queued_rounds": {"pairs_per_s": round(n / t_form[(1, 0)], 3),
+                                                        "split_ms": split_form[(1, 0)],
+                                                        "bit_identical": bool(same[(1, 0)])},
+                   "one_stream_prepare_graph_rounds": {"pairs_per_s": round(n / t_form[(1, 1)], 3),
+                                                       "split_ms": split_form[(1, 1)],
+                                                       "bit_identical": bool(same[(1, 1)])},
+                   "graph_timed": "capture of every round, instantiation, one replay, synchronisation"},
+            "timed": "plain: cost volume + pyramid + BP + map per pair in turn; batched: every pair's cost volume, "
+                     "pyramid and BP set-up, then hbp_bp_run_batch over all pairs"}
 
 
 SERVE_MIX = ((0.4, 640, 480, 64), (0.4, 1280, 720, 128), (0.2, 1920, 1080, 256))

@@ -4,6 +4,8 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
+#include <vector>
 
 #include "../../include/hbp_hip.h"
 #include "hbp_internal.h"
@@ -107,6 +109,17 @@ int hbp_inject_fault(const char *what, long long after) {
     HBP_API_END
 }
 
+// a context's stream, pools and pinned words, after its queued work
+static void ctx_teardown(Ctx &c) {
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    c.scratch.release();
+    c.counters.release();
+    c.free_host_box();
+    c.pool.release_all();
+    HBP_CHECK(hipStreamDestroy(c.stream));
+    c.stream = nullptr;
+}
+
 int hbp_ctx_create(int device, hbp_ctx **out) {
     HBP_API_BEGIN
     HBP_REQUIRE(out, "out is NULL");
@@ -134,12 +147,9 @@ int hbp_ctx_destroy(hbp_ctx *ctx) {
                                     " volume, pyramid, BP or server handles of this context are still alive: "
                                     "free them before the context");
     use_device(ctx);
-    HBP_CHECK(hipStreamSynchronize(ctx->c.stream));
-    ctx->c.scratch.release();
-    ctx->c.counters.release();
-    ctx->c.free_host_box();
-    ctx->c.pool.release_all();
-    HBP_CHECK(hipStreamDestroy(ctx->c.stream));
+    for (auto &h : ctx->helpers) ctx_teardown(*h);
+    ctx->helpers.clear();
+    ctx_teardown(ctx->c);
     delete ctx;
     HBP_API_END
 }
@@ -642,6 +652,119 @@ int hbp_bp_create(hbp_ctx *ctx, const hbp_pyramid *pyr, const hbp_volume *vol, f
         throw;
     }
     *out = adopt_handle(ctx, b);
+    HBP_API_END
+}
+
+// one pair's cost volume, pyramid and BP object on context c (its stream synchronised after)
+struct PreparedPair {
+    std::unique_ptr<hbp_volume> vol;
+    std::unique_ptr<hbp_pyramid> pyr;
+    std::unique_ptr<hbp_bp> bp;
+};
+static void prepare_pair(Ctx &c, const hbp_pair_dev &q, float alpha, float tau_s, int use_cost_vol,
+                         PreparedPair &out) {
+    PreparedPair r;
+    r.vol.reset(new hbp_volume());
+    volume_build(c, r.vol->v, q.left, q.right, q.width, q.height, q.labels);
+    r.pyr.reset(new hbp_pyramid());
+    pyramid_build(c, r.pyr->p, q.left, q.width, q.height, q.num_nb, q.linkage, 1, 0.0);
+    const int L = r.pyr->p.levels();
+    HBP_REQUIRE(L >= 2, "image too small for a pyramid of two levels");
+    const int ml = q.min_level < 0 ? L - 1 + q.min_level : q.min_level;
+    r.bp.reset(new hbp_bp());
+    BP &b = r.bp->b;
+    b.pyr = &r.pyr->p;
+    b.vol = &r.vol->v;
+    b.lambda = alpha;
+    b.tau_s = tau_s;
+    b.min_level = std::max(1, std::min(ml, L - 1));
+    b.use_cost_vol = use_cost_vol ? 1 : 0;
+    bp_setup(c, b);
+    HBP_CHECK(hipStreamSynchronize(c.stream));
+    out = std::move(r);
+}
+
+int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float alpha, float tau_smooth,
+                          int use_cost_vol, int workers, hbp_volume **vols, hbp_pyramid **pyrs, hbp_bp **bps) {
+    HBP_API_BEGIN
+    HBP_REQUIRE(ctx && n >= 0 && (n == 0 || (pairs && vols && pyrs && bps)), "NULL argument");
+    HBP_REQUIRE(workers >= 1 && workers <= 16, "workers must be in [1, 16]");
+    HBP_REQUIRE(std::isfinite(alpha) && std::isfinite(tau_smooth), "alpha and tau_smooth must be finite");
+    for (int i = 0; i < n; ++i) {
+        vols[i] = nullptr;
+        pyrs[i] = nullptr;
+        bps[i] = nullptr;
+        const hbp_pair_dev &q = pairs[i];
+        HBP_REQUIRE(q.left && q.right, "NULL image in a pair");
+        HBP_REQUIRE(q.width > 0 && q.height > 0 && q.labels >= 1, "bad pair size");
+        check_seg_args(q.linkage, q.num_nb, 0.0);
+    }
+    if (n == 0) return 0;
+    use_device(ctx);
+    Ctx &m = ctx->c;
+    // the caller's queued work (e.g. the images' copies) first
+    HBP_CHECK(hipStreamSynchronize(m.stream));
+    while ((int)ctx->helpers.size() < workers) {
+        auto h = std::make_unique<Ctx>();
+        h->device = m.device;
+        h->num_cu = m.num_cu;
+        h->coop_blocks = m.coop_blocks;
+        h->coop_forbidden = true;
+        HBP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        ctx->helpers.push_back(std::move(h));
+    }
+    const int nw = std::min(workers, n);
+    for (int w = 0; w < nw; ++w) {   // the context's kernel forms
+        Ctx &h = *ctx->helpers[w];
+        h.coarse_min_pix = m.coarse_min_pix;
+        h.single_hb = m.single_hb;
+        h.batch_hb = m.batch_hb;
+        h.hamming_cols = m.hamming_cols;
+        h.seg_len = m.seg_len;
+        std::memcpy(h.level_form, m.level_form, sizeof(h.level_form));
+    }
+    std::vector<PreparedPair> done(n);
+    std::vector<char> coop(n, 0);
+    std::vector<std::string> err(nw);
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    auto work = [&](int w) {
+        if (hipSetDevice(m.device) != hipSuccess) {
+            err[w] = "hipSetDevice failed in a worker";
+            failed = true;
+            return;
+        }
+        Ctx &h = *ctx->helpers[w];
+        for (int i = next++; i < n && !failed; i = next++) {
+            try {
+                prepare_pair(h, pairs[i], alpha, tau_smooth, use_cost_vol, done[i]);
+            } catch (const CoopNeeded &) {
+                (void)hipStreamSynchronize(h.stream);
+                coop[i] = 1;   // redone below, where a cooperative launch is safe
+            } catch (const HipError &e) {
+                (void)hipStreamSynchronize(h.stream);
+                err[w] = e.msg;
+                failed = true;
+            } catch (const std::bad_alloc &) {
+                (void)hipStreamSynchronize(h.stream);
+                err[w] = "host allocation failed";
+                failed = true;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nw; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto &t : th) t.join();
+    for (int w = 0; w < nw; ++w) HBP_REQUIRE(err[w].empty(), err[w]);   // done[] frees what was built
+    // pairs whose CRTrees needed the cooperative form: on the caller's context, nothing else queued
+    for (int i = 0; i < n; ++i)
+        if (coop[i]) prepare_pair(m, pairs[i], alpha, tau_smooth, use_cost_vol, done[i]);
+    for (int i = 0; i < n; ++i) {
+        vols[i] = adopt_handle(ctx, done[i].vol.release());
+        pyrs[i] = adopt_handle(ctx, done[i].pyr.release());
+        bps[i] = adopt_handle(ctx, done[i].bp.release());
+    }
     HBP_API_END
 }
 

@@ -1,12 +1,18 @@
 // Opaque handle types of the C ABI and the error-catching wrapper of every entry point.
 #pragma once
+#include <memory>
 #include <new>
+#include <vector>
 
 #include "hbp_internal.h"
 
 struct hbp_ctx {
     hbp::Ctx c;
     int live = 0;   // volumes, pyramids, BP objects and servers made on it and not yet freed
+    // worker contexts of hbp_pairs_prepare_dev (a stream and memory pool each), made on first
+    // use; the objects they build belong to this context but their memory stays in the
+    // workers' pools, so the workers live until the context is destroyed
+    std::vector<std::unique_ptr<hbp::Ctx>> helpers;
 };
 // every handle remembers the context whose pool holds its device memory
 struct hbp_volume {

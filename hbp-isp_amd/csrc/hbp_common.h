@@ -25,6 +25,10 @@ void set_error(const std::string &msg);
 struct HipError {
     std::string msg;
 };
+// thrown instead of a cooperative launch on a context that forbids them (a worker whose stream
+// runs beside other streams' kernels, so the grid's co-residency could not be guaranteed); the
+// caller redoes the work where a cooperative launch is safe
+struct CoopNeeded {};
 #define HBP_CHECK(expr)                                                                    \
     do {                                                                                   \
         hipError_t _e = (expr);                                                            \
@@ -362,6 +366,8 @@ struct Ctx {
     // server's admission context, whose work runs beside the BP rounds of the main stream:
     // a cooperative grid needs every one of its blocks resident at once)
     hipStream_t drain_before_coop = nullptr;
+    // no cooperative launch on this context: CoopNeeded is thrown instead
+    bool coop_forbidden = false;
     // pinned host words that kernels write counts into, read after one stream wait (no copies)
     int *hbox = nullptr;
     int *host_box() {

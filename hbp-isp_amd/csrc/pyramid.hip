@@ -738,17 +738,19 @@ static int gauss_taps(double sigma, GaussTaps &t, float *wsum) {
     return S;
 }
 
-static void upload_srgb_lut(Ctx &c) {
-    static double lut[256];
-    static bool init = false;
-    if (!init) {
-        for (int v = 0; v < 256; ++v) {
-            double cc = (double)(float)v / 255.0;
-            lut[v] = (cc > 0.04045) ? std::pow((cc + 0.055) / 1.055, 2.4) : cc / 12.92;
+struct SrgbLut {
+    double v[256];
+    SrgbLut() {
+        for (int i = 0; i < 256; ++i) {
+            double cc = (double)(float)i / 255.0;
+            v[i] = (cc > 0.04045) ? std::pow((cc + 0.055) / 1.055, 2.4) : cc / 12.92;
         }
-        init = true;
     }
-    HBP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_srgb_lut), lut, sizeof(lut), 0, hipMemcpyHostToDevice, c.stream));
+};
+static void upload_srgb_lut(Ctx &c) {
+    static const SrgbLut lut;   // built once, thread-safe (several host threads build pyramids)
+    HBP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_srgb_lut), lut.v, sizeof(lut.v), 0, hipMemcpyHostToDevice,
+                                     c.stream));
 }
 
 // device memory of one CRTrees run: 8 node arrays, the loop flags / status words of the
@@ -815,6 +817,7 @@ static void crt_cooperative(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
     long need = ((long)V + 255) / 256;
     if (need < blocks) blocks = (int)need;
     void *args[] = {(void *)&d_nn, (void *)&V, (void *)&s, (void *)&d_label};
+    if (c.coop_forbidden) throw CoopNeeded{};
     if (c.drain_before_coop) HBP_CHECK(hipStreamSynchronize(c.drain_before_coop));
     HBP_CHECK(hipLaunchCooperativeKernel((const void *)crtrees_kernel<true>, dim3(blocks), dim3(256), args, 0,
                                          c.stream));

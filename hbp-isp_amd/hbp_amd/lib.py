@@ -80,6 +80,8 @@ SIGNATURES = {
     "hbp_bp_level_size": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "hbp_bp_run": (C.c_int, [_vp, _vp, _vp, _vp]),
     "hbp_bp_run_batch": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "hbp_pairs_prepare_dev": (C.c_int, [_vp, C.c_int, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
+                                        C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "hbp_server_create": (C.c_int, [_vp, C.c_int, C.c_longlong, C.c_float, C.c_float, _pp]),
     "hbp_server_destroy": (C.c_int, [_vp]),
     "hbp_submit": (C.c_int, [_vp, _u8, _u8, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
@@ -165,6 +167,12 @@ def inject_fault(what, after):
     _check(load().hbp_inject_fault(what.encode(), int(after)))
 
 
+class PairDev(C.Structure):
+    """hbp_pair_dev: one pair's device images and sizes for hbp_pairs_prepare_dev"""
+    _fields_ = [("left", C.c_void_p), ("right", C.c_void_p), ("width", C.c_int), ("height", C.c_int),
+                ("labels", C.c_int), ("linkage", C.c_int), ("num_nb", C.c_int), ("min_level", C.c_int)]
+
+
 class _Handle:
     """A library object owned by a context: freed before the context is."""
 
@@ -173,6 +181,15 @@ class _Handle:
     def _adopt(self, ctx, h):
         self.ctx, self.h = ctx, h
         ctx._children.add(self)
+
+    @classmethod
+    def _wrap(cls, ctx, h, **attrs):
+        """an object for a handle the library made (already owned by ctx)"""
+        obj = cls.__new__(cls)
+        obj._adopt(ctx, h)
+        for k, v in attrs.items():
+            setattr(obj, k, v)
+        return obj
 
     def free(self):
         h = getattr(self, "h", None)
@@ -272,6 +289,23 @@ class Ctx:
     # ---- (c)
     def bp(self, pyr, vol, alpha=1000.1, tau_s=0.6, min_level=2, use_cost_vol=1):
         return BP(self, pyr, vol, alpha, tau_s, min_level, use_cost_vol)
+
+    def prepare_pairs_dev(self, pairs, alpha=1000.1, tau_s=0.6, use_cost_vol=1, workers=4, link=2, num_nb=4):
+        """Cost volumes, pyramids and BP objects of many pairs built by `workers` host threads on
+        streams of their own (hbp_pairs_prepare_dev).  pairs: (left_dev, right_dev, W, H, D,
+        min_level) with device pointers to the BGR images; returns [(Volume, Pyramid, BP)]."""
+        n = len(pairs)
+        arr = (PairDev * max(n, 1))(*[PairDev(l, r, W, H, D, link, num_nb, ml) for l, r, W, H, D, ml in pairs])
+        vols, pyrs, bps = (_vp * max(n, 1))(), (_vp * max(n, 1))(), (_vp * max(n, 1))()
+        _check(self.lib.hbp_pairs_prepare_dev(self.h, n, C.cast(arr, _vp), alpha, tau_s, int(use_cost_vol),
+                                              int(workers), vols, pyrs, bps))
+        out = []
+        for i, (_, _, W, H, D, _) in enumerate(pairs):
+            v = Volume._wrap(self, vols[i], W=W, H=H, D=D, f32=False)
+            p = Pyramid._wrap(self, pyrs[i], W=W, H=H)
+            b = BP._wrap(self, bps[i], W=W, H=H, pyr=p, vol=v)
+            out.append((v, p, b))
+        return out
 
     def bp_run_batch(self, bps):
         """Runs the remaining sweeps of several BP objects together (hbp_bp_run_batch); each

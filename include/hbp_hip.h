@@ -164,6 +164,24 @@ int hbp_bp_level_size(const hbp_bp *bp, int level, int *num_clus, int *num_bd);
 /* map_msg (hbp_isp.cu:1081-1105): disp f32 [H][W] and plane f32 [H][W][3] on the host
  * (either may be NULL); runs the remaining sweeps first */
 int hbp_bp_run(hbp_ctx *ctx, hbp_bp *bp, float *disp, float *plane3);
+/* ---- many pairs' cost volumes, pyramids and BP objects at once (no counterpart in the
+ *      reference, which builds one pair per run_ms; for hbp_bp_run_batch).  Each pair: device
+ *      BGR images (u8 [height][width][3]), the Census cost with `labels` disparities,
+ *      hbp_pyramid_build_dev(linkage, num_nb, sigma 0), hbp_bp_create(alpha, tau_smooth,
+ *      min_level, use_cost_vol) with min_level < 0 meaning BP on -min_level levels below the
+ *      top and min_level clamped into [1, levels - 1].  `workers` (1-16) host threads build
+ *      the pairs, each on a stream and memory pool of its own that the context keeps, so the
+ *      pairs' latency-bound pyramid levels and set-up waits overlap; the objects are the same
+ *      bits as the one-pair calls, belong to ctx (free them with ctx) and are complete when the
+ *      call returns.  A pair whose CRTrees needs the cooperative fallback is redone on ctx's
+ *      own stream after the workers finish.  On error nothing is returned (all NULL). */
+typedef struct {
+    const uint8_t *left, *right;
+    int width, height, labels, linkage, num_nb, min_level;
+} hbp_pair_dev;
+int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float alpha, float tau_smooth,
+                          int use_cost_vol, int workers, hbp_volume **vols, hbp_pyramid **pyrs, hbp_bp **bps);
+
 /* batched BP (no counterpart in the reference, which runs one pair per HBP_ISP::run_ms): runs
  * the remaining sweeps of n BP objects of one context together, sweep i of every pair in the
  * same launches, then the map of each.  Every pair's disparity and planes are bit-identical to
