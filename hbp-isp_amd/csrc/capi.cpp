@@ -704,7 +704,8 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
     Ctx &m = ctx->c;
     // the caller's queued work (e.g. the images' copies) first
     HBP_CHECK(hipStreamSynchronize(m.stream));
-    while ((int)ctx->helpers.size() < workers) {
+    const int nw = std::min(workers, n);
+    while ((int)ctx->helpers.size() < nw) {
         auto h = std::make_unique<Ctx>();
         h->device = m.device;
         h->num_cu = m.num_cu;
@@ -713,7 +714,6 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
         HBP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         ctx->helpers.push_back(std::move(h));
     }
-    const int nw = std::min(workers, n);
     for (int w = 0; w < nw; ++w) {   // the context's kernel forms
         Ctx &h = *ctx->helpers[w];
         h.coarse_min_pix = m.coarse_min_pix;
@@ -749,11 +749,21 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
                 (void)hipStreamSynchronize(h.stream);
                 err[w] = "host allocation failed";
                 failed = true;
+            } catch (...) {
+                (void)hipStreamSynchronize(h.stream);
+                err[w] = "unknown error in a worker";
+                failed = true;
             }
         }
     };
     std::vector<std::thread> th;
-    for (int w = 1; w < nw; ++w) th.emplace_back(work, w);
+    try {
+        for (int w = 1; w < nw; ++w) th.emplace_back(work, w);
+    } catch (...) {   // a thread could not start: the started ones stop at their next pair
+        failed = true;
+        for (auto &t : th) t.join();
+        throw HipError{"could not start the worker threads"};
+    }
     work(0);
     for (auto &t : th) t.join();
     for (int w = 0; w < nw; ++w) HBP_REQUIRE(err[w].empty(), err[w]);   // done[] frees what was built
