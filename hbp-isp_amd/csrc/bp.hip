@@ -126,6 +126,11 @@ __global__ void node_major_kernel(const int2 *__restrict__ bd, const int *__rest
         revl[(long)sn * maxdeg + i] = rev[e];
     }
 }
+// per edge: the slot of its reverse edge among the reverse's source's edges
+__global__ void rloc_kernel(const int *__restrict__ eloc, const int *__restrict__ rev, long E, int *__restrict__ rloc) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (long)gridDim.x * blockDim.x)
+        rloc[e] = eloc[rev[e]];
+}
 // WTA disparity of the sorted pixels
 __global__ void pw_kernel(const uint32_t *__restrict__ pxy, long n, const float *__restrict__ dwta, int W,
                           float *__restrict__ pw) {
@@ -356,22 +361,24 @@ constexpr size_t PC_LDS_BYTES = (size_t)MAXC * P * 16 + (size_t)MAXC * P * 8 + (
 // sum / clear of the rrep replicas of one accumulator; unrolled to the largest replica count
 // so that all loads are in flight together
 constexpr int kMaxReplicas = 8;
-__device__ __forceinline__ int64_t acc_sum(const SweepArgs &a, long id, int k) {
-    const unsigned long long *__restrict__ p = a.acc + (size_t)id * a.rrep * K + k;
+__device__ __forceinline__ int64_t acc_sum_at(const unsigned long long *__restrict__ acc, int rrep, long id, int k) {
+    const unsigned long long *__restrict__ p = acc + (size_t)id * rrep * K + k;
     unsigned long long v[kMaxReplicas];
 #pragma unroll
-    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < a.rrep ? p[r * K] : 0ull;
+    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < rrep ? p[r * K] : 0ull;
     unsigned long long t = 0ull;
 #pragma unroll
     for (int r = 0; r < kMaxReplicas; ++r) t += v[r];
     return (int64_t)t;
 }
-__device__ __forceinline__ void acc_zero(const SweepArgs &a, long id, int k) {
-    unsigned long long *__restrict__ p = a.acc + (size_t)id * a.rrep * K + k;
+__device__ __forceinline__ void acc_zero_at(unsigned long long *__restrict__ acc, int rrep, long id, int k) {
+    unsigned long long *__restrict__ p = acc + (size_t)id * rrep * K + k;
 #pragma unroll
     for (int r = 0; r < kMaxReplicas; ++r)
-        if (r < a.rrep) p[r * K] = 0ull;
+        if (r < rrep) p[r * K] = 0ull;
 }
+__device__ __forceinline__ int64_t acc_sum(const SweepArgs &a, long id, int k) { return acc_sum_at(a.acc, a.rrep, id, k); }
+__device__ __forceinline__ void acc_zero(const SweepArgs &a, long id, int k) { acc_zero_at(a.acc, a.rrep, id, k); }
 
 // init_bp (hbp_isp.cu:930-950): copy_prior_plane (56-73), init_rand_seed_k (570-591), plus the
 // first sweep's RANSAC candidates.
@@ -986,7 +993,22 @@ __device__ __forceinline__ void row_revl(int rv, int (&r)[sizeof...(I)], std::in
     ((r[I] = row_bcast<I>(rv)), ...);
 }
 
-__device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn) {
+// Inputs of a node update that are not in place: the state a run of fused sweeps leaves (see
+// chain_kernel).  Planes of slots < 2P and their costs from pl / co, the candidates (slots
+// >= 2P) from cand, the RANSAC points of those candidates from pt; the update writes its
+// outputs to the SweepArgs state and also clears a second set of accumulators.
+struct NodeIO {
+    const float4 *pl;
+    const float *co;
+    const float4 *cand;
+    const uint4 *pt;
+    unsigned long long *acc2;
+    unsigned int *nan2;
+};
+
+template <bool IO>
+__device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_kind, float max_dz, float max_dn,
+                           const NodeIO &io) {
     constexpr int MAXDEG = kNodeMaxDeg;   // incoming messages held in registers (the rest are re-read)
     const int lane = threadIdx.x & 63;
     const int k = lane & 15;
@@ -998,9 +1020,10 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     const int ppos = a.pix_off[s], pn = a.pix_off[s + 1] - ppos;
     const unsigned int nb = a.accnan[s];
     const uint32_t s0 = a.s_ransac[s];
-    const float cost_in = a.cost[s * K + kk];
+    const float cost_in = IO ? io.co[s * K + kk] : a.cost[s * K + kk];
     const int64_t fsum = acc_sum(a, s, kk >= 2 * P ? kk : 2 * P);
-    const float4 pin = a.plane[s * K + kk];
+    const float4 pin = !IO ? a.plane[s * K + kk] : (kk < 2 * P ? io.pl[s * K + kk] : io.cand[s * K + kk]);
+    const uint4 pt_in = IO ? io.pt[s] : make_uint4(0u, 0u, 0u, 0u);
     const float2 mxy = a.mxy[s];
     const long nm = (long)s * a.maxdeg;
     // rows nm .. nm+15, past deg into the next node's rows or the arrays' kNodeMaxDeg padding
@@ -1030,10 +1053,12 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     }
     // ---- sums and the selection
     if (k == 2 * P) a.accnan[s] = 0u;
+    if (IO && k == 2 * P) io.nan2[s] = 0u;
     float cost = cost_in;
     if (kk >= 2 * P) {
         cost = from_fixed(fsum, (nb >> kk) & 1u, a.shift);
         acc_zero(a, s, kk);
+        if (IO) acc_zero_at(io.acc2, a.rrep, s, kk);
     }
     // belief = cost + incoming messages in edge order
     float bel = cost;
@@ -1065,6 +1090,8 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
         a.cost[s * K + k] = cost_n;
         a.belief[s * K + k] = bel_n;
     }
+    // the points of the candidates just costed, unless a new RANSAC draw replaces them below
+    if (IO && k == K - 1 && next_kind != kRansac) a.point[s] = pt_in;
     float *msg = (float *)a.msg;
     if (k < 2 * P) {
 #pragma unroll
@@ -1105,6 +1132,214 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
 }
 
 // ----------------------------------------------------------------------------------------
+// Fused sweeps (coarse levels, one pair).  A RANSAC sweep's candidates depend on the seeds
+// alone, not on the previous sweep's selection, so a run of sweeps j0, j0+1, .., jL (j0 any
+// non-spatial sweep, the others RANSAC sweeps of the same level) runs as one launch per sweep
+// instead of two.  Launch u (sweep j = j0+u) holds three kinds of wavefronts:
+//   segments   the plane costs of sweep j's candidates (drawn one launch earlier);
+//   edges      sweep j's messages.  For u >= 1 the selection that ends sweep j-1 (belief,
+//              rank, compaction: node_group's arithmetic) is recomputed inside the wavefront
+//              for both ends of each edge from that selection's inputs, which launch u-1 left;
+//   nodes      the same selection of every superpixel, written out for launch u+1, the seed
+//              advance, and the draw of sweep j+1's RANSAC candidates.
+// No wavefront reads what another writes in the same launch: the selection's inputs and
+// outputs, the candidates and the sums rotate through separate buffers (ChainIO).  After
+// launch L a node update (node_group<true>) writes the standard state, so the rest of the
+// schedule (spatial and random sweeps, the map, the next level) reads it unchanged.  Every
+// value is computed by the same arithmetic as in the split sweeps, so the bits are the same.
+// ----------------------------------------------------------------------------------------
+struct ChainIO {
+    // inputs of the selection that ends sweep j-1
+    const float4 *pl_in;                 // [V][K] slots < 2P: planes after the selection of sweep j-2
+    const float *co_in;                  // [V][K] slots < 2P: their costs
+    const float4 *cand_in;               // [V][K] slots >= 2P: the candidates of sweep j-1
+    const unsigned long long *acc_in;    // their fixed-point sums [V][rrep][K]
+    const unsigned int *nan_in;          // their NaN bits [V]
+    const float *mt_in;                  // sweep j-1's messages, node-major [V][maxdeg][K]
+    const int *rloc;                     // per edge: the reverse edge's slot among its source's edges [E]
+    // sweep j (its candidates are SweepArgs::plane slots >= 2P)
+    const float4 *cand_cur;              // [V][K] slots >= 2P
+    const uint4 *pt_cur;                 // their RANSAC points [V]
+    // node wavefronts
+    float4 *pl_out;                      // [V][K] slots < 2P: the selection that ends sweep j-1
+    float *co_out;
+    float4 *cand_next;                   // [V][K] slots >= 2P: sweep j+1's RANSAC candidates
+    uint4 *pt_next;
+    unsigned long long *acc_clear;       // sums to clear for a later launch, or null
+    unsigned int *nan_clear;
+    int materialize;                     // u >= 1: the selection of sweep j-1 is computed here
+    int draw;                            // sweep j+1 is a RANSAC sweep of the run
+};
+
+// node_group's selection of superpixel n from a chain's inputs; lane k of a 16-lane row owns
+// slot k.  Loads first (one round trip), then the arithmetic.
+struct SelIn {
+    int deg;
+    unsigned int nb;
+    float cost_in;
+    int64_t fsum;
+    float4 pin;
+    float mt[kNodeMaxDeg];
+};
+struct SelOut {
+    float bel, cost;
+    float4 pl;
+    int src;
+};
+__device__ __forceinline__ void sel_load(const SweepArgs &a, const ChainIO &io, long n, int kk, SelIn &in) {
+    in.deg = a.off[n + 1] - a.off[n];
+    in.nb = io.nan_in[n];
+    in.cost_in = io.co_in[n * K + kk];
+    in.fsum = acc_sum_at(io.acc_in, a.rrep, n, kk >= 2 * P ? kk : 2 * P);
+    in.pin = kk < 2 * P ? io.pl_in[n * K + kk] : io.cand_in[n * K + kk];
+    const float *__restrict__ mtp = io.mt_in + n * a.maxdeg * K + kk;
+#pragma unroll
+    for (int i = 0; i < kNodeMaxDeg; ++i) in.mt[i] = mtp[i * K];
+}
+__device__ __forceinline__ SelOut sel_compute(const SweepArgs &a, const ChainIO &io, long n, const SelIn &in, int k,
+                                              int kk) {
+    const int g = (threadIdx.x & 63) & ~15;
+    const bool own = k < K;
+    const float cost = kk >= 2 * P ? from_fixed(in.fsum, (in.nb >> kk) & 1u, a.shift) : in.cost_in;
+    float bel = cost;
+#pragma unroll
+    for (int i = 0; i < kNodeMaxDeg; ++i) bel = i < in.deg ? bel + in.mt[i] : bel;
+    for (int i = kNodeMaxDeg; i < in.deg; ++i) bel += io.mt_in[(n * a.maxdeg + i) * K + kk];
+    const int order = belief_rank(bel, k, std::make_integer_sequence<int, K>{});
+    const bool sel = own && order < P;
+    const uint32_t m = (uint32_t)(__ballot(sel) >> g) & 0xfffu;
+    const int nsel = __popc(m);
+    int src = k;
+    if (k < nsel) {
+        uint32_t mm = m;
+        for (int i = 0; i < k; ++i) mm &= mm - 1u;
+        src = __ffs(mm) - 1;
+    }
+    return SelOut{bel, cost, in.pin, src};
+}
+
+// one step p of the message minimum over t's 2P particles (edge_wave's arithmetic), the
+// particle's values broadcast from lane p of the row
+template <int p>
+__device__ __forceinline__ void fused_msg_step(float tm, float tx, float ty, float tz, float4 psl, const float (&X)[6],
+                                               const float (&Y)[6], float c0, float dpos, float tau_s, float &out,
+                                               float &minimum) {
+    if (p == P) {
+        out += minimum;
+        minimum = FLT_MAX;
+    }
+    const float m = row_bcast<p>(tm);
+    const float dpx = psl.x - row_bcast<p>(tx);
+    const float dpy = psl.y - row_bcast<p>(ty);
+    const float dpz = psl.z - row_bcast<p>(tz);
+    float phi = fabsf(dpx * X[0] + dpy * Y[0] + dpz);
+#pragma unroll
+    for (int j = 1; j < 6; ++j) phi = phi + fabsf(dpx * X[j] + dpy * Y[j] + dpz);
+    phi = fminf(div3_nonneg(phi) / dpos, tau_s);
+    const float v = m + c0 * phi;
+    if (v < minimum) minimum = v;
+}
+template <int... p>
+__device__ __forceinline__ void fused_msg(float tm, float tx, float ty, float tz, float4 psl, const float (&X)[6],
+                                          const float (&Y)[6], float c0, float dpos, float tau_s, float &out,
+                                          float &minimum, std::integer_sequence<int, p...>) {
+    (fused_msg_step<p>(tm, tx, ty, tz, psl, X, Y, c0, dpos, tau_s, out, minimum), ...);
+}
+template <int... J>
+__device__ __forceinline__ float row_sum(float v, std::integer_sequence<int, J...>) {
+    float s = 0.f;
+    ((s += row_bcast<J>(v)), ...);
+    return s;
+}
+
+// sweep j's messages of FUSED_EDGES edges per wavefront, 16 lanes per edge e = (s, t): lane y
+// computes the message into s for s's slot y (edge_wave), with the selections of s and t that
+// end sweep j-1 recomputed in the row
+constexpr int FUSED_EDGES = 4;
+__device__ void edge_wave_fused(const SweepArgs &a, const ChainIO &io, long wv) {
+    const int lane = threadIdx.x & 63;
+    const int k = lane & 15, g = lane & ~15;
+    const int kk = k < K ? k : 0;
+    const long e = wv * FUSED_EDGES + (lane >> 4);
+    const bool live = e < a.E;
+    const long ec = live ? e : a.E - 1;
+    const int2 b = a.bd[ec];
+    const int eslot = a.eloc[ec], rslot = io.rloc[ec];
+    const float c0 = a.c0[ec], dpos = a.dpos[ec];
+    SelIn is, it;
+    sel_load(a, io, b.x, kk, is);
+    sel_load(a, io, b.y, kk, it);
+    // t's message from s (on t's slot rslot), the points, s's candidates of this sweep
+    const float mrow = io.mt_in[((long)b.y * a.maxdeg + rslot) * K + kk];
+    const uint4 ps = io.pt_cur[b.x], pt = io.pt_cur[b.y];
+    const float4 cc = io.cand_cur[(long)b.x * K + (kk >= 2 * P ? kk : 2 * P)];
+    const SelOut ss = sel_compute(a, io, b.x, is, k, kk);
+    const SelOut st = sel_compute(a, io, b.y, it, k, kk);
+    // s's planes after its selection (slots < 2P) and this sweep's candidates
+    float4 psl = make_float4(__shfl(ss.pl.x, g + ss.src), __shfl(ss.pl.y, g + ss.src), __shfl(ss.pl.z, g + ss.src), 0.f);
+    if (k >= 2 * P) psl = cc;
+    // lanes p < 2P: t's particle p after its selection, its belief less the message from s
+    const float tm = __shfl(st.bel, g + st.src) - __shfl(mrow, g + st.src);
+    const float tx = __shfl(st.pl.x, g + st.src), ty = __shfl(st.pl.y, g + st.src), tz = __shfl(st.pl.z, g + st.src);
+    float X[6], Y[6];
+    X[0] = (float)unpack_x(ps.x); Y[0] = (float)unpack_y(ps.x);
+    X[1] = (float)unpack_x(ps.y); Y[1] = (float)unpack_y(ps.y);
+    X[2] = (float)unpack_x(ps.z); Y[2] = (float)unpack_y(ps.z);
+    X[3] = (float)unpack_x(pt.x); Y[3] = (float)unpack_y(pt.x);
+    X[4] = (float)unpack_x(pt.y); Y[4] = (float)unpack_y(pt.y);
+    X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
+    float out = 0.f, minimum = FLT_MAX;
+    fused_msg(tm, tx, ty, tz, psl, X, Y, c0, dpos, a.tau_s, out, minimum, std::make_integer_sequence<int, 2 * P>{});
+    out += minimum;
+    // normalize_msg_k: mean over the 12 slots, summed in slot order
+    float mean = row_sum(out, std::make_integer_sequence<int, K>{});
+    mean /= K;
+    if (live && k < K) a.msg_t[((long)b.x * a.maxdeg + eslot) * K + k] = out - mean;
+}
+
+// node wavefronts of a fused launch, 16 lanes per superpixel s: the selection that ends sweep
+// j-1 (materialize), the seed advance it makes (its next sweep, j, is a RANSAC sweep), and the
+// draw of sweep j+1's RANSAC candidates from the advanced seed (node_group's draw)
+__device__ void chain_node(const SweepArgs &a, const ChainIO &io, long s) {
+    const int lane = threadIdx.x & 63;
+    const int k = lane & 15, g = lane & ~15;
+    const int kk = k < K ? k : 0;
+    const bool cand = k >= 2 * P && k < K;
+    const int ppos = a.pix_off[s], pn = a.pix_off[s + 1] - ppos;
+    uint32_t s0 = a.s_ransac[s];
+    if (io.materialize) {
+        SelIn in;
+        sel_load(a, io, s, kk, in);
+        const SelOut so = sel_compute(a, io, s, in, k, kk);
+        const float4 pl_n = make_float4(__shfl(so.pl.x, g + so.src), __shfl(so.pl.y, g + so.src),
+                                        __shfl(so.pl.z, g + so.src), __shfl(so.pl.w, g + so.src));
+        const float co_n = __shfl(so.cost, g + so.src);
+        if (k < 2 * P) {
+            io.pl_out[s * K + k] = pl_n;
+            io.co_out[s * K + k] = co_n;
+        }
+        constexpr LcgJump j3 = lcg_jump_coef(3 * P);
+        s0 = (j3.a * s0 + j3.c) % 2147483648u;
+        if (k == 2 * P) a.s_ransac[s] = s0;
+    }
+    if (io.acc_clear && cand) {
+        acc_zero_at(io.acc_clear, a.rrep, s, k);
+        if (k == 2 * P) io.nan_clear[s] = 0u;
+    }
+    if (io.draw && cand) {
+        uint32_t sk = lcg_skip<3>(s0, k - 2 * P);
+        int p0, p1, p2;
+        ransac_draw(sk, pn, p0, p1, p2);
+        const uint4 q = make_uint4(a.pxy[ppos + p0], a.pxy[ppos + p1], a.pxy[ppos + p2], 0u);
+        const float4 A = make_float4((float)unpack_x(q.x), (float)unpack_y(q.x), a.pw[ppos + p0], 0.f);
+        const float4 B = make_float4((float)unpack_x(q.y), (float)unpack_y(q.y), a.pw[ppos + p1], 0.f);
+        const float4 C = make_float4((float)unpack_x(q.z), (float)unpack_y(q.z), a.pw[ppos + p2], 0.f);
+        io.cand_next[s * K + k] = fit_plane(A, B, C);
+        if (k == K - 1) io.pt_next[s] = q;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
 // kernels
 // ----------------------------------------------------------------------------------------
 #define GRID_STRIDE(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
@@ -1114,7 +1349,7 @@ __global__ __launch_bounds__(256) void init_cost_kernel(SweepArgs a) { GRID_STRI
 __global__ __launch_bounds__(256) void spatial_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) spatial_node(a, id); }
 __global__ __launch_bounds__(256) void node_kernel(SweepArgs a, int this_kind, int next_kind, float ndz, float ndn) {
     const long s = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
-    if (s < a.V) node_group(a, s, this_kind, next_kind, ndz, ndn);
+    if (s < a.V) node_group<false>(a, s, this_kind, next_kind, ndz, ndn, NodeIO{});
 }
 
 // Plane costs of slots [slot0, slot0+NS) and, for NS == P, the sweep's messages.  Coarse
@@ -1154,6 +1389,34 @@ __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, 
     } else if (NS == P) {
         edge_wave(a, (long)(b - npc_blocks) * 4 + wave, COARSE && spatial != 0);
     }
+}
+
+// One launch of a run of fused sweeps (see ChainIO): segment blocks first (XCD-contiguous),
+// then edge blocks, then node blocks of 16 superpixels.  FIRST: the run's first sweep, whose
+// messages read the standard state (edge_wave) and whose node wavefronts only draw.
+template <bool FIRST, int MODE, int HB>
+__global__ __launch_bounds__(256) void chain_kernel(SweepArgs a, ChainIO io, int npc, int nedge_blocks) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = blockIdx.x;
+    const int npc_blocks = (npc + 3) / 4;
+    if (b < npc_blocks) {
+        const int w = xcd_contiguous(b, npc_blocks) * 4 + wave;
+        if (w < npc) pc_segment<P, MODE, HB>(a, w, 2 * P, false);
+    } else if (b < npc_blocks + nedge_blocks) {
+        const long wv = (long)(b - npc_blocks) * 4 + wave;
+        if (FIRST) edge_wave(a, wv, false);
+        else edge_wave_fused(a, io, wv);
+    } else {
+        const long s = ((long)(b - npc_blocks - nedge_blocks) * 256 + threadIdx.x) >> 4;
+        if (s < a.V) chain_node(a, io, s);
+    }
+}
+
+// the node update that ends a run of fused sweeps: the run's buffers in, the standard state out
+__global__ __launch_bounds__(256) void node_io_kernel(SweepArgs a, NodeIO io, int this_kind, int next_kind, float ndz,
+                                                      float ndn) {
+    const long s = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
+    if (s < a.V) node_group<true>(a, s, this_kind, next_kind, ndz, ndn, io);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1235,7 +1498,7 @@ __global__ __launch_bounds__(256) void node_batch_kernel(const SweepArgs *__rest
     if (local >= it.node_blocks) return;
     const SweepArgs a = rebased_args(args, it.args);
     const long s = ((long)local * 256 + threadIdx.x) >> 4;
-    if (s < a.V) node_group(a, s, it.kind, it.next_kind, it.ndz, it.ndn);
+    if (s < a.V) node_group<false>(a, s, it.kind, it.next_kind, it.ndz, it.ndn, NodeIO{});
 }
 __global__ __launch_bounds__(256) void edge_batch_kernel(const SweepArgs *__restrict__ args,
                                                          const BatchItem *__restrict__ items, int nitems) {
@@ -1593,6 +1856,12 @@ void bp_setup(Ctx &c, BP &b) {
         if (B.E > 0)
             hipLaunchKernelGGL(node_major_kernel, dim3(grid_for(c, B.E)), dim3(256), 0, c.stream, B.bd, B.off.p,
                                B.rev.p, B.E, B.maxdeg, B.eloc.p, B.revl.p);
+        if (B.coarse) {
+            B.rloc.alloc(c.pool, std::max<long>(B.E, 1));
+            if (B.E > 0)
+                hipLaunchKernelGGL(rloc_kernel, dim3(grid_for(c, B.E)), dim3(256), 0, c.stream, B.eloc.p, B.rev.p,
+                                   B.E, B.rloc.p);
+        }
     }
     b.msg.alloc(c.pool, (size_t)maxE * 2);
     b.msg_t.alloc(c.pool, (slots + kNodeMaxDeg) * K);
@@ -1709,23 +1978,171 @@ static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float
     HBP_CHECK(hipGetLastError());
 }
 
+template <bool FIRST, int HB>
+static void launch_chain_hb(Ctx &c, const ProfScope &ps, const SweepArgs &a, const ChainIO &io, int mode, int blocks,
+                            int npc, int nedge_blocks) {
+    if (mode == kCostLeft)
+        HBP_LAUNCH(ps, (chain_kernel<FIRST, kCostLeft, HB>), dim3(blocks), dim3(256), 0, c.stream, a, io, npc,
+                   nedge_blocks);
+    else if (mode == kCostRight)
+        HBP_LAUNCH(ps, (chain_kernel<FIRST, kCostRight, HB>), dim3(blocks), dim3(256), 0, c.stream, a, io, npc,
+                   nedge_blocks);
+    else if (mode == kCostF32)
+        HBP_LAUNCH(ps, (chain_kernel<FIRST, kCostF32, HB>), dim3(blocks), dim3(256), 0, c.stream, a, io, npc,
+                   nedge_blocks);
+    else
+        HBP_LAUNCH(ps, (chain_kernel<FIRST, kCostWta, HB>), dim3(blocks), dim3(256), 0, c.stream, a, io, npc,
+                   nedge_blocks);
+}
+
+// one launch of a fused run: plane costs of the sweep's candidates (unless the level init
+// costed them), its messages, and the node wavefronts
+template <bool FIRST>
+static void launch_chain(Ctx &c, const SweepArgs &a, const ChainIO &io, const BPLevel &B, int l, bool costs) {
+    const int mode = !a.use_cost_vol ? kCostWta : (a.volf ? kCostF32 : (a.right ? kCostRight : kCostLeft));
+    const int npc = costs ? B.nseg : 0;
+    const int nedge_w = a.E > 0 ? div_up(a.E, FIRST ? EDGES_PER_WAVE : FUSED_EDGES) : 0;
+    const int nedge_blocks = div_up(nedge_w, 4);
+    const int node_blocks = div_up((long)a.V * 16, 256);
+    const int blocks = div_up(npc, 4) + nedge_blocks + node_blocks;
+    const double bytes = (costs ? plane_cost_bytes(a.N, a.V, P) : 0.0) + edge_bytes(a.E, a.V) +
+                         (FIRST ? 0.0 : node_bytes(a.E, a.V));
+    ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes, true);
+    if (c.single_hb == 2) launch_chain_hb<FIRST, 2>(c, ps, a, io, mode, blocks, npc, nedge_blocks);
+    else launch_chain_hb<FIRST, 4>(c, ps, a, io, mode, blocks, npc, nedge_blocks);
+}
+
+// the buffers of the fused runs (ChainIO), sized by the largest segment-form level; made on
+// the first run, so that objects that never run one (batched and served pairs) do not hold them
+static void chain_alloc(Ctx &c, BP &b) {
+    if (b.ch_cand[0].p) return;
+    size_t vc = 0, mc = 0, ac = 0;
+    for (int l = b.min_level; l < b.L; ++l) {
+        const BPLevel &B = b.lv[l];
+        if (!B.coarse) continue;
+        vc = std::max(vc, (size_t)B.V);
+        mc = std::max(mc, (size_t)B.V * B.maxdeg);
+        ac = std::max(ac, (size_t)B.V * acc_replicas(B.V) * K);
+    }
+    HBP_REQUIRE(vc > 0, "no segment-form BP level");
+    for (int i = 0; i < 2; ++i) {
+        b.ch_pl[i].alloc(c.pool, vc * K);
+        b.ch_co[i].alloc(c.pool, vc * K);
+        b.ch_pt[i].alloc(c.pool, vc);
+        b.ch_acc[i].alloc(c.pool, ac);
+        b.ch_nan[i].alloc(c.pool, vc);
+        HBP_CHECK(hipMemsetAsync(b.ch_acc[i].p, 0, sizeof(unsigned long long) * ac, c.stream));
+        HBP_CHECK(hipMemsetAsync(b.ch_nan[i].p, 0, sizeof(unsigned int) * vc, c.stream));
+    }
+    for (int i = 0; i < 3; ++i) b.ch_cand[i].alloc(c.pool, vc * K);
+    b.ch_mt.alloc(c.pool, (mc + kNodeMaxDeg) * K);
+}
+
+// sweeps j0 .. j0+n-1 (n >= 2) of one coarse level as a fused run (see ChainIO).  Launch u
+// writes its messages to MT(u) and its sums to ACC(u), reads the previous launch's from
+// MT(u-1) / ACC(u-1), and clears ACC(u-2); the selection that ends sweep j0+v goes to Q(v);
+// the candidates of sweep j0+u, u >= 1, are CAND(u) with RANSAC points PT(u).  The first
+// sweep's inputs and candidates are the standard state; the closing node update writes it.
+static void run_chain(Ctx &c, BP &b, size_t j0, int n) {
+    const Sweep &sw0 = b.sched[j0];
+    const int l = sw0.level;
+    const BPLevel &B = b.lv[l];
+    const SweepArgs a = sweep_args(b, l);
+    chain_alloc(c, b);
+    if (sw0.first) run_level_init(c, b, a, l);
+    const int uL = n - 1;
+    auto CAND = [&](int u) { return b.ch_cand[u % 3].p; };
+    auto PT = [&](int u) { return b.ch_pt[u % 2].p; };
+    auto ACC = [&](int u) { return u % 3 == 0 ? b.acc.p : b.ch_acc[u % 3 - 1].p; };
+    auto NANS = [&](int u) { return u % 3 == 0 ? b.accnan.p : b.ch_nan[u % 3 - 1].p; };
+    auto MT = [&](int u) { return u % 2 == 0 ? b.msg_t.p : b.ch_mt.p; };
+    for (int u = 0; u <= uL; ++u) {
+        SweepArgs ax = a;
+        ax.acc = ACC(u);
+        ax.accnan = NANS(u);
+        ax.msg_t = MT(u);
+        ChainIO io{};
+        io.rloc = B.rloc.p;
+        io.draw = u < uL ? 1 : 0;
+        io.cand_next = CAND(u + 1);
+        io.pt_next = PT(u + 1);
+        if (u == 0) {
+            launch_chain<true>(c, ax, io, B, l, !sw0.first);
+            continue;
+        }
+        ax.plane = CAND(u);
+        io.pl_in = u == 1 ? a.plane : b.ch_pl[u % 2].p;   // Q(u-2)
+        io.co_in = u == 1 ? a.cost : b.ch_co[u % 2].p;
+        io.cand_in = u == 1 ? a.plane : CAND(u - 1);
+        io.acc_in = ACC(u - 1);
+        io.nan_in = NANS(u - 1);
+        io.mt_in = MT(u - 1);
+        io.cand_cur = CAND(u);
+        io.pt_cur = PT(u);
+        io.pl_out = b.ch_pl[(u - 1) % 2].p;   // Q(u-1)
+        io.co_out = b.ch_co[(u - 1) % 2].p;
+        io.materialize = 1;
+        if (u >= 2) {
+            io.acc_clear = ACC(u - 2);
+            io.nan_clear = NANS(u - 2);
+        }
+        launch_chain<false>(c, ax, io, B, l, true);
+    }
+    // the node update of the run's last sweep, into the standard state
+    const size_t jL = j0 + (size_t)uL;
+    const Sweep &swL = b.sched[jL];
+    int next_kind = kNone;
+    float ndz = 0.f, ndn = 0.f;
+    if (!swL.last && jL + 1 < b.sched.size()) {
+        const Sweep &nx = b.sched[jL + 1];
+        next_kind = nx.kind;
+        ndz = nx.dz;
+        ndn = nx.dn;
+    }
+    SweepArgs an = a;
+    an.acc = ACC(uL);
+    an.accnan = NANS(uL);
+    an.msg_t = MT(uL);
+    const NodeIO nio{b.ch_pl[(uL - 1) % 2].p, b.ch_co[(uL - 1) % 2].p, CAND(uL), PT(uL), ACC(uL - 1), NANS(uL - 1)};
+    {
+        ProfScope psn(c, "node_L" + std::to_string(l), node_bytes(a.E, a.V), true);
+        HBP_LAUNCH(psn, node_io_kernel, dim3(div_up((long)a.V * 16, 256)), dim3(256), 0, c.stream, an, nio, swL.kind,
+                   next_kind, ndz, ndn);
+    }
+    HBP_CHECK(hipGetLastError());
+}
+
 int bp_step(Ctx &c, BP &b, int max_sweeps) {
     int done = 0;
     while (done < max_sweeps && b.next < b.sched.size()) {
         const Sweep &sw = b.sched[b.next];
-        int next_kind = kNone;
-        float ndz = 0.f, ndn = 0.f;
-        if (!sw.last && b.next + 1 < b.sched.size()) {
-            const Sweep &nx = b.sched[b.next + 1];
-            next_kind = nx.kind;
-            ndz = nx.dz;
-            ndn = nx.dn;
+        const BPLevel &B = b.lv[sw.level];
+        // a fused run: this sweep (not spatial) and the RANSAC sweeps after it on its level
+        int n = 1;
+        if (c.fuse && B.coarse && B.rloc.p && sw.kind != kSpatial) {
+            while (done + n < max_sweeps && b.next + n < b.sched.size()) {
+                const Sweep &nx = b.sched[b.next + n];
+                if (nx.first || nx.level != sw.level || nx.kind != kRansac) break;
+                ++n;
+            }
         }
-        run_sweep_split(c, b, sw, next_kind, ndz, ndn);
+        if (n >= 2) {
+            run_chain(c, b, b.next, n);
+        } else {
+            int next_kind = kNone;
+            float ndz = 0.f, ndn = 0.f;
+            if (!sw.last && b.next + 1 < b.sched.size()) {
+                const Sweep &nx = b.sched[b.next + 1];
+                next_kind = nx.kind;
+                ndz = nx.dz;
+                ndn = nx.dn;
+            }
+            run_sweep_split(c, b, sw, next_kind, ndz, ndn);
+        }
         b.cur = sw.level;
-        b.next++;
-        done++;
-        b.sweeps++;
+        b.next += (size_t)n;
+        done += n;
+        b.sweeps += n;
     }
     if (b.next >= b.sched.size()) b.cur = b.min_level - 1;
     return done;

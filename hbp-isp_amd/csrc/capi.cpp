@@ -295,12 +295,15 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
     } else if (n == "batch_graph") {
         HBP_REQUIRE(value == 0 || value == 1, "batch_graph must be 0 or 1");
         ctx->c.batch_graph = value;
+    } else if (n == "fuse") {
+        HBP_REQUIRE(value == 0 || value == 1, "fuse must be 0 or 1");
+        ctx->c.fuse = value;
     } else if (n.rfind("level_form.", 0) == 0) {
         const int l = level_form_index(n);
         HBP_REQUIRE(value >= -1 && value <= 1, "a level form is -1 (by coarse_min_pix), 0 (chunks) or 1 (segments)");
         ctx->c.level_form[l] = value;
     } else {
-        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, level_form.<l>)"};
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, level_form.<l>)"};
     }
     HBP_API_END
 }
@@ -314,9 +317,10 @@ int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
     else if (n == "batch_hb") *value = ctx->c.batch_hb;
     else if (n == "hamming_cols") *value = ctx->c.hamming_cols;
     else if (n == "batch_graph") *value = ctx->c.batch_graph;
+    else if (n == "fuse") *value = ctx->c.fuse;
     else if (n == "seg_len") *value = ctx->c.seg_len;
     else if (n.rfind("level_form.", 0) == 0) *value = ctx->c.level_form[level_form_index(n)];
-    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, level_form.<l>)"};
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, level_form.<l>)"};
     HBP_API_END
 }
 
@@ -721,6 +725,7 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
         h.batch_hb = m.batch_hb;
         h.hamming_cols = m.hamming_cols;
         h.seg_len = m.seg_len;
+        h.fuse = m.fuse;
         std::memcpy(h.level_form, m.level_form, sizeof(h.level_form));
     }
     std::vector<PreparedPair> done(n);

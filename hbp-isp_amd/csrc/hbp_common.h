@@ -357,6 +357,10 @@ struct Ctx {
     // batched BP: 1 captures the whole schedule of rounds in one HIP graph and replays it, 0
     // queues the rounds one by one from the host (the same launches, the same bits)
     int batch_graph = 0;
+    // one-pair BP on coarse levels: 1 runs each RANSAC sweep that follows another non-spatial
+    // sweep of its level as one launch (the previous sweep's node update recomputed where it is
+    // read), 0 as two launches like every other sweep (the same bits)
+    int fuse = 1;
     // per BP level: -1 the threshold above decides, 0 chunks of sorted pixels, 1 per-superpixel
     // segments (a schedule of forms the tuner can set level by level; the bits are the same)
     static constexpr int kFormLevels = 24;

@@ -52,6 +52,7 @@ struct BPLevel {
     int maxdeg = 1;                // largest out-degree
     DevBuf<int> eloc;              // slot of each edge among its source's edges [E]
     DevBuf<int> revl;              // reverse edge per (node, slot) [V][maxdeg]
+    DevBuf<int> rloc;              // coarse: slot of each edge's reverse among its source's edges [E]
     DevBuf<float> pw;              // WTA disparity of the sorted pixels [N]
     DevBuf<float> c0, dpos;        // per edge: max(dc0, .01) * bdl and max(dc1, 1) [E]
     DevBuf<float2> mxy;            // normalised mean position [V]
@@ -82,6 +83,15 @@ struct BP {
     int crx_stride = 0;
     DevBuf<unsigned long long> acc;  // [V][12] fixed-point plane-cost sums
     DevBuf<unsigned int> accnan;     // [V] NaN bits per slot
+    // runs of fused sweeps on coarse levels (bp.hip, ChainIO): selections [Vc][K], candidates
+    // [Vc][K], RANSAC points [Vc], a second sweep-message buffer and a second and third set of
+    // sums, Vc the largest coarse level
+    DevBuf<float4> ch_pl[2], ch_cand[3];
+    DevBuf<float> ch_co[2];
+    DevBuf<uint4> ch_pt[2];
+    DevBuf<float> ch_mt;
+    DevBuf<unsigned long long> ch_acc[2];
+    DevBuf<unsigned int> ch_nan[2];
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;

@@ -69,7 +69,10 @@ int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long 
  *      Hamming cost kernel; "seg_len" (a multiple of 64 in [64, 512], default 512): pixels per
  *      segment wavefront of the segment form; "batch_graph" (0 or 1, default 0): hbp_bp_run_batch queues its
  *      rounds one by one from the host (0) or captures the whole schedule of rounds in one HIP
- *      graph and replays it (1); "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
+ *      graph and replays it (1); "fuse" (0 or 1, default 1): one-pair BP on segment-form levels
+ *      runs each RANSAC sweep that follows another non-spatial sweep of its level as one launch,
+ *      the previous sweep's node update recomputed where it is read (1), or every sweep as two
+ *      launches (0); "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
  *      planes by coarse_min_pix's rule, in chunks of sorted pixels, or in per-superpixel
  *      segments.  Applies to BP objects created afterwards (and to the batched
  *      rounds run afterwards).  hbp_amd.tune picks them by timing the caller's workload. */

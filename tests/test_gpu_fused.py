@@ -1,0 +1,110 @@
+"""Fused sweeps (tuning "fuse", on by default): on segment-form BP levels a run of RANSAC sweeps
+takes one launch per sweep, the previous sweep's node update recomputed where it is read.  The
+results must be the bits of the two-launch sweeps for every cost form (census of either view,
+the WTA indicator, a float volume), both segment step batches, any split of the schedule into
+hbp_bp_step calls, and the oracle's."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from hbp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same_bits(a, b):
+    a, b = np.ascontiguousarray(a, np.float32), np.ascontiguousarray(b, np.float32)
+    na, nb = np.isnan(a), np.isnan(b)
+    return a.shape == b.shape and np.array_equal(na, nb) and np.array_equal(a.view(np.uint32)[~na],
+                                                                            b.view(np.uint32)[~nb])
+
+
+@pytest.fixture
+def tuned(ctx):
+    before = ctx.get_tuning()
+    yield ctx
+    for k, v in before.items():
+        ctx.set_tuning(k, v)
+    for l in range(24):
+        ctx.set_tuning(f"level_form.{l}", -1)
+
+
+def make_bp(ctx, left, right, D, ml, right_view=False, cost="Census", **kw):
+    pyr = ctx.pyramid(right if right_view else left)
+    vol = ctx.cost_volume(left, right, D, right_view, cost=cost)
+    return ctx.bp(pyr, vol, min_level=ml, **kw)
+
+
+CASES = [
+    # seed, W, H, D, min_level, right_view, cost, bp keywords
+    (71, 320, 240, 48, 1, False, "Census", {}),
+    (72, 256, 200, 32, 2, True, "Census", {}),
+    (73, 240, 176, 40, 2, False, "Census", {"use_cost_vol": 0, "alpha": 1.3}),
+    (74, 240, 160, 32, 1, False, "AD", {}),
+    (75, 200, 152, 24, 2, False, "Census", {"alpha": -3.7, "tau_s": 2.5}),
+]
+
+
+@pytest.mark.parametrize("seed,W,H,D,ml,right_view,cost,kw", CASES)
+@pytest.mark.parametrize("hb", [4, 2])
+def test_fused_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_view, cost, kw, hb):
+    say(f"fused vs split seed={seed} {W}x{H} D={D} ml={ml} right={right_view} {cost} {kw} hb={hb}")
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
+    # every level in the segment form, so that runs are fused on every BP level
+    tuned.set_tuning("coarse_min_pix", 1)
+    tuned.set_tuning("single_hb", hb)
+    out = {}
+    for fuse in (0, 1):
+        tuned.set_tuning("fuse", fuse)
+        out[fuse] = make_bp(tuned, left, right, D, ml, right_view, cost, **kw).run()
+    assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("chunk", [2, 3, 5, 7, 13])
+def test_fused_state_after_every_step_call(tuned, chunk):
+    """hbp_bp_step(n) ends a fused run where the call ends: the state read back after each call
+    is the split sweeps' state, planes, costs, beliefs and messages alike."""
+    say(f"fused state, steps of {chunk}")
+    left, right, _ = synth.make_pair(76, 288, 216, 40, (3, 8))
+    tuned.set_tuning("coarse_min_pix", 1)
+    bps = {}
+    for fuse in (0, 1):
+        tuned.set_tuning("fuse", fuse)
+        bps[fuse] = make_bp(tuned, left, right, 40, 2)
+    total = bps[0].total_sweeps
+    done = 0
+    calls = 0
+    while done < total and calls < 40:
+        n = [bps[f].step(chunk) for f in (0, 1)]
+        assert n[0] == n[1]
+        done += n[0]
+        calls += 1
+        s0, s1 = bps[0].state(), bps[1].state()
+        assert s0["level"] == s1["level"]
+        for key in ("planes", "cost", "belief", "msg"):
+            assert same_bits(s0[key], s1[key]), f"{key} differs after {done} sweeps"
+    d0, p0 = bps[0].run()
+    d1, p1 = bps[1].run()
+    assert same_bits(d0, d1) and same_bits(p0, p1)
+
+
+@pytest.mark.parametrize("seed,W,H,D,ml", [(77, 200, 150, 32, 1), (78, 320, 240, 64, 2)])
+def test_fused_runs_match_the_oracle(tuned, seed, W, H, D, ml):
+    say(f"fused vs oracle seed={seed} {W}x{H} D={D} ml={ml}")
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
+    vo, wo = O.cost_volume(left, right, D)
+    bo = O.BP(O.Pyramid(left), vo, wo, D, min_level=ml)
+    bo.step()
+    do, plo = bo.map()
+    for cmp in (1, 96):
+        tuned.set_tuning("coarse_min_pix", cmp)
+        tuned.set_tuning("fuse", 1)
+        d, p = make_bp(tuned, left, right, D, ml).run()
+        assert same_bits(d, do) and same_bits(p, plo), cmp
