@@ -361,21 +361,25 @@ constexpr size_t PC_LDS_BYTES = (size_t)MAXC * P * 16 + (size_t)MAXC * P * 8 + (
 // sum / clear of the rrep replicas of one accumulator; unrolled to the largest replica count
 // so that all loads are in flight together
 constexpr int kMaxReplicas = 8;
-__device__ __forceinline__ int64_t acc_sum_at(const unsigned long long *__restrict__ acc, int rrep, long id, int k) {
-    const unsigned long long *__restrict__ p = acc + (size_t)id * rrep * K + k;
+// A buffer of slot stride S (K, or P for one sweep's candidates alone) holds slots K-S .. K-1 of
+// each node: slot k of node n, replica r is at (n rrep + r) S + k - (K - S).
+__device__ __forceinline__ int64_t acc_sum_at(const unsigned long long *__restrict__ acc, int rrep, long id, int k,
+                                              int stride = K) {
+    const unsigned long long *__restrict__ p = acc + (size_t)id * rrep * stride + (k - (K - stride));
     unsigned long long v[kMaxReplicas];
 #pragma unroll
-    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < rrep ? p[r * K] : 0ull;
+    for (int r = 0; r < kMaxReplicas; ++r) v[r] = r < rrep ? p[r * stride] : 0ull;
     unsigned long long t = 0ull;
 #pragma unroll
     for (int r = 0; r < kMaxReplicas; ++r) t += v[r];
     return (int64_t)t;
 }
-__device__ __forceinline__ void acc_zero_at(unsigned long long *__restrict__ acc, int rrep, long id, int k) {
-    unsigned long long *__restrict__ p = acc + (size_t)id * rrep * K + k;
+__device__ __forceinline__ void acc_zero_at(unsigned long long *__restrict__ acc, int rrep, long id, int k,
+                                            int stride = K) {
+    unsigned long long *__restrict__ p = acc + (size_t)id * rrep * stride + (k - (K - stride));
 #pragma unroll
     for (int r = 0; r < kMaxReplicas; ++r)
-        if (r < rrep) p[r * K] = 0ull;
+        if (r < rrep) p[r * stride] = 0ull;
 }
 __device__ __forceinline__ int64_t acc_sum(const SweepArgs &a, long id, int k) { return acc_sum_at(a.acc, a.rrep, id, k); }
 __device__ __forceinline__ void acc_zero(const SweepArgs &a, long id, int k) { acc_zero_at(a.acc, a.rrep, id, k); }
@@ -1002,8 +1006,9 @@ struct NodeIO {
     const float *co;
     const float4 *cand;
     const uint4 *pt;
-    unsigned long long *acc2;
+    unsigned long long *acc2;   // or null
     unsigned int *nan2;
+    int cs, as;                 // slot strides of cand and of SweepArgs::acc (K or P)
 };
 
 template <bool IO>
@@ -1021,8 +1026,10 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     const unsigned int nb = a.accnan[s];
     const uint32_t s0 = a.s_ransac[s];
     const float cost_in = IO ? io.co[s * K + kk] : a.cost[s * K + kk];
-    const int64_t fsum = acc_sum(a, s, kk >= 2 * P ? kk : 2 * P);
-    const float4 pin = !IO ? a.plane[s * K + kk] : (kk < 2 * P ? io.pl[s * K + kk] : io.cand[s * K + kk]);
+    const int ast = IO ? io.as : K;
+    const int64_t fsum = acc_sum_at(a.acc, a.rrep, s, kk >= 2 * P ? kk : 2 * P, ast);
+    const float4 pin = !IO ? a.plane[s * K + kk]
+                           : (kk < 2 * P ? io.pl[s * K + kk] : io.cand[s * io.cs + kk - (K - io.cs)]);
     const uint4 pt_in = IO ? io.pt[s] : make_uint4(0u, 0u, 0u, 0u);
     const float2 mxy = a.mxy[s];
     const long nm = (long)s * a.maxdeg;
@@ -1053,12 +1060,12 @@ __device__ void node_group(const SweepArgs &a, long s, int this_kind, int next_k
     }
     // ---- sums and the selection
     if (k == 2 * P) a.accnan[s] = 0u;
-    if (IO && k == 2 * P) io.nan2[s] = 0u;
+    if (IO && io.nan2 && k == 2 * P) io.nan2[s] = 0u;
     float cost = cost_in;
     if (kk >= 2 * P) {
         cost = from_fixed(fsum, (nb >> kk) & 1u, a.shift);
-        acc_zero(a, s, kk);
-        if (IO) acc_zero_at(io.acc2, a.rrep, s, kk);
+        acc_zero_at(a.acc, a.rrep, s, kk, ast);
+        if (IO && io.acc2) acc_zero_at(io.acc2, a.rrep, s, kk);
     }
     // belief = cost + incoming messages in edge order
     float bel = cost;
@@ -1152,14 +1159,15 @@ struct ChainIO {
     // inputs of the selection that ends sweep j-1
     const float4 *pl_in;                 // [V][K] slots < 2P: planes after the selection of sweep j-2
     const float *co_in;                  // [V][K] slots < 2P: their costs
-    const float4 *cand_in;               // [V][K] slots >= 2P: the candidates of sweep j-1
-    const unsigned long long *acc_in;    // their fixed-point sums [V][rrep][K]
+    const float4 *cand_in;               // slots >= 2P: the candidates of sweep j-1 (slot stride cs_in)
+    const unsigned long long *acc_in;    // their fixed-point sums [V][rrep][as_in]
     const unsigned int *nan_in;          // their NaN bits [V]
     const float *mt_in;                  // sweep j-1's messages, node-major [V][maxdeg][K]
     const int *rloc;                     // per edge: the reverse edge's slot among its source's edges [E]
     // sweep j (its candidates are SweepArgs::plane slots >= 2P)
-    const float4 *cand_cur;              // [V][K] slots >= 2P
+    const float4 *cand_cur;              // slots >= 2P (slot stride cs_cur)
     const uint4 *pt_cur;                 // their RANSAC points [V]
+    int cs_in, as_in, cs_cur;            // slot strides (K or P)
     // node wavefronts
     float4 *pl_out;                      // [V][K] slots < 2P: the selection that ends sweep j-1
     float *co_out;
@@ -1169,6 +1177,7 @@ struct ChainIO {
     unsigned int *nan_clear;
     int materialize;                     // u >= 1: the selection of sweep j-1 is computed here
     int draw;                            // sweep j+1 is a RANSAC sweep of the run
+    int prev_spatial;                    // sweep j-1 was a spatial sweep (its update advances the parents' seeds)
 };
 
 // node_group's selection of superpixel n from a chain's inputs; lane k of a 16-lane row owns
@@ -1190,8 +1199,8 @@ __device__ __forceinline__ void sel_load(const SweepArgs &a, const ChainIO &io, 
     in.deg = a.off[n + 1] - a.off[n];
     in.nb = io.nan_in[n];
     in.cost_in = io.co_in[n * K + kk];
-    in.fsum = acc_sum_at(io.acc_in, a.rrep, n, kk >= 2 * P ? kk : 2 * P);
-    in.pin = kk < 2 * P ? io.pl_in[n * K + kk] : io.cand_in[n * K + kk];
+    in.fsum = acc_sum_at(io.acc_in, a.rrep, n, kk >= 2 * P ? kk : 2 * P, io.as_in);
+    in.pin = kk < 2 * P ? io.pl_in[n * K + kk] : io.cand_in[n * io.cs_in + kk - (K - io.cs_in)];
     const float *__restrict__ mtp = io.mt_in + n * a.maxdeg * K + kk;
 #pragma unroll
     for (int i = 0; i < kNodeMaxDeg; ++i) in.mt[i] = mtp[i * K];
@@ -1272,7 +1281,7 @@ __device__ void edge_wave_fused(const SweepArgs &a, const ChainIO &io, long wv) 
     // t's message from s (on t's slot rslot), the points, s's candidates of this sweep
     const float mrow = io.mt_in[((long)b.y * a.maxdeg + rslot) * K + kk];
     const uint4 ps = io.pt_cur[b.x], pt = io.pt_cur[b.y];
-    const float4 cc = io.cand_cur[(long)b.x * K + (kk >= 2 * P ? kk : 2 * P)];
+    const float4 cc = io.cand_cur[(long)b.x * io.cs_cur + (kk >= 2 * P ? kk : 2 * P) - (K - io.cs_cur)];
     const SelOut ss = sel_compute(a, io, b.x, is, k, kk);
     const SelOut st = sel_compute(a, io, b.y, it, k, kk);
     // s's planes after its selection (slots < 2P) and this sweep's candidates
@@ -1321,6 +1330,12 @@ __device__ void chain_node(const SweepArgs &a, const ChainIO &io, long s) {
         constexpr LcgJump j3 = lcg_jump_coef(3 * P);
         s0 = (j3.a * s0 + j3.c) % 2147483648u;
         if (k == 2 * P) a.s_ransac[s] = s0;
+        // after a spatial sweep: the seeds of the parent with this index (node_group's rule)
+        if (io.prev_spatial && k == 0 && s < a.Vh) {
+            constexpr LcgJump jp = lcg_jump_coef(P);
+            a.s_child[s] = (jp.a * a.s_child[s] + jp.c) % 2147483648u;
+            a.s_neighbor[s] = (jp.a * a.s_neighbor[s] + jp.c) % 2147483648u;
+        }
     }
     if (io.acc_clear && cand) {
         acc_zero_at(io.acc_clear, a.rrep, s, k);
@@ -1340,14 +1355,151 @@ __device__ void chain_node(const SweepArgs &a, const ChainIO &io, long s) {
 }
 
 // ----------------------------------------------------------------------------------------
+// RANSAC plane costs ahead of the BP chain (tuning fuse = 2).  A level's seeds advance by a
+// fixed number of LCG draws per node update (3P before a RANSAC sweep, 4P before a random one,
+// none before a spatial one), so the seed from which each RANSAC sweep's candidates are drawn
+// is an affine map of the seed the level init leaves, and every RANSAC sweep's candidates,
+// points and plane costs can be computed before the sweeps run: the candidates in one launch
+// at the level's start, the costs by persistent launches on a second stream that run beside
+// the chain of message and node launches.  Sweep r of the level's RANSAC list (its first sweep
+// excluded, whose candidates the level init draws and costs) keeps its candidates [V][P], its
+// points [V], its sums [V][rrep][P] and NaN words [V] in slice r.
+// ----------------------------------------------------------------------------------------
+struct PipeArgs {
+    float4 *cand;                 // [nR][V][P]
+    uint4 *pt;                    // [nR][V]
+    unsigned long long *acc;      // [nR][V][rrep][P]
+    unsigned int *nan;            // [nR][V]
+    int rd_first, rd_count;       // the level's random sweeps: first iteration, count
+    long cslice, aslice;          // V P, V rrep P
+};
+
+// LCG steps from the seed the level init leaves to the seed listed sweep r's candidates are
+// drawn from: 3P per RANSAC sweep and 4P per random sweep before it (after the level's first
+// sweep).  A level's iterations hold 4 RANSAC sweeps each (listed sweep r is the level's
+// (r+1)-th, in iteration (r+1)/4) and its random sweeps end iterations rd_first ..
+// rd_first+rd_count-1.
+__host__ __device__ inline long ransac_draw_steps(int r, int rd_first, int rd_count) {
+    const int i = (r + 1) / 4;
+    const int nd = i - rd_first < 0 ? 0 : (i - rd_first > rd_count ? rd_count : i - rd_first);
+    return 3L * P * r + 4L * P * nd;
+}
+// n steps of the LCG as one affine map, by squaring (all such maps commute)
+__host__ __device__ inline LcgJump lcg_jump_pow(unsigned long n) {
+    LcgJump r{1u, 0u}, b{1103515245u, 12345u};
+    while (n) {
+        if (n & 1u) r = LcgJump{b.a * r.a, b.a * r.c + b.c};
+        b = LcgJump{b.a * b.a, b.a * b.c + b.c};
+        n >>= 1;
+    }
+    return r;
+}
+
+// candidates p of every RANSAC sweep r of the list for node n (node_group's draw)
+__device__ void draw_ahead(const SweepArgs &a, const PipeArgs &pa, long i) {
+    const int p = (int)(i % P);
+    const long rn = i / P;
+    const int n = (int)(rn % a.V), r = (int)(rn / a.V);
+    const int ppos = a.pix_off[n], pn = a.pix_off[n + 1] - ppos;
+    const LcgJump m = lcg_jump_pow((unsigned long)ransac_draw_steps(r, pa.rd_first, pa.rd_count));
+    const uint32_t s0 = (m.a * a.s_ransac[n] + m.c) % 2147483648u;
+    uint32_t sk = lcg_skip<3>(s0, p);
+    int p0, p1, p2;
+    ransac_draw(sk, pn, p0, p1, p2);
+    const uint4 q = make_uint4(a.pxy[ppos + p0], a.pxy[ppos + p1], a.pxy[ppos + p2], 0u);
+    const float4 A = make_float4((float)unpack_x(q.x), (float)unpack_y(q.x), a.pw[ppos + p0], 0.f);
+    const float4 B = make_float4((float)unpack_x(q.y), (float)unpack_y(q.y), a.pw[ppos + p1], 0.f);
+    const float4 C = make_float4((float)unpack_x(q.z), (float)unpack_y(q.z), a.pw[ppos + p2], 0.f);
+    pa.cand[r * pa.cslice + (long)n * P + p] = fit_plane(A, B, C);
+    if (p == P - 1) pa.pt[(long)r * a.V + n] = q;
+}
+
+// plane costs of the candidates of sweeps r0 .. r0+nr-1 (nr <= 64 / P) for one segment: the
+// segment's positions are read once, then each sweep's P planes as in pc_segment (the same
+// terms and sums); lane P g + k ends with sweep r0+g's sum of slot k, lane g with its NaN bits
+template <int MODE, int HB>
+__device__ void pc_segment_multi(const SweepArgs &a, const PipeArgs &pa, int w, int r0, int nr) {
+    const int lane = threadIdx.x & 63;
+    int4 sg = a.seg[w];
+    pin(sg.x); pin(sg.y); pin(sg.z); pin(sg.w);
+    if (sg.y <= sg.x) return;
+    const int c = __builtin_amdgcn_readfirstlane(sg.z);
+    const int first = __builtin_amdgcn_readfirstlane(sg.x), end = __builtin_amdgcn_readfirstlane(sg.y);
+    const int steps = (end - first + 63) >> 6;
+    const int full = (end - first) >> 6;
+    uint32_t q[SEG_R];
+#pragma unroll
+    for (int r = 0; r < SEG_R; ++r) {
+        const int j = first + r * 64 + lane;
+        q[r] = ld_at<uint32_t>(a.pxy, (uint32_t)(j < end ? j : end - 1) << 2);
+    }
+    const float4 *__restrict__ cp = pa.cand + r0 * pa.cslice + (long)c * P;
+    float4 pl[P], pn[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) pn[k] = cp[k];
+    unsigned int mynan = 0u;
+    unsigned long long mine = 0ull;
+#pragma unroll 1
+    for (int g = 0; g < nr; ++g) {
+        // this sweep's planes; the next sweep's are loaded now, in flight with this one's gathers
+#pragma unroll
+        for (int k = 0; k < P; ++k) pl[k] = pn[k];
+        const int gn = g + 1 < nr ? g + 1 : g;
+#pragma unroll
+        for (int k = 0; k < P; ++k) pn[k] = cp[gn * pa.cslice + k];
+        bool tame = a.D >= 2;
+#pragma unroll
+        for (int k = 0; k < P; ++k) tame = tame && tame_plane(pl[k]);
+        const bool fast = __builtin_amdgcn_readfirstlane((int)tame) != 0;
+        double acc[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc[k] = 0.0;
+        unsigned int nanm = 0u;
+#pragma unroll
+        for (int h = 0; h < SEG_R / HB; ++h) {
+            const int rb = h * HB;
+            const int rem = steps - rb;
+            if (rem <= 0) break;
+            if (rem >= HB) {
+                if (fast && rb + HB <= full) pc_batch<MODE, HB, false>(a, pl, 2 * P, first, end, rb, q + rb, acc, nanm);
+                else pc_batch<MODE, HB, true>(a, pl, 2 * P, first, end, rb, q + rb, acc, nanm);
+            } else if (rem == 1) {
+                pc_batch<MODE, 1, true>(a, pl, 2 * P, first, end, rb, q + rb, acc, nanm);
+            } else {
+                pc_batch<MODE, 2, true>(a, pl, 2 * P, first, end, rb, q + rb, acc, nanm);
+                if (rem == 3) pc_batch<MODE, 1, true>(a, pl, 2 * P, first, end, rb + 2, q + rb + 2, acc, nanm);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const long long m = (long long)wave_sum_exact(acc[k]);
+            if (lane == g * P + k) mine = (unsigned long long)(a.lambda < 0.f ? -m : m);
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) nanm |= __shfl_xor(nanm, o);
+        if (lane == g) mynan = nanm;
+    }
+    const int rep = w % a.rrep;
+    if (lane < nr * P && mine)
+        atomicAdd(&pa.acc[(r0 + lane / P) * pa.aslice + ((long)c * a.rrep + rep) * P + lane % P], mine);
+    if (lane < nr && mynan) atomicOr(&pa.nan[(long)(r0 + lane) * a.V + c], mynan);
+}
+
+// ----------------------------------------------------------------------------------------
 // kernels
 // ----------------------------------------------------------------------------------------
 #define GRID_STRIDE(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+// The launches of a pair's chain of sweeps raise their wavefronts' priority (s_setprio): with
+// fuse 2 the plane costs computed ahead on the second stream share the SIMDs, and the SIMD's
+// issue arbiter then serves the latency-bound chain first and the ahead launch with what is left.
+__device__ __forceinline__ void chain_priority() { __builtin_amdgcn_s_setprio(2); }
 
 __global__ __launch_bounds__(256) void init_level_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) init_node(a, id); }
 __global__ __launch_bounds__(256) void init_cost_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) init_cost_node(a, id); }
 __global__ __launch_bounds__(256) void spatial_kernel(SweepArgs a) { GRID_STRIDE(id, a.V) spatial_node(a, id); }
 __global__ __launch_bounds__(256) void node_kernel(SweepArgs a, int this_kind, int next_kind, float ndz, float ndn) {
+    chain_priority();
     const long s = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
     if (s < a.V) node_group<false>(a, s, this_kind, next_kind, ndz, ndn, NodeIO{});
 }
@@ -1367,6 +1519,7 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n) {
 // the sweep's messages alone, 4 waves of EDGES_PER_WAVE edges per block (fine levels: its
 // register budget is the message code's alone, for more resident waves)
 __global__ __launch_bounds__(256) void edge_kernel(SweepArgs a) {
+    chain_priority();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     edge_wave(a, (long)blockIdx.x * 4 + wave, false);
 }
@@ -1374,6 +1527,7 @@ __global__ __launch_bounds__(256) void edge_kernel(SweepArgs a) {
 template <int NS, bool COARSE, int MODE, int HB = 4>
 __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, int spatial, int npc) {
     extern __shared__ unsigned long long lds_pc[];
+    chain_priority();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = blockIdx.x;
     // plane-cost blocks first (XCD-contiguous), then the message blocks in dispatch order
@@ -1396,6 +1550,7 @@ __global__ __launch_bounds__(256) void cost_edge_kernel(SweepArgs a, int slot0, 
 // messages read the standard state (edge_wave) and whose node wavefronts only draw.
 template <bool FIRST, int MODE, int HB>
 __global__ __launch_bounds__(256) void chain_kernel(SweepArgs a, ChainIO io, int npc, int nedge_blocks) {
+    chain_priority();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = blockIdx.x;
     const int npc_blocks = (npc + 3) / 4;
@@ -1412,9 +1567,30 @@ __global__ __launch_bounds__(256) void chain_kernel(SweepArgs a, ChainIO io, int
     }
 }
 
+// every RANSAC candidate of the level's list (nR sweeps), P lanes per (sweep, node)
+__global__ __launch_bounds__(256) void draw_ahead_kernel(SweepArgs a, PipeArgs pa, int nR) {
+    GRID_STRIDE(i, (long)nR * a.V * P) draw_ahead(a, pa, i);
+}
+
+// persistent plane costs of sweeps r0 .. r0+nr-1 over the npc segments: the blocks of XCD x
+// (block b runs on XCD b mod 8) walk that XCD's contiguous share of the segments (the grid holds
+// at least 8 blocks, so every share has a block)
+template <int MODE, int HB>
+__global__ __launch_bounds__(256) void pc_ahead_kernel(SweepArgs a, PipeArgs pa, int r0, int nr, int npc) {
+    constexpr int X = 8;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int x = blockIdx.x % X;
+    const int nbx = ((int)gridDim.x - x + X - 1) / X;
+    const int qn = npc / X, rn = npc % X;
+    const int start = x * qn + (x < rn ? x : rn), len = qn + (x < rn ? 1 : 0);
+    for (int i = (int)(blockIdx.x / X) * 4 + wave; i < len; i += nbx * 4)
+        pc_segment_multi<MODE, HB>(a, pa, start + i, r0, nr);
+}
+
 // the node update that ends a run of fused sweeps: the run's buffers in, the standard state out
 __global__ __launch_bounds__(256) void node_io_kernel(SweepArgs a, NodeIO io, int this_kind, int next_kind, float ndz,
                                                       float ndn) {
+    chain_priority();
     const long s = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
     if (s < a.V) node_group<true>(a, s, this_kind, next_kind, ndz, ndn, io);
 }
@@ -1876,6 +2052,47 @@ void bp_setup(Ctx &c, BP &b) {
     b.disp.alloc(c.pool, N);
     b.plane3.alloc(c.pool, (size_t)N * 3);
     b.sched = schedule(b.L, b.min_level, b.D);
+    // per level, the RANSAC sweeps after its first and the seed maps of their draws: node
+    // update j draws sweep j+1's candidates from the seed it reads, then advances it by 3P
+    // draws before a RANSAC sweep, 4P before a random one, none before a spatial one, so every
+    // map is a number of LCG steps; draw_ahead takes that number from ransac_draw_steps, checked
+    // here against the walk over the schedule
+    b.ridx.assign(b.sched.size(), -1);
+    b.rcount.assign(b.L, 0);
+    b.rd_first.assign(b.L, 0);
+    b.rd_count.assign(b.L, 0);
+    {
+        int lv = -1, iter = 0;
+        for (const Sweep &sw : b.sched) {   // random sweeps: their count, in consecutive iterations
+            if (sw.first) {
+                lv = sw.level;
+                iter = 0;
+                b.rd_first[lv] = (10 + lv * 10) / 4 * 3 + 1;
+            }
+            if (sw.kind == kRandom) {   // it ends the iteration whose spatial sweep was counted
+                HBP_REQUIRE(iter - 1 == b.rd_first[lv] + b.rd_count[lv], "random sweeps outside the expected iterations");
+                ++b.rd_count[lv];
+            }
+            if (sw.kind == kSpatial) ++iter;
+        }
+        long steps = 0;
+        for (size_t j = 0; j < b.sched.size(); ++j) {
+            const Sweep &sw = b.sched[j];
+            if (sw.first) {
+                lv = sw.level;
+                steps = 0;
+                continue;
+            }
+            if (sw.kind == kRansac) {
+                const int r = b.rcount[lv]++;
+                b.ridx[j] = r;
+                HBP_REQUIRE(steps == ransac_draw_steps(r, b.rd_first[lv], b.rd_count[lv]),
+                            "seed map of a RANSAC sweep differs from its closed form");
+            }
+            steps += sw.kind == kRansac ? 3 * P : (sw.kind == kRandom ? 4 * P : 0);
+        }
+    }
+    b.pp_level = -1;
     b.next = 0;
     b.cur = b.L - 2;
     b.sweeps = 0;
@@ -1954,6 +2171,8 @@ static void run_level_init(Ctx &c, BP &b, const SweepArgs &a, int l) {
     HBP_CHECK(hipGetLastError());
 }
 
+static void level_start(Ctx &c, BP &b, const SweepArgs &a, int l);
+
 // one sweep as separate kernels
 static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float ndz, float ndn) {
     int l = sw.level;
@@ -1961,7 +2180,7 @@ static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float
     SweepArgs a = sweep_args(b, l);
     bool spatial = sw.kind == kSpatial;
     if (sw.first) {
-        run_level_init(c, b, a, l);
+        level_start(c, b, a, l);
         launch_cost_edge<P>(c, a, B, l, 2 * P, false, false, a.E > 0);
     } else {
         if (spatial && !B.coarse) {
@@ -2038,55 +2257,197 @@ static void chain_alloc(Ctx &c, BP &b) {
     b.ch_mt.alloc(c.pool, (mc + kNodeMaxDeg) * K);
 }
 
+// ---- RANSAC plane costs ahead (fuse 2), host side
+static double pc_ahead_bytes(long N, int V, int nr) { return 8.0 * N + nr * (4.0 * N + V * (24.0 * P + 4)); }
+
+static PipeArgs pipe_args(const BP &b, int l) {
+    const BPLevel &B = b.lv[l];
+    PipeArgs pa{};
+    pa.cand = b.pp_cand.p;
+    pa.pt = b.pp_pt.p;
+    pa.acc = b.pp_acc.p;
+    pa.nan = b.pp_nan.p;
+    pa.rd_first = b.rd_first[l];
+    pa.rd_count = b.rd_count[l];
+    pa.cslice = (long)B.V * P;
+    pa.aslice = (long)B.V * acc_replicas(B.V) * P;
+    return pa;
+}
+
+// the slices of the largest coarse level's list, made on first use
+static void pipe_alloc(Ctx &c, BP &b) {
+    if (b.pp_cand.p) return;
+    size_t nc = 1, na = 1, nv = 1;
+    for (int l = b.min_level; l < b.L; ++l) {
+        const BPLevel &B = b.lv[l];
+        if (!B.coarse) continue;
+        const size_t nR = (size_t)b.rcount[l];
+        nc = std::max(nc, nR * B.V * P);
+        na = std::max(na, nR * B.V * acc_replicas(B.V) * P);
+        nv = std::max(nv, nR * B.V);
+    }
+    b.pp_cand.alloc(c.pool, nc);
+    b.pp_pt.alloc(c.pool, nv);
+    b.pp_acc.alloc(c.pool, na);
+    b.pp_nan.alloc(c.pool, nv);
+}
+
+// Level start (after the level init, before any node update of the level): every listed
+// RANSAC sweep's candidates on the main stream, then on the second stream their sums cleared
+// and their plane costs in launches of pipe_batch sweeps, each followed by an event.
+static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
+    const BPLevel &B = b.lv[l];
+    const int nR = b.rcount[l];
+    b.pp_level = l;
+    b.pp_batches = 0;
+    b.pp_waited = -1;
+    if (nR == 0) return;
+    pipe_alloc(c, b);
+    const PipeArgs pa = pipe_args(b, l);
+    hipLaunchKernelGGL(draw_ahead_kernel, dim3(grid_for(c, (long)nR * a.V * P)), dim3(256), 0, c.stream, a, pa, nR);
+    HBP_CHECK(hipGetLastError());
+    hipStream_t s2 = c.side_stream();
+    // the second stream waits for the candidates (and for every earlier use of the slices)
+    HBP_CHECK(hipEventRecord(c.side_event(0), c.stream));
+    HBP_CHECK(hipStreamWaitEvent(s2, c.side_event(0), 0));
+    HBP_CHECK(hipMemsetAsync(pa.acc, 0, sizeof(unsigned long long) * (size_t)nR * pa.aslice, s2));
+    HBP_CHECK(hipMemsetAsync(pa.nan, 0, sizeof(unsigned int) * (size_t)nR * a.V, s2));
+    const int nb = c.pipe_batch;
+    const int mode = !a.use_cost_vol ? kCostWta : (a.volf ? kCostF32 : (a.right ? kCostRight : kCostLeft));
+    const int npc = B.nseg;
+    // at least one block per XCD: each XCD's blocks cost that XCD's share of the segments
+    const int blocks = std::max(8, std::min(c.pipe_blocks, div_up(npc, 4)));
+    const bool hb2 = c.batch_hb == 2;   // throughput-bound, like the batched launches
+    int bi = 0;
+    for (int r0 = 0; r0 < nR; r0 += nb, ++bi) {
+        const int nr = std::min(nb, nR - r0);
+        ProfScope ps(c, "plane_cost_ahead_L" + std::to_string(l), pc_ahead_bytes(a.N, a.V, nr), true);
+#define HBP_AHEAD(M)                                                                                             \
+    do {                                                                                                         \
+        if (hb2) HBP_LAUNCH(ps, (pc_ahead_kernel<M, 2>), dim3(blocks), dim3(256), 0, s2, a, pa, r0, nr, npc);    \
+        else HBP_LAUNCH(ps, (pc_ahead_kernel<M, 4>), dim3(blocks), dim3(256), 0, s2, a, pa, r0, nr, npc);        \
+    } while (0)
+        if (mode == kCostLeft) HBP_AHEAD(kCostLeft);
+        else if (mode == kCostRight) HBP_AHEAD(kCostRight);
+        else if (mode == kCostF32) HBP_AHEAD(kCostF32);
+        else HBP_AHEAD(kCostWta);
+#undef HBP_AHEAD
+        HBP_CHECK(hipGetLastError());
+        HBP_CHECK(hipEventRecord(c.side_event(1 + (size_t)bi), s2));
+    }
+    b.pp_batches = bi;
+    b.pp_batch = nb;
+}
+
+// the main stream waits for the launch that holds listed sweep r's sums
+static void pipe_wait(Ctx &c, BP &b, int r) {
+    const int bi = r / b.pp_batch;
+    if (bi <= b.pp_waited) return;
+    HBP_CHECK(hipStreamWaitEvent(c.stream, c.side_event(1 + (size_t)bi), 0));
+    b.pp_waited = bi;
+}
+
+// the main stream waits for everything issued on the second stream for this object
+static void pipe_drain(Ctx &c, BP &b) {
+    if (b.pp_batches > 0 && b.pp_waited < b.pp_batches - 1) pipe_wait(c, b, (b.pp_batches - 1) * b.pp_batch);
+}
+
+// level init, and the ahead work of fuse 2
+static void level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
+    run_level_init(c, b, a, l);
+    if (c.fuse == 2 && b.lv[l].coarse) pipe_level_start(c, b, a, l);
+}
+
 // sweeps j0 .. j0+n-1 (n >= 2) of one coarse level as a fused run (see ChainIO).  Launch u
-// writes its messages to MT(u) and its sums to ACC(u), reads the previous launch's from
-// MT(u-1) / ACC(u-1), and clears ACC(u-2); the selection that ends sweep j0+v goes to Q(v);
-// the candidates of sweep j0+u, u >= 1, are CAND(u) with RANSAC points PT(u).  The first
-// sweep's inputs and candidates are the standard state; the closing node update writes it.
+// (sweep j0+u) writes its messages to MT(u), reads the previous launch's from MT(u-1), and
+// recomputes the selection that ends sweep j0+u-1, which its node wavefronts write to Q(u-1).
+// The candidates, points and sums of sweep j0+u: u = 0, the standard state (candidates and
+// points from the previous node update or the level init, sums in the standard buffers);
+// with fuse 2, the listed RANSAC sweeps' slices (costed on the second stream); otherwise
+// CAND(u), PT(u) (drawn by launch u-1) and ACC(u) (costed by launch u, cleared by launch u+2).
+// The closing node update writes the standard state.
 static void run_chain(Ctx &c, BP &b, size_t j0, int n) {
     const Sweep &sw0 = b.sched[j0];
     const int l = sw0.level;
     const BPLevel &B = b.lv[l];
     const SweepArgs a = sweep_args(b, l);
     chain_alloc(c, b);
-    if (sw0.first) run_level_init(c, b, a, l);
+    if (sw0.first) level_start(c, b, a, l);
+    const bool pipe = c.fuse == 2 && b.pp_level == l;
+    const PipeArgs pa = pipe ? pipe_args(b, l) : PipeArgs{};
     const int uL = n - 1;
-    auto CAND = [&](int u) { return b.ch_cand[u % 3].p; };
-    auto PT = [&](int u) { return b.ch_pt[u % 2].p; };
-    auto ACC = [&](int u) { return u % 3 == 0 ? b.acc.p : b.ch_acc[u % 3 - 1].p; };
-    auto NANS = [&](int u) { return u % 3 == 0 ? b.accnan.p : b.ch_nan[u % 3 - 1].p; };
+    // where sweep j0+u keeps its candidates, points and sums
+    struct Loc {
+        const float4 *cand;
+        const uint4 *pt;
+        unsigned long long *acc;
+        unsigned int *nan;
+        int cstride, astride;   // slot strides of cand and acc: K, or P for a listed sweep's slice
+        int r;                  // index in the level's list when costed ahead, else -1
+    };
+    auto loc = [&](int u) -> Loc {
+        const int r = pipe ? b.ridx[j0 + (size_t)u] : -1;
+        if (u == 0) {
+            if (r >= 0) return Loc{a.plane, a.point, pa.acc + r * pa.aslice, pa.nan + (long)r * a.V, K, P, r};
+            return Loc{a.plane, a.point, b.acc.p, b.accnan.p, K, K, -1};
+        }
+        if (r >= 0) return Loc{pa.cand + r * pa.cslice, pa.pt + (long)r * a.V, pa.acc + r * pa.aslice,
+                               pa.nan + (long)r * a.V, P, P, r};
+        const int i = u % 3;
+        return Loc{b.ch_cand[i].p, b.ch_pt[u % 2].p, i == 0 ? b.acc.p : b.ch_acc[i - 1].p,
+                   i == 0 ? b.accnan.p : b.ch_nan[i - 1].p, K, K, -1};
+    };
+    // a standard-stride sum buffer is cleared by the second launch after the one that read it
+    // (in the one launch both ends of an edge and each node may still read it); the slices of
+    // the listed sweeps are cleared at the level start
     auto MT = [&](int u) { return u % 2 == 0 ? b.msg_t.p : b.ch_mt.p; };
     for (int u = 0; u <= uL; ++u) {
+        const Loc cu = loc(u);
         SweepArgs ax = a;
-        ax.acc = ACC(u);
-        ax.accnan = NANS(u);
+        ax.acc = cu.acc;
+        ax.accnan = cu.nan;
         ax.msg_t = MT(u);
         ChainIO io{};
         io.rloc = B.rloc.p;
-        io.draw = u < uL ? 1 : 0;
-        io.cand_next = CAND(u + 1);
-        io.pt_next = PT(u + 1);
+        // fuse 1: launch u draws sweep j0+u+1's candidates (the listed ones exist already)
+        const Loc nx = u < uL ? loc(u + 1) : cu;
+        io.draw = (u < uL && nx.r < 0) ? 1 : 0;
+        io.cand_next = (float4 *)nx.cand;
+        io.pt_next = (uint4 *)nx.pt;
+        // plane costs in this launch unless costed ahead or by the level init
+        const bool costs = cu.r < 0 && !(u == 0 && sw0.first);
         if (u == 0) {
-            launch_chain<true>(c, ax, io, B, l, !sw0.first);
+            // a spatial first sweep (fuse 2): its own launch, its node update done by launch 1
+            if (sw0.kind == kSpatial) launch_cost_edge<P>(c, ax, B, l, 2 * P, true, true, a.E > 0);
+            else launch_chain<true>(c, ax, io, B, l, costs);
             continue;
         }
-        ax.plane = CAND(u);
+        io.prev_spatial = (u == 1 && sw0.kind == kSpatial) ? 1 : 0;
+        const Loc cp = loc(u - 1);
+        if (cp.r >= 0) pipe_wait(c, b, cp.r);
+        ax.plane = (float4 *)cu.cand;   // segments (standard stride only)
         io.pl_in = u == 1 ? a.plane : b.ch_pl[u % 2].p;   // Q(u-2)
         io.co_in = u == 1 ? a.cost : b.ch_co[u % 2].p;
-        io.cand_in = u == 1 ? a.plane : CAND(u - 1);
-        io.acc_in = ACC(u - 1);
-        io.nan_in = NANS(u - 1);
+        io.cand_in = cp.cand;
+        io.cs_in = cp.cstride;
+        io.acc_in = cp.acc;
+        io.as_in = cp.astride;
+        io.nan_in = cp.nan;
         io.mt_in = MT(u - 1);
-        io.cand_cur = CAND(u);
-        io.pt_cur = PT(u);
+        io.cand_cur = cu.cand;
+        io.cs_cur = cu.cstride;
+        io.pt_cur = cu.pt;
         io.pl_out = b.ch_pl[(u - 1) % 2].p;   // Q(u-1)
         io.co_out = b.ch_co[(u - 1) % 2].p;
         io.materialize = 1;
         if (u >= 2) {
-            io.acc_clear = ACC(u - 2);
-            io.nan_clear = NANS(u - 2);
+            const Loc cc = loc(u - 2);
+            if (cc.r < 0) {
+                io.acc_clear = cc.acc;
+                io.nan_clear = cc.nan;
+            }
         }
-        launch_chain<false>(c, ax, io, B, l, true);
+        launch_chain<false>(c, ax, io, B, l, costs);
     }
     // the node update of the run's last sweep, into the standard state
     const size_t jL = j0 + (size_t)uL;
@@ -2099,11 +2460,18 @@ static void run_chain(Ctx &c, BP &b, size_t j0, int n) {
         ndz = nx.dz;
         ndn = nx.dn;
     }
+    const Loc cl = loc(uL), cp = loc(uL - 1);
+    if (cl.r >= 0) pipe_wait(c, b, cl.r);
     SweepArgs an = a;
-    an.acc = ACC(uL);
-    an.accnan = NANS(uL);
+    an.acc = cl.acc;
+    an.accnan = cl.nan;
     an.msg_t = MT(uL);
-    const NodeIO nio{b.ch_pl[(uL - 1) % 2].p, b.ch_co[(uL - 1) % 2].p, CAND(uL), PT(uL), ACC(uL - 1), NANS(uL - 1)};
+    NodeIO nio{b.ch_pl[(uL - 1) % 2].p, b.ch_co[(uL - 1) % 2].p, cl.cand, cl.pt, nullptr, nullptr, cl.cstride,
+               cl.astride};
+    if (cp.r < 0) {
+        nio.acc2 = cp.acc;
+        nio.nan2 = cp.nan;
+    }
     {
         ProfScope psn(c, "node_L" + std::to_string(l), node_bytes(a.E, a.V), true);
         HBP_LAUNCH(psn, node_io_kernel, dim3(div_up((long)a.V * 16, 256)), dim3(256), 0, c.stream, an, nio, swL.kind,
@@ -2117,9 +2485,11 @@ int bp_step(Ctx &c, BP &b, int max_sweeps) {
     while (done < max_sweeps && b.next < b.sched.size()) {
         const Sweep &sw = b.sched[b.next];
         const BPLevel &B = b.lv[sw.level];
-        // a fused run: this sweep (not spatial) and the RANSAC sweeps after it on its level
+        // a fused run: this sweep and the RANSAC sweeps after it on its level (a spatial first
+        // sweep only with fuse 2, whose RANSAC candidates exist before the run)
         int n = 1;
-        if (c.fuse && B.coarse && B.rloc.p && sw.kind != kSpatial) {
+        const bool ahead = c.fuse == 2 && b.pp_level == sw.level;
+        if (c.fuse && B.coarse && B.rloc.p && (sw.kind != kSpatial || ahead)) {
             while (done + n < max_sweeps && b.next + n < b.sched.size()) {
                 const Sweep &nx = b.sched[b.next + n];
                 if (nx.first || nx.level != sw.level || nx.kind != kRansac) break;
@@ -2145,6 +2515,8 @@ int bp_step(Ctx &c, BP &b, int max_sweeps) {
         b.sweeps += n;
     }
     if (b.next >= b.sched.size()) b.cur = b.min_level - 1;
+    // nothing of this object stays queued on the second stream past the call
+    pipe_drain(c, b);
     return done;
 }
 

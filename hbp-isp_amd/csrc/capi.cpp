@@ -112,6 +112,7 @@ int hbp_inject_fault(const char *what, long long after) {
 // a context's stream, pools and pinned words, after its queued work
 static void ctx_teardown(Ctx &c) {
     HBP_CHECK(hipStreamSynchronize(c.stream));
+    c.free_side();
     c.scratch.release();
     c.counters.release();
     c.free_host_box();
@@ -296,14 +297,20 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
         HBP_REQUIRE(value == 0 || value == 1, "batch_graph must be 0 or 1");
         ctx->c.batch_graph = value;
     } else if (n == "fuse") {
-        HBP_REQUIRE(value == 0 || value == 1, "fuse must be 0 or 1");
+        HBP_REQUIRE(value >= 0 && value <= 2, "fuse must be 0, 1 or 2");
         ctx->c.fuse = value;
+    } else if (n == "pipe_batch") {
+        HBP_REQUIRE(value >= 1 && value <= 16, "pipe_batch must be in [1, 16]");
+        ctx->c.pipe_batch = value;
+    } else if (n == "pipe_blocks") {
+        HBP_REQUIRE(value >= 8 && value <= 65536, "pipe_blocks must be in [8, 65536]");
+        ctx->c.pipe_blocks = value;
     } else if (n.rfind("level_form.", 0) == 0) {
         const int l = level_form_index(n);
         HBP_REQUIRE(value >= -1 && value <= 1, "a level form is -1 (by coarse_min_pix), 0 (chunks) or 1 (segments)");
         ctx->c.level_form[l] = value;
     } else {
-        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, level_form.<l>)"};
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, level_form.<l>)"};
     }
     HBP_API_END
 }
@@ -318,9 +325,11 @@ int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
     else if (n == "hamming_cols") *value = ctx->c.hamming_cols;
     else if (n == "batch_graph") *value = ctx->c.batch_graph;
     else if (n == "fuse") *value = ctx->c.fuse;
+    else if (n == "pipe_batch") *value = ctx->c.pipe_batch;
+    else if (n == "pipe_blocks") *value = ctx->c.pipe_blocks;
     else if (n == "seg_len") *value = ctx->c.seg_len;
     else if (n.rfind("level_form.", 0) == 0) *value = ctx->c.level_form[level_form_index(n)];
-    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, level_form.<l>)"};
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, level_form.<l>)"};
     HBP_API_END
 }
 
@@ -726,6 +735,8 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
         h.hamming_cols = m.hamming_cols;
         h.seg_len = m.seg_len;
         h.fuse = m.fuse;
+        h.pipe_batch = m.pipe_batch;
+        h.pipe_blocks = m.pipe_blocks;
         std::memcpy(h.level_form, m.level_form, sizeof(h.level_form));
     }
     std::vector<PreparedPair> done(n);

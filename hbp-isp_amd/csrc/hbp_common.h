@@ -359,8 +359,35 @@ struct Ctx {
     int batch_graph = 0;
     // one-pair BP on coarse levels: 1 runs each RANSAC sweep that follows another non-spatial
     // sweep of its level as one launch (the previous sweep's node update recomputed where it is
-    // read), 0 as two launches like every other sweep (the same bits)
-    int fuse = 1;
+    // read), 0 as two launches like every other sweep; 2 as 1, with every RANSAC sweep's plane
+    // costs computed ahead on a second stream (the same bits in every form)
+    int fuse = 2;
+    // fuse 2: RANSAC sweeps per plane-cost launch of the second stream, and the blocks of
+    // such a persistent launch (fewer leave room on every CU for the chain's launches)
+    int pipe_batch = 4, pipe_blocks = 512;
+    // the second stream (default priority) and events on it: [0] marks the main stream's
+    // level start, [1 + i] the end of the i-th plane-cost launch of a level; made on first use
+    hipStream_t side = nullptr;
+    std::vector<hipEvent_t> side_ev;
+    hipStream_t side_stream() {
+        if (!side) HBP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        return side;
+    }
+    hipEvent_t side_event(size_t i) {
+        while (side_ev.size() <= i) {
+            hipEvent_t e;
+            HBP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            side_ev.push_back(e);
+        }
+        return side_ev[i];
+    }
+    void free_side() {
+        if (side) (void)hipStreamSynchronize(side);
+        for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);
+        side_ev.clear();
+        if (side) (void)hipStreamDestroy(side);
+        side = nullptr;
+    }
     // per BP level: -1 the threshold above decides, 0 chunks of sorted pixels, 1 per-superpixel
     // segments (a schedule of forms the tuner can set level by level; the bits are the same)
     static constexpr int kFormLevels = 24;

@@ -92,6 +92,16 @@ struct BP {
     DevBuf<float> ch_mt;
     DevBuf<unsigned long long> ch_acc[2];
     DevBuf<unsigned int> ch_nan[2];
+    // RANSAC plane costs ahead (fuse 2): per sweep of sched its index in its level's list of
+    // RANSAC sweeps (the level's first sweep excluded) or -1; per level the list's length and
+    // the first iteration and count of its random sweeps; the slices of the level pp_level
+    // (bp.hip, PipeArgs), the launches issued for it and the last one the main stream waited for
+    std::vector<int> ridx, rcount, rd_first, rd_count;
+    DevBuf<float4> pp_cand;
+    DevBuf<uint4> pp_pt;
+    DevBuf<unsigned long long> pp_acc;
+    DevBuf<unsigned int> pp_nan;
+    int pp_level = -1, pp_batches = 0, pp_batch = 4, pp_waited = -1;
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;

@@ -220,6 +220,7 @@ Server::~Server() {
     admit_ctx.scratch.release();
     admit_ctx.counters.release();
     admit_ctx.free_host_box();
+    admit_ctx.free_side();
     admit_ctx.pool.release_all();
     if (admit_ctx.stream) (void)hipStreamDestroy(admit_ctx.stream);
 }

@@ -238,7 +238,8 @@ class Ctx:
         _check(self.lib.hbp_ctx_sync(self.h))
 
     # ---- kernel-form tuning (results are the same for every value)
-    TUNING = ("coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len", "batch_graph", "fuse")
+    TUNING = ("coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len", "batch_graph", "fuse",
+              "pipe_batch", "pipe_blocks")
 
     def set_tuning(self, name, value):
         _check(self.lib.hbp_ctx_set_tuning(self.h, name.encode(), int(value)))

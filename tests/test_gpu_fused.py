@@ -1,8 +1,9 @@
-"""Fused sweeps (tuning "fuse", on by default): on segment-form BP levels a run of RANSAC sweeps
-takes one launch per sweep, the previous sweep's node update recomputed where it is read.  The
-results must be the bits of the two-launch sweeps for every cost form (census of either view,
-the WTA indicator, a float volume), both segment step batches, any split of the schedule into
-hbp_bp_step calls, and the oracle's."""
+"""Fused sweeps (tuning "fuse"): on segment-form BP levels a run of RANSAC sweeps takes one launch
+per sweep, the previous sweep's node update recomputed where it is read (fuse 1), and with fuse 2
+(the default) every RANSAC sweep's plane costs are computed ahead on a second stream.  The
+results must be the bits of the two-launch sweeps (fuse 0) for every cost form (census of either
+view, the WTA indicator, a float volume), both segment step batches, any split of the schedule
+into hbp_bp_step calls, any batch of sweeps per ahead launch, and the oracle's."""
 import sys
 
 import numpy as np
@@ -60,29 +61,49 @@ def test_fused_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_v
     # every level in the segment form, so that runs are fused on every BP level
     tuned.set_tuning("coarse_min_pix", 1)
     tuned.set_tuning("single_hb", hb)
+    tuned.set_tuning("batch_hb", 6 - hb)
     out = {}
-    for fuse in (0, 1):
+    for fuse in (0, 1, 2):
         tuned.set_tuning("fuse", fuse)
         out[fuse] = make_bp(tuned, left, right, D, ml, right_view, cost, **kw).run()
-    assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][1], out[1][1])
+    for fuse in (1, 2):
+        assert same_bits(out[0][0], out[fuse][0]) and same_bits(out[0][1], out[fuse][1]), fuse
 
 
+@pytest.mark.parametrize("batch,blocks", [(1, 8), (3, 64), (7, 512), (16, 4096)])
+def test_ahead_launch_shapes_give_the_same_bits(tuned, batch, blocks):
+    say(f"ahead batch={batch} blocks={blocks}")
+    left, right, _ = synth.make_pair(79, 320, 240, 48, (3, 8))
+    tuned.set_tuning("coarse_min_pix", 1)
+    tuned.set_tuning("fuse", 0)
+    d0, p0 = make_bp(tuned, left, right, 48, 1).run()
+    tuned.set_tuning("fuse", 2)
+    tuned.set_tuning("pipe_batch", batch)
+    tuned.set_tuning("pipe_blocks", blocks)
+    d1, p1 = make_bp(tuned, left, right, 48, 1).run()
+    assert same_bits(d0, d1) and same_bits(p0, p1)
+
+
+@pytest.mark.parametrize("fuse", [1, 2])
 @pytest.mark.parametrize("chunk", [2, 3, 5, 7, 13])
-def test_fused_state_after_every_step_call(tuned, chunk):
+def test_fused_state_after_every_step_call(tuned, chunk, fuse):
     """hbp_bp_step(n) ends a fused run where the call ends: the state read back after each call
     is the split sweeps' state, planes, costs, beliefs and messages alike."""
-    say(f"fused state, steps of {chunk}")
+    say(f"fused {fuse} state, steps of {chunk}")
     left, right, _ = synth.make_pair(76, 288, 216, 40, (3, 8))
     tuned.set_tuning("coarse_min_pix", 1)
-    bps = {}
-    for fuse in (0, 1):
-        tuned.set_tuning("fuse", fuse)
-        bps[fuse] = make_bp(tuned, left, right, 40, 2)
+    modes = (0, fuse)   # the form is the context's at each step call
+    bps = [make_bp(tuned, left, right, 40, 2) for _ in modes]
+
+    def step(i, n):
+        tuned.set_tuning("fuse", modes[i])
+        return bps[i].step(n)
+
     total = bps[0].total_sweeps
     done = 0
     calls = 0
     while done < total and calls < 40:
-        n = [bps[f].step(chunk) for f in (0, 1)]
+        n = [step(i, chunk) for i in (0, 1)]
         assert n[0] == n[1]
         done += n[0]
         calls += 1
@@ -90,12 +111,16 @@ def test_fused_state_after_every_step_call(tuned, chunk):
         assert s0["level"] == s1["level"]
         for key in ("planes", "cost", "belief", "msg"):
             assert same_bits(s0[key], s1[key]), f"{key} differs after {done} sweeps"
+    step(0, 1 << 30)
+    step(1, 1 << 30)
     d0, p0 = bps[0].run()
     d1, p1 = bps[1].run()
     assert same_bits(d0, d1) and same_bits(p0, p1)
 
 
-@pytest.mark.parametrize("seed,W,H,D,ml", [(77, 200, 150, 32, 1), (78, 320, 240, 64, 2)])
+# the 90x70 pair's segment-form levels have about 20 segments: fewer ahead blocks than XCDs
+@pytest.mark.parametrize("seed,W,H,D,ml", [(77, 200, 150, 32, 1), (78, 320, 240, 64, 2), (74, 90, 70, 16, 2),
+                                           (75, 96, 72, 24, 2)])
 def test_fused_runs_match_the_oracle(tuned, seed, W, H, D, ml):
     say(f"fused vs oracle seed={seed} {W}x{H} D={D} ml={ml}")
     left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
@@ -104,7 +129,8 @@ def test_fused_runs_match_the_oracle(tuned, seed, W, H, D, ml):
     bo.step()
     do, plo = bo.map()
     for cmp in (1, 96):
-        tuned.set_tuning("coarse_min_pix", cmp)
-        tuned.set_tuning("fuse", 1)
-        d, p = make_bp(tuned, left, right, D, ml).run()
-        assert same_bits(d, do) and same_bits(p, plo), cmp
+        for fuse in (1, 2):
+            tuned.set_tuning("coarse_min_pix", cmp)
+            tuned.set_tuning("fuse", fuse)
+            d, p = make_bp(tuned, left, right, D, ml).run()
+            assert same_bits(d, do) and same_bits(p, plo), (cmp, fuse)

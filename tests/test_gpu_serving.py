@@ -4,6 +4,7 @@ launches), cancellation of queued and running pairs, and contexts that refuse to
 under live handles.  Every kernel exits normally in these tests: a forced failure throws on the
 host before the work it stands for is queued."""
 import ctypes as C
+import time
 
 import numpy as np
 import pytest
@@ -61,12 +62,23 @@ def _disarm():
     H.inject_fault("round", 0)
 
 
-def _step_until_active(srv, n_active, limit=100000):
+def _step_until_active(srv, n_active, limit=20000, tickets=()):
+    """Steps until n_active pairs are on the device.  Between steps the main stream drains: the
+    admission stream may share a hardware queue with it (a process holds more streams than the
+    device's hardware queues), and then admission work runs only behind the queued rounds."""
+    a = p = 0
     for _ in range(limit):
         a, p, _ = srv.step()
         if a >= n_active and p == 0:
             return
-    raise AssertionError("pairs were not admitted")
+        time.sleep(0.0005)
+    errors = {}
+    for t in tickets:
+        try:
+            errors[t] = "done" if srv.poll(t) is not None else "pending"
+        except H.HbpError as e:
+            errors[t] = str(e)
+    raise AssertionError(f"pairs were not admitted: active {a}, queued {p}, requests {errors}")
 
 
 def test_allocation_failure_fails_one_admission_only(ctx):
@@ -94,13 +106,15 @@ def test_allocation_failure_fails_one_admission_only(ctx):
 
 def test_round_failure_fails_its_pairs_and_the_server_keeps_serving(ctx):
     say("server: injected round failure")
-    a = (_pair(111, 96, 72, 24), 24, 2)
+    # a's rounds (min_level 1 on 320x240: about 1500 sweeps) outlast b's admission, so both are
+    # on the device together before the failing round
+    a = (_pair(111, 320, 240, 48), 48, 1)
     b = (_pair(112, 64, 48, 16), 16, 1)
     c = (_pair(113, 120, 90, 32), 32, 2)
     srv = H.Server(ctx, max_active=4)
     ta = srv.submit(a[0][0], a[0][1], a[1], a[2])
     tb = srv.submit(b[0][0], b[0][1], b[1], b[2])
-    _step_until_active(srv, 2)
+    _step_until_active(srv, 2, tickets=(ta, tb))
     H.inject_fault("round", 1)
     out = _drive(srv, [ta, tb])
     assert all(isinstance(out[t], H.HbpError) and "round" in str(out[t]) for t in (ta, tb))

@@ -36,7 +36,7 @@ def tuned(ctx):
 
 
 SETTINGS = [dict(coarse_min_pix=96, single_hb=4, batch_hb=2, seg_len=512, batch_graph=0, fuse=0),
-            dict(coarse_min_pix=1, single_hb=2, batch_hb=4, seg_len=64, batch_graph=1, fuse=1),
+            dict(coarse_min_pix=1, single_hb=2, batch_hb=4, seg_len=64, batch_graph=1, fuse=2),
             dict(coarse_min_pix=1 << 24, single_hb=4, batch_hb=4, seg_len=256, batch_graph=0, fuse=1),
             dict(coarse_min_pix=40, single_hb=2, batch_hb=2, seg_len=320, batch_graph=1, fuse=0),
             dict(coarse_min_pix=96, single_hb=4, batch_hb=2, seg_len=448, batch_graph=1, fuse=1)]
@@ -110,12 +110,16 @@ def test_tuning_parameters_are_checked(tuned):
         tuned.set_tuning("batch_graph", 2)
     with pytest.raises(H.HbpError, match="fuse"):
         tuned.set_tuning("fuse", 3)
+    with pytest.raises(H.HbpError, match="pipe_batch"):
+        tuned.set_tuning("pipe_batch", 17)
+    with pytest.raises(H.HbpError, match="pipe_blocks"):
+        tuned.set_tuning("pipe_blocks", 4)
     with pytest.raises(H.HbpError, match="level form"):
         tuned.set_tuning("level_form.3", 2)
     with pytest.raises(H.HbpError, match="out of range"):
         tuned.set_tuning("level_form.24", 0)
     assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len",
-                                      "batch_graph", "fuse"}
+                                      "batch_graph", "fuse", "pipe_batch", "pipe_blocks"}
 
 
 def test_tuner_sets_the_fastest_grid_point(tuned):
