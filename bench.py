@@ -27,6 +27,13 @@ HBM_PEAK_GBS = 8000.0
 # with 1754 launches per pair ten steps still time about 600 launches while the events' own
 # cost (about 8 us per sampled launch) stays near 1 % of the step
 EVENT_EVERY = 29
+# the RANSAC plane costs computed ahead (340 launches per pair, on the second stream): one in 7
+AHEAD_EVENT_EVERY = 7
+# wavefront VALU instructions per second of the whole chip: 256 CUs x 4 SIMDs, one wave64
+# instruction per SIMD every 4 cycles at 2.4 GHz
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
+# profile names of the kernels whose counter summaries profiles/ holds, and their rocprof names
+PROFILE_KERNELS = {"plane_cost": "cost_edge_kernel<4, true, 0, 4>", "plane_cost_ahead": "pc_ahead_kernel<0, 2>"}
 # host threads (each with a stream of its own) that build the batched leg's pairs
 PREP_WORKERS = 8
 METRIC = "stereo pairs/s + Mpix·disp/s; p50/p99 latency, batched/served vs plain"
@@ -78,6 +85,11 @@ def main():
     ap.add_argument("--batch-kernels", action="store_true",
                     help="only the batched BP kernels over the 64-pair mixed batch: each kernel's time "
                          "(HIP events around every launch), launches and algorithmic GB/s against the HBM peak")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "PAIRS"), default=None,
+                    help="per-pair table of a rocprofv3 --kernel-trace --stats kernel_stats.csv over PAIRS pairs")
+    ap.add_argument("--valu-summary", metavar="VALU_CSV", default=None,
+                    help="summarise a rocprofv3 --pmc SQ_INSTS_VALU CSV: wavefront VALU instructions per launch "
+                         "of the kernel --traffic-kernel names")
     ap.add_argument("--traffic-kernel", default=None,
                     help="with --traffic-summary: the kernel (a name prefix) to summarise instead of the plain path's "
                          "plane-cost kernel")
@@ -92,6 +104,12 @@ def main():
 
     if args.traffic_summary:
         print(json.dumps(traffic_summary(*args.traffic_summary, kernel=args.traffic_kernel), indent=1))
+        return
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary[0], int(args.trace_summary[1])))
+        return
+    if args.valu_summary:
+        print(json.dumps(valu_summary(args.valu_summary, kernel=args.traffic_kernel), indent=1))
         return
 
     import ctypes as C
@@ -174,14 +192,25 @@ def main():
         from hbp_amd import tune
         tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2), "seg_len": (512, 384, 256)})
         tuning["levels"] = tune.tune_levels(ctx, left, right, D, info["min_level"])
+        tuning["pipe"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid=tune.PIPE_GRID)
         tuning["hamming"] = tune.tune_hamming(ctx, left, right, D)
     for _ in range(args.warmup):
         one_pair()
     ctx.sync()
-    # timed region: HIP events around the dominant kernel's launches only
-    dom = "plane_cost"
+    # the dominant kernel: the most time in one pair with every kernel timed (outside the timed
+    # region); with the RANSAC plane costs computed ahead on the second stream it is that launch
     ctx.profile(True)
-    ctx.profile_only(dom, EVENT_EVERY)
+    ctx.profile_reset()
+    one_pair()
+    ctx.sync()
+    first_stats = ctx.profile_stats()
+    ctx.profile(False)
+    dom = max(group(first_stats).items(), key=lambda kv: kv[1]["ms"])[0]
+    # the ahead launches are few (one per 4 sweeps) and off the chain's critical path: sample more
+    every = EVENT_EVERY if dom == "plane_cost" else AHEAD_EVENT_EVERY
+    # timed region: HIP events around the dominant kernel's launches only
+    ctx.profile(True)
+    ctx.profile_only(dom, every)
     ctx.profile_reset()
     elapsed, lat = timed(args.steps)
     stats = ctx.profile_stats()
@@ -247,16 +276,6 @@ def main():
     p50 = statistics.median(lat_ms)
     p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
 
-    def group(st):
-        out = {}
-        for name, v in st.items():
-            g = name.split("_L")[0]
-            a = out.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
-            a["ms"] += v["ms"]
-            a["bytes"] += v["bytes"]
-            a["launches"] += v["launches"]
-        return out
-
     gd = group(stats).get(dom, dict(ms=0.0, bytes=0.0, launches=0))
     avg_ms = gd["ms"] / max(gd["launches"], 1)
     avg_bytes = gd["bytes"] / max(gd["launches"], 1)
@@ -265,14 +284,26 @@ def main():
                        gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None)
                for g, v in sorted(group(pair_stats).items(), key=lambda kv: -kv[1]["ms"])}
 
-    # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    traffic = None
+    # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
+    # and its VALU instructions per launch (SQ_INSTS_VALU), from the committed summaries
+    traffic, valu = None, None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
+    vfile = os.path.join(ROOT, "profiles", f"valu_{dom}_{args.workload}.json")
+    if os.path.exists(vfile) and avg_ms > 0:
+        try:
+            vi = json.load(open(vfile)).get("valu_instr_per_launch")
+            if vi:
+                rate = vi / (avg_ms * 1e-3)
+                valu = {"achieved": round(rate / 1e9, 1), "peak": round(VALU_PEAK_WAVE_INSTR / 1e9, 1),
+                        "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_WAVE_INSTR, 4),
+                        "instr_per_launch": vi}
+        except (OSError, ValueError):
+            valu = None
 
     post, post_io = (None, None) if args.no_post else post_process(ctx, left, right, D, info["min_level"])
     if batch_pairs and not args.no_autotune:
@@ -313,7 +344,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": round(avg_bytes),
                      "avg_launch_us": round(1e3 * avg_ms, 2), "launches_timed": gd["launches"],
-                     "event_sampling": f"1 in {EVENT_EVERY} launches in the timed region"},
+                     "event_sampling": f"1 in {every} launches in the timed region", "valu": valu},
         "kernels_one_pair": kernels,
         "cost_volume_kernels": cost_kernels,
         "launch_us_by_level": {n: round(1e3 * v["ms"] / max(v["launches"], 1), 2)
@@ -335,6 +366,18 @@ DOMINANT_KERNEL = "cost_edge_kernel<4, true, 0, 4>"   # plane cost, coarse form,
 CALIB_KIB = 262144.0                                  # the calibration copy reads and writes 256 MiB
 
 
+def group(st):
+    """Per-kernel profile entries summed over BP levels (names "<kernel>_L<level>")."""
+    out = {}
+    for name, v in st.items():
+        g = name.split("_L")[0]
+        a = out.setdefault(g, dict(ms=0.0, bytes=0.0, launches=0))
+        a["ms"] += v["ms"]
+        a["bytes"] += v["bytes"]
+        a["launches"] += v["launches"]
+    return out
+
+
 def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=None):
     """HBM bytes per launch of `kernel`: the mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
     over its launches, each scaled by the ratio of the known bytes to the counter's reading on
@@ -351,7 +394,7 @@ def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=N
                     grids.setdefault(row["Grid_Size"], []).append(v)
         return vals, grids
 
-    kernel = kernel or DOMINANT_KERNEL
+    kernel = PROFILE_KERNELS.get(kernel, kernel) if kernel else DOMINANT_KERNEL
     cf, _ = per_launch(cal_fetch_csv, "calib_copy_kernel")
     cw, _ = per_launch(cal_write_csv, "calib_copy_kernel")
     sf = CALIB_KIB / (sum(cf) / len(cf))
@@ -368,6 +411,36 @@ def traffic_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, kernel=N
             "by_grid_size": {g: {"launches": len(v), "fetch_kib": round(sum(v) / len(v), 2),
                                  "write_kib": round(sum(wg.get(g, [0])) / max(len(wg.get(g, [])), 1), 2)}
                              for g, v in sorted(fg.items())}}
+
+
+def trace_summary(stats_csv, pairs):
+    """Launches per pair, mean duration and time per pair of every kernel in a rocprofv3
+    kernel_stats.csv that covers `pairs` pairs."""
+    import csv
+    rows = []
+    with open(stats_csv) as f:
+        for row in csv.DictReader(f):
+            rows.append((row["Name"], int(row["Calls"]), float(row["TotalDurationNs"])))
+    rows.sort(key=lambda r: -r[2])
+    total = sum(r[2] for r in rows)
+    lines = [f"total kernel time {total / 1e6:.3f} ms over {pairs} pairs = {total / 1e6 / pairs:.3f} ms per pair "
+             "(kernels of both streams, overlapping ones counted each)",
+             "  per pair    avg_us  ms/pair  kernel"]
+    for name, calls, ns in rows:
+        lines.append(f"{calls / pairs:10.1f} {ns / calls / 1e3:9.2f} {ns / 1e6 / pairs:8.3f}  {name[:110]}")
+    return "\n".join(lines)
+
+
+def valu_summary(valu_csv, kernel=None):
+    """Wavefront VALU instructions per launch of `kernel`: the mean SQ_INSTS_VALU per dispatch."""
+    import csv
+    kernel = PROFILE_KERNELS.get(kernel, kernel) if kernel else DOMINANT_KERNEL
+    vals = []
+    with open(valu_csv) as f:
+        for row in csv.DictReader(f):
+            if kernel in row["Kernel_Name"]:
+                vals.append(float(row["Counter_Value"]))
+    return {"kernel": kernel, "valu_instr_per_launch": round(sum(vals) / len(vals)), "launches": len(vals)}
 
 
 def batch_spec(n, seed):

@@ -2258,7 +2258,9 @@ static void chain_alloc(Ctx &c, BP &b) {
 }
 
 // ---- RANSAC plane costs ahead (fuse 2), host side
-static double pc_ahead_bytes(long N, int V, int nr) { return 8.0 * N + nr * (4.0 * N + V * (24.0 * P + 4)); }
+// algorithmic bytes of one ahead launch: each pixel's position and two census words once
+// (the sweeps re-gather the same right census rows), per sweep the planes, sums and NaN words
+static double pc_ahead_bytes(long N, int V, int nr) { return 12.0 * N + nr * (V * (24.0 * P + 4)); }
 
 static PipeArgs pipe_args(const BP &b, int l) {
     const BPLevel &B = b.lv[l];
@@ -2317,7 +2319,7 @@ static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
     const int npc = B.nseg;
     // at least one block per XCD: each XCD's blocks cost that XCD's share of the segments
     const int blocks = std::max(8, std::min(c.pipe_blocks, div_up(npc, 4)));
-    const bool hb2 = c.batch_hb == 2;   // throughput-bound, like the batched launches
+    const bool hb2 = c.pipe_hb == 2;
     int bi = 0;
     for (int r0 = 0; r0 < nR; r0 += nb, ++bi) {
         const int nr = std::min(nb, nR - r0);

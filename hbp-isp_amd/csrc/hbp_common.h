@@ -362,9 +362,10 @@ struct Ctx {
     // read), 0 as two launches like every other sweep; 2 as 1, with every RANSAC sweep's plane
     // costs computed ahead on a second stream (the same bits in every form)
     int fuse = 2;
-    // fuse 2: RANSAC sweeps per plane-cost launch of the second stream, and the blocks of
-    // such a persistent launch (fewer leave room on every CU for the chain's launches)
-    int pipe_batch = 4, pipe_blocks = 512;
+    // fuse 2: RANSAC sweeps per plane-cost launch of the second stream, the blocks of such a
+    // persistent launch (fewer leave room on every CU for the chain's launches) and the steps
+    // per batch of its segment wavefronts (2 or 4)
+    int pipe_batch = 4, pipe_blocks = 512, pipe_hb = 2;
     // the second stream (default priority) and events on it: [0] marks the main stream's
     // level start, [1 + i] the end of the i-th plane-cost launch of a level; made on first use
     hipStream_t side = nullptr;

@@ -10,13 +10,17 @@ cluster sums are exact integers in any order), so tuning only trades speed:
     batched launches: 4 keeps more gathers in flight, 2 fewer registers and more wavefronts;
   * seg_len: pixels per segment wavefront of the segment form (a multiple of 64 up to 512);
   * hamming_cols: columns per lane of the Hamming cost kernel (4 or 8);
-  * batch_graph: batched rounds queued one by one or replayed from one captured HIP graph.
+  * batch_graph: batched rounds queued one by one or replayed from one captured HIP graph;
+  * pipe_blocks / pipe_batch / pipe_hb: the persistent launches that compute RANSAC sweeps'
+    plane costs ahead on a second stream (fuse 2): their blocks (fewer leave more of every CU to
+    the chain of message and node launches), RANSAC sweeps per launch, steps per batch.
 The reference's kernels have one form each; this module has no counterpart there.
 """
 import itertools
 import time
 
 SINGLE_GRID = {"coarse_min_pix": (32, 64, 96, 128, 256), "single_hb": (4, 2)}
+PIPE_GRID = {"pipe_blocks": (384, 512, 640, 768), "pipe_batch": (4, 8)}
 BATCH_GRID = {"coarse_min_pix": (64, 96, 128), "batch_hb": (2, 4)}
 
 

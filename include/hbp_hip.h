@@ -69,10 +69,14 @@ int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long 
  *      Hamming cost kernel; "seg_len" (a multiple of 64 in [64, 512], default 512): pixels per
  *      segment wavefront of the segment form; "batch_graph" (0 or 1, default 0): hbp_bp_run_batch queues its
  *      rounds one by one from the host (0) or captures the whole schedule of rounds in one HIP
- *      graph and replays it (1); "fuse" (0 or 1, default 1): one-pair BP on segment-form levels
+ *      graph and replays it (1); "fuse" (0, 1, 2): one-pair BP on segment-form levels
  *      runs each RANSAC sweep that follows another non-spatial sweep of its level as one launch,
  *      the previous sweep's node update recomputed where it is read (1), or every sweep as two
- *      launches (0); "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
+ *      launches (0); 2 (the default) as 1, with every RANSAC sweep's plane costs computed at the
+ *      level's start on a second stream of the context; "pipe_batch" (1..16, default 4),
+ *      "pipe_blocks" (8..65536, default 512), "pipe_hb" (2 or 4, default 2): RANSAC sweeps per
+ *      such launch, its persistent blocks and the steps per batch of its segment wavefronts;
+ *      "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
  *      planes by coarse_min_pix's rule, in chunks of sorted pixels, or in per-superpixel
  *      segments.  Applies to BP objects created afterwards (and to the batched
  *      rounds run afterwards).  hbp_amd.tune picks them by timing the caller's workload. */

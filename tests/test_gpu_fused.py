@@ -61,7 +61,7 @@ def test_fused_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_v
     # every level in the segment form, so that runs are fused on every BP level
     tuned.set_tuning("coarse_min_pix", 1)
     tuned.set_tuning("single_hb", hb)
-    tuned.set_tuning("batch_hb", 6 - hb)
+    tuned.set_tuning("pipe_hb", 6 - hb)
     out = {}
     for fuse in (0, 1, 2):
         tuned.set_tuning("fuse", fuse)

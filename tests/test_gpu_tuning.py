@@ -119,7 +119,8 @@ def test_tuning_parameters_are_checked(tuned):
     with pytest.raises(H.HbpError, match="out of range"):
         tuned.set_tuning("level_form.24", 0)
     assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len",
-                                      "batch_graph", "fuse", "pipe_batch", "pipe_blocks"}
+                                      "batch_graph", "fuse", "pipe_batch", "pipe_blocks",
+                                      "pipe_hb"}
 
 
 def test_tuner_sets_the_fastest_grid_point(tuned):
