@@ -1587,6 +1587,140 @@ __global__ __launch_bounds__(256) void pc_ahead_kernel(SweepArgs a, PipeArgs pa,
         pc_segment_multi<MODE, HB>(a, pa, start + i, r0, nr);
 }
 
+// ----------------------------------------------------------------------------------------
+// Tiny levels (fuse 2): a run's RANSAC sweeps after its first launch and the closing node update
+// in one workgroup of 1024 threads.  The RANSAC sweeps' plane costs exist (second stream), so
+// what remains per sweep is the selection of every superpixel (phase A, 16 lanes each) and the
+// messages of every edge (phase B, 16 lanes each), with a workgroup barrier between them instead
+// of a launch.  The selection's outputs (planes, costs, beliefs and source slots of slots < 2P)
+// and the sweep messages (two node-major buffers) live in LDS.  The arithmetic is node_group's and
+// edge_wave's: the same bits as the fused launches.  Limits: V <= RUN_V, maxdeg <= kNodeMaxDeg,
+// V maxdeg <= RUN_MT (the host checks them).
+// ----------------------------------------------------------------------------------------
+constexpr int RUN_V = 64, RUN_MT = 384;
+constexpr size_t RUN_LDS_BYTES = (size_t)RUN_V * 2 * P * (16 + 4 + 4 + 4) + 2 * (size_t)RUN_MT * K * 4;
+struct RunArgs {
+    const unsigned long long *acc0;   // sums of the run's first sweep (slot stride K), cleared at the end
+    unsigned int *nan0;
+    const float *mt0;                 // its messages (global, node-major)
+    const int *rloc;                  // per edge: the reverse edge's slot among its source's edges
+    int n;                            // RANSAC sweeps after the first
+    int r1;                           // list index of the first of them (the others follow)
+    int prev_spatial;                 // the first sweep is a spatial one
+    PipeArgs pa;
+};
+
+__global__ __launch_bounds__(1024) void run_kernel(SweepArgs a, RunArgs ra) {
+    chain_priority();
+    extern __shared__ unsigned long long lds_run[];
+    float4 *sp = (float4 *)lds_run;                    // [RUN_V][2P]
+    float *sc = (float *)(sp + RUN_V * 2 * P);         // [RUN_V][2P]
+    float *sb = sc + RUN_V * 2 * P;                    // [RUN_V][2P]
+    int *ss = (int *)(sb + RUN_V * 2 * P);             // [RUN_V][2P]
+    float *smt = (float *)(ss + RUN_V * 2 * P);        // [2][RUN_MT][K]
+    const PipeArgs &pa = ra.pa;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = lane & 15, g = lane & ~15;
+    const bool own = k < K;
+    const int kk = own ? k : 0;
+    const int row = wave * 4 + (lane >> 4);            // 64 rows of 16 lanes
+    for (int u = 1; u <= ra.n + 1; ++u) {
+        // ---- phase A: the selection that ends sweep j0+u-1 (u = n+1: the run's last, closing)
+        const int n = row;
+        if (n < a.V) {
+            const bool first = u == 1;                 // sweep j0+u-1 is the run's first
+            const float *mtp = first ? ra.mt0 : smt + (size_t)((u - 1) & 1) * RUN_MT * K;
+            const long nm = (long)n * a.maxdeg;
+            SelIn in;
+            in.deg = a.off[n + 1] - a.off[n];
+            const int rp = ra.r1 + u - 2;              // list index of sweep j0+u-1 (u >= 2)
+            in.nb = first ? ra.nan0[n] : pa.nan[(long)rp * a.V + n];
+            in.cost_in = first ? a.cost[(long)n * K + kk] : sc[n * 2 * P + (kk < 2 * P ? kk : 0)];
+            in.fsum = first ? acc_sum_at(ra.acc0, a.rrep, n, kk >= 2 * P ? kk : 2 * P)
+                            : acc_sum_at(pa.acc + rp * pa.aslice, a.rrep, n, kk >= 2 * P ? kk : 2 * P, P);
+            if (kk < 2 * P) in.pin = first ? a.plane[(long)n * K + kk] : sp[n * 2 * P + kk];
+            else in.pin = first ? a.plane[(long)n * K + kk] : pa.cand[rp * pa.cslice + (long)n * P + kk - 2 * P];
+#pragma unroll
+            for (int i = 0; i < kNodeMaxDeg; ++i) in.mt[i] = i < in.deg ? mtp[(nm + i) * K + kk] : 0.f;
+            ChainIO none{};
+            const SelOut so = sel_compute(a, none, n, in, k, kk);
+            const float4 pl_n = make_float4(__shfl(so.pl.x, g + so.src), __shfl(so.pl.y, g + so.src),
+                                            __shfl(so.pl.z, g + so.src), __shfl(so.pl.w, g + so.src));
+            const float co_n = __shfl(so.cost, g + so.src), be_n = __shfl(so.bel, g + so.src);
+            if (u <= ra.n) {
+                if (k < 2 * P) {
+                    sp[n * 2 * P + k] = pl_n;
+                    sc[n * 2 * P + k] = co_n;
+                    sb[n * 2 * P + k] = be_n;
+                    ss[n * 2 * P + k] = so.src;
+                }
+                // the selection's seed advance (its next sweep is a RANSAC sweep)
+                if (k == 2 * P) {
+                    constexpr LcgJump j3 = lcg_jump_coef(3 * P);
+                    a.s_ransac[n] = (j3.a * a.s_ransac[n] + j3.c) % 2147483648u;
+                }
+                if (first && ra.prev_spatial && k == 0 && n < a.Vh) {
+                    constexpr LcgJump jp = lcg_jump_coef(P);
+                    a.s_child[n] = (jp.a * a.s_child[n] + jp.c) % 2147483648u;
+                    a.s_neighbor[n] = (jp.a * a.s_neighbor[n] + jp.c) % 2147483648u;
+                }
+            } else {
+                // the closing node update (node_group, next sweep spatial or none): standard state
+                if (own) {
+                    a.plane[(long)n * K + k] = pl_n;
+                    a.cost[(long)n * K + k] = co_n;
+                    a.belief[(long)n * K + k] = be_n;
+                }
+                float *msg = (float *)a.msg;
+#pragma unroll
+                for (int i = 0; i < kNodeMaxDeg; ++i) {
+                    const float oi = __shfl(in.mt[i], g + so.src);
+                    if (i < in.deg && k < 2 * P) msg[(long)a.revl[nm + i] * 8 + k] = oi;
+                }
+                if (k == K - 1) a.point[n] = pa.pt[(long)(ra.r1 + ra.n - 1) * a.V + n];
+                if (kk >= 2 * P) acc_zero_at((unsigned long long *)ra.acc0, a.rrep, n, kk);
+                if (k == 2 * P) ra.nan0[n] = 0u;
+            }
+        }
+        __syncthreads();
+        if (u > ra.n) break;
+        // ---- phase B: the messages of sweep j0+u
+        const float *mtp = u == 1 ? ra.mt0 : smt + (size_t)((u - 1) & 1) * RUN_MT * K;
+        float *mto = smt + (size_t)(u & 1) * RUN_MT * K;
+        const int rc = ra.r1 + u - 1;                  // list index of sweep j0+u
+        for (long e = row; e < a.E; e += RUN_V) {
+            const int2 b = a.bd[e];
+            const int sn = b.x, tn = b.y;
+            const int eslot = a.eloc[e], rslot = ra.rloc[e];
+            const float c0 = a.c0[e], dpos = a.dpos[e];
+            // slots >= 2P: this sweep's candidates (lanes past the K slots read slot 2P's)
+            const int kc = kk >= 2 * P ? kk - 2 * P : 0;
+            const float4 psl = k < 2 * P ? sp[sn * 2 * P + k] : pa.cand[rc * pa.cslice + (long)sn * P + kc];
+            // lanes p < 2P: t's particle p after its selection, its belief less the message from s
+            const int pp = k < 2 * P ? k : 0;
+            const int tsrc = ss[tn * 2 * P + pp];
+            const float tm = sb[tn * 2 * P + pp] - mtp[((long)tn * a.maxdeg + rslot) * K + tsrc];
+            const float4 tp = sp[tn * 2 * P + pp];
+            const uint4 ps = pa.pt[(long)rc * a.V + sn], pt = pa.pt[(long)rc * a.V + tn];
+            float X[6], Y[6];
+            X[0] = (float)unpack_x(ps.x); Y[0] = (float)unpack_y(ps.x);
+            X[1] = (float)unpack_x(ps.y); Y[1] = (float)unpack_y(ps.y);
+            X[2] = (float)unpack_x(ps.z); Y[2] = (float)unpack_y(ps.z);
+            X[3] = (float)unpack_x(pt.x); Y[3] = (float)unpack_y(pt.x);
+            X[4] = (float)unpack_x(pt.y); Y[4] = (float)unpack_y(pt.y);
+            X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
+            float out = 0.f, minimum = FLT_MAX;
+            fused_msg(tm, tp.x, tp.y, tp.z, psl, X, Y, c0, dpos, a.tau_s, out, minimum,
+                      std::make_integer_sequence<int, 2 * P>{});
+            out += minimum;
+            float mean = row_sum(out, std::make_integer_sequence<int, K>{});
+            mean /= K;
+            if (own) mto[((long)sn * a.maxdeg + eslot) * K + k] = out - mean;
+        }
+        __syncthreads();
+    }
+}
+
 // the node update that ends a run of fused sweeps: the run's buffers in, the standard state out
 __global__ __launch_bounds__(256) void node_io_kernel(SweepArgs a, NodeIO io, int this_kind, int next_kind, float ndz,
                                                       float ndn) {
@@ -2314,15 +2448,19 @@ static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
     HBP_CHECK(hipStreamWaitEvent(s2, c.side_event(0), 0));
     HBP_CHECK(hipMemsetAsync(pa.acc, 0, sizeof(unsigned long long) * (size_t)nR * pa.aslice, s2));
     HBP_CHECK(hipMemsetAsync(pa.nan, 0, sizeof(unsigned int) * (size_t)nR * a.V, s2));
+    // the first launch costs listed sweeps [0, f), f = nb - 1 (1 for nb 1), each later one nb
+    // sweeps: with nb 4 a launch holds one iteration's RANSAC sweeps (the level's first sweep,
+    // costed by the level init, is not listed), so a run waits for one launch only
     const int nb = c.pipe_batch;
+    const int f = nb > 1 ? nb - 1 : 1;
     const int mode = !a.use_cost_vol ? kCostWta : (a.volf ? kCostF32 : (a.right ? kCostRight : kCostLeft));
     const int npc = B.nseg;
     // at least one block per XCD: each XCD's blocks cost that XCD's share of the segments
     const int blocks = std::max(8, std::min(c.pipe_blocks, div_up(npc, 4)));
     const bool hb2 = c.pipe_hb == 2;
     int bi = 0;
-    for (int r0 = 0; r0 < nR; r0 += nb, ++bi) {
-        const int nr = std::min(nb, nR - r0);
+    for (int r0 = 0; r0 < nR; r0 += (bi == 0 ? f : nb), ++bi) {
+        const int nr = std::min(bi == 0 ? f : nb, nR - r0);
         ProfScope ps(c, "plane_cost_ahead_L" + std::to_string(l), pc_ahead_bytes(a.N, a.V, nr), true);
 #define HBP_AHEAD(M)                                                                                             \
     do {                                                                                                         \
@@ -2339,11 +2477,12 @@ static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
     }
     b.pp_batches = bi;
     b.pp_batch = nb;
+    b.pp_first = f;
 }
 
 // the main stream waits for the launch that holds listed sweep r's sums
 static void pipe_wait(Ctx &c, BP &b, int r) {
-    const int bi = r / b.pp_batch;
+    const int bi = r < b.pp_first ? 0 : 1 + (r - b.pp_first) / b.pp_batch;
     if (bi <= b.pp_waited) return;
     HBP_CHECK(hipStreamWaitEvent(c.stream, c.side_event(1 + (size_t)bi), 0));
     b.pp_waited = bi;
@@ -2351,7 +2490,10 @@ static void pipe_wait(Ctx &c, BP &b, int r) {
 
 // the main stream waits for everything issued on the second stream for this object
 static void pipe_drain(Ctx &c, BP &b) {
-    if (b.pp_batches > 0 && b.pp_waited < b.pp_batches - 1) pipe_wait(c, b, (b.pp_batches - 1) * b.pp_batch);
+    if (b.pp_batches > 0 && b.pp_waited < b.pp_batches - 1) {
+        HBP_CHECK(hipStreamWaitEvent(c.stream, c.side_event((size_t)b.pp_batches), 0));
+        b.pp_waited = b.pp_batches - 1;
+    }
 }
 
 // level init, and the ahead work of fuse 2
@@ -2403,7 +2545,36 @@ static void run_chain(Ctx &c, BP &b, size_t j0, int n) {
     // (in the one launch both ends of an edge and each node may still read it); the slices of
     // the listed sweeps are cleared at the level start
     auto MT = [&](int u) { return u % 2 == 0 ? b.msg_t.p : b.ch_mt.p; };
+    // tiny levels (fuse 2): the launches after the first and the closing node update in one
+    // workgroup (run_kernel), when the run's sweeps after the first are all costed ahead and
+    // the sweep after the run is a spatial one or none
+    int next_kind_l = kNone;
+    {
+        const size_t jl = j0 + (size_t)uL;
+        if (!b.sched[jl].last && jl + 1 < b.sched.size()) next_kind_l = b.sched[jl + 1].kind;
+    }
+    bool tiny = pipe && a.V <= RUN_V && a.E <= c.run_max_e && B.maxdeg <= kNodeMaxDeg &&
+                (long)a.V * B.maxdeg <= RUN_MT && (next_kind_l == kNone || next_kind_l == kSpatial) &&
+                loc(0).r < 0;
+    for (int u = 1; tiny && u <= uL; ++u) tiny = loc(u).r >= 0 && loc(u).r == loc(1).r + u - 1;
     for (int u = 0; u <= uL; ++u) {
+        if (tiny && u == 1) {
+            const Loc c0l = loc(0);
+            pipe_wait(c, b, loc(uL).r);
+            RunArgs ra{};
+            ra.acc0 = c0l.acc;
+            ra.nan0 = c0l.nan;
+            ra.mt0 = MT(0);
+            ra.rloc = B.rloc.p;
+            ra.n = uL;
+            ra.r1 = loc(1).r;
+            ra.prev_spatial = sw0.kind == kSpatial ? 1 : 0;
+            ra.pa = pa;
+            ProfScope psr(c, "run_L" + std::to_string(l), uL * (edge_bytes(a.E, a.V) + node_bytes(a.E, a.V)), true);
+            HBP_LAUNCH(psr, run_kernel, dim3(1), dim3(1024), RUN_LDS_BYTES, c.stream, a, ra);
+            HBP_CHECK(hipGetLastError());
+            return;
+        }
         const Loc cu = loc(u);
         SweepArgs ax = a;
         ax.acc = cu.acc;

@@ -305,6 +305,9 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
     } else if (n == "pipe_blocks") {
         HBP_REQUIRE(value >= 8 && value <= 65536, "pipe_blocks must be in [8, 65536]");
         ctx->c.pipe_blocks = value;
+    } else if (n == "run_max_e") {
+        HBP_REQUIRE(value >= 0 && value <= 4096, "run_max_e must be in [0, 4096]");
+        ctx->c.run_max_e = value;
     } else if (n == "pipe_hb") {
         HBP_REQUIRE(value == 2 || value == 4, "pipe_hb must be 2 or 4");
         ctx->c.pipe_hb = value;
@@ -313,7 +316,7 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
         HBP_REQUIRE(value >= -1 && value <= 1, "a level form is -1 (by coarse_min_pix), 0 (chunks) or 1 (segments)");
         ctx->c.level_form[l] = value;
     } else {
-        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, level_form.<l>)"};
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, run_max_e, level_form.<l>)"};
     }
     HBP_API_END
 }
@@ -331,9 +334,10 @@ int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
     else if (n == "pipe_batch") *value = ctx->c.pipe_batch;
     else if (n == "pipe_blocks") *value = ctx->c.pipe_blocks;
     else if (n == "pipe_hb") *value = ctx->c.pipe_hb;
+    else if (n == "run_max_e") *value = ctx->c.run_max_e;
     else if (n == "seg_len") *value = ctx->c.seg_len;
     else if (n.rfind("level_form.", 0) == 0) *value = ctx->c.level_form[level_form_index(n)];
-    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, level_form.<l>)"};
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, run_max_e, level_form.<l>)"};
     HBP_API_END
 }
 
@@ -742,6 +746,7 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
         h.pipe_batch = m.pipe_batch;
         h.pipe_blocks = m.pipe_blocks;
         h.pipe_hb = m.pipe_hb;
+        h.run_max_e = m.run_max_e;
         std::memcpy(h.level_form, m.level_form, sizeof(h.level_form));
     }
     std::vector<PreparedPair> done(n);

@@ -101,7 +101,7 @@ struct BP {
     DevBuf<uint4> pp_pt;
     DevBuf<unsigned long long> pp_acc;
     DevBuf<unsigned int> pp_nan;
-    int pp_level = -1, pp_batches = 0, pp_batch = 4, pp_waited = -1;
+    int pp_level = -1, pp_batches = 0, pp_batch = 4, pp_first = 3, pp_waited = -1;
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;

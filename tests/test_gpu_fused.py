@@ -70,6 +70,22 @@ def test_fused_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_v
         assert same_bits(out[0][0], out[fuse][0]) and same_bits(out[0][1], out[fuse][1]), fuse
 
 
+@pytest.mark.parametrize("seed,W,H,D,ml,right_view,cost,kw", CASES)
+@pytest.mark.parametrize("max_e", [64, 4096])
+def test_one_workgroup_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_view, cost, kw, max_e):
+    """Levels of at most 64 superpixels (and run_max_e edges) run a fused run's sweeps after its
+    first in one workgroup; with run_max_e 4096 every such level does."""
+    say(f"one-workgroup runs seed={seed} {W}x{H} D={D} ml={ml} right={right_view} {cost} {kw} max_e={max_e}")
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
+    tuned.set_tuning("coarse_min_pix", 1)
+    tuned.set_tuning("fuse", 0)
+    d0, p0 = make_bp(tuned, left, right, D, ml, right_view, cost, **kw).run()
+    tuned.set_tuning("fuse", 2)
+    tuned.set_tuning("run_max_e", max_e)
+    d1, p1 = make_bp(tuned, left, right, D, ml, right_view, cost, **kw).run()
+    assert same_bits(d0, d1) and same_bits(p0, p1)
+
+
 @pytest.mark.parametrize("batch,blocks", [(1, 8), (3, 64), (7, 512), (16, 4096)])
 def test_ahead_launch_shapes_give_the_same_bits(tuned, batch, blocks):
     say(f"ahead batch={batch} blocks={blocks}")
@@ -84,7 +100,7 @@ def test_ahead_launch_shapes_give_the_same_bits(tuned, batch, blocks):
     assert same_bits(d0, d1) and same_bits(p0, p1)
 
 
-@pytest.mark.parametrize("fuse", [1, 2])
+@pytest.mark.parametrize("fuse", [1, 2, 3])
 @pytest.mark.parametrize("chunk", [2, 3, 5, 7, 13])
 def test_fused_state_after_every_step_call(tuned, chunk, fuse):
     """hbp_bp_step(n) ends a fused run where the call ends: the state read back after each call
@@ -92,6 +108,9 @@ def test_fused_state_after_every_step_call(tuned, chunk, fuse):
     say(f"fused {fuse} state, steps of {chunk}")
     left, right, _ = synth.make_pair(76, 288, 216, 40, (3, 8))
     tuned.set_tuning("coarse_min_pix", 1)
+    if fuse == 3:        # fuse 2 with one-workgroup runs on every level of at most 64 superpixels
+        tuned.set_tuning("run_max_e", 4096)
+        fuse = 2
     modes = (0, fuse)   # the form is the context's at each step call
     bps = [make_bp(tuned, left, right, 40, 2) for _ in modes]
 

@@ -120,7 +120,7 @@ def test_tuning_parameters_are_checked(tuned):
         tuned.set_tuning("level_form.24", 0)
     assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len",
                                       "batch_graph", "fuse", "pipe_batch", "pipe_blocks",
-                                      "pipe_hb"}
+                                      "pipe_hb", "run_max_e"}
 
 
 def test_tuner_sets_the_fastest_grid_point(tuned):
