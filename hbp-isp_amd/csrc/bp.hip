@@ -351,6 +351,9 @@ struct SweepArgs {
     int rrep;
     unsigned int *accnan;        // [V]
     uint32_t *s_ransac, *s_neighbor, *s_child;
+    // a spatial sweep of a pair whose level start computed its candidates' superpixels
+    // (spatial_ahead): [V][P], else null (the candidates are then resolved from the seeds)
+    const int *spc;
 };
 
 // fine levels: a 256-thread block costs CHUNK consecutive sorted pixels, 4 per lane
@@ -443,6 +446,21 @@ __device__ __forceinline__ float4 spatial_candidate(const SweepArgs &a, long id,
 }
 __device__ void spatial_node(const SweepArgs &a, long id) {
     for (int p = 0; p < P; ++p) a.plane[id * K + 2 * P + p] = spatial_candidate(a, id, p);
+}
+// The superpixel whose best plane is candidate p of node id in the m-th spatial sweep of the
+// level: spatial_candidate's draws, from the seeds that sweep reads.  The level init sets the
+// seeds of parent q to the level's off[q] (neighbour draws) and child_off[q] (child draws), and
+// only the node updates of spatial sweeps change them, by P draws each, so the m-th spatial
+// sweep reads them m P draws later.
+__device__ int spatial_ahead(const SweepArgs &a, int m, long id, int p) {
+    const LcgJump jm = lcg_jump_pow((unsigned long)m * P + p + 1);
+    const int Pp = a.clus[id];
+    const int nb_pos = a.off_h[Pp], nb_num = a.off_h[Pp + 1] - nb_pos;
+    const uint32_t sn = (jm.a * (uint32_t)a.off[Pp] + jm.c) % 2147483648u;
+    const int t = a.bd_h[(int)umod0(sn, nb_num) + nb_pos].y;
+    const uint32_t sc = (jm.a * (uint32_t)a.child_off[t] + jm.c) % 2147483648u;
+    const int cpos = a.child_off[t], ccnt = a.child_off[t + 1] - cpos;
+    return a.child[cpos + (int)umod0(sc, ccnt)];
 }
 
 // Plane-cost modes: the census cost volume of the left view, of the right view
@@ -738,7 +756,7 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
     for (int g0 = 0; g0 < NS; g0 += P) {
         if (spatial) {
             float4 cand = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane < P) cand = spatial_candidate(a, c, lane);
+            if (lane < P) cand = a.spc ? a.plane[(long)a.spc[(long)c * P + lane] * K] : spatial_candidate(a, c, lane);
             if (sg.w == 0 && lane < P) a.plane[(long)c * K + 2 * P + lane] = cand;
 #pragma unroll
             for (int k = 0; k < P; ++k)
@@ -934,7 +952,9 @@ __device__ void edge_wave(const SweepArgs &a, long wv, bool spatial) {
         X[5] = (float)unpack_x(pt.z); Y[5] = (float)unpack_y(pt.z);
         float c0 = a.c0[e];
         float dpos = a.dpos[e];
-        float4 psl = (spatial && y >= 2 * P) ? spatial_candidate(a, s, y - 2 * P) : a.plane[(long)s * K + y];
+        float4 psl = (spatial && y >= 2 * P)
+                         ? (a.spc ? a.plane[(long)a.spc[(long)s * P + y - 2 * P] * K] : spatial_candidate(a, s, y - 2 * P))
+                         : a.plane[(long)s * K + y];
         float4 m0 = a.msg[2 * e], m1 = a.msg[2 * e + 1];
         float mh[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
         float minimum = FLT_MAX;
@@ -1384,16 +1404,7 @@ __host__ __device__ inline long ransac_draw_steps(int r, int rd_first, int rd_co
     const int nd = i - rd_first < 0 ? 0 : (i - rd_first > rd_count ? rd_count : i - rd_first);
     return 3L * P * r + 4L * P * nd;
 }
-// n steps of the LCG as one affine map, by squaring (all such maps commute)
-__host__ __device__ inline LcgJump lcg_jump_pow(unsigned long n) {
-    LcgJump r{1u, 0u}, b{1103515245u, 12345u};
-    while (n) {
-        if (n & 1u) r = LcgJump{b.a * r.a, b.a * r.c + b.c};
-        b = LcgJump{b.a * b.a, b.a * b.c + b.c};
-        n >>= 1;
-    }
-    return r;
-}
+
 
 // candidates p of every RANSAC sweep r of the list for node n (node_group's draw)
 __device__ void draw_ahead(const SweepArgs &a, const PipeArgs &pa, long i) {
@@ -1564,6 +1575,15 @@ __global__ __launch_bounds__(256) void chain_kernel(SweepArgs a, ChainIO io, int
     } else {
         const long s = ((long)(b - npc_blocks - nedge_blocks) * 256 + threadIdx.x) >> 4;
         if (s < a.V) chain_node(a, io, s);
+    }
+}
+
+// the candidates' superpixels of every spatial sweep of the level (nS sweeps)
+__global__ __launch_bounds__(256) void spatial_ahead_kernel(SweepArgs a, int *spc, int nS) {
+    GRID_STRIDE(i, (long)nS * a.V * P) {
+        const int p = (int)(i % P);
+        const long rn = i / P;
+        spc[i] = spatial_ahead(a, (int)(rn / a.V), rn % a.V, p);
     }
 }
 
@@ -2191,6 +2211,10 @@ void bp_setup(Ctx &c, BP &b) {
     // draws before a RANSAC sweep, 4P before a random one, none before a spatial one, so every
     // map is a number of LCG steps; draw_ahead takes that number from ransac_draw_steps, checked
     // here against the walk over the schedule
+    b.sidx.assign(b.sched.size(), -1);
+    b.scount.assign(b.L, 0);
+    for (size_t j = 0; j < b.sched.size(); ++j)
+        if (b.sched[j].kind == kSpatial) b.sidx[j] = b.scount[b.sched[j].level]++;
     b.ridx.assign(b.sched.size(), -1);
     b.rcount.assign(b.L, 0);
     b.rd_first.assign(b.L, 0);
@@ -2306,6 +2330,7 @@ static void run_level_init(Ctx &c, BP &b, const SweepArgs &a, int l) {
 }
 
 static void level_start(Ctx &c, BP &b, const SweepArgs &a, int l);
+static const int *pipe_spc(const BP &b, const Sweep &sw, size_t j);
 
 // one sweep as separate kernels
 static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float ndz, float ndn) {
@@ -2313,6 +2338,7 @@ static void run_sweep_split(Ctx &c, BP &b, const Sweep &sw, int next_kind, float
     const BPLevel &B = b.lv[l];
     SweepArgs a = sweep_args(b, l);
     bool spatial = sw.kind == kSpatial;
+    if (spatial && B.coarse) a.spc = pipe_spc(b, sw, (size_t)(&sw - b.sched.data()));
     if (sw.first) {
         level_start(c, b, a, l);
         launch_cost_edge<P>(c, a, B, l, 2 * P, false, false, a.E > 0);
@@ -2413,7 +2439,7 @@ static PipeArgs pipe_args(const BP &b, int l) {
 // the slices of the largest coarse level's list, made on first use
 static void pipe_alloc(Ctx &c, BP &b) {
     if (b.pp_cand.p) return;
-    size_t nc = 1, na = 1, nv = 1;
+    size_t nc = 1, na = 1, nv = 1, ns = 1;
     for (int l = b.min_level; l < b.L; ++l) {
         const BPLevel &B = b.lv[l];
         if (!B.coarse) continue;
@@ -2421,7 +2447,9 @@ static void pipe_alloc(Ctx &c, BP &b) {
         nc = std::max(nc, nR * B.V * P);
         na = std::max(na, nR * B.V * acc_replicas(B.V) * P);
         nv = std::max(nv, nR * B.V);
+        ns = std::max(ns, (size_t)b.scount[l] * B.V * P);
     }
+    b.pp_spc.alloc(c.pool, ns);
     b.pp_cand.alloc(c.pool, nc);
     b.pp_pt.alloc(c.pool, nv);
     b.pp_acc.alloc(c.pool, na);
@@ -2437,8 +2465,16 @@ static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
     b.pp_level = l;
     b.pp_batches = 0;
     b.pp_waited = -1;
-    if (nR == 0) return;
     pipe_alloc(c, b);
+    // the spatial sweeps' candidate superpixels (their seeds advance only in spatial sweeps'
+    // node updates) need the level's parent structure
+    const int nS = b.scount[l];
+    if (nS > 0 && l + 1 < b.L) {
+        hipLaunchKernelGGL(spatial_ahead_kernel, dim3(grid_for(c, (long)nS * a.V * P)), dim3(256), 0, c.stream, a,
+                           b.pp_spc.p, nS);
+        HBP_CHECK(hipGetLastError());
+    }
+    if (nR == 0) return;
     const PipeArgs pa = pipe_args(b, l);
     hipLaunchKernelGGL(draw_ahead_kernel, dim3(grid_for(c, (long)nR * a.V * P)), dim3(256), 0, c.stream, a, pa, nR);
     HBP_CHECK(hipGetLastError());
@@ -2478,6 +2514,12 @@ static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
     b.pp_batches = bi;
     b.pp_batch = nb;
     b.pp_first = f;
+}
+
+// a spatial sweep's candidate superpixels when its level start computed them, else null
+static const int *pipe_spc(const BP &b, const Sweep &sw, size_t j) {
+    if (sw.kind != kSpatial || b.pp_level != sw.level || b.sidx[j] < 0 || !b.pp_spc.p) return nullptr;
+    return b.pp_spc.p + (size_t)b.sidx[j] * b.lv[sw.level].V * P;
 }
 
 // the main stream waits for the launch that holds listed sweep r's sums
@@ -2591,7 +2633,10 @@ static void run_chain(Ctx &c, BP &b, size_t j0, int n) {
         const bool costs = cu.r < 0 && !(u == 0 && sw0.first);
         if (u == 0) {
             // a spatial first sweep (fuse 2): its own launch, its node update done by launch 1
-            if (sw0.kind == kSpatial) launch_cost_edge<P>(c, ax, B, l, 2 * P, true, true, a.E > 0);
+            if (sw0.kind == kSpatial) {
+                ax.spc = pipe_spc(b, sw0, j0);
+                launch_cost_edge<P>(c, ax, B, l, 2 * P, true, true, a.E > 0);
+            }
             else launch_chain<true>(c, ax, io, B, l, costs);
             continue;
         }

@@ -182,6 +182,16 @@ __host__ __device__ inline uint32_t lcg_skip(uint32_t s, int j) {
     const uint32_t c = j == 1 ? j1.c : (j == 2 ? j2.c : j3.c);
     return j == 0 ? s : (a * s + c) % 2147483648u;
 }
+// n steps of lcg_next as one affine map, by squaring (all such maps commute)
+__host__ __device__ inline LcgJump lcg_jump_pow(unsigned long n) {
+    LcgJump r{1u, 0u}, b{1103515245u, 12345u};
+    while (n) {
+        if (n & 1u) r = LcgJump{b.a * r.a, b.a * r.c + b.c};
+        b = LcgJump{b.a * b.a, b.a * b.c + b.c};
+        n >>= 1;
+    }
+    return r;
+}
 __host__ __device__ inline float lcg_uniform(uint32_t &s, float f) {
     s = lcg_next(s);
     return (float)s / 2147483648.0f * f;

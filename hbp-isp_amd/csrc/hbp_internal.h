@@ -97,6 +97,10 @@ struct BP {
     // the first iteration and count of its random sweeps; the slices of the level pp_level
     // (bp.hip, PipeArgs), the launches issued for it and the last one the main stream waited for
     std::vector<int> ridx, rcount, rd_first, rd_count;
+    // per sweep its index among its level's spatial sweeps or -1, per level their count; the
+    // candidates' superpixels of every spatial sweep of the level pp_level [nS][V][P]
+    std::vector<int> sidx, scount;
+    DevBuf<int> pp_spc;
     DevBuf<float4> pp_cand;
     DevBuf<uint4> pp_pt;
     DevBuf<unsigned long long> pp_acc;
