@@ -111,13 +111,13 @@ int hbp_inject_fault(const char *what, long long after) {
 
 // a context's stream, pools and pinned words, after its queued work
 static void ctx_teardown(Ctx &c) {
-    HBP_CHECK(hipStreamSynchronize(c.stream));
+    if (c.stream) HBP_CHECK(hipStreamSynchronize(c.stream));
     c.free_side();
     c.scratch.release();
     c.counters.release();
     c.free_host_box();
     c.pool.release_all();
-    HBP_CHECK(hipStreamDestroy(c.stream));
+    if (c.stream) HBP_CHECK(hipStreamDestroy(c.stream));
     c.stream = nullptr;
 }
 
@@ -732,9 +732,26 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
         h->num_cu = m.num_cu;
         h->coop_blocks = m.coop_blocks;
         h->coop_forbidden = true;
-        HBP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         ctx->helpers.push_back(std::move(h));
     }
+    // The workers' streams live for this call only (their memory pools stay with the context,
+    // which owns what the workers build): a process's streams share the device's few hardware
+    // queues, and streams kept idle push later ones, such as a server's admission stream, onto
+    // the queue of a busy one.
+    struct StreamsOf {
+        hbp_ctx *ctx;
+        ~StreamsOf() {
+            for (auto &h : ctx->helpers)
+                if (h->stream) {
+                    (void)hipStreamSynchronize(h->stream);
+                    (void)hipStreamDestroy(h->stream);
+                    h->stream = nullptr;
+                }
+        }
+    } streams_of{ctx};
+    for (int w = 0; w < nw; ++w)
+        if (!ctx->helpers[w]->stream)
+            HBP_CHECK(hipStreamCreateWithFlags(&ctx->helpers[w]->stream, hipStreamNonBlocking));
     for (int w = 0; w < nw; ++w) {   // the context's kernel forms
         Ctx &h = *ctx->helpers[w];
         h.coarse_min_pix = m.coarse_min_pix;
