@@ -206,6 +206,11 @@ __device__ __forceinline__ void flag_or(int *flags, int slot, int local) {
 __device__ __forceinline__ int flag_read(const int *flags, int slot) {
     return __hip_atomic_load(&flags[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// sets a loop flag (it only ever goes from 0 to 1): many blocks end with this, so those that
+// find it set skip the atomic, whose updates of one word are serialised at the memory side
+__device__ __forceinline__ void flag_set(int *flags, int slot) {
+    if (!flag_read(flags, slot)) atomicOr(&flags[slot], 1);
+}
 __device__ __forceinline__ void flag_clear(int *flags, int slot) {
     __hip_atomic_store(&flags[slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -381,8 +386,8 @@ __global__ void crt_check_kernel(int V, CRTBuf s, CRTFlags *f, int r) {
     int loc = 0;
     CRT_NODES(i) if (s.mask[i] && s.indeg[i] > 1) loc = 1;
     if (__syncthreads_or(loc) && threadIdx.x == 0) {
-        atomicOr(&f->more[r], 1);
-        if (r == kCrtRounds) atomicOr(&f->overflow, 1);
+        flag_set(f->more, r);
+        if (r == kCrtRounds) flag_set(&f->overflow, 0);
     }
 }
 __global__ void crt_delink_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
@@ -396,8 +401,8 @@ __global__ void crt_jump_kernel(int V, CRTBuf s, CRTFlags *f, int r, int j) {
     int jl = 0;
     CRT_NODES(i) jl |= crt_jump(s, p, q, i);
     if (__syncthreads_or(jl) && threadIdx.x == 0) {
-        atomicOr(&f->jl[r][j], 1);
-        if (j == kCrtJumps - 1) atomicOr(&f->overflow, 1);
+        flag_set(f->jl[r], j);
+        if (j == kCrtJumps - 1) flag_set(&f->overflow, 0);
     }
 }
 __global__ void crt_relink_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
@@ -426,8 +431,8 @@ __global__ void crt_max_kernel(int V, CRTBuf s, CRTFlags *f, int j) {
     int jl = 0;
     CRT_NODES(i) jl |= crt_max_jump(p, q, mx, cl, i);
     if (__syncthreads_or(jl) && threadIdx.x == 0) {
-        atomicOr(&f->jm[j], 1);
-        if (j == kCrtMaxJumps - 1) atomicOr(&f->overflow, 1);
+        flag_set(f->jm, j);
+        if (j == kCrtMaxJumps - 1) flag_set(&f->overflow, 0);
     }
 }
 // the last step changes no label, so M and C hold the same labels.  After an overflow the
@@ -789,21 +794,23 @@ static void crt_enqueue(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
         HBP_CHECK(hipGetLastError());
         return;
     }
-    const dim3 g(grid_for(c, V)), b(256);
+    // the phases that end in a loop flag run on 2 blocks per CU (grid-stride over the nodes), so
+    // that few blocks set the flag
+    const dim3 g(grid_for(c, V)), gf(std::min(grid_for(c, V), 2 * c.num_cu)), b(256);
     CRTFlags *f = r.f;
     hipLaunchKernelGGL(crt_init_kernel, g, b, 0, c.stream, d_nn, V, s);
     for (int rd = 0; rd <= kCrtRounds; ++rd) {
         hipLaunchKernelGGL(crt_count_kernel, g, b, 0, c.stream, V, s, f, rd);
-        hipLaunchKernelGGL(crt_check_kernel, g, b, 0, c.stream, V, s, f, rd);
+        hipLaunchKernelGGL(crt_check_kernel, gf, b, 0, c.stream, V, s, f, rd);
         if (rd == kCrtRounds) break;
         hipLaunchKernelGGL(crt_delink_kernel, g, b, 0, c.stream, V, s, f, rd);
         for (int j = 0; j < kCrtJumps; ++j)
-            hipLaunchKernelGGL(crt_jump_kernel, g, b, 0, c.stream, V, s, f, rd, j);
+            hipLaunchKernelGGL(crt_jump_kernel, gf, b, 0, c.stream, V, s, f, rd, j);
         hipLaunchKernelGGL(crt_relink_kernel, g, b, 0, c.stream, V, s, f, rd);
         hipLaunchKernelGGL(crt_relink_all_kernel, g, b, 0, c.stream, V, s, f, rd);
     }
     hipLaunchKernelGGL(crt_max_init_kernel, g, b, 0, c.stream, V, s, f);
-    for (int j = 0; j < kCrtMaxJumps; ++j) hipLaunchKernelGGL(crt_max_kernel, g, b, 0, c.stream, V, s, f, j);
+    for (int j = 0; j < kCrtMaxJumps; ++j) hipLaunchKernelGGL(crt_max_kernel, gf, b, 0, c.stream, V, s, f, j);
     hipLaunchKernelGGL(crt_final_kernel, g, b, 0, c.stream, V, s, f, d_label);
     HBP_CHECK(hipGetLastError());
 }
