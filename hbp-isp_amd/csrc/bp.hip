@@ -2459,11 +2459,46 @@ static void pipe_alloc(Ctx &c, BP &b) {
 // Level start (after the level init, before any node update of the level): every listed
 // RANSAC sweep's candidates on the main stream, then on the second stream their sums cleared
 // and their plane costs in launches of pipe_batch sweeps, each followed by an event.
+// issues the ahead launches [b.pp_issued, upto) of the level b.pp_level on the second stream
+static void pipe_issue(Ctx &c, BP &b, int upto) {
+    upto = std::min(upto, b.pp_batches);
+    if (b.pp_issued >= upto) return;
+    const int l = b.pp_level;
+    const SweepArgs a = sweep_args(b, l);
+    const PipeArgs pa = pipe_args(b, l);
+    const int nR = b.rcount[l];
+    const int nb = b.pp_batch, f = b.pp_first;
+    const int mode = !a.use_cost_vol ? kCostWta : (a.volf ? kCostF32 : (a.right ? kCostRight : kCostLeft));
+    const int npc = b.lv[l].nseg;
+    // at least one block per XCD: each XCD's blocks cost that XCD's share of the segments
+    const int blocks = std::max(8, std::min(c.pipe_blocks, div_up(npc, 4)));
+    const bool hb2 = c.pipe_hb == 2;
+    hipStream_t s2 = c.side_stream();
+    for (int bi = b.pp_issued; bi < upto; ++bi) {
+        const int r0 = bi == 0 ? 0 : f + (bi - 1) * nb;
+        const int nr = std::min(bi == 0 ? f : nb, nR - r0);
+        ProfScope ps(c, "plane_cost_ahead_L" + std::to_string(l), pc_ahead_bytes(a.N, a.V, nr), true);
+#define HBP_AHEAD(M)                                                                                             \
+    do {                                                                                                         \
+        if (hb2) HBP_LAUNCH(ps, (pc_ahead_kernel<M, 2>), dim3(blocks), dim3(256), 0, s2, a, pa, r0, nr, npc);    \
+        else HBP_LAUNCH(ps, (pc_ahead_kernel<M, 4>), dim3(blocks), dim3(256), 0, s2, a, pa, r0, nr, npc);        \
+    } while (0)
+        if (mode == kCostLeft) HBP_AHEAD(kCostLeft);
+        else if (mode == kCostRight) HBP_AHEAD(kCostRight);
+        else if (mode == kCostF32) HBP_AHEAD(kCostF32);
+        else HBP_AHEAD(kCostWta);
+#undef HBP_AHEAD
+        HBP_CHECK(hipGetLastError());
+        HBP_CHECK(hipEventRecord(c.side_event(1 + (size_t)bi), s2));
+    }
+    b.pp_issued = upto;
+}
+
 static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
-    const BPLevel &B = b.lv[l];
     const int nR = b.rcount[l];
     b.pp_level = l;
     b.pp_batches = 0;
+    b.pp_issued = 0;
     b.pp_waited = -1;
     pipe_alloc(c, b);
     // the spatial sweeps' candidate superpixels (their seeds advance only in spatial sweeps'
@@ -2489,31 +2524,13 @@ static void pipe_level_start(Ctx &c, BP &b, const SweepArgs &a, int l) {
     // costed by the level init, is not listed), so a run waits for one launch only
     const int nb = c.pipe_batch;
     const int f = nb > 1 ? nb - 1 : 1;
-    const int mode = !a.use_cost_vol ? kCostWta : (a.volf ? kCostF32 : (a.right ? kCostRight : kCostLeft));
-    const int npc = B.nseg;
-    // at least one block per XCD: each XCD's blocks cost that XCD's share of the segments
-    const int blocks = std::max(8, std::min(c.pipe_blocks, div_up(npc, 4)));
-    const bool hb2 = c.pipe_hb == 2;
-    int bi = 0;
-    for (int r0 = 0; r0 < nR; r0 += (bi == 0 ? f : nb), ++bi) {
-        const int nr = std::min(bi == 0 ? f : nb, nR - r0);
-        ProfScope ps(c, "plane_cost_ahead_L" + std::to_string(l), pc_ahead_bytes(a.N, a.V, nr), true);
-#define HBP_AHEAD(M)                                                                                             \
-    do {                                                                                                         \
-        if (hb2) HBP_LAUNCH(ps, (pc_ahead_kernel<M, 2>), dim3(blocks), dim3(256), 0, s2, a, pa, r0, nr, npc);    \
-        else HBP_LAUNCH(ps, (pc_ahead_kernel<M, 4>), dim3(blocks), dim3(256), 0, s2, a, pa, r0, nr, npc);        \
-    } while (0)
-        if (mode == kCostLeft) HBP_AHEAD(kCostLeft);
-        else if (mode == kCostRight) HBP_AHEAD(kCostRight);
-        else if (mode == kCostF32) HBP_AHEAD(kCostF32);
-        else HBP_AHEAD(kCostWta);
-#undef HBP_AHEAD
-        HBP_CHECK(hipGetLastError());
-        HBP_CHECK(hipEventRecord(c.side_event(1 + (size_t)bi), s2));
-    }
-    b.pp_batches = bi;
     b.pp_batch = nb;
     b.pp_first = f;
+    b.pp_batches = nR <= f ? 1 : 1 + div_up(nR - f, nb);
+    // with a window w only the first w + 1 launches are issued here, the rest as the main
+    // stream's waits reach them (pipe_wait), so that the main stream's next launches are not
+    // queued behind a burst of host-side launches
+    pipe_issue(c, b, c.pipe_window > 0 ? c.pipe_window + 1 : b.pp_batches);
 }
 
 // a spatial sweep's candidate superpixels when its level start computed them, else null
@@ -2526,12 +2543,14 @@ static const int *pipe_spc(const BP &b, const Sweep &sw, size_t j) {
 static void pipe_wait(Ctx &c, BP &b, int r) {
     const int bi = r < b.pp_first ? 0 : 1 + (r - b.pp_first) / b.pp_batch;
     if (bi <= b.pp_waited) return;
+    pipe_issue(c, b, c.pipe_window > 0 ? bi + 1 + c.pipe_window : b.pp_batches);
     HBP_CHECK(hipStreamWaitEvent(c.stream, c.side_event(1 + (size_t)bi), 0));
     b.pp_waited = bi;
 }
 
 // the main stream waits for everything issued on the second stream for this object
 static void pipe_drain(Ctx &c, BP &b) {
+    pipe_issue(c, b, b.pp_batches);
     if (b.pp_batches > 0 && b.pp_waited < b.pp_batches - 1) {
         HBP_CHECK(hipStreamWaitEvent(c.stream, c.side_event((size_t)b.pp_batches), 0));
         b.pp_waited = b.pp_batches - 1;

@@ -376,6 +376,9 @@ struct Ctx {
     // persistent launch (fewer leave room on every CU for the chain's launches) and the steps
     // per batch of its segment wavefronts (2 or 4)
     int pipe_batch = 4, pipe_blocks = 512, pipe_hb = 2;
+    // fuse 2: plane-cost launches issued ahead of the one the main stream waits for (0: every
+    // launch of a level at its start)
+    int pipe_window = 4;
     // fuse 2: BP levels with at most this many directed edges (and at most 64 superpixels) run a
     // fused run's sweeps after its first in one workgroup (0: never)
     int run_max_e = 64;

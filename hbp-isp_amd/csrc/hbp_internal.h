@@ -95,7 +95,7 @@ struct BP {
     // RANSAC plane costs ahead (fuse 2): per sweep of sched its index in its level's list of
     // RANSAC sweeps (the level's first sweep excluded) or -1; per level the list's length and
     // the first iteration and count of its random sweeps; the slices of the level pp_level
-    // (bp.hip, PipeArgs), the launches issued for it and the last one the main stream waited for
+    // (bp.hip, PipeArgs), its launches, those issued so far and the last one the main stream waited for
     std::vector<int> ridx, rcount, rd_first, rd_count;
     // per sweep its index among its level's spatial sweeps or -1, per level their count; the
     // candidates' superpixels of every spatial sweep of the level pp_level [nS][V][P]
@@ -105,7 +105,7 @@ struct BP {
     DevBuf<uint4> pp_pt;
     DevBuf<unsigned long long> pp_acc;
     DevBuf<unsigned int> pp_nan;
-    int pp_level = -1, pp_batches = 0, pp_batch = 4, pp_first = 3, pp_waited = -1;
+    int pp_level = -1, pp_batches = 0, pp_batch = 4, pp_first = 3, pp_waited = -1, pp_issued = 0;
     DevBuf<uint32_t> s_ransac, s_neighbor, s_child;
     // schedule state (hbp_g, hbp_isp.cu:1283-1311)
     int cur = -1;

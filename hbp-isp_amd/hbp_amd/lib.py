@@ -239,7 +239,7 @@ class Ctx:
 
     # ---- kernel-form tuning (results are the same for every value)
     TUNING = ("coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len", "batch_graph", "fuse",
-              "pipe_batch", "pipe_blocks", "pipe_hb", "run_max_e")
+              "pipe_batch", "pipe_blocks", "pipe_hb", "pipe_window", "run_max_e")
 
     def set_tuning(self, name, value):
         _check(self.lib.hbp_ctx_set_tuning(self.h, name.encode(), int(value)))

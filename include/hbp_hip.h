@@ -76,6 +76,8 @@ int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long 
  *      level's start on a second stream of the context; "pipe_batch" (1..16, default 4),
  *      "pipe_blocks" (8..65536, default 512), "pipe_hb" (2 or 4, default 2): RANSAC sweeps per
  *      such launch, its persistent blocks and the steps per batch of its segment wavefronts;
+ *      "pipe_window" (0..1024, default 4): such launches issued ahead of the one the first
+ *      stream waits for (0: all of a level's at its start);
  *      "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs
  *      planes by coarse_min_pix's rule, in chunks of sorted pixels, or in per-superpixel
  *      segments.  Applies to BP objects created afterwards (and to the batched

@@ -86,9 +86,10 @@ def test_one_workgroup_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml,
     assert same_bits(d0, d1) and same_bits(p0, p1)
 
 
-@pytest.mark.parametrize("batch,blocks", [(1, 8), (3, 64), (7, 512), (16, 4096)])
-def test_ahead_launch_shapes_give_the_same_bits(tuned, batch, blocks):
-    say(f"ahead batch={batch} blocks={blocks}")
+@pytest.mark.parametrize("batch,blocks,window", [(1, 8, 0), (3, 64, 1), (7, 512, 0), (16, 4096, 2), (4, 512, 1),
+                                                 (1, 512, 3)])
+def test_ahead_launch_shapes_give_the_same_bits(tuned, batch, blocks, window):
+    say(f"ahead batch={batch} blocks={blocks} window={window}")
     left, right, _ = synth.make_pair(79, 320, 240, 48, (3, 8))
     tuned.set_tuning("coarse_min_pix", 1)
     tuned.set_tuning("fuse", 0)
@@ -96,11 +97,12 @@ def test_ahead_launch_shapes_give_the_same_bits(tuned, batch, blocks):
     tuned.set_tuning("fuse", 2)
     tuned.set_tuning("pipe_batch", batch)
     tuned.set_tuning("pipe_blocks", blocks)
+    tuned.set_tuning("pipe_window", window)
     d1, p1 = make_bp(tuned, left, right, 48, 1).run()
     assert same_bits(d0, d1) and same_bits(p0, p1)
 
 
-@pytest.mark.parametrize("fuse", [1, 2, 3])
+@pytest.mark.parametrize("fuse", [1, 2, 3, 4])
 @pytest.mark.parametrize("chunk", [2, 3, 5, 7, 13])
 def test_fused_state_after_every_step_call(tuned, chunk, fuse):
     """hbp_bp_step(n) ends a fused run where the call ends: the state read back after each call
@@ -110,6 +112,9 @@ def test_fused_state_after_every_step_call(tuned, chunk, fuse):
     tuned.set_tuning("coarse_min_pix", 1)
     if fuse == 3:        # fuse 2 with one-workgroup runs on every level of at most 64 superpixels
         tuned.set_tuning("run_max_e", 4096)
+        fuse = 2
+    elif fuse == 4:      # fuse 2 with the plane-cost launches issued one ahead of the waits
+        tuned.set_tuning("pipe_window", 1)
         fuse = 2
     modes = (0, fuse)   # the form is the context's at each step call
     bps = [make_bp(tuned, left, right, 40, 2) for _ in modes]

@@ -114,13 +114,15 @@ def test_tuning_parameters_are_checked(tuned):
         tuned.set_tuning("pipe_batch", 17)
     with pytest.raises(H.HbpError, match="pipe_blocks"):
         tuned.set_tuning("pipe_blocks", 4)
+    with pytest.raises(H.HbpError, match="pipe_window"):
+        tuned.set_tuning("pipe_window", -1)
     with pytest.raises(H.HbpError, match="level form"):
         tuned.set_tuning("level_form.3", 2)
     with pytest.raises(H.HbpError, match="out of range"):
         tuned.set_tuning("level_form.24", 0)
     assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len",
                                       "batch_graph", "fuse", "pipe_batch", "pipe_blocks",
-                                      "pipe_hb", "run_max_e"}
+                                      "pipe_hb", "pipe_window", "run_max_e"}
 
 
 def test_tuner_sets_the_fastest_grid_point(tuned):
