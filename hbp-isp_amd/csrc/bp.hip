@@ -701,25 +701,38 @@ __device__ __forceinline__ void pc_batch(const SweepArgs &a, const float4 (&pl)[
     }
 }
 
-// exact sum over the wavefront of integer valued doubles below 2^53 (any order is exact):
-// xor 1 and xor 2 within quads, rotations by 4 and 8 within rows of 16 (every lane then holds
-// its row's sum), then the four rows
+// Sums over the wavefront of integer valued doubles below 2^53 are exact in any order.
+// dpp_f64: a DPP exchange of a double (both halves with the same pattern)
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
     return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true),
                             __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true));
 }
-__device__ __forceinline__ double wave_sum_exact(double v) {
-    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
-    v += dpp_f64<0x124>(v);   // row_ror 4
-    v += dpp_f64<0x128>(v);   // row_ror 8
-    double r = 0.0;
+// the P = 4 sums of a group of planes at once: lane l returns the sum over the wavefront of
+// v[l mod 4].  Within a quad, lanes l and l^1 split the pairs {v0, v1} and {v2, v3}: each keeps
+// the value its bit 0 names and adds the one its partner sends (a select, then one exchange per
+// pair); lanes l and l^2 then split the two pair sums the same way, so that every lane holds its
+// quad's sum of v[l mod 4].  Rotations by 4 and 8 within rows of 16 keep l mod 4, and two
+// exchanges across rows finish.  A value only ever meets values of its own index.
+static_assert(P == 4, "four sums per group");
+__device__ __forceinline__ double wave_sum4_exact(const double (&v)[4]) {
+    const int lane = threadIdx.x & 63;
+    const bool b0 = (lane & 1) != 0, b1 = (lane & 2) != 0;
+    const double x0 = (b0 ? v[1] : v[0]) + dpp_f64<0xB1>(b0 ? v[0] : v[1]);   // quad_perm [1,0,3,2]
+    const double x1 = (b0 ? v[3] : v[2]) + dpp_f64<0xB1>(b0 ? v[2] : v[3]);
+    double y = (b1 ? x1 : x0) + dpp_f64<0x4E>(b1 ? x0 : x1);                  // quad_perm [2,3,0,1]
+    y += dpp_f64<0x124>(y);   // row_ror 4
+    y += dpp_f64<0x128>(y);   // row_ror 8
+    y += __shfl_xor(y, 16);
+    y += __shfl_xor(y, 32);
+    return y;
+}
+// OR over the wavefront, skipped (one ballot) when every lane holds 0
+__device__ __forceinline__ unsigned int wave_or_rare(unsigned int m) {
+    if (__builtin_amdgcn_ballot_w64(m != 0u) == 0ull) return 0u;
 #pragma unroll
-    for (int row = 0; row < 4; ++row)
-        r += __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * row),
-                              __builtin_amdgcn_readlane(__double2loint(v), 16 * row));
-    return r;
+    for (int o = 32; o >= 1; o >>= 1) m |= __shfl_xor(m, o);
+    return m;
 }
 
 // HB: steps per batch.  4 keeps 16 census gathers in flight per wavefront (a single pair, whose
@@ -792,16 +805,12 @@ __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
                 if (rem == 3) pc_batch<MODE, 1, true>(a, pl, sl, first, end, r0 + 2, q + r0 + 2, acc, nanm);
             }
         }
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            // every cost is >= 0, so the sum takes lambda's sign
-            // the terms carry the costs' signs and |lambda|; the sum takes lambda's sign
-            const long long m = (long long)wave_sum_exact(acc[k]);
-            if (lane == g0 + k) mine = (unsigned long long)(a.lambda < 0.f ? -m : m);
-        }
+        // lane g0 + k takes slot slot0 + g0 + k (k = lane mod P); the terms carry the costs'
+        // signs and |lambda|, the sum takes lambda's sign
+        const long long m = (long long)wave_sum4_exact(acc);
+        if ((lane & ~(P - 1)) == g0) mine = (unsigned long long)(a.lambda < 0.f ? -m : m);
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) nanm |= __shfl_xor(nanm, o);
+    nanm = wave_or_rare(nanm);
     const int rep = w % a.rrep;
     if (lane < NS && mine) atomicAdd(&a.acc[((long)c * a.rrep + rep) * K + slot0 + lane], mine);
     if (lane == 0 && nanm) atomicOr(&a.accnan[c], nanm);
@@ -1481,13 +1490,9 @@ __device__ void pc_segment_multi(const SweepArgs &a, const PipeArgs &pa, int w, 
                 if (rem == 3) pc_batch<MODE, 1, true>(a, pl, 2 * P, first, end, rb + 2, q + rb + 2, acc, nanm);
             }
         }
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const long long m = (long long)wave_sum_exact(acc[k]);
-            if (lane == g * P + k) mine = (unsigned long long)(a.lambda < 0.f ? -m : m);
-        }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) nanm |= __shfl_xor(nanm, o);
+        const long long m = (long long)wave_sum4_exact(acc);   // lane l: slot l mod P
+        if ((lane & ~(P - 1)) == g * P) mine = (unsigned long long)(a.lambda < 0.f ? -m : m);
+        nanm = wave_or_rare(nanm);
         if (lane == g) mynan = nanm;
     }
     const int rep = w % a.rrep;
