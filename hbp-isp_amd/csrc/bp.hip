@@ -1583,6 +1583,20 @@ __global__ __launch_bounds__(256) void chain_kernel(SweepArgs a, ChainIO io, int
     }
 }
 
+// a fused launch that costs no planes (its RANSAC sweep costed ahead): edge blocks, then node
+// blocks of 16 superpixels; without the plane-cost code of chain_kernel
+__global__ __launch_bounds__(256) void chain_msg_kernel(SweepArgs a, ChainIO io, int nedge_blocks) {
+    chain_priority();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = blockIdx.x;
+    if (b < nedge_blocks) {
+        edge_wave_fused(a, io, (long)b * 4 + wave);
+    } else {
+        const long s = ((long)(b - nedge_blocks) * 256 + threadIdx.x) >> 4;
+        if (s < a.V) chain_node(a, io, s);
+    }
+}
+
 // the candidates' superpixels of every spatial sweep of the level (nS sweeps)
 __global__ __launch_bounds__(256) void spatial_ahead_kernel(SweepArgs a, int *spc, int nS) {
     GRID_STRIDE(i, (long)nS * a.V * P) {
@@ -2392,6 +2406,10 @@ static void launch_chain(Ctx &c, const SweepArgs &a, const ChainIO &io, const BP
     const double bytes = (costs ? plane_cost_bytes(a.N, a.V, P) : 0.0) + edge_bytes(a.E, a.V) +
                          (FIRST ? 0.0 : node_bytes(a.E, a.V));
     ProfScope ps(c, (costs ? "plane_cost_L" : "edge_L") + std::to_string(l), bytes, true);
+    if (!FIRST && npc == 0) {
+        HBP_LAUNCH(ps, chain_msg_kernel, dim3(nedge_blocks + node_blocks), dim3(256), 0, c.stream, a, io, nedge_blocks);
+        return;
+    }
     if (c.single_hb == 2) launch_chain_hb<FIRST, 2>(c, ps, a, io, mode, blocks, npc, nedge_blocks);
     else launch_chain_hb<FIRST, 4>(c, ps, a, io, mode, blocks, npc, nedge_blocks);
 }
