@@ -735,9 +735,9 @@ __device__ __forceinline__ unsigned int wave_or_rare(unsigned int m) {
     return m;
 }
 
-// HB: steps per batch.  4 keeps 16 census gathers in flight per wavefront (a single pair, whose
-// coarse launches have fewer wavefronts than the chip holds); 2 halves the registers the
-// gathers hold, for more resident wavefronts when many pairs share a launch.
+// HB: steps per batch.  4 keeps 16 census gathers in flight per wavefront; 2 halves the registers
+// the gathers hold, for more resident wavefronts (beside the plane costs computed ahead on the
+// second stream, or when many pairs share a launch).
 template <int NS, int MODE, int HB = 4>
 __device__ void pc_segment(const SweepArgs &a, int w, int slot0, bool spatial) {
     static_assert(NS % P == 0, "planes are costed in groups of P");

@@ -355,9 +355,11 @@ struct Ctx {
     // are exact in any order): the average pixels per superpixel from which a BP level takes
     // the segment form of the plane cost (below it a segment wavefront holds one small
     // superpixel; measured crossover 64-128), and the steps per batch of its wavefronts in
-    // single-pair launches (4: more gathers in flight for a latency-bound launch) and in
-    // batched ones (2: fewer registers, more resident wavefronts for a throughput-bound one)
-    int coarse_min_pix = 96, single_hb = 4, batch_hb = 2;
+    // single-pair launches and in batched ones (2: fewer registers, so that with the plane
+    // costs computed ahead on the second stream two or three of a spatial sweep's wavefronts
+    // fit beside the ahead kernel's on a SIMD, and more resident wavefronts when many pairs
+    // share a launch; 4: more gathers in flight per wavefront)
+    int coarse_min_pix = 96, single_hb = 2, batch_hb = 2;
     // columns per lane of the Hamming cost kernel (4 or 8; the wider form writes 512 instead of
     // 256 contiguous bytes per wavefront store)
     int hamming_cols = 4;

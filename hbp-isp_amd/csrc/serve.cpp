@@ -37,7 +37,7 @@ namespace hbp {
 struct Request {
     long long id = 0;
     int W = 0, H = 0, D = 0, min_level = 2;
-    int coarse_min_pix = 96, single_hb = 4, batch_hb = 2, hamming_cols = 4, seg_len = 512;   // the serving context's forms
+    int coarse_min_pix = 96, single_hb = 2, batch_hb = 2, hamming_cols = 4, seg_len = 512;   // the serving context's forms
     int level_form[Ctx::kFormLevels] = {};
     std::vector<uint8_t> left, right;
     size_t est = 0;           // device bytes held while on the device (without the u8 volume)

@@ -64,7 +64,7 @@ int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long 
  *      form each).  Every value gives the same results bit for bit; only the speed changes.
  *      "coarse_min_pix" (>= 1, default 96): average pixels per superpixel from which a BP level
  *      costs planes in the per-superpixel segment form rather than in chunks; "single_hb",
- *      "batch_hb" (2 or 4; defaults 4, 2): steps per batch of the segment wavefronts in one-pair
+ *      "batch_hb" (2 or 4; defaults 2, 2): steps per batch of the segment wavefronts in one-pair
  *      and in batched launches; "hamming_cols" (4 or 8, default 4): columns per lane of the
  *      Hamming cost kernel; "seg_len" (a multiple of 64 in [64, 512], default 512): pixels per
  *      segment wavefront of the segment form; "batch_graph" (0 or 1, default 0): hbp_bp_run_batch queues its
