@@ -33,7 +33,7 @@ AHEAD_EVENT_EVERY = 7
 # instruction per SIMD every 4 cycles at 2.4 GHz
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 # profile names of the kernels whose counter summaries profiles/ holds, and their rocprof names
-PROFILE_KERNELS = {"plane_cost": "cost_edge_kernel<4, true, 0, 4>", "plane_cost_ahead": "pc_ahead_kernel<0, 2>"}
+PROFILE_KERNELS = {"plane_cost": "cost_edge_kernel<4, true, 0, 2>", "plane_cost_ahead": "pc_ahead_kernel<0, 2>"}
 # host threads (each with a stream of its own) that build the batched leg's pairs
 PREP_WORKERS = 8
 METRIC = "stereo pairs/s + Mpix·disp/s; p50/p99 latency, batched/served vs plain"
