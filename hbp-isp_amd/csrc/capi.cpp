@@ -314,12 +314,21 @@ int hbp_ctx_set_tuning(hbp_ctx *ctx, const char *name, int value) {
     } else if (n == "pipe_window") {
         HBP_REQUIRE(value >= 0 && value <= 1024, "pipe_window must be in [0, 1024]");
         ctx->c.pipe_window = value;
+    } else if (n == "crt_rounds") {
+        HBP_REQUIRE(value >= 1 && value <= CrtSched{}.rounds, "crt_rounds must be in [1, 5]");
+        ctx->c.crt.rounds = value;
+    } else if (n == "crt_jumps") {
+        HBP_REQUIRE(value >= 1 && value <= CrtSched{}.jumps, "crt_jumps must be in [1, 8]");
+        ctx->c.crt.jumps = value;
+    } else if (n == "crt_max_jumps") {
+        HBP_REQUIRE(value >= 1 && value <= CrtSched{}.max_jumps, "crt_max_jumps must be in [1, 8]");
+        ctx->c.crt.max_jumps = value;
     } else if (n.rfind("level_form.", 0) == 0) {
         const int l = level_form_index(n);
         HBP_REQUIRE(value >= -1 && value <= 1, "a level form is -1 (by coarse_min_pix), 0 (chunks) or 1 (segments)");
         ctx->c.level_form[l] = value;
     } else {
-        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, pipe_window, run_max_e, level_form.<l>)"};
+        throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, pipe_window, run_max_e, crt_rounds, crt_jumps, crt_max_jumps, level_form.<l>)"};
     }
     HBP_API_END
 }
@@ -338,10 +347,13 @@ int hbp_ctx_get_tuning(hbp_ctx *ctx, const char *name, int *value) {
     else if (n == "pipe_blocks") *value = ctx->c.pipe_blocks;
     else if (n == "pipe_hb") *value = ctx->c.pipe_hb;
     else if (n == "pipe_window") *value = ctx->c.pipe_window;
+    else if (n == "crt_rounds") *value = ctx->c.crt.rounds;
+    else if (n == "crt_jumps") *value = ctx->c.crt.jumps;
+    else if (n == "crt_max_jumps") *value = ctx->c.crt.max_jumps;
     else if (n == "run_max_e") *value = ctx->c.run_max_e;
     else if (n == "seg_len") *value = ctx->c.seg_len;
     else if (n.rfind("level_form.", 0) == 0) *value = ctx->c.level_form[level_form_index(n)];
-    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, pipe_window, run_max_e, level_form.<l>)"};
+    else throw HipError{"unknown tuning parameter " + n + " (coarse_min_pix, single_hb, batch_hb, hamming_cols, seg_len, batch_graph, fuse, pipe_batch, pipe_blocks, pipe_hb, pipe_window, run_max_e, crt_rounds, crt_jumps, crt_max_jumps, level_form.<l>)"};
     HBP_API_END
 }
 
@@ -768,6 +780,7 @@ int hbp_pairs_prepare_dev(hbp_ctx *ctx, int n, const hbp_pair_dev *pairs, float 
         h.pipe_blocks = m.pipe_blocks;
         h.pipe_hb = m.pipe_hb;
         h.pipe_window = m.pipe_window;
+        h.crt = m.crt;
         h.run_max_e = m.run_max_e;
         std::memcpy(h.level_form, m.level_form, sizeof(h.level_form));
     }

@@ -344,6 +344,12 @@ struct Profiler {
     }
 };
 
+// a CRTrees schedule of the scheduled form: rounds, pointer-jumping steps per round,
+// max-jumping steps (the defaults are the full bounds)
+struct CrtSched {
+    int rounds = 5, jumps = 8, max_jumps = 8;
+};
+
 struct Ctx {
     int device = 0;
     Profiler prof;
@@ -381,6 +387,8 @@ struct Ctx {
     // fuse 2: plane-cost launches issued ahead of the one the main stream waits for (0: every
     // launch of a level at its start)
     int pipe_window = 4;
+    // the CRTrees schedule of the next scheduled pyramid level (learned from the levels before)
+    CrtSched crt;
     // fuse 2: BP levels with at most this many directed edges (and at most 64 superpixels) run a
     // fused run's sweeps after its first in one workgroup (0: never)
     int run_max_e = 64;

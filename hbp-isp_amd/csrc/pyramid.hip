@@ -172,11 +172,14 @@ __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int
 // kernel (the oracle checks both forms give the same labels).  Three executions of it:
 //  * levels up to kCrtSingleMax nodes: one 1024-thread block, phases separated by
 //    __syncthreads;
-//  * larger levels: one kernel per phase, launched in a fixed schedule of kCrtRounds rounds of
-//    kCrtJumps pointer-jumping steps and kCrtMaxJumps max-jumping steps.  The loop flags live
-//    in device memory; a kernel whose loop has already ended returns at once, and the
-//    ping-pong buffer of jumping step j is fixed by the parity of j, so the host never waits
-//    inside the loop.  A loop that needs more steps than scheduled sets the overflow word;
+//  * larger levels: one kernel per phase, launched in a schedule of at most kCrtRounds rounds
+//    of at most kCrtJumps pointer-jumping steps and at most kCrtMaxJumps max-jumping steps.
+//    The loop flags live in device memory; a kernel whose loop has already ended returns at
+//    once (a launch all the same), and the ping-pong buffer of jumping step j is fixed by the
+//    parity of j, so the host never waits inside the loop.  A loop that needs more steps than
+//    scheduled sets the overflow word.  The schedule of a level is the steps the context's
+//    previous levels used plus one of each (CrtSched), so that few dead launches are queued; a
+//    level that overflows it runs again with the full schedule;
 //  * then (rare) one cooperative kernel with grid barriers recomputes the level with the full
 //    bounds (64 rounds, 64 jumps).
 // ----------------------------------------------------------------------------------------
@@ -185,6 +188,8 @@ struct CRTBuf {
 };
 constexpr int kCrtSingleMax = 16000;
 constexpr int kCrtRounds = 5, kCrtJumps = 8, kCrtMaxJumps = 8;
+static_assert(CrtSched{}.rounds == kCrtRounds && CrtSched{}.jumps == kCrtJumps && CrtSched{}.max_jumps == kCrtMaxJumps,
+              "the default schedule is the full one");
 struct CRTFlags {               // zeroed before a scheduled run
     int more[kCrtRounds + 1];   // round r continues (some node has indegree > 1)
     int jl[kCrtRounds][kCrtJumps];
@@ -381,20 +386,21 @@ __global__ void crt_count_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
     if (!crt_round_open(f, r)) return;
     CRT_NODES(i) if (s.mask[i]) atomicAdd(&s.indeg[s.A[i]], 1);
 }
-__global__ void crt_check_kernel(int V, CRTBuf s, CRTFlags *f, int r) {
+// rounds: the schedule's rounds (the check after the last one overflows if the loop goes on)
+__global__ void crt_check_kernel(int V, CRTBuf s, CRTFlags *f, int r, int rounds) {
     if (!crt_round_open(f, r)) return;
     int loc = 0;
     CRT_NODES(i) if (s.mask[i] && s.indeg[i] > 1) loc = 1;
     if (__syncthreads_or(loc) && threadIdx.x == 0) {
         flag_set(f->more, r);
-        if (r == kCrtRounds) flag_set(&f->overflow, 0);
+        if (r == rounds) flag_set(&f->overflow, 0);
     }
 }
 __global__ void crt_delink_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
     if (!flag_read(f->more, r)) return;
     CRT_NODES(i) crt_delink(s, i);
 }
-__global__ void crt_jump_kernel(int V, CRTBuf s, CRTFlags *f, int r, int j) {
+__global__ void crt_jump_kernel(int V, CRTBuf s, CRTFlags *f, int r, int j, int jumps) {
     if (!flag_read(f->more, r) || (j > 0 && !flag_read(f->jl[r], j - 1))) return;
     const int *p = (j & 1) ? s.B : s.A;
     int *q = (j & 1) ? s.A : s.B;
@@ -402,7 +408,7 @@ __global__ void crt_jump_kernel(int V, CRTBuf s, CRTFlags *f, int r, int j) {
     CRT_NODES(i) jl |= crt_jump(s, p, q, i);
     if (__syncthreads_or(jl) && threadIdx.x == 0) {
         flag_set(f->jl[r], j);
-        if (j == kCrtJumps - 1) flag_set(&f->overflow, 0);
+        if (j == jumps - 1) flag_set(&f->overflow, 0);
     }
 }
 __global__ void crt_relink_kernel(int V, CRTBuf s, const CRTFlags *f, int r) {
@@ -422,7 +428,7 @@ __global__ void crt_max_init_kernel(int V, CRTBuf s, const CRTFlags *f) {
     if (flag_read(&f->overflow, 0)) return;
     CRT_NODES(i) s.M[i] = s.A[i];
 }
-__global__ void crt_max_kernel(int V, CRTBuf s, CRTFlags *f, int j) {
+__global__ void crt_max_kernel(int V, CRTBuf s, CRTFlags *f, int j, int max_jumps) {
     if (flag_read(&f->overflow, 0) || (j > 0 && !flag_read(f->jm, j - 1))) return;
     const int *p = (j & 1) ? s.B : s.A;
     int *q = (j & 1) ? s.A : s.B;
@@ -432,15 +438,34 @@ __global__ void crt_max_kernel(int V, CRTBuf s, CRTFlags *f, int j) {
     CRT_NODES(i) jl |= crt_max_jump(p, q, mx, cl, i);
     if (__syncthreads_or(jl) && threadIdx.x == 0) {
         flag_set(f->jm, j);
-        if (j == kCrtMaxJumps - 1) flag_set(&f->overflow, 0);
+        if (j == max_jumps - 1) flag_set(&f->overflow, 0);
     }
 }
 // the last step changes no label, so M and C hold the same labels.  After an overflow the
 // labels are the identity: in range for the compaction that is queued behind this kernel before
-// the host sees the overflow word (the cooperative rerun then overwrites them)
-__global__ void crt_final_kernel(int V, const CRTBuf s, const CRTFlags *f, int *__restrict__ label) {
+// the host sees the overflow word (a rerun then overwrites them).  Thread 0 also leaves in
+// status[1..3] the steps the run used: rounds that went on, the most jumping steps of a round
+// and the max-jumping steps (a step counts when it ran; the last of a loop finds nothing more)
+__global__ void crt_final_kernel(int V, const CRTBuf s, const CRTFlags *f, int *__restrict__ label, int rounds,
+                                 int jumps, int max_jumps) {
     const bool ov = flag_read(&f->overflow, 0) != 0;
     CRT_NODES(i) label[i] = ov ? (int)i : s.M[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int ur = 0, uj = 0, um = 1;
+        for (int r = 0; r < rounds; ++r) {
+            if (!flag_read(f->more, r)) break;
+            ++ur;
+            int n = 1;
+            for (int j = 0; j < jumps; ++j)
+                if (flag_read(f->jl[r], j)) n = j + 2;
+            uj = max(uj, n);
+        }
+        for (int j = 0; j < max_jumps; ++j)
+            if (flag_read(f->jm, j)) um = j + 2;
+        s.status[1] = ur;
+        s.status[2] = uj;
+        s.status[3] = um;
+    }
 }
 
 // CRTrees::compact_clus_label (CRTrees.cu:387-402): rank of each label among the labels used.
@@ -784,8 +809,9 @@ static void crt_setup(Ctx &c, CRTRun &r, int V) {
     r.f = (CRTFlags *)(s.flags + 8);
     HBP_CHECK(hipMemsetAsync(s.flags, 0, sizeof(int) * words, c.stream));
 }
-// enqueues the run without waiting: one block for small levels, else the scheduled kernels
-static void crt_enqueue(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
+// enqueues the run without waiting: one block for small levels, else the scheduled kernels of
+// the schedule sc (each count at most its kCrt* bound)
+static void crt_enqueue(Ctx &c, CRTRun &r, const int *d_nn, int *d_label, const CrtSched &sc) {
     const int V = r.V;
     const CRTBuf &s = r.s;
     ProfScope ps(c, "crtrees", 16.0 * V);
@@ -798,20 +824,24 @@ static void crt_enqueue(Ctx &c, CRTRun &r, const int *d_nn, int *d_label) {
     // that few blocks set the flag
     const dim3 g(grid_for(c, V)), gf(std::min(grid_for(c, V), 2 * c.num_cu)), b(256);
     CRTFlags *f = r.f;
+    HBP_REQUIRE(sc.rounds >= 1 && sc.rounds <= kCrtRounds && sc.jumps >= 1 && sc.jumps <= kCrtJumps &&
+                    sc.max_jumps >= 1 && sc.max_jumps <= kCrtMaxJumps,
+                "CRTrees schedule out of its bounds");
     hipLaunchKernelGGL(crt_init_kernel, g, b, 0, c.stream, d_nn, V, s);
-    for (int rd = 0; rd <= kCrtRounds; ++rd) {
+    for (int rd = 0; rd <= sc.rounds; ++rd) {
         hipLaunchKernelGGL(crt_count_kernel, g, b, 0, c.stream, V, s, f, rd);
-        hipLaunchKernelGGL(crt_check_kernel, gf, b, 0, c.stream, V, s, f, rd);
-        if (rd == kCrtRounds) break;
+        hipLaunchKernelGGL(crt_check_kernel, gf, b, 0, c.stream, V, s, f, rd, sc.rounds);
+        if (rd == sc.rounds) break;
         hipLaunchKernelGGL(crt_delink_kernel, g, b, 0, c.stream, V, s, f, rd);
-        for (int j = 0; j < kCrtJumps; ++j)
-            hipLaunchKernelGGL(crt_jump_kernel, gf, b, 0, c.stream, V, s, f, rd, j);
+        for (int j = 0; j < sc.jumps; ++j)
+            hipLaunchKernelGGL(crt_jump_kernel, gf, b, 0, c.stream, V, s, f, rd, j, sc.jumps);
         hipLaunchKernelGGL(crt_relink_kernel, g, b, 0, c.stream, V, s, f, rd);
         hipLaunchKernelGGL(crt_relink_all_kernel, g, b, 0, c.stream, V, s, f, rd);
     }
     hipLaunchKernelGGL(crt_max_init_kernel, g, b, 0, c.stream, V, s, f);
-    for (int j = 0; j < kCrtMaxJumps; ++j) hipLaunchKernelGGL(crt_max_kernel, gf, b, 0, c.stream, V, s, f, j);
-    hipLaunchKernelGGL(crt_final_kernel, g, b, 0, c.stream, V, s, f, d_label);
+    for (int j = 0; j < sc.max_jumps; ++j)
+        hipLaunchKernelGGL(crt_max_kernel, gf, b, 0, c.stream, V, s, f, j, sc.max_jumps);
+    hipLaunchKernelGGL(crt_final_kernel, g, b, 0, c.stream, V, s, f, d_label, sc.rounds, sc.jumps, sc.max_jumps);
     HBP_CHECK(hipGetLastError());
 }
 // the one-kernel form over a cooperative grid (a scheduled run that overflowed)
@@ -855,7 +885,7 @@ static void read_words(Ctx &c, const WordPtrs &w, int n, int *out) {
 void crtrees_labels(Ctx &c, const int *d_nn, int V, int *d_label) {
     CRTRun r;
     crt_setup(c, r, V);
-    crt_enqueue(c, r, d_nn, d_label);
+    crt_enqueue(c, r, d_nn, d_label, CrtSched{});
     int w[2] = {0, 0};
     read_words(c, WordPtrs{{r.s.status, &r.f->overflow}}, 2, w);
     if (w[1]) {
@@ -892,19 +922,35 @@ int compact_labels(Ctx &c, int *d_label, int V) {
     return compact_labels_and_read(c, d_label, V, nullptr, nullptr, 0, false);
 }
 
-// labels + compaction of one pyramid level with one host wait (two after an overflow); en, when
-// not null, is a device word (the previous edge reduction's edge count) read in the same wait
+// labels + compaction of one pyramid level with one host wait (more after an overflow); en, when
+// not null, is a device word (the previous edge reduction's edge count) read in the same wait.
+// The scheduled form runs the context's schedule (c.crt), then, on overflow, the full schedule,
+// then the cooperative form; the steps a scheduled run used set the next level's schedule: one
+// more of each than used, shrinking by at most one per level.
 static int crtrees_level(Ctx &c, const int *d_nn, int V, int *d_label, const int *en, int *en_host) {
     CRTRun r;
     crt_setup(c, r, V);
-    crt_enqueue(c, r, d_nn, d_label);
-    const int *extra[3] = {r.s.status, &r.f->overflow, en};
-    int w[3] = {0, 0, 0};
-    int Vn = compact_labels_and_read(c, d_label, V, extra, w, en ? 3 : 2, true);
-    if (en) *en_host = w[2];
+    const CrtSched full{};
+    const CrtSched sc{std::min(std::max(c.crt.rounds, 1), kCrtRounds), std::min(std::max(c.crt.jumps, 1), kCrtJumps),
+                      std::min(std::max(c.crt.max_jumps, 1), kCrtMaxJumps)};
+    const bool scheduled = V > kCrtSingleMax;
+    crt_enqueue(c, r, d_nn, d_label, sc);
+    const int *extra[6] = {r.s.status, &r.f->overflow, r.s.status + 1, r.s.status + 2, r.s.status + 3, en};
+    int w[6] = {0, 0, 0, 0, 0, 0};
+    int Vn = compact_labels_and_read(c, d_label, V, extra, w, en ? 6 : 5, true);
+    if (en) *en_host = w[5];
+    if (w[1] && scheduled && (sc.rounds < full.rounds || sc.jumps < full.jumps || sc.max_jumps < full.max_jumps)) {
+        HBP_CHECK(hipMemsetAsync(r.s.flags, 0, sizeof(int) * (8 + (sizeof(CRTFlags) + 3) / 4), c.stream));
+        crt_enqueue(c, r, d_nn, d_label, full);
+        Vn = compact_labels_and_read(c, d_label, V, extra, w, 5, true);
+    }
     if (w[1]) {
         crt_cooperative(c, r, d_nn, d_label);
         Vn = compact_labels_and_read(c, d_label, V, extra, w, 1, true);
+    } else if (scheduled) {
+        c.crt.rounds = std::max(w[2] + 1, sc.rounds - 1);
+        c.crt.jumps = std::max(w[3] + 1, sc.jumps - 1);
+        c.crt.max_jumps = std::max(w[4] + 1, sc.max_jumps - 1);
     }
     crt_require(w[0]);
     return Vn;

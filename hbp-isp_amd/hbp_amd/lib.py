@@ -239,7 +239,8 @@ class Ctx:
 
     # ---- kernel-form tuning (results are the same for every value)
     TUNING = ("coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len", "batch_graph", "fuse",
-              "pipe_batch", "pipe_blocks", "pipe_hb", "pipe_window", "run_max_e")
+              "pipe_batch", "pipe_blocks", "pipe_hb", "pipe_window", "run_max_e",
+              "crt_rounds", "crt_jumps", "crt_max_jumps")
 
     def set_tuning(self, name, value):
         _check(self.lib.hbp_ctx_set_tuning(self.h, name.encode(), int(value)))

@@ -597,3 +597,36 @@ def test_server_rejects_what_does_not_fit(ctx):
     with pytest.raises(H.HbpError):
         srv.poll(big)
     srv.close()
+
+
+@pytest.mark.parametrize("sched", [(1, 1, 1), (2, 3, 2), (5, 8, 8)])
+def test_pyramid_under_every_crtrees_schedule(ctx, sched):
+    """The scheduled CRTrees phases run the schedule the context learned from its previous levels
+    (crt_rounds, crt_jumps, crt_max_jumps); a level that needs more steps overflows it and runs
+    again with the full schedule.  Every starting schedule gives the oracle's levels, and the
+    context's schedule afterwards stays inside the bounds."""
+    rounds, jumps, max_jumps = sched
+    say(f"pyramid 640x480 under CRTrees schedule {sched}")
+    left, _, _ = synth.make_pair(12, 640, 480, 64, (6, 12))
+    po = O.Pyramid(left)
+    before = {k: ctx.get_tuning(k) for k in ("crt_rounds", "crt_jumps", "crt_max_jumps")}
+    try:
+        for _ in range(2):   # the second build starts from the schedule the first one learned
+            ctx.set_tuning("crt_rounds", rounds)
+            ctx.set_tuning("crt_jumps", jumps)
+            ctx.set_tuning("crt_max_jumps", max_jumps)
+            pg = ctx.pyramid(left)
+            assert pg.levels == po.levels
+            for l in range(po.levels):
+                a, b = po.level(l), pg.level(l)
+                assert (a["V"], a["E"]) == (b["V"], b["E"]), l
+                assert np.array_equal(a["clus"], b["clus"]), l
+                assert np.array_equal(a["bd"], b["bd"]), l
+                assert same_bits(a["mean"], b["mean"]), l
+            assert 1 <= ctx.get_tuning("crt_rounds") <= 5
+            assert 1 <= ctx.get_tuning("crt_jumps") <= 8
+            assert 1 <= ctx.get_tuning("crt_max_jumps") <= 8
+            rounds, jumps, max_jumps = (ctx.get_tuning(k) for k in ("crt_rounds", "crt_jumps", "crt_max_jumps"))
+    finally:
+        for k, v in before.items():
+            ctx.set_tuning(k, v)

@@ -116,13 +116,18 @@ def test_tuning_parameters_are_checked(tuned):
         tuned.set_tuning("pipe_blocks", 4)
     with pytest.raises(H.HbpError, match="pipe_window"):
         tuned.set_tuning("pipe_window", -1)
+    with pytest.raises(H.HbpError, match="crt_rounds"):
+        tuned.set_tuning("crt_rounds", 0)
+    with pytest.raises(H.HbpError, match="crt_jumps"):
+        tuned.set_tuning("crt_jumps", 9)
     with pytest.raises(H.HbpError, match="level form"):
         tuned.set_tuning("level_form.3", 2)
     with pytest.raises(H.HbpError, match="out of range"):
         tuned.set_tuning("level_form.24", 0)
     assert set(tuned.get_tuning()) == {"coarse_min_pix", "single_hb", "batch_hb", "hamming_cols", "seg_len",
                                       "batch_graph", "fuse", "pipe_batch", "pipe_blocks",
-                                      "pipe_hb", "pipe_window", "run_max_e"}
+                                      "pipe_hb", "pipe_window", "run_max_e", "crt_rounds", "crt_jumps",
+                                      "crt_max_jumps"}
 
 
 def test_tuner_sets_the_fastest_grid_point(tuned):
