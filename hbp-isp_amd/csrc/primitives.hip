@@ -158,13 +158,48 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K *__restrict
     }
 }
 
+// up to SC_TILE tiles: one workgroup per digit d scans that digit's row of tile counts,
+// offs[d][t] = hist[d][0] + ... + hist[d][t-1], and leaves the row's total in totals[d]; the
+// scatter adds the digits' bases (the exclusive scan of totals), which makes the offsets of the
+// digit-major scan in one launch instead of three
+__global__ __launch_bounds__(SC_THREADS) void rs_rowscan_kernel(const int *__restrict__ hist, int ntiles,
+                                                              int *__restrict__ offs, int *__restrict__ totals) {
+    __shared__ int w[SC_THREADS / 64];
+    const long row = (long)blockIdx.x * ntiles;
+    const int base = threadIdx.x * SC_ITEMS;
+    int v[SC_ITEMS], s = 0;
+#pragma unroll
+    for (int i = 0; i < SC_ITEMS; ++i) {
+        v[i] = base + i < ntiles ? hist[row + base + i] : 0;
+        s += v[i];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, w, total);
+#pragma unroll
+    for (int i = 0; i < SC_ITEMS; ++i) {
+        if (base + i < ntiles) offs[row + base + i] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+// totals: null when offs holds the digit-major scan itself, else the digits' totals
+// (rs_rowscan_kernel; offs then holds each digit's row scan)
 template <class K, class V, bool HAS_V>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K *__restrict__ kin, K *__restrict__ kout,
                                                                 const V *__restrict__ vin, V *__restrict__ vout,
                                                                 long n, int shift, unsigned mask, int ntiles,
-                                                                const int *__restrict__ offs) {
+                                                                const int *__restrict__ offs,
+                                                                const int *__restrict__ totals) {
+    static_assert(RS_BINS == RS_THREADS, "one digit per thread");
     __shared__ int wc[RS_WAVES][RS_BINS];   // per wavefront: running counts, then first positions
+    __shared__ int dbase[RS_BINS];          // with totals: the keys of lower digits
+    __shared__ int ws[RS_THREADS / 64];
     for (int i = threadIdx.x; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wc[0][0])[i] = 0;
+    if (totals) {
+        int all;
+        dbase[threadIdx.x] = block_exclusive_scan(totals[threadIdx.x], ws, all);
+    }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -201,7 +236,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K *__restr
     __syncthreads();
     // first position of each (wavefront, digit): the tile's offset plus the lower wavefronts' counts
     for (int d = threadIdx.x; d < RS_BINS; d += RS_THREADS) {
-        int pos = offs[(long)d * ntiles + blockIdx.x];
+        int pos = offs[(long)d * ntiles + blockIdx.x] + (totals ? dbase[d] : 0);
 #pragma unroll
         for (int w = 0; w < RS_WAVES; ++w) {
             const int cnt = wc[w][d];
@@ -323,9 +358,11 @@ static bool radix_sort(Ctx &c, K *keys, K *keys_tmp, V *vals, V *vals_tmp, long 
         return false;
     }
     const long ntiles = div_up(n, RS_TILE);
-    // scratch: hist [BINS][tiles], its exclusive scan, the scan's tile sums
+    // scratch: hist [BINS][tiles], its exclusive scan (or each digit's row scan), the scan's tile
+    // sums (or the digits' totals)
     const long nh = (long)RS_BINS * ntiles;
-    int *hist = (int *)c.scratch_bytes(sizeof(int) * (size_t)(2 * nh + scan_tiles(nh)));
+    const bool rows = ntiles <= SC_TILE;
+    int *hist = (int *)c.scratch_bytes(sizeof(int) * (size_t)(2 * nh + std::max<long>(scan_tiles(nh), RS_BINS)));
     int *offs = hist + nh, *sums = offs + nh;
     K *ka = keys, *kb = keys_tmp;
     V *va = vals, *vb = vals_tmp;
@@ -335,9 +372,14 @@ static bool radix_sort(Ctx &c, K *keys, K *keys_tmp, V *vals, V *vals_tmp, long 
         const unsigned mask = (1u << bits) - 1u;
         hipLaunchKernelGGL(rs_hist_kernel<K>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, c.stream, ka, n, shift,
                            mask, (int)ntiles, hist);
-        scan_exclusive_buf(c, hist, offs, nh, sums);
+        if (rows)
+            hipLaunchKernelGGL(rs_rowscan_kernel, dim3(RS_BINS), dim3(SC_THREADS), 0, c.stream, (const int *)hist,
+                               (int)ntiles, offs, sums);
+        else
+            scan_exclusive_buf(c, hist, offs, nh, sums);
         hipLaunchKernelGGL((rs_scatter_kernel<K, V, HAS_V>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, c.stream, ka,
-                           kb, va, vb, n, shift, mask, (int)ntiles, (const int *)offs);
+                           kb, va, vb, n, shift, mask, (int)ntiles, (const int *)offs,
+                           rows ? (const int *)sums : (const int *)nullptr);
         HBP_CHECK(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);

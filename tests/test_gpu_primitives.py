@@ -25,9 +25,11 @@ def sort_dev(ctx, keys, vals, end_bit):
     return k, v
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 63, 4095, 4096, 4097, 65537, 1 << 20, 2_900_000])
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 4095, 4096, 4097, 65537, 1 << 20, 2_900_000, 8_388_609])
 @pytest.mark.parametrize("key_bits", [32, 64])
 def test_radix_sort_is_numpys_stable_sort(ctx, n, key_bits):
+    """Up to 2048 tiles of 4096 keys a pass scans each digit's row of tile counts in one launch;
+    above (8388609 keys) the whole digit-major table is scanned in tiles."""
     rng = np.random.default_rng(n * 7 + key_bits)
     for end_bit in sorted({1, 7, 8, 13, 24, key_bits}):
         say(f"sort n={n} keys={key_bits} end_bit={end_bit}")
