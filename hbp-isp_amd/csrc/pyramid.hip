@@ -186,7 +186,7 @@ __global__ void nn_grid_kernel(const float4 *__restrict__ lab, int W, int H, int
 struct CRTBuf {
     int *nnw, *A, *B, *M, *C, *mask, *maskc, *indeg, *flags, *status;
 };
-constexpr int kCrtSingleMax = 16000;
+constexpr int kCrtSingleMax = 8000;
 constexpr int kCrtRounds = 5, kCrtJumps = 8, kCrtMaxJumps = 8;
 static_assert(CrtSched{}.rounds == kCrtRounds && CrtSched{}.jumps == kCrtJumps && CrtSched{}.max_jumps == kCrtMaxJumps,
               "the default schedule is the full one");

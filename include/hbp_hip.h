@@ -79,7 +79,7 @@ int hbp_primitive_scan(hbp_ctx *ctx, const int32_t *in, int32_t *out, long long 
  *      "pipe_window" (0..1024, default 4): such launches issued ahead of the one the first
  *      stream waits for (0: all of a level's at its start);
  *      "crt_rounds" (1..5), "crt_jumps" (1..8), "crt_max_jumps" (1..8): the schedule of the
- *      next pyramid level's CRTrees phase kernels (levels above 16000 nodes); the context
+ *      next pyramid level's CRTrees phase kernels (levels above 8000 nodes); the context
  *      keeps adapting it to the steps its levels use (a level that needs more reruns with the
  *      full schedule);
  *      "level_form.<l>" (l < 24; -1 default, 0, 1): BP level l costs

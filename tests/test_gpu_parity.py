@@ -113,7 +113,7 @@ def test_pyramid_large_pair(ctx):
 
 @pytest.mark.parametrize("W,H,kind", [(20, 1000, "flat"), (24, 900, "columns")])
 def test_pyramid_level_whose_crtrees_outgrows_the_schedule(ctx, W, H, kind):
-    """A level above the one-block size (> 16000 nodes) whose 1-NN chains are longer than the
+    """A level above the one-block size (> 8000 nodes) whose 1-NN chains are longer than the
     scheduled pointer jumping covers (2^8): in a flat image every pixel's first minimum is the
     pixel above it, so each column is one chain of H nodes.  The scheduled kernels overflow and
     the cooperative kernel redoes the level; the labels the compaction reads in between must
@@ -122,7 +122,7 @@ def test_pyramid_level_whose_crtrees_outgrows_the_schedule(ctx, W, H, kind):
     left = np.full((H, W, 3), 90, np.uint8)
     if kind == "columns":
         left[:, :, 1] = (np.arange(W) * 9 % 256).astype(np.uint8)[None, :]
-    assert W * H > 16000 and H > 2 ** 8
+    assert W * H > 8000 and H > 2 ** 8
     po = O.Pyramid(left)
     pg = ctx.pyramid(left)
     assert pg.levels == po.levels
