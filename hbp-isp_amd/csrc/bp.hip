@@ -153,13 +153,22 @@ __global__ void seg_count_kernel(const int *__restrict__ pix_off, int V, int sl,
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c <= V; c += (long)gridDim.x * blockDim.x)
         cnt[c] = c < V ? (pix_off[c + 1] - pix_off[c] + sl - 1) / sl : 0;
 }
+// one thread per segment k < sego[V]: its superpixel is the last c with sego[c] <= k (superpixels
+// of the coarse levels hold up to tens of thousands of pixels, so one thread per superpixel left
+// a few threads writing a hundred segments each)
 __global__ void seg_fill_kernel(const int *__restrict__ pix_off, const int *__restrict__ sego, int V, int sl,
-                                int4 *__restrict__ seg) {
-    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < V; c += (long)gridDim.x * blockDim.x) {
-        int b = pix_off[c], e = pix_off[c + 1];
-        int o = sego[c];
-        for (int i = 0; b + i * sl < e; ++i)
-            seg[o + i] = make_int4(b + i * sl, min(b + (i + 1) * sl, e), (int)c, i);
+                                long nmax, int4 *__restrict__ seg) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nmax; k += (long)gridDim.x * blockDim.x) {
+        if (k >= sego[V]) break;
+        int lo = 0, hi = V;   // sego[lo] <= k < sego[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sego[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        const int i = (int)(k - sego[lo]);
+        const int b = pix_off[lo], e = pix_off[lo + 1];
+        seg[k] = make_int4(b + i * sl, min(b + (i + 1) * sl, e), lo, i);
     }
 }
 // reverse_bd_id_k (hbp_isp.cu:154-175): first r in t's CSR range with bd[r].y == s
@@ -2135,8 +2144,8 @@ void bp_setup(Ctx &c, BP &b) {
                 hipLaunchKernelGGL(seg_count_kernel, dim3(grid_for(c, (long)V + 1)), dim3(256), 0, c.stream,
                                    B.pix_off.p, V, sl, scnt.p);
                 scan_exclusive_int(c, scnt.p, B.sego.p, V + 1, nullptr);
-                hipLaunchKernelGGL(seg_fill_kernel, dim3(grid_for(c, V)), dim3(256), 0, c.stream, B.pix_off.p,
-                                   B.sego.p, V, sl, B.seg.p);
+                hipLaunchKernelGGL(seg_fill_kernel, dim3(grid_for(c, B.nseg_max)), dim3(256), 0, c.stream, B.pix_off.p,
+                                   B.sego.p, V, sl, (long)B.nseg_max, B.seg.p);
             }
             // CSR by source, reverse edges, edge weights, mean positions
             B.off.alloc(c.pool, (size_t)V + 1);
