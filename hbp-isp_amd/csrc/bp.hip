@@ -1243,15 +1243,11 @@ __device__ __forceinline__ void sel_load(const SweepArgs &a, const ChainIO &io, 
 #pragma unroll
     for (int i = 0; i < kNodeMaxDeg; ++i) in.mt[i] = mtp[i * K];
 }
-__device__ __forceinline__ SelOut sel_compute(const SweepArgs &a, const ChainIO &io, long n, const SelIn &in, int k,
-                                              int kk) {
+// the rank and compaction of a selection from each slot's belief: src = the slot whose values
+// output slot k takes
+__device__ __forceinline__ SelOut sel_rank(float bel, float cost, float4 pin, int k) {
     const int g = (threadIdx.x & 63) & ~15;
     const bool own = k < K;
-    const float cost = kk >= 2 * P ? from_fixed(in.fsum, (in.nb >> kk) & 1u, a.shift) : in.cost_in;
-    float bel = cost;
-#pragma unroll
-    for (int i = 0; i < kNodeMaxDeg; ++i) bel = i < in.deg ? bel + in.mt[i] : bel;
-    for (int i = kNodeMaxDeg; i < in.deg; ++i) bel += io.mt_in[(n * a.maxdeg + i) * K + kk];
     const int order = belief_rank(bel, k, std::make_integer_sequence<int, K>{});
     const bool sel = own && order < P;
     const uint32_t m = (uint32_t)(__ballot(sel) >> g) & 0xfffu;
@@ -1262,9 +1258,17 @@ __device__ __forceinline__ SelOut sel_compute(const SweepArgs &a, const ChainIO 
         for (int i = 0; i < k; ++i) mm &= mm - 1u;
         src = __ffs(mm) - 1;
     }
-    return SelOut{bel, cost, in.pin, src};
+    return SelOut{bel, cost, pin, src};
 }
-
+__device__ __forceinline__ SelOut sel_compute(const SweepArgs &a, const ChainIO &io, long n, const SelIn &in, int k,
+                                              int kk) {
+    const float cost = kk >= 2 * P ? from_fixed(in.fsum, (in.nb >> kk) & 1u, a.shift) : in.cost_in;
+    float bel = cost;
+#pragma unroll
+    for (int i = 0; i < kNodeMaxDeg; ++i) bel = i < in.deg ? bel + in.mt[i] : bel;
+    for (int i = kNodeMaxDeg; i < in.deg; ++i) bel += io.mt_in[(n * a.maxdeg + i) * K + kk];
+    return sel_rank(bel, cost, in.pin, k);
+}
 // one step p of the message minimum over t's 2P particles (edge_wave's arithmetic), the
 // particle's values broadcast from lane p of the row
 template <int p>
@@ -1640,13 +1644,24 @@ __global__ __launch_bounds__(256) void pc_ahead_kernel(SweepArgs a, PipeArgs pa,
 // in one workgroup of 1024 threads.  The RANSAC sweeps' plane costs exist (second stream), so
 // what remains per sweep is the selection of every superpixel (phase A, 16 lanes each) and the
 // messages of every edge (phase B, 16 lanes each), with a workgroup barrier between them instead
-// of a launch.  The selection's outputs (planes, costs, beliefs and source slots of slots < 2P)
-// and the sweep messages (two node-major buffers) live in LDS.  The arithmetic is node_group's and
-// edge_wave's: the same bits as the fused launches.  Limits: V <= RUN_V, maxdeg <= kNodeMaxDeg,
-// V maxdeg <= RUN_MT (the host checks them).
+// of a launch.  Everything the run reads besides the first sweep's state is small (V <= RUN_V
+// superpixels, E <= RUN_MT edges, at most RUN_NR listed sweeps), so a prologue loads all of it
+// into LDS in one memory round trip, together with the first selection's inputs: the listed
+// sweeps' candidates, points and costs (their replica sums and NaN words folded into the float
+// cost, as the selection computes it), the edges' ends, slots and weights, and the first
+// sweep's messages.  After it every phase reads LDS and registers only.  The selection's outputs
+// (planes, costs, beliefs and source slots of slots < 2P) and the sweep messages (two node-major
+// buffers) live in LDS as well.  The arithmetic is node_group's and edge_wave's: the same bits
+// as the fused launches.  Limits: V <= RUN_V, maxdeg <= kNodeMaxDeg, V maxdeg <= RUN_MT (so
+// E <= RUN_MT), n <= RUN_NR (the host checks them).
 // ----------------------------------------------------------------------------------------
-constexpr int RUN_V = 64, RUN_MT = 384;
-constexpr size_t RUN_LDS_BYTES = (size_t)RUN_V * 2 * P * (16 + 4 + 4 + 4) + 2 * (size_t)RUN_MT * K * 4;
+constexpr int RUN_V = 32, RUN_MT = 384, RUN_NR = 4;
+constexpr size_t RUN_LDS_BYTES = (size_t)RUN_V * 2 * P * (16 + 4 + 4 + 4)   // sp, sc, sb, ss
+                                 + 2 * (size_t)RUN_MT * K * 4                // smt
+                                 + (size_t)RUN_NR * RUN_V * P * (16 + 4)     // pcand, pcost
+                                 + (size_t)RUN_NR * RUN_V * 16               // ppt
+                                 + (size_t)RUN_MT * (16 + 8);                // edges
+static_assert(RUN_LDS_BYTES <= 65536, "one workgroup's LDS");
 struct RunArgs {
     const unsigned long long *acc0;   // sums of the run's first sweep (slot stride K), cleared at the end
     unsigned int *nan0;
@@ -1661,95 +1676,108 @@ struct RunArgs {
 __global__ __launch_bounds__(1024) void run_kernel(SweepArgs a, RunArgs ra) {
     chain_priority();
     extern __shared__ unsigned long long lds_run[];
-    float4 *sp = (float4 *)lds_run;                    // [RUN_V][2P]
-    float *sc = (float *)(sp + RUN_V * 2 * P);         // [RUN_V][2P]
+    float4 *sp = (float4 *)lds_run;                    // [RUN_V][2P] selected planes
+    float4 *pcand = sp + RUN_V * 2 * P;                // [RUN_NR][RUN_V][P] listed sweeps' candidates
+    uint4 *ppt = (uint4 *)(pcand + RUN_NR * RUN_V * P);   // [RUN_NR][RUN_V] their RANSAC points
+    int4 *e4 = (int4 *)(ppt + RUN_NR * RUN_V);         // [RUN_MT] edge (source, target, slot, reverse slot)
+    float2 *ecd = (float2 *)(e4 + RUN_MT);             // [RUN_MT] edge (c0, dpos)
+    float *pcost = (float *)(ecd + RUN_MT);            // [RUN_NR][RUN_V][P] candidates' costs
+    float *sc = pcost + RUN_NR * RUN_V * P;            // [RUN_V][2P]
     float *sb = sc + RUN_V * 2 * P;                    // [RUN_V][2P]
     int *ss = (int *)(sb + RUN_V * 2 * P);             // [RUN_V][2P]
     float *smt = (float *)(ss + RUN_V * 2 * P);        // [2][RUN_MT][K]
     const PipeArgs &pa = ra.pa;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int k = lane & 15, g = lane & ~15;
     const bool own = k < K;
     const int kk = own ? k : 0;
     const int row = wave * 4 + (lane >> 4);            // 64 rows of 16 lanes
-    for (int u = 1; u <= ra.n + 1; ++u) {
-        // ---- phase A: the selection that ends sweep j0+u-1 (u = n+1: the run's last, closing)
-        const int n = row;
-        if (n < a.V) {
-            const bool first = u == 1;                 // sweep j0+u-1 is the run's first
-            const float *mtp = first ? ra.mt0 : smt + (size_t)((u - 1) & 1) * RUN_MT * K;
-            const long nm = (long)n * a.maxdeg;
-            SelIn in;
-            in.deg = a.off[n + 1] - a.off[n];
-            const int rp = ra.r1 + u - 2;              // list index of sweep j0+u-1 (u >= 2)
-            in.nb = first ? ra.nan0[n] : pa.nan[(long)rp * a.V + n];
-            in.cost_in = first ? a.cost[(long)n * K + kk] : sc[n * 2 * P + (kk < 2 * P ? kk : 0)];
-            in.fsum = first ? acc_sum_at(ra.acc0, a.rrep, n, kk >= 2 * P ? kk : 2 * P)
-                            : acc_sum_at(pa.acc + rp * pa.aslice, a.rrep, n, kk >= 2 * P ? kk : 2 * P, P);
-            if (kk < 2 * P) in.pin = first ? a.plane[(long)n * K + kk] : sp[n * 2 * P + kk];
-            else in.pin = first ? a.plane[(long)n * K + kk] : pa.cand[rp * pa.cslice + (long)n * P + kk - 2 * P];
+    // ---- prologue: the first selection's inputs (row n = row) in registers, the rest in LDS
+    const int n = row;
+    const bool node = n < a.V;
+    const int nc = node ? n : 0;
+    const long nm = (long)nc * a.maxdeg;
+    const int deg = node ? a.off[nc + 1] - a.off[nc] : 0;
+    uint32_t seed = a.s_ransac[nc];
+    const bool child_seeds = ra.prev_spatial && k == 0 && node && n < a.Vh;
+    uint32_t seed_c = child_seeds ? a.s_child[nc] : 0u, seed_n = child_seeds ? a.s_neighbor[nc] : 0u;
+    SelIn in;
+    in.deg = deg;
+    in.nb = ra.nan0[nc];
+    in.cost_in = a.cost[(long)nc * K + kk];
+    in.fsum = acc_sum_at(ra.acc0, a.rrep, nc, kk >= 2 * P ? kk : 2 * P);
+    in.pin = a.plane[(long)nc * K + kk];
 #pragma unroll
-            for (int i = 0; i < kNodeMaxDeg; ++i) in.mt[i] = i < in.deg ? mtp[(nm + i) * K + kk] : 0.f;
-            ChainIO none{};
-            const SelOut so = sel_compute(a, none, n, in, k, kk);
-            const float4 pl_n = make_float4(__shfl(so.pl.x, g + so.src), __shfl(so.pl.y, g + so.src),
-                                            __shfl(so.pl.z, g + so.src), __shfl(so.pl.w, g + so.src));
-            const float co_n = __shfl(so.cost, g + so.src), be_n = __shfl(so.bel, g + so.src);
-            if (u <= ra.n) {
-                if (k < 2 * P) {
-                    sp[n * 2 * P + k] = pl_n;
-                    sc[n * 2 * P + k] = co_n;
-                    sb[n * 2 * P + k] = be_n;
-                    ss[n * 2 * P + k] = so.src;
-                }
-                // the selection's seed advance (its next sweep is a RANSAC sweep)
-                if (k == 2 * P) {
-                    constexpr LcgJump j3 = lcg_jump_coef(3 * P);
-                    a.s_ransac[n] = (j3.a * a.s_ransac[n] + j3.c) % 2147483648u;
-                }
-                if (first && ra.prev_spatial && k == 0 && n < a.Vh) {
-                    constexpr LcgJump jp = lcg_jump_coef(P);
-                    a.s_child[n] = (jp.a * a.s_child[n] + jp.c) % 2147483648u;
-                    a.s_neighbor[n] = (jp.a * a.s_neighbor[n] + jp.c) % 2147483648u;
-                }
-            } else {
-                // the closing node update (node_group, next sweep spatial or none): standard state
-                if (own) {
-                    a.plane[(long)n * K + k] = pl_n;
-                    a.cost[(long)n * K + k] = co_n;
-                    a.belief[(long)n * K + k] = be_n;
-                }
-                float *msg = (float *)a.msg;
-#pragma unroll
-                for (int i = 0; i < kNodeMaxDeg; ++i) {
-                    const float oi = __shfl(in.mt[i], g + so.src);
-                    if (i < in.deg && k < 2 * P) msg[(long)a.revl[nm + i] * 8 + k] = oi;
-                }
-                if (k == K - 1) a.point[n] = pa.pt[(long)(ra.r1 + ra.n - 1) * a.V + n];
-                if (kk >= 2 * P) acc_zero_at((unsigned long long *)ra.acc0, a.rrep, n, kk);
-                if (k == 2 * P) ra.nan0[n] = 0u;
-            }
+    for (int i = 0; i < kNodeMaxDeg; ++i) in.mt[i] = i < deg ? ra.mt0[(nm + i) * K + kk] : 0.f;
+    // lane i of the row: the reverse (node, slot) of incoming slot i (for the closing update)
+    const int rv = k < deg ? a.revl[nm + k] : 0;
+    {
+        const int nvp = a.V * P;
+        for (int i = tid; i < ra.n * nvp; i += 1024) {
+            const int r = i / nvp, v = (i % nvp) / P, p = i % P;
+            const int rp = ra.r1 + r;
+            const float4 cand = pa.cand[rp * pa.cslice + (long)v * P + p];
+            const int64_t fsum = acc_sum_at(pa.acc + rp * pa.aslice, a.rrep, v, 2 * P + p, P);
+            const unsigned int nb = pa.nan[(long)rp * a.V + v];
+            pcand[(r * RUN_V + v) * P + p] = cand;
+            pcost[(r * RUN_V + v) * P + p] = from_fixed(fsum, (nb >> (2 * P + p)) & 1u, a.shift);
         }
-        __syncthreads();
-        if (u > ra.n) break;
-        // ---- phase B: the messages of sweep j0+u
-        const float *mtp = u == 1 ? ra.mt0 : smt + (size_t)((u - 1) & 1) * RUN_MT * K;
-        float *mto = smt + (size_t)(u & 1) * RUN_MT * K;
-        const int rc = ra.r1 + u - 1;                  // list index of sweep j0+u
-        for (long e = row; e < a.E; e += RUN_V) {
+        for (int i = tid; i < ra.n * a.V; i += 1024) {
+            const int r = i / a.V, v = i % a.V;
+            ppt[r * RUN_V + v] = pa.pt[(long)(ra.r1 + r) * a.V + v];
+        }
+        for (int e = tid; e < a.E; e += 1024) {
             const int2 b = a.bd[e];
-            const int sn = b.x, tn = b.y;
-            const int eslot = a.eloc[e], rslot = ra.rloc[e];
-            const float c0 = a.c0[e], dpos = a.dpos[e];
+            e4[e] = make_int4(b.x, b.y, a.eloc[e], ra.rloc[e]);
+            ecd[e] = make_float2(a.c0[e], a.dpos[e]);
+        }
+        // the first sweep's messages, read by the first sweep's successor as smt buffer 0
+        for (int i = tid; i < a.V * a.maxdeg * K; i += 1024) smt[i] = ra.mt0[i];
+    }
+    // a selection that is not the run's last: its outputs to LDS and the seed advance (its next
+    // sweep is a RANSAC sweep)
+    auto keep = [&](const SelOut &so, bool first) {
+        const float4 pl_n = make_float4(__shfl(so.pl.x, g + so.src), __shfl(so.pl.y, g + so.src),
+                                        __shfl(so.pl.z, g + so.src), __shfl(so.pl.w, g + so.src));
+        const float co_n = __shfl(so.cost, g + so.src), be_n = __shfl(so.bel, g + so.src);
+        if (k < 2 * P) {
+            sp[n * 2 * P + k] = pl_n;
+            sc[n * 2 * P + k] = co_n;
+            sb[n * 2 * P + k] = be_n;
+            ss[n * 2 * P + k] = so.src;
+        }
+        constexpr LcgJump j3 = lcg_jump_coef(3 * P);
+        seed = (j3.a * seed + j3.c) % 2147483648u;
+        if (k == 2 * P) a.s_ransac[n] = seed;
+        if (first && child_seeds) {
+            constexpr LcgJump jp = lcg_jump_coef(P);
+            a.s_child[n] = (jp.a * seed_c + jp.c) % 2147483648u;
+            a.s_neighbor[n] = (jp.a * seed_n + jp.c) % 2147483648u;
+        }
+    };
+    // ---- phase A of the run's first sweep (never its last: n >= 1)
+    if (node) keep(sel_compute(a, ChainIO{}, n, in, k, kk), true);
+    __syncthreads();
+    for (int u = 1; u <= ra.n; ++u) {
+        // ---- phase B: the messages of sweep j0+u (listed sweep u-1)
+        const float *mtp = smt + (size_t)((u - 1) & 1) * RUN_MT * K;
+        float *mto = smt + (size_t)(u & 1) * RUN_MT * K;
+        const int rl = (u - 1) * RUN_V;
+        for (int e = row; e < a.E; e += 64) {
+            const int4 ed = e4[e];
+            const int sn = ed.x, tn = ed.y, eslot = ed.z, rslot = ed.w;
+            const float2 cd = ecd[e];
+            const float c0 = cd.x, dpos = cd.y;
             // slots >= 2P: this sweep's candidates (lanes past the K slots read slot 2P's)
             const int kc = kk >= 2 * P ? kk - 2 * P : 0;
-            const float4 psl = k < 2 * P ? sp[sn * 2 * P + k] : pa.cand[rc * pa.cslice + (long)sn * P + kc];
+            const float4 psl = k < 2 * P ? sp[sn * 2 * P + k] : pcand[(rl + sn) * P + kc];
             // lanes p < 2P: t's particle p after its selection, its belief less the message from s
             const int pp = k < 2 * P ? k : 0;
             const int tsrc = ss[tn * 2 * P + pp];
             const float tm = sb[tn * 2 * P + pp] - mtp[((long)tn * a.maxdeg + rslot) * K + tsrc];
             const float4 tp = sp[tn * 2 * P + pp];
-            const uint4 ps = pa.pt[(long)rc * a.V + sn], pt = pa.pt[(long)rc * a.V + tn];
+            const uint4 ps = ppt[rl + sn], pt = ppt[rl + tn];
             float X[6], Y[6];
             X[0] = (float)unpack_x(ps.x); Y[0] = (float)unpack_y(ps.x);
             X[1] = (float)unpack_x(ps.y); Y[1] = (float)unpack_y(ps.y);
@@ -1764,6 +1792,46 @@ __global__ __launch_bounds__(1024) void run_kernel(SweepArgs a, RunArgs ra) {
             float mean = row_sum(out, std::make_integer_sequence<int, K>{});
             mean /= K;
             if (own) mto[((long)sn * a.maxdeg + eslot) * K + k] = out - mean;
+        }
+        __syncthreads();
+        // ---- phase A: the selection that ends sweep j0+u (listed sweep u-1); u = n: the run's
+        // last, the closing node update (node_group, next sweep spatial or none) into the
+        // standard state
+        if (node) {
+            const int ps = n * 2 * P + (kk < 2 * P ? kk : 0);
+            const int pc = ((u - 1) * RUN_V + n) * P + (kk >= 2 * P ? kk - 2 * P : 0);
+            const float cost = kk >= 2 * P ? pcost[pc] : sc[ps];
+            const float4 pin = kk < 2 * P ? sp[ps] : pcand[pc];
+            float mt[kNodeMaxDeg];
+#pragma unroll
+            for (int i = 0; i < kNodeMaxDeg; ++i) mt[i] = i < deg ? mto[(nm + i) * K + kk] : 0.f;
+            float bel = cost;
+#pragma unroll
+            for (int i = 0; i < kNodeMaxDeg; ++i) bel = i < deg ? bel + mt[i] : bel;
+            const SelOut so = sel_rank(bel, cost, pin, k);
+            if (u < ra.n) {
+                keep(so, false);
+            } else {
+                const float4 pl_n = make_float4(__shfl(so.pl.x, g + so.src), __shfl(so.pl.y, g + so.src),
+                                                __shfl(so.pl.z, g + so.src), __shfl(so.pl.w, g + so.src));
+                const float co_n = __shfl(so.cost, g + so.src), be_n = __shfl(so.bel, g + so.src);
+                if (own) {
+                    a.plane[(long)n * K + k] = pl_n;
+                    a.cost[(long)n * K + k] = co_n;
+                    a.belief[(long)n * K + k] = be_n;
+                }
+                int r[kNodeMaxDeg];
+                row_revl(rv, r, std::make_integer_sequence<int, kNodeMaxDeg>{});
+                float *msg = (float *)a.msg;
+#pragma unroll
+                for (int i = 0; i < kNodeMaxDeg; ++i) {
+                    const float oi = __shfl(mt[i], g + so.src);
+                    if (i < deg && k < 2 * P) msg[(long)r[i] * 8 + k] = oi;
+                }
+                if (k == K - 1) a.point[n] = ppt[(ra.n - 1) * RUN_V + n];
+                if (kk >= 2 * P) acc_zero_at((unsigned long long *)ra.acc0, a.rrep, n, kk);
+                if (k == 2 * P) ra.nan0[n] = 0u;
+            }
         }
         __syncthreads();
     }
@@ -2646,9 +2714,9 @@ static void run_chain(Ctx &c, BP &b, size_t j0, int n) {
         const size_t jl = j0 + (size_t)uL;
         if (!b.sched[jl].last && jl + 1 < b.sched.size()) next_kind_l = b.sched[jl + 1].kind;
     }
-    bool tiny = pipe && a.V <= RUN_V && a.E <= c.run_max_e && B.maxdeg <= kNodeMaxDeg &&
-                (long)a.V * B.maxdeg <= RUN_MT && (next_kind_l == kNone || next_kind_l == kSpatial) &&
-                loc(0).r < 0;
+    bool tiny = pipe && a.V <= RUN_V && a.E <= c.run_max_e && a.E <= RUN_MT && B.maxdeg <= kNodeMaxDeg &&
+                (long)a.V * B.maxdeg <= RUN_MT && uL >= 1 && uL <= RUN_NR &&
+                (next_kind_l == kNone || next_kind_l == kSpatial) && loc(0).r < 0;
     for (int u = 1; tiny && u <= uL; ++u) tiny = loc(u).r >= 0 && loc(u).r == loc(1).r + u - 1;
     for (int u = 0; u <= uL; ++u) {
         if (tiny && u == 1) {
