@@ -73,7 +73,7 @@ def test_fused_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_v
 @pytest.mark.parametrize("seed,W,H,D,ml,right_view,cost,kw", CASES)
 @pytest.mark.parametrize("max_e", [64, 4096])
 def test_one_workgroup_runs_give_the_split_sweeps_bits(tuned, seed, W, H, D, ml, right_view, cost, kw, max_e):
-    """Levels of at most 64 superpixels (and run_max_e edges) run a fused run's sweeps after its
+    """Levels of at most 32 superpixels (and run_max_e edges) run a fused run's sweeps after its
     first in one workgroup; with run_max_e 4096 every such level does."""
     say(f"one-workgroup runs seed={seed} {W}x{H} D={D} ml={ml} right={right_view} {cost} {kw} max_e={max_e}")
     left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
@@ -110,7 +110,7 @@ def test_fused_state_after_every_step_call(tuned, chunk, fuse):
     say(f"fused {fuse} state, steps of {chunk}")
     left, right, _ = synth.make_pair(76, 288, 216, 40, (3, 8))
     tuned.set_tuning("coarse_min_pix", 1)
-    if fuse == 3:        # fuse 2 with one-workgroup runs on every level of at most 64 superpixels
+    if fuse == 3:        # fuse 2 with one-workgroup runs on every level of at most 32 superpixels
         tuned.set_tuning("run_max_e", 4096)
         fuse = 2
     elif fuse == 4:      # fuse 2 with the plane-cost launches issued one ahead of the waits
