@@ -50,6 +50,7 @@ CASES = [
     (73, 240, 176, 40, 2, False, "Census", {"use_cost_vol": 0, "alpha": 1.3}),
     (74, 240, 160, 32, 1, False, "AD", {}),
     (75, 200, 152, 24, 2, False, "Census", {"alpha": -3.7, "tau_s": 2.5}),
+    (77, 217, 151, 27, 1, True, "Census", {}),     # odd sizes and disparity count
 ]
 
 

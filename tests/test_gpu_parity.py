@@ -330,7 +330,7 @@ def test_bp_state_after_each_of_the_first_sweeps(ctx, seed, W, H, D, ml):
 
 
 @pytest.mark.parametrize("seed,W,H,D,ml", [(1, 64, 48, 16, 2), (2, 96, 72, 24, 1), (6, 120, 90, 32, 2),
-                                           (8, 160, 120, 48, 3)])
+                                           (8, 160, 120, 48, 3), (11, 101, 67, 20, 1), (12, 203, 131, 37, 2)])
 def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
     say(f"bp full seed={seed} {W}x{H} D={D} min_level={ml}")
     left, right = _bp_pair(seed, W, H, D)
