@@ -347,6 +347,25 @@ def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
     assert same_bits(plo, plg)
 
 
+@pytest.mark.parametrize("seed,W,H,D,ml", [(21, 48, 36, 2, 1), (22, 53, 41, 3, 1), (23, 80, 60, 1, 1),
+                                           (24, 64, 40, 4, 2)])
+def test_bp_tiny_disparity_counts_bit_exact(ctx, seed, W, H, D, ml):
+    """D of 1-4: the plane cost's clamp into [0, D-2] and the range test at its ends (D < 2
+    takes the general form everywhere)"""
+    say(f"bp tiny D seed={seed} {W}x{H} D={D} min_level={ml}")
+    left, right, _ = synth.make_pair(seed, W, H, max(D, 2), (3, 6))
+    vo, wo = O.cost_volume(left, right, D)
+    po = O.Pyramid(left)
+    bo = O.BP(po, vo, wo, D, min_level=ml)
+    bo.step()
+    do, plo = bo.map()
+    vg = ctx.cost_volume(left, right, D)
+    pg = ctx.pyramid(left)
+    dg, plg = ctx.bp(pg, vg, min_level=ml).run()
+    assert same_bits(do, dg), f"{mismatch(do, dg).mean():.4%} of disparities differ"
+    assert same_bits(plo, plg)
+
+
 def test_bp_use_cost_vol_false(ctx):
     say("bp use_cost_vol=0")
     left, right = _bp_pair(3, 80, 60, 16)
