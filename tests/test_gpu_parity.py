@@ -348,10 +348,10 @@ def test_bp_full_run_bit_exact(ctx, seed, W, H, D, ml):
 
 
 @pytest.mark.parametrize("seed,W,H,D,ml", [(21, 48, 36, 2, 1), (22, 53, 41, 3, 1), (23, 80, 60, 1, 1),
-                                           (24, 64, 40, 4, 2)])
+                                           (24, 64, 40, 4, 2), (36, 9, 7, 5, 1)])
 def test_bp_tiny_disparity_counts_bit_exact(ctx, seed, W, H, D, ml):
     """D of 1-4: the plane cost's clamp into [0, D-2] and the range test at its ends (D < 2
-    takes the general form everywhere)"""
+    takes the general form everywhere); and the smallest image the census window allows"""
     say(f"bp tiny D seed={seed} {W}x{H} D={D} min_level={ml}")
     left, right, _ = synth.make_pair(seed, W, H, max(D, 2), (3, 6))
     vo, wo = O.cost_volume(left, right, D)
