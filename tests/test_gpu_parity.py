@@ -545,6 +545,13 @@ def test_batch_larger_pairs(ctx):
     _check_batch(ctx, _batch_inputs(ctx, specs))
 
 
+def test_batch_odd_sizes(ctx):
+    """Odd widths, heights and disparity counts (tile and segment edges) in the same launches."""
+    say("batch of odd-sized pairs")
+    specs = [(71, 101, 67, 20, 1), (72, 203, 131, 37, 2), (73, 9, 7, 5, 1), (74, 333, 217, 3, 2)]
+    _check_batch(ctx, _batch_inputs(ctx, specs))
+
+
 def test_batch_rejects_mixed_cost_forms(ctx):
     left, right = _bp_pair(61, 64, 48, 16)
     a = ctx.bp(ctx.pyramid(left), ctx.cost_volume(left, right, 16), min_level=2)
