@@ -462,7 +462,7 @@ def test_bp_rejects_bad_min_level(ctx):
 
 
 # ---------------------------------------------------------------- (d)
-@pytest.mark.parametrize("seed,W,H,D", [(1, 64, 48, 16), (5, 120, 90, 32)])
+@pytest.mark.parametrize("seed,W,H,D", [(1, 64, 48, 16), (5, 120, 90, 32), (7, 101, 67, 20), (8, 217, 151, 27)])
 def test_left_right_check_and_fill_bit_exact(ctx, seed, W, H, D):
     say(f"lr check seed={seed} {W}x{H}")
     left, right = _bp_pair(seed, W, H, D)
