@@ -45,7 +45,7 @@ def test_f32_volume_and_wta(ctx, right_view):
 
 
 @pytest.mark.parametrize("seed,W,H,D,ml,right_view", [(4, 96, 72, 24, 2, False), (5, 128, 96, 32, 1, False),
-                                                      (6, 120, 90, 32, 2, True)])
+                                                      (6, 120, 90, 32, 2, True), (9, 103, 71, 21, 1, False)])
 def test_bp_on_f32_volume_bit_exact(ctx, seed, W, H, D, ml, right_view):
     say(f"mc-cnn bp seed={seed} {W}x{H} D={D} min_level={ml} right={right_view}")
     left, right, raw = mccnn_like(seed, W, H, D)
@@ -95,7 +95,8 @@ def test_ad_volume_and_wta(ctx, seed, W, H, D, right_view):
     assert same_bits(vg, vo) and same_bits(wg, wo)
 
 
-@pytest.mark.parametrize("seed,W,H,D,ml,right_view", [(5, 96, 72, 24, 2, False), (6, 120, 90, 32, 1, True)])
+@pytest.mark.parametrize("seed,W,H,D,ml,right_view", [(5, 96, 72, 24, 2, False), (6, 120, 90, 32, 1, True),
+                                                      (8, 101, 67, 19, 1, True)])
 def test_bp_on_ad_volume_bit_exact(ctx, seed, W, H, D, ml, right_view):
     say(f"ad bp seed={seed} {W}x{H} D={D} min_level={ml} right={right_view}")
     left, right, _ = synth.make_pair(seed, W, H, D, (3, 8))
