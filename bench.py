@@ -188,11 +188,15 @@ def main():
     tuning = {}
     if not args.no_autotune:
         # single_hb for this pair's shape; coarse_min_pix matters for the mixed sizes below (the
-        # coarse levels of this pair all hold thousands of pixels per superpixel)
+        # coarse levels of this pair all hold thousands of pixels per superpixel).  The segment
+        # length (512) and the sweeps per ahead launch (4) stay at the shape the dominant
+        # kernel's traffic and VALU profiles were measured at, so that `roofline` reads them for
+        # the launches it times (their other values measured within the noise of these)
         from hbp_amd import tune
-        tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2), "seg_len": (512, 384, 256)})
+        tuning["single"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid={"single_hb": (4, 2)})
         tuning["levels"] = tune.tune_levels(ctx, left, right, D, info["min_level"])
-        tuning["pipe"] = tune.tune_single(ctx, left, right, D, info["min_level"], grid=tune.PIPE_GRID)
+        tuning["pipe"] = tune.tune_single(ctx, left, right, D, info["min_level"],
+                                          grid={"pipe_blocks": tune.PIPE_GRID["pipe_blocks"], "pipe_batch": (4,)})
         tuning["hamming"] = tune.tune_hamming(ctx, left, right, D)
     for _ in range(args.warmup):
         one_pair()
@@ -287,14 +291,17 @@ def main():
     # HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
     # and its VALU instructions per launch (SQ_INSTS_VALU), from the committed summaries
     traffic, valu = None, None
+    # the summaries were taken with the default shape (segments of 512 pixels, 4 RANSAC sweeps
+    # per ahead launch); launches of another shape move other bytes and instructions
+    profiled_shape = ctx.get_tuning("seg_len") == 512 and ctx.get_tuning("pipe_batch") == 4
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
-    if os.path.exists(tfile):
+    vfile = os.path.join(ROOT, "profiles", f"valu_{dom}_{args.workload}.json")
+    if profiled_shape and os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
-    vfile = os.path.join(ROOT, "profiles", f"valu_{dom}_{args.workload}.json")
-    if os.path.exists(vfile) and avg_ms > 0:
+    if profiled_shape and os.path.exists(vfile) and avg_ms > 0:
         try:
             vi = json.load(open(vfile)).get("valu_instr_per_launch")
             if vi:
