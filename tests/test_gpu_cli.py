@@ -79,9 +79,9 @@ def test_cli_census_with_gaussian_preblur(tmp_path):
     assert same_bits(M.read_pfm(out + "_hbp_isp_disp.pfm"), disp)
 
 
-def test_cli_lr_refinement(tmp_path):
-    W, H, D = 80, 64, 16
-    left, right, _ = synth.make_pair(72, W, H, D, (3, 6))
+@pytest.mark.parametrize("seed,W,H,D", [(72, 80, 64, 16), (75, 101, 67, 19)])
+def test_cli_lr_refinement(tmp_path, seed, W, H, D):
+    left, right, _ = synth.make_pair(seed, W, H, D, (3, 6))
     out = run_cli(tmp_path, left, right, D, lr=True)
     vol, wta = O.cost_volume(left, right, D)
     dl, pl = oracle_view(left, vol, wta, D)
