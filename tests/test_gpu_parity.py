@@ -570,9 +570,10 @@ def _single(ctx, left, right, D, ml):
 def test_server_results_equal_single_runs(ctx):
     """Pairs of different sizes submitted at different times, at most 3 on the device at once:
     every result equals the pair's own run (each result the same alone and in the mix)."""
-    say("server: 7 pairs, 3 slots, staggered submits")
+    say("server: 9 pairs, 3 slots, staggered submits")
     specs = [(71, 96, 72, 24, 2), (72, 160, 120, 48, 2), (73, 64, 48, 16, 1), (74, 200, 64, 40, 2),
-             (75, 120, 90, 32, 3), (76, 80, 200, 32, 2), (77, 128, 96, 32, 2)]
+             (75, 120, 90, 32, 3), (76, 80, 200, 32, 2), (77, 128, 96, 32, 2), (78, 101, 67, 19, 1),
+             (79, 9, 7, 3, 1)]
     pairs = [(_bp_pair(s, W, H, D), D, ml) for s, W, H, D, ml in specs]
     srv = H.Server(ctx, max_active=3)
     tickets = {}
